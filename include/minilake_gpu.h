@@ -1,0 +1,412 @@
+/*
+ * minilake_gpu.h — C ABI of the MI355X (gfx950) execution path for MiniLake's
+ * data-parallel core: filter/projection expression kernels, hash aggregate,
+ * hash join build/probe, TopN, fused pipelines, batched fragments and learned
+ * hash-table sizing.
+ *
+ * Everything crossing this boundary is plain C: POD structs, pointers, sizes
+ * and int status codes. No exception or unwinding crosses it. Column data
+ * pointers given to the GPU library are DEVICE pointers; dictionaries, plan
+ * structs and expression programs live in HOST memory.
+ *
+ * The reference interfaces each group of entry points replaces are cited per
+ * declaration (paths relative to the MiniLake repository root).
+ *
+ * Ownership rules (mirror crates/minilake-exec/src/pipeline.rs:32-79):
+ *   - the caller owns inputs for the duration of a call;
+ *   - results are allocated by the library and released with ml_result_free;
+ *   - one ml_ctx (one HIP stream) per host thread; handles are not
+ *     thread-safe, like LocalSink;
+ *   - finalize is called once, after the last sink.
+ */
+#ifndef MINILAKE_GPU_H
+#define MINILAKE_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes: one per MiniLakeError variant
+ * (crates/minilake-core/src/error.rs:8-48), 0 = Ok.                  */
+/* ------------------------------------------------------------------ */
+typedef int32_t ml_status;
+enum {
+  ML_OK = 0,
+  ML_ERR_PARSE = 1,
+  ML_ERR_PLAN = 2,
+  ML_ERR_EXECUTION = 3,
+  ML_ERR_UNSUPPORTED = 4,
+  ML_ERR_IO = 5,
+  ML_ERR_PARQUET = 6,
+  ML_ERR_RESOURCES_EXHAUSTED = 7,
+  ML_ERR_CANCELLED = 8,
+  ML_ERR_INTERNAL = 9,
+  /* Not an error of the reference: a pre-sized hash table was too small.
+   * Raised only by the learned-sizing path, which then falls back to the
+   * plain resize-on-overflow path. */
+  ML_ERR_CAPACITY = 10
+};
+
+/* ------------------------------------------------------------------ */
+/* Data types (crates/minilake-core/src/types.rs:15-28)                */
+/* ------------------------------------------------------------------ */
+enum {
+  ML_TYPE_BOOLEAN = 0, /* 1 byte per value (Vec<bool>) */
+  ML_TYPE_INT32 = 1,
+  ML_TYPE_INT64 = 2,
+  ML_TYPE_FLOAT64 = 3,
+  ML_TYPE_DATE = 4, /* i32 days since 1970-01-01 */
+  ML_TYPE_UTF8 = 5, /* dictionary-encoded: u32 code per row + ml_dict */
+  ML_TYPE_NULL = -1 /* only for a NULL literal */
+};
+
+/* Dictionary of a UTF8 column (DictVec, crates/minilake-core/src/column.rs:126-131).
+ * Host memory. Entry i is bytes[offsets[i] .. offsets[i+1]). */
+typedef struct ml_dict {
+  int32_t len;
+  int32_t _pad;
+  const uint32_t* offsets; /* len + 1 entries */
+  const uint8_t* bytes;
+} ml_dict;
+
+/* One column (Column/ColumnData, crates/minilake-core/src/column.rs:135-200). */
+typedef struct ml_column {
+  int32_t type;
+  int32_t _pad;
+  const void* data;          /* device pointer, `len` values of the type's width */
+  int64_t len;
+  const uint64_t* validity;  /* LSB-first bitmap; NULL = all valid. The GPU path
+                                requires NULL (every configured table is NULL-free)
+                                and returns ML_ERR_UNSUPPORTED otherwise. */
+  const ml_dict* dict;       /* UTF8 only */
+} ml_column;
+
+/* A batch (crates/minilake-core/src/batch.rs:25-29): equal-length columns plus
+ * an optional ascending selection vector (batch.rs:14). */
+typedef struct ml_batch {
+  const ml_column* columns;
+  int32_t num_columns;
+  int32_t _pad;
+  int64_t num_rows;
+  const uint32_t* selection; /* device pointer, ascending; NULL = all rows */
+  int64_t selection_len;
+} ml_batch;
+
+/* ------------------------------------------------------------------ */
+/* Expressions: PhysicalExpr (crates/minilake-exec/src/expr/mod.rs:109-173)
+ * flattened to postfix. Column indices refer to the row layout at the
+ * point the expression is evaluated (see ml_operator).               */
+/* ------------------------------------------------------------------ */
+enum {
+  ML_EXPR_COLUMN = 0,
+  ML_EXPR_LITERAL = 1,
+  ML_EXPR_BINARY = 2,
+  ML_EXPR_NOT = 3,
+  ML_EXPR_NEGATE = 4,
+  ML_EXPR_CAST = 5
+};
+/* BinaryOp, same order as crates/minilake-exec/src/expr/mod.rs:20-47 */
+enum {
+  ML_OP_ADD = 0, ML_OP_SUB, ML_OP_MUL, ML_OP_DIV, ML_OP_MOD,
+  ML_OP_EQ, ML_OP_NOTEQ, ML_OP_LT, ML_OP_LTEQ, ML_OP_GT, ML_OP_GTEQ,
+  ML_OP_AND, ML_OP_OR
+};
+
+typedef struct ml_expr_node {
+  int32_t kind;   /* ML_EXPR_* */
+  int32_t op;     /* ML_OP_* for BINARY */
+  int32_t type;   /* literal type / cast target */
+  int32_t column; /* COLUMN index */
+  int64_t i64;    /* INT32/INT64/DATE/BOOLEAN literal */
+  double f64;     /* FLOAT64 literal */
+  const char* str; /* UTF8 literal (host memory) */
+  int32_t str_len;
+  int32_t _pad;
+} ml_expr_node;
+
+typedef struct ml_expr {
+  const ml_expr_node* nodes;
+  int32_t num_nodes;
+  int32_t _pad;
+} ml_expr;
+
+/* ------------------------------------------------------------------ */
+/* Execution context (ExecConfig, crates/minilake-exec/src/context.rs:14-34) */
+/* ------------------------------------------------------------------ */
+typedef struct ml_exec_config {
+  int64_t batch_rows;       /* device batch (morsel) size; default 2^21 */
+  int32_t device;           /* HIP device ordinal */
+  int32_t block_size;       /* threads per workgroup for streaming kernels (0 = tuned) */
+  int32_t grid_per_cu;      /* workgroups per CU for grid-stride kernels (0 = tuned) */
+  int32_t use_bloom;        /* 1: join tables carry a blocked Bloom filter (-1 = auto) */
+  void* stream;             /* hipStream_t to launch on; NULL = library-owned stream */
+} ml_exec_config;
+
+typedef struct ml_ctx ml_ctx;
+typedef struct ml_result ml_result;
+typedef struct ml_hashagg ml_hashagg;
+typedef struct ml_join_build ml_join_build;
+typedef struct ml_join_table ml_join_table;
+typedef struct ml_topn ml_topn;
+
+void ml_default_config(ml_exec_config* cfg);
+ml_status ml_ctx_create(const ml_exec_config* cfg, ml_ctx** out);
+void ml_ctx_destroy(ml_ctx* ctx);
+/* Message of the last error raised on this context ("" if none). */
+const char* ml_last_error(const ml_ctx* ctx);
+/* Wait for all work queued on the context's stream. */
+ml_status ml_ctx_synchronize(ml_ctx* ctx);
+/* The context's hipStream_t. */
+void* ml_ctx_stream(ml_ctx* ctx);
+
+/* Results: library-owned device columns (host copies on request). */
+int64_t ml_result_num_rows(const ml_result* r);
+int32_t ml_result_num_columns(const ml_result* r);
+/* Column i of the result; its data pointer is device memory. */
+const ml_column* ml_result_column(const ml_result* r, int32_t i);
+/* Copy column i to host memory (`dst` must hold num_rows * width bytes). */
+ml_status ml_result_copy_column(ml_ctx* ctx, const ml_result* r, int32_t i, void* dst);
+/* Copy column i's validity as one byte per row (1 = valid) to host memory.
+ * Result columns carry validity this way; ml_column.validity of a result is NULL. */
+ml_status ml_result_copy_validity(ml_ctx* ctx, const ml_result* r, int32_t i, uint8_t* dst);
+void ml_result_free(ml_result* r);
+
+/* ------------------------------------------------------------------ */
+/* Stateless operators                                                */
+/* ------------------------------------------------------------------ */
+
+/* Result type of `expr` over `batch`'s columns (after the coercions of
+ * crates/minilake-exec/src/expr/eval.rs:218-301). */
+ml_status ml_expr_type(ml_ctx* ctx, const ml_expr* expr, const ml_batch* batch, int32_t* out_type);
+
+/* FilterOperator::execute (crates/minilake-exec/src/operators/filter.rs:19-25)
+ * via select (expr/eval.rs:167-196) + mask_to_selection (kernels/cmp.rs:85-118):
+ * writes the ascending indices of active rows where `predicate` is TRUE
+ * (NULL dropped) to `out_sel` (device, capacity = active rows). */
+ml_status ml_filter(ml_ctx* ctx, const ml_expr* predicate, const ml_batch* in,
+                    uint32_t* out_sel, int64_t* out_len);
+
+/* ProjectionOperator::execute (crates/minilake-exec/src/operators/projection.rs:21-32)
+ * via evaluate_to_column (expr/eval.rs:154-160): evaluates each expression over
+ * every physical row; the selection is kept by the caller. Result column i holds
+ * exprs[i] (with its validity bitmap if any row is NULL). */
+ml_status ml_project(ml_ctx* ctx, const ml_expr* exprs, int32_t num_exprs, const ml_batch* in,
+                     ml_result** out);
+
+/* hash_columns (crates/minilake-exec/src/kernels/hash.rs:16-23): one 64-bit
+ * hash per active row over the key expressions, bit-identical to the reference
+ * (SEED start, combine per column, hash_bytes per dictionary entry). */
+ml_status ml_hash_columns(ml_ctx* ctx, const ml_expr* keys, int32_t num_keys, const ml_batch* in,
+                          uint64_t* out_hashes);
+
+/* ------------------------------------------------------------------ */
+/* Hash aggregate sink (HashAggregateSink / HashAggState,
+ * crates/minilake-exec/src/operators/aggregate/hash.rs:55-172,285-484;
+ * accumulators aggregate/accumulator.rs:99-544)                      */
+/* ------------------------------------------------------------------ */
+enum { /* AggregateFunction, accumulator.rs:24-38 */
+  ML_AGG_COUNT_STAR = 0, ML_AGG_COUNT = 1, ML_AGG_SUM = 2, ML_AGG_AVG = 3,
+  ML_AGG_MIN = 4, ML_AGG_MAX = 5
+};
+enum { /* AggMode, aggregate/mod.rs:50-57 */
+  ML_AGG_MODE_SINGLE = 0, ML_AGG_MODE_PARTIAL = 1, ML_AGG_MODE_FINAL = 2
+};
+enum { /* how the group table is sized */
+  ML_SIZING_PLAIN = 0,   /* start small, double and rehash before a batch that could
+                            exceed 7/8 load (swiss.rs:219-232 applied per device batch) */
+  ML_SIZING_HINT = 1,    /* pre-size from capacity_hint; overflow -> ML_ERR_CAPACITY */
+  ML_SIZING_LEARNED = 2  /* pre-size from the learned estimator; on overflow fall back
+                            to ML_SIZING_PLAIN automatically */
+};
+
+typedef struct ml_aggregate {
+  int32_t func;     /* ML_AGG_* */
+  int32_t arg_type; /* type of the argument expression (ignored for COUNT(*)) */
+  ml_expr arg;      /* empty for COUNT(*) */
+} ml_aggregate;
+
+typedef struct ml_agg_spec {
+  const ml_expr* group_exprs;
+  const int32_t* key_types;
+  int32_t num_keys;
+  int32_t num_aggs;
+  const ml_aggregate* aggs;
+  int32_t mode;          /* ML_AGG_MODE_SINGLE only on the GPU */
+  int32_t sizing;        /* ML_SIZING_* */
+  int64_t capacity_hint; /* expected distinct groups (HINT / LEARNED) */
+} ml_agg_spec;
+
+ml_status ml_hashagg_create(ml_ctx* ctx, const ml_agg_spec* spec, ml_hashagg** out);
+/* HashAggLocal::sink (aggregate/hash.rs:429-437). */
+ml_status ml_hashagg_sink(ml_hashagg* agg, const ml_batch* batch);
+/* HashAggregateSink::finalize (aggregate/hash.rs:369-425): key columns then one
+ * column per aggregate (finish semantics of accumulator.rs:500-526). */
+ml_status ml_hashagg_finalize(ml_hashagg* agg, ml_result** out);
+/* Groups currently held (valid at any point; synchronizes). */
+ml_status ml_hashagg_num_groups(ml_hashagg* agg, int64_t* out);
+/* Slots allocated and number of grow-and-rehash steps taken so far. */
+ml_status ml_hashagg_table_stats(ml_hashagg* agg, int64_t* capacity, int32_t* resizes);
+void ml_hashagg_free(ml_hashagg* agg);
+
+/* ------------------------------------------------------------------ */
+/* Hash join (crates/minilake-exec/src/operators/join.rs:36-338)      */
+/* ------------------------------------------------------------------ */
+typedef struct ml_join_build_spec {
+  const ml_expr* keys;       /* build key expressions */
+  const int32_t* key_types;  /* shared key types (physical_planner.rs:112-121 casts) */
+  int32_t num_keys;
+  int32_t _pad;
+} ml_join_build_spec;
+
+/* JoinBuildSink::new / create_local / sink (join.rs:72-101,217-222). */
+ml_status ml_join_build_create(ml_ctx* ctx, const ml_join_build_spec* spec, ml_join_build** out);
+ml_status ml_join_build_sink(ml_join_build* b, const ml_batch* batch);
+/* JoinBuildSink::finalize + build_table (join.rs:122-203). The table keeps every
+ * build column; NULL keys never match. Consumes the builder. */
+ml_status ml_join_build_finalize(ml_join_build* b, ml_join_table** out);
+int64_t ml_join_table_num_rows(const ml_join_table* t);
+/* 1 if every build key is distinct (each probe row matches at most once). */
+int32_t ml_join_table_unique(const ml_join_table* t);
+void ml_join_table_free(ml_join_table* t);
+
+/* HashJoinProbe::execute (join.rs:263-315): all matching (probe row, build row)
+ * pairs. Result columns: probe row index (INT64, physical row of `probe`) and
+ * build row index (INT64). Pair order is unspecified. */
+ml_status ml_join_probe(ml_ctx* ctx, const ml_join_table* t, const ml_expr* probe_keys,
+                        int32_t num_keys, const ml_batch* probe, ml_result** out_pairs);
+
+/* ------------------------------------------------------------------ */
+/* Fused pipelines: Source -> Operator* -> Sink
+ * (Pipeline::run_worker/push, crates/minilake-exec/src/pipeline.rs:109-161).
+ * One kernel runs the whole chain per row; nothing between operators is
+ * materialized in HBM.                                               */
+/* ------------------------------------------------------------------ */
+enum { ML_PIPE_FILTER = 0, ML_PIPE_PROBE = 1, ML_PIPE_PROJECT = 2 };
+
+typedef struct ml_operator {
+  int32_t kind;
+  int32_t build_is_left;      /* PROBE: output layout build ++ probe (join.rs:331-335) */
+  ml_expr predicate;          /* FILTER */
+  const ml_join_table* table; /* PROBE */
+  const ml_expr* keys;        /* PROBE: probe key expressions */
+  int32_t num_keys;           /* PROBE */
+  int32_t num_exprs;          /* PROJECT */
+  const ml_expr* exprs;       /* PROJECT: the new layout is these expressions */
+} ml_operator;
+
+/* Run a pipeline whose sink is a hash aggregate / join build / TopN. */
+ml_status ml_pipeline_hashagg(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops,
+                              int32_t num_ops, ml_hashagg* sink);
+ml_status ml_pipeline_join_build(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops,
+                                 int32_t num_ops, ml_join_build* sink);
+ml_status ml_pipeline_topn(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops,
+                           int32_t num_ops, ml_topn* sink);
+/* Pipeline ending in a collect sink (CollectSink, operators/collect.rs:11-53):
+ * every column of the final layout, compacted to the surviving rows. */
+ml_status ml_pipeline_collect(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops,
+                              int32_t num_ops, ml_result** out);
+
+/* ------------------------------------------------------------------ */
+/* TopN sink (SortSink with limit, crates/minilake-exec/src/operators/sort.rs:43-268) */
+/* ------------------------------------------------------------------ */
+typedef struct ml_sort_key {
+  ml_expr expr;
+  int32_t asc;
+  int32_t nulls_first;
+} ml_sort_key;
+
+typedef struct ml_sort_spec {
+  const ml_sort_key* keys;
+  int32_t num_keys;
+  int32_t _pad;
+  int64_t limit; /* ORDER BY ... LIMIT n (physical_planner.rs:150-178) */
+} ml_sort_spec;
+
+ml_status ml_topn_create(ml_ctx* ctx, const ml_sort_spec* spec, ml_topn** out);
+ml_status ml_topn_sink(ml_topn* t, const ml_batch* batch);
+/* Best `limit` rows, sorted; every column of the sunk batches is gathered. */
+ml_status ml_topn_finalize(ml_topn* t, ml_result** out);
+void ml_topn_free(ml_topn* t);
+
+/* ------------------------------------------------------------------ */
+/* Batched fragments: many independent "SELECT g, SUM(m) FROM probe JOIN build
+ * USING (key) GROUP BY g" fragments in one persistent-kernel launch, optionally
+ * replayed through a captured hipGraph. No reference counterpart (its nearest
+ * analog is the morsel queue, crates/minilake-exec/src/scheduler.rs:44-68).   */
+/* ------------------------------------------------------------------ */
+typedef struct ml_fragment {
+  const int64_t* build_keys;   /* device, unique */
+  const int64_t* build_groups; /* device, group key of each build row, in [0, num_groups) */
+  int64_t build_rows;
+  const int64_t* probe_keys;   /* device */
+  const double* probe_values;  /* device */
+  int64_t probe_rows;
+  int64_t num_groups;          /* group key domain (dense) */
+  double* out_sums;            /* device, num_groups */
+  int64_t* out_counts;         /* device, num_groups */
+} ml_fragment;
+
+typedef struct ml_fragment_tuning {
+  int32_t block_size;      /* threads per workgroup */
+  int32_t tile_rows;       /* probe rows per work item */
+  int32_t blocks_per_cu;   /* persistent grid = CUs * blocks_per_cu */
+  int32_t lds_groups;      /* groups up to this domain aggregate in LDS first */
+} ml_fragment_tuning;
+
+typedef struct ml_fragments ml_fragments;
+
+/* Plan a set of fragments: allocates every hash table and the work queue. */
+ml_status ml_fragments_create(ml_ctx* ctx, const ml_fragment* frags, int32_t n,
+                              const ml_fragment_tuning* tuning, ml_fragments** out);
+/* Run all fragments: one build launch and one probe launch (persistent kernels). */
+ml_status ml_fragments_run(ml_fragments* f);
+/* Capture ml_fragments_run into a hipGraph (once) and replay it. */
+ml_status ml_fragments_run_graph(ml_fragments* f);
+/* Baseline: the same fragments, two kernel launches per fragment. */
+ml_status ml_fragments_run_per_fragment(ml_fragments* f);
+/* Autotune: time a small grid of tunings and keep the fastest. */
+ml_status ml_fragments_autotune(ml_fragments* f, ml_fragment_tuning* best);
+void ml_fragments_free(ml_fragments* f);
+
+/* ------------------------------------------------------------------ */
+/* Learned cardinality estimation (replaces the crude estimate_rows,
+ * crates/minilake-sql/src/physical_planner.rs:251-295, for table sizing).
+ * MLP 32 -> 64 -> 64 -> 2, FP32, ReLU.                                 */
+/* ------------------------------------------------------------------ */
+#define ML_MLP_IN 32
+#define ML_MLP_H1 64
+#define ML_MLP_H2 64
+#define ML_MLP_OUT 2
+#define ML_MLP_NUM_PARAMS \
+  (ML_MLP_IN * ML_MLP_H1 + ML_MLP_H1 + ML_MLP_H1 * ML_MLP_H2 + ML_MLP_H2 + ML_MLP_H2 * ML_MLP_OUT + ML_MLP_OUT)
+
+typedef struct ml_mlp ml_mlp;
+
+/* Parameters in host memory: W1[64][32], b1[64], W2[64][64], b2[64], W3[2][64], b3[2]. */
+ml_status ml_mlp_create(ml_ctx* ctx, const float* params, ml_mlp** out);
+ml_status ml_mlp_get_params(ml_mlp* m, float* params);
+/* Forward on device features x[n][32] -> y[n][2]. */
+ml_status ml_mlp_forward(ml_mlp* m, const float* x, int64_t n, float* y);
+/* Mean-squared-error loss and parameter gradients on one batch (device x, t). */
+ml_status ml_mlp_loss_grad(ml_mlp* m, const float* x, const float* t, int64_t n, float* loss,
+                           float* grads_host);
+/* One Adam step over a minibatch; returns the batch loss. */
+ml_status ml_mlp_train_step(ml_mlp* m, const float* x, const float* t, int64_t n, float lr,
+                            float* loss);
+void ml_mlp_free(ml_mlp* m);
+
+/* Features of one aggregation/join input (device key column): row-group
+ * min/max/null-count statistics (crates/minilake-storage/src/stats.rs:15-22)
+ * plus a KMV sketch of a `sample`-row sample, 32 floats. */
+ml_status ml_sizing_features(ml_ctx* ctx, const int64_t* keys, int64_t n, int64_t row_group_rows,
+                             int32_t sample, uint64_t seed, float* out_features);
+/* Attach a trained estimator to the context: ML_SIZING_LEARNED sinks use it. */
+ml_status ml_ctx_set_estimator(ml_ctx* ctx, ml_mlp* m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MINILAKE_GPU_H */

@@ -1,0 +1,274 @@
+// Host-side structures of the gfx950 execution path (C++, no exceptions cross
+// the C ABI: every entry point catches and returns an ml_status).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/minilake_gpu.h"
+#include "kernels.h"
+#include "mlgpu.h"
+
+namespace mlg {
+
+// MiniLakeError as a C++ exception inside the library.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] inline void fail(int code, const std::string& m) { throw Error(code, m); }
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(ML_ERR_INTERNAL, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+inline const char* type_name(int t) {
+  switch (t) {
+    case ML_TYPE_BOOLEAN: return "BOOLEAN";
+    case ML_TYPE_INT32: return "INT32";
+    case ML_TYPE_INT64: return "INT64";
+    case ML_TYPE_FLOAT64: return "FLOAT64";
+    case ML_TYPE_DATE: return "DATE";
+    case ML_TYPE_UTF8: return "VARCHAR";
+    default: return "NULL";
+  }
+}
+inline int type_width(int t) {
+  switch (t) {
+    case ML_TYPE_BOOLEAN: return 1;
+    case ML_TYPE_INT32: case ML_TYPE_DATE: case ML_TYPE_UTF8: return 4;
+    default: return 8;
+  }
+}
+
+// ------------------------------------------------------------------ memory
+// Stream-ordered caching allocator: blocks are recycled in power-of-two size
+// classes and only returned to the driver when the context is destroyed. All
+// work of a context runs on its one stream, so reuse after free is ordered.
+class DeviceArena {
+ public:
+  explicit DeviceArena(int device) : device_(device) {}
+  ~DeviceArena();
+  void* alloc(size_t bytes);
+  void free(void* p);
+  size_t bytes_in_use() const { return in_use_; }
+
+ private:
+  int device_;
+  std::map<size_t, std::vector<void*>> free_;
+  std::map<void*, size_t> live_;
+  size_t in_use_ = 0;
+};
+
+struct Buffer {
+  DeviceArena* arena = nullptr;
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  Buffer() = default;
+  Buffer(DeviceArena* a, size_t n) : arena(a), ptr(n ? a->alloc(n) : nullptr), bytes(n) {}
+  Buffer(const Buffer&) = delete;
+  Buffer& operator=(const Buffer&) = delete;
+  Buffer(Buffer&& o) noexcept { *this = std::move(o); }
+  Buffer& operator=(Buffer&& o) noexcept {
+    if (this != &o) {
+      reset();
+      arena = o.arena; ptr = o.ptr; bytes = o.bytes;
+      o.ptr = nullptr; o.bytes = 0;
+    }
+    return *this;
+  }
+  ~Buffer() { reset(); }
+  void reset() {
+    if (ptr && arena) arena->free(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// --------------------------------------------------------------- dictionary
+// Host copy of an ml_dict, shared by columns and results.
+struct HostDict {
+  std::vector<std::string> entries;
+  std::vector<uint32_t> offsets;
+  std::string bytes;
+  ml_dict view{};
+  static std::shared_ptr<HostDict> from(const ml_dict* d);
+  void finish_view();
+};
+
+// ------------------------------------------------------------------ context
+struct MlpModel;
+}  // namespace mlg
+
+struct ml_ctx {
+  ml_exec_config cfg{};
+  int device = 0;
+  int num_cus = 256;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string last_error;
+  std::unique_ptr<mlg::DeviceArena> arena;
+  mlg::MlpModel* estimator = nullptr;
+  // small per-context device scratch: counters and flags
+  mlg::Buffer counters;  // 64 x u64
+  std::map<const ml_dict*, std::shared_ptr<mlg::HostDict>> dicts;
+  std::shared_ptr<mlg::HostDict> dict(const ml_dict* d);
+};
+
+namespace mlg {
+
+// ------------------------------------------------------------ result column
+struct ResultColumn {
+  int type = ML_TYPE_INT64;
+  Buffer data;
+  Buffer valid;                     // u8 per row, or empty
+  std::shared_ptr<HostDict> dict;   // UTF8
+  Buffer bitmap;                    // LSB-first validity words exported via ml_column
+};
+
+}  // namespace mlg
+
+struct ml_result {
+  ml_ctx* ctx = nullptr;
+  int64_t num_rows = 0;
+  std::vector<mlg::ResultColumn> cols;
+  std::vector<ml_column> views;
+  void finish_views();
+};
+
+namespace mlg {
+
+// -------------------------------------------------------------- expressions
+struct Scalar {
+  int type = ML_TYPE_NULL;  // ML_TYPE_NULL = SQL NULL
+  int64_t i = 0;
+  double f = 0;
+  std::string s;
+  bool is_null() const { return type == ML_TYPE_NULL; }
+};
+
+// A column of a pipeline's row layout.
+struct VCol;
+struct Node;
+using NodePtr = std::shared_ptr<const Node>;
+
+struct VCol {
+  int type = ML_TYPE_INT64;
+  int stage = 0;
+  const void* data = nullptr;       // device
+  const uint8_t* valid = nullptr;   // device u8 validity or null
+  std::shared_ptr<HostDict> dict;   // UTF8
+  NodePtr expr;                     // a projected expression (inlined where used)
+};
+
+struct Node {
+  int kind = ML_EXPR_COLUMN;
+  int op = 0;
+  int type = 0;
+  int column = 0;
+  Scalar lit;
+  NodePtr left, right;
+  const VCol* col = nullptr;  // resolved physical column
+};
+
+NodePtr parse_expr(const ml_expr& e);
+
+// Compiles expression trees against a row layout into device instructions.
+class Compiler {
+ public:
+  Compiler(ml_ctx* ctx, std::vector<Instr>& ins, std::vector<Buffer>& tables)
+      : ctx_(ctx), ins_(ins), tables_(tables) {}
+  // Result type of a resolved tree (the reference's expression typing).
+  int type_of(const NodePtr& n);
+  // Compile a resolved tree; the value is cast to `want` (evaluate_to_column
+  // semantics) unless want < 0. Returns the program.
+  Prog compile(const NodePtr& n, int want, int* out_type = nullptr);
+  // Group/join key word: cast to `key_type`, f64 normalized.
+  Prog compile_key(const NodePtr& n, int key_type);
+  // hash_columns input: a word to hash, or an already-hashed value (dictionaries).
+  Prog compile_hash_input(const NodePtr& n, bool* prehashed);
+  // A plain load of a layout column.
+  Prog compile_load(const VCol& c);
+
+ private:
+  struct Val {
+    bool is_const;
+    Scalar s;   // constant
+    int type;   // runtime type when not constant
+  };
+  Val gen(const NodePtr& n);               // emits code for non-constant values
+  Val gen_binary(const Node& n);
+  Val materialize(const Val& v);           // constant -> CONST instruction
+  void emit_cast(int from, int to);
+  Val cast_val(const Val& v, int to);
+  void push(uint16_t op, uint64_t imm = 0, uint64_t imm2 = 0, uint8_t stage = 0, uint32_t flags = 0);
+  void unary(uint16_t op, uint64_t imm = 0);
+  void binary(uint16_t op);
+  const uint8_t* dict_predicate(const HostDict& d, int op, const std::string& lit);
+
+  ml_ctx* ctx_;
+  std::vector<Instr>& ins_;
+  std::vector<Buffer>& tables_;
+  int depth_ = 0;
+  int max_depth_ = 0;
+  bool nullable_ = false;
+  std::vector<const Node*> consts_;
+};
+
+// ------------------------------------------------------------- join tables
+struct JoinTableImpl {
+  int64_t nrows = 0;
+  std::vector<int> key_types;
+  std::vector<std::shared_ptr<HostDict>> key_dicts;
+  std::vector<int> col_types;
+  std::vector<std::shared_ptr<HostDict>> col_dicts;
+  std::vector<Buffer> cols;
+  std::vector<Buffer> col_valid;
+  Buffer slots;
+  Buffer bloom;
+  DevJoinTable dev{};
+};
+
+}  // namespace mlg
+
+struct ml_join_table {
+  ml_ctx* ctx = nullptr;
+  mlg::JoinTableImpl impl;
+};
+
+// -------------------------------------------------------------- the pipeline
+namespace mlg {
+
+struct PipelineBuild {
+  ml_ctx* ctx;
+  Plan P{};
+  std::vector<Instr> ins;
+  std::vector<Buffer> tables;  // device lookup tables referenced by instructions
+  std::vector<const VCol*> layout;
+  std::vector<std::unique_ptr<VCol>> owned;
+  std::vector<std::vector<uint8_t>> host_tables;
+  int nstages = 1;
+  int64_t active_rows = 0;
+  bool expanding = false;
+  Compiler comp;
+  Buffer dev_ins;
+
+  PipelineBuild(ml_ctx* c) : ctx(c), comp(c, ins, tables) {}
+  void source(const ml_batch* b, bool use_selection = true);
+  void add_ops(const ml_operator* ops, int n);
+  void upload();  // instructions -> device; fills P.ins
+  int grid(int block = 256) const;
+  NodePtr resolve(const ml_expr& e) const;
+  const VCol* add_col(VCol c);
+};
+
+NodePtr resolve_layout(const NodePtr& n, const std::vector<const VCol*>& layout);
+
+}  // namespace mlg

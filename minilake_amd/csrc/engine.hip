@@ -1,0 +1,1684 @@
+// Host side of the gfx950 execution path: context, device memory, pipeline
+// planning and the sinks behind the C ABI (include/minilake_gpu.h).
+#include <algorithm>
+#include <cmath>
+#include <deque>
+#include <numeric>
+
+#include "engine.h"
+
+namespace mlg {
+
+// ------------------------------------------------------------------ memory
+static size_t size_class(size_t n) {
+  size_t c = 256;
+  while (c < n) c <<= 1;
+  return c;
+}
+
+void* DeviceArena::alloc(size_t bytes) {
+  const size_t c = size_class(bytes);
+  auto& fl = free_[c];
+  void* p = nullptr;
+  if (!fl.empty()) {
+    p = fl.back();
+    fl.pop_back();
+  } else {
+    hipError_t e = hipMalloc(&p, c);
+    if (e != hipSuccess) {
+      // release every cached block and retry once
+      (void)hipGetLastError();
+      hip_check(hipDeviceSynchronize(), "synchronize before trim");
+      for (auto& kv : free_)
+        for (void* q : kv.second) (void)hipFree(q);
+      free_.clear();
+      e = hipMalloc(&p, c);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(ML_ERR_RESOURCES_EXHAUSTED, "device allocation of " + std::to_string(c) + " bytes failed");
+      }
+    }
+  }
+  live_[p] = c;
+  in_use_ += c;
+  return p;
+}
+
+void DeviceArena::free(void* p) {
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  in_use_ -= it->second;
+  free_[it->second].push_back(p);
+  live_.erase(it);
+}
+
+DeviceArena::~DeviceArena() {
+  (void)hipSetDevice(device_);
+  for (auto& kv : free_)
+    for (void* q : kv.second) (void)hipFree(q);
+  for (auto& kv : live_) (void)hipFree(kv.first);
+}
+
+static void h2d(ml_ctx* ctx, void* dst, const void* src, size_t n) {
+  if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, ctx->stream), "copy to device");
+}
+static void d2h(ml_ctx* ctx, void* dst, const void* src, size_t n) {
+  if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
+  hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+}
+static void d2d(ml_ctx* ctx, void* dst, const void* src, size_t n) {
+  if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, ctx->stream), "device copy");
+}
+static void zero(ml_ctx* ctx, void* p, size_t bytes) {  // a fill kernel, so the work stays capturable
+  launch_fill_u64((uint64_t*)p, 0, (int64_t)((bytes + 7) / 8), ctx->stream);
+}
+
+// ------------------------------------------------------------ dictionaries
+std::shared_ptr<HostDict> HostDict::from(const ml_dict* d) {
+  auto h = std::make_shared<HostDict>();
+  for (int i = 0; i < d->len; i++)
+    h->entries.emplace_back((const char*)d->bytes + d->offsets[i], d->offsets[i + 1] - d->offsets[i]);
+  h->finish_view();
+  return h;
+}
+void HostDict::finish_view() {
+  offsets.assign(1, 0);
+  bytes.clear();
+  for (auto& e : entries) {
+    bytes += e;
+    offsets.push_back((uint32_t)bytes.size());
+  }
+  view.len = (int32_t)entries.size();
+  view.offsets = offsets.data();
+  view.bytes = (const uint8_t*)bytes.data();
+}
+
+static bool same_dict(const std::shared_ptr<HostDict>& a, const std::shared_ptr<HostDict>& b) {
+  return a == b || (a && b && a->entries == b->entries);
+}
+
+// ------------------------------------------------------------------ pipeline
+const VCol* PipelineBuild::add_col(VCol c) {
+  owned.push_back(std::make_unique<VCol>(std::move(c)));
+  return owned.back().get();
+}
+
+NodePtr PipelineBuild::resolve(const ml_expr& e) const { return resolve_layout(parse_expr(e), layout); }
+
+void PipelineBuild::source(const ml_batch* b, bool use_selection) {
+  if (!b) fail(ML_ERR_INTERNAL, "null batch");
+  layout.clear();
+  for (int i = 0; i < b->num_columns; i++) {
+    const ml_column& mc = b->columns[i];
+    if (mc.validity) fail(ML_ERR_UNSUPPORTED, "the GPU path takes NULL-free input columns (validity must be NULL)");
+    if (mc.len != b->num_rows) fail(ML_ERR_INTERNAL, "column length differs from batch rows");
+    if (mc.type < ML_TYPE_BOOLEAN || mc.type > ML_TYPE_UTF8) fail(ML_ERR_PLAN, "bad column type");
+    VCol c;
+    c.type = mc.type;
+    c.stage = 0;
+    c.data = mc.data;
+    if (mc.type == ML_TYPE_UTF8) {
+      if (!mc.dict) fail(ML_ERR_UNSUPPORTED, "UTF8 columns must be dictionary-encoded");
+      c.dict = ctx->dict(mc.dict);
+    }
+    if (b->num_rows > 0 && !mc.data) fail(ML_ERR_INTERNAL, "null column data");
+    layout.push_back(add_col(std::move(c)));
+  }
+  P.sel = (use_selection && b->selection) ? b->selection : nullptr;
+  active_rows = P.sel ? b->selection_len : b->num_rows;
+  P.begin = 0;
+  P.end = active_rows;
+  P.expand_step = -1;
+  P.nsteps = 0;
+}
+
+void PipelineBuild::add_ops(const ml_operator* ops, int n) {
+  int ntables = 0;
+  for (int i = 0; i < n; i++) {
+    const ml_operator& op = ops[i];
+    switch (op.kind) {
+      case ML_PIPE_FILTER: {
+        NodePtr pred = resolve(op.predicate);
+        if (comp.type_of(pred) != ML_TYPE_BOOLEAN) fail(ML_ERR_EXECUTION, "filter predicate is not boolean");
+        if (P.nsteps >= MAX_STEPS) fail(ML_ERR_UNSUPPORTED, "too many operators in one pipeline");
+        Step& st = P.steps[P.nsteps++];
+        st = Step{};
+        st.kind = STEP_FILTER;
+        st.pred = comp.compile(pred, -1);
+        break;
+      }
+      case ML_PIPE_PROBE: {
+        const ml_join_table* t = op.table;
+        if (!t) fail(ML_ERR_INTERNAL, "probe before build");
+        const JoinTableImpl& J = t->impl;
+        if (op.num_keys != (int)J.key_types.size()) fail(ML_ERR_PLAN, "probe and build key counts differ");
+        if (P.nsteps >= MAX_STEPS || nstages >= MAX_STAGES) fail(ML_ERR_UNSUPPORTED, "too many joins in one pipeline");
+        const int stage = nstages++;
+        Step& st = P.steps[P.nsteps];
+        st = Step{};
+        st.kind = STEP_PROBE;
+        st.stage = stage;
+        st.table = ntables;
+        st.nkeys = op.num_keys;
+        for (int k = 0; k < op.num_keys; k++) {
+          NodePtr kn = resolve(op.keys[k]);
+          if (J.key_types[k] == ML_TYPE_UTF8) {
+            const VCol* c = kn->kind == ML_EXPR_COLUMN ? kn->col : nullptr;
+            if (!c || !same_dict(c->dict, J.key_dicts[k]))
+              fail(ML_ERR_UNSUPPORTED, "string join keys must share the build side's dictionary");
+          }
+          st.keys[k] = comp.compile_key(kn, J.key_types[k]);
+        }
+        P.tables[ntables++] = J.dev;
+        if (!J.dev.unique) {
+          if (expanding) fail(ML_ERR_UNSUPPORTED, "more than one join with duplicate build keys in one pipeline");
+          expanding = true;
+          P.expand_step = P.nsteps;
+        }
+        P.nsteps++;
+        std::vector<const VCol*> build;
+        for (size_t c = 0; c < J.col_types.size(); c++) {
+          VCol v;
+          v.type = J.col_types[c];
+          v.stage = stage;
+          v.data = J.cols[c].ptr;
+          v.valid = J.col_valid[c].ptr ? J.col_valid[c].as<uint8_t>() : nullptr;
+          v.dict = J.col_dicts[c];
+          build.push_back(add_col(std::move(v)));
+        }
+        // output layout: build ++ probe or probe ++ build (join.rs:331-335)
+        std::vector<const VCol*> nl;
+        if (op.build_is_left) {
+          nl = build;
+          nl.insert(nl.end(), layout.begin(), layout.end());
+        } else {
+          nl = layout;
+          nl.insert(nl.end(), build.begin(), build.end());
+        }
+        layout = nl;
+        break;
+      }
+      case ML_PIPE_PROJECT: {
+        std::vector<const VCol*> nl;
+        for (int k = 0; k < op.num_exprs; k++) {
+          NodePtr e = resolve(op.exprs[k]);
+          VCol v;
+          v.type = comp.type_of(e);
+          if (v.type == ML_TYPE_NULL) v.type = ML_TYPE_INT64;
+          v.expr = e;
+          if (e->kind == ML_EXPR_COLUMN) {
+            v.dict = e->col->dict;
+            v.stage = e->col->stage;
+          }
+          nl.push_back(add_col(std::move(v)));
+        }
+        layout = nl;
+        break;
+      }
+      default: fail(ML_ERR_PLAN, "unknown operator kind");
+    }
+  }
+}
+
+void PipelineBuild::upload() {
+  if (ins.empty()) ins.push_back(Instr{});
+  dev_ins = Buffer(ctx->arena.get(), ins.size() * sizeof(Instr));
+  h2d(ctx, dev_ins.ptr, ins.data(), ins.size() * sizeof(Instr));
+  P.ins = dev_ins.as<Instr>();
+}
+
+int PipelineBuild::grid(int block) const {
+  int64_t g = (P.end - P.begin + block - 1) / block;
+  const int per_cu = ctx->cfg.grid_per_cu > 0 ? ctx->cfg.grid_per_cu : 8;
+  const int64_t cap = (int64_t)ctx->num_cus * per_cu;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+static int block_of(ml_ctx* ctx) { return ctx->cfg.block_size > 0 ? ctx->cfg.block_size : 256; }
+
+// Columns of a layout materialized on the device (collect, join payload).
+struct Materialized {
+  std::vector<int> types;
+  std::vector<std::shared_ptr<HostDict>> dicts;
+  std::vector<Buffer> cols;
+  std::vector<Buffer> valid;
+  int64_t rows = 0;
+};
+
+// Compile a layout column for materialization.
+static Prog column_prog(PipelineBuild& B, const VCol* c, bool* nullable) {
+  Prog p;
+  if (c->expr) p = B.comp.compile(c->expr, c->type);
+  else p = B.comp.compile_load(*c);
+  *nullable = p.nullable != 0;
+  return p;
+}
+
+// Run B with a COLLECT sink (optionally with per-stage row ids); rows come back in `out`.
+static void run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buffer* pair_rows, int pair_stages) {
+  ml_ctx* ctx = B.ctx;
+  uint64_t* counters = ctx->counters.as<uint64_t>();
+  // capacity: active rows, unless one probe can match several build rows
+  uint64_t cap = (uint64_t)B.active_rows;
+  if (B.expanding) {
+    B.P.sink = SINK_COUNT_PAIRS;
+    B.P.out_count = counters;
+    zero(ctx, counters, 16);
+    B.upload();
+    launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+    uint64_t n = 0;
+    d2h(ctx, &n, counters, 8);
+    cap = n;
+  }
+  if (with_cols) {
+    if ((int)B.layout.size() > MAX_COLS) fail(ML_ERR_UNSUPPORTED, "too many columns to materialize");
+    B.P.ncols = (int)B.layout.size();
+    for (size_t c = 0; c < B.layout.size(); c++) {
+      const VCol* v = B.layout[c];
+      bool nullable = false;
+      B.P.cols[c] = column_prog(B, v, &nullable);
+      const int w = type_width(v->type);
+      B.P.col_width[c] = w;
+      out.types.push_back(v->type);
+      out.dicts.push_back(v->dict);
+      out.cols.emplace_back(ctx->arena.get(), std::max<uint64_t>(cap, 1) * w);
+      out.valid.emplace_back();
+      if (nullable) out.valid.back() = Buffer(ctx->arena.get(), std::max<uint64_t>(cap, 1));
+      B.P.col_out[c] = out.cols.back().ptr;
+      B.P.col_valid[c] = out.valid.back().ptr ? out.valid.back().as<uint8_t>() : nullptr;
+    }
+  } else {
+    B.P.ncols = 0;
+  }
+  if (pair_rows) {
+    *pair_rows = Buffer(ctx->arena.get(), std::max<uint64_t>(cap, 1) * pair_stages * 8);
+    B.P.pair_out = pair_rows->as<int64_t>();
+    B.P.pair_stages = pair_stages;
+  } else {
+    B.P.pair_out = nullptr;
+  }
+  B.P.sink = SINK_COLLECT;
+  B.P.out_count = counters;
+  B.P.out_capacity = cap;
+  B.P.out_overflow = (uint32_t*)(counters + 1);
+  zero(ctx, counters, 16);
+  B.upload();
+  launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+  uint64_t hc[2] = {0, 0};
+  d2h(ctx, hc, counters, 16);
+  if (hc[1] & 1) fail(ML_ERR_INTERNAL, "collect output overflowed its capacity");
+  out.rows = (int64_t)hc[0];
+}
+
+static ml_result* make_result(ml_ctx* ctx, Materialized& m) {
+  auto* r = new ml_result();
+  r->ctx = ctx;
+  r->num_rows = m.rows;
+  for (size_t c = 0; c < m.types.size(); c++) {
+    ResultColumn rc;
+    rc.type = m.types[c];
+    rc.data = std::move(m.cols[c]);
+    rc.valid = std::move(m.valid[c]);
+    rc.dict = m.dicts[c];
+    r->cols.push_back(std::move(rc));
+  }
+  r->finish_views();
+  return r;
+}
+
+// ------------------------------------------------------------- aggregates
+struct AggDef {
+  int func;
+  int arg_type;
+  NodePtr arg;  // unresolved (parsed) argument, null for COUNT(*)
+  const ml_expr* raw;
+};
+
+struct OutDef {
+  int out_type;
+  int finish;       // FIN_*
+  int state;        // slot index
+  int count;        // slot whose value > 0 means "valid" (-1: always valid)
+};
+
+}  // namespace mlg
+
+struct ml_hashagg {
+  ml_ctx* ctx = nullptr;
+  std::vector<ml_expr> group_raw;
+  std::vector<int> key_types;
+  std::vector<mlg::AggDef> aggs;
+  int sizing = ML_SIZING_PLAIN;
+  int64_t hint = 0;
+  // storage of copied expressions
+  std::vector<std::vector<ml_expr_node>> node_store;
+  std::deque<std::string> str_store;
+
+  // plan, fixed at the first sink
+  bool planned = false;
+  bool direct = false;
+  bool key_nullable = false;
+  std::vector<mlg::AccSlot> slots;   // arg = index into aggs (-1 = rows)
+  std::vector<int> slot_cast;        // accumulator input type of the slot's argument
+  std::vector<mlg::OutDef> outs;
+  std::vector<std::shared_ptr<mlg::HostDict>> key_dicts;
+  // direct
+  int64_t ngroups = 0;
+  std::vector<int64_t> dim_size, dim_stride;
+  mlg::Buffer totals;
+  // hash
+  int nkw = 1;
+  uint64_t cap = 0;
+  mlg::Buffer table;
+  std::vector<mlg::Buffer> acc;
+  mlg::Buffer flags;   // [0] group count (u64), [1] overflow bits (u32)
+  uint64_t len_host = 0;
+  bool len_known = true;
+  int resizes = 0;
+  bool failed = false;
+};
+
+namespace mlg {
+
+static uint64_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+// SwissTable::with_capacity sizing (crates/minilake-hashtable/src/swiss.rs:86-96)
+static uint64_t swiss_capacity(uint64_t n) { return std::max<uint64_t>(16, next_pow2(n * 8 / 7 + 1)); }
+
+// Plan accumulator slots (accumulator.rs:104-165 state kinds) for the aggregate list.
+static void plan_agg(ml_hashagg* A, PipelineBuild& B) {
+  A->slots.clear();
+  A->slot_cast.clear();
+  A->outs.clear();
+  A->slots.push_back({ACC_COUNT, -1});  // rows of the group
+  A->slot_cast.push_back(-1);
+  auto add_slot = [&](int kind, int arg, int cast) {
+    for (size_t s = 0; s < A->slots.size(); s++)
+      if (A->slots[s].kind == kind && A->slots[s].arg == arg && A->slot_cast[s] == cast) return (int)s;
+    A->slots.push_back({kind, arg});
+    A->slot_cast.push_back(cast);
+    return (int)A->slots.size() - 1;
+  };
+  for (size_t a = 0; a < A->aggs.size(); a++) {
+    const AggDef& d = A->aggs[a];
+    bool nullable = false;
+    if (d.func != ML_AGG_COUNT_STAR) {
+      NodePtr n = B.resolve(*d.raw);
+      std::vector<Instr> tmp;
+      std::vector<Buffer> tt;
+      Compiler c(B.ctx, tmp, tt);
+      Prog p = c.compile(n, d.arg_type);
+      nullable = p.nullable != 0;
+    }
+    const int ia = (int)a;
+    OutDef o{ML_TYPE_INT64, FIN_COPY, 0, -1};
+    switch (d.func) {
+      case ML_AGG_COUNT_STAR: o.state = 0; break;
+      case ML_AGG_COUNT: o.state = nullable ? add_slot(ACC_COUNT, ia, d.arg_type) : 0; break;
+      case ML_AGG_SUM:
+        if (d.arg_type == ML_TYPE_INT32 || d.arg_type == ML_TYPE_INT64 || d.arg_type == ML_TYPE_BOOLEAN) {
+          o.out_type = ML_TYPE_INT64;
+          o.state = add_slot(ACC_SUM_I64, ia, ML_TYPE_INT64);
+        } else if (d.arg_type == ML_TYPE_FLOAT64) {
+          o.out_type = ML_TYPE_FLOAT64;
+          o.state = add_slot(ACC_SUM_F64, ia, ML_TYPE_FLOAT64);
+        } else {
+          fail(ML_ERR_PLAN, std::string("sum(") + type_name(d.arg_type) + ") not supported");
+        }
+        o.count = nullable ? add_slot(ACC_COUNT, ia, ML_TYPE_INT64) : 0;
+        break;
+      case ML_AGG_AVG:
+        if (d.arg_type != ML_TYPE_INT32 && d.arg_type != ML_TYPE_INT64 && d.arg_type != ML_TYPE_FLOAT64)
+          fail(ML_ERR_PLAN, std::string("avg(") + type_name(d.arg_type) + ") not supported");
+        o.out_type = ML_TYPE_FLOAT64;
+        o.finish = FIN_AVG;
+        o.state = add_slot(ACC_SUM_F64, ia, ML_TYPE_FLOAT64);
+        o.count = nullable ? add_slot(ACC_COUNT, ia, ML_TYPE_FLOAT64) : 0;
+        break;
+      case ML_AGG_MIN: case ML_AGG_MAX: {
+        const bool mn = d.func == ML_AGG_MIN;
+        if (d.arg_type == ML_TYPE_INT32 || d.arg_type == ML_TYPE_INT64 || d.arg_type == ML_TYPE_DATE) {
+          o.out_type = d.arg_type;
+          o.finish = d.arg_type == ML_TYPE_INT64 ? FIN_COPY : FIN_I32;
+          o.state = add_slot(mn ? ACC_MIN_I64 : ACC_MAX_I64, ia, ML_TYPE_INT64);
+        } else if (d.arg_type == ML_TYPE_FLOAT64) {
+          o.out_type = ML_TYPE_FLOAT64;
+          o.state = add_slot(mn ? ACC_MIN_F64 : ACC_MAX_F64, ia, ML_TYPE_FLOAT64);
+        } else {
+          fail(ML_ERR_UNSUPPORTED, "min/max of strings");
+        }
+        o.count = nullable ? add_slot(ACC_COUNT, ia, ML_TYPE_INT64) : 0;
+        break;
+      }
+      default: fail(ML_ERR_PLAN, "unknown aggregate function");
+    }
+    A->outs.push_back(o);
+  }
+  if ((int)A->slots.size() > MAX_AGG_SLOTS) fail(ML_ERR_UNSUPPORTED, "too many aggregate states");
+}
+
+// Compile keys/args of one sink call into B.P (shared by hash and direct modes).
+static void compile_agg(ml_hashagg* A, PipelineBuild& B, bool first) {
+  if ((int)A->key_types.size() > MAX_KEYS) fail(ML_ERR_UNSUPPORTED, "more than 4 group keys");
+  B.P.nkeys = (int)A->key_types.size();
+  bool key_nullable = false;
+  std::vector<std::shared_ptr<HostDict>> dicts;
+  for (size_t k = 0; k < A->key_types.size(); k++) {
+    NodePtr n = B.resolve(A->group_raw[k]);
+    std::shared_ptr<HostDict> d;
+    if (A->key_types[k] == ML_TYPE_UTF8) {
+      if (n->kind != ML_EXPR_COLUMN || !n->col->dict) fail(ML_ERR_UNSUPPORTED, "string group keys must be dictionary columns");
+      d = n->col->dict;
+    }
+    dicts.push_back(d);
+    B.P.keys[k] = B.comp.compile_key(n, A->key_types[k]);
+    key_nullable = key_nullable || B.P.keys[k].nullable;
+  }
+  if (first) {
+    A->key_dicts = dicts;
+    A->key_nullable = key_nullable;
+  } else {
+    for (size_t k = 0; k < dicts.size(); k++)
+      if (!same_dict(dicts[k], A->key_dicts[k])) fail(ML_ERR_UNSUPPORTED, "every batch must share the group key dictionaries");
+    if (key_nullable != A->key_nullable) fail(ML_ERR_INTERNAL, "key nullability changed between batches");
+  }
+  B.P.key_nullable = A->key_nullable ? 1 : 0;
+  // argument programs, one per (aggregate, input cast), deduplicated by code
+  std::vector<std::pair<int, int>> arg_keys;  // (agg, cast)
+  std::vector<int> arg_prog;                  // index into B.P.args
+  std::vector<std::pair<int, int>> prog_span;
+  B.P.nargs = 0;
+  B.P.nacc = (int)A->slots.size();
+  for (size_t s = 0; s < A->slots.size(); s++) {
+    const AccSlot& sl = A->slots[s];
+    B.P.acc[s].kind = sl.kind;
+    if (sl.arg < 0) {
+      B.P.acc[s].arg = -1;
+      continue;
+    }
+    const std::pair<int, int> key{sl.arg, A->slot_cast[s]};
+    int found = -1;
+    for (size_t q = 0; q < arg_keys.size(); q++)
+      if (arg_keys[q] == key) found = arg_prog[q];
+    if (found < 0) {
+      const AggDef& d = A->aggs[(size_t)sl.arg];
+      NodePtr x = B.resolve(*d.raw);
+      int want = d.arg_type;
+      if (A->slot_cast[s] >= 0 && A->slot_cast[s] != d.arg_type) {
+        // evaluate_to_column(arg, arg_type), then prepare_input's cast (accumulator.rs:229-236)
+        auto c = std::make_shared<Node>();
+        c->kind = ML_EXPR_CAST;
+        c->type = d.arg_type;
+        c->left = x;
+        x = c;
+        want = A->slot_cast[s];
+      }
+      Prog p = B.comp.compile(x, want);
+      // dedupe identical code
+      for (int q = 0; q < B.P.nargs && found < 0; q++) {
+        const Prog& o = B.P.args[q];
+        if (o.count == p.count && o.nullable == p.nullable &&
+            std::memcmp(&B.ins[(size_t)o.first], &B.ins[(size_t)p.first], sizeof(Instr) * (size_t)p.count) == 0) {
+          found = q;
+          B.ins.resize((size_t)p.first);
+        }
+      }
+      if (found < 0) {
+        if (B.P.nargs >= MAX_AGG_SLOTS) fail(ML_ERR_UNSUPPORTED, "too many aggregate arguments");
+        B.P.args[B.P.nargs] = p;
+        found = B.P.nargs++;
+      }
+      arg_keys.push_back(key);
+      arg_prog.push_back(found);
+    }
+    B.P.acc[s].arg = found;
+  }
+}
+
+static AccSlotList slot_kinds(const ml_hashagg* A) {
+  AccSlotList l{};
+  for (size_t s = 0; s < A->slots.size(); s++) l.kind[s] = A->slots[s].kind;
+  return l;
+}
+
+static void init_acc(ml_hashagg* A, std::vector<Buffer>& acc, uint64_t cap) {
+  acc.clear();
+  for (size_t s = 0; s < A->slots.size(); s++) {
+    acc.emplace_back(A->ctx->arena.get(), cap * 8);
+    launch_fill_u64(acc.back().as<uint64_t>(), acc_init_bits(A->slots[s].kind), (int64_t)cap, A->ctx->stream);
+  }
+}
+
+// Grow the group table to hold `needed` groups (SwissTable::reserve, swiss.rs:219-232).
+static void agg_reserve(ml_hashagg* A, uint64_t needed) {
+  ml_ctx* ctx = A->ctx;
+  if (A->cap && needed * 8 <= A->cap * 7) return;
+  const uint64_t len = A->len_host;
+  const uint64_t ncap = swiss_capacity(std::max(needed, len * 2));
+  Buffer nt(ctx->arena.get(), ncap * (1 + A->nkw) * 8);
+  zero(ctx, nt.ptr, nt.bytes);
+  std::vector<Buffer> nacc;
+  init_acc(A, nacc, ncap);
+  if (A->cap) {
+    AccMove mv{};
+    mv.n = (int)A->slots.size();
+    for (int s = 0; s < mv.n; s++) {
+      mv.src[s] = A->acc[s].ptr;
+      mv.dst[s] = nacc[s].ptr;
+    }
+    launch_agg_rehash(A->table.as<uint64_t>(), A->cap, A->nkw, nt.as<uint64_t>(), ncap - 1, mv, ctx->stream);
+    A->resizes++;
+  }
+  A->table = std::move(nt);
+  A->acc = std::move(nacc);
+  A->cap = ncap;
+}
+
+static uint64_t agg_len(ml_hashagg* A) {
+  if (!A->len_known) {
+    uint64_t n = 0;
+    d2h(A->ctx, &n, A->flags.ptr, 8);
+    A->len_host = n;
+    A->len_known = true;
+  }
+  return A->len_host;
+}
+
+static void agg_reset_table(ml_hashagg* A) {
+  A->table.reset();
+  A->acc.clear();
+  A->cap = 0;
+  A->len_host = 0;
+  A->len_known = true;
+  zero(A->ctx, A->flags.ptr, 16);
+}
+
+// One sink call: run the pipeline in B into the aggregate.
+static void agg_run(ml_hashagg* A, PipelineBuild& B) {
+  ml_ctx* ctx = A->ctx;
+  const bool first = !A->planned;
+  if (first) {
+    plan_agg(A, B);
+    // direct (small-domain) aggregation when every key is a dictionary or boolean
+    bool direct = !A->key_types.empty();
+    int64_t groups = 1;
+    std::vector<int64_t> sizes;
+    for (size_t k = 0; k < A->key_types.size() && direct; k++) {
+      NodePtr n = B.resolve(A->group_raw[k]);
+      std::vector<Instr> tmp;
+      std::vector<Buffer> tt;
+      Compiler c(ctx, tmp, tt);
+      const bool nl = c.compile_key(n, A->key_types[k]).nullable != 0;
+      int64_t dom;
+      if (A->key_types[k] == ML_TYPE_UTF8 && n->kind == ML_EXPR_COLUMN && n->col->dict) dom = (int64_t)n->col->dict->entries.size();
+      else if (A->key_types[k] == ML_TYPE_BOOLEAN) dom = 2;
+      else { direct = false; break; }
+      dom += nl ? 1 : 0;
+      sizes.push_back(std::max<int64_t>(dom, 1));
+      groups *= sizes.back();
+      if (groups > 4096) direct = false;
+    }
+    if (direct && groups * (int64_t)A->slots.size() * 8 > 48 * 1024) direct = false;
+    A->direct = direct;
+    if (direct) {
+      A->ngroups = groups;
+      A->dim_size = sizes;
+      A->dim_stride.assign(sizes.size(), 1);
+      for (int k = (int)sizes.size() - 2; k >= 0; k--) A->dim_stride[k] = A->dim_stride[k + 1] * sizes[k + 1];
+      A->totals = Buffer(ctx->arena.get(), (size_t)groups * A->slots.size() * 8);
+      AccSlotList kinds = slot_kinds(A);
+      std::vector<uint64_t> init((size_t)groups * A->slots.size());
+      for (size_t c = 0; c < init.size(); c++) init[c] = acc_init_bits(kinds.kind[c % A->slots.size()]);
+      h2d(ctx, A->totals.ptr, init.data(), init.size() * 8);
+      hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    }
+    A->flags = Buffer(ctx->arena.get(), 16);
+    zero(ctx, A->flags.ptr, 16);
+    A->planned = true;
+  }
+  compile_agg(A, B, first);
+  if (first) A->nkw = (int)A->key_types.size() + (A->key_nullable ? 1 : 0);
+
+  if (A->direct) {
+    B.P.sink = SINK_DIRECT_AGG;
+    for (size_t k = 0; k < A->dim_size.size(); k++) {
+      B.P.dim_size[k] = A->dim_size[k];
+      B.P.dim_stride[k] = A->dim_stride[k];
+    }
+    B.P.ngroups = (int32_t)A->ngroups;
+    const int grid = B.grid();
+    const int ncells = (int)(A->ngroups * (int64_t)A->slots.size());
+    Buffer partials(ctx->arena.get(), (size_t)grid * ncells * 8);
+    B.P.direct_partials = partials.ptr;
+    B.upload();
+    launch_pipeline(B.P, grid, block_of(ctx), (size_t)ncells * 8, ctx->stream);
+    launch_direct_reduce(partials.as<uint64_t>(), grid, ncells, (int)A->slots.size(), slot_kinds(A),
+                         A->totals.as<uint64_t>(), ctx->stream);
+    return;
+  }
+
+  // hash mode
+  B.P.sink = SINK_HASH_AGG;
+  B.P.agg_nkw = A->nkw;
+  B.P.agg_count = A->flags.as<uint64_t>();
+  B.P.agg_overflow = (uint32_t*)(A->flags.as<uint64_t>() + 1);
+  B.upload();
+  const int64_t total = B.active_rows;
+  auto launch_range = [&](int64_t b, int64_t e, uint64_t fill_limit) {
+    B.P.begin = b;
+    B.P.end = e;
+    B.P.agg_slots = A->table.as<uint64_t>();
+    B.P.agg_mask = A->cap - 1;
+    B.P.agg_fill_limit = fill_limit;
+    for (size_t s = 0; s < A->slots.size(); s++) B.P.acc_data[s] = A->acc[s].ptr;
+    launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+    A->len_known = false;
+  };
+  if (A->sizing == ML_SIZING_PLAIN) {
+    // reserve(len + batch) before every device batch, doubling as needed
+    const int64_t batch = ctx->cfg.batch_rows > 0 ? ctx->cfg.batch_rows : (1 << 21);
+    for (int64_t b = 0; b < total || (b == 0 && total == 0); b += batch) {
+      const int64_t e = std::min(total, b + batch);
+      if (e <= b) break;
+      agg_reserve(A, agg_len(A) + (uint64_t)(e - b));
+      launch_range(b, e, A->cap);
+    }
+  } else {
+    if (!A->cap) {
+      A->len_host = 0;
+      agg_reserve(A, std::max<uint64_t>(1, (uint64_t)A->hint));
+    }
+    launch_range(0, total, A->cap * 7 / 8);
+  }
+}
+
+static uint32_t agg_overflow_bits(ml_hashagg* A) {
+  uint64_t f[2] = {0, 0};
+  d2h(A->ctx, f, A->flags.ptr, 16);
+  A->len_host = f[0];
+  A->len_known = true;
+  return (uint32_t)f[1];
+}
+
+static ml_result* agg_finalize(ml_hashagg* A) {
+  ml_ctx* ctx = A->ctx;
+  auto* r = new ml_result();
+  r->ctx = ctx;
+  const size_t nk = A->key_types.size();
+  if (!A->planned) {  // no input at all: empty result of the right shape
+    for (size_t k = 0; k < nk; k++) {
+      ResultColumn c;
+      c.type = A->key_types[k];
+      r->cols.push_back(std::move(c));
+    }
+    for (auto& d : A->aggs) {
+      ResultColumn c;
+      c.type = (d.func == ML_AGG_COUNT || d.func == ML_AGG_COUNT_STAR) ? ML_TYPE_INT64
+               : (d.func == ML_AGG_AVG ? ML_TYPE_FLOAT64 : (d.func == ML_AGG_SUM && d.arg_type != ML_TYPE_FLOAT64 ? ML_TYPE_INT64 : d.arg_type));
+      r->cols.push_back(std::move(c));
+    }
+    r->finish_views();
+    return r;
+  }
+  if (A->direct) {
+    const size_t ns = A->slots.size();
+    std::vector<uint64_t> tot((size_t)A->ngroups * ns);
+    d2h(ctx, tot.data(), A->totals.ptr, tot.size() * 8);
+    std::vector<int64_t> groups;
+    for (int64_t g = 0; g < A->ngroups; g++)
+      if (tot[(size_t)g * ns] > 0) groups.push_back(g);
+    const size_t n = groups.size();
+    r->num_rows = (int64_t)n;
+    for (size_t k = 0; k < nk; k++) {
+      ResultColumn c;
+      c.type = A->key_types[k];
+      c.dict = A->key_dicts[k];
+      const int w = type_width(c.type);
+      std::vector<uint8_t> host(std::max<size_t>(n, 1) * w);
+      std::vector<uint8_t> valid(std::max<size_t>(n, 1), 1);
+      bool any_null = false;
+      for (size_t j = 0; j < n; j++) {
+        int64_t code = (groups[j] / A->dim_stride[k]) % A->dim_size[k];
+        const bool is_null = A->key_nullable && code == A->dim_size[k] - 1;
+        if (is_null) { valid[j] = 0; any_null = true; code = 0; }
+        if (w == 4) { uint32_t v = (uint32_t)code; std::memcpy(&host[j * 4], &v, 4); }
+        else host[j] = (uint8_t)code;
+      }
+      c.data = Buffer(ctx->arena.get(), host.size());
+      h2d(ctx, c.data.ptr, host.data(), host.size());
+      if (any_null) {
+        c.valid = Buffer(ctx->arena.get(), valid.size());
+        h2d(ctx, c.valid.ptr, valid.data(), valid.size());
+      }
+      r->cols.push_back(std::move(c));
+    }
+    for (auto& o : A->outs) {
+      ResultColumn c;
+      c.type = o.out_type;
+      const int w = type_width(c.type);
+      std::vector<uint8_t> host(std::max<size_t>(n, 1) * w);
+      std::vector<uint8_t> valid(std::max<size_t>(n, 1), 1);
+      bool any_null = false;
+      for (size_t j = 0; j < n; j++) {
+        const uint64_t* st = &tot[(size_t)groups[j] * ns];
+        uint64_t v = st[o.state];
+        const int64_t cnt = o.count >= 0 ? (int64_t)st[o.count] : 1;
+        if (o.finish == FIN_AVG) {
+          double s;
+          std::memcpy(&s, &v, 8);
+          const double avg = cnt == 0 ? 0.0 : s / (double)cnt;
+          std::memcpy(&v, &avg, 8);
+        }
+        if (o.count >= 0 && cnt <= 0) { valid[j] = 0; any_null = true; }
+        if (w == 8) std::memcpy(&host[j * 8], &v, 8);
+        else { uint32_t x = (uint32_t)v; std::memcpy(&host[j * 4], &x, 4); }
+      }
+      c.data = Buffer(ctx->arena.get(), host.size());
+      h2d(ctx, c.data.ptr, host.data(), host.size());
+      if (any_null) {
+        c.valid = Buffer(ctx->arena.get(), valid.size());
+        h2d(ctx, c.valid.ptr, valid.data(), valid.size());
+      }
+      r->cols.push_back(std::move(c));
+    }
+    hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    r->finish_views();
+    return r;
+  }
+  // hash mode: compact the occupied slots (ordered by slot) and decode
+  if (agg_overflow_bits(A) & 2u) fail(ML_ERR_CAPACITY, "pre-sized group table overflowed");
+  const uint64_t cap = A->cap;
+  Buffer flags(ctx->arena.get(), std::max<uint64_t>(cap, 1));
+  launch_slot_flags(A->table.as<uint64_t>(), cap, A->nkw, flags.as<uint8_t>(), ctx->stream);
+  const int64_t nb = compact_blocks((int64_t)cap);
+  Buffer bc(ctx->arena.get(), (size_t)std::max<int64_t>(nb, 1) * 4);
+  Buffer offs(ctx->arena.get(), (size_t)(nb + 1) * 8);
+  Buffer idx(ctx->arena.get(), std::max<uint64_t>(A->len_host, 1) * 8);
+  launch_compact(flags.as<uint8_t>(), (int64_t)cap, bc.as<uint32_t>(), offs.as<uint64_t>(), nullptr, idx.ptr, 0,
+                 ctx->stream);
+  uint64_t n = 0;
+  d2h(ctx, &n, offs.as<uint64_t>() + nb, 8);
+  if (n != A->len_host) fail(ML_ERR_INTERNAL, "group count mismatch");
+  r->num_rows = (int64_t)n;
+  EmitPlan E{};
+  E.nkeys = (int)nk;
+  E.key_nullable = A->key_nullable ? 1 : 0;
+  for (size_t k = 0; k < nk; k++) {
+    ResultColumn c;
+    c.type = A->key_types[k];
+    c.dict = A->key_dicts[k];
+    const int w = type_width(c.type);
+    c.data = Buffer(ctx->arena.get(), std::max<uint64_t>(n, 1) * w);
+    if (A->key_nullable) c.valid = Buffer(ctx->arena.get(), std::max<uint64_t>(n, 1));
+    E.key_width[k] = w;
+    E.key_out[k] = c.data.ptr;
+    E.key_valid[k] = c.valid.ptr ? c.valid.as<uint8_t>() : nullptr;
+    r->cols.push_back(std::move(c));
+  }
+  E.nout = (int)A->outs.size();
+  for (size_t a = 0; a < A->outs.size(); a++) {
+    const OutDef& o = A->outs[a];
+    ResultColumn c;
+    c.type = o.out_type;
+    const int w = type_width(c.type);
+    c.data = Buffer(ctx->arena.get(), std::max<uint64_t>(n, 1) * w);
+    if (o.count > 0) c.valid = Buffer(ctx->arena.get(), std::max<uint64_t>(n, 1));
+    E.finish[a] = o.finish;
+    E.out_width[a] = w;
+    E.state[a] = A->acc[(size_t)o.state].ptr;
+    E.count_state[a] = o.count >= 0 ? A->acc[(size_t)o.count].ptr : nullptr;
+    E.out[a] = c.data.ptr;
+    E.out_valid[a] = c.valid.ptr ? c.valid.as<uint8_t>() : nullptr;
+    r->cols.push_back(std::move(c));
+  }
+  launch_agg_emit(A->table.as<uint64_t>(), A->nkw, idx.as<int64_t>(), (int64_t)n, E, ctx->stream);
+  hip_check(hipStreamSynchronize(ctx->stream), "sync");
+  r->finish_views();
+  return r;
+}
+
+}  // namespace mlg
+
+// ---------------------------------------------------------------- join build
+struct ml_join_build {
+  ml_ctx* ctx = nullptr;
+  std::vector<ml_expr> keys_raw;
+  std::vector<int> key_types;
+  std::vector<std::vector<ml_expr_node>> node_store;
+  std::deque<std::string> str_store;
+  bool schema_fixed = false;
+  std::vector<int> col_types;
+  std::vector<std::shared_ptr<mlg::HostDict>> col_dicts;
+  std::vector<std::shared_ptr<mlg::HostDict>> key_dicts;
+  std::vector<bool> col_nullable;
+  struct Chunk {
+    std::vector<mlg::Buffer> cols, valid;
+    mlg::Buffer keys, key_null;
+    int64_t rows = 0;
+  };
+  std::vector<Chunk> chunks;
+};
+
+namespace mlg {
+
+static void join_build_run(ml_join_build* J, PipelineBuild& B) {
+  ml_ctx* ctx = J->ctx;
+  if ((int)J->key_types.size() > MAX_KEYS) fail(ML_ERR_UNSUPPORTED, "more than 4 join keys");
+  if ((int)B.layout.size() > MAX_COLS) fail(ML_ERR_UNSUPPORTED, "too many build columns");
+  // schema of the build side
+  std::vector<int> types;
+  std::vector<std::shared_ptr<HostDict>> dicts;
+  for (const VCol* v : B.layout) {
+    types.push_back(v->type);
+    dicts.push_back(v->dict);
+  }
+  std::vector<std::shared_ptr<HostDict>> kd;
+  B.P.nkeys = (int)J->key_types.size();
+  for (size_t k = 0; k < J->key_types.size(); k++) {
+    NodePtr n = B.resolve(J->keys_raw[k]);
+    std::shared_ptr<HostDict> d;
+    if (J->key_types[k] == ML_TYPE_UTF8) {
+      if (n->kind != ML_EXPR_COLUMN || !n->col->dict) fail(ML_ERR_UNSUPPORTED, "string join keys must be dictionary columns");
+      d = n->col->dict;
+    }
+    kd.push_back(d);
+    B.P.keys[k] = B.comp.compile_key(n, J->key_types[k]);
+  }
+  if (!J->schema_fixed) {
+    J->col_types = types;
+    J->col_dicts = dicts;
+    J->key_dicts = kd;
+    J->schema_fixed = true;
+  } else {
+    if (types != J->col_types) fail(ML_ERR_INTERNAL, "build batches with different schemas");
+    for (size_t c = 0; c < dicts.size(); c++)
+      if (!same_dict(dicts[c], J->col_dicts[c])) fail(ML_ERR_UNSUPPORTED, "build batches must share dictionaries");
+  }
+  uint64_t* counters = ctx->counters.as<uint64_t>();
+  uint64_t cap = (uint64_t)B.active_rows;
+  if (B.expanding) {
+    B.P.sink = SINK_COUNT_PAIRS;
+    B.P.out_count = counters;
+    zero(ctx, counters, 16);
+    B.upload();
+    launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+    uint64_t n = 0;
+    d2h(ctx, &n, counters, 8);
+    cap = n;
+  }
+  ml_join_build::Chunk ch;
+  B.P.ncols = (int)B.layout.size();
+  std::vector<bool> nullable(B.layout.size());
+  for (size_t c = 0; c < B.layout.size(); c++) {
+    bool nl = false;
+    B.P.cols[c] = column_prog(B, B.layout[c], &nl);
+    nullable[c] = nl;
+    const int w = type_width(B.layout[c]->type);
+    B.P.col_width[c] = w;
+    ch.cols.emplace_back(ctx->arena.get(), std::max<uint64_t>(cap, 1) * w);
+    ch.valid.emplace_back();
+    if (nl) ch.valid.back() = Buffer(ctx->arena.get(), std::max<uint64_t>(cap, 1));
+    B.P.col_out[c] = ch.cols.back().ptr;
+    B.P.col_valid[c] = ch.valid.back().ptr ? ch.valid.back().as<uint8_t>() : nullptr;
+  }
+  if (J->col_nullable.empty()) J->col_nullable = nullable;
+  else
+    for (size_t c = 0; c < nullable.size(); c++) J->col_nullable[c] = J->col_nullable[c] || nullable[c];
+  ch.keys = Buffer(ctx->arena.get(), std::max<uint64_t>(cap, 1) * J->key_types.size() * 8);
+  ch.key_null = Buffer(ctx->arena.get(), std::max<uint64_t>(cap, 1));
+  B.P.key_out = ch.keys.as<uint64_t>();
+  B.P.key_null = ch.key_null.as<uint8_t>();
+  B.P.sink = SINK_JOIN_BUILD;
+  B.P.out_count = counters;
+  B.P.out_capacity = cap;
+  B.P.out_overflow = (uint32_t*)(counters + 1);
+  zero(ctx, counters, 16);
+  B.upload();
+  launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+  uint64_t hc[2] = {0, 0};
+  d2h(ctx, hc, counters, 16);
+  if (hc[1] & 1) fail(ML_ERR_INTERNAL, "join build output overflowed");
+  ch.rows = (int64_t)hc[0];
+  J->chunks.push_back(std::move(ch));
+}
+
+static ml_join_table* join_build_finalize(ml_join_build* J) {
+  ml_ctx* ctx = J->ctx;
+  auto* t = new ml_join_table();
+  t->ctx = ctx;
+  JoinTableImpl& T = t->impl;
+  T.key_types = J->key_types;
+  T.key_dicts = J->key_dicts;
+  T.col_types = J->col_types;
+  T.col_dicts = J->col_dicts;
+  if (T.key_dicts.size() < T.key_types.size()) T.key_dicts.resize(T.key_types.size());
+  int64_t n = 0;
+  for (auto& c : J->chunks) n += c.rows;
+  T.nrows = n;
+  const size_t nk = J->key_types.size();
+  Buffer keys, key_null;
+  if (J->chunks.size() == 1) {
+    auto& c = J->chunks[0];
+    T.cols = std::move(c.cols);
+    T.col_valid = std::move(c.valid);
+    keys = std::move(c.keys);
+    key_null = std::move(c.key_null);
+  } else {
+    // Batch::concat of the collected batches (batch.rs:168-184)
+    for (size_t ci = 0; ci < T.col_types.size(); ci++) {
+      const int w = type_width(T.col_types[ci]);
+      T.cols.emplace_back(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1) * w);
+      const bool nl = ci < J->col_nullable.size() && J->col_nullable[ci];
+      T.col_valid.emplace_back();
+      if (nl) {
+        T.col_valid.back() = Buffer(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1));
+        launch_fill_u64(T.col_valid.back().as<uint64_t>(), 0x0101010101010101ull,
+                        (std::max<int64_t>(n, 1) + 7) / 8, ctx->stream);
+      }
+      int64_t off = 0;
+      for (auto& c : J->chunks) {
+        d2d(ctx, T.cols[ci].as<uint8_t>() + off * w, c.cols[ci].ptr, (size_t)c.rows * w);
+        if (nl && c.valid[ci].ptr) d2d(ctx, T.col_valid[ci].as<uint8_t>() + off, c.valid[ci].ptr, (size_t)c.rows);
+        off += c.rows;
+      }
+    }
+    keys = Buffer(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1) * nk * 8);
+    key_null = Buffer(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1));
+    int64_t off = 0;
+    for (auto& c : J->chunks) {
+      d2d(ctx, keys.as<uint64_t>() + off * nk, c.keys.ptr, (size_t)c.rows * nk * 8);
+      d2d(ctx, key_null.as<uint8_t>() + off, c.key_null.ptr, (size_t)c.rows);
+      off += c.rows;
+    }
+  }
+  J->chunks.clear();
+  if (T.col_valid.size() < T.cols.size()) T.col_valid.resize(T.cols.size());
+  // open-addressing table at load <= 3/4
+  const uint64_t cap = std::max<uint64_t>(16, next_pow2((uint64_t)((n * 4 + 2) / 3)));
+  T.dev.nkw = (int)nk;
+  T.dev.mask = cap - 1;
+  T.slots = Buffer(ctx->arena.get(), cap * (1 + nk) * 8);
+  zero(ctx, T.slots.ptr, T.slots.bytes);
+  const int use_bloom = ctx->cfg.use_bloom;
+  const bool bloom = use_bloom > 0 || (use_bloom < 0 && n >= (1 << 16));
+  if (bloom) {
+    const uint64_t words = std::max<uint64_t>(64, next_pow2((uint64_t)n / 8 + 1));
+    T.bloom = Buffer(ctx->arena.get(), words * 8);
+    zero(ctx, T.bloom.ptr, T.bloom.bytes);
+    T.dev.bloom = T.bloom.as<uint64_t>();
+    T.dev.bloom_mask = words - 1;
+  }
+  T.dev.slots = T.slots.as<uint64_t>();
+  launch_join_insert(T.dev, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, T.bloom.as<uint64_t>(), ctx->stream);
+  uint32_t* dup = (uint32_t*)ctx->counters.as<uint64_t>();
+  zero(ctx, dup, 8);
+  launch_join_unique(T.dev, dup, ctx->stream);
+  uint32_t hdup = 0;
+  d2h(ctx, &hdup, dup, 4);
+  T.dev.unique = hdup ? 0 : 1;
+  return t;
+}
+
+}  // namespace mlg
+
+// -------------------------------------------------------------------- TopN
+struct ml_topn {
+  ml_ctx* ctx = nullptr;
+  std::vector<ml_expr> keys_raw;
+  std::vector<int> asc, nulls_first;
+  int64_t limit = 0;
+  std::vector<std::vector<ml_expr_node>> node_store;
+  std::deque<std::string> str_store;
+  bool schema_fixed = false;
+  mlg::Materialized best;  // the best `limit` rows seen so far
+};
+
+namespace mlg {
+
+// Sort a materialized table's rows by the TopN keys and keep the first `limit`.
+static void topn_reduce(ml_topn* T, Materialized& m) {
+  ml_ctx* ctx = T->ctx;
+  const int64_t n = m.rows;
+  const int64_t L = T->limit;
+  if (n == 0 || L == 0) {
+    m.rows = 0;
+    return;
+  }
+  // layout over the materialized columns
+  PipelineBuild B(ctx);
+  for (size_t c = 0; c < m.types.size(); c++) {
+    VCol v;
+    v.type = m.types[c];
+    v.data = m.cols[c].ptr;
+    v.valid = m.valid[c].ptr ? m.valid[c].as<uint8_t>() : nullptr;
+    v.dict = m.dicts[c];
+    B.layout.push_back(B.add_col(std::move(v)));
+  }
+  SortKeys K{};
+  K.nkeys = (int)T->keys_raw.size();
+  if (K.nkeys > MAX_SORT_KEYS) fail(ML_ERR_UNSUPPORTED, "more than 4 sort keys");
+  int nw = 0;
+  for (int k = 0; k < K.nkeys; k++) {
+    NodePtr e = B.resolve(T->keys_raw[(size_t)k]);
+    const int t = B.comp.type_of(e);
+    Prog p;
+    if (t == ML_TYPE_UTF8) {
+      // strings sort bytewise: rank of each dictionary entry
+      if (e->kind != ML_EXPR_COLUMN || !e->col->dict) fail(ML_ERR_UNSUPPORTED, "string sort keys must be dictionary columns");
+      const auto& ent = e->col->dict->entries;
+      std::vector<uint32_t> order(ent.size());
+      std::iota(order.begin(), order.end(), 0u);
+      std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return ent[a] < ent[b]; });
+      std::vector<uint64_t> rank(ent.size() + 1, 0);
+      for (size_t r = 0; r < order.size(); r++) {
+        uint64_t rk = r;
+        if (r > 0 && ent[order[r]] == ent[order[r - 1]]) rk = rank[order[r - 1]];
+        rank[order[r]] = rk;
+      }
+      Buffer tb(ctx->arena.get(), rank.size() * 8);
+      h2d(ctx, tb.ptr, rank.data(), rank.size() * 8);
+      hip_check(hipStreamSynchronize(ctx->stream), "sync");
+      p = B.comp.compile(e, -1);
+      Instr in{};
+      in.op = OP_LOOKUP_U64;
+      in.depth = 1;
+      in.imm = (uint64_t)(uintptr_t)tb.as<uint64_t>();
+      B.ins.push_back(in);
+      p.count++;
+      B.tables.push_back(std::move(tb));
+      K.kind[k] = SK_UNSIGNED;
+    } else {
+      p = B.comp.compile(e, -1);
+      K.kind[k] = t == ML_TYPE_FLOAT64 ? SK_F64 : (t == ML_TYPE_BOOLEAN ? SK_UNSIGNED : SK_INT);
+    }
+    K.prog[k] = p;
+    K.asc[k] = T->asc[(size_t)k];
+    K.nulls_first[k] = T->nulls_first[(size_t)k];
+    K.nullable[k] = p.nullable;
+    nw += p.nullable ? 2 : 1;
+  }
+  K.nwords = nw;
+  if (nw + 1 > 6) fail(ML_ERR_UNSUPPORTED, "sort keys too wide");
+  if (L > 256) fail(ML_ERR_UNSUPPORTED, "TopN limit above 256");
+  B.upload();
+  const int W = nw + 1;
+  Buffer words(ctx->arena.get(), (size_t)n * W * 8);
+  launch_sort_keys(B.P, K, n, words.as<uint64_t>(), ctx->stream);
+  // repeated tile selection until one tile remains
+  int64_t cur = n;
+  Buffer a = std::move(words);
+  for (;;) {
+    const int64_t tiles = (cur + TOPN_TILE - 1) / TOPN_TILE;
+    const int keep = (int)std::min<int64_t>(L, TOPN_TILE);
+    Buffer b(ctx->arena.get(), (size_t)tiles * keep * W * 8);
+    launch_topn_tiles(a.as<uint64_t>(), cur, nw, keep, b.as<uint64_t>(), ctx->stream);
+    a = std::move(b);
+    cur = tiles * keep;
+    if (tiles == 1) break;
+  }
+  const int64_t m_out = std::min<int64_t>(L, n);
+  Buffer rows(ctx->arena.get(), (size_t)m_out * 8);
+  launch_topn_rows(a.as<uint64_t>(), nw, m_out, rows.as<int64_t>(), ctx->stream);
+  // gather every column of the winners
+  for (size_t c = 0; c < m.types.size(); c++) {
+    const int w = type_width(m.types[c]);
+    Buffer d(ctx->arena.get(), (size_t)std::max<int64_t>(m_out, 1) * w);
+    Buffer v;
+    if (m.valid[c].ptr) v = Buffer(ctx->arena.get(), (size_t)std::max<int64_t>(m_out, 1));
+    launch_gather(m.cols[c].ptr, w, rows.as<int64_t>(), m_out, d.ptr, m.valid[c].as<uint8_t>(),
+                  v.ptr ? v.as<uint8_t>() : nullptr, ctx->stream);
+    m.cols[c] = std::move(d);
+    m.valid[c] = std::move(v);
+  }
+  hip_check(hipStreamSynchronize(ctx->stream), "sync");
+  m.rows = m_out;
+}
+
+static void topn_run(ml_topn* T, PipelineBuild& B) {
+  Materialized m;
+  run_collect(B, m, true, nullptr, 0);
+  if (!T->schema_fixed) {
+    T->best.types = m.types;
+    T->best.dicts = m.dicts;
+    T->best.cols.resize(m.types.size());
+    T->best.valid.resize(m.types.size());
+    T->best.rows = 0;
+    T->schema_fixed = true;
+  } else if (m.types != T->best.types) {
+    fail(ML_ERR_INTERNAL, "TopN batches with different schemas");
+  }
+  topn_reduce(T, m);
+  // merge with the best rows so far: concatenate, then reduce again
+  Materialized& b = T->best;
+  if (b.rows == 0) {
+    b = std::move(m);
+    return;
+  }
+  ml_ctx* ctx = T->ctx;
+  Materialized cat;
+  cat.types = b.types;
+  cat.dicts = b.dicts;
+  cat.rows = b.rows + m.rows;
+  for (size_t c = 0; c < b.types.size(); c++) {
+    const int w = type_width(b.types[c]);
+    Buffer d(ctx->arena.get(), (size_t)cat.rows * w);
+    d2d(ctx, d.ptr, b.cols[c].ptr, (size_t)b.rows * w);
+    d2d(ctx, d.as<uint8_t>() + b.rows * w, m.cols[c].ptr, (size_t)m.rows * w);
+    Buffer v;
+    if (b.valid[c].ptr || m.valid[c].ptr) {
+      v = Buffer(ctx->arena.get(), (size_t)cat.rows);
+      launch_fill_u64(v.as<uint64_t>(), 0x0101010101010101ull, (cat.rows + 7) / 8, ctx->stream);
+      if (b.valid[c].ptr) d2d(ctx, v.ptr, b.valid[c].ptr, (size_t)b.rows);
+      if (m.valid[c].ptr) d2d(ctx, v.as<uint8_t>() + b.rows, m.valid[c].ptr, (size_t)m.rows);
+    }
+    cat.cols.push_back(std::move(d));
+    cat.valid.push_back(std::move(v));
+  }
+  topn_reduce(T, cat);
+  b = std::move(cat);
+}
+
+}  // namespace mlg
+
+// ------------------------------------------------------------------ results
+void ml_result::finish_views() {
+  views.clear();
+  for (auto& c : cols) {
+    ml_column v{};
+    v.type = c.type;
+    v.data = c.data.ptr;
+    v.len = num_rows;
+    v.validity = nullptr;
+    v.dict = c.dict ? &c.dict->view : nullptr;
+    views.push_back(v);
+  }
+}
+
+std::shared_ptr<mlg::HostDict> ml_ctx::dict(const ml_dict* d) {
+  auto it = dicts.find(d);
+  auto h = mlg::HostDict::from(d);
+  if (it != dicts.end() && it->second->entries == h->entries) return it->second;
+  dicts[d] = h;
+  return h;
+}
+
+// ===================================================================== C ABI
+using namespace mlg;
+
+template <class F>
+static ml_status guarded(ml_ctx* ctx, F f) {
+  try {
+    f();
+    if (ctx) ctx->last_error.clear();
+    return ML_OK;
+  } catch (const Error& e) {
+    if (ctx) ctx->last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (ctx) ctx->last_error = "host allocation failed";
+    return ML_ERR_RESOURCES_EXHAUSTED;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->last_error = e.what();
+    return ML_ERR_INTERNAL;
+  }
+}
+
+template <class Obj>
+static ml_expr own_expr(Obj* o, const ml_expr& e) {
+  std::vector<ml_expr_node> nodes(e.nodes, e.nodes + e.num_nodes);
+  for (auto& n : nodes)
+    if (n.kind == ML_EXPR_LITERAL && n.type == ML_TYPE_UTF8) o->str_store.emplace_back(n.str ? n.str : "", (size_t)n.str_len);
+  o->node_store.push_back(std::move(nodes));
+  auto& st = o->node_store.back();
+  // string literals point at owned copies (a deque never moves its elements)
+  size_t si = o->str_store.size();
+  for (auto it = st.rbegin(); it != st.rend(); ++it)
+    if (it->kind == ML_EXPR_LITERAL && it->type == ML_TYPE_UTF8) it->str = o->str_store[--si].c_str();
+  ml_expr out = e;
+  out.nodes = st.data();
+  return out;
+}
+
+extern "C" {
+
+void ml_default_config(ml_exec_config* cfg) {
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->batch_rows = 1 << 21;
+  cfg->device = 0;
+  cfg->block_size = 256;
+  cfg->grid_per_cu = 8;
+  cfg->use_bloom = -1;
+  cfg->stream = nullptr;
+}
+
+ml_status ml_ctx_create(const ml_exec_config* cfg, ml_ctx** out) {
+  auto* ctx = new ml_ctx();
+  ml_status st = guarded(nullptr, [&] {
+    if (cfg) ctx->cfg = *cfg;
+    else ml_default_config(&ctx->cfg);
+    if (ctx->cfg.batch_rows <= 0) ctx->cfg.batch_rows = 1 << 21;
+    ctx->device = ctx->cfg.device;
+    int ndev = 0;
+    hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+    if (ctx->device < 0 || ctx->device >= ndev) fail(ML_ERR_EXECUTION, "no such HIP device");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, ctx->device), "hipGetDeviceProperties");
+    ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (ctx->cfg.stream) {
+      ctx->stream = (hipStream_t)ctx->cfg.stream;
+    } else {
+      hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+      ctx->own_stream = true;
+    }
+    ctx->arena = std::make_unique<DeviceArena>(ctx->device);
+    ctx->counters = Buffer(ctx->arena.get(), 64 * 8);
+    zero(ctx, ctx->counters.ptr, 64 * 8);
+    hip_check(hipStreamSynchronize(ctx->stream), "sync");
+  });
+  if (st != ML_OK) {
+    delete ctx;
+    *out = nullptr;
+    return st;
+  }
+  *out = ctx;
+  return ML_OK;
+}
+
+void ml_ctx_destroy(ml_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->counters.reset();
+  ctx->arena.reset();
+  if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* ml_last_error(const ml_ctx* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+ml_status ml_ctx_synchronize(ml_ctx* ctx) {
+  return guarded(ctx, [&] { hip_check(hipStreamSynchronize(ctx->stream), "synchronize"); });
+}
+
+void* ml_ctx_stream(ml_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int64_t ml_result_num_rows(const ml_result* r) { return r ? r->num_rows : 0; }
+int32_t ml_result_num_columns(const ml_result* r) { return r ? (int32_t)r->cols.size() : 0; }
+const ml_column* ml_result_column(const ml_result* r, int32_t i) {
+  if (!r || i < 0 || (size_t)i >= r->views.size()) return nullptr;
+  return &r->views[(size_t)i];
+}
+
+ml_status ml_result_copy_column(ml_ctx* ctx, const ml_result* r, int32_t i, void* dst) {
+  return guarded(ctx, [&] {
+    if (!r || i < 0 || (size_t)i >= r->cols.size()) fail(ML_ERR_INTERNAL, "no such result column");
+    const auto& c = r->cols[(size_t)i];
+    d2h(ctx, dst, c.data.ptr, (size_t)r->num_rows * type_width(c.type));
+  });
+}
+
+ml_status ml_result_copy_validity(ml_ctx* ctx, const ml_result* r, int32_t i, uint8_t* dst) {
+  return guarded(ctx, [&] {
+    if (!r || i < 0 || (size_t)i >= r->cols.size()) fail(ML_ERR_INTERNAL, "no such result column");
+    const auto& c = r->cols[(size_t)i];
+    if (c.valid.ptr) d2h(ctx, dst, c.valid.ptr, (size_t)r->num_rows);
+    else std::memset(dst, 1, (size_t)r->num_rows);
+  });
+}
+
+void ml_result_free(ml_result* r) {
+  if (!r) return;
+  if (r->ctx) (void)hipSetDevice(r->ctx->device);
+  delete r;
+}
+
+ml_status ml_expr_type(ml_ctx* ctx, const ml_expr* expr, const ml_batch* batch, int32_t* out_type) {
+  return guarded(ctx, [&] {
+    PipelineBuild B(ctx);
+    B.source(batch);
+    *out_type = B.comp.type_of(B.resolve(*expr));
+  });
+}
+
+ml_status ml_filter(ml_ctx* ctx, const ml_expr* predicate, const ml_batch* in, uint32_t* out_sel, int64_t* out_len) {
+  return guarded(ctx, [&] {
+    PipelineBuild B(ctx);
+    B.source(in);
+    ml_operator op{};
+    op.kind = ML_PIPE_FILTER;
+    op.predicate = *predicate;
+    B.add_ops(&op, 1);
+    const int64_t n = B.active_rows;
+    Buffer flags(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1));
+    B.P.sink = SINK_FLAGS;
+    B.P.flag_out = flags.as<uint8_t>();
+    B.upload();
+    if (n > 0) launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+    const int64_t nb = compact_blocks(n);
+    Buffer bc(ctx->arena.get(), (size_t)std::max<int64_t>(nb, 1) * 4);
+    Buffer offs(ctx->arena.get(), (size_t)(nb + 1) * 8);
+    launch_compact(flags.as<uint8_t>(), n, bc.as<uint32_t>(), offs.as<uint64_t>(), B.P.sel, out_sel, 1, ctx->stream);
+    uint64_t total = 0;
+    d2h(ctx, &total, offs.as<uint64_t>() + nb, 8);
+    *out_len = (int64_t)total;
+  });
+}
+
+ml_status ml_project(ml_ctx* ctx, const ml_expr* exprs, int32_t num_exprs, const ml_batch* in, ml_result** out) {
+  return guarded(ctx, [&] {
+    PipelineBuild B(ctx);
+    B.source(in, false);  // evaluate over every physical row (eval.rs:61-66)
+    if (num_exprs > MAX_COLS) fail(ML_ERR_UNSUPPORTED, "too many expressions");
+    Materialized m;
+    const int64_t n = in->num_rows;
+    m.rows = n;
+    B.P.ncols = num_exprs;
+    for (int c = 0; c < num_exprs; c++) {
+      NodePtr e = B.resolve(exprs[c]);
+      int t = B.comp.type_of(e);
+      if (t == ML_TYPE_NULL) t = ML_TYPE_INT64;
+      if (t == ML_TYPE_UTF8) fail(ML_ERR_UNSUPPORTED, "string projections");
+      Prog p = B.comp.compile(e, t);
+      B.P.cols[c] = p;
+      const int w = type_width(t);
+      B.P.col_width[c] = w;
+      m.types.push_back(t);
+      m.dicts.push_back(nullptr);
+      m.cols.emplace_back(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1) * w);
+      m.valid.emplace_back();
+      if (p.nullable) m.valid.back() = Buffer(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1));
+      B.P.col_out[c] = m.cols.back().ptr;
+      B.P.col_valid[c] = m.valid.back().ptr ? m.valid.back().as<uint8_t>() : nullptr;
+    }
+    B.P.sink = SINK_PROJECT;
+    B.upload();
+    if (n > 0) launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+    hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    *out = make_result(ctx, m);
+  });
+}
+
+ml_status ml_hash_columns(ml_ctx* ctx, const ml_expr* keys, int32_t num_keys, const ml_batch* in, uint64_t* out_hashes) {
+  return guarded(ctx, [&] {
+    PipelineBuild B(ctx);
+    B.source(in);
+    if (num_keys > MAX_KEYS) fail(ML_ERR_UNSUPPORTED, "more than 4 key columns");
+    B.P.nkeys = num_keys;
+    B.P.key_prehashed = 0;
+    for (int k = 0; k < num_keys; k++) {
+      bool pre = false;
+      B.P.keys[k] = B.comp.compile_hash_input(B.resolve(keys[k]), &pre);
+      if (pre) B.P.key_prehashed |= 1 << k;
+    }
+    B.P.sink = SINK_HASH;
+    B.P.hash_out = out_hashes;
+    B.upload();
+    if (B.active_rows > 0) launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+    hip_check(hipStreamSynchronize(ctx->stream), "sync");
+  });
+}
+
+// ------------------------------------------------------------- hash aggregate
+ml_status ml_hashagg_create(ml_ctx* ctx, const ml_agg_spec* spec, ml_hashagg** out) {
+  auto* A = new ml_hashagg();
+  A->ctx = ctx;
+  ml_status st = guarded(ctx, [&] {
+    if (spec->num_keys < 0 || spec->num_aggs < 0) fail(ML_ERR_PLAN, "bad aggregate spec");
+    if (spec->mode != ML_AGG_MODE_SINGLE) fail(ML_ERR_UNSUPPORTED, "partial/final aggregation modes on the GPU");
+    for (int k = 0; k < spec->num_keys; k++) {
+      A->group_raw.push_back(own_expr(A, spec->group_exprs[k]));
+      parse_expr(A->group_raw.back());
+      A->key_types.push_back(spec->key_types[k]);
+    }
+    for (int a = 0; a < spec->num_aggs; a++) {
+      AggDef d;
+      d.func = spec->aggs[a].func;
+      d.arg_type = spec->aggs[a].arg_type;
+      d.raw = nullptr;
+      if (d.func != ML_AGG_COUNT_STAR) {
+        A->group_raw.push_back(own_expr(A, spec->aggs[a].arg));
+        d.arg = parse_expr(A->group_raw.back());
+      }
+      A->aggs.push_back(d);
+    }
+    // point each aggregate at its stored argument (after all push_backs)
+    size_t gi = (size_t)spec->num_keys;
+    for (auto& d : A->aggs)
+      if (d.func != ML_AGG_COUNT_STAR) d.raw = &A->group_raw[gi++];
+    A->group_raw.resize((size_t)spec->num_keys + (gi - (size_t)spec->num_keys));
+    A->sizing = spec->sizing;
+    A->hint = spec->capacity_hint;
+    if (A->sizing != ML_SIZING_PLAIN && A->hint <= 0) A->hint = 1;
+  });
+  if (st != ML_OK) {
+    delete A;
+    *out = nullptr;
+    return st;
+  }
+  *out = A;
+  return ML_OK;
+}
+
+ml_status ml_hashagg_sink(ml_hashagg* agg, const ml_batch* batch) {
+  return guarded(agg->ctx, [&] {
+    PipelineBuild B(agg->ctx);
+    B.source(batch);
+    agg_run(agg, B);
+  });
+}
+
+ml_status ml_pipeline_hashagg(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops, int32_t num_ops,
+                              ml_hashagg* sink) {
+  return guarded(ctx, [&] {
+    const bool fresh = !sink->planned;
+    {
+      PipelineBuild B(ctx);
+      B.source(source);
+      B.add_ops(ops, num_ops);
+      agg_run(sink, B);
+    }
+    if (!sink->direct && sink->sizing != ML_SIZING_PLAIN) {
+      // a pre-sized table that filled up lost rows: the learned path reruns
+      // this batch with the plain resize-on-overflow sizing
+      if (agg_overflow_bits(sink) & 2u) {
+        if (sink->sizing == ML_SIZING_HINT) fail(ML_ERR_CAPACITY, "pre-sized group table overflowed");
+        if (!fresh) fail(ML_ERR_CAPACITY, "learned-size table overflowed after earlier batches");
+        agg_reset_table(sink);
+        sink->sizing = ML_SIZING_PLAIN;
+        sink->resizes = 0;
+        PipelineBuild B(ctx);
+        B.source(source);
+        B.add_ops(ops, num_ops);
+        agg_run(sink, B);
+      }
+    }
+  });
+}
+
+ml_status ml_hashagg_finalize(ml_hashagg* agg, ml_result** out) {
+  return guarded(agg->ctx, [&] { *out = agg_finalize(agg); });
+}
+
+ml_status ml_hashagg_num_groups(ml_hashagg* agg, int64_t* out) {
+  return guarded(agg->ctx, [&] {
+    if (agg->direct) {
+      const size_t ns = agg->slots.size();
+      std::vector<uint64_t> tot((size_t)agg->ngroups * ns);
+      d2h(agg->ctx, tot.data(), agg->totals.ptr, tot.size() * 8);
+      int64_t n = 0;
+      for (int64_t g = 0; g < agg->ngroups; g++) n += tot[(size_t)g * ns] > 0;
+      *out = n;
+    } else {
+      agg->len_known = false;
+      *out = agg->planned ? (int64_t)agg_len(agg) : 0;
+    }
+  });
+}
+
+ml_status ml_hashagg_table_stats(ml_hashagg* agg, int64_t* capacity, int32_t* resizes) {
+  return guarded(agg->ctx, [&] {
+    *capacity = agg->direct ? agg->ngroups : (int64_t)agg->cap;
+    *resizes = agg->resizes;
+  });
+}
+
+void ml_hashagg_free(ml_hashagg* agg) { delete agg; }
+
+// ------------------------------------------------------------------ join
+ml_status ml_join_build_create(ml_ctx* ctx, const ml_join_build_spec* spec, ml_join_build** out) {
+  auto* J = new ml_join_build();
+  J->ctx = ctx;
+  ml_status st = guarded(ctx, [&] {
+    for (int k = 0; k < spec->num_keys; k++) {
+      J->keys_raw.push_back(own_expr(J, spec->keys[k]));
+      parse_expr(J->keys_raw.back());
+      J->key_types.push_back(spec->key_types[k]);
+      if (spec->key_types[k] < ML_TYPE_BOOLEAN || spec->key_types[k] > ML_TYPE_UTF8) fail(ML_ERR_PLAN, "bad key type");
+    }
+  });
+  if (st != ML_OK) {
+    delete J;
+    *out = nullptr;
+    return st;
+  }
+  *out = J;
+  return ML_OK;
+}
+
+ml_status ml_join_build_sink(ml_join_build* b, const ml_batch* batch) {
+  return guarded(b->ctx, [&] {
+    PipelineBuild B(b->ctx);
+    B.source(batch);
+    join_build_run(b, B);
+  });
+}
+
+ml_status ml_pipeline_join_build(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops, int32_t num_ops,
+                                 ml_join_build* sink) {
+  return guarded(ctx, [&] {
+    PipelineBuild B(ctx);
+    B.source(source);
+    B.add_ops(ops, num_ops);
+    join_build_run(sink, B);
+  });
+}
+
+ml_status ml_join_build_finalize(ml_join_build* b, ml_join_table** out) {
+  ml_ctx* ctx = b->ctx;
+  ml_status st = guarded(ctx, [&] {
+    if (!b->schema_fixed) fail(ML_ERR_EXECUTION, "join build received no batch");
+    *out = join_build_finalize(b);
+  });
+  delete b;
+  return st;
+}
+
+int64_t ml_join_table_num_rows(const ml_join_table* t) { return t ? t->impl.nrows : 0; }
+int32_t ml_join_table_unique(const ml_join_table* t) { return t ? t->impl.dev.unique : 0; }
+void ml_join_table_free(ml_join_table* t) { delete t; }
+
+ml_status ml_join_probe(ml_ctx* ctx, const ml_join_table* t, const ml_expr* probe_keys, int32_t num_keys,
+                        const ml_batch* probe, ml_result** out_pairs) {
+  return guarded(ctx, [&] {
+    PipelineBuild B(ctx);
+    B.source(probe);
+    ml_operator op{};
+    op.kind = ML_PIPE_PROBE;
+    op.table = t;
+    op.keys = probe_keys;
+    op.num_keys = num_keys;
+    B.add_ops(&op, 1);
+    Materialized m;
+    Buffer pairs;
+    run_collect(B, m, false, &pairs, 2);
+    // split the (probe row, build row) pairs into two INT64 columns
+    Materialized o;
+    o.rows = m.rows;
+    for (int c = 0; c < 2; c++) {
+      o.types.push_back(ML_TYPE_INT64);
+      o.dicts.push_back(nullptr);
+      o.cols.emplace_back(ctx->arena.get(), (size_t)std::max<int64_t>(m.rows, 1) * 8);
+      o.valid.emplace_back();
+    }
+    if (m.rows > 0) {
+      hip_check(hipMemcpy2DAsync(o.cols[0].ptr, 8, pairs.ptr, 16, 8, (size_t)m.rows, hipMemcpyDeviceToDevice, ctx->stream), "split");
+      hip_check(hipMemcpy2DAsync(o.cols[1].ptr, 8, pairs.as<uint8_t>() + 8, 16, 8, (size_t)m.rows, hipMemcpyDeviceToDevice,
+                                 ctx->stream), "split");
+    }
+    hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    *out_pairs = make_result(ctx, o);
+  });
+}
+
+ml_status ml_pipeline_collect(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops, int32_t num_ops,
+                              ml_result** out) {
+  return guarded(ctx, [&] {
+    PipelineBuild B(ctx);
+    B.source(source);
+    B.add_ops(ops, num_ops);
+    Materialized m;
+    run_collect(B, m, true, nullptr, 0);
+    *out = make_result(ctx, m);
+  });
+}
+
+// -------------------------------------------------------------------- TopN
+ml_status ml_topn_create(ml_ctx* ctx, const ml_sort_spec* spec, ml_topn** out) {
+  auto* T = new ml_topn();
+  T->ctx = ctx;
+  ml_status st = guarded(ctx, [&] {
+    for (int k = 0; k < spec->num_keys; k++) {
+      T->keys_raw.push_back(own_expr(T, spec->keys[k].expr));
+      parse_expr(T->keys_raw.back());
+      T->asc.push_back(spec->keys[k].asc != 0);
+      T->nulls_first.push_back(spec->keys[k].nulls_first != 0);
+    }
+    if (spec->limit < 0) fail(ML_ERR_UNSUPPORTED, "unbounded sort (only ORDER BY ... LIMIT n)");
+    T->limit = spec->limit;
+  });
+  if (st != ML_OK) {
+    delete T;
+    *out = nullptr;
+    return st;
+  }
+  *out = T;
+  return ML_OK;
+}
+
+ml_status ml_topn_sink(ml_topn* t, const ml_batch* batch) {
+  return guarded(t->ctx, [&] {
+    PipelineBuild B(t->ctx);
+    B.source(batch);
+    topn_run(t, B);
+  });
+}
+
+ml_status ml_pipeline_topn(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops, int32_t num_ops, ml_topn* sink) {
+  return guarded(ctx, [&] {
+    PipelineBuild B(ctx);
+    B.source(source);
+    B.add_ops(ops, num_ops);
+    topn_run(sink, B);
+  });
+}
+
+ml_status ml_topn_finalize(ml_topn* t, ml_result** out) {
+  return guarded(t->ctx, [&] {
+    Materialized& b = t->best;
+    if (!t->schema_fixed) b.rows = 0;
+    *out = make_result(t->ctx, b);
+    t->best = Materialized();
+    t->schema_fixed = false;
+  });
+}
+
+void ml_topn_free(ml_topn* t) { delete t; }
+
+}  // extern "C"

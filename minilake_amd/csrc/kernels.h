@@ -1,0 +1,75 @@
+// Launch wrappers for the gfx950 kernels (kernels.hip) and the small structs
+// they take by value.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mlgpu.h"
+
+namespace mlg {
+
+struct AccSlotList {
+  int32_t kind[MAX_AGG_SLOTS];
+};
+
+// accumulator arrays moved by a rehash (old -> new slot)
+struct AccMove {
+  int32_t n;
+  const void* src[MAX_AGG_SLOTS];
+  void* dst[MAX_AGG_SLOTS];
+};
+
+enum FinishKind : int32_t { FIN_COPY = 0, FIN_AVG = 1, FIN_I32 = 2 };
+
+// hash aggregate output: key columns then one column per aggregate
+struct EmitPlan {
+  int32_t nkeys;
+  int32_t key_nullable;
+  int32_t key_width[MAX_KEYS];
+  void* key_out[MAX_KEYS];
+  uint8_t* key_valid[MAX_KEYS];
+  int32_t nout;
+  int32_t finish[MAX_AGG_SLOTS];
+  int32_t out_width[MAX_AGG_SLOTS];
+  const void* state[MAX_AGG_SLOTS];        // value state (sum / min / max / count)
+  const void* count_state[MAX_AGG_SLOTS];  // valid-input count (AVG divisor, SUM/MIN/MAX seen)
+  void* out[MAX_AGG_SLOTS];
+  uint8_t* out_valid[MAX_AGG_SLOTS];
+};
+
+enum SortKeyKind : int32_t { SK_INT = 0, SK_F64 = 1, SK_UNSIGNED = 2 };
+constexpr int MAX_SORT_KEYS = 4;
+constexpr int TOPN_TILE = 1024;
+
+struct SortKeys {
+  int32_t nkeys;
+  int32_t nwords;
+  Prog prog[MAX_SORT_KEYS];
+  int32_t kind[MAX_SORT_KEYS];
+  int32_t asc[MAX_SORT_KEYS];
+  int32_t nulls_first[MAX_SORT_KEYS];
+  int32_t nullable[MAX_SORT_KEYS];
+};
+
+void launch_pipeline(const Plan& P, int grid, int block, size_t lds, hipStream_t s);
+void launch_direct_reduce(const uint64_t* partials, int nblocks, int ncells, int nacc, const AccSlotList& accs,
+                          uint64_t* out, hipStream_t s);
+void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
+void launch_gather(const void* src, int width, const int64_t* idx, int64_t n, void* dst, const uint8_t* src_valid,
+                   uint8_t* dst_valid, hipStream_t s);
+void launch_join_insert(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                        uint64_t* bloom, hipStream_t s);
+void launch_join_unique(const DevJoinTable& T, uint32_t* dup_flag, hipStream_t s);
+void launch_agg_rehash(const uint64_t* old_slots, uint64_t old_cap, int nkw, uint64_t* new_slots, uint64_t new_mask,
+                       const AccMove& mv, hipStream_t s);
+void launch_slot_flags(const uint64_t* slots, uint64_t cap, int nkw, uint8_t* flags, hipStream_t s);
+void launch_agg_emit(const uint64_t* slots, int nkw, const int64_t* idx, int64_t n, const EmitPlan& E, hipStream_t s);
+int64_t compact_blocks(int64_t n);
+void launch_compact(const uint8_t* flags, int64_t n, uint32_t* block_counts, uint64_t* offsets, const uint32_t* sel,
+                    void* out, int out_u32, hipStream_t s);
+void launch_sort_keys(const Plan& P, const SortKeys& K, int64_t n, uint64_t* words, hipStream_t s);
+void launch_topn_tiles(const uint64_t* in, int64_t n, int nw, int keep, uint64_t* out, hipStream_t s);
+void launch_topn_rows(const uint64_t* words, int nw, int64_t n, int64_t* rows_out, hipStream_t s);
+
+}  // namespace mlg

@@ -1,0 +1,1013 @@
+// gfx950 kernels of the execution path: the fused pipeline interpreter and
+// its sinks, hash-table build/rehash/finalize, ordered compaction and TopN.
+//
+// Layout and launch notes (MI355X, CDNA4):
+//  * wave64; streaming kernels use 256-thread workgroups and grid-stride loops
+//    over rows, sized to a multiple of the 256 CUs;
+//  * expression programs are read with scalar loads (constant address space),
+//    so instruction dispatch is uniform SALU work;
+//  * FP64 accumulation uses hardware atomics (global_atomic_add_f64,
+//    ds_add_f64); per-row arithmetic is compiled with -ffp-contract=off so
+//    every row value is bit-identical to an IEEE evaluation in source order.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "mlgpu.h"
+
+namespace mlg {
+
+typedef __attribute__((address_space(4))) const Instr CInstr;
+
+__device__ __forceinline__ double u2f(uint64_t x) { return __longlong_as_double((long long)x); }
+__device__ __forceinline__ uint64_t f2u(double x) { return (uint64_t)__double_as_longlong(x); }
+
+__device__ __forceinline__ int64_t row_of(const int64_t (&rows)[MAX_STAGES], int s) {
+  switch (s) {
+    case 0: return rows[0];
+    case 1: return rows[1];
+    case 2: return rows[2];
+    case 3: return rows[3];
+    case 4: return rows[4];
+    case 5: return rows[5];
+    case 6: return rows[6];
+    default: return rows[7];
+  }
+}
+__device__ __forceinline__ void set_row(int64_t (&rows)[MAX_STAGES], int s, int64_t v) {
+  switch (s) {
+    case 0: rows[0] = v; break;
+    case 1: rows[1] = v; break;
+    case 2: rows[2] = v; break;
+    case 3: rows[3] = v; break;
+    case 4: rows[4] = v; break;
+    case 5: rows[5] = v; break;
+    case 6: rows[6] = v; break;
+    default: rows[7] = v; break;
+  }
+}
+
+// Rust `f as i64` / `f as i32`: saturating, NaN -> 0.
+__device__ __forceinline__ int64_t sat_i64(double x) {
+  if (x != x) return 0;
+  if (x >= 9223372036854775808.0) return INT64_MAX;
+  if (x <= -9223372036854775808.0) return INT64_MIN;
+  return (int64_t)x;
+}
+__device__ __forceinline__ int32_t sat_i32(double x) {
+  if (x != x) return 0;
+  if (x >= 2147483647.0) return INT32_MAX;
+  if (x <= -2147483648.0) return INT32_MIN;
+  return (int32_t)x;
+}
+__device__ __forceinline__ uint64_t sx32(uint32_t x) { return (uint64_t)(int64_t)(int32_t)x; }
+
+// ----------------------------------------------------------------- interpreter
+// Stack slots live in registers: every access below indexes r[] with a
+// constant inside a switch on the (uniform) depth, so nothing spills.
+__device__ __forceinline__ uint64_t get_slot(const uint64_t (&r)[MAX_DEPTH], int d) {
+  switch (d) {
+    case 0: return r[0];
+    case 1: return r[1];
+    case 2: return r[2];
+    case 3: return r[3];
+    case 4: return r[4];
+    default: return r[5];
+  }
+}
+__device__ __forceinline__ void set_slot(uint64_t (&r)[MAX_DEPTH], int d, uint64_t v) {
+  switch (d) {
+    case 0: r[0] = v; break;
+    case 1: r[1] = v; break;
+    case 2: r[2] = v; break;
+    case 3: r[3] = v; break;
+    case 4: r[4] = v; break;
+    default: r[5] = v; break;
+  }
+}
+
+// Evaluate a program for one row; returns validity, value in `val`.
+__device__ __forceinline__ bool eval(CInstr* ins, Prog p, const int64_t (&rows)[MAX_STAGES], uint64_t& val) {
+  uint64_t r[MAX_DEPTH];
+#pragma unroll
+  for (int i = 0; i < MAX_DEPTH; i++) r[i] = 0;
+  uint32_t ok = 0xFFFFFFFFu;
+#pragma unroll 1
+  for (int pc = 0; pc < p.count; pc++) {
+    CInstr& in = ins[p.first + pc];
+    const uint32_t op = in.op;
+    const int d = in.depth;
+    const uint64_t imm = in.imm;
+    uint64_t res;
+    uint32_t vres = 1;
+    int dst;
+    if (op <= OP_CONST) {  // push
+      dst = d;
+      if (op == OP_CONST) {
+        res = imm;
+        vres = in.flags & 1u;
+      } else {
+        const int64_t row = row_of(rows, in.stage);
+        if (op == OP_LOAD_I32) res = sx32((uint32_t)((const int32_t*)imm)[row]);
+        else if (op == OP_LOAD_U32) res = (uint64_t)((const uint32_t*)imm)[row];
+        else if (op == OP_LOAD_I64) res = ((const uint64_t*)imm)[row];
+        else res = ((const uint8_t*)imm)[row] != 0;
+        const uint64_t vp = in.imm2;
+        if (p.nullable && vp) vres = ((const uint8_t*)vp)[row] != 0;
+      }
+    } else if (op < OP_ADD_I32) {  // unary on the top slot
+      dst = d - 1;
+      const uint64_t a = get_slot(r, dst);
+      vres = (ok >> dst) & 1u;
+      switch (op) {
+        case OP_NOT: res = a ^ 1ull; break;
+        case OP_NEG_I32: res = sx32(0u - (uint32_t)a); break;
+        case OP_NEG_I64: res = 0ull - a; break;
+        case OP_NEG_F64: res = a ^ 0x8000000000000000ull; break;
+        case OP_I2F: res = f2u((double)(int64_t)a); break;
+        case OP_F2I64: res = (uint64_t)sat_i64(u2f(a)); break;
+        case OP_F2I32: res = sx32((uint32_t)sat_i32(u2f(a))); break;
+        case OP_I64_TO_I32: res = sx32((uint32_t)a); break;
+        case OP_LOOKUP_U8: res = ((const uint8_t*)imm)[a]; break;
+        case OP_LOOKUP_U64: res = ((const uint64_t*)imm)[a]; break;
+        case OP_KEY_F64:
+          res = (a == 0x8000000000000000ull) ? 0ull : ((u2f(a) != u2f(a)) ? 0x7FF8000000000000ull : a);
+          break;
+        default: res = u2f(a) == 0.0 ? 0ull : a; break;  // OP_HASH_F64
+      }
+    } else {  // binary: slots d-2 (left), d-1 (right) -> d-2
+      dst = d - 2;
+      const uint64_t a = get_slot(r, dst), b = get_slot(r, d - 1);
+      const uint32_t va = (ok >> dst) & 1u, vb = (ok >> (d - 1)) & 1u;
+      vres = va & vb;
+      switch (op) {
+        case OP_ADD_I32: res = sx32((uint32_t)a + (uint32_t)b); break;
+        case OP_SUB_I32: res = sx32((uint32_t)a - (uint32_t)b); break;
+        case OP_MUL_I32: res = sx32((uint32_t)a * (uint32_t)b); break;
+        case OP_MOD_I32: {  // 0 and NULL for x % 0; wrapping_rem for -1
+          const int32_t x = (int32_t)a, y = (int32_t)b;
+          res = (y == 0 || y == -1) ? 0ull : sx32((uint32_t)(x % y));
+          if (y == 0) vres = 0;
+          break;
+        }
+        case OP_ADD_I64: res = a + b; break;
+        case OP_SUB_I64: res = a - b; break;
+        case OP_MUL_I64: res = a * b; break;
+        case OP_MOD_I64: {
+          const int64_t x = (int64_t)a, y = (int64_t)b;
+          res = (y == 0 || y == -1) ? 0ull : (uint64_t)(x % y);
+          if (y == 0) vres = 0;
+          break;
+        }
+        case OP_ADD_F64: res = f2u(u2f(a) + u2f(b)); break;
+        case OP_SUB_F64: res = f2u(u2f(a) - u2f(b)); break;
+        case OP_MUL_F64: res = f2u(u2f(a) * u2f(b)); break;
+        case OP_DIV_F64:  // x / 0 is NULL (crates/minilake-exec/src/expr/eval.rs:260-270)
+          res = f2u(u2f(a) / u2f(b));
+          if (u2f(b) == 0.0) vres = 0;
+          break;
+        case OP_MOD_F64: res = f2u(fmod(u2f(a), u2f(b))); break;
+        case OP_EQ_I: res = (int64_t)a == (int64_t)b; break;
+        case OP_NE_I: res = (int64_t)a != (int64_t)b; break;
+        case OP_LT_I: res = (int64_t)a < (int64_t)b; break;
+        case OP_LE_I: res = (int64_t)a <= (int64_t)b; break;
+        case OP_GT_I: res = (int64_t)a > (int64_t)b; break;
+        case OP_GE_I: res = (int64_t)a >= (int64_t)b; break;
+        case OP_EQ_F: res = u2f(a) == u2f(b); break;
+        case OP_NE_F: res = u2f(a) != u2f(b); break;
+        case OP_LT_F: res = u2f(a) < u2f(b); break;
+        case OP_LE_F: res = u2f(a) <= u2f(b); break;
+        case OP_GT_F: res = u2f(a) > u2f(b); break;
+        case OP_GE_F: res = u2f(a) >= u2f(b); break;
+        case OP_AND:  // Kleene AND (crates/minilake-exec/src/kernels/boolean.rs:18-36)
+          res = a & b;
+          vres = (va & vb) | (va & (uint32_t)(a == 0)) | (vb & (uint32_t)(b == 0));
+          break;
+        default:  // OP_OR, Kleene (boolean.rs:40-58)
+          res = a | b;
+          vres = (va & vb) | (va & (uint32_t)(a != 0)) | (vb & (uint32_t)(b != 0));
+          break;
+      }
+    }
+    set_slot(r, dst, res);
+    ok = (ok & ~(1u << dst)) | (vres << dst);
+  }
+  val = r[0];
+  return (ok & 1u) != 0;
+}
+
+__device__ __forceinline__ void set_word(uint64_t (&w)[MAX_KEYS + 1], int k, uint64_t v) {
+  switch (k) {
+    case 0: w[0] = v; break;
+    case 1: w[1] = v; break;
+    case 2: w[2] = v; break;
+    case 3: w[3] = v; break;
+    default: w[4] = v; break;
+  }
+}
+
+__device__ __forceinline__ uint64_t key_hash(const uint64_t* w, int n) {
+  uint64_t h = HSEED;
+#pragma unroll
+  for (int k = 0; k < MAX_KEYS + 1; k++)
+    if (k < n) h = hash_combine(h, hash_u64(w[k]));
+  return h;
+}
+
+__device__ __forceinline__ bool bloom_maybe(const DevJoinTable& T, uint64_t h) {
+  if (!T.bloom) return true;
+  const uint64_t g = hash_u64(h);
+  const uint64_t bits = (1ull << (g & 63)) | (1ull << ((g >> 6) & 63)) | (1ull << ((g >> 12) & 63));
+  return (T.bloom[(g >> 18) & T.bloom_mask] & bits) == bits;
+}
+
+// First slot at or after `pos` whose key equals w; false at the first empty slot.
+__device__ __forceinline__ bool join_next(const DevJoinTable& T, const uint64_t (&w)[MAX_KEYS + 1], uint64_t h,
+                                          uint64_t& pos, int64_t& row) {
+  const int stride = 1 + T.nkw;
+  const uint32_t tag = (uint32_t)(h >> 32);
+  for (;;) {
+    const uint64_t* s = T.slots + pos * stride;
+    const uint64_t hdr = s[0];
+    if (hdr == 0) return false;
+    if ((uint32_t)(hdr >> 32) == tag) {
+      bool eq = true;
+#pragma unroll
+      for (int k = 0; k < MAX_KEYS; k++)
+        if (k < T.nkw) eq = eq && (s[1 + k] == w[k]);
+      if (eq) {
+        row = (int64_t)(uint32_t)hdr - 1;
+        return true;
+      }
+    }
+    pos = (pos + 1) & T.mask;
+  }
+}
+
+__device__ __forceinline__ bool probe_keys(const Plan& P, CInstr* ins, const Step& st,
+                                           const int64_t (&rows)[MAX_STAGES], uint64_t (&w)[MAX_KEYS + 1]) {
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = 0;
+#pragma unroll 1
+  for (int k = 0; k < st.nkeys; k++) {
+    uint64_t v;
+    ok = eval(ins, st.keys[k], rows, v) && ok;
+    set_word(w, k, v);
+  }
+  return ok;
+}
+
+// Filters and unique probes in [from, to); false once the row is dropped.
+__device__ __forceinline__ bool run_steps(const Plan& P, CInstr* ins, int from, int to,
+                                          int64_t (&rows)[MAX_STAGES]) {
+  bool alive = true;
+  for (int s = from; s < to; s++) {
+    if (!alive) break;
+    const Step& st = P.steps[s];
+    if (st.kind == STEP_FILTER) {
+      uint64_t v;
+      const bool ok = eval(ins, st.pred, rows, v);
+      alive = ok && v != 0;
+    } else {
+      uint64_t w[MAX_KEYS + 1];
+      if (!probe_keys(P, ins, st, rows, w)) {  // NULL keys never match (join.rs:284-286)
+        alive = false;
+        continue;
+      }
+      const DevJoinTable& T = P.tables[st.table];
+      const uint64_t h = key_hash(w, st.nkeys);
+      uint64_t pos = h & T.mask;
+      int64_t row = 0;
+      if (!bloom_maybe(T, h) || !join_next(T, w, h, pos, row)) {
+        alive = false;
+        continue;
+      }
+      set_row(rows, st.stage, row);
+    }
+  }
+  return alive;
+}
+
+// ------------------------------------------------------------- wave helpers
+__device__ __forceinline__ uint64_t wave_alloc(uint64_t* counter, uint32_t n_per_lane) {
+  // one atomic per wave for the active lanes; each lane gets its own range
+  const uint64_t active = __ballot(1);
+  const int lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)active) - 1;
+  const uint64_t below = active & ((1ull << lane) - 1ull);
+  const uint32_t rank = (uint32_t)__popcll(below);
+  const uint32_t total = (uint32_t)__popcll(active) * n_per_lane;
+  uint64_t base = 0;
+  if (lane == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)total);
+  base = __shfl(base, leader);
+  return base + (uint64_t)rank * n_per_lane;
+}
+
+// --------------------------------------------------------------- aggregates
+__device__ __forceinline__ void atomic_min_f64(double* p, double v) {
+  unsigned long long* q = (unsigned long long*)p;
+  unsigned long long cur = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (v < __longlong_as_double((long long)cur)) {
+    unsigned long long prev = atomicCAS(q, cur, (unsigned long long)__double_as_longlong(v));
+    if (prev == cur) break;
+    cur = prev;
+  }
+}
+__device__ __forceinline__ void atomic_max_f64(double* p, double v) {
+  unsigned long long* q = (unsigned long long*)p;
+  unsigned long long cur = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (v > __longlong_as_double((long long)cur)) {
+    unsigned long long prev = atomicCAS(q, cur, (unsigned long long)__double_as_longlong(v));
+    if (prev == cur) break;
+    cur = prev;
+  }
+}
+
+// accumulate into a global (hash table) state slot
+__device__ __forceinline__ void acc_apply_global(int kind, void* data, int64_t slot, uint64_t v) {
+  switch (kind) {
+    case ACC_SUM_F64: unsafeAtomicAdd((double*)data + slot, u2f(v)); break;
+    case ACC_SUM_I64: atomicAdd((unsigned long long*)data + slot, (unsigned long long)v); break;
+    case ACC_COUNT: atomicAdd((unsigned long long*)data + slot, 1ull); break;
+    case ACC_MIN_I64: atomicMin((long long*)data + slot, (long long)v); break;
+    case ACC_MAX_I64: atomicMax((long long*)data + slot, (long long)v); break;
+    case ACC_MIN_F64: atomic_min_f64((double*)data + slot, u2f(v)); break;
+    case ACC_MAX_F64: atomic_max_f64((double*)data + slot, u2f(v)); break;
+    default: break;
+  }
+}
+
+// accumulate into an LDS state word
+__device__ __forceinline__ void acc_apply_lds(int kind, uint64_t* w, uint64_t v) {
+  switch (kind) {
+    case ACC_SUM_F64: unsafeAtomicAdd((double*)w, u2f(v)); break;
+    case ACC_SUM_I64: atomicAdd((unsigned long long*)w, (unsigned long long)v); break;
+    case ACC_COUNT: atomicAdd((unsigned long long*)w, 1ull); break;
+    case ACC_MIN_I64: atomicMin((long long*)w, (long long)v); break;
+    case ACC_MAX_I64: atomicMax((long long*)w, (long long)v); break;
+    case ACC_MIN_F64: {
+      unsigned long long* q = (unsigned long long*)w;
+      unsigned long long cur = *q;
+      while (u2f(v) < __longlong_as_double((long long)cur)) {
+        unsigned long long prev = atomicCAS(q, cur, (unsigned long long)v);
+        if (prev == cur) break;
+        cur = prev;
+      }
+      break;
+    }
+    case ACC_MAX_F64: {
+      unsigned long long* q = (unsigned long long*)w;
+      unsigned long long cur = *q;
+      while (u2f(v) > __longlong_as_double((long long)cur)) {
+        unsigned long long prev = atomicCAS(q, cur, (unsigned long long)v);
+        if (prev == cur) break;
+        cur = prev;
+      }
+      break;
+    }
+    default: break;
+  }
+}
+
+// Find or insert the group with key words w; returns its slot or -1 (table full).
+__device__ __forceinline__ int64_t agg_slot(const Plan& P, const uint64_t (&w)[MAX_KEYS + 1], int nkw, uint64_t h) {
+  const uint64_t hdr = (h & ~3ull) | 2ull;
+  const uint64_t stride = 1 + (uint64_t)nkw;
+  uint64_t pos = h & P.agg_mask;
+  uint64_t probes = 0;
+  while (probes <= P.agg_mask) {
+    uint64_t* s = P.agg_slots + pos * stride;
+    uint64_t cur = __hip_atomic_load(s, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == 0) {
+      const uint64_t prev = atomicCAS((unsigned long long*)s, 0ull, 1ull);
+      if (prev == 0) {
+        const uint64_t n = atomicAdd((unsigned long long*)P.agg_count, 1ull);
+        if (n >= P.agg_fill_limit) atomicOr(P.agg_overflow, 1u);  // past the load-factor limit
+#pragma unroll
+        for (int k = 0; k < MAX_KEYS + 1; k++)
+          if (k < nkw) s[1 + k] = w[k];
+        __hip_atomic_store(s, hdr, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return (int64_t)pos;
+      }
+      cur = prev;
+      if (cur != 1) cur = __hip_atomic_load(s, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (cur == 1) continue;  // another lane is writing this slot's key: read it again
+    if (cur == hdr) {
+      bool eq = true;
+#pragma unroll
+      for (int k = 0; k < MAX_KEYS + 1; k++)
+        if (k < nkw) eq = eq && (s[1 + k] == w[k]);
+      if (eq) return (int64_t)pos;
+    }
+    pos = (pos + 1) & P.agg_mask;
+    probes++;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ int agg_keys(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES],
+                                        uint64_t (&w)[MAX_KEYS + 1]) {
+  uint64_t nullmask = 0;
+#pragma unroll
+  for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = 0;
+#pragma unroll 1
+  for (int k = 0; k < P.nkeys; k++) {
+    uint64_t v;
+    if (!eval(ins, P.keys[k], rows, v)) {  // NULL keys form one group (GROUP BY semantics)
+      v = 0;
+      nullmask |= 1ull << k;
+    }
+    set_word(w, k, v);
+  }
+  if (P.key_nullable) set_word(w, P.nkeys, nullmask);
+  return P.nkeys + (P.key_nullable ? 1 : 0);
+}
+
+__device__ __forceinline__ void sink_hash_agg(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES]) {
+  uint64_t w[MAX_KEYS + 1];
+  const int nkw = agg_keys(P, ins, rows, w);
+  const uint64_t h = key_hash(w, nkw);
+  const int64_t slot = agg_slot(P, w, nkw, h);
+  if (slot < 0) {  // table full: this row's contribution is lost, the caller must rerun
+    atomicOr(P.agg_overflow, 2u);
+    return;
+  }
+  for (int a = 0; a < P.nargs; a++) {
+    uint64_t v;
+    const bool ok = eval(ins, P.args[a], rows, v);
+    if (!ok) continue;  // NULL arguments are skipped by every accumulator
+    for (int s = 0; s < P.nacc; s++)
+      if (P.acc[s].arg == a) acc_apply_global(P.acc[s].kind, P.acc_data[s], slot, v);
+  }
+  for (int s = 0; s < P.nacc; s++)
+    if (P.acc[s].arg < 0) atomicAdd((unsigned long long*)P.acc_data[s] + slot, 1ull);
+}
+
+__device__ __forceinline__ void sink_direct_agg(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES],
+                                                uint64_t* lds) {
+  int64_t g = 0;
+#pragma unroll 1
+  for (int k = 0; k < P.nkeys; k++) {
+    uint64_t v;
+    const bool ok = eval(ins, P.keys[k], rows, v);
+    // a NULL key takes the extra last value of its dimension
+    const int64_t code = ok ? (int64_t)v : P.dim_size[k] - 1;
+    g += code * P.dim_stride[k];
+  }
+  uint64_t* st = lds + g * P.nacc;
+  for (int a = 0; a < P.nargs; a++) {
+    uint64_t v;
+    const bool ok = eval(ins, P.args[a], rows, v);
+    if (!ok) continue;
+    for (int s = 0; s < P.nacc; s++)
+      if (P.acc[s].arg == a) acc_apply_lds(P.acc[s].kind, st + s, v);
+  }
+  for (int s = 0; s < P.nacc; s++)
+    if (P.acc[s].arg < 0) atomicAdd((unsigned long long*)(st + s), 1ull);
+}
+
+// Materialize every column of the layout for one row at output position pos.
+__device__ __forceinline__ void write_cols(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES], uint64_t pos) {
+  for (int c = 0; c < P.ncols; c++) {
+    uint64_t v;
+    const bool ok = eval(ins, P.cols[c], rows, v);
+    const int w = P.col_width[c];
+    if (w == 8) ((uint64_t*)P.col_out[c])[pos] = v;
+    else if (w == 4) ((uint32_t*)P.col_out[c])[pos] = (uint32_t)v;
+    else ((uint8_t*)P.col_out[c])[pos] = (uint8_t)v;
+    if (P.col_valid[c]) P.col_valid[c][pos] = ok;
+  }
+}
+
+__device__ __forceinline__ void sink_join_build(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES]) {
+  const uint64_t pos = wave_alloc(P.out_count, 1);
+  if (pos >= P.out_capacity) {
+    atomicOr(P.out_overflow, 1u);
+    return;
+  }
+  bool any_null = false;
+#pragma unroll 1
+  for (int k = 0; k < P.nkeys; k++) {
+    uint64_t v;
+    if (!eval(ins, P.keys[k], rows, v)) {
+      any_null = true;
+      v = 0;
+    }
+    P.key_out[pos * P.nkeys + k] = v;
+  }
+  P.key_null[pos] = any_null;
+  write_cols(P, ins, rows, pos);
+}
+
+__device__ __forceinline__ void sink_collect(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES]) {
+  const uint64_t pos = wave_alloc(P.out_count, 1);
+  if (pos >= P.out_capacity) {
+    atomicOr(P.out_overflow, 1u);
+    return;
+  }
+  write_cols(P, ins, rows, pos);
+  if (P.pair_out) {
+    for (int s = 0; s < P.pair_stages; s++) P.pair_out[pos * P.pair_stages + s] = row_of(rows, s);
+  }
+}
+
+template <int SINK>
+__device__ __forceinline__ void do_sink(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES], int64_t i,
+                                        uint64_t* lds) {
+  if constexpr (SINK == SINK_HASH_AGG) sink_hash_agg(P, ins, rows);
+  else if constexpr (SINK == SINK_DIRECT_AGG) sink_direct_agg(P, ins, rows, lds);
+  else if constexpr (SINK == SINK_JOIN_BUILD) sink_join_build(P, ins, rows);
+  else if constexpr (SINK == SINK_COLLECT) sink_collect(P, ins, rows);
+  else if constexpr (SINK == SINK_COUNT_PAIRS) wave_alloc(P.out_count, 1);
+  else if constexpr (SINK == SINK_PROJECT) write_cols(P, ins, rows, (uint64_t)rows[0]);
+  else if constexpr (SINK == SINK_HASH) {
+    uint64_t h = HSEED;
+#pragma unroll 1
+    for (int k = 0; k < P.nkeys; k++) {
+      uint64_t v;
+      const bool ok = eval(ins, P.keys[k], rows, v);
+      const uint64_t x = !ok ? HNULL : (((P.key_prehashed >> k) & 1) ? v : hash_u64(v));
+      h = hash_combine(h, x);
+    }
+    P.hash_out[i] = h;
+  }
+}
+
+// The fused pipeline: Source -> (filter | probe)* -> sink, one row per lane.
+template <int SINK>
+__global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
+  extern __shared__ uint64_t lds[];
+  CInstr* ins = (CInstr*)P.ins;
+  const int ncells = (SINK == SINK_DIRECT_AGG) ? P.ngroups * P.nacc : 0;
+  if constexpr (SINK == SINK_DIRECT_AGG) {
+    for (int c = threadIdx.x; c < ncells; c += blockDim.x) lds[c] = acc_init_bits(P.acc[c % P.nacc].kind);
+    __syncthreads();
+  }
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = P.begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.end; i += stride) {
+    int64_t rows[MAX_STAGES];
+#pragma unroll
+    for (int s = 0; s < MAX_STAGES; s++) rows[s] = 0;
+    rows[0] = P.sel ? (int64_t)P.sel[i] : i;
+    if (P.expand_step < 0) {
+      const bool alive = run_steps(P, ins, 0, P.nsteps, rows);
+      if constexpr (SINK == SINK_FLAGS) P.flag_out[i] = alive;
+      else if (alive) do_sink<SINK>(P, ins, rows, i, lds);
+    } else {
+      // one probe may match several build rows: the rest of the chain runs per match
+      const int e = P.expand_step;
+      bool alive = run_steps(P, ins, 0, e, rows);
+      if (alive) {
+        const Step& st = P.steps[e];
+        uint64_t w[MAX_KEYS + 1];
+        if (probe_keys(P, ins, st, rows, w)) {
+          const DevJoinTable& T = P.tables[st.table];
+          const uint64_t h = key_hash(w, st.nkeys);
+          uint64_t pos = h & T.mask;
+          int64_t row = 0;
+          if (bloom_maybe(T, h)) {
+            while (join_next(T, w, h, pos, row)) {
+              set_row(rows, st.stage, row);
+              if (run_steps(P, ins, e + 1, P.nsteps, rows)) do_sink<SINK>(P, ins, rows, i, lds);
+              pos = (pos + 1) & T.mask;
+            }
+          }
+        }
+      }
+    }
+  }
+  if constexpr (SINK == SINK_DIRECT_AGG) {
+    __syncthreads();
+    uint64_t* out = (uint64_t*)P.direct_partials + (uint64_t)blockIdx.x * ncells;
+    for (int c = threadIdx.x; c < ncells; c += blockDim.x) out[c] = lds[c];
+  }
+}
+
+// ---------------------------------------------------------- direct agg reduce
+// One thread per (group, state): combine block partials in block order.
+__global__ void direct_reduce_kernel(const uint64_t* partials, int nblocks, int ncells, int nacc, AccSlotList accs,
+                                     uint64_t* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int kind = accs.kind[c % nacc];
+  uint64_t acc = out[c];  // running totals of earlier batches
+  for (int b = 0; b < nblocks; b++) {
+    const uint64_t v = partials[(uint64_t)b * ncells + c];
+    switch (kind) {
+      case ACC_SUM_F64: acc = f2u(u2f(acc) + u2f(v)); break;
+      case ACC_SUM_I64: case ACC_COUNT: acc += v; break;
+      case ACC_MIN_I64: acc = (int64_t)v < (int64_t)acc ? v : acc; break;
+      case ACC_MAX_I64: acc = (int64_t)v > (int64_t)acc ? v : acc; break;
+      case ACC_MIN_F64: acc = u2f(v) < u2f(acc) ? v : acc; break;
+      case ACC_MAX_F64: acc = u2f(v) > u2f(acc) ? v : acc; break;
+      default: break;
+    }
+  }
+  out[c] = acc;
+}
+
+// ------------------------------------------------------------ fills / gathers
+__global__ void fill_u64_kernel(uint64_t* p, uint64_t v, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+__global__ void gather_kernel(const void* src, int width, const int64_t* idx, int64_t n, void* dst,
+                              const uint8_t* src_valid, uint8_t* dst_valid) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t r = idx[i];
+    if (width == 8) ((uint64_t*)dst)[i] = ((const uint64_t*)src)[r];
+    else if (width == 4) ((uint32_t*)dst)[i] = ((const uint32_t*)src)[r];
+    else ((uint8_t*)dst)[i] = ((const uint8_t*)src)[r];
+    if (dst_valid) dst_valid[i] = src_valid ? src_valid[r] : 1;
+  }
+}
+
+// ------------------------------------------------------------------ join table
+__global__ void join_insert_kernel(DevJoinTable T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                                   uint64_t* bloom) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int nkw = T.nkw;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+    if (key_null[r]) continue;  // NULL build keys are never inserted (join.rs:182-184)
+    uint64_t w[MAX_KEYS + 1];
+#pragma unroll
+    for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < nkw ? keys[r * nkw + k] : 0;
+    const uint64_t h = key_hash(w, nkw);
+    const uint64_t hdr = ((h >> 32) << 32) | (uint64_t)(r + 1);
+    uint64_t pos = h & T.mask;
+    for (;;) {
+      uint64_t* s = T.slots + pos * (1 + nkw);
+      if (atomicCAS((unsigned long long*)s, 0ull, (unsigned long long)hdr) == 0ull) {
+#pragma unroll
+        for (int k = 0; k < MAX_KEYS; k++)
+          if (k < nkw) s[1 + k] = w[k];
+        break;
+      }
+      pos = (pos + 1) & T.mask;
+    }
+    if (bloom) {
+      const uint64_t g = hash_u64(h);
+      const uint64_t bits = (1ull << (g & 63)) | (1ull << ((g >> 6) & 63)) | (1ull << ((g >> 12) & 63));
+      atomicOr((unsigned long long*)&bloom[(g >> 18) & T.bloom_mask], (unsigned long long)bits);
+    }
+  }
+}
+
+// unique = 0 if two occupied slots hold the same key (walk each slot's run from its home).
+__global__ void join_unique_kernel(DevJoinTable T, uint32_t* dup_flag) {
+  const uint64_t cap = T.mask + 1;
+  const int nkw = T.nkw;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < (int64_t)cap; p += stride) {
+    const uint64_t* s = T.slots + (uint64_t)p * (1 + nkw);
+    if (s[0] == 0) continue;
+    uint64_t w[MAX_KEYS + 1];
+#pragma unroll
+    for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < nkw ? s[1 + k] : 0;
+    const uint64_t h = key_hash(w, nkw);
+    uint64_t q = h & T.mask;
+    while (q != (uint64_t)p) {
+      const uint64_t* t = T.slots + q * (1 + nkw);
+      bool eq = t[0] != 0 && (t[0] >> 32) == (s[0] >> 32);
+#pragma unroll
+      for (int k = 0; k < MAX_KEYS; k++)
+        if (k < nkw) eq = eq && t[1 + k] == w[k];
+      if (eq) {
+        atomicOr(dup_flag, 1u);
+        break;
+      }
+      q = (q + 1) & T.mask;
+    }
+  }
+}
+
+// --------------------------------------------------------------- agg tables
+__global__ void agg_rehash_kernel(const uint64_t* old_slots, uint64_t old_cap, int nkw, uint64_t* new_slots,
+                                  uint64_t new_mask, AccMove mv) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < (int64_t)old_cap; p += stride) {
+    const uint64_t* s = old_slots + (uint64_t)p * (1 + nkw);
+    const uint64_t hdr = s[0];
+    if (hdr == 0) continue;
+    uint64_t w[MAX_KEYS + 1];
+#pragma unroll
+    for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < nkw ? s[1 + k] : 0;
+    const uint64_t h = key_hash(w, nkw);
+    uint64_t pos = h & new_mask;
+    for (;;) {
+      uint64_t* t = new_slots + pos * (1 + nkw);
+      if (atomicCAS((unsigned long long*)t, 0ull, (unsigned long long)hdr) == 0ull) {
+#pragma unroll
+        for (int k = 0; k < MAX_KEYS + 1; k++)
+          if (k < nkw) t[1 + k] = w[k];
+        for (int a = 0; a < mv.n; a++) ((uint64_t*)mv.dst[a])[pos] = ((const uint64_t*)mv.src[a])[p];
+        break;
+      }
+      pos = (pos + 1) & new_mask;
+    }
+  }
+}
+
+__global__ void slot_flags_kernel(const uint64_t* slots, uint64_t cap, int nkw, uint8_t* flags) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < (int64_t)cap; p += stride)
+    flags[p] = slots[(uint64_t)p * (1 + nkw)] != 0;
+}
+
+// Decode group keys and finish aggregates for output rows j (slot = idx[j]).
+__global__ void agg_emit_kernel(const uint64_t* slots, int nkw, const int64_t* idx, int64_t n, EmitPlan E) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    const int64_t slot = idx[j];
+    const uint64_t* s = slots + (uint64_t)slot * (1 + nkw);
+    const uint64_t nullmask = E.key_nullable ? s[1 + E.nkeys] : 0;
+    for (int k = 0; k < E.nkeys; k++) {
+      const uint64_t w = s[1 + k];
+      if (E.key_width[k] == 8) ((uint64_t*)E.key_out[k])[j] = w;
+      else if (E.key_width[k] == 4) ((uint32_t*)E.key_out[k])[j] = (uint32_t)w;
+      else ((uint8_t*)E.key_out[k])[j] = (uint8_t)w;
+      if (E.key_valid[k]) E.key_valid[k][j] = ((nullmask >> k) & 1) == 0;
+    }
+    for (int a = 0; a < E.nout; a++) {
+      const uint64_t v = ((const uint64_t*)E.state[a])[slot];
+      const int64_t cnt = E.count_state[a] ? ((const int64_t*)E.count_state[a])[slot] : 1;
+      uint64_t out = v;
+      switch (E.finish[a]) {
+        case FIN_AVG: out = f2u(cnt == 0 ? 0.0 : u2f(v) / (double)cnt); break;
+        case FIN_I32: out = (uint64_t)(uint32_t)v; break;
+        default: break;
+      }
+      if (E.out_width[a] == 8) ((uint64_t*)E.out[a])[j] = out;
+      else ((uint32_t*)E.out[a])[j] = (uint32_t)out;
+      if (E.out_valid[a]) E.out_valid[a][j] = cnt > 0;
+    }
+  }
+}
+
+// ------------------------------------------------------- ordered compaction
+constexpr int CT = 256;          // threads
+constexpr int CE = 4;            // elements per thread
+constexpr int CB = CT * CE;      // elements per block
+
+__global__ void flag_counts_kernel(const uint8_t* flags, int64_t n, uint32_t* block_counts) {
+  __shared__ uint32_t red[CT / 64];
+  const int64_t base = (int64_t)blockIdx.x * CB + (int64_t)threadIdx.x * CE;
+  uint32_t c = 0;
+#pragma unroll
+  for (int e = 0; e < CE; e++) c += (base + e < n) ? (flags[base + e] != 0) : 0;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < CT / 64; w++) t += red[w];
+    block_counts[blockIdx.x] = t;
+  }
+}
+
+// exclusive scan of block counts (single workgroup, chunked); total at offsets[nb]
+__global__ void scan_counts_kernel(const uint32_t* counts, int64_t nb, uint64_t* offsets) {
+  __shared__ uint64_t buf[1024];
+  uint64_t carry = 0;
+  for (int64_t base = 0; base < nb; base += 1024) {
+    const int64_t i = base + threadIdx.x;
+    const uint64_t v = i < nb ? counts[i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const uint64_t t = threadIdx.x >= (unsigned)o ? buf[threadIdx.x - o] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) offsets[i] = carry + buf[threadIdx.x] - v;
+    const uint64_t total = buf[1023];
+    __syncthreads();
+    carry += total;
+  }
+  if (threadIdx.x == 0) offsets[nb] = carry;
+}
+
+__global__ void flag_scatter_kernel(const uint8_t* flags, int64_t n, const uint64_t* offsets, const uint32_t* sel,
+                                    void* out, int out_u32) {
+  __shared__ uint32_t warp_tot[CT / 64];
+  const int64_t base = (int64_t)blockIdx.x * CB + (int64_t)threadIdx.x * CE;
+  uint32_t f[CE];
+  uint32_t c = 0;
+#pragma unroll
+  for (int e = 0; e < CE; e++) {
+    f[e] = (base + e < n) ? (flags[base + e] != 0) : 0;
+    c += f[e];
+  }
+  // inclusive scan within the wave
+  const int lane = threadIdx.x & 63;
+  uint32_t incl = c;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) warp_tot[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  uint32_t wbase = 0;
+  for (int w = 0; w < (int)(threadIdx.x >> 6); w++) wbase += warp_tot[w];
+  uint64_t pos = offsets[blockIdx.x] + wbase + incl - c;
+#pragma unroll
+  for (int e = 0; e < CE; e++) {
+    if (f[e]) {
+      const int64_t i = base + e;
+      const uint64_t v = sel ? (uint64_t)sel[i] : (uint64_t)i;
+      if (out_u32) ((uint32_t*)out)[pos] = (uint32_t)v;
+      else ((int64_t*)out)[pos] = (int64_t)v;
+      pos++;
+    }
+  }
+}
+
+// ------------------------------------------------------------------- TopN
+// Sort words: lexicographic u64 words, then row index ascending.
+__device__ __forceinline__ bool sort_less(const uint64_t* a, const uint64_t* b, int nw) {
+  for (int k = 0; k < nw; k++) {
+    if (a[k] != b[k]) return a[k] < b[k];
+  }
+  return a[nw] < b[nw];  // row index tie-break
+}
+
+__global__ void sort_keys_kernel(Plan P, SortKeys K, int64_t n, uint64_t* words) {
+  CInstr* ins = (CInstr*)P.ins;
+  const int nw = K.nwords;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    int64_t rows[MAX_STAGES];
+#pragma unroll
+    for (int s = 0; s < MAX_STAGES; s++) rows[s] = 0;
+    rows[0] = i;
+    uint64_t* out = words + (uint64_t)i * (nw + 1);
+    int wi = 0;
+    for (int k = 0; k < K.nkeys; k++) {
+      uint64_t v;
+      const bool ok = eval(ins, K.prog[k], rows, v);
+      uint64_t word;
+      if (K.kind[k] == SK_F64) {  // f64::total_cmp order
+        const int64_t b = (int64_t)v;
+        word = (uint64_t)(b ^ (int64_t)(((uint64_t)(b >> 63)) >> 1)) ^ 0x8000000000000000ull;
+      } else if (K.kind[k] == SK_INT) {
+        word = v ^ 0x8000000000000000ull;
+      } else {
+        word = v;
+      }
+      if (!K.asc[k]) word = ~word;
+      if (K.nullable[k]) {
+        const uint64_t nflag = ok ? (K.nulls_first[k] ? 1 : 0) : (K.nulls_first[k] ? 0 : 1);
+        out[wi++] = nflag;
+        out[wi++] = ok ? word : 0;
+      } else {
+        out[wi++] = word;
+      }
+    }
+    out[nw] = (uint64_t)i;
+  }
+}
+
+// Bitonic-sort tiles of TILE entries (nw+1 words each) in LDS; keep the first
+// `keep` entries of every tile.
+__global__ void __launch_bounds__(256) topn_tile_kernel(const uint64_t* in, int64_t n, int nw, int keep,
+                                                         uint64_t* out) {
+  extern __shared__ uint64_t sh[];
+  const int W = nw + 1;
+  const int64_t tile0 = (int64_t)blockIdx.x * TOPN_TILE;
+  const int cnt = (int)((n - tile0) < TOPN_TILE ? (n - tile0) : TOPN_TILE);
+  for (int e = threadIdx.x; e < TOPN_TILE; e += blockDim.x) {
+    for (int k = 0; k < W; k++)
+      sh[e * W + k] = e < cnt ? in[(uint64_t)(tile0 + e) * W + k] : ~0ull;
+  }
+  __syncthreads();
+  for (int size = 2; size <= TOPN_TILE; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < TOPN_TILE / 2; t += blockDim.x) {
+        const int lo = 2 * stride * (t / stride) + (t % stride);
+        const int hi = lo + stride;
+        const bool up = ((lo & size) == 0);
+        uint64_t* a = sh + lo * W;
+        uint64_t* b = sh + hi * W;
+        const bool swap = up ? sort_less(b, a, nw) : sort_less(a, b, nw);
+        if (swap)
+          for (int k = 0; k < W; k++) {
+            const uint64_t x = a[k];
+            a[k] = b[k];
+            b[k] = x;
+          }
+      }
+      __syncthreads();
+    }
+  }
+  const int m = keep < cnt ? keep : cnt;
+  for (int e = threadIdx.x; e < m; e += blockDim.x)
+    for (int k = 0; k < W; k++) out[((uint64_t)blockIdx.x * keep + e) * W + k] = sh[e * W + k];
+  // pad missing entries of a short tile with +inf keys
+  for (int e = m + threadIdx.x; e < keep; e += blockDim.x)
+    for (int k = 0; k < W; k++) out[((uint64_t)blockIdx.x * keep + e) * W + k] = ~0ull;
+}
+
+__global__ void topn_rows_kernel(const uint64_t* words, int nw, int64_t n, int64_t* rows_out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) rows_out[i] = (int64_t)words[(uint64_t)i * (nw + 1) + nw];
+}
+
+// =================================================================== launchers
+static int grid_for(int64_t n, int block, int max_blocks) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > max_blocks) g = max_blocks;
+  return (int)g;
+}
+
+void launch_pipeline(const Plan& P, int grid, int block, size_t lds, hipStream_t s) {
+  switch (P.sink) {
+    case SINK_HASH_AGG: pipeline_kernel<SINK_HASH_AGG><<<grid, block, lds, s>>>(P); break;
+    case SINK_DIRECT_AGG: pipeline_kernel<SINK_DIRECT_AGG><<<grid, block, lds, s>>>(P); break;
+    case SINK_JOIN_BUILD: pipeline_kernel<SINK_JOIN_BUILD><<<grid, block, lds, s>>>(P); break;
+    case SINK_COLLECT: pipeline_kernel<SINK_COLLECT><<<grid, block, lds, s>>>(P); break;
+    case SINK_COUNT_PAIRS: pipeline_kernel<SINK_COUNT_PAIRS><<<grid, block, lds, s>>>(P); break;
+    case SINK_PROJECT: pipeline_kernel<SINK_PROJECT><<<grid, block, lds, s>>>(P); break;
+    case SINK_HASH: pipeline_kernel<SINK_HASH><<<grid, block, lds, s>>>(P); break;
+    case SINK_FLAGS: pipeline_kernel<SINK_FLAGS><<<grid, block, lds, s>>>(P); break;
+    default: break;
+  }
+}
+
+void launch_direct_reduce(const uint64_t* partials, int nblocks, int ncells, int nacc, const AccSlotList& accs,
+                          uint64_t* out, hipStream_t s) {
+  direct_reduce_kernel<<<(ncells + 63) / 64, 64, 0, s>>>(partials, nblocks, ncells, nacc, accs, out);
+}
+
+void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  fill_u64_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(p, v, n);
+}
+
+void launch_gather(const void* src, int width, const int64_t* idx, int64_t n, void* dst, const uint8_t* src_valid,
+                   uint8_t* dst_valid, hipStream_t s) {
+  if (n <= 0) return;
+  gather_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(src, width, idx, n, dst, src_valid, dst_valid);
+}
+
+void launch_join_insert(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                        uint64_t* bloom, hipStream_t s) {
+  if (n <= 0) return;
+  join_insert_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(T, keys, key_null, n, bloom);
+}
+
+void launch_join_unique(const DevJoinTable& T, uint32_t* dup_flag, hipStream_t s) {
+  join_unique_kernel<<<grid_for((int64_t)T.mask + 1, 256, 8192), 256, 0, s>>>(T, dup_flag);
+}
+
+void launch_agg_rehash(const uint64_t* old_slots, uint64_t old_cap, int nkw, uint64_t* new_slots, uint64_t new_mask,
+                       const AccMove& mv, hipStream_t s) {
+  agg_rehash_kernel<<<grid_for((int64_t)old_cap, 256, 8192), 256, 0, s>>>(old_slots, old_cap, nkw, new_slots,
+                                                                          new_mask, mv);
+}
+
+void launch_slot_flags(const uint64_t* slots, uint64_t cap, int nkw, uint8_t* flags, hipStream_t s) {
+  slot_flags_kernel<<<grid_for((int64_t)cap, 256, 8192), 256, 0, s>>>(slots, cap, nkw, flags);
+}
+
+void launch_agg_emit(const uint64_t* slots, int nkw, const int64_t* idx, int64_t n, const EmitPlan& E,
+                     hipStream_t s) {
+  if (n <= 0) return;
+  agg_emit_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(slots, nkw, idx, n, E);
+}
+
+int64_t compact_blocks(int64_t n) { return (n + CB - 1) / CB; }
+
+void launch_compact(const uint8_t* flags, int64_t n, uint32_t* block_counts, uint64_t* offsets, const uint32_t* sel,
+                    void* out, int out_u32, hipStream_t s) {
+  const int64_t nb = compact_blocks(n);
+  if (nb == 0) {
+    fill_u64_kernel<<<1, 64, 0, s>>>(offsets, 0, 1);
+    return;
+  }
+  flag_counts_kernel<<<(unsigned)nb, CT, 0, s>>>(flags, n, block_counts);
+  scan_counts_kernel<<<1, 1024, 0, s>>>(block_counts, nb, offsets);
+  flag_scatter_kernel<<<(unsigned)nb, CT, 0, s>>>(flags, n, offsets, sel, out, out_u32);
+}
+
+void launch_sort_keys(const Plan& P, const SortKeys& K, int64_t n, uint64_t* words, hipStream_t s) {
+  if (n <= 0) return;
+  sort_keys_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(P, K, n, words);
+}
+
+void launch_topn_tiles(const uint64_t* in, int64_t n, int nw, int keep, uint64_t* out, hipStream_t s) {
+  const int64_t tiles = (n + TOPN_TILE - 1) / TOPN_TILE;
+  const size_t lds = (size_t)TOPN_TILE * (nw + 1) * 8;
+  topn_tile_kernel<<<(unsigned)tiles, 256, lds, s>>>(in, n, nw, keep, out);
+}
+
+void launch_topn_rows(const uint64_t* words, int nw, int64_t n, int64_t* rows_out, hipStream_t s) {
+  if (n <= 0) return;
+  topn_rows_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(words, nw, n, rows_out);
+}
+
+}  // namespace mlg

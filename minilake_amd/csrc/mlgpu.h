@@ -1,0 +1,206 @@
+// Shared host/device definitions of the gfx950 execution path.
+//
+// Plans reach the GPU as a `Plan` kernel argument (uniform, scalar-loaded)
+// plus an instruction array in device memory. Expression programs are typed
+// register code: each instruction names the stack slot it writes (its depth),
+// so the interpreter never indexes registers dynamically.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mlg {
+
+constexpr int MAX_DEPTH = 6;      // expression stack slots held in registers
+constexpr int MAX_STAGES = 8;     // row-index stages: 0 = source, k = k-th probe
+constexpr int MAX_STEPS = 12;     // filters + probes in one fused pipeline
+constexpr int MAX_KEYS = 4;       // key columns (one 64-bit word each)
+constexpr int MAX_AGG_SLOTS = 24; // accumulator state slots per aggregate sink
+constexpr int MAX_COLS = 32;      // materialized columns (collect / join payload)
+
+// MiniLake's hash constants (crates/minilake-hashtable/src/hash.rs:12-17).
+constexpr uint64_t HSEED = 0x243F6A8885A308D3ull;
+constexpr uint64_t HMUL = 0x9E3779B97F4A7C15ull;
+constexpr uint64_t HMUL2 = 0xBF58476D1CE4E5B9ull;
+constexpr uint64_t HNULL = 0x7C159E3779B97F4Aull;
+
+__host__ __device__ inline uint64_t fold_mul(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (a * b) ^ __umul64hi(a, b);
+#else
+  unsigned __int128 r = (unsigned __int128)a * b;
+  return (uint64_t)r ^ (uint64_t)(r >> 64);
+#endif
+}
+__host__ __device__ inline uint64_t hash_u64(uint64_t x) { return fold_mul(x ^ HSEED, HMUL); }
+__host__ __device__ inline uint64_t hash_combine(uint64_t h, uint64_t v) { return fold_mul(h ^ v, HMUL2); }
+
+// ------------------------------------------------------------------ opcodes
+enum Opcode : uint16_t {
+  // push (write slot `depth`)
+  OP_LOAD_I32 = 0,  // sign-extended
+  OP_LOAD_U32,      // dictionary codes
+  OP_LOAD_I64,
+  OP_LOAD_U8,       // booleans (normalized to 0/1)
+  OP_CONST,
+  // unary on slot depth-1
+  OP_NOT,
+  OP_NEG_I32,
+  OP_NEG_I64,
+  OP_NEG_F64,
+  OP_I2F,           // i64 (or sign-extended i32 / bool) -> f64
+  OP_F2I64,         // saturating, NaN -> 0 (Rust `as`)
+  OP_F2I32,
+  OP_I64_TO_I32,    // truncate, sign-extend
+  OP_LOOKUP_U8,     // slot = table[slot] (dictionary predicate per entry)
+  OP_LOOKUP_U64,    // slot = table[slot] (dictionary hash / rank per entry)
+  OP_KEY_F64,       // -0.0 -> +0.0 and one canonical NaN (group/join key words)
+  OP_HASH_F64,      // hash input word: 0 if value == 0.0 else bits (kernels/hash.rs:53)
+  // binary: slots depth-2, depth-1 -> depth-2
+  OP_ADD_I32, OP_SUB_I32, OP_MUL_I32, OP_MOD_I32,
+  OP_ADD_I64, OP_SUB_I64, OP_MUL_I64, OP_MOD_I64,
+  OP_ADD_F64, OP_SUB_F64, OP_MUL_F64, OP_DIV_F64, OP_MOD_F64,
+  OP_EQ_I, OP_NE_I, OP_LT_I, OP_LE_I, OP_GT_I, OP_GE_I,
+  OP_EQ_F, OP_NE_F, OP_LT_F, OP_LE_F, OP_GT_F, OP_GE_F,
+  OP_AND, OP_OR,
+  OP_COUNT
+};
+
+struct Instr {
+  uint16_t op;
+  uint8_t depth;   // stack depth before the instruction
+  uint8_t stage;   // LOAD: which row index
+  uint32_t flags;  // CONST: 1 = valid
+  uint64_t imm;    // LOAD: data pointer; CONST: value bits; LOOKUP: table pointer
+  uint64_t imm2;   // LOAD: u8 validity pointer or 0
+};
+
+struct Prog {
+  int32_t first = 0;     // index into the plan's instruction array
+  int16_t count = 0;
+  int16_t nullable = 0;  // 1 if the program can produce NULL (tracks validity bits)
+};
+
+// ---------------------------------------------------------------- hash tables
+// Join table: open addressing, linear probing; slot = header word then key
+// words. header = (tag << 32) | (build_row + 1), 0 = empty.
+struct DevJoinTable {
+  uint64_t* slots = nullptr;
+  uint64_t mask = 0;       // capacity - 1
+  int32_t nkw = 1;         // key words
+  int32_t unique = 1;      // every key distinct: stop at the first match
+  const uint64_t* bloom = nullptr;  // blocked Bloom filter (one u64 per block) or null
+  uint64_t bloom_mask = 0;
+};
+
+// Aggregate table: header word per slot (0 empty, 1 being written,
+// otherwise (hash & ~3) | 2), key words AoS, accumulators SoA by slot.
+enum AccKind : int32_t {
+  ACC_SUM_F64 = 0,   // f64 += v
+  ACC_SUM_I64 = 1,   // wrapping i64 += v
+  ACC_COUNT = 2,     // i64 += 1 (when the argument is valid)
+  ACC_MIN_I64 = 3,
+  ACC_MAX_I64 = 4,
+  ACC_MIN_F64 = 5,
+  ACC_MAX_F64 = 6,
+};
+// Identity element of each accumulator state.
+__host__ __device__ inline uint64_t acc_init_bits(int kind) {
+  switch (kind) {
+    case ACC_MIN_I64: return 0x7FFFFFFFFFFFFFFFull;
+    case ACC_MAX_I64: return 0x8000000000000000ull;
+    case ACC_MIN_F64: return 0x7FF0000000000000ull;  // +inf
+    case ACC_MAX_F64: return 0xFFF0000000000000ull;  // -inf
+    default: return 0;
+  }
+}
+
+struct AccSlot {
+  int32_t kind;
+  int32_t arg;       // index of the argument program (-1 = rows counter)
+};
+
+enum SinkKind : int32_t {
+  SINK_NONE = 0,
+  SINK_HASH_AGG = 1,
+  SINK_DIRECT_AGG = 2,
+  SINK_JOIN_BUILD = 3,
+  SINK_COLLECT = 4,
+  SINK_COUNT_PAIRS = 5,
+  SINK_PROJECT = 6,   // write every column at its physical row
+  SINK_HASH = 7,      // hash_columns: one hash per active row
+  SINK_FLAGS = 8,     // filter: one 0/1 flag per active row
+};
+
+enum StepKind : int32_t { STEP_FILTER = 0, STEP_PROBE = 1 };
+
+struct Step {
+  int32_t kind;
+  int32_t stage;       // PROBE: stage whose row index it sets
+  int32_t table;       // PROBE: index into Plan::tables
+  int32_t nkeys;
+  Prog pred;           // FILTER
+  Prog keys[MAX_KEYS]; // PROBE: key word programs
+};
+
+struct Plan {
+  // source rows: i in [begin, end) of the selection (or of the rows)
+  const uint32_t* sel;
+  int64_t begin;
+  int64_t end;
+  const Instr* ins;
+
+  int32_t nsteps;
+  int32_t expand_step;  // index of the one probe that may match many rows, or -1
+  Step steps[MAX_STEPS];
+  DevJoinTable tables[MAX_STAGES];
+
+  // ---- sink
+  int32_t sink;
+  int32_t nkeys;             // AGG / JOIN_BUILD: key programs
+  Prog keys[MAX_KEYS];
+  int32_t key_nullable;      // AGG: keys may be NULL (adds a null-mask word)
+  int32_t key_prehashed;     // HASH: bit k set = key k already is a 64-bit hash
+  uint64_t* hash_out;        // HASH: one hash per active row
+  uint8_t* flag_out;         // FILTER: one predicate result per active row
+
+  // hash aggregate
+  uint64_t* agg_slots;       // header + key words, stride 1 + agg_nkw
+  int32_t agg_nkw;
+  int32_t nacc;
+  uint64_t agg_mask;
+  uint64_t* agg_count;       // number of groups (device counter)
+  uint32_t* agg_overflow;    // set when the table passes its fill limit
+  uint64_t agg_fill_limit;
+  AccSlot acc[MAX_AGG_SLOTS];
+  int32_t nargs;
+  Prog args[MAX_AGG_SLOTS];  // argument programs
+  void* acc_data[MAX_AGG_SLOTS];  // SoA arrays (f64 or i64) of capacity slots
+
+  // direct (small-domain) aggregate: key k contributes (word * dim_stride[k])
+  int64_t dim_stride[MAX_KEYS];
+  int64_t dim_size[MAX_KEYS];
+  int32_t ngroups;
+  void* direct_partials;     // [grid][ngroups][nacc] 8-byte words
+
+  // join build / collect: materialized columns
+  int32_t ncols;
+  Prog cols[MAX_COLS];
+  int32_t col_width[MAX_COLS];
+  void* col_out[MAX_COLS];
+  uint8_t* col_valid[MAX_COLS];  // u8 validity or null
+  uint64_t* key_out;         // JOIN_BUILD: key words [rows][nkeys]
+  uint8_t* key_null;         // JOIN_BUILD: 1 if any key is NULL
+  uint64_t* out_count;       // rows written so far (device counter)
+  uint64_t out_capacity;
+  uint32_t* out_overflow;
+  int64_t* pair_out;         // COLLECT with row ids: [rows][nstages]
+  int32_t pair_stages;
+};
+
+// Sortable key words for TopN.
+struct SortPlan {
+  int32_t nwords;
+};
+
+}  // namespace mlg

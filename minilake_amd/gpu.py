@@ -1,0 +1,391 @@
+"""Host-side mirror of MiniLake's operator interface over the gfx950 C ABI.
+
+``GpuEngine`` exposes the operators and sinks of the reference executor
+(crates/minilake-exec/src/pipeline.rs:32-79) on device-resident tables:
+
+  join_build   JoinBuildSink + build_table        (operators/join.rs:72-203)
+  hashagg      HashAggregateSink                  (operators/aggregate/hash.rs:285-484)
+  topn         SortSink with a limit              (operators/sort.rs:145-268)
+  collect      CollectSink                        (operators/collect.rs:11-53)
+  filter       FilterOperator / select            (operators/filter.rs:19-25)
+  project      ProjectionOperator                 (operators/projection.rs:21-32)
+  hash_columns kernels/hash.rs:16-23
+  join_probe   HashJoinProbe::execute             (operators/join.rs:263-315)
+
+Every pipeline (source -> filters/probes -> sink) runs as one fused kernel.
+PyTorch only provides device memory and the stream; all compute is in
+``libminilake_gpu.so``. There is no CPU fallback: without the library or a
+GPU, constructing an engine raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import abi
+from .table import NP_DTYPE, HostColumn, HostTable, make_dict
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libminilake_gpu.so")
+_lib = None
+
+# Every symbol include/minilake_gpu.h declares.
+EXPORTED = [
+    "ml_default_config", "ml_ctx_create", "ml_ctx_destroy", "ml_last_error", "ml_ctx_synchronize", "ml_ctx_stream",
+    "ml_result_num_rows", "ml_result_num_columns", "ml_result_column", "ml_result_copy_column",
+    "ml_result_copy_validity", "ml_result_free", "ml_expr_type", "ml_filter", "ml_project", "ml_hash_columns",
+    "ml_hashagg_create", "ml_hashagg_sink", "ml_hashagg_finalize", "ml_hashagg_num_groups",
+    "ml_hashagg_table_stats", "ml_hashagg_free", "ml_join_build_create", "ml_join_build_sink",
+    "ml_join_build_finalize", "ml_join_table_num_rows", "ml_join_table_unique", "ml_join_table_free",
+    "ml_join_probe", "ml_pipeline_hashagg", "ml_pipeline_join_build", "ml_pipeline_topn", "ml_pipeline_collect",
+    "ml_topn_create", "ml_topn_sink", "ml_topn_finalize", "ml_topn_free",
+]
+
+
+class GpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{abi.STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libminilake_gpu.so (building it first if it is absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            from . import build
+            build.build_gpu_library()
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        P = C.POINTER
+        sig = {
+            "ml_default_config": ([P(abi.MlExecConfig)], None),
+            "ml_ctx_create": ([P(abi.MlExecConfig), P(vp)], i32),
+            "ml_ctx_destroy": ([vp], None),
+            "ml_last_error": ([vp], C.c_char_p),
+            "ml_ctx_synchronize": ([vp], i32),
+            "ml_ctx_stream": ([vp], vp),
+            "ml_result_num_rows": ([vp], i64),
+            "ml_result_num_columns": ([vp], i32),
+            "ml_result_column": ([vp, i32], P(abi.MlColumn)),
+            "ml_result_copy_column": ([vp, vp, i32, vp], i32),
+            "ml_result_copy_validity": ([vp, vp, i32, vp], i32),
+            "ml_result_free": ([vp], None),
+            "ml_expr_type": ([vp, P(abi.MlExpr), P(abi.MlBatch), P(i32)], i32),
+            "ml_filter": ([vp, P(abi.MlExpr), P(abi.MlBatch), vp, P(i64)], i32),
+            "ml_project": ([vp, P(abi.MlExpr), i32, P(abi.MlBatch), P(vp)], i32),
+            "ml_hash_columns": ([vp, P(abi.MlExpr), i32, P(abi.MlBatch), vp], i32),
+            "ml_hashagg_create": ([vp, P(abi.MlAggSpec), P(vp)], i32),
+            "ml_hashagg_sink": ([vp, P(abi.MlBatch)], i32),
+            "ml_hashagg_finalize": ([vp, P(vp)], i32),
+            "ml_hashagg_num_groups": ([vp, P(i64)], i32),
+            "ml_hashagg_table_stats": ([vp, P(i64), P(i32)], i32),
+            "ml_hashagg_free": ([vp], None),
+            "ml_join_build_create": ([vp, P(abi.MlJoinBuildSpec), P(vp)], i32),
+            "ml_join_build_sink": ([vp, P(abi.MlBatch)], i32),
+            "ml_join_build_finalize": ([vp, P(vp)], i32),
+            "ml_join_table_num_rows": ([vp], i64),
+            "ml_join_table_unique": ([vp], i32),
+            "ml_join_table_free": ([vp], None),
+            "ml_join_probe": ([vp, vp, P(abi.MlExpr), i32, P(abi.MlBatch), P(vp)], i32),
+            "ml_pipeline_hashagg": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, vp], i32),
+            "ml_pipeline_join_build": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, vp], i32),
+            "ml_pipeline_topn": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, vp], i32),
+            "ml_pipeline_collect": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, P(vp)], i32),
+            "ml_topn_create": ([vp, P(abi.MlSortSpec), P(vp)], i32),
+            "ml_topn_sink": ([vp, P(abi.MlBatch)], i32),
+            "ml_topn_finalize": ([vp, P(vp)], i32),
+            "ml_topn_free": ([vp], None),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols(path: str = LIB_PATH):
+    """Names from EXPORTED that the built library actually exports (no GPU needed)."""
+    L = C.CDLL(path)
+    return [n for n in EXPORTED if hasattr(L, n)]
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("the gfx950 engine needs a visible GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+_TORCH_DT = {abi.BOOLEAN: "uint8", abi.INT32: "int32", abi.INT64: "int64", abi.FLOAT64: "float64",
+             abi.DATE: "int32", abi.UTF8: "int32"}
+
+
+class DeviceTable:
+    """A HostTable uploaded to HBM (one tensor per column, dictionaries kept on the host)."""
+
+    def __init__(self, host: HostTable, device="cuda"):
+        torch = _torch()
+        self.host = host
+        self.num_rows = host.num_rows
+        self.types = host.types
+        self.tensors = []
+        for c in host.columns:
+            if c.validity is not None and not np.all(c.validity):
+                raise GpuError(abi.ML_ERR_UNSUPPORTED, "the GPU path takes NULL-free input columns")
+            v = c.values.view(np.int32) if c.type == abi.UTF8 else c.values
+            self.tensors.append(torch.from_numpy(np.ascontiguousarray(v)).to(device))
+        self.dicts = [c.dictionary for c in host.columns]
+
+    def nbytes(self):
+        return sum(t.numel() * t.element_size() for t in self.tensors)
+
+    def ml_batch(self, keep: abi.Keep, selection=None) -> abi.MlBatch:
+        cols = keep((abi.MlColumn * max(1, len(self.tensors)))())
+        dcache = {}
+        for i, (t, ty) in enumerate(zip(self.tensors, self.types)):
+            cols[i].type = ty
+            cols[i].data = t.data_ptr()
+            cols[i].len = self.num_rows
+            if ty == abi.UTF8:
+                key = id(self.dicts[i])
+                if key not in dcache:
+                    dcache[key] = make_dict(self.dicts[i], keep)
+                cols[i].dict = C.pointer(dcache[key])
+        sel_ptr, sel_len = None, 0
+        if selection is not None:
+            sel_ptr, sel_len = selection.data_ptr(), selection.numel()
+        return keep(abi.MlBatch(cols, len(self.tensors), 0, self.num_rows, sel_ptr, sel_len))
+
+
+class GpuResult:
+    """A library-owned device result (ml_result): usable as a source or copied to the host."""
+
+    def __init__(self, engine: "GpuEngine", handle):
+        self.engine, self.handle = engine, handle
+        L = lib()
+        self.num_rows = L.ml_result_num_rows(handle)
+        self.num_columns = L.ml_result_num_columns(handle)
+        self.types = [L.ml_result_column(handle, i).contents.type for i in range(self.num_columns)]
+
+    def ml_batch(self, keep: abi.Keep, selection=None) -> abi.MlBatch:
+        L = lib()
+        cols = keep((abi.MlColumn * max(1, self.num_columns))())
+        for i in range(self.num_columns):
+            cols[i] = L.ml_result_column(self.handle, i).contents
+        sel_ptr, sel_len = None, 0
+        if selection is not None:
+            sel_ptr, sel_len = selection.data_ptr(), selection.numel()
+        return keep(abi.MlBatch(cols, self.num_columns, 0, self.num_rows, sel_ptr, sel_len))
+
+    def to_host(self) -> HostTable:
+        L = lib()
+        cols = []
+        for i in range(self.num_columns):
+            mc = L.ml_result_column(self.handle, i).contents
+            dt = NP_DTYPE[mc.type]
+            vals = np.zeros(max(1, self.num_rows), dtype=dt)
+            valid = np.zeros(max(1, self.num_rows), dtype=np.uint8)
+            self.engine._check(L.ml_result_copy_column(self.engine.ctx, self.handle, i, vals.ctypes.data))
+            self.engine._check(L.ml_result_copy_validity(self.engine.ctx, self.handle, i, valid.ctypes.data))
+            vals, valid = vals[:self.num_rows], valid[:self.num_rows].astype(bool)
+            d = None
+            if mc.type == abi.UTF8:
+                md = mc.dict.contents
+                offs = [md.offsets[k] for k in range(md.len + 1)]
+                raw = C.string_at(md.bytes, offs[-1]) if offs[-1] else b""
+                d = [raw[offs[k]:offs[k + 1]].decode() for k in range(md.len)]
+            cols.append(HostColumn(mc.type, vals, d, None if valid.all() else valid))
+        return HostTable(cols, num_rows=self.num_rows)
+
+    def __del__(self):
+        if self.handle and _lib is not None:
+            _lib.ml_result_free(self.handle)
+            self.handle = None
+
+
+class GpuJoin:
+    def __init__(self, handle):
+        self.handle = handle
+
+    def num_rows(self):
+        return lib().ml_join_table_num_rows(self.handle)
+
+    def unique(self):
+        return bool(lib().ml_join_table_unique(self.handle))
+
+    def __del__(self):
+        if self.handle and _lib is not None:
+            _lib.ml_join_table_free(self.handle)
+            self.handle = None
+
+
+class GpuEngine:
+    """The gfx950 executor behind the engine interface of minilake_amd.queries."""
+
+    name = "gpu"
+
+    def __init__(self, device: int = 0, batch_rows: int = 1 << 21, use_bloom: int = -1, block_size: int = 256,
+                 grid_per_cu: int = 8):
+        torch = _torch()
+        L = lib()
+        self.torch = torch
+        torch.cuda.set_device(device)
+        cfg = abi.MlExecConfig()
+        L.ml_default_config(C.byref(cfg))
+        cfg.batch_rows, cfg.device, cfg.use_bloom = batch_rows, device, use_bloom
+        cfg.block_size, cfg.grid_per_cu = block_size, grid_per_cu
+        cfg.stream = torch.cuda.current_stream(device).cuda_stream
+        ctx = C.c_void_p()
+        rc = L.ml_ctx_create(C.byref(cfg), C.byref(ctx))
+        if rc != 0:
+            raise GpuError(rc, "ml_ctx_create failed")
+        self.ctx = ctx.value
+        self.device = device
+
+    def close(self):
+        if self.ctx:
+            lib().ml_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GpuError(rc, lib().ml_last_error(self.ctx).decode())
+
+    def synchronize(self):
+        self._check(lib().ml_ctx_synchronize(self.ctx))
+
+    # ------------------------------------------------------------ tables
+    def table(self, t) -> DeviceTable:
+        if isinstance(t, (DeviceTable, GpuResult)):
+            return t
+        dt = DeviceTable(t, device=f"cuda:{self.device}")
+        self.torch.cuda.synchronize()
+        return dt
+
+    def _src(self, src, keep, selection=None):
+        src = self.table(src)
+        sel = None
+        if selection is not None:
+            sel = self.torch.from_numpy(np.ascontiguousarray(selection, dtype=np.uint32).view(np.int32)).to(
+                f"cuda:{self.device}")
+            keep(sel)
+        return src, src.ml_batch(keep, sel)
+
+    def _ops(self, ops, keep):
+        return abi.op_array(ops, keep, table_handle=lambda j: j.handle)
+
+    # ------------------------------------------------------------ sinks
+    def join_build(self, src, ops, keys, key_types) -> GpuJoin:
+        L, keep = lib(), abi.Keep()
+        _, b = self._src(src, keep)
+        spec = abi.join_build_spec(keys, key_types, keep)
+        jb = C.c_void_p()
+        self._check(L.ml_join_build_create(self.ctx, C.byref(spec), C.byref(jb)))
+        rc = L.ml_pipeline_join_build(self.ctx, C.byref(b), self._ops(ops, keep), len(ops), jb)
+        if rc != 0:
+            msg = L.ml_last_error(self.ctx).decode()
+            t = C.c_void_p()
+            L.ml_join_build_finalize(jb, C.byref(t))  # frees the builder
+            if t.value:
+                L.ml_join_table_free(t)
+            raise GpuError(rc, msg)
+        t = C.c_void_p()
+        self._check(L.ml_join_build_finalize(jb, C.byref(t)))
+        return GpuJoin(t.value)
+
+    def hashagg(self, src, ops, group_exprs, key_types, aggs, sizing=abi.SIZING_PLAIN, capacity_hint=0,
+                stats: Optional[dict] = None) -> GpuResult:
+        L, keep = lib(), abi.Keep()
+        _, b = self._src(src, keep)
+        spec = abi.agg_spec(group_exprs, key_types, aggs, keep, sizing, capacity_hint)
+        h = C.c_void_p()
+        self._check(L.ml_hashagg_create(self.ctx, C.byref(spec), C.byref(h)))
+        try:
+            self._check(L.ml_pipeline_hashagg(self.ctx, C.byref(b), self._ops(ops, keep), len(ops), h))
+            out = C.c_void_p()
+            self._check(L.ml_hashagg_finalize(h, C.byref(out)))
+            if stats is not None:
+                cap, res = C.c_int64(), C.c_int32()
+                self._check(L.ml_hashagg_table_stats(h, C.byref(cap), C.byref(res)))
+                stats["capacity"], stats["resizes"] = cap.value, res.value
+        finally:
+            L.ml_hashagg_free(h)
+        return GpuResult(self, out.value)
+
+    def topn(self, src, ops, sort_keys, limit) -> GpuResult:
+        L, keep = lib(), abi.Keep()
+        _, b = self._src(src, keep)
+        spec = abi.sort_spec(sort_keys, limit, keep)
+        h = C.c_void_p()
+        self._check(L.ml_topn_create(self.ctx, C.byref(spec), C.byref(h)))
+        try:
+            self._check(L.ml_pipeline_topn(self.ctx, C.byref(b), self._ops(ops, keep), len(ops), h))
+            out = C.c_void_p()
+            self._check(L.ml_topn_finalize(h, C.byref(out)))
+        finally:
+            L.ml_topn_free(h)
+        return GpuResult(self, out.value)
+
+    def collect(self, src, ops) -> GpuResult:
+        L, keep = lib(), abi.Keep()
+        _, b = self._src(src, keep)
+        out = C.c_void_p()
+        self._check(L.ml_pipeline_collect(self.ctx, C.byref(b), self._ops(ops, keep), len(ops), C.byref(out)))
+        return GpuResult(self, out.value)
+
+    # -------------------------------------------------------- operators
+    def expr_type(self, e, src):
+        L, keep = lib(), abi.Keep()
+        _, b = self._src(src, keep)
+        me = abi.to_ml_expr(e, keep)
+        t = C.c_int32()
+        self._check(L.ml_expr_type(self.ctx, C.byref(me), C.byref(b), C.byref(t)))
+        return t.value
+
+    def filter(self, pred, src, selection: Optional[np.ndarray] = None) -> np.ndarray:
+        L, keep = lib(), abi.Keep()
+        s, b = self._src(src, keep, selection)
+        n = s.num_rows if selection is None else len(selection)
+        out = self.torch.zeros(max(1, n), dtype=self.torch.int32, device=f"cuda:{self.device}")
+        me = abi.to_ml_expr(pred, keep)
+        cnt = C.c_int64()
+        self._check(L.ml_filter(self.ctx, C.byref(me), C.byref(b), out.data_ptr(), C.byref(cnt)))
+        return out[:cnt.value].cpu().numpy().view(np.uint32).copy()
+
+    def project(self, exprs: Sequence, src) -> HostTable:
+        L, keep = lib(), abi.Keep()
+        _, b = self._src(src, keep)
+        arr = abi.expr_array(exprs, keep)
+        out = C.c_void_p()
+        self._check(L.ml_project(self.ctx, arr, len(exprs), C.byref(b), C.byref(out)))
+        return GpuResult(self, out.value).to_host()
+
+    def hash_columns(self, keys: Sequence, src, selection: Optional[np.ndarray] = None) -> np.ndarray:
+        L, keep = lib(), abi.Keep()
+        s, b = self._src(src, keep, selection)
+        n = s.num_rows if selection is None else len(selection)
+        out = self.torch.zeros(max(1, n), dtype=self.torch.int64, device=f"cuda:{self.device}")
+        arr = abi.expr_array(keys, keep)
+        self._check(L.ml_hash_columns(self.ctx, arr, len(keys), C.byref(b), out.data_ptr()))
+        return out[:n].cpu().numpy().view(np.uint64).copy()
+
+    def join_probe(self, jt: GpuJoin, keys: Sequence, src, selection: Optional[np.ndarray] = None):
+        L, keep = lib(), abi.Keep()
+        _, b = self._src(src, keep, selection)
+        arr = abi.expr_array(keys, keep)
+        out = C.c_void_p()
+        self._check(L.ml_join_probe(self.ctx, jt.handle, arr, len(keys), C.byref(b), C.byref(out)))
+        t = GpuResult(self, out.value).to_host()
+        return t.columns[0].values, t.columns[1].values

@@ -1,0 +1,252 @@
+"""CPU ORACLE — test infrastructure only.
+
+ctypes front end of ``liboracle.so``, the C++ restatement of MiniLake's CPU
+executor (see ref_core.h). Only ``tests/``, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` use it, and only as the checker / the
+timed baseline; the product never imports it.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Optional, Sequence
+
+import numpy as np
+
+from minilake_amd import abi
+from minilake_amd.table import HostColumn, HostTable, column_from_ml, host_batch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{abi.STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class MloOptions(C.Structure):
+    _fields_ = [("threads", C.c_int32), ("batch_size", C.c_int32), ("morsel_rows", C.c_int64)]
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64
+        L.mlo_last_error.restype = C.c_char_p
+        for name in ("mlo_fold_mul", "mlo_combine"):
+            getattr(L, name).argtypes = [u64, u64]
+            getattr(L, name).restype = u64
+        L.mlo_hash_u64.argtypes = [u64]
+        L.mlo_hash_u64.restype = u64
+        L.mlo_hash_bytes.argtypes = [C.c_char_p, u64]
+        L.mlo_hash_bytes.restype = u64
+        L.mlo_match_tag.argtypes = [u64, C.c_uint8]
+        L.mlo_match_tag.restype = u64
+        L.mlo_match_empty.argtypes = [u64]
+        L.mlo_match_empty.restype = u64
+        L.mlo_swiss_group_ids.argtypes = [vp, i64, i64, i32, vp, C.POINTER(i64), C.POINTER(i32)]
+        L.mlo_swiss_group_ids.restype = i64
+        L.mlo_expr_type.argtypes = [C.POINTER(abi.MlExpr), C.POINTER(abi.MlBatch), C.POINTER(i32)]
+        L.mlo_filter.argtypes = [C.POINTER(abi.MlExpr), C.POINTER(abi.MlBatch), vp, C.POINTER(i64)]
+        L.mlo_project.argtypes = [C.POINTER(abi.MlExpr), i32, C.POINTER(abi.MlBatch), C.POINTER(vp)]
+        L.mlo_hash_columns.argtypes = [C.POINTER(abi.MlExpr), i32, C.POINTER(abi.MlBatch), vp]
+        L.mlo_pipeline_hashagg.argtypes = [C.POINTER(abi.MlBatch), C.POINTER(abi.MlOperator), i32,
+                                           C.POINTER(abi.MlAggSpec), C.POINTER(MloOptions), C.POINTER(vp)]
+        L.mlo_pipeline_join_build.argtypes = [C.POINTER(abi.MlBatch), C.POINTER(abi.MlOperator), i32,
+                                              C.POINTER(abi.MlJoinBuildSpec), C.POINTER(MloOptions), C.POINTER(vp)]
+        L.mlo_pipeline_topn.argtypes = [C.POINTER(abi.MlBatch), C.POINTER(abi.MlOperator), i32,
+                                        C.POINTER(abi.MlSortSpec), C.POINTER(MloOptions), C.POINTER(vp)]
+        L.mlo_pipeline_collect.argtypes = [C.POINTER(abi.MlBatch), C.POINTER(abi.MlOperator), i32,
+                                           C.POINTER(MloOptions), C.POINTER(vp)]
+        L.mlo_join_probe.argtypes = [vp, C.POINTER(abi.MlExpr), i32, C.POINTER(abi.MlBatch), C.POINTER(vp)]
+        L.mlo_join_num_rows.argtypes = [vp]
+        L.mlo_join_num_rows.restype = i64
+        L.mlo_join_num_keys.argtypes = [vp]
+        L.mlo_join_num_keys.restype = i64
+        L.mlo_join_free.argtypes = [vp]
+        L.mlo_table_num_rows.argtypes = [vp]
+        L.mlo_table_num_rows.restype = i64
+        L.mlo_table_num_columns.argtypes = [vp]
+        L.mlo_table_num_columns.restype = i32
+        L.mlo_table_column.argtypes = [vp, i32]
+        L.mlo_table_column.restype = C.POINTER(abi.MlColumn)
+        L.mlo_table_resizes.argtypes = [vp]
+        L.mlo_table_resizes.restype = i32
+        L.mlo_table_capacity.argtypes = [vp]
+        L.mlo_table_capacity.restype = i64
+        L.mlo_table_free.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise OracleError(rc, lib().mlo_last_error().decode())
+
+
+# ------------------------------------------------------------- primitives
+def fold_mul(a, b): return lib().mlo_fold_mul(a, b)
+def hash_u64(x): return lib().mlo_hash_u64(x)
+def combine(h, v): return lib().mlo_combine(h, v)
+def hash_bytes(b: bytes): return lib().mlo_hash_bytes(b, len(b))
+def match_tag(group, tag): return lib().mlo_match_tag(group, tag)
+def match_empty(group): return lib().mlo_match_empty(group)
+
+
+def swiss_group_ids(keys: np.ndarray, batch: int = 0, weak: bool = False):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    out = np.zeros(len(keys), dtype=np.uint32)
+    cap, res = C.c_int64(), C.c_int32()
+    n = lib().mlo_swiss_group_ids(keys.ctypes.data, len(keys), batch, int(weak), out.ctypes.data,
+                                  C.byref(cap), C.byref(res))
+    if n < 0:
+        raise AssertionError("SwissTable lost an entry")
+    return out, int(n), int(cap.value), int(res.value)
+
+
+# ------------------------------------------------------------------ engine
+class OracleTable:
+    """A result of the oracle: host columns (owned by the C++ side until copied)."""
+
+    def __init__(self, host: HostTable, resizes=0, capacity=0):
+        self.host, self.resizes, self.capacity = host, resizes, capacity
+
+    def to_host(self) -> HostTable:
+        return self.host
+
+
+def _take_table(ptr) -> OracleTable:
+    L = lib()
+    n = L.mlo_table_num_rows(ptr)
+    cols = [column_from_ml(L.mlo_table_column(ptr, i).contents, n) for i in range(L.mlo_table_num_columns(ptr))]
+    t = OracleTable(HostTable(cols, num_rows=n), L.mlo_table_resizes(ptr), L.mlo_table_capacity(ptr))
+    L.mlo_table_free(ptr)
+    return t
+
+
+class OracleJoin:
+    def __init__(self, handle):
+        self.handle = handle
+
+    def num_rows(self):
+        return lib().mlo_join_num_rows(self.handle)
+
+    def __del__(self):
+        if self.handle and _lib is not None:
+            _lib.mlo_join_free(self.handle)
+            self.handle = None
+
+
+class OracleEngine:
+    """The reference executor (C++ restatement) behind the engine interface of minilake_amd.queries."""
+
+    name = "oracle"
+
+    def __init__(self, threads: int = 1, batch_size: int = 2048, morsel_rows: int = 122_880):
+        self.opts = MloOptions(threads, batch_size, morsel_rows)
+
+    def table(self, t: HostTable):
+        return t
+
+    @staticmethod
+    def _host(t):
+        return t.to_host() if isinstance(t, OracleTable) else t
+
+    def _ops(self, ops, keep):
+        return abi.op_array(ops, keep, table_handle=lambda j: j.handle)
+
+    def join_build(self, src, ops, keys, key_types):
+        keep = abi.Keep()
+        src = self._host(src)
+        b = host_batch(src, keep)
+        spec = abi.join_build_spec(keys, key_types, keep)
+        out = C.c_void_p()
+        _check(lib().mlo_pipeline_join_build(C.byref(b), self._ops(ops, keep), len(ops), C.byref(spec),
+                                             C.byref(self.opts), C.byref(out)))
+        return OracleJoin(out.value)
+
+    def hashagg(self, src, ops, group_exprs, key_types, aggs, sizing=abi.SIZING_PLAIN, capacity_hint=0):
+        keep = abi.Keep()
+        src = self._host(src)
+        b = host_batch(src, keep)
+        spec = abi.agg_spec(group_exprs, key_types, aggs, keep, sizing, capacity_hint)
+        out = C.c_void_p()
+        _check(lib().mlo_pipeline_hashagg(C.byref(b), self._ops(ops, keep), len(ops), C.byref(spec),
+                                          C.byref(self.opts), C.byref(out)))
+        return _take_table(out.value)
+
+    def topn(self, src, ops, sort_keys, limit):
+        keep = abi.Keep()
+        src = self._host(src)
+        b = host_batch(src, keep)
+        spec = abi.sort_spec(sort_keys, limit, keep)
+        out = C.c_void_p()
+        _check(lib().mlo_pipeline_topn(C.byref(b), self._ops(ops, keep), len(ops), C.byref(spec),
+                                       C.byref(self.opts), C.byref(out)))
+        return _take_table(out.value)
+
+    def collect(self, src, ops):
+        keep = abi.Keep()
+        b = host_batch(self._host(src), keep)
+        out = C.c_void_p()
+        _check(lib().mlo_pipeline_collect(C.byref(b), self._ops(ops, keep), len(ops), C.byref(self.opts),
+                                          C.byref(out)))
+        return _take_table(out.value)
+
+    def expr_type(self, e, src):
+        keep = abi.Keep()
+        b = host_batch(self._host(src), keep)
+        me = abi.to_ml_expr(e, keep)
+        t = C.c_int32()
+        _check(lib().mlo_expr_type(C.byref(me), C.byref(b), C.byref(t)))
+        return t.value
+
+    def filter(self, pred, src, selection: Optional[np.ndarray] = None) -> np.ndarray:
+        keep = abi.Keep()
+        src = self._host(src)
+        b = host_batch(src, keep, selection)
+        me = abi.to_ml_expr(pred, keep)
+        out = np.zeros(max(1, src.num_rows), dtype=np.uint32)
+        n = C.c_int64()
+        _check(lib().mlo_filter(C.byref(me), C.byref(b), out.ctypes.data, C.byref(n)))
+        return out[:n.value].copy()
+
+    def project(self, exprs: Sequence, src) -> HostTable:
+        keep = abi.Keep()
+        b = host_batch(self._host(src), keep)
+        arr = abi.expr_array(exprs, keep)
+        out = C.c_void_p()
+        _check(lib().mlo_project(arr, len(exprs), C.byref(b), C.byref(out)))
+        return _take_table(out.value).host
+
+    def hash_columns(self, keys: Sequence, src, selection: Optional[np.ndarray] = None) -> np.ndarray:
+        keep = abi.Keep()
+        src = self._host(src)
+        b = host_batch(src, keep, selection)
+        arr = abi.expr_array(keys, keep)
+        n = src.num_rows if selection is None else len(selection)
+        out = np.zeros(max(1, n), dtype=np.uint64)
+        _check(lib().mlo_hash_columns(arr, len(keys), C.byref(b), out.ctypes.data))
+        return out[:n].copy()
+
+    def join_probe(self, jt: OracleJoin, keys: Sequence, src, selection: Optional[np.ndarray] = None):
+        keep = abi.Keep()
+        b = host_batch(self._host(src), keep, selection)
+        arr = abi.expr_array(keys, keep)
+        out = C.c_void_p()
+        _check(lib().mlo_join_probe(jt.handle, arr, len(keys), C.byref(b), C.byref(out)))
+        t = _take_table(out.value).host
+        return t.columns[0].values, t.columns[1].values

@@ -1,0 +1,209 @@
+"""GPU parity of the single operators against the CPU oracle: filter
+(ascending selection, bit-exact), projection (per-row values bit-exact,
+NULLs from x/0 and x%0), hash_columns (bit-identical to the reference hash),
+join probe with duplicate build keys, hash aggregate edge cases, TopN."""
+
+import numpy as np
+import pytest
+
+from minilake_amd import abi
+from minilake_amd.abi import (AGG_AVG, AGG_COUNT, AGG_COUNT_STAR, AGG_MAX, AGG_MIN, AGG_SUM, BOOLEAN, DATE,
+                              FLOAT64, INT32, INT64, UTF8, Agg, col, date_lit, lit)
+from minilake_amd.table import HostColumn, HostTable
+from tests.compare import assert_tables_equal, rows_of
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_table(n=5000, seed=3):
+    rng = np.random.default_rng(seed)
+    i32 = rng.integers(-1000, 1000, n).astype(np.int32)
+    i64 = rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+    f64 = rng.normal(0, 100, n)
+    f64[::17] = 0.0
+    f64[5::31] = -0.0
+    b = (rng.integers(0, 2, n)).astype(np.uint8)
+    d = rng.integers(8000, 9000, n).astype(np.int32)
+    s = rng.integers(0, 4, n).astype(np.uint32)
+    small = rng.integers(-3, 4, n).astype(np.int32)  # includes 0 divisors
+    return HostTable([
+        HostColumn(INT32, i32), HostColumn(INT64, i64), HostColumn(FLOAT64, f64), HostColumn(BOOLEAN, b),
+        HostColumn(DATE, d), HostColumn(UTF8, s, ["apple", "banana", "cherry", "date"]), HostColumn(INT32, small),
+    ])
+
+
+PREDICATES = [
+    col(0).lt(lit(0)),
+    col(1).ge(lit(0)) & col(2).lt(lit(50.0)),
+    col(4).le(date_lit(8500)) | col(3).eq(lit(True)),
+    col(5).eq(lit("banana")),
+    col(5).gt(lit("b")),
+    ~(col(6).eq(lit(0))),
+    (col(0) % col(6)).eq(lit(1)),          # NULL where col6 == 0
+    (col(2) / col(6)).gt(lit(1.0)),         # NULL where col6 == 0
+    col(0).cast(FLOAT64).gt(col(2)),
+    lit(True) & col(3).eq(lit(False)),
+]
+
+
+@pytest.mark.parametrize("k", range(len(PREDICATES)))
+def test_filter_matches_oracle(gpu_engine, oracle_engine, k):
+    t = mixed_table()
+    pred = PREDICATES[k]
+    want = oracle_engine.filter(pred, t)
+    got = gpu_engine.filter(pred, t)
+    np.testing.assert_array_equal(got, want)
+    sel = np.arange(0, t.num_rows, 3, dtype=np.uint32)
+    np.testing.assert_array_equal(gpu_engine.filter(pred, t, sel), oracle_engine.filter(pred, t, sel))
+
+
+EXPRS = [
+    col(2) * (lit(1.0) - col(2)),
+    (col(2) * (lit(1.0) - col(2))) * (lit(1.0) + col(2)),
+    col(0) + col(6),
+    col(0) * lit(2147483) ,                # wrapping i32 multiply
+    col(1) * lit(12345678901),             # wrapping i64 multiply
+    col(0) / col(6),                       # f64, NULL on zero
+    col(0) % col(6),                       # NULL on zero
+    col(1) % col(6),
+    col(2) % lit(3.0),
+    col(4) + lit(30),                      # date arithmetic stays a date
+    col(4) - col(4),
+    -col(2),
+    -col(0),
+    col(2).cast(INT64),
+    col(2).cast(INT32),
+    col(1).cast(INT32),
+    col(3).cast(FLOAT64),
+    col(0) + col(1),
+    col(0) + lit(1.5),
+]
+
+
+@pytest.mark.parametrize("k", range(len(EXPRS)))
+def test_project_matches_oracle_bit_exact(gpu_engine, oracle_engine, k):
+    t = mixed_table()
+    e = EXPRS[k]
+    want = oracle_engine.project([e], t)
+    got = gpu_engine.project([e], t)
+    assert got.types == want.types
+    g, w = got.columns[0], want.columns[0]
+    gv = np.ones(t.num_rows, bool) if g.validity is None else g.validity
+    wv = np.ones(t.num_rows, bool) if w.validity is None else w.validity
+    np.testing.assert_array_equal(gv, wv)
+    # values must be bit-identical wherever valid
+    np.testing.assert_array_equal(g.values.view(np.uint8).reshape(t.num_rows, -1)[wv],
+                                  w.values.view(np.uint8).reshape(t.num_rows, -1)[wv])
+
+
+HASH_KEYS = [[col(0)], [col(1)], [col(2)], [col(3)], [col(4)], [col(5)], [col(1), col(0), col(5)],
+             [col(0) / col(6)], [col(2), col(4)]]
+
+
+@pytest.mark.parametrize("k", range(len(HASH_KEYS)))
+def test_hash_columns_bit_identical(gpu_engine, oracle_engine, k):
+    t = mixed_table()
+    keys = HASH_KEYS[k]
+    np.testing.assert_array_equal(gpu_engine.hash_columns(keys, t), oracle_engine.hash_columns(keys, t))
+    sel = np.arange(1, t.num_rows, 7, dtype=np.uint32)
+    np.testing.assert_array_equal(gpu_engine.hash_columns(keys, t, sel), oracle_engine.hash_columns(keys, t, sel))
+
+
+def test_join_probe_with_duplicate_build_keys(gpu_engine, oracle_engine):
+    rng = np.random.default_rng(5)
+    build = HostTable([HostColumn(INT64, rng.integers(0, 300, 1000)), HostColumn(INT32, np.arange(1000, dtype=np.int32))])
+    probe = HostTable([HostColumn(INT64, rng.integers(-10, 320, 4000))])
+    for eng in (gpu_engine, oracle_engine):
+        pass
+    jo = oracle_engine.join_build(build, [], [col(0)], [INT64])
+    jg = gpu_engine.join_build(build, [], [col(0)], [INT64])
+    assert not jg.unique()
+    po, bo = oracle_engine.join_probe(jo, [col(0)], probe)
+    pg, bg = gpu_engine.join_probe(jg, [col(0)], probe)
+    assert sorted(zip(pg.tolist(), bg.tolist())) == sorted(zip(po.tolist(), bo.tolist()))
+
+
+def test_expanding_probe_into_aggregate(gpu_engine, oracle_engine):
+    rng = np.random.default_rng(9)
+    build = HostTable([HostColumn(INT64, rng.integers(0, 50, 400)), HostColumn(FLOAT64, rng.random(400))])
+    probe = HostTable([HostColumn(INT64, rng.integers(0, 60, 3000)), HostColumn(INT32, rng.integers(0, 5, 3000).astype(np.int32))])
+    ops = [abi.probe_op(None, [col(0)])]
+    aggs = [Agg(AGG_SUM, col(3), FLOAT64), count_star_agg(), Agg(AGG_MAX, col(3), FLOAT64)]
+    for eng, t in ((oracle_engine, None), (gpu_engine, None)):
+        pass
+    jo = oracle_engine.join_build(build, [], [col(0)], [INT64])
+    jg = gpu_engine.join_build(build, [], [col(0)], [INT64])
+    want = oracle_engine.hashagg(probe, [abi.probe_op(jo, [col(0)])], [col(1)], [INT32], aggs).to_host()
+    got = gpu_engine.hashagg(probe, [abi.probe_op(jg, [col(0)])], [col(1)], [INT32], aggs).to_host()
+    assert_tables_equal(got, want, rtol=1e-12)
+
+
+def count_star_agg():
+    return Agg(AGG_COUNT_STAR)
+
+
+def test_hashagg_all_functions_and_null_keys(gpu_engine, oracle_engine):
+    t = mixed_table(20000)
+    aggs = [Agg(AGG_SUM, col(0), INT32), Agg(AGG_SUM, col(2), FLOAT64), Agg(AGG_AVG, col(0), INT32),
+            Agg(AGG_AVG, col(2), FLOAT64), Agg(AGG_MIN, col(4), DATE), Agg(AGG_MAX, col(1), INT64),
+            Agg(AGG_MIN, col(2), FLOAT64), Agg(AGG_COUNT, col(0) / col(6), FLOAT64), count_star_agg(),
+            Agg(AGG_SUM, col(0) % col(6), INT32)]
+    # group by an expression that is NULL when col6 == 0, and a dictionary column
+    group = [col(6) % col(6).cast(INT32) + col(0) % lit(7), col(5)]
+    want = oracle_engine.hashagg(t, [], group, [INT32, UTF8], aggs).to_host()
+    got = gpu_engine.hashagg(t, [], group, [INT32, UTF8], aggs).to_host()
+    assert_tables_equal(got, want, rtol=1e-12)
+
+
+def test_hashagg_high_cardinality_plain_resizes(gpu_engine, oracle_engine):
+    rng = np.random.default_rng(1)
+    n = 400_000
+    t = HostTable([HostColumn(INT64, rng.integers(0, 150_000, n)), HostColumn(FLOAT64, rng.random(n))])
+    aggs = [Agg(AGG_SUM, col(1), FLOAT64), count_star_agg()]
+    want = oracle_engine.hashagg(t, [], [col(0)], [INT64], aggs).to_host()
+    stats = {}
+    from minilake_amd.gpu import GpuEngine
+    small = GpuEngine(batch_rows=50_000)
+    try:
+        got = small.hashagg(t, [], [col(0)], [INT64], aggs, stats=stats).to_host()
+    finally:
+        small.close()
+    assert stats["resizes"] >= 2
+    assert_tables_equal(got, want, rtol=1e-12)
+
+
+def test_hashagg_learned_underestimate_falls_back(gpu_engine, oracle_engine):
+    rng = np.random.default_rng(2)
+    n = 200_000
+    t = HostTable([HostColumn(INT64, rng.integers(0, 50_000, n)), HostColumn(INT64, rng.integers(0, 100, n))])
+    aggs = [Agg(AGG_SUM, col(1), INT64), count_star_agg()]
+    want = oracle_engine.hashagg(t, [], [col(0)], [INT64], aggs).to_host()
+    got = gpu_engine.hashagg(t, [], [col(0)], [INT64], aggs, sizing=abi.SIZING_LEARNED, capacity_hint=100).to_host()
+    assert_tables_equal(got, want)
+    with pytest.raises(Exception):
+        gpu_engine.hashagg(t, [], [col(0)], [INT64], aggs, sizing=abi.SIZING_HINT, capacity_hint=100)
+
+
+def test_topn_keys_and_ties(gpu_engine, oracle_engine):
+    rng = np.random.default_rng(4)
+    n = 50_000
+    t = HostTable([HostColumn(FLOAT64, np.round(rng.normal(0, 10, n), 1)), HostColumn(DATE, rng.integers(0, 50, n).astype(np.int32)),
+                   HostColumn(INT64, np.arange(n, dtype=np.int64))])
+    keys = [(col(0), False, False), (col(1), True, False)]
+    for limit in (1, 10, 100):
+        want = rows_of(oracle_engine.topn(t, [], keys, limit).to_host())
+        got = rows_of(gpu_engine.topn(t, [], keys, limit).to_host())
+        assert [(r[0], r[1]) for r in got] == [(r[0], r[1]) for r in want]
+
+
+def test_collect_through_probe(gpu_engine, oracle_engine):
+    rng = np.random.default_rng(8)
+    build = HostTable([HostColumn(INT64, np.arange(1000, dtype=np.int64)), HostColumn(UTF8, rng.integers(0, 3, 1000).astype(np.uint32), ["x", "y", "z"])])
+    probe = HostTable([HostColumn(INT64, rng.integers(0, 2000, 5000)), HostColumn(FLOAT64, rng.random(5000))])
+    jo = oracle_engine.join_build(build, [], [col(0)], [INT64])
+    jg = gpu_engine.join_build(build, [], [col(0)], [INT64])
+    ops_o = [abi.filter_op(col(1).lt(lit(0.7))), abi.probe_op(jo, [col(0)], build_is_left=True)]
+    ops_g = [abi.filter_op(col(1).lt(lit(0.7))), abi.probe_op(jg, [col(0)], build_is_left=True)]
+    want = oracle_engine.collect(probe, ops_o).to_host()
+    got = gpu_engine.collect(probe, ops_g).to_host()
+    assert_tables_equal(got, want, rtol=0.0)
