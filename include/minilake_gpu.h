@@ -162,6 +162,20 @@ ml_status ml_ctx_synchronize(ml_ctx* ctx);
 /* The context's hipStream_t. */
 void* ml_ctx_stream(ml_ctx* ctx);
 
+/* Per-launch statistics of fused pipeline kernels (the GPU analog of
+ * OperatorMetrics rows_in / time, crates/minilake-exec/src/metrics.rs:13-40):
+ * kernel time from HIP events recorded on the launch stream. */
+typedef struct ml_launch_stat {
+  int32_t sink;   /* 1 hash agg, 2 direct agg, 3 join build, 4 collect, ... */
+  int32_t _pad;
+  int64_t rows;   /* source rows the launch covered */
+  double ms;      /* kernel duration */
+} ml_launch_stat;
+ml_status ml_ctx_enable_stats(ml_ctx* ctx, int32_t on);
+/* Synchronize, copy up to `cap` recorded launches to `out`, clear the record;
+ * returns the number written. */
+int32_t ml_ctx_take_stats(ml_ctx* ctx, ml_launch_stat* out, int32_t cap);
+
 /* Results: library-owned device columns (host copies on request). */
 int64_t ml_result_num_rows(const ml_result* r);
 int32_t ml_result_num_columns(const ml_result* r);
@@ -270,6 +284,10 @@ ml_status ml_join_build_finalize(ml_join_build* b, ml_join_table** out);
 int64_t ml_join_table_num_rows(const ml_join_table* t);
 /* 1 if every build key is distinct (each probe row matches at most once). */
 int32_t ml_join_table_unique(const ml_join_table* t);
+/* The build side's columns in table row order (JoinTable.batch, join.rs:36-45);
+ * build row indices returned by ml_join_probe index these. */
+int32_t ml_join_table_num_columns(const ml_join_table* t);
+const ml_column* ml_join_table_column(const ml_join_table* t, int32_t i);
 void ml_join_table_free(ml_join_table* t);
 
 /* HashJoinProbe::execute (join.rs:263-315): all matching (probe row, build row)

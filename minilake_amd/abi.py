@@ -115,6 +115,10 @@ class MlFragment(C.Structure):
                 ("num_groups", C.c_int64), ("out_sums", C.c_void_p), ("out_counts", C.c_void_p)]
 
 
+class MlLaunchStat(C.Structure):
+    _fields_ = [("sink", C.c_int32), ("_pad", C.c_int32), ("rows", C.c_int64), ("ms", C.c_double)]
+
+
 class MlFragmentTuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("tile_rows", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("lds_groups", C.c_int32)]

@@ -368,7 +368,7 @@ static bool compare_bytes(int op, const std::string& a, const std::string& b) { 
 const uint8_t* Compiler::dict_predicate(const HostDict& d, int op, const std::string& lit) {
   std::vector<uint8_t> per_entry(d.entries.size() + 1, 0);
   for (size_t k = 0; k < d.entries.size(); k++) per_entry[k] = compare_bytes(op, d.entries[k], lit);
-  Buffer b(ctx_->arena.get(), per_entry.size());
+  Buffer b(ctx_->arena, per_entry.size());
   hip_check(hipMemcpyAsync(b.ptr, per_entry.data(), per_entry.size(), hipMemcpyHostToDevice, ctx_->stream), "upload");
   hip_check(hipStreamSynchronize(ctx_->stream), "sync");
   const uint8_t* p = b.as<uint8_t>();
@@ -577,7 +577,7 @@ Prog Compiler::compile_hash_input(const NodePtr& n, bool* prehashed) {
       }
       hs[k] = h;
     }
-    Buffer b(ctx_->arena.get(), hs.size() * 8);
+    Buffer b(ctx_->arena, hs.size() * 8);
     hip_check(hipMemcpyAsync(b.ptr, hs.data(), hs.size() * 8, hipMemcpyHostToDevice, ctx_->stream), "upload");
     hip_check(hipStreamSynchronize(ctx_->stream), "sync");
     const uint64_t* tp = b.as<uint64_t>();

@@ -66,19 +66,21 @@ class DeviceArena {
   size_t in_use_ = 0;
 };
 
+// A device allocation; it keeps its arena alive, so results and tables may
+// outlive the context that made them.
 struct Buffer {
-  DeviceArena* arena = nullptr;
+  std::shared_ptr<DeviceArena> arena;
   void* ptr = nullptr;
   size_t bytes = 0;
   Buffer() = default;
-  Buffer(DeviceArena* a, size_t n) : arena(a), ptr(n ? a->alloc(n) : nullptr), bytes(n) {}
+  Buffer(const std::shared_ptr<DeviceArena>& a, size_t n) : arena(a), ptr(n ? a->alloc(n) : nullptr), bytes(n) {}
   Buffer(const Buffer&) = delete;
   Buffer& operator=(const Buffer&) = delete;
   Buffer(Buffer&& o) noexcept { *this = std::move(o); }
   Buffer& operator=(Buffer&& o) noexcept {
     if (this != &o) {
       reset();
-      arena = o.arena; ptr = o.ptr; bytes = o.bytes;
+      arena = std::move(o.arena); ptr = o.ptr; bytes = o.bytes;
       o.ptr = nullptr; o.bytes = 0;
     }
     return *this;
@@ -88,6 +90,7 @@ struct Buffer {
     if (ptr && arena) arena->free(ptr);
     ptr = nullptr;
     bytes = 0;
+    arena.reset();
   }
   template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
@@ -114,12 +117,21 @@ struct ml_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::string last_error;
-  std::unique_ptr<mlg::DeviceArena> arena;
+  std::shared_ptr<mlg::DeviceArena> arena;
   mlg::MlpModel* estimator = nullptr;
   // small per-context device scratch: counters and flags
   mlg::Buffer counters;  // 64 x u64
   std::map<const ml_dict*, std::shared_ptr<mlg::HostDict>> dicts;
   std::shared_ptr<mlg::HostDict> dict(const ml_dict* d);
+  // launch statistics
+  bool stats_on = false;
+  struct PendingStat {
+    int sink;
+    int64_t rows;
+    hipEvent_t start, stop;
+  };
+  std::vector<PendingStat> stats;
+  void launch(const mlg::Plan& P, int grid, int block, size_t lds);
 };
 
 namespace mlg {
@@ -137,6 +149,7 @@ struct ResultColumn {
 
 struct ml_result {
   ml_ctx* ctx = nullptr;
+  int device = 0;
   int64_t num_rows = 0;
   std::vector<mlg::ResultColumn> cols;
   std::vector<ml_column> views;
@@ -234,6 +247,7 @@ struct JoinTableImpl {
   Buffer slots;
   Buffer bloom;
   DevJoinTable dev{};
+  std::vector<ml_column> views;
 };
 
 }  // namespace mlg

@@ -222,7 +222,7 @@ void PipelineBuild::add_ops(const ml_operator* ops, int n) {
 
 void PipelineBuild::upload() {
   if (ins.empty()) ins.push_back(Instr{});
-  dev_ins = Buffer(ctx->arena.get(), ins.size() * sizeof(Instr));
+  dev_ins = Buffer(ctx->arena, ins.size() * sizeof(Instr));
   h2d(ctx, dev_ins.ptr, ins.data(), ins.size() * sizeof(Instr));
   P.ins = dev_ins.as<Instr>();
 }
@@ -267,7 +267,7 @@ static void run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buf
     B.P.out_count = counters;
     zero(ctx, counters, 16);
     B.upload();
-    launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+    ctx->launch(B.P, B.grid(), block_of(ctx), 0);
     uint64_t n = 0;
     d2h(ctx, &n, counters, 8);
     cap = n;
@@ -283,9 +283,9 @@ static void run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buf
       B.P.col_width[c] = w;
       out.types.push_back(v->type);
       out.dicts.push_back(v->dict);
-      out.cols.emplace_back(ctx->arena.get(), std::max<uint64_t>(cap, 1) * w);
+      out.cols.emplace_back(ctx->arena, std::max<uint64_t>(cap, 1) * w);
       out.valid.emplace_back();
-      if (nullable) out.valid.back() = Buffer(ctx->arena.get(), std::max<uint64_t>(cap, 1));
+      if (nullable) out.valid.back() = Buffer(ctx->arena, std::max<uint64_t>(cap, 1));
       B.P.col_out[c] = out.cols.back().ptr;
       B.P.col_valid[c] = out.valid.back().ptr ? out.valid.back().as<uint8_t>() : nullptr;
     }
@@ -293,7 +293,7 @@ static void run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buf
     B.P.ncols = 0;
   }
   if (pair_rows) {
-    *pair_rows = Buffer(ctx->arena.get(), std::max<uint64_t>(cap, 1) * pair_stages * 8);
+    *pair_rows = Buffer(ctx->arena, std::max<uint64_t>(cap, 1) * pair_stages * 8);
     B.P.pair_out = pair_rows->as<int64_t>();
     B.P.pair_stages = pair_stages;
   } else {
@@ -305,7 +305,7 @@ static void run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buf
   B.P.out_overflow = (uint32_t*)(counters + 1);
   zero(ctx, counters, 16);
   B.upload();
-  launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+  ctx->launch(B.P, B.grid(), block_of(ctx), 0);
   uint64_t hc[2] = {0, 0};
   d2h(ctx, hc, counters, 16);
   if (hc[1] & 1) fail(ML_ERR_INTERNAL, "collect output overflowed its capacity");
@@ -315,6 +315,7 @@ static void run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buf
 static ml_result* make_result(ml_ctx* ctx, Materialized& m) {
   auto* r = new ml_result();
   r->ctx = ctx;
+  r->device = ctx->device;
   r->num_rows = m.rows;
   for (size_t c = 0; c < m.types.size(); c++) {
     ResultColumn rc;
@@ -550,7 +551,7 @@ static AccSlotList slot_kinds(const ml_hashagg* A) {
 static void init_acc(ml_hashagg* A, std::vector<Buffer>& acc, uint64_t cap) {
   acc.clear();
   for (size_t s = 0; s < A->slots.size(); s++) {
-    acc.emplace_back(A->ctx->arena.get(), cap * 8);
+    acc.emplace_back(A->ctx->arena, cap * 8);
     launch_fill_u64(acc.back().as<uint64_t>(), acc_init_bits(A->slots[s].kind), (int64_t)cap, A->ctx->stream);
   }
 }
@@ -561,7 +562,7 @@ static void agg_reserve(ml_hashagg* A, uint64_t needed) {
   if (A->cap && needed * 8 <= A->cap * 7) return;
   const uint64_t len = A->len_host;
   const uint64_t ncap = swiss_capacity(std::max(needed, len * 2));
-  Buffer nt(ctx->arena.get(), ncap * (1 + A->nkw) * 8);
+  Buffer nt(ctx->arena, ncap * (1 + A->nkw) * 8);
   zero(ctx, nt.ptr, nt.bytes);
   std::vector<Buffer> nacc;
   init_acc(A, nacc, ncap);
@@ -631,14 +632,14 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
       A->dim_size = sizes;
       A->dim_stride.assign(sizes.size(), 1);
       for (int k = (int)sizes.size() - 2; k >= 0; k--) A->dim_stride[k] = A->dim_stride[k + 1] * sizes[k + 1];
-      A->totals = Buffer(ctx->arena.get(), (size_t)groups * A->slots.size() * 8);
+      A->totals = Buffer(ctx->arena, (size_t)groups * A->slots.size() * 8);
       AccSlotList kinds = slot_kinds(A);
       std::vector<uint64_t> init((size_t)groups * A->slots.size());
       for (size_t c = 0; c < init.size(); c++) init[c] = acc_init_bits(kinds.kind[c % A->slots.size()]);
       h2d(ctx, A->totals.ptr, init.data(), init.size() * 8);
       hip_check(hipStreamSynchronize(ctx->stream), "sync");
     }
-    A->flags = Buffer(ctx->arena.get(), 16);
+    A->flags = Buffer(ctx->arena, 16);
     zero(ctx, A->flags.ptr, 16);
     A->planned = true;
   }
@@ -654,10 +655,10 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
     B.P.ngroups = (int32_t)A->ngroups;
     const int grid = B.grid();
     const int ncells = (int)(A->ngroups * (int64_t)A->slots.size());
-    Buffer partials(ctx->arena.get(), (size_t)grid * ncells * 8);
+    Buffer partials(ctx->arena, (size_t)grid * ncells * 8);
     B.P.direct_partials = partials.ptr;
     B.upload();
-    launch_pipeline(B.P, grid, block_of(ctx), (size_t)ncells * 8, ctx->stream);
+    ctx->launch(B.P, grid, block_of(ctx), (size_t)ncells * 8);
     launch_direct_reduce(partials.as<uint64_t>(), grid, ncells, (int)A->slots.size(), slot_kinds(A),
                          A->totals.as<uint64_t>(), ctx->stream);
     return;
@@ -677,7 +678,7 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
     B.P.agg_mask = A->cap - 1;
     B.P.agg_fill_limit = fill_limit;
     for (size_t s = 0; s < A->slots.size(); s++) B.P.acc_data[s] = A->acc[s].ptr;
-    launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+    ctx->launch(B.P, B.grid(), block_of(ctx), 0);
     A->len_known = false;
   };
   if (A->sizing == ML_SIZING_PLAIN) {
@@ -710,6 +711,7 @@ static ml_result* agg_finalize(ml_hashagg* A) {
   ml_ctx* ctx = A->ctx;
   auto* r = new ml_result();
   r->ctx = ctx;
+  r->device = ctx->device;
   const size_t nk = A->key_types.size();
   if (!A->planned) {  // no input at all: empty result of the right shape
     for (size_t k = 0; k < nk; k++) {
@@ -750,10 +752,10 @@ static ml_result* agg_finalize(ml_hashagg* A) {
         if (w == 4) { uint32_t v = (uint32_t)code; std::memcpy(&host[j * 4], &v, 4); }
         else host[j] = (uint8_t)code;
       }
-      c.data = Buffer(ctx->arena.get(), host.size());
+      c.data = Buffer(ctx->arena, host.size());
       h2d(ctx, c.data.ptr, host.data(), host.size());
       if (any_null) {
-        c.valid = Buffer(ctx->arena.get(), valid.size());
+        c.valid = Buffer(ctx->arena, valid.size());
         h2d(ctx, c.valid.ptr, valid.data(), valid.size());
       }
       r->cols.push_back(std::move(c));
@@ -779,10 +781,10 @@ static ml_result* agg_finalize(ml_hashagg* A) {
         if (w == 8) std::memcpy(&host[j * 8], &v, 8);
         else { uint32_t x = (uint32_t)v; std::memcpy(&host[j * 4], &x, 4); }
       }
-      c.data = Buffer(ctx->arena.get(), host.size());
+      c.data = Buffer(ctx->arena, host.size());
       h2d(ctx, c.data.ptr, host.data(), host.size());
       if (any_null) {
-        c.valid = Buffer(ctx->arena.get(), valid.size());
+        c.valid = Buffer(ctx->arena, valid.size());
         h2d(ctx, c.valid.ptr, valid.data(), valid.size());
       }
       r->cols.push_back(std::move(c));
@@ -794,12 +796,12 @@ static ml_result* agg_finalize(ml_hashagg* A) {
   // hash mode: compact the occupied slots (ordered by slot) and decode
   if (agg_overflow_bits(A) & 2u) fail(ML_ERR_CAPACITY, "pre-sized group table overflowed");
   const uint64_t cap = A->cap;
-  Buffer flags(ctx->arena.get(), std::max<uint64_t>(cap, 1));
+  Buffer flags(ctx->arena, std::max<uint64_t>(cap, 1));
   launch_slot_flags(A->table.as<uint64_t>(), cap, A->nkw, flags.as<uint8_t>(), ctx->stream);
   const int64_t nb = compact_blocks((int64_t)cap);
-  Buffer bc(ctx->arena.get(), (size_t)std::max<int64_t>(nb, 1) * 4);
-  Buffer offs(ctx->arena.get(), (size_t)(nb + 1) * 8);
-  Buffer idx(ctx->arena.get(), std::max<uint64_t>(A->len_host, 1) * 8);
+  Buffer bc(ctx->arena, (size_t)std::max<int64_t>(nb, 1) * 4);
+  Buffer offs(ctx->arena, (size_t)(nb + 1) * 8);
+  Buffer idx(ctx->arena, std::max<uint64_t>(A->len_host, 1) * 8);
   launch_compact(flags.as<uint8_t>(), (int64_t)cap, bc.as<uint32_t>(), offs.as<uint64_t>(), nullptr, idx.ptr, 0,
                  ctx->stream);
   uint64_t n = 0;
@@ -814,8 +816,8 @@ static ml_result* agg_finalize(ml_hashagg* A) {
     c.type = A->key_types[k];
     c.dict = A->key_dicts[k];
     const int w = type_width(c.type);
-    c.data = Buffer(ctx->arena.get(), std::max<uint64_t>(n, 1) * w);
-    if (A->key_nullable) c.valid = Buffer(ctx->arena.get(), std::max<uint64_t>(n, 1));
+    c.data = Buffer(ctx->arena, std::max<uint64_t>(n, 1) * w);
+    if (A->key_nullable) c.valid = Buffer(ctx->arena, std::max<uint64_t>(n, 1));
     E.key_width[k] = w;
     E.key_out[k] = c.data.ptr;
     E.key_valid[k] = c.valid.ptr ? c.valid.as<uint8_t>() : nullptr;
@@ -827,8 +829,8 @@ static ml_result* agg_finalize(ml_hashagg* A) {
     ResultColumn c;
     c.type = o.out_type;
     const int w = type_width(c.type);
-    c.data = Buffer(ctx->arena.get(), std::max<uint64_t>(n, 1) * w);
-    if (o.count > 0) c.valid = Buffer(ctx->arena.get(), std::max<uint64_t>(n, 1));
+    c.data = Buffer(ctx->arena, std::max<uint64_t>(n, 1) * w);
+    if (o.count > 0) c.valid = Buffer(ctx->arena, std::max<uint64_t>(n, 1));
     E.finish[a] = o.finish;
     E.out_width[a] = w;
     E.state[a] = A->acc[(size_t)o.state].ptr;
@@ -907,7 +909,7 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
     B.P.out_count = counters;
     zero(ctx, counters, 16);
     B.upload();
-    launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+    ctx->launch(B.P, B.grid(), block_of(ctx), 0);
     uint64_t n = 0;
     d2h(ctx, &n, counters, 8);
     cap = n;
@@ -921,17 +923,17 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
     nullable[c] = nl;
     const int w = type_width(B.layout[c]->type);
     B.P.col_width[c] = w;
-    ch.cols.emplace_back(ctx->arena.get(), std::max<uint64_t>(cap, 1) * w);
+    ch.cols.emplace_back(ctx->arena, std::max<uint64_t>(cap, 1) * w);
     ch.valid.emplace_back();
-    if (nl) ch.valid.back() = Buffer(ctx->arena.get(), std::max<uint64_t>(cap, 1));
+    if (nl) ch.valid.back() = Buffer(ctx->arena, std::max<uint64_t>(cap, 1));
     B.P.col_out[c] = ch.cols.back().ptr;
     B.P.col_valid[c] = ch.valid.back().ptr ? ch.valid.back().as<uint8_t>() : nullptr;
   }
   if (J->col_nullable.empty()) J->col_nullable = nullable;
   else
     for (size_t c = 0; c < nullable.size(); c++) J->col_nullable[c] = J->col_nullable[c] || nullable[c];
-  ch.keys = Buffer(ctx->arena.get(), std::max<uint64_t>(cap, 1) * J->key_types.size() * 8);
-  ch.key_null = Buffer(ctx->arena.get(), std::max<uint64_t>(cap, 1));
+  ch.keys = Buffer(ctx->arena, std::max<uint64_t>(cap, 1) * J->key_types.size() * 8);
+  ch.key_null = Buffer(ctx->arena, std::max<uint64_t>(cap, 1));
   B.P.key_out = ch.keys.as<uint64_t>();
   B.P.key_null = ch.key_null.as<uint8_t>();
   B.P.sink = SINK_JOIN_BUILD;
@@ -940,7 +942,7 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
   B.P.out_overflow = (uint32_t*)(counters + 1);
   zero(ctx, counters, 16);
   B.upload();
-  launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+  ctx->launch(B.P, B.grid(), block_of(ctx), 0);
   uint64_t hc[2] = {0, 0};
   d2h(ctx, hc, counters, 16);
   if (hc[1] & 1) fail(ML_ERR_INTERNAL, "join build output overflowed");
@@ -973,11 +975,11 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
     // Batch::concat of the collected batches (batch.rs:168-184)
     for (size_t ci = 0; ci < T.col_types.size(); ci++) {
       const int w = type_width(T.col_types[ci]);
-      T.cols.emplace_back(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1) * w);
+      T.cols.emplace_back(ctx->arena, (size_t)std::max<int64_t>(n, 1) * w);
       const bool nl = ci < J->col_nullable.size() && J->col_nullable[ci];
       T.col_valid.emplace_back();
       if (nl) {
-        T.col_valid.back() = Buffer(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1));
+        T.col_valid.back() = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1));
         launch_fill_u64(T.col_valid.back().as<uint64_t>(), 0x0101010101010101ull,
                         (std::max<int64_t>(n, 1) + 7) / 8, ctx->stream);
       }
@@ -988,8 +990,8 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
         off += c.rows;
       }
     }
-    keys = Buffer(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1) * nk * 8);
-    key_null = Buffer(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1));
+    keys = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1) * nk * 8);
+    key_null = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1));
     int64_t off = 0;
     for (auto& c : J->chunks) {
       d2d(ctx, keys.as<uint64_t>() + off * nk, c.keys.ptr, (size_t)c.rows * nk * 8);
@@ -1003,13 +1005,13 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   const uint64_t cap = std::max<uint64_t>(16, next_pow2((uint64_t)((n * 4 + 2) / 3)));
   T.dev.nkw = (int)nk;
   T.dev.mask = cap - 1;
-  T.slots = Buffer(ctx->arena.get(), cap * (1 + nk) * 8);
+  T.slots = Buffer(ctx->arena, cap * (1 + nk) * 8);
   zero(ctx, T.slots.ptr, T.slots.bytes);
   const int use_bloom = ctx->cfg.use_bloom;
   const bool bloom = use_bloom > 0 || (use_bloom < 0 && n >= (1 << 16));
   if (bloom) {
     const uint64_t words = std::max<uint64_t>(64, next_pow2((uint64_t)n / 8 + 1));
-    T.bloom = Buffer(ctx->arena.get(), words * 8);
+    T.bloom = Buffer(ctx->arena, words * 8);
     zero(ctx, T.bloom.ptr, T.bloom.bytes);
     T.dev.bloom = T.bloom.as<uint64_t>();
     T.dev.bloom_mask = words - 1;
@@ -1022,6 +1024,14 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   uint32_t hdup = 0;
   d2h(ctx, &hdup, dup, 4);
   T.dev.unique = hdup ? 0 : 1;
+  for (size_t c = 0; c < T.col_types.size(); c++) {
+    ml_column v{};
+    v.type = T.col_types[c];
+    v.data = T.cols[c].ptr;
+    v.len = n;
+    v.dict = T.col_dicts[c] ? &T.col_dicts[c]->view : nullptr;
+    T.views.push_back(v);
+  }
   return t;
 }
 
@@ -1081,7 +1091,7 @@ static void topn_reduce(ml_topn* T, Materialized& m) {
         if (r > 0 && ent[order[r]] == ent[order[r - 1]]) rk = rank[order[r - 1]];
         rank[order[r]] = rk;
       }
-      Buffer tb(ctx->arena.get(), rank.size() * 8);
+      Buffer tb(ctx->arena, rank.size() * 8);
       h2d(ctx, tb.ptr, rank.data(), rank.size() * 8);
       hip_check(hipStreamSynchronize(ctx->stream), "sync");
       p = B.comp.compile(e, -1);
@@ -1108,7 +1118,7 @@ static void topn_reduce(ml_topn* T, Materialized& m) {
   if (L > 256) fail(ML_ERR_UNSUPPORTED, "TopN limit above 256");
   B.upload();
   const int W = nw + 1;
-  Buffer words(ctx->arena.get(), (size_t)n * W * 8);
+  Buffer words(ctx->arena, (size_t)n * W * 8);
   launch_sort_keys(B.P, K, n, words.as<uint64_t>(), ctx->stream);
   // repeated tile selection until one tile remains
   int64_t cur = n;
@@ -1116,21 +1126,21 @@ static void topn_reduce(ml_topn* T, Materialized& m) {
   for (;;) {
     const int64_t tiles = (cur + TOPN_TILE - 1) / TOPN_TILE;
     const int keep = (int)std::min<int64_t>(L, TOPN_TILE);
-    Buffer b(ctx->arena.get(), (size_t)tiles * keep * W * 8);
+    Buffer b(ctx->arena, (size_t)tiles * keep * W * 8);
     launch_topn_tiles(a.as<uint64_t>(), cur, nw, keep, b.as<uint64_t>(), ctx->stream);
     a = std::move(b);
     cur = tiles * keep;
     if (tiles == 1) break;
   }
   const int64_t m_out = std::min<int64_t>(L, n);
-  Buffer rows(ctx->arena.get(), (size_t)m_out * 8);
+  Buffer rows(ctx->arena, (size_t)m_out * 8);
   launch_topn_rows(a.as<uint64_t>(), nw, m_out, rows.as<int64_t>(), ctx->stream);
   // gather every column of the winners
   for (size_t c = 0; c < m.types.size(); c++) {
     const int w = type_width(m.types[c]);
-    Buffer d(ctx->arena.get(), (size_t)std::max<int64_t>(m_out, 1) * w);
+    Buffer d(ctx->arena, (size_t)std::max<int64_t>(m_out, 1) * w);
     Buffer v;
-    if (m.valid[c].ptr) v = Buffer(ctx->arena.get(), (size_t)std::max<int64_t>(m_out, 1));
+    if (m.valid[c].ptr) v = Buffer(ctx->arena, (size_t)std::max<int64_t>(m_out, 1));
     launch_gather(m.cols[c].ptr, w, rows.as<int64_t>(), m_out, d.ptr, m.valid[c].as<uint8_t>(),
                   v.ptr ? v.as<uint8_t>() : nullptr, ctx->stream);
     m.cols[c] = std::move(d);
@@ -1167,12 +1177,12 @@ static void topn_run(ml_topn* T, PipelineBuild& B) {
   cat.rows = b.rows + m.rows;
   for (size_t c = 0; c < b.types.size(); c++) {
     const int w = type_width(b.types[c]);
-    Buffer d(ctx->arena.get(), (size_t)cat.rows * w);
+    Buffer d(ctx->arena, (size_t)cat.rows * w);
     d2d(ctx, d.ptr, b.cols[c].ptr, (size_t)b.rows * w);
     d2d(ctx, d.as<uint8_t>() + b.rows * w, m.cols[c].ptr, (size_t)m.rows * w);
     Buffer v;
     if (b.valid[c].ptr || m.valid[c].ptr) {
-      v = Buffer(ctx->arena.get(), (size_t)cat.rows);
+      v = Buffer(ctx->arena, (size_t)cat.rows);
       launch_fill_u64(v.as<uint64_t>(), 0x0101010101010101ull, (cat.rows + 7) / 8, ctx->stream);
       if (b.valid[c].ptr) d2d(ctx, v.ptr, b.valid[c].ptr, (size_t)b.rows);
       if (m.valid[c].ptr) d2d(ctx, v.as<uint8_t>() + b.rows, m.valid[c].ptr, (size_t)m.rows);
@@ -1198,6 +1208,20 @@ void ml_result::finish_views() {
     v.dict = c.dict ? &c.dict->view : nullptr;
     views.push_back(v);
   }
+}
+
+void ml_ctx::launch(const mlg::Plan& P, int grid, int block, size_t lds) {
+  if (!stats_on) {
+    mlg::launch_pipeline(P, grid, block, lds, stream);
+    return;
+  }
+  PendingStat ps{P.sink, P.end - P.begin, nullptr, nullptr};
+  mlg::hip_check(hipEventCreate(&ps.start), "hipEventCreate");
+  mlg::hip_check(hipEventCreate(&ps.stop), "hipEventCreate");
+  mlg::hip_check(hipEventRecord(ps.start, stream), "hipEventRecord");
+  mlg::launch_pipeline(P, grid, block, lds, stream);
+  mlg::hip_check(hipEventRecord(ps.stop, stream), "hipEventRecord");
+  stats.push_back(ps);
 }
 
 std::shared_ptr<mlg::HostDict> ml_ctx::dict(const ml_dict* d) {
@@ -1277,8 +1301,8 @@ ml_status ml_ctx_create(const ml_exec_config* cfg, ml_ctx** out) {
       hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
       ctx->own_stream = true;
     }
-    ctx->arena = std::make_unique<DeviceArena>(ctx->device);
-    ctx->counters = Buffer(ctx->arena.get(), 64 * 8);
+    ctx->arena = std::make_shared<DeviceArena>(ctx->device);
+    ctx->counters = Buffer(ctx->arena, 64 * 8);
     zero(ctx, ctx->counters.ptr, 64 * 8);
     hip_check(hipStreamSynchronize(ctx->stream), "sync");
   });
@@ -1309,6 +1333,31 @@ ml_status ml_ctx_synchronize(ml_ctx* ctx) {
 
 void* ml_ctx_stream(ml_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+ml_status ml_ctx_enable_stats(ml_ctx* ctx, int32_t on) {
+  return guarded(ctx, [&] { ctx->stats_on = on != 0; });
+}
+
+int32_t ml_ctx_take_stats(ml_ctx* ctx, ml_launch_stat* out, int32_t cap) {
+  int32_t n = 0;
+  ml_status st = guarded(ctx, [&] {
+    hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+    for (auto& ps : ctx->stats) {
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, ps.start, ps.stop), "hipEventElapsedTime");
+      if (n < cap) {
+        out[n].sink = ps.sink;
+        out[n].rows = ps.rows;
+        out[n].ms = ms;
+        n++;
+      }
+      (void)hipEventDestroy(ps.start);
+      (void)hipEventDestroy(ps.stop);
+    }
+    ctx->stats.clear();
+  });
+  return st == ML_OK ? n : -1;
+}
+
 int64_t ml_result_num_rows(const ml_result* r) { return r ? r->num_rows : 0; }
 int32_t ml_result_num_columns(const ml_result* r) { return r ? (int32_t)r->cols.size() : 0; }
 const ml_column* ml_result_column(const ml_result* r, int32_t i) {
@@ -1335,7 +1384,7 @@ ml_status ml_result_copy_validity(ml_ctx* ctx, const ml_result* r, int32_t i, ui
 
 void ml_result_free(ml_result* r) {
   if (!r) return;
-  if (r->ctx) (void)hipSetDevice(r->ctx->device);
+  (void)hipSetDevice(r->device);
   delete r;
 }
 
@@ -1356,14 +1405,14 @@ ml_status ml_filter(ml_ctx* ctx, const ml_expr* predicate, const ml_batch* in, u
     op.predicate = *predicate;
     B.add_ops(&op, 1);
     const int64_t n = B.active_rows;
-    Buffer flags(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1));
+    Buffer flags(ctx->arena, (size_t)std::max<int64_t>(n, 1));
     B.P.sink = SINK_FLAGS;
     B.P.flag_out = flags.as<uint8_t>();
     B.upload();
-    if (n > 0) launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+    if (n > 0) ctx->launch(B.P, B.grid(), block_of(ctx), 0);
     const int64_t nb = compact_blocks(n);
-    Buffer bc(ctx->arena.get(), (size_t)std::max<int64_t>(nb, 1) * 4);
-    Buffer offs(ctx->arena.get(), (size_t)(nb + 1) * 8);
+    Buffer bc(ctx->arena, (size_t)std::max<int64_t>(nb, 1) * 4);
+    Buffer offs(ctx->arena, (size_t)(nb + 1) * 8);
     launch_compact(flags.as<uint8_t>(), n, bc.as<uint32_t>(), offs.as<uint64_t>(), B.P.sel, out_sel, 1, ctx->stream);
     uint64_t total = 0;
     d2h(ctx, &total, offs.as<uint64_t>() + nb, 8);
@@ -1391,15 +1440,15 @@ ml_status ml_project(ml_ctx* ctx, const ml_expr* exprs, int32_t num_exprs, const
       B.P.col_width[c] = w;
       m.types.push_back(t);
       m.dicts.push_back(nullptr);
-      m.cols.emplace_back(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1) * w);
+      m.cols.emplace_back(ctx->arena, (size_t)std::max<int64_t>(n, 1) * w);
       m.valid.emplace_back();
-      if (p.nullable) m.valid.back() = Buffer(ctx->arena.get(), (size_t)std::max<int64_t>(n, 1));
+      if (p.nullable) m.valid.back() = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1));
       B.P.col_out[c] = m.cols.back().ptr;
       B.P.col_valid[c] = m.valid.back().ptr ? m.valid.back().as<uint8_t>() : nullptr;
     }
     B.P.sink = SINK_PROJECT;
     B.upload();
-    if (n > 0) launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+    if (n > 0) ctx->launch(B.P, B.grid(), block_of(ctx), 0);
     hip_check(hipStreamSynchronize(ctx->stream), "sync");
     *out = make_result(ctx, m);
   });
@@ -1420,7 +1469,7 @@ ml_status ml_hash_columns(ml_ctx* ctx, const ml_expr* keys, int32_t num_keys, co
     B.P.sink = SINK_HASH;
     B.P.hash_out = out_hashes;
     B.upload();
-    if (B.active_rows > 0) launch_pipeline(B.P, B.grid(), block_of(ctx), 0, ctx->stream);
+    if (B.active_rows > 0) ctx->launch(B.P, B.grid(), block_of(ctx), 0);
     hip_check(hipStreamSynchronize(ctx->stream), "sync");
   });
 }
@@ -1582,6 +1631,11 @@ ml_status ml_join_build_finalize(ml_join_build* b, ml_join_table** out) {
 
 int64_t ml_join_table_num_rows(const ml_join_table* t) { return t ? t->impl.nrows : 0; }
 int32_t ml_join_table_unique(const ml_join_table* t) { return t ? t->impl.dev.unique : 0; }
+int32_t ml_join_table_num_columns(const ml_join_table* t) { return t ? (int32_t)t->impl.col_types.size() : 0; }
+const ml_column* ml_join_table_column(const ml_join_table* t, int32_t i) {
+  if (!t || i < 0 || (size_t)i >= t->impl.views.size()) return nullptr;
+  return &t->impl.views[(size_t)i];
+}
 void ml_join_table_free(ml_join_table* t) { delete t; }
 
 ml_status ml_join_probe(ml_ctx* ctx, const ml_join_table* t, const ml_expr* probe_keys, int32_t num_keys,
@@ -1604,7 +1658,7 @@ ml_status ml_join_probe(ml_ctx* ctx, const ml_join_table* t, const ml_expr* prob
     for (int c = 0; c < 2; c++) {
       o.types.push_back(ML_TYPE_INT64);
       o.dicts.push_back(nullptr);
-      o.cols.emplace_back(ctx->arena.get(), (size_t)std::max<int64_t>(m.rows, 1) * 8);
+      o.cols.emplace_back(ctx->arena, (size_t)std::max<int64_t>(m.rows, 1) * 8);
       o.valid.emplace_back();
     }
     if (m.rows > 0) {

@@ -41,8 +41,13 @@ EXPORTED = [
     "ml_hashagg_create", "ml_hashagg_sink", "ml_hashagg_finalize", "ml_hashagg_num_groups",
     "ml_hashagg_table_stats", "ml_hashagg_free", "ml_join_build_create", "ml_join_build_sink",
     "ml_join_build_finalize", "ml_join_table_num_rows", "ml_join_table_unique", "ml_join_table_free",
+    "ml_join_table_num_columns", "ml_join_table_column",
     "ml_join_probe", "ml_pipeline_hashagg", "ml_pipeline_join_build", "ml_pipeline_topn", "ml_pipeline_collect",
-    "ml_topn_create", "ml_topn_sink", "ml_topn_finalize", "ml_topn_free",
+    "ml_topn_create", "ml_topn_sink", "ml_topn_finalize", "ml_topn_free", "ml_ctx_enable_stats",
+    "ml_ctx_take_stats", "ml_fragments_create", "ml_fragments_run", "ml_fragments_run_graph",
+    "ml_fragments_run_per_fragment", "ml_fragments_autotune", "ml_fragments_free", "ml_mlp_create",
+    "ml_mlp_get_params", "ml_mlp_forward", "ml_mlp_loss_grad", "ml_mlp_train_step", "ml_mlp_free",
+    "ml_sizing_features", "ml_ctx_set_estimator",
 ]
 
 
@@ -91,6 +96,8 @@ def lib():
             "ml_join_table_num_rows": ([vp], i64),
             "ml_join_table_unique": ([vp], i32),
             "ml_join_table_free": ([vp], None),
+            "ml_join_table_num_columns": ([vp], i32),
+            "ml_join_table_column": ([vp, i32], P(abi.MlColumn)),
             "ml_join_probe": ([vp, vp, P(abi.MlExpr), i32, P(abi.MlBatch), P(vp)], i32),
             "ml_pipeline_hashagg": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, vp], i32),
             "ml_pipeline_join_build": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, vp], i32),
@@ -100,6 +107,22 @@ def lib():
             "ml_topn_sink": ([vp, P(abi.MlBatch)], i32),
             "ml_topn_finalize": ([vp, P(vp)], i32),
             "ml_topn_free": ([vp], None),
+            "ml_ctx_enable_stats": ([vp, i32], i32),
+            "ml_ctx_take_stats": ([vp, P(abi.MlLaunchStat), i32], i32),
+            "ml_fragments_create": ([vp, P(abi.MlFragment), i32, P(abi.MlFragmentTuning), P(vp)], i32),
+            "ml_fragments_run": ([vp], i32),
+            "ml_fragments_run_graph": ([vp], i32),
+            "ml_fragments_run_per_fragment": ([vp], i32),
+            "ml_fragments_autotune": ([vp, P(abi.MlFragmentTuning)], i32),
+            "ml_fragments_free": ([vp], None),
+            "ml_mlp_create": ([vp, vp, P(vp)], i32),
+            "ml_mlp_get_params": ([vp, vp], i32),
+            "ml_mlp_forward": ([vp, vp, i64, vp], i32),
+            "ml_mlp_loss_grad": ([vp, vp, vp, i64, P(C.c_float), vp], i32),
+            "ml_mlp_train_step": ([vp, vp, vp, i64, C.c_float, P(C.c_float)], i32),
+            "ml_mlp_free": ([vp], None),
+            "ml_sizing_features": ([vp, vp, i64, i64, i32, C.c_uint64, vp], i32),
+            "ml_ctx_set_estimator": ([vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -113,6 +136,21 @@ def exported_symbols(path: str = LIB_PATH):
     """Names from EXPORTED that the built library actually exports (no GPU needed)."""
     L = C.CDLL(path)
     return [n for n in EXPORTED if hasattr(L, n)]
+
+
+_hip = None
+
+
+def device_to_host(dst: int, src: int, nbytes: int) -> None:
+    """hipMemcpy device -> host (synchronous), for inspecting library-owned buffers."""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipMemcpy.restype = C.c_int
+    rc = _hip.hipMemcpy(dst, src, nbytes, 2)
+    if rc != 0:
+        raise GpuError(abi.ML_ERR_INTERNAL, f"hipMemcpy failed ({rc})")
 
 
 def _torch():
@@ -218,6 +256,15 @@ class GpuJoin:
 
     def unique(self):
         return bool(lib().ml_join_table_unique(self.handle))
+
+    def column(self, i: int) -> np.ndarray:
+        """Build column i in table row order, copied to the host."""
+        mc = lib().ml_join_table_column(self.handle, i).contents
+        n = self.num_rows()
+        out = np.zeros(max(n, 1), dtype=NP_DTYPE[mc.type])
+        if n:
+            device_to_host(out.ctypes.data, mc.data, n * out.itemsize)
+        return out[:n]
 
     def __del__(self):
         if self.handle and _lib is not None:

@@ -41,9 +41,13 @@ def assert_tables_equal(got: HostTable, want: HostTable, rtol=1e-12, atol=0.0, o
         assert g.type == w.type, f"column {i}: type {abi.TYPE_NAMES[g.type]} != {abi.TYPE_NAMES[w.type]}"
     float_cols = {i for i, c in enumerate(want.columns) if c.type == abi.FLOAT64}
     gr, wr = rows_of(got), rows_of(want)
+    if rtol == 0 and atol == 0:
+        float_cols_key = set()  # exact comparison: order rows by every column
+    else:
+        float_cols_key = float_cols
     if not ordered:
-        gr = sorted(gr, key=lambda r: _key(r, float_cols))
-        wr = sorted(wr, key=lambda r: _key(r, float_cols))
+        gr = sorted(gr, key=lambda r: _key(r, float_cols_key))
+        wr = sorted(wr, key=lambda r: _key(r, float_cols_key))
     for k, (a, b) in enumerate(zip(gr, wr)):
         for i, (x, y) in enumerate(zip(a, b)):
             if i in float_cols:

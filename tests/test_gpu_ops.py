@@ -111,26 +111,26 @@ def test_hash_columns_bit_identical(gpu_engine, oracle_engine, k):
 
 def test_join_probe_with_duplicate_build_keys(gpu_engine, oracle_engine):
     rng = np.random.default_rng(5)
-    build = HostTable([HostColumn(INT64, rng.integers(0, 300, 1000)), HostColumn(INT32, np.arange(1000, dtype=np.int32))])
+    bkeys = rng.integers(0, 300, 1000)
+    build = HostTable([HostColumn(INT64, bkeys), HostColumn(INT32, np.arange(1000, dtype=np.int32))])
     probe = HostTable([HostColumn(INT64, rng.integers(-10, 320, 4000))])
-    for eng in (gpu_engine, oracle_engine):
-        pass
     jo = oracle_engine.join_build(build, [], [col(0)], [INT64])
     jg = gpu_engine.join_build(build, [], [col(0)], [INT64])
-    assert not jg.unique()
+    assert not jg.unique() and jg.num_rows() == 1000
     po, bo = oracle_engine.join_probe(jo, [col(0)], probe)
     pg, bg = gpu_engine.join_probe(jg, [col(0)], probe)
-    assert sorted(zip(pg.tolist(), bg.tolist())) == sorted(zip(po.tolist(), bo.tolist()))
+    # build row ids are table-internal: compare (probe row, original build row id)
+    ids_g = jg.column(1)
+    assert sorted(zip(pg.tolist(), ids_g[bg].tolist())) == sorted(zip(po.tolist(), bo.tolist()))
+    # every pair joins equal keys
+    assert (jg.column(0)[bg] == probe.columns[0].values[pg]).all()
 
 
 def test_expanding_probe_into_aggregate(gpu_engine, oracle_engine):
     rng = np.random.default_rng(9)
     build = HostTable([HostColumn(INT64, rng.integers(0, 50, 400)), HostColumn(FLOAT64, rng.random(400))])
     probe = HostTable([HostColumn(INT64, rng.integers(0, 60, 3000)), HostColumn(INT32, rng.integers(0, 5, 3000).astype(np.int32))])
-    ops = [abi.probe_op(None, [col(0)])]
     aggs = [Agg(AGG_SUM, col(3), FLOAT64), count_star_agg(), Agg(AGG_MAX, col(3), FLOAT64)]
-    for eng, t in ((oracle_engine, None), (gpu_engine, None)):
-        pass
     jo = oracle_engine.join_build(build, [], [col(0)], [INT64])
     jg = gpu_engine.join_build(build, [], [col(0)], [INT64])
     want = oracle_engine.hashagg(probe, [abi.probe_op(jo, [col(0)])], [col(1)], [INT32], aggs).to_host()
