@@ -84,11 +84,11 @@ def main():
     L.ml_ctx_take_stats(eng.ctx, stats_buf, 4096)  # clear
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    start.record()
+    start.record(eng.stream)
     t_wall = time.time()
     for _ in range(args.steps):
         top, _ = Q.run_q5(eng, dev)
-    stop.record()
+    stop.record(eng.stream)
     torch.cuda.synchronize()
     wall = time.time() - t_wall
     ms_total = start.elapsed_time(stop)

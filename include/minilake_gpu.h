@@ -323,9 +323,20 @@ ml_status ml_pipeline_join_build(ml_ctx* ctx, const ml_batch* source, const ml_o
 ml_status ml_pipeline_topn(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops,
                            int32_t num_ops, ml_topn* sink);
 /* Pipeline ending in a collect sink (CollectSink, operators/collect.rs:11-53):
- * every column of the final layout, compacted to the surviving rows. */
+ * every column of the final layout, compacted to the surviving rows. When a
+ * probe can match several build rows the output size is unknown, so an exact
+ * counting pass runs first. */
 ml_status ml_pipeline_collect(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops,
                               int32_t num_ops, ml_result** out);
+/* The same with the output pre-sized to `capacity_hint` rows (a learned
+ * estimate) instead of the counting pass; if the rows do not fit, the exact
+ * path reruns (`*used_fallback` = 1). Results are identical either way. */
+ml_status ml_pipeline_collect_sized(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops,
+                                    int32_t num_ops, int64_t capacity_hint, int32_t* used_fallback,
+                                    ml_result** out);
+/* Rows reaching the end of the pipeline (COUNT(*) over a join), no output. */
+ml_status ml_pipeline_count(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops, int32_t num_ops,
+                            int64_t* out_rows);
 
 /* ------------------------------------------------------------------ */
 /* TopN sink (SortSink with limit, crates/minilake-exec/src/operators/sort.rs:43-268) */

@@ -257,12 +257,29 @@ static Prog column_prog(PipelineBuild& B, const VCol* c, bool* nullable) {
 }
 
 // Run B with a COLLECT sink (optionally with per-stage row ids); rows come back in `out`.
-static void run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buffer* pair_rows, int pair_stages) {
+static uint64_t run_count(PipelineBuild& B) {
+  ml_ctx* ctx = B.ctx;
+  uint64_t* counters = ctx->counters.as<uint64_t>();
+  B.P.sink = SINK_COUNT_PAIRS;
+  B.P.out_count = counters;
+  zero(ctx, counters, 16);
+  B.upload();
+  if (B.active_rows > 0) ctx->launch(B.P, B.grid(), block_of(ctx), 0);
+  uint64_t n = 0;
+  d2h(ctx, &n, counters, 8);
+  return n;
+}
+
+// Returns false when a capacity hint was too small (nothing usable written).
+static bool run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buffer* pair_rows, int pair_stages,
+                        int64_t capacity_hint = -1) {
   ml_ctx* ctx = B.ctx;
   uint64_t* counters = ctx->counters.as<uint64_t>();
   // capacity: active rows, unless one probe can match several build rows
   uint64_t cap = (uint64_t)B.active_rows;
-  if (B.expanding) {
+  if (capacity_hint >= 0) {
+    cap = (uint64_t)capacity_hint;
+  } else if (B.expanding) {
     B.P.sink = SINK_COUNT_PAIRS;
     B.P.out_count = counters;
     zero(ctx, counters, 16);
@@ -308,8 +325,12 @@ static void run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buf
   ctx->launch(B.P, B.grid(), block_of(ctx), 0);
   uint64_t hc[2] = {0, 0};
   d2h(ctx, hc, counters, 16);
-  if (hc[1] & 1) fail(ML_ERR_INTERNAL, "collect output overflowed its capacity");
+  if (hc[1] & 1) {
+    if (capacity_hint >= 0) return false;
+    fail(ML_ERR_INTERNAL, "collect output overflowed its capacity");
+  }
   out.rows = (int64_t)hc[0];
+  return true;
 }
 
 static ml_result* make_result(ml_ctx* ctx, Materialized& m) {
@@ -1680,6 +1701,40 @@ ml_status ml_pipeline_collect(ml_ctx* ctx, const ml_batch* source, const ml_oper
     Materialized m;
     run_collect(B, m, true, nullptr, 0);
     *out = make_result(ctx, m);
+  });
+}
+
+ml_status ml_pipeline_collect_sized(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops, int32_t num_ops,
+                                    int64_t capacity_hint, int32_t* used_fallback, ml_result** out) {
+  return guarded(ctx, [&] {
+    if (used_fallback) *used_fallback = 0;
+    {
+      PipelineBuild B(ctx);
+      B.source(source);
+      B.add_ops(ops, num_ops);
+      Materialized m;
+      if (run_collect(B, m, true, nullptr, 0, std::max<int64_t>(capacity_hint, 0))) {
+        *out = make_result(ctx, m);
+        return;
+      }
+    }
+    if (used_fallback) *used_fallback = 1;
+    PipelineBuild B(ctx);
+    B.source(source);
+    B.add_ops(ops, num_ops);
+    Materialized m;
+    run_collect(B, m, true, nullptr, 0);
+    *out = make_result(ctx, m);
+  });
+}
+
+ml_status ml_pipeline_count(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops, int32_t num_ops,
+                            int64_t* out_rows) {
+  return guarded(ctx, [&] {
+    PipelineBuild B(ctx);
+    B.source(source);
+    B.add_ops(ops, num_ops);
+    *out_rows = (int64_t)run_count(B);
   });
 }
 

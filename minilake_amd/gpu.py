@@ -41,7 +41,7 @@ EXPORTED = [
     "ml_hashagg_create", "ml_hashagg_sink", "ml_hashagg_finalize", "ml_hashagg_num_groups",
     "ml_hashagg_table_stats", "ml_hashagg_free", "ml_join_build_create", "ml_join_build_sink",
     "ml_join_build_finalize", "ml_join_table_num_rows", "ml_join_table_unique", "ml_join_table_free",
-    "ml_join_table_num_columns", "ml_join_table_column",
+    "ml_join_table_num_columns", "ml_join_table_column", "ml_pipeline_collect_sized", "ml_pipeline_count",
     "ml_join_probe", "ml_pipeline_hashagg", "ml_pipeline_join_build", "ml_pipeline_topn", "ml_pipeline_collect",
     "ml_topn_create", "ml_topn_sink", "ml_topn_finalize", "ml_topn_free", "ml_ctx_enable_stats",
     "ml_ctx_take_stats", "ml_fragments_create", "ml_fragments_run", "ml_fragments_run_graph",
@@ -103,6 +103,8 @@ def lib():
             "ml_pipeline_join_build": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, vp], i32),
             "ml_pipeline_topn": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, vp], i32),
             "ml_pipeline_collect": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, P(vp)], i32),
+            "ml_pipeline_collect_sized": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, i64, P(i32), P(vp)], i32),
+            "ml_pipeline_count": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, P(i64)], i32),
             "ml_topn_create": ([vp, P(abi.MlSortSpec), P(vp)], i32),
             "ml_topn_sink": ([vp, P(abi.MlBatch)], i32),
             "ml_topn_finalize": ([vp, P(vp)], i32),
@@ -179,6 +181,17 @@ class DeviceTable:
             v = c.values.view(np.int32) if c.type == abi.UTF8 else c.values
             self.tensors.append(torch.from_numpy(np.ascontiguousarray(v)).to(device))
         self.dicts = [c.dictionary for c in host.columns]
+
+    @classmethod
+    def from_tensors(cls, tensors, types, dicts=None, num_rows=None) -> "DeviceTable":
+        """Wrap device tensors (already in HBM) as a table without a host copy."""
+        t = cls.__new__(cls)
+        t.host = None
+        t.tensors = list(tensors)
+        t.types = list(types)
+        t.dicts = list(dicts) if dicts else [None] * len(t.tensors)
+        t.num_rows = int(num_rows if num_rows is not None else (t.tensors[0].numel() if t.tensors else 0))
+        return t
 
     def nbytes(self):
         return sum(t.numel() * t.element_size() for t in self.tensors)
@@ -283,11 +296,16 @@ class GpuEngine:
         L = lib()
         self.torch = torch
         torch.cuda.set_device(device)
+        # one stream for torch's copies/allocations and the library's kernels,
+        # so everything the engine does is stream-ordered
+        self.stream = torch.cuda.Stream(device)
+        self._prev_stream = torch.cuda.current_stream(device)
+        torch.cuda.set_stream(self.stream)
         cfg = abi.MlExecConfig()
         L.ml_default_config(C.byref(cfg))
         cfg.batch_rows, cfg.device, cfg.use_bloom = batch_rows, device, use_bloom
         cfg.block_size, cfg.grid_per_cu = block_size, grid_per_cu
-        cfg.stream = torch.cuda.current_stream(device).cuda_stream
+        cfg.stream = self.stream.cuda_stream
         ctx = C.c_void_p()
         rc = L.ml_ctx_create(C.byref(cfg), C.byref(ctx))
         if rc != 0:
@@ -299,6 +317,8 @@ class GpuEngine:
         if self.ctx:
             lib().ml_ctx_destroy(self.ctx)
             self.ctx = None
+            if self.torch.cuda.current_stream(self.device) == self.stream:
+                self.torch.cuda.set_stream(self._prev_stream)
 
     def __del__(self):
         try:
@@ -391,6 +411,42 @@ class GpuEngine:
         out = C.c_void_p()
         self._check(L.ml_pipeline_collect(self.ctx, C.byref(b), self._ops(ops, keep), len(ops), C.byref(out)))
         return GpuResult(self, out.value)
+
+    def count_distinct(self, src, ops, group_exprs, key_types, sizing=abi.SIZING_PLAIN, capacity_hint=0,
+                       stats: Optional[dict] = None) -> int:
+        """Number of groups of a GROUP BY (the aggregate's table, no result rows emitted)."""
+        L, keep = lib(), abi.Keep()
+        _, b = self._src(src, keep)
+        spec = abi.agg_spec(group_exprs, key_types, [abi.count_star()], keep, sizing, capacity_hint)
+        h = C.c_void_p()
+        self._check(L.ml_hashagg_create(self.ctx, C.byref(spec), C.byref(h)))
+        try:
+            self._check(L.ml_pipeline_hashagg(self.ctx, C.byref(b), self._ops(ops, keep), len(ops), h))
+            n = C.c_int64()
+            self._check(L.ml_hashagg_num_groups(h, C.byref(n)))
+            if stats is not None:
+                cap, res = C.c_int64(), C.c_int32()
+                self._check(L.ml_hashagg_table_stats(h, C.byref(cap), C.byref(res)))
+                stats["capacity"], stats["resizes"] = cap.value, res.value
+        finally:
+            L.ml_hashagg_free(h)
+        return n.value
+
+    def collect_sized(self, src, ops, capacity_hint: int):
+        """Collect with a pre-sized output; returns (result, used_fallback)."""
+        L, keep = lib(), abi.Keep()
+        _, b = self._src(src, keep)
+        out, fb = C.c_void_p(), C.c_int32()
+        self._check(L.ml_pipeline_collect_sized(self.ctx, C.byref(b), self._ops(ops, keep), len(ops), int(capacity_hint),
+                                                C.byref(fb), C.byref(out)))
+        return GpuResult(self, out.value), bool(fb.value)
+
+    def count(self, src, ops) -> int:
+        L, keep = lib(), abi.Keep()
+        _, b = self._src(src, keep)
+        n = C.c_int64()
+        self._check(L.ml_pipeline_count(self.ctx, C.byref(b), self._ops(ops, keep), len(ops), C.byref(n)))
+        return n.value
 
     # -------------------------------------------------------- operators
     def expr_type(self, e, src):

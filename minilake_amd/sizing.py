@@ -75,6 +75,7 @@ class Mlp:
         xd = self._dev(x)
         y = self.engine.torch.zeros((len(x), 2), dtype=self.engine.torch.float32, device=xd.device)
         self.engine._check(lib().ml_mlp_forward(self.handle, xd.data_ptr(), len(x), y.data_ptr()))
+        self.engine.synchronize()
         return y.cpu().numpy()
 
     def loss_grad(self, x: np.ndarray, t: np.ndarray) -> Tuple[float, np.ndarray]:
@@ -126,27 +127,157 @@ def column_features(engine, keys_dev, n: int, seed: int = 0) -> np.ndarray:
     return f
 
 
+# ----------------------------------------------------------- training tables
+def _splitmix_np(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def sample_rows(n: int, seed: int, s: int = SAMPLE) -> np.ndarray:
+    """Row ids of a `s`-row sample of a table of n rows (seeded splitmix64)."""
+    with np.errstate(over="ignore"):
+        r = _splitmix_np(np.arange(s, dtype=np.uint64) + np.uint64(seed))
+    return (r % np.uint64(max(n, 1))).astype(np.int64)
+
+
+def features(engine, keys_dev, n: int, build_table, build_rows: int, build_distinct: int, seed: int = 0):
+    """All 32 features for GROUP BY key and probe(key) against `build_table`."""
+    from .gpu import DeviceTable
+    from .abi import col, probe_op
+    f = column_features(engine, keys_dev, n, seed)
+    idx = engine.torch.from_numpy(sample_rows(n, seed + 1)).to(keys_dev.device)
+    samp = DeviceTable.from_tensors([keys_dev[idx].contiguous()], [abi.INT64])
+    matches = engine.count(samp, [probe_op(build_table, [col(0)])]) if build_table is not None else 0
+    return join_features(f, build_rows, build_distinct, matches, matches)
+
+
 def join_features(f: np.ndarray, build_rows: int, build_distinct: int, sample_hits: int, sample_matches: int):
-    """Fill features 27-31 from the build side: rows, distinct keys, share of the
-    probe sample that finds a key, matches per sampled probe row."""
+    """Features 27-31 from the build side: rows, distinct keys, rows per key,
+    matches per sampled probe row, and the scaled-up match estimate."""
     f = f.copy()
     f[27] = math.log2(max(1, build_rows))
     f[28] = math.log2(max(1, build_distinct))
     f[29] = build_rows / max(1, build_distinct)
-    f[30] = sample_hits / SAMPLE
+    f[30] = sample_matches / SAMPLE
     f[31] = math.log2(1 + sample_matches * (2 ** float(f[0])) / SAMPLE)
     return f
 
 
-# ----------------------------------------------------------- training tables
-def synthetic_table(seed: int, rows: int, domain: int, s: float, build_mult: int, build_share: float):
-    """A key column of `rows` Zipf(s) draws over `domain` keys (scrambled so key
-    order carries no information) and a build table holding a `build_share`
-    fraction of the domain's keys, each `build_mult` times."""
-    ranks = W.zipf(seed, 901, rows, s, domain)
-    perm_seed = seed * 7919 + 13
-    keys = W.permutation_of(perm_seed, 902, domain, ranks - 1)
-    nb = max(1, int(domain * build_share))
-    bkeys = W.permutation(perm_seed, 903, domain, count=nb)
-    bkeys = np.repeat(bkeys, build_mult)
-    return keys.astype(np.int64), bkeys.astype(np.int64)
+class KeyGen:
+    """Synthetic key columns generated in HBM (device twin of workloads.zipf_keys)."""
+
+    def __init__(self, engine, max_rows: int):
+        self.engine = engine
+        torch = engine.torch
+        self.keys = torch.empty(max(1, max_rows), dtype=torch.int64, device=f"cuda:{engine.device}")
+        self.bkeys = torch.empty(max(1, max_rows), dtype=torch.int64, device=f"cuda:{engine.device}")
+        self.stream = engine.stream.cuda_stream
+
+    def probe(self, n, seed, s, domain):
+        W.gpu_zipf_keys(self.keys.data_ptr(), n, seed, 11, s, domain, 12, self.stream)
+        return self.keys[:n]
+
+    def build(self, nkeys, mult, seed, domain, hot: bool = True):
+        """hot: the build side holds the most frequent probe keys; else random keys of the domain."""
+        W.gpu_build_keys(self.bkeys.data_ptr(), nkeys, mult, seed, 12 if hot else 13, domain, self.stream)
+        return self.bkeys[:nkeys * mult]
+
+
+def labelled_table(engine, gen: KeyGen, seed: int, rows: int, domain: int, s: float, share: float, mult: int,
+                   hot: bool = True):
+    """One training pair: features and the exact (log2 distinct, log2 1+join rows)
+    computed by the plain engine."""
+    from .gpu import DeviceTable
+    from .abi import INT64, col, probe_op
+    keys = gen.probe(rows, seed, s, domain)
+    nb = max(1, min(int(domain * share), gen.bkeys.numel() // mult))
+    bkeys = gen.build(nb, mult, seed, domain, hot)
+    probe_t = DeviceTable.from_tensors([keys], [INT64])
+    build_t = engine.join_build(DeviceTable.from_tensors([bkeys], [INT64]), [], [col(0)], [INT64])
+    x = features(engine, keys, rows, build_t, nb * mult, nb, seed=seed)
+    distinct = engine.count_distinct(probe_t, [], [col(0)], [INT64], sizing=abi.SIZING_HINT,
+                                     capacity_hint=min(rows, domain))
+    join_rows = engine.count(probe_t, [probe_op(build_t, [col(0)])])
+    return x, np.array([math.log2(max(1, distinct)), math.log2(1 + join_rows)], dtype=np.float32)
+
+
+def training_set(engine, ntables: int, seed: int = 1, min_rows: float = 1e4, max_rows: float = 4e6,
+                 max_domain: float = 1e8, log=None):
+    rng = np.random.default_rng(seed)
+    gen = KeyGen(engine, int(max_rows))
+    X = np.zeros((ntables, 32), np.float32)
+    Y = np.zeros((ntables, 2), np.float32)
+    for t in range(ntables):
+        rows = int(10 ** rng.uniform(math.log10(min_rows), math.log10(max_rows)))
+        domain = max(1, int(10 ** rng.uniform(0, math.log10(max_domain))))
+        s = float(rng.uniform(0, 1.5))
+        share = float(10 ** rng.uniform(-3, 0))
+        mult = int(rng.integers(1, 4))
+        hot = bool(rng.integers(0, 2))
+        X[t], Y[t] = labelled_table(engine, gen, seed * 1_000_003 + t, rows, domain, s, share, mult, hot)
+        if log and (t % 2000 == 0 or t == ntables - 1):
+            log(f"training table {t}/{ntables}")
+    return X, Y
+
+
+def load_params(path: str = WEIGHTS_PATH) -> Optional[np.ndarray]:
+    if not os.path.exists(path):
+        return None
+    return np.load(path, allow_pickle=False).astype(np.float32)
+
+
+def capacity_from_log2(pred: float, margin: float = 1.15) -> int:
+    return int(math.ceil((2.0 ** float(pred)) * margin))
+
+
+# ------------------------------------------------------------ evaluation
+def zipf_workload(engine, rows: int = 60_000_000, domain: int = 100_000_000, s: float = 1.1,
+                  build_keys: int = 2_000_000, mult: int = 2, seed: int = 42):
+    """The learned-sizing evaluation: a `rows`-row fact table of Zipf(s) int64 keys
+    (scrambled over `domain`), a measure w = key mod 1009, and a build table of
+    `build_keys` random keys, each `mult` times, with payload b = key mod 7."""
+    from .gpu import DeviceTable
+    from .abi import INT64
+    torch = engine.torch
+    gen = KeyGen(engine, max(rows, build_keys * mult))
+    keys = gen.probe(rows, seed, s, domain).clone()
+    bkeys = gen.build(build_keys, mult, seed, domain, hot=False).clone()
+    w = torch.remainder(keys, 1009)
+    b = torch.remainder(bkeys, 7)
+    fact = DeviceTable.from_tensors([keys, w], [INT64, INT64])
+    build = DeviceTable.from_tensors([bkeys, b], [INT64, INT64])
+    return fact, build
+
+
+def run_plain(engine, fact, build):
+    """GROUP BY key (COUNT, SUM) with resize-on-overflow sizing, and the join
+    collected after an exact counting pass."""
+    from .abi import AGG_SUM, INT64, Agg, col, count_star, probe_op
+    agg = engine.hashagg(fact, [], [col(0)], [INT64], [count_star(), Agg(AGG_SUM, col(1), INT64)],
+                         sizing=abi.SIZING_PLAIN)
+    jt = engine.join_build(build, [], [col(0)], [INT64])
+    joined = engine.collect(fact, [probe_op(jt, [col(0)])])
+    return agg, joined
+
+
+def predict(engine, mlp: "Mlp", fact, build_rows: int, build_distinct: int, jt, seed: int = 0):
+    x = features(engine, fact.tensors[0], fact.num_rows, jt, build_rows, build_distinct, seed=seed)
+    return mlp.forward(x[None, :])[0]
+
+
+def run_learned(engine, mlp: "Mlp", fact, build, build_distinct: int, margin: float = 1.15):
+    """The same two queries with both outputs pre-sized from the network's
+    prediction (features + inference included); too-small predictions fall
+    back to the exact path inside the library."""
+    from .abi import AGG_SUM, INT64, Agg, col, count_star, probe_op
+    jt = engine.join_build(build, [], [col(0)], [INT64])
+    y = predict(engine, mlp, fact, build.num_rows, build_distinct, jt)
+    groups = capacity_from_log2(y[0], margin)
+    out_rows = max(1, capacity_from_log2(y[1], margin) - 1)
+    agg = engine.hashagg(fact, [], [col(0)], [INT64], [count_star(), Agg(AGG_SUM, col(1), INT64)],
+                         sizing=abi.SIZING_LEARNED, capacity_hint=groups)
+    joined, fell_back = engine.collect_sized(fact, [probe_op(jt, [col(0)])], out_rows)
+    return agg, joined, y, fell_back

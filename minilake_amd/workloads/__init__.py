@@ -30,12 +30,50 @@ _LIB_PATH = os.path.join(_HERE, "libminilake_workloads.so")
 _lib = None
 
 
+_GPU_LIB_PATH = os.path.join(_HERE, "libminilake_workloads_gpu.so")
+_gpu_lib = None
+
+
 def build_library() -> str:
     src = os.path.join(_HERE, "gen.cpp")
     cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread", "-o", _LIB_PATH + ".tmp", src]
     subprocess.run(cmd, check=True)
     os.replace(_LIB_PATH + ".tmp", _LIB_PATH)
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    arch = os.environ.get("MINILAKE_GPU_ARCH", "gfx950")
+    cmd = [hipcc, f"--offload-arch={arch}", "-O3", "-std=c++17", "-fPIC", "-shared", "-Xclang", "-target-feature",
+           "-Xclang", "-packed-fp32-ops", "-o", _GPU_LIB_PATH + ".tmp", os.path.join(_HERE, "gen_gpu.hip")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for gen_gpu.hip:\n" + r.stderr)
+    os.replace(_GPU_LIB_PATH + ".tmp", _GPU_LIB_PATH)
     return _LIB_PATH
+
+
+def gpu_lib():
+    """Device generator (libminilake_workloads_gpu.so)."""
+    global _gpu_lib
+    if _gpu_lib is None:
+        if not os.path.exists(_GPU_LIB_PATH):
+            build_library()
+        L = C.CDLL(_GPU_LIB_PATH)
+        i64, u64, vp = C.c_int64, C.c_uint64, C.c_void_p
+        L.mlw_gpu_zipf_keys.argtypes = [vp, i64, u64, u64, C.c_double, i64, u64, vp]
+        L.mlw_gpu_zipf_keys.restype = C.c_int
+        L.mlw_gpu_build_keys.argtypes = [vp, i64, i64, u64, u64, i64, vp]
+        L.mlw_gpu_build_keys.restype = C.c_int
+        _gpu_lib = L
+    return _gpu_lib
+
+
+def gpu_zipf_keys(out_ptr, n, seed, stream, s, domain, perm_stream, hip_stream=None):
+    if gpu_lib().mlw_gpu_zipf_keys(out_ptr, n, seed, stream, float(s), int(domain), perm_stream, hip_stream) != 0:
+        raise RuntimeError("device key generation failed")
+
+
+def gpu_build_keys(out_ptr, nkeys, mult, seed, perm_stream, domain, hip_stream=None):
+    if gpu_lib().mlw_gpu_build_keys(out_ptr, nkeys, mult, seed, perm_stream, int(domain), hip_stream) != 0:
+        raise RuntimeError("device key generation failed")
 
 
 def lib():
@@ -51,6 +89,7 @@ def lib():
         L.mlw_zipf.argtypes = [u64, u64, i64, i64, C.c_double, i64, i64p]
         L.mlw_permute.argtypes = [u64, u64, i64, i64, i64, i64p]
         L.mlw_gather8.argtypes = [i64p, i64p, i64, i64p]
+        L.mlw_permute_values.argtypes = [u64, u64, i64, i64, i64p, i64p]
         _lib = L
     return _lib
 
@@ -89,6 +128,19 @@ def permutation(seed, stream, n, count=None, offset=0) -> np.ndarray:
     out = np.empty(count, dtype=np.int64)
     lib().mlw_permute(seed, stream, offset, count, n, _p(out, C.c_int64))
     return out
+
+
+def permute_values(seed, stream, n, values) -> np.ndarray:
+    v = np.ascontiguousarray(values, dtype=np.int64)
+    out = np.empty(len(v), dtype=np.int64)
+    lib().mlw_permute_values(seed, stream, len(v), n, _p(v, C.c_int64), _p(out, C.c_int64))
+    return out
+
+
+def zipf_keys(seed, stream, n, s, domain, perm_stream) -> np.ndarray:
+    """Host twin of the device generator: scrambled Zipf(s) keys in [0, domain)."""
+    ranks = zipf(seed, stream, n, s, domain) if s > 0 else uniform_int(seed, stream, n, 1, domain)
+    return permute_values(seed, perm_stream, domain, ranks - 1)
 
 
 def gather8(src: np.ndarray, idx: np.ndarray) -> np.ndarray:

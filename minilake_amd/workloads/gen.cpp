@@ -128,6 +128,29 @@ void mlw_permute(uint64_t seed, uint64_t stream, int64_t offset, int64_t count, 
   });
 }
 
+// out[i] = the same Feistel permutation applied to values[i] (each in [0, n)).
+void mlw_permute_values(uint64_t seed, uint64_t stream, int64_t count, int64_t n, const int64_t* values, int64_t* out) {
+  uint64_t key = stream_key(seed, stream);
+  int bits = 2;
+  while ((1ull << bits) < (uint64_t)n) bits += 2;
+  int half = bits / 2;
+  uint64_t mask = (1ull << half) - 1;
+  parallel_for(count, [&](int64_t j) {
+    uint64_t x = (uint64_t)values[j];
+    do {
+      uint64_t l = x >> half, r = x & mask;
+      for (int round = 0; round < 4; round++) {
+        uint64_t f = draw(key, (r << 2) | (uint64_t)round) & mask;
+        uint64_t nl = r, nr = l ^ f;
+        l = nl;
+        r = nr;
+      }
+      x = (l << half) | r;
+    } while (x >= (uint64_t)n);
+    out[j] = (int64_t)x;
+  });
+}
+
 // Parallel gather: out[i] = src[idx[i]] for 8-byte and 4-byte elements.
 void mlw_gather8(const int64_t* src, const int64_t* idx, int64_t n, int64_t* out) {
   parallel_for(n, [&](int64_t i) { out[i] = src[idx[i]]; });
