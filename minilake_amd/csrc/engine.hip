@@ -238,6 +238,130 @@ int PipelineBuild::grid(int block) const {
 
 static int block_of(ml_ctx* ctx) { return ctx->cfg.block_size > 0 ? ctx->cfg.block_size : 256; }
 
+// Stages whose row index a program's loads read.
+static uint32_t prog_stages(const std::vector<Instr>& ins, const Prog& p) {
+  uint32_t m = 0;
+  for (int i = 0; i < p.count; i++) {
+    const Instr& in = ins[(size_t)p.first + i];
+    if (in.op < OP_CONST) m |= 1u << in.stage;
+  }
+  return m;
+}
+
+static uint32_t step_stages(const std::vector<Instr>& ins, const Step& st) {
+  if (st.kind == STEP_FILTER) return prog_stages(ins, st.pred);
+  uint32_t m = 0;
+  for (int k = 0; k < st.nkeys; k++) m |= prog_stages(ins, st.keys[k]);
+  return m;
+}
+
+struct FlowLaunch {
+  int grid = 0;
+  size_t lds = 0;
+  int64_t nwaves = 0;
+};
+
+constexpr int FLOW_BLOCK = 256;
+constexpr int64_t FLOW_MIN_CHUNK = 512;  // source rows per wave at least
+constexpr size_t FLOW_MAX_LDS = 64 * 1024;
+
+// Prepare B.P for flow_kernel: cut the steps into segments (a queue after
+// every step, or after every probe when there are more steps than levels),
+// keep in each queue only the stages read further down, and size the waves
+// so every CU is filled. `sink_stages` are the stages the sink reads. Returns
+// false when the general kernel must run instead (a probe that can match
+// several rows, row indices beyond 32 bits, or queues too large for LDS).
+static bool plan_flow(PipelineBuild& B, int sink, uint32_t sink_stages, size_t cells_bytes, FlowLaunch& F) {
+  Plan& P = B.P;
+  P.nseg = 0;
+  const int64_t active = P.end - P.begin;
+  if (B.expanding || P.expand_step >= 0 || active <= 0) return false;
+  if (P.end > (int64_t)UINT32_MAX) return false;
+  std::vector<int> cuts;
+  for (int s = 1; s <= P.nsteps; s++) cuts.push_back(s);
+  if ((int)cuts.size() > MAX_LEVELS) {
+    std::vector<int> c2;
+    for (int s = 1; s < P.nsteps; s++)
+      if (P.steps[s - 1].kind == STEP_PROBE) c2.push_back(s);
+    c2.push_back(P.nsteps);
+    if ((int)c2.size() > MAX_LEVELS) c2.erase(c2.begin() + (MAX_LEVELS - 1), c2.end() - 1);
+    cuts = c2;
+  }
+  const int nseg = (int)cuts.size() + 1;
+  uint32_t used_from[MAX_STEPS + 1];
+  used_from[P.nsteps] = sink_stages;
+  for (int s = P.nsteps - 1; s >= 0; s--) used_from[s] = used_from[s + 1] | step_stages(B.ins, P.steps[s]);
+  uint32_t assigned = 1;
+  int qw = 0;
+  int prev = 0;
+  for (int l = 0; l < nseg; l++) {
+    const int end = l < (int)cuts.size() ? cuts[l] : P.nsteps;
+    P.seg_end[l] = (int16_t)end;
+    if (l >= 1) {
+      const uint32_t live = assigned & used_from[prev];
+      P.live[l] = (uint8_t)live;
+      P.qoff[l] = (int16_t)qw;
+      qw += __builtin_popcount(live) * QCAP;
+    }
+    for (int s = prev; s < end; s++)
+      if (P.steps[s].kind == STEP_PROBE) assigned |= 1u << P.steps[s].stage;
+    prev = end;
+  }
+  const size_t lds = cells_bytes + (size_t)(FLOW_BLOCK / 64) * qw * 4;
+  if (lds > FLOW_MAX_LDS) return false;
+  P.nseg = nseg;
+  P.qwords = qw;
+  // the stages a ROWS sink records
+  P.tup_mask = (uint8_t)sink_stages;
+  // one wave per resident slot, each with a contiguous chunk of source rows
+  const int bpc = flow_blocks_per_cu(sink, FLOW_BLOCK, lds);
+  int64_t nwaves = (int64_t)B.ctx->num_cus * bpc * (FLOW_BLOCK / 64);
+  int64_t chunk = (active + nwaves - 1) / nwaves;
+  if (chunk < FLOW_MIN_CHUNK) {
+    nwaves = (active + FLOW_MIN_CHUNK - 1) / FLOW_MIN_CHUNK;
+    nwaves = (nwaves + 3) / 4 * 4;
+    chunk = (active + nwaves - 1) / nwaves;
+  }
+  P.chunk = chunk;
+  P.staging = (uint64_t)nwaves * (uint64_t)chunk;
+  F.grid = (int)(nwaves / 4);
+  F.lds = lds;
+  F.nwaves = nwaves;
+  return true;
+}
+
+// Stages read by the sink programs of a pipeline's aggregate / materialization.
+static uint32_t sink_read_stages(const PipelineBuild& B) {
+  const Plan& P = B.P;
+  uint32_t m = 0;
+  for (int k = 0; k < P.nkeys; k++) m |= prog_stages(B.ins, P.keys[k]);
+  for (int a = 0; a < P.nargs; a++) m |= prog_stages(B.ins, P.args[a]);
+  for (int c = 0; c < P.ncols; c++) m |= prog_stages(B.ins, P.cols[c]);
+  if (P.pair_out)
+    for (int s = 0; s < P.pair_stages; s++) m |= 1u << s;
+  return m;
+}
+
+// Run a ROWS flow and materialize its rows densely (sink = SINK_COLLECT or
+// SINK_JOIN_BUILD, whose outputs are already set in B.P). Returns the row count.
+static uint64_t run_flow_rows(PipelineBuild& B, int sink, const FlowLaunch& F) {
+  ml_ctx* ctx = B.ctx;
+  Buffer tuples(ctx->arena, std::max<uint64_t>(1, (uint64_t)__builtin_popcount(B.P.tup_mask) * B.P.staging) * 4);
+  Buffer counts(ctx->arena, (size_t)F.nwaves * 4);
+  Buffer offsets(ctx->arena, (size_t)(F.nwaves + 1) * 8);
+  B.P.tuples = tuples.as<uint32_t>();
+  B.P.wave_counts = counts.as<uint32_t>();
+  const int keep = B.P.sink;
+  B.P.sink = SINK_ROWS;
+  ctx->launch(B.P, F.grid, FLOW_BLOCK, F.lds);
+  B.P.sink = keep;
+  launch_scan_counts(counts.as<uint32_t>(), F.nwaves, offsets.as<uint64_t>(), ctx->stream);
+  launch_materialize(B.P, sink, offsets.as<uint64_t>(), F.nwaves, ctx->stream);
+  uint64_t total = 0;
+  d2h(ctx, &total, offsets.as<uint64_t>() + F.nwaves, 8);
+  return total;
+}
+
 // Columns of a layout materialized on the device (collect, join payload).
 struct Materialized {
   std::vector<int> types;
@@ -261,9 +385,20 @@ static uint64_t run_count(PipelineBuild& B) {
   ml_ctx* ctx = B.ctx;
   uint64_t* counters = ctx->counters.as<uint64_t>();
   B.P.sink = SINK_COUNT_PAIRS;
+  B.upload();
+  FlowLaunch F;
+  if (plan_flow(B, SINK_COUNT_PAIRS, 0, 0, F)) {
+    Buffer counts(ctx->arena, (size_t)F.nwaves * 4);
+    Buffer offsets(ctx->arena, (size_t)(F.nwaves + 1) * 8);
+    B.P.wave_counts = counts.as<uint32_t>();
+    ctx->launch(B.P, F.grid, FLOW_BLOCK, F.lds);
+    launch_scan_counts(counts.as<uint32_t>(), F.nwaves, offsets.as<uint64_t>(), ctx->stream);
+    uint64_t n = 0;
+    d2h(ctx, &n, offsets.as<uint64_t>() + F.nwaves, 8);
+    return n;
+  }
   B.P.out_count = counters;
   zero(ctx, counters, 16);
-  B.upload();
   if (B.active_rows > 0) ctx->launch(B.P, B.grid(), block_of(ctx), 0);
   uint64_t n = 0;
   d2h(ctx, &n, counters, 8);
@@ -320,8 +455,13 @@ static bool run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buf
   B.P.out_count = counters;
   B.P.out_capacity = cap;
   B.P.out_overflow = (uint32_t*)(counters + 1);
-  zero(ctx, counters, 16);
   B.upload();
+  FlowLaunch F;
+  if (cap >= (uint64_t)B.active_rows && plan_flow(B, SINK_ROWS, sink_read_stages(B), 0, F)) {
+    out.rows = (int64_t)run_flow_rows(B, SINK_COLLECT, F);
+    return true;
+  }
+  zero(ctx, counters, 16);
   ctx->launch(B.P, B.grid(), block_of(ctx), 0);
   uint64_t hc[2] = {0, 0};
   d2h(ctx, hc, counters, 16);
@@ -674,12 +814,23 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
       B.P.dim_stride[k] = A->dim_stride[k];
     }
     B.P.ngroups = (int32_t)A->ngroups;
-    const int grid = B.grid();
     const int ncells = (int)(A->ngroups * (int64_t)A->slots.size());
+    B.upload();
+    FlowLaunch F;
+    int grid, block;
+    size_t lds;
+    if (plan_flow(B, SINK_DIRECT_AGG, sink_read_stages(B), (size_t)ncells * 8, F)) {
+      grid = F.grid;
+      block = FLOW_BLOCK;
+      lds = F.lds;
+    } else {
+      grid = B.grid();
+      block = block_of(ctx);
+      lds = (size_t)ncells * 8;
+    }
     Buffer partials(ctx->arena, (size_t)grid * ncells * 8);
     B.P.direct_partials = partials.ptr;
-    B.upload();
-    ctx->launch(B.P, grid, block_of(ctx), (size_t)ncells * 8);
+    ctx->launch(B.P, grid, block, lds);
     launch_direct_reduce(partials.as<uint64_t>(), grid, ncells, (int)A->slots.size(), slot_kinds(A),
                          A->totals.as<uint64_t>(), ctx->stream);
     return;
@@ -699,7 +850,9 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
     B.P.agg_mask = A->cap - 1;
     B.P.agg_fill_limit = fill_limit;
     for (size_t s = 0; s < A->slots.size(); s++) B.P.acc_data[s] = A->acc[s].ptr;
-    ctx->launch(B.P, B.grid(), block_of(ctx), 0);
+    FlowLaunch F;
+    if (plan_flow(B, SINK_HASH_AGG, sink_read_stages(B), 0, F)) ctx->launch(B.P, F.grid, FLOW_BLOCK, F.lds);
+    else ctx->launch(B.P, B.grid(), block_of(ctx), 0);
     A->len_known = false;
   };
   if (A->sizing == ML_SIZING_PLAIN) {
@@ -961,13 +1114,18 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
   B.P.out_count = counters;
   B.P.out_capacity = cap;
   B.P.out_overflow = (uint32_t*)(counters + 1);
-  zero(ctx, counters, 16);
   B.upload();
-  ctx->launch(B.P, B.grid(), block_of(ctx), 0);
-  uint64_t hc[2] = {0, 0};
-  d2h(ctx, hc, counters, 16);
-  if (hc[1] & 1) fail(ML_ERR_INTERNAL, "join build output overflowed");
-  ch.rows = (int64_t)hc[0];
+  FlowLaunch F;
+  if (plan_flow(B, SINK_ROWS, sink_read_stages(B), 0, F)) {
+    ch.rows = (int64_t)run_flow_rows(B, SINK_JOIN_BUILD, F);
+  } else {
+    zero(ctx, counters, 16);
+    ctx->launch(B.P, B.grid(), block_of(ctx), 0);
+    uint64_t hc[2] = {0, 0};
+    d2h(ctx, hc, counters, 16);
+    if (hc[1] & 1) fail(ML_ERR_INTERNAL, "join build output overflowed");
+    ch.rows = (int64_t)hc[0];
+  }
   J->chunks.push_back(std::move(ch));
 }
 
@@ -1233,14 +1391,16 @@ void ml_result::finish_views() {
 
 void ml_ctx::launch(const mlg::Plan& P, int grid, int block, size_t lds) {
   if (!stats_on) {
-    mlg::launch_pipeline(P, grid, block, lds, stream);
+    if (P.nseg > 0) mlg::launch_flow(P, grid, block, lds, stream);
+    else mlg::launch_pipeline(P, grid, block, lds, stream);
     return;
   }
   PendingStat ps{P.sink, P.end - P.begin, nullptr, nullptr};
   mlg::hip_check(hipEventCreate(&ps.start), "hipEventCreate");
   mlg::hip_check(hipEventCreate(&ps.stop), "hipEventCreate");
   mlg::hip_check(hipEventRecord(ps.start, stream), "hipEventRecord");
-  mlg::launch_pipeline(P, grid, block, lds, stream);
+  if (P.nseg > 0) mlg::launch_flow(P, grid, block, lds, stream);
+  else mlg::launch_pipeline(P, grid, block, lds, stream);
   mlg::hip_check(hipEventRecord(ps.stop, stream), "hipEventRecord");
   stats.push_back(ps);
 }

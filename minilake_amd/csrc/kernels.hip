@@ -21,7 +21,8 @@ typedef __attribute__((address_space(4))) const Instr CInstr;
 __device__ __forceinline__ double u2f(uint64_t x) { return __longlong_as_double((long long)x); }
 __device__ __forceinline__ uint64_t f2u(double x) { return (uint64_t)__double_as_longlong(x); }
 
-__device__ __forceinline__ int64_t row_of(const int64_t (&rows)[MAX_STAGES], int s) {
+// Row indices are 32-bit: sources and build sides are limited to 2^32 - 1 rows.
+__device__ __forceinline__ uint32_t row_of(const uint32_t (&rows)[MAX_STAGES], int s) {
   switch (s) {
     case 0: return rows[0];
     case 1: return rows[1];
@@ -33,7 +34,7 @@ __device__ __forceinline__ int64_t row_of(const int64_t (&rows)[MAX_STAGES], int
     default: return rows[7];
   }
 }
-__device__ __forceinline__ void set_row(int64_t (&rows)[MAX_STAGES], int s, int64_t v) {
+__device__ __forceinline__ void set_row(uint32_t (&rows)[MAX_STAGES], int s, uint32_t v) {
   switch (s) {
     case 0: rows[0] = v; break;
     case 1: rows[1] = v; break;
@@ -86,7 +87,7 @@ __device__ __forceinline__ void set_slot(uint64_t (&r)[MAX_DEPTH], int d, uint64
 }
 
 // Evaluate a program for one row; returns validity, value in `val`.
-__device__ __forceinline__ bool eval(CInstr* ins, Prog p, const int64_t (&rows)[MAX_STAGES], uint64_t& val) {
+__device__ __forceinline__ bool eval(CInstr* ins, Prog p, const uint32_t (&rows)[MAX_STAGES], uint64_t& val) {
   uint64_t r[MAX_DEPTH];
 #pragma unroll
   for (int i = 0; i < MAX_DEPTH; i++) r[i] = 0;
@@ -106,7 +107,7 @@ __device__ __forceinline__ bool eval(CInstr* ins, Prog p, const int64_t (&rows)[
         res = imm;
         vres = in.flags & 1u;
       } else {
-        const int64_t row = row_of(rows, in.stage);
+        const uint64_t row = row_of(rows, in.stage);
         if (op == OP_LOAD_I32) res = sx32((uint32_t)((const int32_t*)imm)[row]);
         else if (op == OP_LOAD_U32) res = (uint64_t)((const uint32_t*)imm)[row];
         else if (op == OP_LOAD_I64) res = ((const uint64_t*)imm)[row];
@@ -244,7 +245,7 @@ __device__ __forceinline__ bool join_next(const DevJoinTable& T, const uint64_t 
 }
 
 __device__ __forceinline__ bool probe_keys(const Plan& P, CInstr* ins, const Step& st,
-                                           const int64_t (&rows)[MAX_STAGES], uint64_t (&w)[MAX_KEYS + 1]) {
+                                           const uint32_t (&rows)[MAX_STAGES], uint64_t (&w)[MAX_KEYS + 1]) {
   bool ok = true;
 #pragma unroll
   for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = 0;
@@ -259,7 +260,7 @@ __device__ __forceinline__ bool probe_keys(const Plan& P, CInstr* ins, const Ste
 
 // Filters and unique probes in [from, to); false once the row is dropped.
 __device__ __forceinline__ bool run_steps(const Plan& P, CInstr* ins, int from, int to,
-                                          int64_t (&rows)[MAX_STAGES]) {
+                                          uint32_t (&rows)[MAX_STAGES]) {
   bool alive = true;
   for (int s = from; s < to; s++) {
     if (!alive) break;
@@ -406,7 +407,7 @@ __device__ __forceinline__ int64_t agg_slot(const Plan& P, const uint64_t (&w)[M
   return -1;
 }
 
-__device__ __forceinline__ int agg_keys(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES],
+__device__ __forceinline__ int agg_keys(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES],
                                         uint64_t (&w)[MAX_KEYS + 1]) {
   uint64_t nullmask = 0;
 #pragma unroll
@@ -424,7 +425,7 @@ __device__ __forceinline__ int agg_keys(const Plan& P, CInstr* ins, const int64_
   return P.nkeys + (P.key_nullable ? 1 : 0);
 }
 
-__device__ __forceinline__ void sink_hash_agg(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES]) {
+__device__ __forceinline__ void sink_hash_agg(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES]) {
   uint64_t w[MAX_KEYS + 1];
   const int nkw = agg_keys(P, ins, rows, w);
   const uint64_t h = key_hash(w, nkw);
@@ -444,7 +445,7 @@ __device__ __forceinline__ void sink_hash_agg(const Plan& P, CInstr* ins, const 
     if (P.acc[s].arg < 0) atomicAdd((unsigned long long*)P.acc_data[s] + slot, 1ull);
 }
 
-__device__ __forceinline__ void sink_direct_agg(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES],
+__device__ __forceinline__ void sink_direct_agg(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES],
                                                 uint64_t* lds) {
   int64_t g = 0;
 #pragma unroll 1
@@ -468,7 +469,7 @@ __device__ __forceinline__ void sink_direct_agg(const Plan& P, CInstr* ins, cons
 }
 
 // Materialize every column of the layout for one row at output position pos.
-__device__ __forceinline__ void write_cols(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES], uint64_t pos) {
+__device__ __forceinline__ void write_cols(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES], uint64_t pos) {
   for (int c = 0; c < P.ncols; c++) {
     uint64_t v;
     const bool ok = eval(ins, P.cols[c], rows, v);
@@ -480,7 +481,7 @@ __device__ __forceinline__ void write_cols(const Plan& P, CInstr* ins, const int
   }
 }
 
-__device__ __forceinline__ void sink_join_build(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES]) {
+__device__ __forceinline__ void sink_join_build(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES]) {
   const uint64_t pos = wave_alloc(P.out_count, 1);
   if (pos >= P.out_capacity) {
     atomicOr(P.out_overflow, 1u);
@@ -500,7 +501,7 @@ __device__ __forceinline__ void sink_join_build(const Plan& P, CInstr* ins, cons
   write_cols(P, ins, rows, pos);
 }
 
-__device__ __forceinline__ void sink_collect(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES]) {
+__device__ __forceinline__ void sink_collect(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES]) {
   const uint64_t pos = wave_alloc(P.out_count, 1);
   if (pos >= P.out_capacity) {
     atomicOr(P.out_overflow, 1u);
@@ -513,7 +514,7 @@ __device__ __forceinline__ void sink_collect(const Plan& P, CInstr* ins, const i
 }
 
 template <int SINK>
-__device__ __forceinline__ void do_sink(const Plan& P, CInstr* ins, const int64_t (&rows)[MAX_STAGES], int64_t i,
+__device__ __forceinline__ void do_sink(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES], int64_t i,
                                         uint64_t* lds) {
   if constexpr (SINK == SINK_HASH_AGG) sink_hash_agg(P, ins, rows);
   else if constexpr (SINK == SINK_DIRECT_AGG) sink_direct_agg(P, ins, rows, lds);
@@ -546,10 +547,10 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
   }
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = P.begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.end; i += stride) {
-    int64_t rows[MAX_STAGES];
+    uint32_t rows[MAX_STAGES];
 #pragma unroll
     for (int s = 0; s < MAX_STAGES; s++) rows[s] = 0;
-    rows[0] = P.sel ? (int64_t)P.sel[i] : i;
+    rows[0] = P.sel ? P.sel[i] : (uint32_t)i;
     if (P.expand_step < 0) {
       const bool alive = run_steps(P, ins, 0, P.nsteps, rows);
       if constexpr (SINK == SINK_FLAGS) P.flag_out[i] = alive;
@@ -584,27 +585,199 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
   }
 }
 
-// ---------------------------------------------------------- direct agg reduce
-// One thread per (group, state): combine block partials in block order.
-__global__ void direct_reduce_kernel(const uint64_t* partials, int nblocks, int ncells, int nacc, AccSlotList accs,
-                                     uint64_t* out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
-  const int kind = accs.kind[c % nacc];
-  uint64_t acc = out[c];  // running totals of earlier batches
-  for (int b = 0; b < nblocks; b++) {
-    const uint64_t v = partials[(uint64_t)b * ncells + c];
-    switch (kind) {
-      case ACC_SUM_F64: acc = f2u(u2f(acc) + u2f(v)); break;
-      case ACC_SUM_I64: case ACC_COUNT: acc += v; break;
-      case ACC_MIN_I64: acc = (int64_t)v < (int64_t)acc ? v : acc; break;
-      case ACC_MAX_I64: acc = (int64_t)v > (int64_t)acc ? v : acc; break;
-      case ACC_MIN_F64: acc = u2f(v) < u2f(acc) ? v : acc; break;
-      case ACC_MAX_F64: acc = u2f(v) > u2f(acc) ? v : acc; break;
-      default: break;
+// ------------------------------------------------------ wave-queued pipeline
+// Queue heads and counts of levels 1..MAX_LEVELS, 7 bits each (0..127),
+// packed into one wave-uniform 64-bit word so they stay in scalar registers.
+__device__ __forceinline__ uint32_t qget(uint64_t packed, int l) { return (uint32_t)(packed >> (7 * l)) & 127u; }
+__device__ __forceinline__ uint64_t qset(uint64_t packed, int l, uint32_t v) {
+  return (packed & ~(127ull << (7 * l))) | ((uint64_t)v << (7 * l));
+}
+
+// Row indices of the stages in `mask`, stored as u32 at base[k * stride]
+// for the k-th stage of the mask. Unrolled over stages so rows[] stays in
+// registers.
+__device__ __forceinline__ void store_stages(uint32_t* base, uint64_t stride, uint32_t mask,
+                                             const uint32_t (&rows)[MAX_STAGES]) {
+  int k = 0;
+#pragma unroll
+  for (int s = 0; s < MAX_STAGES; s++)
+    if (mask >> s & 1) base[(uint64_t)(k++) * stride] = rows[s];
+}
+__device__ __forceinline__ void load_stages(const uint32_t* base, uint64_t stride, uint32_t mask,
+                                            uint32_t (&rows)[MAX_STAGES]) {
+  int k = 0;
+#pragma unroll
+  for (int s = 0; s < MAX_STAGES; s++)
+    if (mask >> s & 1) rows[s] = base[(uint64_t)(k++) * stride];
+}
+
+// Each wave owns source rows [begin + w*chunk, begin + (w+1)*chunk). Level 0
+// reads 64 source rows; level l >= 1 pops up to 64 queued rows. A round runs
+// the level's segment; survivors go to the next level's queue (or the sink
+// after the last segment). The deepest level holding a full wave runs first,
+// so queues never exceed QCAP entries, and rows reach the sink in source
+// order: every queued row precedes the rows of shallower levels.
+template <int SINK>
+__global__ void __launch_bounds__(256) flow_kernel(Plan P) {
+  extern __shared__ uint64_t lds[];
+  CInstr* ins = (CInstr*)P.ins;
+  const int ncells = (SINK == SINK_DIRECT_AGG) ? P.ngroups * P.nacc : 0;
+  if constexpr (SINK == SINK_DIRECT_AGG) {
+    for (int c = threadIdx.x; c < ncells; c += blockDim.x) lds[c] = acc_init_bits(P.acc[c % P.nacc].kind);
+    __syncthreads();
+  }
+  const int lane = __lane_id();
+  const int wv = threadIdx.x >> 6;
+  uint32_t* q = (uint32_t*)(lds + ncells) + wv * P.qwords;
+  const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
+  int64_t next = P.begin + gw * P.chunk;
+  const int64_t stop = min(P.end, next + P.chunk);
+  static_assert(QCAP == 128 && 7 * (MAX_LEVELS + 1) <= 64, "queue state packing");
+  uint64_t qh = 0, qn = 0;
+  const int last = P.nseg - 1;
+  uint64_t nout = 0;
+  for (;;) {
+    int lvl = -1;
+#pragma unroll
+    for (int l = MAX_LEVELS; l >= 1; l--)
+      if (lvl < 0 && l <= last && qget(qn, l) >= 64) lvl = l;
+    if (lvl < 0 && next < stop) lvl = 0;
+    if (lvl < 0) {
+#pragma unroll
+      for (int l = MAX_LEVELS; l >= 1; l--)
+        if (lvl < 0 && l <= last && qget(qn, l) > 0) lvl = l;
+    }
+    if (lvl < 0) break;
+
+    uint32_t rows[MAX_STAGES];
+#pragma unroll
+    for (int s = 0; s < MAX_STAGES; s++) rows[s] = 0;
+    int n;
+    if (lvl == 0) {
+      n = (int)min((int64_t)64, stop - next);
+      if (lane < n) {
+        const int64_t i = next + lane;
+        rows[0] = P.sel ? P.sel[i] : (uint32_t)i;
+      }
+      next += n;
+    } else {
+      const uint32_t h = qget(qh, lvl), c = qget(qn, lvl);
+      n = (int)min(64u, c);
+      if (lane < n) load_stages(q + P.qoff[lvl] + ((h + (uint32_t)lane) & (QCAP - 1)), QCAP, P.live[lvl], rows);
+      qh = qset(qh, lvl, (h + (uint32_t)n) & (QCAP - 1));
+      qn = qset(qn, lvl, c - (uint32_t)n);
+    }
+    const int sb = lvl == 0 ? 0 : P.seg_end[lvl - 1];
+    const bool alive = lane < n && run_steps(P, ins, sb, P.seg_end[lvl], rows);
+
+    if (lvl == last) {
+      if constexpr (SINK == SINK_DIRECT_AGG) {
+        if (alive) sink_direct_agg(P, ins, rows, lds);
+      } else if constexpr (SINK == SINK_HASH_AGG) {
+        if (alive) sink_hash_agg(P, ins, rows);
+      } else {
+        const uint64_t m = __ballot(alive);
+        if constexpr (SINK == SINK_ROWS) {
+          if (alive) {
+            const uint64_t pos =
+                (uint64_t)gw * P.chunk + nout + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+            store_stages(P.tuples + pos, P.staging, P.tup_mask, rows);
+          }
+        }
+        nout += (uint64_t)__popcll(m);
+      }
+    } else {
+      const uint64_t m = __ballot(alive);
+      const int L = lvl + 1;
+      const uint32_t h = qget(qh, L), c = qget(qn, L);
+      if (alive) {
+        const uint32_t slot = (h + c + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))) & (QCAP - 1);
+        store_stages(q + P.qoff[L] + slot, QCAP, P.live[L], rows);
+      }
+      qn = qset(qn, L, c + (uint32_t)__popcll(m));
+    }
+    // queue writes of this round are read by other lanes in later rounds
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if constexpr (SINK == SINK_ROWS || SINK == SINK_COUNT_PAIRS) {
+    if (lane == 0) P.wave_counts[gw] = (uint32_t)nout;
+  }
+  if constexpr (SINK == SINK_DIRECT_AGG) {
+    __syncthreads();
+    uint64_t* out = (uint64_t*)P.direct_partials + (uint64_t)blockIdx.x * ncells;
+    for (int c = threadIdx.x; c < ncells; c += blockDim.x) out[c] = lds[c];
+  }
+}
+
+// Materialize the rows a ROWS flow produced: one workgroup per wave region,
+// written densely at the region's exclusive offset (source order kept).
+template <int SINK>
+__global__ void __launch_bounds__(256) materialize_kernel(Plan P, const uint64_t* offsets) {
+  CInstr* ins = (CInstr*)P.ins;
+  const int64_t w = blockIdx.x;
+  const uint32_t cnt = P.wave_counts[w];
+  const uint64_t base = offsets[w];
+  for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
+    const uint64_t src = (uint64_t)w * P.chunk + k;
+    uint32_t rows[MAX_STAGES];
+#pragma unroll
+    for (int s = 0; s < MAX_STAGES; s++) rows[s] = 0;
+    load_stages(P.tuples + src, P.staging, P.tup_mask, rows);
+    const uint64_t pos = base + k;
+    if constexpr (SINK == SINK_JOIN_BUILD) {
+      bool any_null = false;
+#pragma unroll 1
+      for (int kk = 0; kk < P.nkeys; kk++) {
+        uint64_t v;
+        if (!eval(ins, P.keys[kk], rows, v)) {
+          any_null = true;
+          v = 0;
+        }
+        P.key_out[pos * P.nkeys + kk] = v;
+      }
+      P.key_null[pos] = any_null;
+    }
+    write_cols(P, ins, rows, pos);
+    if constexpr (SINK == SINK_COLLECT) {
+      if (P.pair_out)
+        for (int s = 0; s < P.pair_stages; s++) P.pair_out[pos * P.pair_stages + s] = row_of(rows, s);
     }
   }
-  out[c] = acc;
+}
+
+// ---------------------------------------------------------- direct agg reduce
+// One workgroup per (group, state) cell: a fixed-shape tree over the block
+// partials (thread t folds partials t, t+256, ... in order, then a pairwise
+// LDS tree), added to the running total of earlier batches. The shape depends
+// only on the partial count, so results are identical from run to run.
+__device__ __forceinline__ uint64_t acc_combine(int kind, uint64_t a, uint64_t b) {
+  switch (kind) {
+    case ACC_SUM_F64: return f2u(u2f(a) + u2f(b));
+    case ACC_SUM_I64: case ACC_COUNT: return a + b;
+    case ACC_MIN_I64: return (int64_t)b < (int64_t)a ? b : a;
+    case ACC_MAX_I64: return (int64_t)b > (int64_t)a ? b : a;
+    case ACC_MIN_F64: return u2f(b) < u2f(a) ? b : a;
+    case ACC_MAX_F64: return u2f(b) > u2f(a) ? b : a;
+    default: return a;
+  }
+}
+
+__global__ void __launch_bounds__(256) direct_tree_kernel(const uint64_t* partials, int nblocks, int ncells, int nacc,
+                                                          AccSlotList accs, uint64_t* out) {
+  __shared__ uint64_t buf[256];
+  const int c = blockIdx.x;
+  const int kind = accs.kind[c % nacc];
+  uint64_t acc = acc_init_bits(kind);
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) acc = acc_combine(kind, acc, partials[(uint64_t)b * ncells + c]);
+  buf[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) buf[threadIdx.x] = acc_combine(kind, buf[threadIdx.x], buf[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = acc_combine(kind, out[c], buf[0]);
 }
 
 // ------------------------------------------------------------ fills / gathers
@@ -840,7 +1013,7 @@ __global__ void sort_keys_kernel(Plan P, SortKeys K, int64_t n, uint64_t* words)
   const int nw = K.nwords;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    int64_t rows[MAX_STAGES];
+    uint32_t rows[MAX_STAGES];
 #pragma unroll
     for (int s = 0; s < MAX_STAGES; s++) rows[s] = 0;
     rows[0] = i;
@@ -940,7 +1113,41 @@ void launch_pipeline(const Plan& P, int grid, int block, size_t lds, hipStream_t
 
 void launch_direct_reduce(const uint64_t* partials, int nblocks, int ncells, int nacc, const AccSlotList& accs,
                           uint64_t* out, hipStream_t s) {
-  direct_reduce_kernel<<<(ncells + 63) / 64, 64, 0, s>>>(partials, nblocks, ncells, nacc, accs, out);
+  if (ncells <= 0) return;
+  direct_tree_kernel<<<ncells, 256, 0, s>>>(partials, nblocks, ncells, nacc, accs, out);
+}
+
+void launch_flow(const Plan& P, int grid, int block, size_t lds, hipStream_t s) {
+  switch (P.sink) {
+    case SINK_HASH_AGG: flow_kernel<SINK_HASH_AGG><<<grid, block, lds, s>>>(P); break;
+    case SINK_DIRECT_AGG: flow_kernel<SINK_DIRECT_AGG><<<grid, block, lds, s>>>(P); break;
+    case SINK_ROWS: flow_kernel<SINK_ROWS><<<grid, block, lds, s>>>(P); break;
+    case SINK_COUNT_PAIRS: flow_kernel<SINK_COUNT_PAIRS><<<grid, block, lds, s>>>(P); break;
+    default: break;
+  }
+}
+
+int flow_blocks_per_cu(int sink, int block, size_t lds) {
+  int nb = 0;
+  hipError_t e = hipErrorInvalidValue;
+  switch (sink) {
+    case SINK_HASH_AGG: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flow_kernel<SINK_HASH_AGG>, block, lds); break;
+    case SINK_DIRECT_AGG: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flow_kernel<SINK_DIRECT_AGG>, block, lds); break;
+    case SINK_ROWS: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flow_kernel<SINK_ROWS>, block, lds); break;
+    case SINK_COUNT_PAIRS: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flow_kernel<SINK_COUNT_PAIRS>, block, lds); break;
+    default: break;
+  }
+  return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+void launch_scan_counts(const uint32_t* counts, int64_t n, uint64_t* offsets, hipStream_t s) {
+  scan_counts_kernel<<<1, 1024, 0, s>>>(counts, n, offsets);
+}
+
+void launch_materialize(const Plan& P, int sink, const uint64_t* offsets, int64_t nwaves, hipStream_t s) {
+  if (nwaves <= 0) return;
+  if (sink == SINK_JOIN_BUILD) materialize_kernel<SINK_JOIN_BUILD><<<(unsigned)nwaves, 256, 0, s>>>(P, offsets);
+  else materialize_kernel<SINK_COLLECT><<<(unsigned)nwaves, 256, 0, s>>>(P, offsets);
 }
 
 void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s) {

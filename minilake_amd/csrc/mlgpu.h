@@ -17,6 +17,9 @@ constexpr int MAX_STEPS = 12;     // filters + probes in one fused pipeline
 constexpr int MAX_KEYS = 4;       // key columns (one 64-bit word each)
 constexpr int MAX_AGG_SLOTS = 24; // accumulator state slots per aggregate sink
 constexpr int MAX_COLS = 32;      // materialized columns (collect / join payload)
+constexpr int MAX_LEVELS = 6;     // LDS queue levels of the wave-queued pipeline
+constexpr int QCAP = 128;         // entries per queue level: one wave's pop leaves < 64,
+                                  // one push adds <= 64, so 127 is the most ever held
 
 // MiniLake's hash constants (crates/minilake-hashtable/src/hash.rs:12-17).
 constexpr uint64_t HSEED = 0x243F6A8885A308D3ull;
@@ -130,6 +133,7 @@ enum SinkKind : int32_t {
   SINK_PROJECT = 6,   // write every column at its physical row
   SINK_HASH = 7,      // hash_columns: one hash per active row
   SINK_FLAGS = 8,     // filter: one 0/1 flag per active row
+  SINK_ROWS = 9,      // wave-queued pipeline: surviving rows' stage indices, per-wave regions
 };
 
 enum StepKind : int32_t { STEP_FILTER = 0, STEP_PROBE = 1 };
@@ -196,6 +200,22 @@ struct Plan {
   uint32_t* out_overflow;
   int64_t* pair_out;         // COLLECT with row ids: [rows][nstages]
   int32_t pair_stages;
+
+  // ---- wave-queued execution (flow_kernel). The steps are cut into
+  // segments; between two segments every wave keeps the stage row indices of
+  // its surviving rows in an LDS queue, and runs the next segment only on
+  // full waves of 64 queued rows, so selective filters and probes leave no
+  // lanes idle further down the chain. Each wave owns `chunk` source rows.
+  int32_t nseg;                          // segments (queue levels 1 .. nseg-1)
+  int32_t qwords;                        // u32 words of one wave's queues
+  int16_t seg_end[MAX_LEVELS + 1];       // segment s = steps [seg_end[s-1], seg_end[s])
+  int16_t qoff[MAX_LEVELS + 1];          // level l queue: u32 offset in the wave's region
+  uint8_t live[MAX_LEVELS + 1];          // level l carries these stages' row indices (bit s = stage s)
+  uint8_t tup_mask;                      // ROWS: stages recorded per row
+  int64_t chunk;                         // source rows per wave
+  uint32_t* tuples;                      // ROWS: [k][staging] row index of the k-th recorded stage
+  uint64_t staging;                      // waves * chunk
+  uint32_t* wave_counts;                 // ROWS / COUNT_PAIRS: rows produced per wave
 };
 
 // Sortable key words for TopN.
