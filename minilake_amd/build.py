@@ -35,9 +35,9 @@ def hipcc() -> str:
     return p
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(src: str, build_dir: str = BUILD, defines=()) -> str:
+    obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
+    cmd = [hipcc(), *HIPCC_FLAGS, *[f"-D{d}" for d in defines], "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         msg = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
@@ -45,18 +45,20 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build_gpu_library(jobs: int = 8) -> str:
-    os.makedirs(BUILD, exist_ok=True)
+def build_gpu_library(jobs: int = 8, out: str = LIB, build_dir: str = BUILD, defines=()) -> str:
+    """Compile and link the library; `defines` (NAME=VALUE) override the
+    tuning constants of the sources (ML_FAST_U, ML_BLOOM_KEYS_PER_WORD, ML_NT_KEYS)."""
+    os.makedirs(build_dir, exist_ok=True)
     sources = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with cf.ThreadPoolExecutor(max_workers=max(1, min(jobs, len(sources)))) as ex:
-        objs = list(ex.map(_compile, sources))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda src: _compile(src, build_dir, defines), sources))
+    tmp = out + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 def build_all() -> None:

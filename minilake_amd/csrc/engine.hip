@@ -225,6 +225,7 @@ void PipelineBuild::upload() {
   dev_ins = Buffer(ctx->arena, ins.size() * sizeof(Instr));
   h2d(ctx, dev_ins.ptr, ins.data(), ins.size() * sizeof(Instr));
   P.ins = dev_ins.as<Instr>();
+  P.nins = (int32_t)ins.size();
 }
 
 int PipelineBuild::grid(int block) const {
@@ -235,6 +236,10 @@ int PipelineBuild::grid(int block) const {
   if (g < 1) g = 1;
   return (int)g;
 }
+
+#ifndef ML_BLOOM_KEYS_PER_WORD
+#define ML_BLOOM_KEYS_PER_WORD 8
+#endif
 
 static int block_of(ml_ctx* ctx) { return ctx->cfg.block_size > 0 ? ctx->cfg.block_size : 256; }
 
@@ -288,6 +293,26 @@ static bool plan_flow(PipelineBuild& B, int sink, uint32_t sink_stages, size_t c
     cuts = c2;
   }
   const int nseg = (int)cuts.size() + 1;
+  // streaming first probe: step 0 alone in segment 0, probing a unique table
+  // with plain non-null stage-0 column loads as key words
+  P.fast = 0;
+  if (nseg >= 2 && cuts[0] == 1 && P.steps[0].kind == STEP_PROBE && P.steps[0].nkeys <= 2 &&
+      P.tables[P.steps[0].table].unique) {
+    bool plain = true;
+    for (int k = 0; k < P.steps[0].nkeys; k++) {
+      const Prog& kp = P.steps[0].keys[k];
+      const Instr& in = B.ins[(size_t)kp.first];
+      plain = plain && kp.count == 1 && !kp.nullable && in.op < OP_CONST && in.stage == 0 && in.imm2 == 0;
+      if (plain) {
+        P.fast_op[k] = in.op;
+        P.fast_col[k] = (const void*)in.imm;
+      }
+    }
+    if (plain) {
+      P.fast = 1;
+      P.fast_nkeys = P.steps[0].nkeys;
+    }
+  }
   uint32_t used_from[MAX_STEPS + 1];
   used_from[P.nsteps] = sink_stages;
   for (int s = P.nsteps - 1; s >= 0; s--) used_from[s] = used_from[s + 1] | step_stages(B.ins, P.steps[s]);
@@ -296,12 +321,13 @@ static bool plan_flow(PipelineBuild& B, int sink, uint32_t sink_stages, size_t c
   int prev = 0;
   for (int l = 0; l < nseg; l++) {
     const int end = l < (int)cuts.size() ? cuts[l] : P.nsteps;
-    P.seg_end[l] = (int16_t)end;
+    P.seg_end[l] = end;
     if (l >= 1) {
       const uint32_t live = assigned & used_from[prev];
-      P.live[l] = (uint8_t)live;
-      P.qoff[l] = (int16_t)qw;
-      qw += __builtin_popcount(live) * QCAP;
+      P.live[l] = live;
+      P.qoff[l] = qw;
+      P.qcap[l] = (l == 1 && P.fast) ? QCAP_FAST : QCAP;
+      qw += __builtin_popcount(live) * P.qcap[l];
     }
     for (int s = prev; s < end; s++)
       if (P.steps[s].kind == STEP_PROBE) assigned |= 1u << P.steps[s].stage;
@@ -312,7 +338,7 @@ static bool plan_flow(PipelineBuild& B, int sink, uint32_t sink_stages, size_t c
   P.nseg = nseg;
   P.qwords = qw;
   // the stages a ROWS sink records
-  P.tup_mask = (uint8_t)sink_stages;
+  P.tup_mask = sink_stages;
   // one wave per resident slot, each with a contiguous chunk of source rows
   const int bpc = flow_blocks_per_cu(sink, FLOW_BLOCK, lds);
   int64_t nwaves = (int64_t)B.ctx->num_cus * bpc * (FLOW_BLOCK / 64);
@@ -1185,24 +1211,72 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   T.dev.nkw = (int)nk;
   T.dev.mask = cap - 1;
   T.slots = Buffer(ctx->arena, cap * (1 + nk) * 8);
-  zero(ctx, T.slots.ptr, T.slots.bytes);
+  T.dev.slots = T.slots.as<uint64_t>();
   const int use_bloom = ctx->cfg.use_bloom;
   const bool bloom = use_bloom > 0 || (use_bloom < 0 && n >= (1 << 16));
-  if (bloom) {
-    const uint64_t words = std::max<uint64_t>(64, next_pow2((uint64_t)n / 8 + 1));
-    T.bloom = Buffer(ctx->arena, words * 8);
-    zero(ctx, T.bloom.ptr, T.bloom.bytes);
-    T.dev.bloom = T.bloom.as<uint64_t>();
-    T.dev.bloom_mask = words - 1;
+  // one 64-bit block (3 bits set per key) per ML_BLOOM_KEYS_PER_WORD keys
+  uint64_t bloom_words = bloom ? std::max<uint64_t>(64, next_pow2((uint64_t)n / ML_BLOOM_KEYS_PER_WORD + 1)) : 0;
+  uint32_t* flags = (uint32_t*)ctx->counters.as<uint64_t>();
+  zero(ctx, flags, 8);
+  // Partitioned build: partitions of S slots, each built by one workgroup in
+  // LDS (no global atomics), S chosen so a partition's slots fit in 32 KB.
+  const uint64_t smax = nk == 1 ? 2048 : (nk <= 3 ? 1024 : 512);
+  const uint64_t S = std::min(cap, smax);
+  const uint64_t nparts = cap / S;
+  bool built = false;
+  if (n > 0 && nparts <= 8192 && (uint64_t)n < 0xFFFFFFFFull) {
+    T.dev.wrap = S - 1;
+    T.dev.part_shift = __builtin_ctzll(S);
+    uint64_t W = 0;
+    if (bloom) {
+      W = std::min<uint64_t>(2048, std::max<uint64_t>(1, bloom_words / nparts));
+      T.bloom = Buffer(ctx->arena, W * nparts * 8);
+      T.dev.bloom = T.bloom.as<uint64_t>();
+      T.dev.bloom_shift = __builtin_ctzll(W);
+      T.dev.bloom_mask = W - 1;
+    }
+    const int nblocks = (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + 8191) / 8192));
+    const int64_t rpb = (n + nblocks - 1) / nblocks;
+    const int64_t nh = (int64_t)nparts * nblocks;
+    Buffer hist(ctx->arena, (size_t)nh * 4), offsets(ctx->arena, (size_t)(nh + 1) * 4);
+    Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(nh) + 1) * 4);
+    Buffer srow(ctx->arena, (size_t)n * 4), skeys(ctx->arena, (size_t)n * nk * 8);
+    launch_join_part_hist(T.dev, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, nblocks, rpb, (int)nparts,
+                          hist.as<uint32_t>(), ctx->stream);
+    launch_scan_u32(hist.as<uint32_t>(), nh, offsets.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
+    launch_join_part_scatter(T.dev, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, nblocks, rpb, (int)nparts,
+                             offsets.as<uint32_t>(), srow.as<uint32_t>(), skeys.as<uint64_t>(), ctx->stream);
+    launch_join_part_build(T.dev, (int)nparts, offsets.as<uint32_t>(), nblocks, srow.as<uint32_t>(),
+                           skeys.as<uint64_t>(), T.dev.bloom ? T.bloom.as<uint64_t>() : nullptr, flags,
+                           S * (1 + nk) * 8 + W * 8, ctx->stream);
+    uint32_t hf = 0;
+    d2h(ctx, &hf, flags, 4);
+    if (!(hf & 2)) {
+      T.dev.unique = (hf & 1) ? 0 : 1;
+      built = true;
+    } else {
+      zero(ctx, flags, 8);  // a partition overflowed (many equal keys): build unpartitioned
+    }
   }
-  T.dev.slots = T.slots.as<uint64_t>();
-  launch_join_insert(T.dev, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, T.bloom.as<uint64_t>(), ctx->stream);
-  uint32_t* dup = (uint32_t*)ctx->counters.as<uint64_t>();
-  zero(ctx, dup, 8);
-  launch_join_unique(T.dev, dup, ctx->stream);
-  uint32_t hdup = 0;
-  d2h(ctx, &hdup, dup, 4);
-  T.dev.unique = hdup ? 0 : 1;
+  if (!built) {
+    T.dev.wrap = cap - 1;
+    T.dev.part_shift = __builtin_ctzll(cap);
+    zero(ctx, T.slots.ptr, T.slots.bytes);
+    T.bloom = Buffer();
+    T.dev.bloom = nullptr;
+    if (bloom) {
+      T.bloom = Buffer(ctx->arena, bloom_words * 8);
+      zero(ctx, T.bloom.ptr, T.bloom.bytes);
+      T.dev.bloom = T.bloom.as<uint64_t>();
+      T.dev.bloom_shift = __builtin_ctzll(bloom_words);
+      T.dev.bloom_mask = bloom_words - 1;
+    }
+    launch_join_insert(T.dev, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, T.bloom.as<uint64_t>(), ctx->stream);
+    launch_join_unique(T.dev, flags, ctx->stream);
+    uint32_t hdup = 0;
+    d2h(ctx, &hdup, flags, 4);
+    T.dev.unique = hdup ? 0 : 1;
+  }
   for (size_t c = 0; c < T.col_types.size(); c++) {
     ml_column v{};
     v.type = T.col_types[c];

@@ -66,6 +66,18 @@ void launch_gather(const void* src, int width, const int64_t* idx, int64_t n, vo
 void launch_join_insert(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
                         uint64_t* bloom, hipStream_t s);
 void launch_join_unique(const DevJoinTable& T, uint32_t* dup_flag, hipStream_t s);
+// partitioned build (see join_part_build_kernel)
+void launch_join_part_hist(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                           int nblocks, int64_t rows_per_block, int nparts, uint32_t* hist, hipStream_t s);
+void launch_join_part_scatter(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                              int nblocks, int64_t rows_per_block, int nparts, const uint32_t* offsets,
+                              uint32_t* srow, uint64_t* skeys, hipStream_t s);
+void launch_join_part_build(const DevJoinTable& T, int nparts, const uint32_t* offsets, int nblocks,
+                            const uint32_t* srow, const uint64_t* skeys, uint64_t* bloom, uint32_t* flags,
+                            size_t lds, hipStream_t s);
+// exclusive scan of n u32 counts (any n < 2^32 total); out[n] = total
+void launch_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, uint32_t* block_sums, hipStream_t s);
+int64_t scan_u32_blocks(int64_t n);
 void launch_agg_rehash(const uint64_t* old_slots, uint64_t old_cap, int nkw, uint64_t* new_slots, uint64_t new_mask,
                        const AccMove& mv, hipStream_t s);
 void launch_slot_flags(const uint64_t* slots, uint64_t cap, int nkw, uint8_t* flags, hipStream_t s);

@@ -17,6 +17,23 @@
 namespace mlg {
 
 typedef __attribute__((address_space(4))) const Instr CInstr;
+typedef __attribute__((address_space(4))) const uint64_t CU64;
+
+// A value every active lane holds; readfirstlane lets the compiler keep it
+// (and whatever it indexes in constant memory) in scalar registers.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// The instruction stream of a plan. Instruction indices are the same in every
+// lane; fetching through at() makes them scalar loads. The index is clamped to
+// the program, so a fetch issued while no lane is active stays in the buffer.
+struct Code {
+  CInstr* base;
+  int32_t last;
+  __device__ __forceinline__ CInstr& at(int i) const {
+    const int j = uni(i);
+    return base[j < last ? j : last];
+  }
+};
 
 __device__ __forceinline__ double u2f(uint64_t x) { return __longlong_as_double((long long)x); }
 __device__ __forceinline__ uint64_t f2u(double x) { return (uint64_t)__double_as_longlong(x); }
@@ -87,16 +104,21 @@ __device__ __forceinline__ void set_slot(uint64_t (&r)[MAX_DEPTH], int d, uint64
 }
 
 // Evaluate a program for one row; returns validity, value in `val`.
-__device__ __forceinline__ bool eval(CInstr* ins, Prog p, const uint32_t (&rows)[MAX_STAGES], uint64_t& val) {
+__device__ __forceinline__ bool eval(const Code& ins, Prog p, const uint32_t (&rows)[MAX_STAGES], uint64_t& val) {
   uint64_t r[MAX_DEPTH];
 #pragma unroll
   for (int i = 0; i < MAX_DEPTH; i++) r[i] = 0;
   uint32_t ok = 0xFFFFFFFFu;
+  const int first = uni(p.first), count = uni(p.count), nullable = uni(p.nullable);
 #pragma unroll 1
-  for (int pc = 0; pc < p.count; pc++) {
-    CInstr& in = ins[p.first + pc];
-    const uint32_t op = in.op;
-    const int d = in.depth;
+  for (int pc = 0; pc < count; pc++) {
+    CInstr& in = ins.at(first + pc);
+    // op (u16) | depth (u8) << 16 | stage (u8) << 24 | flags << 32, one scalar load
+    const uint64_t head = *(CU64*)&in;
+    const uint32_t op = (uint32_t)head & 0xFFFFu;
+    const int d = (int)((head >> 16) & 0xFFu);
+    const int stage = (int)((head >> 24) & 0xFFu);
+    const uint32_t flags = (uint32_t)(head >> 32);
     const uint64_t imm = in.imm;
     uint64_t res;
     uint32_t vres = 1;
@@ -105,15 +127,15 @@ __device__ __forceinline__ bool eval(CInstr* ins, Prog p, const uint32_t (&rows)
       dst = d;
       if (op == OP_CONST) {
         res = imm;
-        vres = in.flags & 1u;
+        vres = flags & 1u;
       } else {
-        const uint64_t row = row_of(rows, in.stage);
+        const uint64_t row = row_of(rows, stage);
         if (op == OP_LOAD_I32) res = sx32((uint32_t)((const int32_t*)imm)[row]);
         else if (op == OP_LOAD_U32) res = (uint64_t)((const uint32_t*)imm)[row];
         else if (op == OP_LOAD_I64) res = ((const uint64_t*)imm)[row];
         else res = ((const uint8_t*)imm)[row] != 0;
         const uint64_t vp = in.imm2;
-        if (p.nullable && vp) vres = ((const uint8_t*)vp)[row] != 0;
+        if (nullable && vp) vres = ((const uint8_t*)vp)[row] != 0;
       }
     } else if (op < OP_ADD_I32) {  // unary on the top slot
       dst = d - 1;
@@ -217,8 +239,8 @@ __device__ __forceinline__ uint64_t key_hash(const uint64_t* w, int n) {
 __device__ __forceinline__ bool bloom_maybe(const DevJoinTable& T, uint64_t h) {
   if (!T.bloom) return true;
   const uint64_t g = hash_u64(h);
-  const uint64_t bits = (1ull << (g & 63)) | (1ull << ((g >> 6) & 63)) | (1ull << ((g >> 12) & 63));
-  return (T.bloom[(g >> 18) & T.bloom_mask] & bits) == bits;
+  const uint64_t bits = bloom_bits(g);
+  return (T.bloom[bloom_word(T, h, g)] & bits) == bits;
 }
 
 // First slot at or after `pos` whose key equals w; false at the first empty slot.
@@ -240,11 +262,11 @@ __device__ __forceinline__ bool join_next(const DevJoinTable& T, const uint64_t 
         return true;
       }
     }
-    pos = (pos + 1) & T.mask;
+    pos = join_step(T, pos);
   }
 }
 
-__device__ __forceinline__ bool probe_keys(const Plan& P, CInstr* ins, const Step& st,
+__device__ __forceinline__ bool probe_keys(const Plan& P, const Code& ins, const Step& st,
                                            const uint32_t (&rows)[MAX_STAGES], uint64_t (&w)[MAX_KEYS + 1]) {
   bool ok = true;
 #pragma unroll
@@ -259,11 +281,12 @@ __device__ __forceinline__ bool probe_keys(const Plan& P, CInstr* ins, const Ste
 }
 
 // Filters and unique probes in [from, to); false once the row is dropped.
-__device__ __forceinline__ bool run_steps(const Plan& P, CInstr* ins, int from, int to,
+__device__ __forceinline__ bool run_steps(const Plan& P, const Code& ins, int from, int to,
                                           uint32_t (&rows)[MAX_STAGES]) {
   bool alive = true;
-  for (int s = from; s < to; s++) {
+  for (int s_ = from; s_ < to; s_++) {
     if (!alive) break;
+    const int s = min(uni(s_), MAX_STEPS - 1);
     const Step& st = P.steps[s];
     if (st.kind == STEP_FILTER) {
       uint64_t v;
@@ -407,7 +430,7 @@ __device__ __forceinline__ int64_t agg_slot(const Plan& P, const uint64_t (&w)[M
   return -1;
 }
 
-__device__ __forceinline__ int agg_keys(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES],
+__device__ __forceinline__ int agg_keys(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES],
                                         uint64_t (&w)[MAX_KEYS + 1]) {
   uint64_t nullmask = 0;
 #pragma unroll
@@ -425,7 +448,7 @@ __device__ __forceinline__ int agg_keys(const Plan& P, CInstr* ins, const uint32
   return P.nkeys + (P.key_nullable ? 1 : 0);
 }
 
-__device__ __forceinline__ void sink_hash_agg(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES]) {
+__device__ __forceinline__ void sink_hash_agg(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES]) {
   uint64_t w[MAX_KEYS + 1];
   const int nkw = agg_keys(P, ins, rows, w);
   const uint64_t h = key_hash(w, nkw);
@@ -445,7 +468,7 @@ __device__ __forceinline__ void sink_hash_agg(const Plan& P, CInstr* ins, const 
     if (P.acc[s].arg < 0) atomicAdd((unsigned long long*)P.acc_data[s] + slot, 1ull);
 }
 
-__device__ __forceinline__ void sink_direct_agg(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES],
+__device__ __forceinline__ void sink_direct_agg(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES],
                                                 uint64_t* lds) {
   int64_t g = 0;
 #pragma unroll 1
@@ -469,7 +492,7 @@ __device__ __forceinline__ void sink_direct_agg(const Plan& P, CInstr* ins, cons
 }
 
 // Materialize every column of the layout for one row at output position pos.
-__device__ __forceinline__ void write_cols(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES], uint64_t pos) {
+__device__ __forceinline__ void write_cols(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES], uint64_t pos) {
   for (int c = 0; c < P.ncols; c++) {
     uint64_t v;
     const bool ok = eval(ins, P.cols[c], rows, v);
@@ -481,7 +504,7 @@ __device__ __forceinline__ void write_cols(const Plan& P, CInstr* ins, const uin
   }
 }
 
-__device__ __forceinline__ void sink_join_build(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES]) {
+__device__ __forceinline__ void sink_join_build(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES]) {
   const uint64_t pos = wave_alloc(P.out_count, 1);
   if (pos >= P.out_capacity) {
     atomicOr(P.out_overflow, 1u);
@@ -501,7 +524,7 @@ __device__ __forceinline__ void sink_join_build(const Plan& P, CInstr* ins, cons
   write_cols(P, ins, rows, pos);
 }
 
-__device__ __forceinline__ void sink_collect(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES]) {
+__device__ __forceinline__ void sink_collect(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES]) {
   const uint64_t pos = wave_alloc(P.out_count, 1);
   if (pos >= P.out_capacity) {
     atomicOr(P.out_overflow, 1u);
@@ -514,7 +537,7 @@ __device__ __forceinline__ void sink_collect(const Plan& P, CInstr* ins, const u
 }
 
 template <int SINK>
-__device__ __forceinline__ void do_sink(const Plan& P, CInstr* ins, const uint32_t (&rows)[MAX_STAGES], int64_t i,
+__device__ __forceinline__ void do_sink(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES], int64_t i,
                                         uint64_t* lds) {
   if constexpr (SINK == SINK_HASH_AGG) sink_hash_agg(P, ins, rows);
   else if constexpr (SINK == SINK_DIRECT_AGG) sink_direct_agg(P, ins, rows, lds);
@@ -539,7 +562,7 @@ __device__ __forceinline__ void do_sink(const Plan& P, CInstr* ins, const uint32
 template <int SINK>
 __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
   extern __shared__ uint64_t lds[];
-  CInstr* ins = (CInstr*)P.ins;
+  const Code ins{(CInstr*)P.ins, P.nins - 1};
   const int ncells = (SINK == SINK_DIRECT_AGG) ? P.ngroups * P.nacc : 0;
   if constexpr (SINK == SINK_DIRECT_AGG) {
     for (int c = threadIdx.x; c < ncells; c += blockDim.x) lds[c] = acc_init_bits(P.acc[c % P.nacc].kind);
@@ -571,7 +594,7 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
             while (join_next(T, w, h, pos, row)) {
               set_row(rows, st.stage, row);
               if (run_steps(P, ins, e + 1, P.nsteps, rows)) do_sink<SINK>(P, ins, rows, i, lds);
-              pos = (pos + 1) & T.mask;
+              pos = join_step(T, pos);
             }
           }
         }
@@ -586,11 +609,103 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
 }
 
 // ------------------------------------------------------ wave-queued pipeline
-// Queue heads and counts of levels 1..MAX_LEVELS, 7 bits each (0..127),
+// Queue heads and counts of levels 1..MAX_LEVELS, 10 bits each (0..1023),
 // packed into one wave-uniform 64-bit word so they stay in scalar registers.
-__device__ __forceinline__ uint32_t qget(uint64_t packed, int l) { return (uint32_t)(packed >> (7 * l)) & 127u; }
+__device__ __forceinline__ uint32_t qget(uint64_t packed, int l) { return (uint32_t)(packed >> (10 * (l - 1))) & 1023u; }
 __device__ __forceinline__ uint64_t qset(uint64_t packed, int l, uint32_t v) {
-  return (packed & ~(127ull << (7 * l))) | ((uint64_t)v << (7 * l));
+  return (packed & ~(1023ull << (10 * (l - 1)))) | ((uint64_t)v << (10 * (l - 1)));
+}
+
+// Streaming first probe for FAST_U rows per lane (rows next + u*64 + lane):
+// every row's key load, Bloom word and first slot are requested before any
+// is used, so a lane keeps FAST_U independent loads in flight per step.
+// Streaming key columns are read once: non-temporal loads keep them from
+// evicting the Bloom filter and table lines that later rows hit in L2.
+#ifndef ML_NT_KEYS
+#define ML_NT_KEYS 1
+#endif
+template <class T>
+__device__ __forceinline__ T stream_load(const T* p) {
+#if ML_NT_KEYS
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ uint64_t fast_key(uint16_t op, const void* col, uint32_t row) {
+  if (op == OP_LOAD_I32) return sx32((uint32_t)stream_load((const int32_t*)col + row));
+  if (op == OP_LOAD_U32) return (uint64_t)stream_load((const uint32_t*)col + row);
+  if (op == OP_LOAD_I64) return stream_load((const uint64_t*)col + row);
+  return stream_load((const uint8_t*)col + row) != 0;
+}
+
+__device__ __forceinline__ void fast_probe(const Plan& P, int64_t next, int64_t stop, int lane, uint32_t (&src)[FAST_U],
+                                           uint32_t (&brow)[FAST_U], bool (&hit)[FAST_U]) {
+  const DevJoinTable& T = P.tables[P.steps[0].table];
+  uint64_t w[FAST_U][MAX_KEYS + 1];
+  uint64_t h[FAST_U];
+#pragma unroll
+  for (int u = 0; u < FAST_U; u++) {
+    const int64_t i = next + u * 64 + lane;
+    hit[u] = i < stop;
+    src[u] = hit[u] ? (P.sel ? P.sel[i] : (uint32_t)i) : 0u;
+  }
+#pragma unroll
+  for (int u = 0; u < FAST_U; u++) {
+#pragma unroll
+    for (int k = 0; k < MAX_KEYS + 1; k++) w[u][k] = 0;
+    if (hit[u]) {
+      w[u][0] = fast_key(P.fast_op[0], P.fast_col[0], src[u]);
+      if (P.fast_nkeys > 1) w[u][1] = fast_key(P.fast_op[1], P.fast_col[1], src[u]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < FAST_U; u++) h[u] = key_hash(w[u], P.fast_nkeys);
+  if (T.bloom) {
+    uint64_t word[FAST_U], bits[FAST_U];
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++) {
+      const uint64_t g = hash_u64(h[u]);
+      bits[u] = bloom_bits(g);
+      word[u] = hit[u] ? T.bloom[bloom_word(T, h[u], g)] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++) hit[u] = hit[u] && (word[u] & bits[u]) == bits[u];
+  }
+  const int stride = 1 + T.nkw;
+  uint64_t pos[FAST_U], hdr[FAST_U], k0[FAST_U];
+#pragma unroll
+  for (int u = 0; u < FAST_U; u++) {
+    pos[u] = h[u] & T.mask;
+    hdr[u] = 0;
+    k0[u] = 0;
+    if (hit[u]) {
+      const uint64_t* sl = T.slots + pos[u] * stride;
+      hdr[u] = sl[0];
+      k0[u] = sl[1];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < FAST_U; u++) {
+    brow[u] = 0;
+    if (!hit[u]) continue;
+    bool found = false, done = false;
+    if (hdr[u] == 0) {
+      done = true;
+    } else if ((uint32_t)(hdr[u] >> 32) == (uint32_t)(h[u] >> 32) && k0[u] == w[u][0] &&
+               (T.nkw < 2 || T.slots[pos[u] * stride + 2] == w[u][1])) {
+      found = true;
+      done = true;
+      brow[u] = (uint32_t)hdr[u] - 1u;
+    }
+    if (!done) {  // the first slot held another key: continue the linear probe
+      uint64_t p = join_step(T, pos[u]);
+      int64_t row = 0;
+      found = join_next(T, w[u], h[u], p, row);
+      brow[u] = (uint32_t)row;
+    }
+    hit[u] = found;
+  }
 }
 
 // Row indices of the stages in `mask`, stored as u32 at base[k * stride]
@@ -616,11 +731,12 @@ __device__ __forceinline__ void load_stages(const uint32_t* base, uint64_t strid
 // the level's segment; survivors go to the next level's queue (or the sink
 // after the last segment). The deepest level holding a full wave runs first,
 // so queues never exceed QCAP entries, and rows reach the sink in source
-// order: every queued row precedes the rows of shallower levels.
+// order: every queued row precedes the rows of shallower levels. With a
+// streaming first probe (P.fast) level 0 takes FAST_U * 64 rows per round.
 template <int SINK>
 __global__ void __launch_bounds__(256) flow_kernel(Plan P) {
   extern __shared__ uint64_t lds[];
-  CInstr* ins = (CInstr*)P.ins;
+  const Code ins{(CInstr*)P.ins, P.nins - 1};
   const int ncells = (SINK == SINK_DIRECT_AGG) ? P.ngroups * P.nacc : 0;
   if constexpr (SINK == SINK_DIRECT_AGG) {
     for (int c = threadIdx.x; c < ncells; c += blockDim.x) lds[c] = acc_init_bits(P.acc[c % P.nacc].kind);
@@ -632,7 +748,7 @@ __global__ void __launch_bounds__(256) flow_kernel(Plan P) {
   const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
   int64_t next = P.begin + gw * P.chunk;
   const int64_t stop = min(P.end, next + P.chunk);
-  static_assert(QCAP == 128 && 7 * (MAX_LEVELS + 1) <= 64, "queue state packing");
+  static_assert(QCAP_FAST <= 1024 && 10 * MAX_LEVELS <= 64, "queue state packing");
   uint64_t qh = 0, qn = 0;
   const int last = P.nseg - 1;
   uint64_t nout = 0;
@@ -648,6 +764,37 @@ __global__ void __launch_bounds__(256) flow_kernel(Plan P) {
         if (lvl < 0 && l <= last && qget(qn, l) > 0) lvl = l;
     }
     if (lvl < 0) break;
+    lvl = uni(lvl);
+
+    if (lvl == 0 && P.fast) {
+      // FAST_U * 64 source rows; the hits go to level 1 in source order
+      uint32_t src[FAST_U], brow[FAST_U];
+      bool hit[FAST_U];
+      fast_probe(P, next, stop, lane, src, brow, hit);
+      next = min(stop, next + FAST_U * 64);
+      const int stage = P.steps[0].stage;
+      const uint32_t cap1 = (uint32_t)P.qcap[1];
+      uint32_t h = qget(qh, 1), c = qget(qn, 1);
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) {
+        const uint64_t m = __ballot(hit[u]);
+        if (hit[u]) {
+          uint32_t rows[MAX_STAGES];
+#pragma unroll
+          for (int s2 = 0; s2 < MAX_STAGES; s2++) rows[s2] = 0;
+          rows[0] = src[u];
+          set_row(rows, stage, brow[u]);
+          const uint32_t slot = (h + c + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))) & (cap1 - 1);
+          store_stages(q + P.qoff[1] + slot, cap1, P.live[1], rows);
+        }
+        c += (uint32_t)__popcll(m);
+      }
+      qn = qset(qn, 1, c);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      continue;
+    }
 
     uint32_t rows[MAX_STAGES];
 #pragma unroll
@@ -663,8 +810,9 @@ __global__ void __launch_bounds__(256) flow_kernel(Plan P) {
     } else {
       const uint32_t h = qget(qh, lvl), c = qget(qn, lvl);
       n = (int)min(64u, c);
-      if (lane < n) load_stages(q + P.qoff[lvl] + ((h + (uint32_t)lane) & (QCAP - 1)), QCAP, P.live[lvl], rows);
-      qh = qset(qh, lvl, (h + (uint32_t)n) & (QCAP - 1));
+      const uint32_t cap = (uint32_t)P.qcap[lvl];
+      if (lane < n) load_stages(q + P.qoff[lvl] + ((h + (uint32_t)lane) & (cap - 1)), cap, P.live[lvl], rows);
+      qh = qset(qh, lvl, (h + (uint32_t)n) & (cap - 1));
       qn = qset(qn, lvl, c - (uint32_t)n);
     }
     const int sb = lvl == 0 ? 0 : P.seg_end[lvl - 1];
@@ -691,8 +839,9 @@ __global__ void __launch_bounds__(256) flow_kernel(Plan P) {
       const int L = lvl + 1;
       const uint32_t h = qget(qh, L), c = qget(qn, L);
       if (alive) {
-        const uint32_t slot = (h + c + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))) & (QCAP - 1);
-        store_stages(q + P.qoff[L] + slot, QCAP, P.live[L], rows);
+        const uint32_t cap = (uint32_t)P.qcap[L];
+        const uint32_t slot = (h + c + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))) & (cap - 1);
+        store_stages(q + P.qoff[L] + slot, cap, P.live[L], rows);
       }
       qn = qset(qn, L, c + (uint32_t)__popcll(m));
     }
@@ -715,7 +864,7 @@ __global__ void __launch_bounds__(256) flow_kernel(Plan P) {
 // written densely at the region's exclusive offset (source order kept).
 template <int SINK>
 __global__ void __launch_bounds__(256) materialize_kernel(Plan P, const uint64_t* offsets) {
-  CInstr* ins = (CInstr*)P.ins;
+  const Code ins{(CInstr*)P.ins, P.nins - 1};
   const int64_t w = blockIdx.x;
   const uint32_t cnt = P.wave_counts[w];
   const uint64_t base = offsets[w];
@@ -819,12 +968,11 @@ __global__ void join_insert_kernel(DevJoinTable T, const uint64_t* keys, const u
           if (k < nkw) s[1 + k] = w[k];
         break;
       }
-      pos = (pos + 1) & T.mask;
+      pos = join_step(T, pos);
     }
     if (bloom) {
       const uint64_t g = hash_u64(h);
-      const uint64_t bits = (1ull << (g & 63)) | (1ull << ((g >> 6) & 63)) | (1ull << ((g >> 12) & 63));
-      atomicOr((unsigned long long*)&bloom[(g >> 18) & T.bloom_mask], (unsigned long long)bits);
+      atomicOr((unsigned long long*)&bloom[bloom_word(T, h, g)], (unsigned long long)bloom_bits(g));
     }
   }
 }
@@ -852,9 +1000,131 @@ __global__ void join_unique_kernel(DevJoinTable T, uint32_t* dup_flag) {
         atomicOr(dup_flag, 1u);
         break;
       }
-      q = (q + 1) & T.mask;
+      q = join_step(T, q);
     }
   }
+}
+
+// ------------------------------------------- partitioned join table build
+// Keys go to the partition of their home slot: (hash & mask) >> part_shift.
+// Workgroup b of the histogram and scatter passes owns build rows
+// [b*rows_per_block, (b+1)*rows_per_block); hist is partition-major
+// (hist[p * nblocks + b]), so its exclusive scan gives each (partition,
+// workgroup) pair its range of the partition-ordered staging arrays.
+__device__ __forceinline__ void load_key(const uint64_t* keys, int64_t r, int nkw, uint64_t (&w)[MAX_KEYS + 1]) {
+#pragma unroll
+  for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < nkw ? keys[r * nkw + k] : 0;
+}
+
+__global__ void __launch_bounds__(256) join_part_hist_kernel(DevJoinTable T, const uint64_t* keys,
+                                                             const uint8_t* key_null, int64_t n,
+                                                             int64_t rows_per_block, int nparts, uint32_t* hist) {
+  extern __shared__ uint32_t cnt[];
+  for (int p = threadIdx.x; p < nparts; p += blockDim.x) cnt[p] = 0;
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * rows_per_block, hi = min(n, lo + rows_per_block);
+  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+    if (key_null[r]) continue;  // NULL build keys are never inserted (join.rs:182-184)
+    uint64_t w[MAX_KEYS + 1];
+    load_key(keys, r, T.nkw, w);
+    const uint64_t h = key_hash(w, T.nkw);
+    atomicAdd(&cnt[(h & T.mask) >> T.part_shift], 1u);
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < nparts; p += blockDim.x) hist[(uint64_t)p * gridDim.x + blockIdx.x] = cnt[p];
+}
+
+__global__ void __launch_bounds__(256) join_part_scatter_kernel(DevJoinTable T, const uint64_t* keys,
+                                                                const uint8_t* key_null, int64_t n,
+                                                                int64_t rows_per_block, int nparts,
+                                                                const uint32_t* offsets, uint32_t* srow,
+                                                                uint64_t* skeys) {
+  extern __shared__ uint32_t cur[];
+  for (int p = threadIdx.x; p < nparts; p += blockDim.x) cur[p] = offsets[(uint64_t)p * gridDim.x + blockIdx.x];
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * rows_per_block, hi = min(n, lo + rows_per_block);
+  const int nkw = T.nkw;
+  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+    if (key_null[r]) continue;
+    uint64_t w[MAX_KEYS + 1];
+    load_key(keys, r, nkw, w);
+    const uint64_t h = key_hash(w, nkw);
+    const uint32_t at = atomicAdd(&cur[(h & T.mask) >> T.part_shift], 1u);
+    srow[at] = (uint32_t)r;
+    for (int k = 0; k < nkw; k++) skeys[(uint64_t)at * nkw + k] = w[k];
+  }
+}
+
+// One workgroup per partition: insert its keys into an LDS image of the
+// partition's slots and Bloom words, check key uniqueness, then write both
+// out with coalesced stores. flags bit 0: a duplicate key; bit 1: a partition
+// with too many keys to keep an empty slot (the caller rebuilds unpartitioned).
+__global__ void __launch_bounds__(256) join_part_build_kernel(DevJoinTable T, const uint32_t* offsets, int nblocks,
+                                                              const uint32_t* srow,
+                                                              const uint64_t* skeys, uint64_t* bloom,
+                                                              uint32_t* flags) {
+  extern __shared__ uint64_t lds[];
+  const int p = blockIdx.x;
+  const int nkw = T.nkw, stride = 1 + nkw;
+  const uint32_t S = (uint32_t)T.wrap + 1u;
+  const uint32_t W = bloom ? (uint32_t)T.bloom_mask + 1u : 0u;
+  uint64_t* sl = lds;
+  uint64_t* bw = lds + (uint64_t)S * stride;
+  const uint32_t lo = offsets[(uint64_t)p * nblocks];
+  const uint32_t hi = offsets[(uint64_t)(p + 1) * nblocks];  // the scan's last entry is the total
+  if (hi - lo >= S) {  // uniform: no empty slot would be left
+    if (threadIdx.x == 0) atomicOr(flags, 2u);
+    return;
+  }
+  for (uint32_t i = threadIdx.x; i < S * stride; i += blockDim.x) sl[i] = 0;
+  for (uint32_t i = threadIdx.x; i < W; i += blockDim.x) bw[i] = 0;
+  __syncthreads();
+  for (uint32_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+    uint64_t w[MAX_KEYS + 1];
+    load_key(skeys, e, nkw, w);
+    const uint64_t h = key_hash(w, nkw);
+    const uint64_t hdr = ((h >> 32) << 32) | (uint64_t)(srow[e] + 1u);
+    uint32_t pos = (uint32_t)(h & T.wrap);
+    for (;;) {  // ends: fewer keys than slots
+      if (atomicCAS((unsigned long long*)&sl[pos * stride], 0ull, (unsigned long long)hdr) == 0ull) {
+        for (int k = 0; k < nkw; k++) sl[pos * stride + 1 + k] = w[k];
+        break;
+      }
+      pos = (pos + 1) & (S - 1);
+    }
+    if (W) {
+      const uint64_t g = hash_u64(h);
+      atomicOr((unsigned long long*)&bw[(g >> 18) & T.bloom_mask], (unsigned long long)bloom_bits(g));
+    }
+  }
+  __syncthreads();
+  // a key is a duplicate if an equal key sits earlier on its probe path
+  bool dup = false;
+  for (uint32_t e = lo + threadIdx.x; e < hi && !dup; e += blockDim.x) {
+    uint64_t w[MAX_KEYS + 1];
+    load_key(skeys, e, nkw, w);
+    const uint64_t h = key_hash(w, nkw);
+    const uint64_t me = ((h >> 32) << 32) | (uint64_t)(srow[e] + 1u);
+    uint32_t pos = (uint32_t)(h & T.wrap);
+    for (;;) {
+      const uint64_t hd = sl[pos * stride];
+      if (hd == me) break;
+      if ((hd >> 32) == (h >> 32)) {
+        bool eq = true;
+        for (int k = 0; k < nkw; k++) eq = eq && sl[pos * stride + 1 + k] == w[k];
+        if (eq) {
+          dup = true;
+          break;
+        }
+      }
+      pos = (pos + 1) & (S - 1);
+    }
+  }
+  if (dup) atomicOr(flags, 1u);
+  uint64_t* out = T.slots + (uint64_t)p * S * stride;
+  for (uint32_t i = threadIdx.x; i < S * stride; i += blockDim.x) out[i] = sl[i];
+  if (W)
+    for (uint32_t i = threadIdx.x; i < W; i += blockDim.x) bloom[(uint64_t)p * W + i] = bw[i];
 }
 
 // --------------------------------------------------------------- agg tables
@@ -999,6 +1269,79 @@ __global__ void flag_scatter_kernel(const uint8_t* flags, int64_t n, const uint6
   }
 }
 
+// ---------------------------------------------- exclusive scan of u32 counts
+// Three passes: per-workgroup sums of SCAN_B elements, one workgroup scanning
+// those sums, then each workgroup scanning its elements from its base.
+constexpr int SCAN_T = 256;
+constexpr int SCAN_E = 16;
+constexpr int SCAN_B = SCAN_T * SCAN_E;
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* red, uint32_t* total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) red[wv] = incl;
+  __syncthreads();
+  uint32_t base = 0, all = 0;
+  for (int w = 0; w < SCAN_T / 64; w++) {
+    if (w < wv) base += red[w];
+    all += red[w];
+  }
+  *total = all;
+  __syncthreads();
+  return base + incl - v;
+}
+
+__global__ void __launch_bounds__(SCAN_T) scan_u32_reduce_kernel(const uint32_t* in, int64_t n, uint32_t* sums) {
+  __shared__ uint32_t red[SCAN_T / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_B + (int64_t)threadIdx.x * SCAN_E;
+  uint32_t c = 0;
+#pragma unroll
+  for (int e = 0; e < SCAN_E; e++) c += (base + e < n) ? in[base + e] : 0u;
+  uint32_t total;
+  block_excl_scan(c, red, &total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// in place: sums[i] <- exclusive prefix, sums[nb] <- total
+__global__ void __launch_bounds__(SCAN_T) scan_u32_top_kernel(uint32_t* sums, int64_t nb) {
+  __shared__ uint32_t red[SCAN_T / 64];
+  uint32_t carry = 0;
+  for (int64_t b0 = 0; b0 < nb; b0 += SCAN_T) {
+    const int64_t i = b0 + threadIdx.x;
+    const uint32_t v = i < nb ? sums[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(v, red, &total);
+    if (i < nb) sums[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) sums[nb] = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_T) scan_u32_down_kernel(const uint32_t* in, int64_t n, const uint32_t* sums,
+                                                               uint32_t* out) {
+  __shared__ uint32_t red[SCAN_T / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_B + (int64_t)threadIdx.x * SCAN_E;
+  uint32_t v[SCAN_E];
+  uint32_t c = 0;
+#pragma unroll
+  for (int e = 0; e < SCAN_E; e++) {
+    v[e] = (base + e < n) ? in[base + e] : 0u;
+    c += v[e];
+  }
+  uint32_t total;
+  uint32_t run = sums[blockIdx.x] + block_excl_scan(c, red, &total);
+#pragma unroll
+  for (int e = 0; e < SCAN_E; e++) {
+    if (base + e < n) out[base + e] = run;
+    run += v[e];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = sums[gridDim.x];
+}
+
 // ------------------------------------------------------------------- TopN
 // Sort words: lexicographic u64 words, then row index ascending.
 __device__ __forceinline__ bool sort_less(const uint64_t* a, const uint64_t* b, int nw) {
@@ -1009,7 +1352,7 @@ __device__ __forceinline__ bool sort_less(const uint64_t* a, const uint64_t* b, 
 }
 
 __global__ void sort_keys_kernel(Plan P, SortKeys K, int64_t n, uint64_t* words) {
-  CInstr* ins = (CInstr*)P.ins;
+  const Code ins{(CInstr*)P.ins, P.nins - 1};
   const int nw = K.nwords;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -1165,6 +1508,37 @@ void launch_join_insert(const DevJoinTable& T, const uint64_t* keys, const uint8
                         uint64_t* bloom, hipStream_t s) {
   if (n <= 0) return;
   join_insert_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(T, keys, key_null, n, bloom);
+}
+
+void launch_join_part_hist(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                           int nblocks, int64_t rows_per_block, int nparts, uint32_t* hist, hipStream_t s) {
+  join_part_hist_kernel<<<nblocks, 256, (size_t)nparts * 4, s>>>(T, keys, key_null, n, rows_per_block, nparts, hist);
+}
+
+void launch_join_part_scatter(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                              int nblocks, int64_t rows_per_block, int nparts, const uint32_t* offsets,
+                              uint32_t* srow, uint64_t* skeys, hipStream_t s) {
+  join_part_scatter_kernel<<<nblocks, 256, (size_t)nparts * 4, s>>>(T, keys, key_null, n, rows_per_block, nparts,
+                                                                    offsets, srow, skeys);
+}
+
+void launch_join_part_build(const DevJoinTable& T, int nparts, const uint32_t* offsets, int nblocks,
+                            const uint32_t* srow, const uint64_t* skeys, uint64_t* bloom, uint32_t* flags,
+                            size_t lds, hipStream_t s) {
+  join_part_build_kernel<<<nparts, 256, lds, s>>>(T, offsets, nblocks, srow, skeys, bloom, flags);
+}
+
+int64_t scan_u32_blocks(int64_t n) { return (n + SCAN_B - 1) / SCAN_B; }
+
+void launch_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, uint32_t* block_sums, hipStream_t s) {
+  const int64_t nb = scan_u32_blocks(n);
+  if (nb == 0) {
+    scan_u32_top_kernel<<<1, SCAN_T, 0, s>>>(out, 0);  // out[0] = 0
+    return;
+  }
+  scan_u32_reduce_kernel<<<(unsigned)nb, SCAN_T, 0, s>>>(in, n, block_sums);
+  scan_u32_top_kernel<<<1, SCAN_T, 0, s>>>(block_sums, nb);
+  scan_u32_down_kernel<<<(unsigned)nb, SCAN_T, 0, s>>>(in, n, block_sums, out);
 }
 
 void launch_join_unique(const DevJoinTable& T, uint32_t* dup_flag, hipStream_t s) {

@@ -20,6 +20,13 @@ constexpr int MAX_COLS = 32;      // materialized columns (collect / join payloa
 constexpr int MAX_LEVELS = 6;     // LDS queue levels of the wave-queued pipeline
 constexpr int QCAP = 128;         // entries per queue level: one wave's pop leaves < 64,
                                   // one push adds <= 64, so 127 is the most ever held
+#ifndef ML_FAST_U
+#define ML_FAST_U 4
+#endif
+constexpr int FAST_U = ML_FAST_U;  // source rows per lane in a streaming first probe
+// level 1 behind it holds < 64 + FAST_U * 64 entries
+constexpr int QCAP_FAST = FAST_U <= 3 ? 256 : FAST_U <= 7 ? 512 : 1024;
+static_assert(QCAP_FAST >= 64 * (FAST_U + 1), "level-1 queue too small");
 
 // MiniLake's hash constants (crates/minilake-hashtable/src/hash.rs:12-17).
 constexpr uint64_t HSEED = 0x243F6A8885A308D3ull;
@@ -69,6 +76,8 @@ enum Opcode : uint16_t {
   OP_COUNT
 };
 
+// The first 8 bytes are fetched as one little-endian word on the device:
+// keep this field order.
 struct Instr {
   uint16_t op;
   uint8_t depth;   // stack depth before the instruction
@@ -86,15 +95,34 @@ struct Prog {
 
 // ---------------------------------------------------------------- hash tables
 // Join table: open addressing, linear probing; slot = header word then key
-// words. header = (tag << 32) | (build_row + 1), 0 = empty.
+// words. header = (tag << 32) | (build_row + 1), 0 = empty. The slots are cut
+// into partitions of 2^part_shift slots (built one per workgroup in LDS); a
+// probe starts at slot hash & mask and wraps inside its partition. A table
+// that could not be partitioned has one partition (part_shift = log2 capacity).
+// The blocked Bloom filter has 2^bloom_shift words per partition.
 struct DevJoinTable {
   uint64_t* slots = nullptr;
   uint64_t mask = 0;       // capacity - 1
+  uint64_t wrap = 0;       // slots per partition - 1
   int32_t nkw = 1;         // key words
   int32_t unique = 1;      // every key distinct: stop at the first match
+  int32_t part_shift = 0;  // log2 slots per partition
+  int32_t bloom_shift = 0; // log2 Bloom words per partition
   const uint64_t* bloom = nullptr;  // blocked Bloom filter (one u64 per block) or null
-  uint64_t bloom_mask = 0;
+  uint64_t bloom_mask = 0; // Bloom words per partition - 1
 };
+
+// next slot of a linear probe (inside the partition)
+__host__ __device__ inline uint64_t join_step(const DevJoinTable& T, uint64_t pos) {
+  return (pos & ~T.wrap) | ((pos + 1) & T.wrap);
+}
+// Bloom word of a key with hash h (g = hash_u64(h) picks the word and bits)
+__host__ __device__ inline uint64_t bloom_word(const DevJoinTable& T, uint64_t h, uint64_t g) {
+  return (((h & T.mask) >> T.part_shift) << T.bloom_shift) | ((g >> 18) & T.bloom_mask);
+}
+__host__ __device__ inline uint64_t bloom_bits(uint64_t g) {
+  return (1ull << (g & 63)) | (1ull << ((g >> 6) & 63)) | (1ull << ((g >> 12) & 63));
+}
 
 // Aggregate table: header word per slot (0 empty, 1 being written,
 // otherwise (hash & ~3) | 2), key words AoS, accumulators SoA by slot.
@@ -153,6 +181,7 @@ struct Plan {
   int64_t begin;
   int64_t end;
   const Instr* ins;
+  int32_t nins;              // instructions in `ins`
 
   int32_t nsteps;
   int32_t expand_step;  // index of the one probe that may match many rows, or -1
@@ -208,14 +237,23 @@ struct Plan {
   // lanes idle further down the chain. Each wave owns `chunk` source rows.
   int32_t nseg;                          // segments (queue levels 1 .. nseg-1)
   int32_t qwords;                        // u32 words of one wave's queues
-  int16_t seg_end[MAX_LEVELS + 1];       // segment s = steps [seg_end[s-1], seg_end[s])
-  int16_t qoff[MAX_LEVELS + 1];          // level l queue: u32 offset in the wave's region
-  uint8_t live[MAX_LEVELS + 1];          // level l carries these stages' row indices (bit s = stage s)
-  uint8_t tup_mask;                      // ROWS: stages recorded per row
+  // (32-bit fields: they are read with scalar loads, which have no sub-dword form)
+  int32_t seg_end[MAX_LEVELS + 1];       // segment s = steps [seg_end[s-1], seg_end[s])
+  int32_t qoff[MAX_LEVELS + 1];          // level l queue: u32 offset in the wave's region
+  int32_t qcap[MAX_LEVELS + 1];          // level l queue: entries (power of two)
+  uint32_t live[MAX_LEVELS + 1];         // level l carries these stages' row indices (bit s = stage s)
+  uint32_t tup_mask;                     // ROWS: stages recorded per row
   int64_t chunk;                         // source rows per wave
   uint32_t* tuples;                      // ROWS: [k][staging] row index of the k-th recorded stage
   uint64_t staging;                      // waves * chunk
   uint32_t* wave_counts;                 // ROWS / COUNT_PAIRS: rows produced per wave
+  // streaming first probe: step 0 probes a unique table with plain stage-0
+  // column keys; level 0 then handles FAST_U rows per lane with independent
+  // key, Bloom and slot loads in flight
+  int32_t fast;
+  int32_t fast_nkeys;
+  uint16_t fast_op[2];
+  const void* fast_col[2];
 };
 
 // Sortable key words for TopN.

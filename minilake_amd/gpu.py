@@ -30,7 +30,9 @@ from . import abi
 from .table import NP_DTYPE, HostColumn, HostTable, make_dict
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libminilake_gpu.so")
+# MINILAKE_GPU_LIB selects another build of the library (e.g. one compiled with
+# different tuning constants by minilake_amd.build.build_gpu_library)
+LIB_PATH = os.environ.get("MINILAKE_GPU_LIB") or os.path.join(_HERE, "libminilake_gpu.so")
 _lib = None
 
 # Every symbol include/minilake_gpu.h declares.

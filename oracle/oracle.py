@@ -206,6 +206,10 @@ class OracleEngine:
                                           C.byref(out)))
         return _take_table(out.value)
 
+    def count(self, src, ops) -> int:
+        """Rows reaching the end of the pipeline (the collect sink's row count)."""
+        return self.collect(src, ops).to_host().num_rows
+
     def expr_type(self, e, src):
         keep = abi.Keep()
         b = host_batch(self._host(src), keep)

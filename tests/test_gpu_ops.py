@@ -207,3 +207,82 @@ def test_collect_through_probe(gpu_engine, oracle_engine):
     want = oracle_engine.collect(probe, ops_o).to_host()
     got = gpu_engine.collect(probe, ops_g).to_host()
     assert_tables_equal(got, want, rtol=0.0)
+
+
+@pytest.mark.parametrize("keys", ["i64", "i32", "i64+i32"])
+def test_streaming_first_probe(gpu_engine, oracle_engine, keys):
+    """A probe as the first pipeline step runs several rows per lane with the
+    Bloom filter (>= 64K build rows) and linear-probe collisions; results must
+    equal the oracle's for one- and two-column keys of both widths."""
+    rng = np.random.default_rng(11)
+    nb, npr = 120_000, 300_000
+    bk64 = rng.permutation(400_000)[:nb].astype(np.int64) * 7 - 1_000_000
+    bk32 = rng.integers(0, 5, nb).astype(np.int32)
+    build = HostTable([HostColumn(INT64, bk64), HostColumn(INT32, bk32), HostColumn(FLOAT64, rng.random(nb))])
+    pick = rng.integers(0, nb, npr)
+    miss = rng.random(npr) < 0.6
+    p64 = np.where(miss, rng.integers(-(1 << 40), 1 << 40, npr), bk64[pick]).astype(np.int64)
+    p32 = np.where(rng.random(npr) < 0.2, rng.integers(0, 5, npr), bk32[pick]).astype(np.int32)
+    probe = HostTable([HostColumn(INT64, p64), HostColumn(INT32, p32), HostColumn(INT64, np.arange(npr, dtype=np.int64))])
+    if keys == "i64":
+        bkeys, btypes, pkeys = [col(0)], [INT64], [col(0)]
+    elif keys == "i32":
+        bkeys, btypes, pkeys = [col(1)], [INT32], [col(1)]
+        build = HostTable([HostColumn(INT64, bk64), HostColumn(INT32, np.arange(nb, dtype=np.int32) * 3),
+                           HostColumn(FLOAT64, rng.random(nb))])
+        probe = HostTable([HostColumn(INT64, p64), HostColumn(INT32, rng.integers(-1000, 3 * nb + 1000, npr).astype(np.int32)),
+                           HostColumn(INT64, np.arange(npr, dtype=np.int64))])
+    else:
+        # (i64, i32) pairs are unique in the build side: the i64 part already is
+        bkeys, btypes, pkeys = [col(0), col(1)], [INT64, INT32], [col(0), col(1)]
+    jo = oracle_engine.join_build(build, [], bkeys, btypes)
+    jg = gpu_engine.join_build(build, [], bkeys, btypes)
+    ops_o = [abi.probe_op(jo, pkeys), abi.filter_op(col(2).lt(lit(0.9)) | col(5).gt(lit(0.5)))]
+    ops_g = [abi.probe_op(jg, pkeys), abi.filter_op(col(2).lt(lit(0.9)) | col(5).gt(lit(0.5)))]
+    want = oracle_engine.collect(probe, ops_o).to_host()
+    got = gpu_engine.collect(probe, ops_g).to_host()
+    assert want.num_rows > 1000
+    assert_tables_equal(got, want, rtol=0.0)
+    assert gpu_engine.count(probe, ops_g) == want.num_rows
+
+
+def test_join_build_heavy_duplicates_falls_back_unpartitioned(gpu_engine, oracle_engine):
+    """Build keys with thousands of copies overflow one LDS partition; the
+    table is then rebuilt unpartitioned and probes still see every copy."""
+    nb = 30_000
+    bk = (np.arange(nb) % 3).astype(np.int64) * 1_000_003
+    build = HostTable([HostColumn(INT64, bk), HostColumn(INT64, np.arange(nb, dtype=np.int64))])
+    probe = HostTable([HostColumn(INT64, np.array([0, 1_000_003, 7, 2_000_006, 0], dtype=np.int64))])
+    jo = oracle_engine.join_build(build, [], [col(0)], [INT64])
+    jg = gpu_engine.join_build(build, [], [col(0)], [INT64])
+    assert not jg.unique()
+    want = oracle_engine.count(probe, [abi.probe_op(jo, [col(0)])])
+    assert want == 4 * 10_000
+    assert gpu_engine.count(probe, [abi.probe_op(jg, [col(0)])]) == want
+    agg_o = oracle_engine.hashagg(probe, [abi.probe_op(jo, [col(0)])], [col(0)], [INT64],
+                                  [Agg(AGG_SUM, col(2), INT64), Agg(AGG_COUNT_STAR, None, INT64)])
+    agg_g = gpu_engine.hashagg(probe, [abi.probe_op(jg, [col(0)])], [col(0)], [INT64],
+                               [Agg(AGG_SUM, col(2), INT64), Agg(AGG_COUNT_STAR, None, INT64)])
+    assert_tables_equal(agg_g.to_host(), agg_o.to_host(), rtol=0.0)
+
+
+@pytest.mark.parametrize("nb", [1, 100, 70_000, 400_000])
+def test_join_build_partitioned_sizes(gpu_engine, oracle_engine, nb):
+    """Partitioned builds from one partition up to hundreds, with and without
+    the Bloom filter (on from 64K rows), unique and with a few duplicates."""
+    rng = np.random.default_rng(nb)
+    bk = rng.integers(-(1 << 50), 1 << 50, nb).astype(np.int64)
+    if nb > 10:
+        bk[: nb // 50] = bk[nb // 50: 2 * (nb // 50)]  # 2% duplicated keys
+    build = HostTable([HostColumn(INT64, bk), HostColumn(INT32, rng.integers(0, 9, nb).astype(np.int32))])
+    npr = 50_000
+    pk = np.where(rng.random(npr) < 0.5, bk[rng.integers(0, nb, npr)], rng.integers(-(1 << 50), 1 << 50, npr))
+    probe = HostTable([HostColumn(INT64, pk.astype(np.int64))])
+    jo = oracle_engine.join_build(build, [], [col(0)], [INT64])
+    jg = gpu_engine.join_build(build, [], [col(0)], [INT64])
+    assert jg.unique() == (nb <= 10)
+    ops_o, ops_g = [abi.probe_op(jo, [col(0)])], [abi.probe_op(jg, [col(0)])]
+    assert gpu_engine.count(probe, ops_g) == oracle_engine.count(probe, ops_o)
+    agg_o = oracle_engine.hashagg(probe, ops_o, [col(2)], [INT32], [Agg(AGG_COUNT_STAR, None, INT64)])
+    agg_g = gpu_engine.hashagg(probe, ops_g, [col(2)], [INT32], [Agg(AGG_COUNT_STAR, None, INT64)])
+    assert_tables_equal(agg_g.to_host(), agg_o.to_host(), rtol=0.0)
