@@ -519,6 +519,48 @@ Compiler::Val Compiler::gen_binary(const Node& n) {
   }
 }
 
+// Peephole over the program starting at `first`: LOAD (no validity) CONST
+// (valid) CMP becomes one OP_CMPC, and LOAD_U32 LOOKUP_U8 one OP_LOOKUPC.
+// Each interpreted instruction costs a dispatch, so filters over a column and
+// a constant run in one.
+void Compiler::fuse(int32_t first) {
+  std::vector<Instr> out;
+  const size_t n = ins_.size();
+  for (size_t i = (size_t)first; i < n; i++) {
+    const Instr& a = ins_[i];
+    if (a.op < OP_CONST && a.imm2 == 0 && i + 2 < n) {
+      const Instr& b = ins_[i + 1];
+      const Instr& c = ins_[i + 2];
+      const bool icmp = c.op >= OP_EQ_I && c.op <= OP_GE_I, fcmp = c.op >= OP_EQ_F && c.op <= OP_GE_F;
+      if (b.op == OP_CONST && (b.flags & 1u) && b.depth == a.depth + 1 && c.depth == a.depth + 2 && (icmp || fcmp) &&
+          !(fcmp && a.op != OP_LOAD_I64)) {
+        Instr f = a;
+        f.op = OP_CMPC;
+        f.imm2 = b.imm;
+        f.flags = ((uint32_t)(icmp ? c.op - OP_EQ_I : c.op - OP_EQ_F) << 8) | (fcmp ? 0x800u : 0u) |
+                  ((uint32_t)a.op << 13);
+        out.push_back(f);
+        i += 2;
+        continue;
+      }
+    }
+    if (a.op == OP_LOAD_U32 && a.imm2 == 0 && i + 1 < n) {
+      const Instr& b = ins_[i + 1];
+      if (b.op == OP_LOOKUP_U8 && b.depth == a.depth + 1) {
+        Instr f = a;
+        f.op = OP_LOOKUPC;
+        f.imm2 = b.imm;
+        out.push_back(f);
+        i += 1;
+        continue;
+      }
+    }
+    out.push_back(a);
+  }
+  ins_.resize((size_t)first);
+  ins_.insert(ins_.end(), out.begin(), out.end());
+}
+
 Prog Compiler::compile(const NodePtr& n, int want, int* out_type) {
   depth_ = 0;
   max_depth_ = 0;
@@ -537,6 +579,7 @@ Prog Compiler::compile(const NodePtr& n, int want, int* out_type) {
   }
   if (max_depth_ > MAX_DEPTH) fail(ML_ERR_UNSUPPORTED, "expression needs more than 6 stack slots");
   if (depth_ != 1) fail(ML_ERR_INTERNAL, "expression compiled to an unbalanced program");
+  fuse(p.first);
   p.count = (int16_t)((int32_t)ins_.size() - p.first);
   p.nullable = nullable_ ? 1 : 0;
   if (out_type) *out_type = v.type;

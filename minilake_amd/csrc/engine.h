@@ -224,6 +224,7 @@ class Compiler {
   void push(uint16_t op, uint64_t imm = 0, uint64_t imm2 = 0, uint8_t stage = 0, uint32_t flags = 0);
   void unary(uint16_t op, uint64_t imm = 0);
   void binary(uint16_t op);
+  void fuse(int32_t first);
   const uint8_t* dict_predicate(const HostDict& d, int op, const std::string& lit);
 
   ml_ctx* ctx_;

@@ -248,7 +248,7 @@ static uint32_t prog_stages(const std::vector<Instr>& ins, const Prog& p) {
   uint32_t m = 0;
   for (int i = 0; i < p.count; i++) {
     const Instr& in = ins[(size_t)p.first + i];
-    if (in.op < OP_CONST) m |= 1u << in.stage;
+    if (reads_column(in.op)) m |= 1u << in.stage;
   }
   return m;
 }
@@ -313,6 +313,30 @@ static bool plan_flow(PipelineBuild& B, int sink, uint32_t sink_stages, size_t c
       P.fast_nkeys = P.steps[0].nkeys;
     }
   }
+  // otherwise the stage-0 columns segment 0 loads (at most two) are loaded
+  // ahead for all rows of a level-0 round
+  P.pf_n = 0;
+  if (!P.fast) {
+    const int end0 = cuts.empty() ? P.nsteps : cuts[0];
+    for (int st = 0; st < end0 && P.pf_n < 2; st++) {
+      const Step& S = P.steps[st];
+      const int np = S.kind == STEP_FILTER ? 1 : S.nkeys;
+      for (int k = 0; k < np && P.pf_n < 2; k++) {
+        const Prog& pg = S.kind == STEP_FILTER ? S.pred : S.keys[k];
+        for (int i = 0; i < pg.count && P.pf_n < 2; i++) {
+          const Instr& in = B.ins[(size_t)pg.first + i];
+          if (!reads_column(in.op) || in.stage != 0) continue;
+          bool seen = false;
+          for (int j = 0; j < P.pf_n; j++) seen = seen || P.pf_col[j] == (const void*)in.imm;
+          if (!seen) {
+            P.pf_col[P.pf_n] = (const void*)in.imm;
+            P.pf_op[P.pf_n] = (uint16_t)column_load_op(in.op, in.flags);
+            P.pf_n++;
+          }
+        }
+      }
+    }
+  }
   uint32_t used_from[MAX_STEPS + 1];
   used_from[P.nsteps] = sink_stages;
   for (int s = P.nsteps - 1; s >= 0; s--) used_from[s] = used_from[s + 1] | step_stages(B.ins, P.steps[s]);
@@ -326,7 +350,7 @@ static bool plan_flow(PipelineBuild& B, int sink, uint32_t sink_stages, size_t c
       const uint32_t live = assigned & used_from[prev];
       P.live[l] = live;
       P.qoff[l] = qw;
-      P.qcap[l] = (l == 1 && P.fast) ? QCAP_FAST : QCAP;
+      P.qcap[l] = l == 1 ? QCAP_FAST : QCAP;
       qw += __builtin_popcount(live) * P.qcap[l];
     }
     for (int s = prev; s < end; s++)

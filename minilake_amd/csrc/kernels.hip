@@ -19,6 +19,19 @@ namespace mlg {
 typedef __attribute__((address_space(4))) const Instr CInstr;
 typedef __attribute__((address_space(4))) const uint64_t CU64;
 
+// Device buffers are global memory. Pointers that reach a kernel through an
+// instruction word or a void* field would otherwise be generic (flat)
+// accesses, which also count against the scalar-load counter: every wait for
+// an instruction fetch would then wait for all outstanding loads as well.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gp(const void* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gpw(void* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 // A value every active lane holds; readfirstlane lets the compiler keep it
 // (and whatever it indexes in constant memory) in scalar registers.
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -103,8 +116,18 @@ __device__ __forceinline__ void set_slot(uint64_t (&r)[MAX_DEPTH], int d, uint64
   }
 }
 
+// Values of up to two stage-0 columns already loaded for the current row
+// (the wave-queued pipeline loads them for several rows at once); a LOAD of
+// one of these columns takes the value from here instead of memory.
+struct Pf {
+  const void* c0 = nullptr;
+  const void* c1 = nullptr;
+  uint64_t v0 = 0, v1 = 0;
+};
+
 // Evaluate a program for one row; returns validity, value in `val`.
-__device__ __forceinline__ bool eval(const Code& ins, Prog p, const uint32_t (&rows)[MAX_STAGES], uint64_t& val) {
+__device__ __forceinline__ bool eval(const Code& ins, Prog p, const uint32_t (&rows)[MAX_STAGES], uint64_t& val,
+                                     const Pf& pf = Pf{}) {
   uint64_t r[MAX_DEPTH];
 #pragma unroll
   for (int i = 0; i < MAX_DEPTH; i++) r[i] = 0;
@@ -123,19 +146,39 @@ __device__ __forceinline__ bool eval(const Code& ins, Prog p, const uint32_t (&r
     uint64_t res;
     uint32_t vres = 1;
     int dst;
-    if (op <= OP_CONST) {  // push
+    if (op <= OP_LOOKUPC) {  // push
       dst = d;
       if (op == OP_CONST) {
         res = imm;
         vres = flags & 1u;
       } else {
         const uint64_t row = row_of(rows, stage);
-        if (op == OP_LOAD_I32) res = sx32((uint32_t)((const int32_t*)imm)[row]);
-        else if (op == OP_LOAD_U32) res = (uint64_t)((const uint32_t*)imm)[row];
-        else if (op == OP_LOAD_I64) res = ((const uint64_t*)imm)[row];
-        else res = ((const uint8_t*)imm)[row] != 0;
-        const uint64_t vp = in.imm2;
-        if (nullable && vp) vres = ((const uint8_t*)vp)[row] != 0;
+        const uint32_t lop = column_load_op(op, flags);
+        uint64_t x;
+        if (stage == 0 && pf.c0 && (const void*)imm == pf.c0) x = pf.v0;
+        else if (stage == 0 && pf.c1 && (const void*)imm == pf.c1) x = pf.v1;
+        else if (lop == OP_LOAD_I32) x = sx32((uint32_t)gp<int32_t>((const void*)imm)[row]);
+        else if (lop == OP_LOAD_U32) x = (uint64_t)gp<uint32_t>((const void*)imm)[row];
+        else if (lop == OP_LOAD_I64) x = gp<uint64_t>((const void*)imm)[row];
+        else x = gp<uint8_t>((const void*)imm)[row] != 0;
+        const uint64_t i2 = in.imm2;
+        if (op == OP_CMPC) {
+          const uint32_t c = (flags >> 8) & 7u;
+          bool r;
+          if (flags & 0x800u) {
+            const double a = u2f(x), b = u2f(i2);
+            r = c == 0 ? a == b : c == 1 ? a != b : c == 2 ? a < b : c == 3 ? a <= b : c == 4 ? a > b : a >= b;
+          } else {
+            const int64_t a = (int64_t)x, b = (int64_t)i2;
+            r = c == 0 ? a == b : c == 1 ? a != b : c == 2 ? a < b : c == 3 ? a <= b : c == 4 ? a > b : a >= b;
+          }
+          res = r;
+        } else if (op == OP_LOOKUPC) {
+          res = gp<uint8_t>((const void*)i2)[x];
+        } else {
+          res = x;
+          if (nullable && i2) vres = gp<uint8_t>((const void*)i2)[row] != 0;
+        }
       }
     } else if (op < OP_ADD_I32) {  // unary on the top slot
       dst = d - 1;
@@ -150,8 +193,8 @@ __device__ __forceinline__ bool eval(const Code& ins, Prog p, const uint32_t (&r
         case OP_F2I64: res = (uint64_t)sat_i64(u2f(a)); break;
         case OP_F2I32: res = sx32((uint32_t)sat_i32(u2f(a))); break;
         case OP_I64_TO_I32: res = sx32((uint32_t)a); break;
-        case OP_LOOKUP_U8: res = ((const uint8_t*)imm)[a]; break;
-        case OP_LOOKUP_U64: res = ((const uint64_t*)imm)[a]; break;
+        case OP_LOOKUP_U8: res = gp<uint8_t>((const void*)imm)[a]; break;
+        case OP_LOOKUP_U64: res = gp<uint64_t>((const void*)imm)[a]; break;
         case OP_KEY_F64:
           res = (a == 0x8000000000000000ull) ? 0ull : ((u2f(a) != u2f(a)) ? 0x7FF8000000000000ull : a);
           break;
@@ -267,14 +310,15 @@ __device__ __forceinline__ bool join_next(const DevJoinTable& T, const uint64_t 
 }
 
 __device__ __forceinline__ bool probe_keys(const Plan& P, const Code& ins, const Step& st,
-                                           const uint32_t (&rows)[MAX_STAGES], uint64_t (&w)[MAX_KEYS + 1]) {
+                                           const uint32_t (&rows)[MAX_STAGES], uint64_t (&w)[MAX_KEYS + 1],
+                                           const Pf& pf = Pf{}) {
   bool ok = true;
 #pragma unroll
   for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = 0;
 #pragma unroll 1
   for (int k = 0; k < st.nkeys; k++) {
     uint64_t v;
-    ok = eval(ins, st.keys[k], rows, v) && ok;
+    ok = eval(ins, st.keys[k], rows, v, pf) && ok;
     set_word(w, k, v);
   }
   return ok;
@@ -282,7 +326,7 @@ __device__ __forceinline__ bool probe_keys(const Plan& P, const Code& ins, const
 
 // Filters and unique probes in [from, to); false once the row is dropped.
 __device__ __forceinline__ bool run_steps(const Plan& P, const Code& ins, int from, int to,
-                                          uint32_t (&rows)[MAX_STAGES]) {
+                                          uint32_t (&rows)[MAX_STAGES], const Pf& pf = Pf{}) {
   bool alive = true;
   for (int s_ = from; s_ < to; s_++) {
     if (!alive) break;
@@ -290,11 +334,11 @@ __device__ __forceinline__ bool run_steps(const Plan& P, const Code& ins, int fr
     const Step& st = P.steps[s];
     if (st.kind == STEP_FILTER) {
       uint64_t v;
-      const bool ok = eval(ins, st.pred, rows, v);
+      const bool ok = eval(ins, st.pred, rows, v, pf);
       alive = ok && v != 0;
     } else {
       uint64_t w[MAX_KEYS + 1];
-      if (!probe_keys(P, ins, st, rows, w)) {  // NULL keys never match (join.rs:284-286)
+      if (!probe_keys(P, ins, st, rows, w, pf)) {  // NULL keys never match (join.rs:284-286)
         alive = false;
         continue;
       }
@@ -350,11 +394,11 @@ __device__ __forceinline__ void atomic_max_f64(double* p, double v) {
 // accumulate into a global (hash table) state slot
 __device__ __forceinline__ void acc_apply_global(int kind, void* data, int64_t slot, uint64_t v) {
   switch (kind) {
-    case ACC_SUM_F64: unsafeAtomicAdd((double*)data + slot, u2f(v)); break;
-    case ACC_SUM_I64: atomicAdd((unsigned long long*)data + slot, (unsigned long long)v); break;
-    case ACC_COUNT: atomicAdd((unsigned long long*)data + slot, 1ull); break;
-    case ACC_MIN_I64: atomicMin((long long*)data + slot, (long long)v); break;
-    case ACC_MAX_I64: atomicMax((long long*)data + slot, (long long)v); break;
+    case ACC_SUM_F64: unsafeAtomicAdd((double*)gpw<double>(data) + slot, u2f(v)); break;
+    case ACC_SUM_I64: atomicAdd((unsigned long long*)gpw<uint64_t>(data) + slot, (unsigned long long)v); break;
+    case ACC_COUNT: atomicAdd((unsigned long long*)gpw<uint64_t>(data) + slot, 1ull); break;
+    case ACC_MIN_I64: atomicMin((long long*)gpw<uint64_t>(data) + slot, (long long)v); break;
+    case ACC_MAX_I64: atomicMax((long long*)gpw<uint64_t>(data) + slot, (long long)v); break;
     case ACC_MIN_F64: atomic_min_f64((double*)data + slot, u2f(v)); break;
     case ACC_MAX_F64: atomic_max_f64((double*)data + slot, u2f(v)); break;
     default: break;
@@ -497,10 +541,10 @@ __device__ __forceinline__ void write_cols(const Plan& P, const Code& ins, const
     uint64_t v;
     const bool ok = eval(ins, P.cols[c], rows, v);
     const int w = P.col_width[c];
-    if (w == 8) ((uint64_t*)P.col_out[c])[pos] = v;
-    else if (w == 4) ((uint32_t*)P.col_out[c])[pos] = (uint32_t)v;
-    else ((uint8_t*)P.col_out[c])[pos] = (uint8_t)v;
-    if (P.col_valid[c]) P.col_valid[c][pos] = ok;
+    if (w == 8) gpw<uint64_t>(P.col_out[c])[pos] = v;
+    else if (w == 4) gpw<uint32_t>(P.col_out[c])[pos] = (uint32_t)v;
+    else gpw<uint8_t>(P.col_out[c])[pos] = (uint8_t)v;
+    if (P.col_valid[c]) gpw<uint8_t>(P.col_valid[c])[pos] = ok;
   }
 }
 
@@ -625,86 +669,130 @@ __device__ __forceinline__ uint64_t qset(uint64_t packed, int l, uint32_t v) {
 #define ML_NT_KEYS 1
 #endif
 template <class T>
-__device__ __forceinline__ T stream_load(const T* p) {
+__device__ __forceinline__ T stream_load(const void* col, uint32_t row) {
 #if ML_NT_KEYS
-  return __builtin_nontemporal_load(p);
+  return __builtin_nontemporal_load(gp<T>(col) + row);
 #else
-  return *p;
+  return gp<T>(col)[row];
 #endif
 }
 __device__ __forceinline__ uint64_t fast_key(uint16_t op, const void* col, uint32_t row) {
-  if (op == OP_LOAD_I32) return sx32((uint32_t)stream_load((const int32_t*)col + row));
-  if (op == OP_LOAD_U32) return (uint64_t)stream_load((const uint32_t*)col + row);
-  if (op == OP_LOAD_I64) return stream_load((const uint64_t*)col + row);
-  return stream_load((const uint8_t*)col + row) != 0;
+  if (op == OP_LOAD_I32) return sx32((uint32_t)stream_load<int32_t>(col, row));
+  if (op == OP_LOAD_U32) return (uint64_t)stream_load<uint32_t>(col, row);
+  if (op == OP_LOAD_I64) return stream_load<uint64_t>(col, row);
+  return stream_load<uint8_t>(col, row) != 0;
 }
 
-__device__ __forceinline__ void fast_probe(const Plan& P, int64_t next, int64_t stop, int lane, uint32_t (&src)[FAST_U],
-                                           uint32_t (&brow)[FAST_U], bool (&hit)[FAST_U]) {
-  const DevJoinTable& T = P.tables[P.steps[0].table];
-  uint64_t w[FAST_U][MAX_KEYS + 1];
-  uint64_t h[FAST_U];
+// FAST_U loads of one column, all issued before any is used: the type
+// dispatch is one uniform branch around the unrolled loads (a branch between
+// loads would make the compiler wait for the earlier ones at the join).
+__device__ __forceinline__ void load_rows(uint16_t op, const void* col, const uint32_t (&row)[FAST_U],
+                                          uint64_t (&out)[FAST_U]) {
+  if (op == OP_LOAD_I64) {
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++) out[u] = stream_load<uint64_t>(col, row[u]);
+  } else if (op == OP_LOAD_I32) {
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++) out[u] = sx32((uint32_t)stream_load<int32_t>(col, row[u]));
+  } else if (op == OP_LOAD_U32) {
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++) out[u] = (uint64_t)stream_load<uint32_t>(col, row[u]);
+  } else {
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++) out[u] = stream_load<uint8_t>(col, row[u]) != 0;
+  }
+}
+
+// Source rows next + u*64 + lane of a level-0 round; rows past `stop` are
+// inactive and read the round's first row instead, so every load below is
+// unconditional and in bounds.
+__device__ __forceinline__ void round_rows(const Plan& P, int64_t next, int64_t stop, int lane,
+                                           uint32_t (&src)[FAST_U], bool (&act)[FAST_U]) {
+  int64_t idx[FAST_U];
 #pragma unroll
   for (int u = 0; u < FAST_U; u++) {
     const int64_t i = next + u * 64 + lane;
-    hit[u] = i < stop;
-    src[u] = hit[u] ? (P.sel ? P.sel[i] : (uint32_t)i) : 0u;
+    act[u] = i < stop;
+    idx[u] = act[u] ? i : next;
   }
+  if (P.sel) {
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++) src[u] = gp<uint32_t>(P.sel)[idx[u]];
+  } else {
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++) src[u] = (uint32_t)idx[u];
+  }
+}
+
+// Streaming first probe for FAST_U rows per lane: every row's key load, Bloom
+// word and slot are requested before any is used, so a lane keeps FAST_U
+// independent loads in flight per step.
+__device__ __forceinline__ void fast_probe(const Plan& P, int64_t next, int64_t stop, int lane, uint32_t (&src)[FAST_U],
+                                           uint32_t (&brow)[FAST_U], bool (&hit)[FAST_U]) {
+  const DevJoinTable& T = P.tables[P.steps[0].table];
+  round_rows(P, next, stop, lane, src, hit);
+  uint64_t k0[FAST_U], k1[FAST_U], h[FAST_U];
+  load_rows(P.fast_op[0], P.fast_col[0], src, k0);
+  const bool two = P.fast_nkeys > 1;
+  if (two) load_rows(P.fast_op[1], P.fast_col[1], src, k1);
 #pragma unroll
   for (int u = 0; u < FAST_U; u++) {
-#pragma unroll
-    for (int k = 0; k < MAX_KEYS + 1; k++) w[u][k] = 0;
-    if (hit[u]) {
-      w[u][0] = fast_key(P.fast_op[0], P.fast_col[0], src[u]);
-      if (P.fast_nkeys > 1) w[u][1] = fast_key(P.fast_op[1], P.fast_col[1], src[u]);
-    }
+    uint64_t hh = hash_combine(HSEED, hash_u64(k0[u]));
+    if (two) hh = hash_combine(hh, hash_u64(k1[u]));
+    h[u] = hh;
   }
-#pragma unroll
-  for (int u = 0; u < FAST_U; u++) h[u] = key_hash(w[u], P.fast_nkeys);
   if (T.bloom) {
     uint64_t word[FAST_U], bits[FAST_U];
 #pragma unroll
     for (int u = 0; u < FAST_U; u++) {
       const uint64_t g = hash_u64(h[u]);
       bits[u] = bloom_bits(g);
-      word[u] = hit[u] ? T.bloom[bloom_word(T, h[u], g)] : 0;
+      word[u] = gp<uint64_t>(T.bloom)[hit[u] ? bloom_word(T, h[u], g) : 0];
     }
 #pragma unroll
     for (int u = 0; u < FAST_U; u++) hit[u] = hit[u] && (word[u] & bits[u]) == bits[u];
   }
+  // Linear probe of all FAST_U rows in lockstep: each step loads the slot of
+  // every row (slot 0 for resolved ones) before testing any of them.
   const int stride = 1 + T.nkw;
-  uint64_t pos[FAST_U], hdr[FAST_U], k0[FAST_U];
+  const auto slots = gp<uint64_t>(T.slots);
+  uint64_t pos[FAST_U];
+  bool pend[FAST_U];
 #pragma unroll
   for (int u = 0; u < FAST_U; u++) {
     pos[u] = h[u] & T.mask;
-    hdr[u] = 0;
-    k0[u] = 0;
-    if (hit[u]) {
-      const uint64_t* sl = T.slots + pos[u] * stride;
-      hdr[u] = sl[0];
-      k0[u] = sl[1];
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < FAST_U; u++) {
+    pend[u] = hit[u];
+    hit[u] = false;
     brow[u] = 0;
-    if (!hit[u]) continue;
-    bool found = false, done = false;
-    if (hdr[u] == 0) {
-      done = true;
-    } else if ((uint32_t)(hdr[u] >> 32) == (uint32_t)(h[u] >> 32) && k0[u] == w[u][0] &&
-               (T.nkw < 2 || T.slots[pos[u] * stride + 2] == w[u][1])) {
-      found = true;
-      done = true;
-      brow[u] = (uint32_t)hdr[u] - 1u;
+  }
+  for (;;) {
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++) any = any || pend[u];
+    if (!__any(any)) break;
+    uint64_t hdr[FAST_U], s0[FAST_U], s1[FAST_U];
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++) {
+      const uint64_t at = (pend[u] ? pos[u] : 0) * stride;
+      hdr[u] = slots[at];
+      s0[u] = slots[at + 1];
     }
-    if (!done) {  // the first slot held another key: continue the linear probe
-      uint64_t p = join_step(T, pos[u]);
-      int64_t row = 0;
-      found = join_next(T, w[u], h[u], p, row);
-      brow[u] = (uint32_t)row;
+    if (two) {
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) s1[u] = slots[(pend[u] ? pos[u] : 0) * stride + 2];
     }
-    hit[u] = found;
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++) {
+      const bool empty = hdr[u] == 0;
+      const bool match = !empty && (uint32_t)(hdr[u] >> 32) == (uint32_t)(h[u] >> 32) && s0[u] == k0[u] &&
+                         (!two || s1[u] == k1[u]);
+      if (pend[u] && match) {
+        hit[u] = true;
+        brow[u] = (uint32_t)hdr[u] - 1u;
+      }
+      pend[u] = pend[u] && !empty && !match;
+      pos[u] = join_step(T, pos[u]);
+    }
   }
 }
 
@@ -731,10 +819,22 @@ __device__ __forceinline__ void load_stages(const uint32_t* base, uint64_t strid
 // the level's segment; survivors go to the next level's queue (or the sink
 // after the last segment). The deepest level holding a full wave runs first,
 // so queues never exceed QCAP entries, and rows reach the sink in source
-// order: every queued row precedes the rows of shallower levels. With a
-// streaming first probe (P.fast) level 0 takes FAST_U * 64 rows per round.
+// order: every queued row precedes the rows of shallower levels. Level 0
+// takes FAST_U * 64 rows per round (so level 1 holds up to QCAP_FAST).
+#ifndef ML_FLOW_WAVES
+#define ML_FLOW_WAVES 0  // minimum waves per SIMD asked of the register allocator (0 = none)
+#endif
+#ifndef ML_PF_COLS
+#define ML_PF_COLS 2     // stage-0 columns loaded ahead at level 0
+#endif
+#if ML_FLOW_WAVES > 0
+#define FLOW_BOUNDS __launch_bounds__(256, ML_FLOW_WAVES)
+#else
+#define FLOW_BOUNDS __launch_bounds__(256)
+#endif
+
 template <int SINK>
-__global__ void __launch_bounds__(256) flow_kernel(Plan P) {
+__global__ void FLOW_BOUNDS flow_kernel(Plan P) {
   extern __shared__ uint64_t lds[];
   const Code ins{(CInstr*)P.ins, P.nins - 1};
   const int ncells = (SINK == SINK_DIRECT_AGG) ? P.ngroups * P.nacc : 0;
@@ -766,59 +866,19 @@ __global__ void __launch_bounds__(256) flow_kernel(Plan P) {
     if (lvl < 0) break;
     lvl = uni(lvl);
 
-    if (lvl == 0 && P.fast) {
-      // FAST_U * 64 source rows; the hits go to level 1 in source order
-      uint32_t src[FAST_U], brow[FAST_U];
-      bool hit[FAST_U];
-      fast_probe(P, next, stop, lane, src, brow, hit);
-      next = min(stop, next + FAST_U * 64);
-      const int stage = P.steps[0].stage;
-      const uint32_t cap1 = (uint32_t)P.qcap[1];
-      uint32_t h = qget(qh, 1), c = qget(qn, 1);
-#pragma unroll
-      for (int u = 0; u < FAST_U; u++) {
-        const uint64_t m = __ballot(hit[u]);
-        if (hit[u]) {
-          uint32_t rows[MAX_STAGES];
-#pragma unroll
-          for (int s2 = 0; s2 < MAX_STAGES; s2++) rows[s2] = 0;
-          rows[0] = src[u];
-          set_row(rows, stage, brow[u]);
-          const uint32_t slot = (h + c + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))) & (cap1 - 1);
-          store_stages(q + P.qoff[1] + slot, cap1, P.live[1], rows);
-        }
-        c += (uint32_t)__popcll(m);
+    // survivors of a segment: into the next level's queue, in lane order
+    auto push = [&](int L, bool alive, const uint32_t (&rows)[MAX_STAGES]) {
+      const uint64_t m = __ballot(alive);
+      const uint32_t h = qget(qh, L), c = qget(qn, L);
+      if (alive) {
+        const uint32_t cap = (uint32_t)P.qcap[L];
+        const uint32_t slot = (h + c + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))) & (cap - 1);
+        store_stages(q + P.qoff[L] + slot, cap, P.live[L], rows);
       }
-      qn = qset(qn, 1, c);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      continue;
-    }
-
-    uint32_t rows[MAX_STAGES];
-#pragma unroll
-    for (int s = 0; s < MAX_STAGES; s++) rows[s] = 0;
-    int n;
-    if (lvl == 0) {
-      n = (int)min((int64_t)64, stop - next);
-      if (lane < n) {
-        const int64_t i = next + lane;
-        rows[0] = P.sel ? P.sel[i] : (uint32_t)i;
-      }
-      next += n;
-    } else {
-      const uint32_t h = qget(qh, lvl), c = qget(qn, lvl);
-      n = (int)min(64u, c);
-      const uint32_t cap = (uint32_t)P.qcap[lvl];
-      if (lane < n) load_stages(q + P.qoff[lvl] + ((h + (uint32_t)lane) & (cap - 1)), cap, P.live[lvl], rows);
-      qh = qset(qh, lvl, (h + (uint32_t)n) & (cap - 1));
-      qn = qset(qn, lvl, c - (uint32_t)n);
-    }
-    const int sb = lvl == 0 ? 0 : P.seg_end[lvl - 1];
-    const bool alive = lane < n && run_steps(P, ins, sb, P.seg_end[lvl], rows);
-
-    if (lvl == last) {
+      qn = qset(qn, L, c + (uint32_t)__popcll(m));
+    };
+    // rows that passed the last segment
+    auto sink = [&](bool alive, const uint32_t (&rows)[MAX_STAGES]) {
       if constexpr (SINK == SINK_DIRECT_AGG) {
         if (alive) sink_direct_agg(P, ins, rows, lds);
       } else if constexpr (SINK == SINK_HASH_AGG) {
@@ -827,23 +887,80 @@ __global__ void __launch_bounds__(256) flow_kernel(Plan P) {
         const uint64_t m = __ballot(alive);
         if constexpr (SINK == SINK_ROWS) {
           if (alive) {
-            const uint64_t pos =
-                (uint64_t)gw * P.chunk + nout + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+            const uint64_t pos = (uint64_t)gw * P.chunk + nout + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
             store_stages(P.tuples + pos, P.staging, P.tup_mask, rows);
           }
         }
         nout += (uint64_t)__popcll(m);
       }
-    } else {
-      const uint64_t m = __ballot(alive);
-      const int L = lvl + 1;
-      const uint32_t h = qget(qh, L), c = qget(qn, L);
-      if (alive) {
-        const uint32_t cap = (uint32_t)P.qcap[L];
-        const uint32_t slot = (h + c + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))) & (cap - 1);
-        store_stages(q + P.qoff[L] + slot, cap, P.live[L], rows);
+    };
+
+    if (lvl == 0 && P.fast) {
+      // FAST_U * 64 source rows through the streaming probe; hits to level 1 in source order
+      uint32_t src[FAST_U], brow[FAST_U];
+      bool hit[FAST_U];
+      fast_probe(P, next, stop, lane, src, brow, hit);
+      next = min(stop, next + FAST_U * 64);
+      const int stage = P.steps[0].stage;
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) {
+        uint32_t rows[MAX_STAGES];
+#pragma unroll
+        for (int s2 = 0; s2 < MAX_STAGES; s2++) rows[s2] = 0;
+        rows[0] = src[u];
+        set_row(rows, stage, brow[u]);
+        push(1, hit[u], rows);
       }
-      qn = qset(qn, L, c + (uint32_t)__popcll(m));
+    } else if (lvl == 0) {
+      // FAST_U * 64 source rows; the stage-0 columns segment 0 reads are
+      // loaded for all of them first, then the segment runs row by row
+      uint32_t src[FAST_U];
+      bool act[FAST_U];
+      uint64_t v0[FAST_U], v1[FAST_U];
+      round_rows(P, next, stop, lane, src, act);
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) v0[u] = v1[u] = 0;
+      if (P.pf_n > 0) load_rows(P.pf_op[0], P.pf_col[0], src, v0);
+      if (ML_PF_COLS > 1 && P.pf_n > 1) load_rows(P.pf_op[1], P.pf_col[1], src, v1);
+      next = min(stop, next + FAST_U * 64);
+#pragma unroll 1
+      for (int u = 0; u < FAST_U; u++) {
+        uint32_t rows[MAX_STAGES];
+#pragma unroll
+        for (int s2 = 0; s2 < MAX_STAGES; s2++) rows[s2] = 0;
+        Pf pf;
+        pf.c0 = P.pf_n > 0 ? P.pf_col[0] : nullptr;
+        pf.c1 = (ML_PF_COLS > 1 && P.pf_n > 1) ? P.pf_col[1] : nullptr;
+        // pick the u-th values with selects (the arrays stay in registers)
+#pragma unroll
+        for (int k = 0; k < FAST_U; k++) {
+          if (k == u) {
+            rows[0] = src[k];
+            pf.v0 = v0[k];
+            pf.v1 = v1[k];
+          }
+        }
+        bool alive = false;
+#pragma unroll
+        for (int k = 0; k < FAST_U; k++)
+          if (k == u) alive = act[k];
+        alive = alive && run_steps(P, ins, 0, P.seg_end[0], rows, pf);
+        if (last == 0) sink(alive, rows);
+        else push(1, alive, rows);
+      }
+    } else {
+      uint32_t rows[MAX_STAGES];
+#pragma unroll
+      for (int s = 0; s < MAX_STAGES; s++) rows[s] = 0;
+      const uint32_t h = qget(qh, lvl), c = qget(qn, lvl);
+      const int n = (int)min(64u, c);
+      const uint32_t cap = (uint32_t)P.qcap[lvl];
+      if (lane < n) load_stages(q + P.qoff[lvl] + ((h + (uint32_t)lane) & (cap - 1)), cap, P.live[lvl], rows);
+      qh = qset(qh, lvl, (h + (uint32_t)n) & (cap - 1));
+      qn = qset(qn, lvl, c - (uint32_t)n);
+      const bool alive = lane < n && run_steps(P, ins, P.seg_end[lvl - 1], P.seg_end[lvl], rows);
+      if (lvl == last) sink(alive, rows);
+      else push(lvl + 1, alive, rows);
     }
     // queue writes of this round are read by other lanes in later rounds
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

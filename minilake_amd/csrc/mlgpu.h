@@ -53,6 +53,10 @@ enum Opcode : uint16_t {
   OP_LOAD_I64,
   OP_LOAD_U8,       // booleans (normalized to 0/1)
   OP_CONST,
+  // fused push forms (a peephole of LOAD CONST CMP and LOAD LOOKUP):
+  OP_CMPC,          // column imm (at `stage`, load kind flags>>13 & 3 = I32/U32/I64/U8) compared
+                    // with the constant imm2: flags>>8 & 7 = EQ,NE,LT,LE,GT,GE, bit 11 = f64
+  OP_LOOKUPC,       // table imm2 indexed by the u32 column imm (a dictionary predicate)
   // unary on slot depth-1
   OP_NOT,
   OP_NEG_I32,
@@ -86,6 +90,17 @@ struct Instr {
   uint64_t imm;    // LOAD: data pointer; CONST: value bits; LOOKUP: table pointer
   uint64_t imm2;   // LOAD: u8 validity pointer or 0
 };
+
+// Instructions that read a column at their stage.
+__host__ __device__ inline bool reads_column(uint32_t op) {
+  return op < OP_CONST || op == OP_CMPC || op == OP_LOOKUPC;
+}
+// The plain load opcode a column-reading instruction uses.
+__host__ __device__ inline uint32_t column_load_op(uint32_t op, uint32_t flags) {
+  if (op == OP_CMPC) return (flags >> 13) & 3u;
+  if (op == OP_LOOKUPC) return OP_LOAD_U32;
+  return op;
+}
 
 struct Prog {
   int32_t first = 0;     // index into the plan's instruction array
@@ -254,6 +269,10 @@ struct Plan {
   int32_t fast_nkeys;
   uint16_t fast_op[2];
   const void* fast_col[2];
+  // otherwise: stage-0 columns segment 0 reads, loaded ahead for FAST_U rows
+  int32_t pf_n;
+  uint16_t pf_op[2];
+  const void* pf_col[2];
 };
 
 // Sortable key words for TopN.
