@@ -247,6 +247,7 @@ struct JoinTableImpl {
   std::vector<Buffer> col_valid;
   Buffer slots;
   Buffer bloom;
+  Buffer dense_rows, dense_bits;  // dense layout (DevJoinTable::dense)
   DevJoinTable dev{};
   std::vector<ml_column> views;
 };

@@ -240,6 +240,9 @@ int PipelineBuild::grid(int block) const {
 #ifndef ML_BLOOM_KEYS_PER_WORD
 #define ML_BLOOM_KEYS_PER_WORD 8
 #endif
+#ifndef ML_JOIN_LOAD_PCT
+#define ML_JOIN_LOAD_PCT 75
+#endif
 
 static int block_of(ml_ctx* ctx) { return ctx->cfg.block_size > 0 ? ctx->cfg.block_size : 256; }
 
@@ -1179,60 +1182,12 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
   J->chunks.push_back(std::move(ch));
 }
 
-static ml_join_table* join_build_finalize(ml_join_build* J) {
-  ml_ctx* ctx = J->ctx;
-  auto* t = new ml_join_table();
-  t->ctx = ctx;
-  JoinTableImpl& T = t->impl;
-  T.key_types = J->key_types;
-  T.key_dicts = J->key_dicts;
-  T.col_types = J->col_types;
-  T.col_dicts = J->col_dicts;
-  if (T.key_dicts.size() < T.key_types.size()) T.key_dicts.resize(T.key_types.size());
-  int64_t n = 0;
-  for (auto& c : J->chunks) n += c.rows;
-  T.nrows = n;
-  const size_t nk = J->key_types.size();
-  Buffer keys, key_null;
-  if (J->chunks.size() == 1) {
-    auto& c = J->chunks[0];
-    T.cols = std::move(c.cols);
-    T.col_valid = std::move(c.valid);
-    keys = std::move(c.keys);
-    key_null = std::move(c.key_null);
-  } else {
-    // Batch::concat of the collected batches (batch.rs:168-184)
-    for (size_t ci = 0; ci < T.col_types.size(); ci++) {
-      const int w = type_width(T.col_types[ci]);
-      T.cols.emplace_back(ctx->arena, (size_t)std::max<int64_t>(n, 1) * w);
-      const bool nl = ci < J->col_nullable.size() && J->col_nullable[ci];
-      T.col_valid.emplace_back();
-      if (nl) {
-        T.col_valid.back() = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1));
-        launch_fill_u64(T.col_valid.back().as<uint64_t>(), 0x0101010101010101ull,
-                        (std::max<int64_t>(n, 1) + 7) / 8, ctx->stream);
-      }
-      int64_t off = 0;
-      for (auto& c : J->chunks) {
-        d2d(ctx, T.cols[ci].as<uint8_t>() + off * w, c.cols[ci].ptr, (size_t)c.rows * w);
-        if (nl && c.valid[ci].ptr) d2d(ctx, T.col_valid[ci].as<uint8_t>() + off, c.valid[ci].ptr, (size_t)c.rows);
-        off += c.rows;
-      }
-    }
-    keys = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1) * nk * 8);
-    key_null = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1));
-    int64_t off = 0;
-    for (auto& c : J->chunks) {
-      d2d(ctx, keys.as<uint64_t>() + off * nk, c.keys.ptr, (size_t)c.rows * nk * 8);
-      d2d(ctx, key_null.as<uint8_t>() + off, c.key_null.ptr, (size_t)c.rows);
-      off += c.rows;
-    }
-  }
-  J->chunks.clear();
-  if (T.col_valid.size() < T.cols.size()) T.col_valid.resize(T.cols.size());
-  // open-addressing table at load <= 3/4
-  const uint64_t cap = std::max<uint64_t>(16, next_pow2((uint64_t)((n * 4 + 2) / 3)));
-  T.dev.nkw = (int)nk;
+// Open-addressing table at load <= ML_JOIN_LOAD_PCT percent, built in LDS
+// partitions when they fit (see join_part_build_kernel), else with global
+// atomics; with a blocked Bloom filter from 64K rows (or as configured).
+static void build_hashed(ml_ctx* ctx, JoinTableImpl& T, Buffer& keys, Buffer& key_null, int64_t n, size_t nk,
+                         uint32_t* flags) {
+  const uint64_t cap = std::max<uint64_t>(16, next_pow2((uint64_t)((n * 100 + ML_JOIN_LOAD_PCT - 1) / ML_JOIN_LOAD_PCT)));
   T.dev.mask = cap - 1;
   T.slots = Buffer(ctx->arena, cap * (1 + nk) * 8);
   T.dev.slots = T.slots.as<uint64_t>();
@@ -1240,7 +1195,6 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   const bool bloom = use_bloom > 0 || (use_bloom < 0 && n >= (1 << 16));
   // one 64-bit block (3 bits set per key) per ML_BLOOM_KEYS_PER_WORD keys
   uint64_t bloom_words = bloom ? std::max<uint64_t>(64, next_pow2((uint64_t)n / ML_BLOOM_KEYS_PER_WORD + 1)) : 0;
-  uint32_t* flags = (uint32_t*)ctx->counters.as<uint64_t>();
   zero(ctx, flags, 8);
   // Partitioned build: partitions of S slots, each built by one workgroup in
   // LDS (no global atomics), S chosen so a partition's slots fit in 32 KB.
@@ -1301,6 +1255,98 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
     d2h(ctx, &hdup, flags, 4);
     T.dev.unique = hdup ? 0 : 1;
   }
+}
+
+static ml_join_table* join_build_finalize(ml_join_build* J) {
+  ml_ctx* ctx = J->ctx;
+  auto* t = new ml_join_table();
+  t->ctx = ctx;
+  JoinTableImpl& T = t->impl;
+  T.key_types = J->key_types;
+  T.key_dicts = J->key_dicts;
+  T.col_types = J->col_types;
+  T.col_dicts = J->col_dicts;
+  if (T.key_dicts.size() < T.key_types.size()) T.key_dicts.resize(T.key_types.size());
+  int64_t n = 0;
+  for (auto& c : J->chunks) n += c.rows;
+  T.nrows = n;
+  const size_t nk = J->key_types.size();
+  Buffer keys, key_null;
+  if (J->chunks.size() == 1) {
+    auto& c = J->chunks[0];
+    T.cols = std::move(c.cols);
+    T.col_valid = std::move(c.valid);
+    keys = std::move(c.keys);
+    key_null = std::move(c.key_null);
+  } else {
+    // Batch::concat of the collected batches (batch.rs:168-184)
+    for (size_t ci = 0; ci < T.col_types.size(); ci++) {
+      const int w = type_width(T.col_types[ci]);
+      T.cols.emplace_back(ctx->arena, (size_t)std::max<int64_t>(n, 1) * w);
+      const bool nl = ci < J->col_nullable.size() && J->col_nullable[ci];
+      T.col_valid.emplace_back();
+      if (nl) {
+        T.col_valid.back() = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1));
+        launch_fill_u64(T.col_valid.back().as<uint64_t>(), 0x0101010101010101ull,
+                        (std::max<int64_t>(n, 1) + 7) / 8, ctx->stream);
+      }
+      int64_t off = 0;
+      for (auto& c : J->chunks) {
+        d2d(ctx, T.cols[ci].as<uint8_t>() + off * w, c.cols[ci].ptr, (size_t)c.rows * w);
+        if (nl && c.valid[ci].ptr) d2d(ctx, T.col_valid[ci].as<uint8_t>() + off, c.valid[ci].ptr, (size_t)c.rows);
+        off += c.rows;
+      }
+    }
+    keys = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1) * nk * 8);
+    key_null = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1));
+    int64_t off = 0;
+    for (auto& c : J->chunks) {
+      d2d(ctx, keys.as<uint64_t>() + off * nk, c.keys.ptr, (size_t)c.rows * nk * 8);
+      d2d(ctx, key_null.as<uint8_t>() + off, c.key_null.ptr, (size_t)c.rows);
+      off += c.rows;
+    }
+  }
+  J->chunks.clear();
+  if (T.col_valid.size() < T.cols.size()) T.col_valid.resize(T.cols.size());
+  T.dev.nkw = (int)nk;
+  uint32_t* flags = (uint32_t*)ctx->counters.as<uint64_t>();
+  // Dense layout: one integer key whose values span at most
+  // max(8n, 2^20) consecutive integers, all distinct. A probe is then one
+  // presence bit and one row load, with no hashing or probing.
+  const int kt0 = T.key_types[0];
+  if (nk == 1 && n > 0 && (uint64_t)n < 0xFFFFFFFFull &&
+      (kt0 == ML_TYPE_INT32 || kt0 == ML_TYPE_INT64 || kt0 == ML_TYPE_DATE)) {
+    const int nparts = (int)std::min<int64_t>(1024, (n + 4095) / 4096);
+    Buffer partials(ctx->arena, (size_t)nparts * 16), mm(ctx->arena, 16);
+    launch_key_minmax(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, partials.as<int64_t>(), nparts,
+                      mm.as<int64_t>(), ctx->stream);
+    int64_t hm[2] = {0, -1};
+    d2h(ctx, hm, mm.ptr, 16);
+    if (hm[0] <= hm[1]) {
+      const unsigned __int128 range = (unsigned __int128)((__int128)hm[1] - (__int128)hm[0]) + 1;
+      const unsigned __int128 limit = std::max<uint64_t>((uint64_t)n * 8, 1ull << 20);
+      if (range <= limit && range < ((unsigned __int128)1 << 31)) {
+        const uint64_t R = (uint64_t)range;
+        Buffer rows(ctx->arena, ((R + 1) / 2) * 8), bits(ctx->arena, ((R + 63) / 64) * 8);
+        zero(ctx, flags, 8);
+        launch_dense_build(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R, rows.as<uint32_t>(),
+                           bits.as<uint64_t>(), flags, ctx->stream);
+        uint32_t hf = 0;
+        d2h(ctx, &hf, flags, 4);
+        if (!(hf & 1)) {
+          T.dense_rows = std::move(rows);
+          T.dense_bits = std::move(bits);
+          T.dev.dense = 1;
+          T.dev.dense_min = hm[0];
+          T.dev.dense_range = R;
+          T.dev.dense_rows = T.dense_rows.as<uint32_t>();
+          T.dev.dense_bits = T.dense_bits.as<uint64_t>();
+          T.dev.unique = 1;
+        }
+      }
+    }
+  }
+  if (!T.dev.dense) build_hashed(ctx, T, keys, key_null, n, nk, flags);
   for (size_t c = 0; c < T.col_types.size(); c++) {
     ml_column v{};
     v.type = T.col_types[c];

@@ -75,6 +75,11 @@ void launch_join_part_scatter(const DevJoinTable& T, const uint64_t* keys, const
 void launch_join_part_build(const DevJoinTable& T, int nparts, const uint32_t* offsets, int nblocks,
                             const uint32_t* srow, const uint64_t* skeys, uint64_t* bloom, uint32_t* flags,
                             size_t lds, hipStream_t s);
+// dense join tables: key range, then rows / presence bits (flag bit 0 = repeated key)
+void launch_key_minmax(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t* partials, int nparts,
+                       int64_t* mm, hipStream_t s);
+void launch_dense_build(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t range,
+                        uint32_t* rows, uint64_t* bits, uint32_t* flag, hipStream_t s);
 // exclusive scan of n u32 counts (any n < 2^32 total); out[n] = total
 void launch_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, uint32_t* block_sums, hipStream_t s);
 int64_t scan_u32_blocks(int64_t n);

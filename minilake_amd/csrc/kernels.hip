@@ -271,19 +271,29 @@ __device__ __forceinline__ void set_word(uint64_t (&w)[MAX_KEYS + 1], int k, uin
   }
 }
 
+// Table hash of key words (n is uniform; the unrolled body keeps w in registers).
 __device__ __forceinline__ uint64_t key_hash(const uint64_t* w, int n) {
-  uint64_t h = HSEED;
+  uint64_t h = tmix(w[0] ^ TSEED);
 #pragma unroll
-  for (int k = 0; k < MAX_KEYS + 1; k++)
-    if (k < n) h = hash_combine(h, hash_u64(w[k]));
+  for (int k = 1; k < MAX_KEYS + 1; k++)
+    if (k < n) h = tmix(h ^ w[k]);
   return h;
 }
 
 __device__ __forceinline__ bool bloom_maybe(const DevJoinTable& T, uint64_t h) {
   if (!T.bloom) return true;
-  const uint64_t g = hash_u64(h);
+  const uint64_t g = bloom_mix(h);
   const uint64_t bits = bloom_bits(g);
   return (T.bloom[bloom_word(T, h, g)] & bits) == bits;
+}
+
+// Dense table lookup of key word k.
+__device__ __forceinline__ bool dense_find(const DevJoinTable& T, uint64_t k, uint32_t& row) {
+  const uint64_t idx = k - (uint64_t)T.dense_min;
+  if (idx >= T.dense_range) return false;
+  if (!((gp<uint64_t>(T.dense_bits)[idx >> 6] >> (idx & 63)) & 1ull)) return false;
+  row = gp<uint32_t>(T.dense_rows)[idx];
+  return true;
 }
 
 // First slot at or after `pos` whose key equals w; false at the first empty slot.
@@ -343,6 +353,15 @@ __device__ __forceinline__ bool run_steps(const Plan& P, const Code& ins, int fr
         continue;
       }
       const DevJoinTable& T = P.tables[st.table];
+      if (T.dense) {
+        uint32_t drow;
+        if (!dense_find(T, w[0], drow)) {
+          alive = false;
+          continue;
+        }
+        set_row(rows, st.stage, drow);
+        continue;
+      }
       const uint64_t h = key_hash(w, st.nkeys);
       uint64_t pos = h & T.mask;
       int64_t row = 0;
@@ -733,19 +752,34 @@ __device__ __forceinline__ void fast_probe(const Plan& P, int64_t next, int64_t 
   round_rows(P, next, stop, lane, src, hit);
   uint64_t k0[FAST_U], k1[FAST_U], h[FAST_U];
   load_rows(P.fast_op[0], P.fast_col[0], src, k0);
+  if (T.dense) {  // presence bits of all rows, then the rows of the present keys
+    uint64_t idx[FAST_U], bw[FAST_U];
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++) {
+      idx[u] = k0[u] - (uint64_t)T.dense_min;
+      hit[u] = hit[u] && idx[u] < T.dense_range;
+      bw[u] = gp<uint64_t>(T.dense_bits)[hit[u] ? (idx[u] >> 6) : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++) {
+      hit[u] = hit[u] && ((bw[u] >> (idx[u] & 63)) & 1ull);
+      brow[u] = gp<uint32_t>(T.dense_rows)[hit[u] ? idx[u] : 0];
+    }
+    return;
+  }
   const bool two = P.fast_nkeys > 1;
   if (two) load_rows(P.fast_op[1], P.fast_col[1], src, k1);
 #pragma unroll
   for (int u = 0; u < FAST_U; u++) {
-    uint64_t hh = hash_combine(HSEED, hash_u64(k0[u]));
-    if (two) hh = hash_combine(hh, hash_u64(k1[u]));
+    uint64_t hh = tmix(k0[u] ^ TSEED);
+    if (two) hh = tmix(hh ^ k1[u]);
     h[u] = hh;
   }
   if (T.bloom) {
     uint64_t word[FAST_U], bits[FAST_U];
 #pragma unroll
     for (int u = 0; u < FAST_U; u++) {
-      const uint64_t g = hash_u64(h[u]);
+      const uint64_t g = bloom_mix(h[u]);
       bits[u] = bloom_bits(g);
       word[u] = gp<uint64_t>(T.bloom)[hit[u] ? bloom_word(T, h[u], g) : 0];
     }
@@ -1088,7 +1122,7 @@ __global__ void join_insert_kernel(DevJoinTable T, const uint64_t* keys, const u
       pos = join_step(T, pos);
     }
     if (bloom) {
-      const uint64_t g = hash_u64(h);
+      const uint64_t g = bloom_mix(h);
       atomicOr((unsigned long long*)&bloom[bloom_word(T, h, g)], (unsigned long long)bloom_bits(g));
     }
   }
@@ -1119,6 +1153,86 @@ __global__ void join_unique_kernel(DevJoinTable T, uint32_t* dup_flag) {
       }
       q = join_step(T, q);
     }
+  }
+}
+
+// ------------------------------------------------- dense join table build
+// min / max of the non-NULL key words: one partial pair per workgroup, then
+// one workgroup folds the partials (mm[0] = min, mm[1] = max).
+__global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                                                         int64_t* partials) {
+  __shared__ int64_t lo[256], hi[256];
+  int64_t a = INT64_MAX, b = INT64_MIN;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    if (key_null[r]) continue;
+    const int64_t k = (int64_t)keys[r];
+    a = k < a ? k : a;
+    b = k > b ? k : b;
+  }
+  lo[threadIdx.x] = a;
+  hi[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      lo[threadIdx.x] = min(lo[threadIdx.x], lo[threadIdx.x + o]);
+      hi[threadIdx.x] = max(hi[threadIdx.x], hi[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = lo[0];
+    partials[2 * blockIdx.x + 1] = hi[0];
+  }
+}
+
+__global__ void __launch_bounds__(256) key_minmax_final_kernel(const int64_t* partials, int nparts, int64_t* mm) {
+  __shared__ int64_t lo[256], hi[256];
+  int64_t a = INT64_MAX, b = INT64_MIN;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    a = min(a, partials[2 * i]);
+    b = max(b, partials[2 * i + 1]);
+  }
+  lo[threadIdx.x] = a;
+  hi[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      lo[threadIdx.x] = min(lo[threadIdx.x], lo[threadIdx.x + o]);
+      hi[threadIdx.x] = max(hi[threadIdx.x], hi[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    mm[0] = lo[0];
+    mm[1] = hi[0];
+  }
+}
+
+// rows[k - min] = r for every non-NULL key (one writer wins for a repeated key)
+__global__ void dense_scatter_kernel(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin,
+                                     uint32_t* rows) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    if (!key_null[r]) rows[(uint64_t)((int64_t)keys[r] - kmin)] = (uint32_t)r;
+}
+
+// flag bit 0: a key whose slot holds another row (a repeated key)
+__global__ void dense_verify_kernel(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin,
+                                    const uint32_t* rows, uint32_t* flag) {
+  bool dup = false;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    if (!key_null[r] && rows[(uint64_t)((int64_t)keys[r] - kmin)] != (uint32_t)r) dup = true;
+  if (__any(dup) && __lane_id() == 0) atomicOr(flag, 1u);
+}
+
+// presence bits from the row array (empty = 0xFFFFFFFF)
+__global__ void dense_bits_kernel(const uint32_t* rows, uint64_t range, uint64_t* bits) {
+  const uint64_t nw = (range + 63) / 64;
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t m = 0;
+    const uint64_t base = w * 64;
+    for (int b = 0; b < 64; b++)
+      if (base + b < range && rows[base + b] != 0xFFFFFFFFu) m |= 1ull << b;
+    bits[w] = m;
   }
 }
 
@@ -1210,7 +1324,7 @@ __global__ void __launch_bounds__(256) join_part_build_kernel(DevJoinTable T, co
       pos = (pos + 1) & (S - 1);
     }
     if (W) {
-      const uint64_t g = hash_u64(h);
+      const uint64_t g = bloom_mix(h);
       atomicOr((unsigned long long*)&bw[(g >> 18) & T.bloom_mask], (unsigned long long)bloom_bits(g));
     }
   }
@@ -1643,6 +1757,21 @@ void launch_join_part_build(const DevJoinTable& T, int nparts, const uint32_t* o
                             const uint32_t* srow, const uint64_t* skeys, uint64_t* bloom, uint32_t* flags,
                             size_t lds, hipStream_t s) {
   join_part_build_kernel<<<nparts, 256, lds, s>>>(T, offsets, nblocks, srow, skeys, bloom, flags);
+}
+
+void launch_key_minmax(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t* partials, int nparts,
+                       int64_t* mm, hipStream_t s) {
+  key_minmax_kernel<<<nparts, 256, 0, s>>>(keys, key_null, n, partials);
+  key_minmax_final_kernel<<<1, 256, 0, s>>>(partials, nparts, mm);
+}
+
+void launch_dense_build(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t range,
+                        uint32_t* rows, uint64_t* bits, uint32_t* flag, hipStream_t s) {
+  fill_u64_kernel<<<grid_for((int64_t)((range + 1) / 2), 256, 8192), 256, 0, s>>>((uint64_t*)rows, ~0ull,
+                                                                                   (int64_t)((range + 1) / 2));
+  dense_scatter_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, rows);
+  dense_verify_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, rows, flag);
+  dense_bits_kernel<<<grid_for((int64_t)((range + 63) / 64), 256, 8192), 256, 0, s>>>(rows, range, bits);
 }
 
 int64_t scan_u32_blocks(int64_t n) { return (n + SCAN_B - 1) / SCAN_B; }

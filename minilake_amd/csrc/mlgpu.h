@@ -45,6 +45,24 @@ __host__ __device__ inline uint64_t fold_mul(uint64_t a, uint64_t b) {
 __host__ __device__ inline uint64_t hash_u64(uint64_t x) { return fold_mul(x ^ HSEED, HMUL); }
 __host__ __device__ inline uint64_t hash_combine(uint64_t h, uint64_t v) { return fold_mul(h ^ v, HMUL2); }
 
+// Hash of key words for this library's own device tables (join and aggregate
+// tables, Bloom filters). Their layout is internal, so they use a cheaper mix
+// than MiniLake's hash (one 64-bit multiply per key word instead of two
+// 64x64->128-bit products, which cost ~16 VALU each on a 32-bit ALU);
+// hash_columns still returns MiniLake's hash.
+constexpr uint64_t TSEED = 0x632BE59BD9B4E019ull;
+__host__ __device__ inline uint64_t tmix(uint64_t x) {
+  x *= 0x9E3779B97F4A7C15ull;
+  return x ^ (x >> 32);
+}
+__host__ __device__ inline uint64_t table_hash(const uint64_t* w, int n) {
+  uint64_t h = tmix(w[0] ^ TSEED);
+  for (int k = 1; k < n; k++) h = tmix(h ^ w[k]);
+  return h;
+}
+// Bloom hash of a table hash: its top bits pick the 3 bits, bits 18.. the word.
+__host__ __device__ inline uint64_t bloom_mix(uint64_t h) { return h * 0xD6E8FEB86659FD93ull; }
+
 // ------------------------------------------------------------------ opcodes
 enum Opcode : uint16_t {
   // push (write slot `depth`)
@@ -125,18 +143,27 @@ struct DevJoinTable {
   int32_t bloom_shift = 0; // log2 Bloom words per partition
   const uint64_t* bloom = nullptr;  // blocked Bloom filter (one u64 per block) or null
   uint64_t bloom_mask = 0; // Bloom words per partition - 1
+  // Dense layout (one integer key word, unique keys spanning a small range):
+  // the row of key k is dense_rows[k - dense_min] when k - dense_min <
+  // dense_range and bit (k - dense_min) of dense_bits is set. Slots and Bloom
+  // filter are then unused.
+  int32_t dense = 0;
+  int64_t dense_min = 0;
+  uint64_t dense_range = 0;
+  const uint32_t* dense_rows = nullptr;
+  const uint64_t* dense_bits = nullptr;
 };
 
 // next slot of a linear probe (inside the partition)
 __host__ __device__ inline uint64_t join_step(const DevJoinTable& T, uint64_t pos) {
   return (pos & ~T.wrap) | ((pos + 1) & T.wrap);
 }
-// Bloom word of a key with hash h (g = hash_u64(h) picks the word and bits)
+// Bloom word of a key with table hash h (g = bloom_mix(h) picks the word and bits)
 __host__ __device__ inline uint64_t bloom_word(const DevJoinTable& T, uint64_t h, uint64_t g) {
   return (((h & T.mask) >> T.part_shift) << T.bloom_shift) | ((g >> 18) & T.bloom_mask);
 }
 __host__ __device__ inline uint64_t bloom_bits(uint64_t g) {
-  return (1ull << (g & 63)) | (1ull << ((g >> 6) & 63)) | (1ull << ((g >> 12) & 63));
+  return (1ull << ((g >> 46) & 63)) | (1ull << ((g >> 52) & 63)) | (1ull << (g >> 58));
 }
 
 // Aggregate table: header word per slot (0 empty, 1 being written,

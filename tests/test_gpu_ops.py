@@ -286,3 +286,35 @@ def test_join_build_partitioned_sizes(gpu_engine, oracle_engine, nb):
     agg_o = oracle_engine.hashagg(probe, ops_o, [col(2)], [INT32], [Agg(AGG_COUNT_STAR, None, INT64)])
     agg_g = gpu_engine.hashagg(probe, ops_g, [col(2)], [INT32], [Agg(AGG_COUNT_STAR, None, INT64)])
     assert_tables_equal(agg_g.to_host(), agg_o.to_host(), rtol=0.0)
+
+
+@pytest.mark.parametrize("case", ["i64_neg", "i32", "dup_falls_back", "null_keys"])
+def test_join_dense_layout(gpu_engine, oracle_engine, case):
+    """Unique integer keys over a small range use the direct-addressed layout;
+    probes below, inside and above the range, NULL build keys, and a repeated
+    key (which must fall back to the hashed layout) all match the oracle."""
+    rng = np.random.default_rng(21)
+    nb = 50_000
+    keys = rng.permutation(200_000)[:nb].astype(np.int64) - 100_000
+    if case == "dup_falls_back":
+        keys[7] = keys[8]
+    ktype = INT32 if case == "i32" else INT64
+    zero = np.zeros(nb, dtype=np.int64)
+    one = np.ones(nb, dtype=np.int64)
+    if case == "null_keys":
+        one[::5] = 0  # key + (0 % 0) is NULL on every fifth row
+    build = HostTable([HostColumn(ktype, keys.astype(np.int32) if ktype == INT32 else keys),
+                       HostColumn(INT64, np.arange(nb, dtype=np.int64)), HostColumn(INT64, zero),
+                       HostColumn(INT64, one)])
+    npr = 200_000
+    pk = rng.integers(-150_000, 150_000, npr)
+    probe = HostTable([HostColumn(ktype, pk.astype(np.int32) if ktype == INT32 else pk.astype(np.int64))])
+    bkey = [col(0) + (col(2) % col(3))] if case == "null_keys" else [col(0)]
+    jo = oracle_engine.join_build(build, [], bkey, [ktype])
+    jg = gpu_engine.join_build(build, [], bkey, [ktype])
+    assert jg.unique() == (case != "dup_falls_back")
+    ops_o, ops_g = [abi.probe_op(jo, [col(0)])], [abi.probe_op(jg, [col(0)])]
+    want = oracle_engine.collect(probe, ops_o).to_host()
+    got = gpu_engine.collect(probe, ops_g).to_host()
+    assert want.num_rows > 10_000
+    assert_tables_equal(got, want, rtol=0.0)
