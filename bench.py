@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--cpu-scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bloom", type=int, default=-1)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain sizing comparison")
+    ap.add_argument("--learned-rows", type=int, default=60_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
@@ -118,6 +120,23 @@ def main():
     del src, dst
     log(f"device copy {copy_gbs:.0f} GB/s")
 
+    # learned vs plain hash-table sizing on the 60M-row Zipf(1.1) aggregate + join
+    learned = None
+    if not args.no_learned:
+        from minilake_amd import sizing
+        params = sizing.load_params()
+        if params is None:
+            learned = {"error": "no trained estimator weights (python -m minilake_amd.train_estimator)"}
+        else:
+            del dev
+            torch.cuda.empty_cache()
+            mlp = sizing.Mlp(eng, params)
+            fact, build = sizing.zipf_workload(eng, rows=args.learned_rows)
+            learned = sizing.compare_plain_learned(eng, mlp, fact, build, build_distinct=2_000_000, log=log)
+            learned["workload"] = (f"{args.learned_rows:,} int64 keys Zipf(1.1) over 1e8, GROUP BY key COUNT(*), "
+                                   f"SUM(key mod 1009); join with 2,000,000 keys x 2 rows")
+            del fact, build, mlp
+
     cpu = None
     if not args.no_cpu:
         from oracle.oracle import OracleEngine
@@ -167,6 +186,7 @@ def main():
                      "algorithmic_bytes": fact_bytes, "kernel_ms": fact_ms_avg,
                      "measured_copy_gbs": copy_gbs},
         "cpu_baseline": cpu,
+        "learned_sizing": learned,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
     }
     print(json.dumps(out))

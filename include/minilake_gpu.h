@@ -181,7 +181,7 @@ int64_t ml_result_num_rows(const ml_result* r);
 int32_t ml_result_num_columns(const ml_result* r);
 /* Column i of the result; its data pointer is device memory. */
 const ml_column* ml_result_column(const ml_result* r, int32_t i);
-/* Copy column i to host memory (`dst` must hold num_rows * width bytes). */
+/* Copy column i to `dst`, host or device memory (num_rows * width bytes). */
 ml_status ml_result_copy_column(ml_ctx* ctx, const ml_result* r, int32_t i, void* dst);
 /* Copy column i's validity as one byte per row (1 = valid) to host memory.
  * Result columns carry validity this way; ml_column.validity of a result is NULL. */

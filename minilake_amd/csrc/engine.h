@@ -248,6 +248,7 @@ struct JoinTableImpl {
   Buffer slots;
   Buffer bloom;
   Buffer dense_rows, dense_bits;  // dense layout (DevJoinTable::dense)
+  Buffer keys;                    // build key words [rows][nkw] (dense layout with several keys)
   DevJoinTable dev{};
   std::vector<ml_column> views;
 };

@@ -1314,12 +1314,22 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   // max(8n, 2^20) consecutive integers, all distinct. A probe is then one
   // presence bit and one row load, with no hashing or probing.
   const int kt0 = T.key_types[0];
+  // (With several key columns the first alone would not reject rows the full
+  // key does, and the hashed layout's Bloom filter does that better.)
   if (nk == 1 && n > 0 && (uint64_t)n < 0xFFFFFFFFull &&
       (kt0 == ML_TYPE_INT32 || kt0 == ML_TYPE_INT64 || kt0 == ML_TYPE_DATE)) {
     const int nparts = (int)std::min<int64_t>(1024, (n + 4095) / 4096);
     Buffer partials(ctx->arena, (size_t)nparts * 16), mm(ctx->arena, 16);
-    launch_key_minmax(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, partials.as<int64_t>(), nparts,
-                      mm.as<int64_t>(), ctx->stream);
+    // the first key word of every row, contiguous
+    Buffer k0;
+    const uint64_t* first = keys.as<uint64_t>();
+    if (nk > 1) {
+      k0 = Buffer(ctx->arena, (size_t)n * 8);
+      launch_strided_u64(keys.as<uint64_t>(), (int)nk, n, k0.as<uint64_t>(), ctx->stream);
+      first = k0.as<uint64_t>();
+    }
+    launch_key_minmax(first, key_null.as<uint8_t>(), n, partials.as<int64_t>(), nparts, mm.as<int64_t>(),
+                      ctx->stream);
     int64_t hm[2] = {0, -1};
     d2h(ctx, hm, mm.ptr, 16);
     if (hm[0] <= hm[1]) {
@@ -1329,7 +1339,7 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
         const uint64_t R = (uint64_t)range;
         Buffer rows(ctx->arena, ((R + 1) / 2) * 8), bits(ctx->arena, ((R + 63) / 64) * 8);
         zero(ctx, flags, 8);
-        launch_dense_build(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R, rows.as<uint32_t>(),
+        launch_dense_build(first, key_null.as<uint8_t>(), n, hm[0], R, rows.as<uint32_t>(),
                            bits.as<uint64_t>(), flags, ctx->stream);
         uint32_t hf = 0;
         d2h(ctx, &hf, flags, 4);
@@ -1341,6 +1351,10 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
           T.dev.dense_range = R;
           T.dev.dense_rows = T.dense_rows.as<uint32_t>();
           T.dev.dense_bits = T.dense_bits.as<uint64_t>();
+          if (nk > 1) {
+            T.keys = std::move(keys);
+            T.dev.dense_keys = T.keys.as<uint64_t>();
+          }
           T.dev.unique = 1;
         }
       }
@@ -1694,7 +1708,9 @@ ml_status ml_result_copy_column(ml_ctx* ctx, const ml_result* r, int32_t i, void
   return guarded(ctx, [&] {
     if (!r || i < 0 || (size_t)i >= r->cols.size()) fail(ML_ERR_INTERNAL, "no such result column");
     const auto& c = r->cols[(size_t)i];
-    d2h(ctx, dst, c.data.ptr, (size_t)r->num_rows * type_width(c.type));
+    const size_t bytes = (size_t)r->num_rows * type_width(c.type);
+    if (bytes) hip_check(hipMemcpyAsync(dst, c.data.ptr, bytes, hipMemcpyDefault, ctx->stream), "copy column");
+    hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
   });
 }
 

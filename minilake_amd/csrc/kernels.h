@@ -76,6 +76,7 @@ void launch_join_part_build(const DevJoinTable& T, int nparts, const uint32_t* o
                             const uint32_t* srow, const uint64_t* skeys, uint64_t* bloom, uint32_t* flags,
                             size_t lds, hipStream_t s);
 // dense join tables: key range, then rows / presence bits (flag bit 0 = repeated key)
+void launch_strided_u64(const uint64_t* src, int stride, int64_t n, uint64_t* dst, hipStream_t s);
 void launch_key_minmax(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t* partials, int nparts,
                        int64_t* mm, hipStream_t s);
 void launch_dense_build(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t range,

@@ -355,7 +355,9 @@ __device__ __forceinline__ bool run_steps(const Plan& P, const Code& ins, int fr
       const DevJoinTable& T = P.tables[st.table];
       if (T.dense) {
         uint32_t drow;
-        if (!dense_find(T, w[0], drow)) {
+        bool ok = dense_find(T, w[0], drow);
+        for (int k = 1; k < T.nkw && ok; k++) ok = gp<uint64_t>(T.dense_keys)[(uint64_t)drow * T.nkw + k] == w[k];
+        if (!ok) {
           alive = false;
           continue;
         }
@@ -752,6 +754,8 @@ __device__ __forceinline__ void fast_probe(const Plan& P, int64_t next, int64_t 
   round_rows(P, next, stop, lane, src, hit);
   uint64_t k0[FAST_U], k1[FAST_U], h[FAST_U];
   load_rows(P.fast_op[0], P.fast_col[0], src, k0);
+  const bool two = P.fast_nkeys > 1;
+  if (two) load_rows(P.fast_op[1], P.fast_col[1], src, k1);
   if (T.dense) {  // presence bits of all rows, then the rows of the present keys
     uint64_t idx[FAST_U], bw[FAST_U];
 #pragma unroll
@@ -765,10 +769,15 @@ __device__ __forceinline__ void fast_probe(const Plan& P, int64_t next, int64_t 
       hit[u] = hit[u] && ((bw[u] >> (idx[u] & 63)) & 1ull);
       brow[u] = gp<uint32_t>(T.dense_rows)[hit[u] ? idx[u] : 0];
     }
+    if (two) {  // the second key word must match the build row's
+      uint64_t kb[FAST_U];
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) kb[u] = gp<uint64_t>(T.dense_keys)[(uint64_t)(hit[u] ? brow[u] : 0) * 2 + 1];
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) hit[u] = hit[u] && kb[u] == k1[u];
+    }
     return;
   }
-  const bool two = P.fast_nkeys > 1;
-  if (two) load_rows(P.fast_op[1], P.fast_col[1], src, k1);
 #pragma unroll
   for (int u = 0; u < FAST_U; u++) {
     uint64_t hh = tmix(k0[u] ^ TSEED);
@@ -1757,6 +1766,15 @@ void launch_join_part_build(const DevJoinTable& T, int nparts, const uint32_t* o
                             const uint32_t* srow, const uint64_t* skeys, uint64_t* bloom, uint32_t* flags,
                             size_t lds, hipStream_t s) {
   join_part_build_kernel<<<nparts, 256, lds, s>>>(T, offsets, nblocks, srow, skeys, bloom, flags);
+}
+
+__global__ void strided_u64_kernel(const uint64_t* src, int stride, int64_t n, uint64_t* dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i * stride];
+}
+
+void launch_strided_u64(const uint64_t* src, int stride, int64_t n, uint64_t* dst, hipStream_t s) {
+  if (n > 0) strided_u64_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(src, stride, n, dst);
 }
 
 void launch_key_minmax(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t* partials, int nparts,

@@ -143,15 +143,17 @@ struct DevJoinTable {
   int32_t bloom_shift = 0; // log2 Bloom words per partition
   const uint64_t* bloom = nullptr;  // blocked Bloom filter (one u64 per block) or null
   uint64_t bloom_mask = 0; // Bloom words per partition - 1
-  // Dense layout (one integer key word, unique keys spanning a small range):
-  // the row of key k is dense_rows[k - dense_min] when k - dense_min <
-  // dense_range and bit (k - dense_min) of dense_bits is set. Slots and Bloom
-  // filter are then unused.
+  // Dense layout (the first key word is an integer, unique, and its values
+  // span a small range): the only candidate row for first key word k is
+  // dense_rows[k - dense_min] when k - dense_min < dense_range and bit
+  // (k - dense_min) of dense_bits is set; further key words are compared with
+  // dense_keys[row * nkw + j]. Slots and Bloom filter are then unused.
   int32_t dense = 0;
   int64_t dense_min = 0;
   uint64_t dense_range = 0;
   const uint32_t* dense_rows = nullptr;
   const uint64_t* dense_bits = nullptr;
+  const uint64_t* dense_keys = nullptr;
 };
 
 // next slot of a linear probe (inside the partition)

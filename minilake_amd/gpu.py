@@ -236,6 +236,31 @@ class GpuResult:
             sel_ptr, sel_len = selection.data_ptr(), selection.numel()
         return keep(abi.MlBatch(cols, self.num_columns, 0, self.num_rows, sel_ptr, sel_len))
 
+    def to_tensors(self):
+        """The result's columns as device tensors (copies), by column type."""
+        L, torch = lib(), self.engine.torch
+        out = []
+        for i in range(self.num_columns):
+            dt = {abi.BOOLEAN: torch.uint8, abi.INT32: torch.int32, abi.DATE: torch.int32, abi.UTF8: torch.int32,
+                  abi.INT64: torch.int64, abi.FLOAT64: torch.float64}[self.types[i]]
+            t = torch.empty(max(1, self.num_rows), dtype=dt, device=f"cuda:{self.engine.device}")
+            self.engine._check(L.ml_result_copy_column(self.engine.ctx, self.handle, i, t.data_ptr()))
+            out.append(t[: self.num_rows])
+        return out
+
+    def fingerprint(self):
+        """Sorted 64-bit per-row fingerprints over every column: two results are
+        the same multiset of rows exactly when these tensors are equal (up to a
+        fingerprint collision)."""
+        torch = self.engine.torch
+        h = torch.full((self.num_rows,), 0x632BE59BD9B4E019, dtype=torch.int64,
+                       device=f"cuda:{self.engine.device}")
+        for c in self.to_tensors():
+            v = c.view(torch.int64) if c.dtype in (torch.int64, torch.float64) else c.to(torch.int64)
+            h = (h ^ v) * -7046029254386353131  # 0x9E3779B97F4A7C15 as a signed 64-bit value
+            h = h ^ (h >> 29)
+        return torch.sort(h).values
+
     def to_host(self) -> HostTable:
         L = lib()
         cols = []

@@ -252,12 +252,12 @@ def zipf_workload(engine, rows: int = 60_000_000, domain: int = 100_000_000, s: 
     return fact, build
 
 
-def run_plain(engine, fact, build):
+def run_plain(engine, fact, build, stats: Optional[dict] = None):
     """GROUP BY key (COUNT, SUM) with resize-on-overflow sizing, and the join
     collected after an exact counting pass."""
     from .abi import AGG_SUM, INT64, Agg, col, count_star, probe_op
     agg = engine.hashagg(fact, [], [col(0)], [INT64], [count_star(), Agg(AGG_SUM, col(1), INT64)],
-                         sizing=abi.SIZING_PLAIN)
+                         sizing=abi.SIZING_PLAIN, stats=stats)
     jt = engine.join_build(build, [], [col(0)], [INT64])
     joined = engine.collect(fact, [probe_op(jt, [col(0)])])
     return agg, joined
@@ -268,7 +268,8 @@ def predict(engine, mlp: "Mlp", fact, build_rows: int, build_distinct: int, jt, 
     return mlp.forward(x[None, :])[0]
 
 
-def run_learned(engine, mlp: "Mlp", fact, build, build_distinct: int, margin: float = 1.15):
+def run_learned(engine, mlp: "Mlp", fact, build, build_distinct: int, margin: float = 1.15,
+                stats: Optional[dict] = None):
     """The same two queries with both outputs pre-sized from the network's
     prediction (features + inference included); too-small predictions fall
     back to the exact path inside the library."""
@@ -278,6 +279,46 @@ def run_learned(engine, mlp: "Mlp", fact, build, build_distinct: int, margin: fl
     groups = capacity_from_log2(y[0], margin)
     out_rows = max(1, capacity_from_log2(y[1], margin) - 1)
     agg = engine.hashagg(fact, [], [col(0)], [INT64], [count_star(), Agg(AGG_SUM, col(1), INT64)],
-                         sizing=abi.SIZING_LEARNED, capacity_hint=groups)
+                         sizing=abi.SIZING_LEARNED, capacity_hint=groups, stats=stats)
     joined, fell_back = engine.collect_sized(fact, [probe_op(jt, [col(0)])], out_rows)
     return agg, joined, y, fell_back
+
+
+def same_rows(a, b) -> bool:
+    """Bit-identical results as multisets of rows (device fingerprints)."""
+    if a.num_rows != b.num_rows or a.types != b.types:
+        return False
+    return bool(a.engine.torch.equal(a.fingerprint(), b.fingerprint()))
+
+
+def compare_plain_learned(engine, mlp: "Mlp", fact, build, build_distinct: int, reps: int = 3, log=None) -> dict:
+    """Time the plain and learned runs of the evaluation queries (after one
+    warm-up each) and check that their results are bit-identical."""
+    import time
+    torch = engine.torch
+
+    def timed(f):
+        f()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = f()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return sorted(ts)[len(ts) // 2], out
+
+    ps, ls = {}, {}
+    plain_ms, (agg_p, join_p) = timed(lambda: run_plain(engine, fact, build, stats=ps))
+    learned_ms, (agg_l, join_l, y, fell_back) = timed(lambda: run_learned(engine, mlp, fact, build, build_distinct,
+                                                                          stats=ls))
+    out = {"plain_ms": plain_ms, "learned_ms": learned_ms, "speedup": plain_ms / learned_ms,
+           "groups": agg_p.num_rows, "predicted_groups": float(2.0 ** y[0]),
+           "join_rows": join_p.num_rows, "predicted_join_rows": float(2.0 ** y[1] - 1),
+           "plain_resizes": ps.get("resizes"), "learned_resizes": ls.get("resizes"),
+           "join_fell_back": bool(fell_back),
+           "identical": same_rows(agg_p, agg_l) and same_rows(join_p, join_l)}
+    if log:
+        log(f"learned sizing: plain {plain_ms:.2f} ms, learned {learned_ms:.2f} ms "
+            f"({out['speedup']:.2f}x), identical={out['identical']}")
+    return out

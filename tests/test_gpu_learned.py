@@ -157,3 +157,20 @@ def test_fragments_match_oracle(gpu_engine, oracle_engine, mode):
         for (gs, gc), (ws, wc) in zip(got, want):
             np.testing.assert_array_equal(gc, wc)
             np.testing.assert_allclose(gs, ws, rtol=1e-12, atol=1e-9)
+
+
+def test_learned_sizing_bit_identical_and_faster(gpu_engine):
+    """The trained estimator pre-sizes the group table and the join output:
+    results are bit-identical to plain resize-on-overflow sizing, the group
+    table never grows, and the run is faster."""
+    from minilake_amd import sizing
+    params = sizing.load_params()
+    assert params is not None, "trained weights (minilake_amd/estimator_weights.npy) are missing"
+    mlp = sizing.Mlp(gpu_engine, params)
+    fact, build = sizing.zipf_workload(gpu_engine, rows=4_000_000, domain=10_000_000, build_keys=200_000)
+    r = sizing.compare_plain_learned(gpu_engine, mlp, fact, build, build_distinct=200_000, reps=3)
+    print(r)
+    assert r["identical"]
+    assert r["plain_resizes"] >= 2
+    assert r["learned_resizes"] == 0
+    assert r["learned_ms"] < r["plain_ms"]

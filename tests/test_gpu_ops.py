@@ -288,7 +288,7 @@ def test_join_build_partitioned_sizes(gpu_engine, oracle_engine, nb):
     assert_tables_equal(agg_g.to_host(), agg_o.to_host(), rtol=0.0)
 
 
-@pytest.mark.parametrize("case", ["i64_neg", "i32", "dup_falls_back", "null_keys"])
+@pytest.mark.parametrize("case", ["i64_neg", "i32", "dup_falls_back", "null_keys", "two_keys_hashed"])
 def test_join_dense_layout(gpu_engine, oracle_engine, case):
     """Unique integer keys over a small range use the direct-addressed layout;
     probes below, inside and above the range, NULL build keys, and a repeated
@@ -310,10 +310,17 @@ def test_join_dense_layout(gpu_engine, oracle_engine, case):
     pk = rng.integers(-150_000, 150_000, npr)
     probe = HostTable([HostColumn(ktype, pk.astype(np.int32) if ktype == INT32 else pk.astype(np.int64))])
     bkey = [col(0) + (col(2) % col(3))] if case == "null_keys" else [col(0)]
-    jo = oracle_engine.join_build(build, [], bkey, [ktype])
-    jg = gpu_engine.join_build(build, [], bkey, [ktype])
+    btypes, pkeys = [ktype], [col(0)]
+    if case == "two_keys_hashed":
+        # (dense unique key, small int): two key columns use the hashed layout
+        tag = (keys % 3).astype(np.int32)
+        build = HostTable(build.columns + [HostColumn(INT32, tag)])
+        bkey, btypes, pkeys = [col(0), col(4)], [INT64, INT32], [col(0), col(1)]
+        probe = HostTable([probe.columns[0], HostColumn(INT32, (pk % 3 + (rng.random(npr) < 0.3)).astype(np.int32))])
+    jo = oracle_engine.join_build(build, [], bkey, btypes)
+    jg = gpu_engine.join_build(build, [], bkey, btypes)
     assert jg.unique() == (case != "dup_falls_back")
-    ops_o, ops_g = [abi.probe_op(jo, [col(0)])], [abi.probe_op(jg, [col(0)])]
+    ops_o, ops_g = [abi.probe_op(jo, pkeys)], [abi.probe_op(jg, pkeys)]
     want = oracle_engine.collect(probe, ops_o).to_host()
     got = gpu_engine.collect(probe, ops_g).to_host()
     assert want.num_rows > 10_000
