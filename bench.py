@@ -33,6 +33,22 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "flow_kernel_star_join_q5.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the lineitem kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,7 +197,9 @@ def main():
                                f"{sj.supplier.num_rows:,}, nation 25, region 5; GROUP BY n_name SUM(price*(1-disc))",
                    "input_rows": input_rows, "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": profiled_traffic(),
+                     "traffic_source": "profiles/flow_kernel_star_join_q5.md (rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                                       "per dispatch, scaled by a 2 GiB copy kernel's counts)",
                      "kernel": "flow_kernel<SINK_DIRECT_AGG> (lineitem probe chain)",
                      "algorithmic_bytes": fact_bytes, "kernel_ms": fact_ms_avg,
                      "measured_copy_gbs": copy_gbs},

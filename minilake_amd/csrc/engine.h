@@ -121,6 +121,10 @@ struct ml_ctx {
   mlg::MlpModel* estimator = nullptr;
   // small per-context device scratch: counters and flags
   mlg::Buffer counters;  // 64 x u64
+  // pinned host staging for small device-to-host reads (counts, flags), so
+  // they skip the pageable-memory staging path of hipMemcpy
+  void* pinned = nullptr;
+  static constexpr size_t PINNED_BYTES = 4096;
   std::map<const ml_dict*, std::shared_ptr<mlg::HostDict>> dicts;
   std::shared_ptr<mlg::HostDict> dict(const ml_dict* d);
   // launch statistics

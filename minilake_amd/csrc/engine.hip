@@ -63,6 +63,12 @@ static void h2d(ml_ctx* ctx, void* dst, const void* src, size_t n) {
   if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, ctx->stream), "copy to device");
 }
 static void d2h(ml_ctx* ctx, void* dst, const void* src, size_t n) {
+  if (n && ctx->pinned && n <= ml_ctx::PINNED_BYTES) {
+    hip_check(hipMemcpyAsync(ctx->pinned, src, n, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
+    hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+    std::memcpy(dst, ctx->pinned, n);
+    return;
+  }
   if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
   hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
 }
@@ -314,6 +320,45 @@ static bool plan_flow(PipelineBuild& B, int sink, uint32_t sink_stages, size_t c
     if (plain) {
       P.fast = 1;
       P.fast_nkeys = P.steps[0].nkeys;
+    }
+  }
+  // a segment-0 filter made only of column atoms: truth table over the atoms
+  P.natoms = 0;
+  if (!P.fast && (cuts.empty() ? P.nsteps : cuts[0]) == 1 && P.steps[0].kind == STEP_FILTER &&
+      !P.steps[0].pred.nullable) {
+    const Prog& pg = P.steps[0].pred;
+    std::vector<FilterAtom> atoms;
+    bool ok = true;
+    for (int i = 0; i < pg.count && ok; i++) {
+      const Instr& in = B.ins[(size_t)pg.first + i];
+      if (in.op == OP_CMPC || in.op == OP_LOOKUPC) {
+        ok = in.stage == 0 && atoms.size() < (size_t)MAX_ATOMS;
+        atoms.push_back(FilterAtom{(const void*)in.imm, in.imm2, in.op, in.flags});
+      } else {
+        ok = in.op == OP_AND || in.op == OP_OR || in.op == OP_NOT || (in.op == OP_CONST && (in.flags & 1u));
+      }
+    }
+    if (ok && !atoms.empty()) {
+      uint32_t truth = 0;
+      for (uint32_t m = 0; m < (1u << atoms.size()); m++) {
+        std::vector<int> st;
+        int a = 0;
+        for (int i = 0; i < pg.count; i++) {
+          const Instr& in = B.ins[(size_t)pg.first + i];
+          if (in.op == OP_CMPC || in.op == OP_LOOKUPC) st.push_back((m >> a++) & 1);
+          else if (in.op == OP_CONST) st.push_back(in.imm != 0);
+          else if (in.op == OP_NOT) st.back() = !st.back();
+          else {
+            const int y = st.back();
+            st.pop_back();
+            st.back() = in.op == OP_AND ? (st.back() && y) : (st.back() || y);
+          }
+        }
+        if (st.size() == 1 && st[0]) truth |= 1u << m;
+      }
+      P.natoms = (int32_t)atoms.size();
+      P.atom_truth = truth;
+      for (size_t k = 0; k < atoms.size(); k++) P.atoms[k] = atoms[k];
     }
   }
   // otherwise the stage-0 columns segment 0 loads (at most two) are loaded
@@ -1643,6 +1688,7 @@ ml_status ml_ctx_create(const ml_exec_config* cfg, ml_ctx** out) {
     ctx->arena = std::make_shared<DeviceArena>(ctx->device);
     ctx->counters = Buffer(ctx->arena, 64 * 8);
     zero(ctx, ctx->counters.ptr, 64 * 8);
+    hip_check(hipHostMalloc(&ctx->pinned, ml_ctx::PINNED_BYTES, hipHostMallocDefault), "hipHostMalloc");
     hip_check(hipStreamSynchronize(ctx->stream), "sync");
   });
   if (st != ML_OK) {
@@ -1660,6 +1706,7 @@ void ml_ctx_destroy(ml_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->counters.reset();
   ctx->arena.reset();
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

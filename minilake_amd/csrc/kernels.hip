@@ -839,6 +839,40 @@ __device__ __forceinline__ void fast_probe(const Plan& P, int64_t next, int64_t 
   }
 }
 
+// Outcome bits of the filter atoms for FAST_U rows: each atom's column is
+// loaded for all rows before any comparison.
+__device__ __forceinline__ void eval_atoms(const Plan& P, const uint32_t (&src)[FAST_U], uint32_t (&bits)[FAST_U]) {
+#pragma unroll
+  for (int u = 0; u < FAST_U; u++) bits[u] = 0;
+#pragma unroll
+  for (int a = 0; a < MAX_ATOMS; a++) {
+    if (a >= P.natoms) break;
+    const FilterAtom& A = P.atoms[a];
+    uint64_t x[FAST_U];
+    load_rows((uint16_t)column_load_op(A.op, A.flags), A.col, src, x);
+    if (A.op == OP_LOOKUPC) {
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) bits[u] |= (uint32_t)(gp<uint8_t>((const void*)A.imm2)[x[u]] != 0) << a;
+    } else {
+      const uint32_t c = (A.flags >> 8) & 7u;
+      const bool f = (A.flags & 0x800u) != 0;
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) {
+        bool lt, gt, eq;
+        if (f) {
+          const double p = u2f(x[u]), q = u2f(A.imm2);
+          lt = p < q, gt = p > q, eq = p == q;
+        } else {
+          const int64_t p = (int64_t)x[u], q = (int64_t)A.imm2;
+          lt = p < q, gt = p > q, eq = p == q;
+        }
+        const bool r = c == 0 ? eq : c == 1 ? !eq : c == 2 ? lt : c == 3 ? (lt || eq) : c == 4 ? gt : (gt || eq);
+        bits[u] |= (uint32_t)r << a;
+      }
+    }
+  }
+}
+
 // Row indices of the stages in `mask`, stored as u32 at base[k * stride]
 // for the k-th stage of the mask. Unrolled over stages so rows[] stays in
 // registers.
@@ -953,6 +987,23 @@ __global__ void FLOW_BOUNDS flow_kernel(Plan P) {
         rows[0] = src[u];
         set_row(rows, stage, brow[u]);
         push(1, hit[u], rows);
+      }
+    } else if (lvl == 0 && P.natoms) {
+      // FAST_U * 64 source rows through a filter of column atoms
+      uint32_t src[FAST_U], bits[FAST_U];
+      bool act[FAST_U];
+      round_rows(P, next, stop, lane, src, act);
+      eval_atoms(P, src, bits);
+      next = min(stop, next + FAST_U * 64);
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) {
+        uint32_t rows[MAX_STAGES];
+#pragma unroll
+        for (int s2 = 0; s2 < MAX_STAGES; s2++) rows[s2] = 0;
+        rows[0] = src[u];
+        const bool alive = act[u] && ((P.atom_truth >> bits[u]) & 1u);
+        if (last == 0) sink(alive, rows);
+        else push(1, alive, rows);
       }
     } else if (lvl == 0) {
       // FAST_U * 64 source rows; the stage-0 columns segment 0 reads are
@@ -1233,15 +1284,16 @@ __global__ void dense_verify_kernel(const uint64_t* keys, const uint8_t* key_nul
   if (__any(dup) && __lane_id() == 0) atomicOr(flag, 1u);
 }
 
-// presence bits from the row array (empty = 0xFFFFFFFF)
+// presence bits from the row array (empty = 0xFFFFFFFF): one wave per 64
+// entries, read coalesced, the word is the wave's ballot
 __global__ void dense_bits_kernel(const uint32_t* rows, uint64_t range, uint64_t* bits) {
   const uint64_t nw = (range + 63) / 64;
-  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t m = 0;
-    const uint64_t base = w * 64;
-    for (int b = 0; b < 64; b++)
-      if (base + b < range && rows[base + b] != 0xFFFFFFFFu) m |= 1ull << b;
-    bits[w] = m;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  const int lane = __lane_id();
+  for (uint64_t w = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); w < nw; w += waves) {
+    const uint64_t i = w * 64 + (uint64_t)lane;
+    const uint64_t m = __ballot(i < range && rows[i] != 0xFFFFFFFFu);
+    if (lane == 0) bits[w] = m;
   }
 }
 
@@ -1789,7 +1841,7 @@ void launch_dense_build(const uint64_t* keys, const uint8_t* key_null, int64_t n
                                                                                    (int64_t)((range + 1) / 2));
   dense_scatter_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, rows);
   dense_verify_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, rows, flag);
-  dense_bits_kernel<<<grid_for((int64_t)((range + 63) / 64), 256, 8192), 256, 0, s>>>(rows, range, bits);
+  dense_bits_kernel<<<grid_for((int64_t)range, 256, 8192), 256, 0, s>>>(rows, range, bits);
 }
 
 int64_t scan_u32_blocks(int64_t n) { return (n + SCAN_B - 1) / SCAN_B; }

@@ -219,6 +219,14 @@ struct Step {
   Prog keys[MAX_KEYS]; // PROBE: key word programs
 };
 
+constexpr int MAX_ATOMS = 4;
+struct FilterAtom {
+  const void* col;   // stage-0 column
+  uint64_t imm2;     // CMPC: the constant; LOOKUPC: the table
+  uint32_t op;       // OP_CMPC or OP_LOOKUPC
+  uint32_t flags;    // the instruction's flags
+};
+
 struct Plan {
   // source rows: i in [begin, end) of the selection (or of the rows)
   const uint32_t* sel;
@@ -302,6 +310,12 @@ struct Plan {
   int32_t pf_n;
   uint16_t pf_op[2];
   const void* pf_col[2];
+  // Segment 0 is one filter made only of fused column atoms (OP_CMPC /
+  // OP_LOOKUPC on stage 0) joined by AND / OR / NOT: the atoms are evaluated
+  // directly and the predicate is bit (atom outcomes) of atom_truth.
+  int32_t natoms;            // 0 = not this form
+  uint32_t atom_truth;       // bit m = predicate when atom i's outcome is bit i of m
+  FilterAtom atoms[MAX_ATOMS];
 };
 
 // Sortable key words for TopN.

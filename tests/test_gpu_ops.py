@@ -325,3 +325,24 @@ def test_join_dense_layout(gpu_engine, oracle_engine, case):
     got = gpu_engine.collect(probe, ops_g).to_host()
     assert want.num_rows > 10_000
     assert_tables_equal(got, want, rtol=0.0)
+
+
+FLOW_PREDICATES = PREDICATES + [
+    col(4).ge(date_lit(8200)) & col(4).lt(date_lit(8600)),                      # range: two atoms
+    (col(0).lt(lit(-500)) | col(0).gt(lit(500))) & ~col(5).eq(lit("apple")),    # 3 atoms, OR, NOT
+    col(1).ne(lit(0)) & col(2).ge(lit(-10.0)) & col(2).le(lit(10.0)) & col(6).eq(lit(2)),  # 4 atoms
+    col(2).gt(lit(float("nan"))) | col(2).ne(lit(float("nan"))),                # NaN compares
+    col(0).lt(lit(0)) & lit(True),                                              # a constant operand
+]
+
+
+@pytest.mark.parametrize("k", range(len(FLOW_PREDICATES)))
+def test_pipeline_filter_collect_matches_oracle(gpu_engine, oracle_engine, k):
+    """Filters as the first pipeline step (column-atom fast path or the
+    interpreter), collected through the wave-queued kernel in source order."""
+    t = mixed_table(n=70_000, seed=9)
+    ops = [abi.filter_op(FLOW_PREDICATES[k])]
+    want = oracle_engine.collect(t, ops).to_host()
+    got = gpu_engine.collect(t, ops).to_host()
+    assert_tables_equal(got, want, rtol=0.0)
+    assert gpu_engine.count(t, ops) == want.num_rows
