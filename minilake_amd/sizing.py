@@ -112,7 +112,10 @@ class Mlp:
         return last
 
     def __del__(self):
-        from . import gpu
+        try:
+            from . import gpu
+        except ImportError:  # interpreter shutdown
+            return
         if getattr(self, "handle", None) and gpu._lib is not None:
             gpu._lib.ml_mlp_free(self.handle)
             self.handle = None
