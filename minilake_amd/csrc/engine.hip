@@ -631,6 +631,7 @@ struct ml_hashagg {
   // hash
   int nkw = 1;
   uint64_t cap = 0;
+  uint64_t wrap = 0;   // probe wrap (cap - 1, or a partition's slots - 1 after a partitioned build)
   mlg::Buffer table;
   std::vector<mlg::Buffer> acc;
   mlg::Buffer flags;   // [0] group count (u64), [1] overflow bits (u32)
@@ -838,6 +839,7 @@ static void agg_reserve(ml_hashagg* A, uint64_t needed) {
   A->table = std::move(nt);
   A->acc = std::move(nacc);
   A->cap = ncap;
+  A->wrap = ncap - 1;
 }
 
 static uint64_t agg_len(ml_hashagg* A) {
@@ -857,6 +859,141 @@ static void agg_reset_table(ml_hashagg* A) {
   A->len_host = 0;
   A->len_known = true;
   zero(A->ctx, A->flags.ptr, 16);
+}
+
+// Partitioned aggregation of a first batch whose group count is known
+// (a capacity hint or the learned estimate): the table is cut into
+// partitions of S slots; the surviving rows' key and argument values are
+// materialized, scattered into partition order, and each partition is
+// aggregated in one workgroup's LDS (a partition with many rows is split into
+// chunks whose partial tables are merged), then written out in the table's
+// slot layout. No global atomics per row, and no grow-and-rehash. Returns
+// false when the pipeline or geometry does not fit (the caller then uses the
+// global table).
+constexpr uint64_t PART_AGG_MIN_GROUPS = 1 << 16;
+constexpr uint32_t PART_CHUNK_ROWS = 1 << 16;
+
+static bool agg_partitioned(ml_hashagg* A, PipelineBuild& B) {
+  ml_ctx* ctx = A->ctx;
+  Plan& P = B.P;
+  const int nkw = A->nkw, nacc = (int)A->slots.size(), nargs = P.nargs, nkeys = P.nkeys;
+  if (nkeys + nargs > MAX_COLS || nkeys < 1) return false;
+  const uint64_t cap = std::max<uint64_t>(1024, next_pow2((uint64_t)A->hint * 2));
+  uint64_t S = 2048;
+  while (S > 256 && S * (1 + nkw + nacc) * 8 > 65536) S >>= 1;
+  S = std::min(S, cap);
+  const uint64_t nparts = cap / S;
+  if (S * (1 + nkw + nacc) * 8 > 65536 || nparts > 8192 || cap >= (1ull << 32)) return false;
+
+  // 1. key and argument values of the surviving rows, in source order
+  P.ncols = nkeys + nargs;
+  std::vector<Buffer> vals, oks;
+  for (int c = 0; c < P.ncols; c++) {
+    const Prog& pg = c < nkeys ? P.keys[c] : P.args[c - nkeys];
+    P.cols[c] = pg;
+    P.col_width[c] = 8;
+  }
+  P.pair_out = nullptr;
+  FlowLaunch F;
+  if (!plan_flow(B, SINK_ROWS, sink_read_stages(B), 0, F)) {
+    P.ncols = 0;
+    return false;
+  }
+  const uint64_t cap_rows = (uint64_t)std::max<int64_t>(1, B.active_rows);
+  for (int c = 0; c < P.ncols; c++) {
+    vals.emplace_back(ctx->arena, cap_rows * 8);
+    oks.emplace_back();
+    if (P.cols[c].nullable) oks.back() = Buffer(ctx->arena, cap_rows);
+    P.col_out[c] = vals.back().ptr;
+    P.col_valid[c] = oks.back().ptr ? oks.back().as<uint8_t>() : nullptr;
+  }
+  const int64_t n = (int64_t)run_flow_rows(B, SINK_COLLECT, F);
+  P.ncols = 0;
+
+  // 2. partition order
+  AggRecords R{};
+  R.nkeys = nkeys;
+  R.nkw = nkw;
+  R.nargs = nargs;
+  R.key_nullable = A->key_nullable ? 1 : 0;
+  bool args_nullable = false;
+  for (int k = 0; k < nkeys; k++) {
+    R.key_val[k] = vals[k].as<uint64_t>();
+    R.key_ok[k] = oks[k].ptr ? oks[k].as<uint8_t>() : nullptr;
+  }
+  for (int a = 0; a < nargs; a++) {
+    R.arg_val[a] = vals[nkeys + a].as<uint64_t>();
+    R.arg_ok[a] = oks[nkeys + a].ptr ? oks[nkeys + a].as<uint8_t>() : nullptr;
+    args_nullable = args_nullable || R.arg_ok[a];
+  }
+  const int part_shift = __builtin_ctzll(S);
+  const int nblocks = (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + 16383) / 16384));
+  const int64_t rpb = std::max<int64_t>(1, (n + nblocks - 1) / nblocks);
+  const int64_t nh = (int64_t)nparts * nblocks;
+  Buffer hist(ctx->arena, (size_t)nh * 4), offsets(ctx->arena, (size_t)(nh + 1) * 4);
+  Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(nh) + 1) * 4);
+  Buffer rkeys(ctx->arena, (size_t)std::max<int64_t>(n, 1) * nkw * 8);
+  Buffer rargs(ctx->arena, (size_t)std::max<int64_t>(n, 1) * std::max(nargs, 1) * 8);
+  Buffer rvalid;
+  if (args_nullable) rvalid = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1) * 4);
+  launch_agg_part_hist(R, n, cap - 1, part_shift, nblocks, rpb, (int)nparts, hist.as<uint32_t>(), ctx->stream);
+  launch_scan_u32(hist.as<uint32_t>(), nh, offsets.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
+  launch_agg_part_scatter(R, n, cap - 1, part_shift, nblocks, rpb, (int)nparts, offsets.as<uint32_t>(),
+                          rkeys.as<uint64_t>(), rargs.as<uint64_t>(), rvalid.as<uint32_t>(), ctx->stream);
+  // partition starts (every nblocks-th offset, and the total)
+  Buffer starts(ctx->arena, (size_t)(nparts + 1) * 4);
+  launch_strided_u32(offsets.as<uint32_t>(), nblocks, (int64_t)nparts + 1, starts.as<uint32_t>(), ctx->stream);
+  std::vector<uint32_t> st((size_t)nparts + 1);
+  d2h(ctx, st.data(), starts.ptr, st.size() * 4);
+  vals.clear();
+  oks.clear();
+
+  // 3. chunks: one per partition, several for partitions with many rows
+  std::vector<PartChunk> chunks;
+  std::vector<uint32_t> heavy, first;
+  uint32_t q = 0;
+  for (uint32_t p = 0; p < nparts; p++) {
+    const uint32_t lo = st[p], hi = st[p + 1];
+    if (hi - lo <= PART_CHUNK_ROWS) {
+      chunks.push_back(PartChunk{p, lo, hi, 0xFFFFFFFFu});
+      continue;
+    }
+    heavy.push_back(p);
+    first.push_back(q);
+    for (uint32_t c = lo; c < hi; c += PART_CHUNK_ROWS) chunks.push_back(PartChunk{p, c, std::min(hi, c + PART_CHUNK_ROWS), q++});
+  }
+  first.push_back(q);
+  A->table = Buffer(ctx->arena, cap * (1 + nkw) * 8);
+  init_acc(A, A->acc, cap);
+  PartAggPlan G{};
+  G.S = (int32_t)S;
+  G.nkw = nkw;
+  G.nargs = nargs;
+  G.nacc = nacc;
+  G.mask = cap - 1;
+  for (int s2 = 0; s2 < nacc; s2++) {
+    G.acc[s2] = P.acc[s2];  // the compiled slots: `arg` indexes the (deduplicated) argument programs
+    G.acc_data[s2] = A->acc[s2].ptr;
+  }
+  G.slots = A->table.as<uint64_t>();
+  G.flags = (uint32_t*)A->flags.ptr;
+  Buffer dchunks(ctx->arena, chunks.size() * sizeof(PartChunk));
+  h2d(ctx, dchunks.ptr, chunks.data(), chunks.size() * sizeof(PartChunk));
+  Buffer partial(ctx->arena, std::max<size_t>(1, (size_t)q * S * (1 + nkw + nacc) * 8));
+  Buffer dheavy(ctx->arena, std::max<size_t>(1, heavy.size()) * 4), dfirst(ctx->arena, first.size() * 4);
+  if (!heavy.empty()) {
+    h2d(ctx, dheavy.ptr, heavy.data(), heavy.size() * 4);
+    h2d(ctx, dfirst.ptr, first.data(), first.size() * 4);
+  }
+  launch_agg_part_chunks(G, dchunks.as<PartChunk>(), (int)chunks.size(), rkeys.as<uint64_t>(), rargs.as<uint64_t>(),
+                         rvalid.as<uint32_t>(), n, partial.as<uint64_t>(), ctx->stream);
+  launch_agg_part_merge(G, dheavy.as<uint32_t>(), dfirst.as<uint32_t>(), (int)heavy.size(), partial.as<uint64_t>(),
+                        ctx->stream);
+  hip_check(hipStreamSynchronize(ctx->stream), "partitioned aggregation");
+  A->cap = cap;
+  A->wrap = S - 1;
+  A->len_known = false;
+  return true;
 }
 
 // One sink call: run the pipeline in B into the aggregate.
@@ -946,13 +1083,32 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
     B.P.end = e;
     B.P.agg_slots = A->table.as<uint64_t>();
     B.P.agg_mask = A->cap - 1;
+    B.P.agg_wrap = A->wrap;
     B.P.agg_fill_limit = fill_limit;
     for (size_t s = 0; s < A->slots.size(); s++) B.P.acc_data[s] = A->acc[s].ptr;
     FlowLaunch F;
-    if (plan_flow(B, SINK_HASH_AGG, sink_read_stages(B), 0, F)) ctx->launch(B.P, F.grid, FLOW_BLOCK, F.lds);
+    // accumulator cache: as many entries (a power of two, <= 1024) as fit 16 KB
+    const int per = 4 + 8 * (int)A->slots.size();
+    int acn = 1024;
+    while (acn > 32 && acn * per > 16384) acn >>= 1;
+    const bool cache = acn * per <= 16384 && (B.P.agg_mask + 1) <= 0xFFFFFFFEull;
+    B.P.ac_n = 0;
+    // the cache sits after the queues (the kernel keeps no cells for this sink);
+    // its bytes are counted in the launch's LDS so the grid matches occupancy
+    bool flow = cache && plan_flow(B, SINK_HASH_AGG, sink_read_stages(B), (size_t)acn * per, F);
+    if (flow) {
+      B.P.ac_n = acn;
+      B.P.ac_off = 2 * B.P.qwords;  // u64 index past 4 waves x qwords u32
+    } else {
+      flow = plan_flow(B, SINK_HASH_AGG, sink_read_stages(B), 0, F);
+    }
+    if (flow) ctx->launch(B.P, F.grid, FLOW_BLOCK, F.lds);
     else ctx->launch(B.P, B.grid(), block_of(ctx), 0);
     A->len_known = false;
   };
+  if (A->sizing != ML_SIZING_PLAIN && !A->cap && (uint64_t)A->hint >= PART_AGG_MIN_GROUPS &&
+      agg_partitioned(A, B))
+    return;
   if (A->sizing == ML_SIZING_PLAIN) {
     // reserve(len + batch) before every device batch, doubling as needed
     const int64_t batch = ctx->cfg.batch_rows > 0 ? ctx->cfg.batch_rows : (1 << 21);

@@ -76,11 +76,52 @@ void launch_join_part_build(const DevJoinTable& T, int nparts, const uint32_t* o
                             const uint32_t* srow, const uint64_t* skeys, uint64_t* bloom, uint32_t* flags,
                             size_t lds, hipStream_t s);
 // dense join tables: key range, then rows / presence bits (flag bit 0 = repeated key)
+void launch_strided_u32(const uint32_t* src, int64_t stride, int64_t n, uint32_t* dst, hipStream_t s);
 void launch_strided_u64(const uint64_t* src, int stride, int64_t n, uint64_t* dst, hipStream_t s);
 void launch_key_minmax(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t* partials, int nparts,
                        int64_t* mm, hipStream_t s);
 void launch_dense_build(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t range,
                         uint32_t* rows, uint64_t* bits, uint32_t* flag, hipStream_t s);
+// ------------------------------------------------- partitioned aggregation
+// Records of a hash aggregate, in partition order: key word w of record i at
+// keys[w * n + i], argument a at args[a * n + i], argument validity bits at
+// valid[i] (bit a) when any argument can be NULL.
+struct AggRecords {
+  int32_t nkeys;              // key columns (values materialized with validity)
+  int32_t nkw;                // key words: nkeys (+ 1 null-mask word)
+  int32_t nargs;
+  int32_t key_nullable;
+  const uint64_t* key_val[MAX_KEYS];
+  const uint8_t* key_ok[MAX_KEYS];   // u8 validity or null
+  const uint64_t* arg_val[MAX_AGG_SLOTS];
+  const uint8_t* arg_ok[MAX_AGG_SLOTS];
+};
+
+// One LDS table per partition: the geometry and accumulator plan.
+struct PartAggPlan {
+  int32_t S;                  // slots per partition (power of two)
+  int32_t nkw, nargs, nacc;
+  uint64_t mask;              // table capacity - 1 (capacity = partitions * S)
+  AccSlot acc[MAX_AGG_SLOTS];
+  uint64_t* slots;            // final table: [slot][1 + nkw]
+  void* acc_data[MAX_AGG_SLOTS];
+  uint32_t* flags;            // [0..1] group count (u64), [2] overflow bits
+};
+
+struct PartChunk {
+  uint32_t part, lo, hi, partial;  // partial = index of its partial table, or ~0 (final)
+};
+
+void launch_agg_part_hist(const AggRecords& R, int64_t n, uint64_t mask, int part_shift, int nblocks,
+                          int64_t rows_per_block, int nparts, uint32_t* hist, hipStream_t s);
+void launch_agg_part_scatter(const AggRecords& R, int64_t n, uint64_t mask, int part_shift, int nblocks,
+                             int64_t rows_per_block, int nparts, const uint32_t* offsets, uint64_t* rkeys,
+                             uint64_t* rargs, uint32_t* rvalid, hipStream_t s);
+void launch_agg_part_chunks(const PartAggPlan& A, const PartChunk* chunks, int nchunks, const uint64_t* rkeys,
+                            const uint64_t* rargs, const uint32_t* rvalid, int64_t n, uint64_t* partial, hipStream_t s);
+void launch_agg_part_merge(const PartAggPlan& A, const uint32_t* parts, const uint32_t* first, int nparts,
+                           const uint64_t* partial, hipStream_t s);
+size_t agg_part_lds(const PartAggPlan& A);
 // exclusive scan of n u32 counts (any n < 2^32 total); out[n] = total
 void launch_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, uint32_t* block_sums, hipStream_t s);
 int64_t scan_u32_blocks(int64_t n);

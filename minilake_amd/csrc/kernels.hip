@@ -464,7 +464,7 @@ __device__ __forceinline__ int64_t agg_slot(const Plan& P, const uint64_t (&w)[M
   const uint64_t stride = 1 + (uint64_t)nkw;
   uint64_t pos = h & P.agg_mask;
   uint64_t probes = 0;
-  while (probes <= P.agg_mask) {
+  while (probes <= P.agg_wrap) {
     uint64_t* s = P.agg_slots + pos * stride;
     uint64_t cur = __hip_atomic_load(s, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     if (cur == 0) {
@@ -489,7 +489,7 @@ __device__ __forceinline__ int64_t agg_slot(const Plan& P, const uint64_t (&w)[M
         if (k < nkw) eq = eq && (s[1 + k] == w[k]);
       if (eq) return (int64_t)pos;
     }
-    pos = (pos + 1) & P.agg_mask;
+    pos = (pos & ~P.agg_wrap) | ((pos + 1) & P.agg_wrap);
     probes++;
   }
   return -1;
@@ -531,6 +531,57 @@ __device__ __forceinline__ void sink_hash_agg(const Plan& P, const Code& ins, co
   }
   for (int s = 0; s < P.nacc; s++)
     if (P.acc[s].arg < 0) atomicAdd((unsigned long long*)P.acc_data[s] + slot, 1ull);
+}
+
+// Merge an accumulator state (not a row value) into a global state slot.
+__device__ __forceinline__ void acc_merge_global(int kind, void* data, int64_t slot, uint64_t v) {
+  switch (kind) {
+    case ACC_SUM_F64: unsafeAtomicAdd((double*)gpw<double>(data) + slot, u2f(v)); break;
+    case ACC_SUM_I64: case ACC_COUNT: atomicAdd((unsigned long long*)gpw<uint64_t>(data) + slot, (unsigned long long)v); break;
+    case ACC_MIN_I64: atomicMin((long long*)gpw<uint64_t>(data) + slot, (long long)v); break;
+    case ACC_MAX_I64: atomicMax((long long*)gpw<uint64_t>(data) + slot, (long long)v); break;
+    case ACC_MIN_F64: atomic_min_f64((double*)gpw<double>(data) + slot, u2f(v)); break;
+    case ACC_MAX_F64: atomic_max_f64((double*)gpw<double>(data) + slot, u2f(v)); break;
+    default: break;
+  }
+}
+
+// Hash aggregate through the workgroup's accumulator cache (ac: P.nacc arrays
+// of P.ac_n states, then P.ac_n tags = slot + 1, 0 = free).
+__device__ __forceinline__ void sink_hash_agg_cached(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES],
+                                                     uint64_t* ac) {
+  uint64_t w[MAX_KEYS + 1];
+  const int nkw = agg_keys(P, ins, rows, w);
+  const uint64_t h = key_hash(w, nkw);
+  const int64_t slot = agg_slot(P, w, nkw, h);
+  if (slot < 0) {
+    atomicOr(P.agg_overflow, 2u);
+    return;
+  }
+  uint32_t* tags = (uint32_t*)(ac + (uint64_t)P.ac_n * P.nacc);
+  const uint32_t ci = (uint32_t)slot & (uint32_t)(P.ac_n - 1);
+  const uint32_t want = (uint32_t)slot + 1u;
+  uint32_t tag = tags[ci];
+  if (tag == 0) {
+    const uint32_t prev = atomicCAS(&tags[ci], 0u, want);
+    tag = prev == 0 ? want : prev;
+  }
+  const bool cached = tag == want;
+  for (int a = 0; a < P.nargs; a++) {
+    uint64_t v;
+    const bool ok = eval(ins, P.args[a], rows, v);
+    if (!ok) continue;
+    for (int s = 0; s < P.nacc; s++)
+      if (P.acc[s].arg == a) {
+        if (cached) acc_apply_lds(P.acc[s].kind, ac + (uint64_t)s * P.ac_n + ci, v);
+        else acc_apply_global(P.acc[s].kind, P.acc_data[s], slot, v);
+      }
+  }
+  for (int s = 0; s < P.nacc; s++)
+    if (P.acc[s].arg < 0) {
+      if (cached) atomicAdd((unsigned long long*)(ac + (uint64_t)s * P.ac_n + ci), 1ull);
+      else atomicAdd((unsigned long long*)gpw<uint64_t>(P.acc_data[s]) + slot, 1ull);
+    }
 }
 
 __device__ __forceinline__ void sink_direct_agg(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES],
@@ -922,6 +973,16 @@ __global__ void FLOW_BOUNDS flow_kernel(Plan P) {
   const int lane = __lane_id();
   const int wv = threadIdx.x >> 6;
   uint32_t* q = (uint32_t*)(lds + ncells) + wv * P.qwords;
+  uint64_t* ac = nullptr;
+  if constexpr (SINK == SINK_HASH_AGG) {
+    if (P.ac_n) {
+      ac = lds + P.ac_off;
+      uint32_t* tags = (uint32_t*)(ac + (uint64_t)P.ac_n * P.nacc);
+      for (int i = threadIdx.x; i < P.ac_n * P.nacc; i += blockDim.x) ac[i] = acc_init_bits(P.acc[i / P.ac_n].kind);
+      for (int i = threadIdx.x; i < P.ac_n; i += blockDim.x) tags[i] = 0;
+      __syncthreads();
+    }
+  }
   const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
   int64_t next = P.begin + gw * P.chunk;
   const int64_t stop = min(P.end, next + P.chunk);
@@ -959,7 +1020,10 @@ __global__ void FLOW_BOUNDS flow_kernel(Plan P) {
       if constexpr (SINK == SINK_DIRECT_AGG) {
         if (alive) sink_direct_agg(P, ins, rows, lds);
       } else if constexpr (SINK == SINK_HASH_AGG) {
-        if (alive) sink_hash_agg(P, ins, rows);
+        if (alive) {
+          if (ac) sink_hash_agg_cached(P, ins, rows, ac);
+          else sink_hash_agg(P, ins, rows);
+        }
       } else {
         const uint64_t m = __ballot(alive);
         if constexpr (SINK == SINK_ROWS) {
@@ -1063,6 +1127,20 @@ __global__ void FLOW_BOUNDS flow_kernel(Plan P) {
   }
   if constexpr (SINK == SINK_ROWS || SINK == SINK_COUNT_PAIRS) {
     if (lane == 0) P.wave_counts[gw] = (uint32_t)nout;
+  }
+  if constexpr (SINK == SINK_HASH_AGG) {
+    if (ac) {  // merge the cached groups into the table
+      __syncthreads();
+      const uint32_t* tags = (const uint32_t*)(ac + (uint64_t)P.ac_n * P.nacc);
+      for (int i = threadIdx.x; i < P.ac_n; i += blockDim.x) {
+        const uint32_t t = tags[i];
+        if (!t) continue;
+        for (int s = 0; s < P.nacc; s++) {
+          const uint64_t v = ac[(uint64_t)s * P.ac_n + i];
+          if (v != acc_init_bits(P.acc[s].kind)) acc_merge_global(P.acc[s].kind, P.acc_data[s], (int64_t)(t - 1u), v);
+        }
+      }
+    }
   }
   if constexpr (SINK == SINK_DIRECT_AGG) {
     __syncthreads();
@@ -1561,6 +1639,204 @@ __global__ void flag_scatter_kernel(const uint8_t* flags, int64_t n, const uint6
   }
 }
 
+// --------------------------------------------- partitioned hash aggregation
+// Key words of materialized record i (NULL keys: word 0 plus the null-mask word).
+__device__ __forceinline__ void record_keys(const AggRecords& R, int64_t i, uint64_t (&w)[MAX_KEYS + 1]) {
+  uint64_t nullmask = 0;
+#pragma unroll
+  for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = 0;
+#pragma unroll
+  for (int k = 0; k < MAX_KEYS; k++) {
+    if (k >= R.nkeys) break;
+    const bool ok = !R.key_ok[k] || R.key_ok[k][i];
+    w[k] = ok ? R.key_val[k][i] : 0;
+    if (!ok) nullmask |= 1ull << k;
+  }
+  if (R.key_nullable) set_word(w, R.nkeys, nullmask);
+}
+
+__global__ void __launch_bounds__(256) agg_part_hist_kernel(AggRecords R, int64_t n, uint64_t mask, int part_shift,
+                                                            int64_t rows_per_block, int nparts, uint32_t* hist) {
+  extern __shared__ uint32_t cnt[];
+  for (int p = threadIdx.x; p < nparts; p += blockDim.x) cnt[p] = 0;
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * rows_per_block, hi = min(n, lo + rows_per_block);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    uint64_t w[MAX_KEYS + 1];
+    record_keys(R, i, w);
+    atomicAdd(&cnt[(key_hash(w, R.nkw) & mask) >> part_shift], 1u);
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < nparts; p += blockDim.x) hist[(uint64_t)p * gridDim.x + blockIdx.x] = cnt[p];
+}
+
+__global__ void __launch_bounds__(256) agg_part_scatter_kernel(AggRecords R, int64_t n, uint64_t mask, int part_shift,
+                                                               int64_t rows_per_block, int nparts,
+                                                               const uint32_t* offsets, uint64_t* rkeys,
+                                                               uint64_t* rargs, uint32_t* rvalid) {
+  extern __shared__ uint32_t cur[];
+  for (int p = threadIdx.x; p < nparts; p += blockDim.x) cur[p] = offsets[(uint64_t)p * gridDim.x + blockIdx.x];
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * rows_per_block, hi = min(n, lo + rows_per_block);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    uint64_t w[MAX_KEYS + 1];
+    record_keys(R, i, w);
+    const uint32_t at = atomicAdd(&cur[(key_hash(w, R.nkw) & mask) >> part_shift], 1u);
+    for (int k = 0; k < R.nkw; k++) rkeys[(uint64_t)k * n + at] = w[k];
+    uint32_t vm = 0;
+    for (int a = 0; a < R.nargs; a++) {
+      rargs[(uint64_t)a * n + at] = R.arg_val[a][i];
+      if (!R.arg_ok[a] || R.arg_ok[a][i]) vm |= 1u << a;
+    }
+    if (rvalid) rvalid[at] = vm;
+  }
+}
+
+// Merge an accumulator state into an LDS state word.
+__device__ __forceinline__ void acc_merge_lds(int kind, uint64_t* w, uint64_t v) {
+  if (kind == ACC_COUNT) atomicAdd((unsigned long long*)w, (unsigned long long)v);
+  else acc_apply_lds(kind, w, v);
+}
+
+// Find or insert the group of key words w (table hash h) in an LDS table of S
+// slots (headers hdr, keys kw). Returns the slot, or -1 when the table is full.
+__device__ __forceinline__ int lds_group(uint64_t* hdr, uint64_t* kw, int S, int nkw,
+                                         const uint64_t (&w)[MAX_KEYS + 1], uint64_t h) {
+  const uint64_t tag = (h & ~3ull) | 2ull;
+  int pos = (int)(h & (uint64_t)(S - 1));
+  int probes = 0;
+  while (probes < S) {
+    uint64_t c = __hip_atomic_load(&hdr[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (c == 0) {
+      const uint64_t prev = atomicCAS((unsigned long long*)&hdr[pos], 0ull, 1ull);
+      if (prev == 0) {
+        for (int k = 0; k < nkw; k++) kw[(uint64_t)pos * nkw + k] = w[k];
+        __hip_atomic_store(&hdr[pos], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return pos;
+      }
+      c = prev;
+    }
+    if (c == 1) continue;  // being written by another lane: read again
+    if (c == tag) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      bool eq = true;
+      for (int k = 0; k < nkw; k++) eq = eq && kw[(uint64_t)pos * nkw + k] == w[k];
+      if (eq) return pos;
+    }
+    pos = (pos + 1) & (S - 1);
+    probes++;
+  }
+  return -1;
+}
+
+// LDS table layout: hdr[S], kw[S * nkw], acc[nacc * S] (u64 each)
+size_t agg_part_lds(const PartAggPlan& A) { return (size_t)A.S * (1 + A.nkw + A.nacc) * 8; }
+
+// Write an LDS table to a table region (slots [base, base + S) of `slots` and
+// accumulator arrays `acc_data`); count its groups into flags.
+__device__ __forceinline__ void part_write(const PartAggPlan& A, const uint64_t* hdr, const uint64_t* kw,
+                                           const uint64_t* acc, uint64_t* slots, void* const* acc_data, uint64_t base,
+                                           bool count) {
+  const int S = A.S, stride = 1 + A.nkw;
+  uint32_t groups = 0;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    const uint64_t h = hdr[i];
+    groups += h != 0;
+    slots[(base + i) * stride] = h;
+    for (int k = 0; k < A.nkw; k++) slots[(base + i) * stride + 1 + k] = kw[(uint64_t)i * A.nkw + k];
+    for (int s2 = 0; s2 < A.nacc; s2++) ((uint64_t*)acc_data[s2])[base + i] = acc[(uint64_t)s2 * S + i];
+  }
+  if (count) {
+    for (int o = 32; o > 0; o >>= 1) groups += __shfl_down(groups, o);
+    if (__lane_id() == 0 && groups) atomicAdd((unsigned long long*)A.flags, (unsigned long long)groups);
+  }
+}
+
+__device__ __forceinline__ void part_init(const PartAggPlan& A, uint64_t* hdr, uint64_t* acc) {
+  for (int i = threadIdx.x; i < A.S; i += blockDim.x) hdr[i] = 0;
+  for (int i = threadIdx.x; i < A.S * A.nacc; i += blockDim.x) acc[i] = acc_init_bits(A.acc[i / A.S].kind);
+  __syncthreads();
+}
+
+// One workgroup per chunk of a partition's records.
+__global__ void __launch_bounds__(256) agg_part_chunk_kernel(PartAggPlan A, const PartChunk* chunks,
+                                                             const uint64_t* rkeys, const uint64_t* rargs,
+                                                             const uint32_t* rvalid, int64_t n, uint64_t* partial) {
+  extern __shared__ uint64_t lds[];
+  const PartChunk C = chunks[blockIdx.x];
+  const int S = A.S;
+  uint64_t* hdr = lds;
+  uint64_t* kw = lds + S;
+  uint64_t* acc = kw + (uint64_t)S * A.nkw;
+  part_init(A, hdr, acc);
+  bool full = false;
+  for (uint32_t e = C.lo + threadIdx.x; e < C.hi; e += blockDim.x) {
+    uint64_t w[MAX_KEYS + 1];
+#pragma unroll
+    for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < A.nkw ? rkeys[(uint64_t)k * n + e] : 0;
+    const int pos = lds_group(hdr, kw, S, A.nkw, w, key_hash(w, A.nkw));
+    if (pos < 0) {
+      full = true;
+      continue;
+    }
+    const uint32_t vm = rvalid ? rvalid[e] : 0xFFFFFFFFu;
+    for (int a = 0; a < A.nargs; a++) {
+      if (!((vm >> a) & 1u)) continue;  // NULL arguments are skipped by every accumulator
+      const uint64_t v = rargs[(uint64_t)a * n + e];
+      for (int s2 = 0; s2 < A.nacc; s2++)
+        if (A.acc[s2].arg == a) acc_apply_lds(A.acc[s2].kind, acc + (uint64_t)s2 * S + pos, v);
+    }
+    for (int s2 = 0; s2 < A.nacc; s2++)
+      if (A.acc[s2].arg < 0) atomicAdd((unsigned long long*)(acc + (uint64_t)s2 * S + pos), 1ull);
+  }
+  if (__any(full) && __lane_id() == 0) atomicOr(A.flags + 2, 2u);
+  __syncthreads();
+  if (C.partial == 0xFFFFFFFFu) {
+    part_write(A, hdr, kw, acc, A.slots, A.acc_data, (uint64_t)C.part * S, true);
+  } else {  // a partial table: [S] headers, [S * nkw] keys, [nacc * S] states
+    uint64_t* q = partial + (uint64_t)C.partial * S * (1 + A.nkw + A.nacc);
+    for (int i = threadIdx.x; i < S * (1 + A.nkw + A.nacc); i += blockDim.x) q[i] = lds[i];
+  }
+}
+
+// One workgroup per partition that was split into several chunks: merge the
+// chunks' partial tables first[j] .. first[j+1]-1 and write the partition.
+__global__ void __launch_bounds__(256) agg_part_merge_kernel(PartAggPlan A, const uint32_t* parts,
+                                                             const uint32_t* first, const uint64_t* partial) {
+  extern __shared__ uint64_t lds[];
+  const int S = A.S;
+  uint64_t* hdr = lds;
+  uint64_t* kw = lds + S;
+  uint64_t* acc = kw + (uint64_t)S * A.nkw;
+  part_init(A, hdr, acc);
+  bool full = false;
+  const size_t words = (size_t)S * (1 + A.nkw + A.nacc);
+  for (uint32_t q = first[blockIdx.x]; q < first[blockIdx.x + 1]; q++) {
+    const uint64_t* t = partial + (uint64_t)q * words;
+    const uint64_t* tk = t + S;
+    const uint64_t* ta = tk + (uint64_t)S * A.nkw;
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+      if (t[i] == 0) continue;
+      uint64_t w[MAX_KEYS + 1];
+#pragma unroll
+      for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < A.nkw ? tk[(uint64_t)i * A.nkw + k] : 0;
+      const int pos = lds_group(hdr, kw, S, A.nkw, w, key_hash(w, A.nkw));
+      if (pos < 0) {
+        full = true;
+        continue;
+      }
+      for (int s2 = 0; s2 < A.nacc; s2++) {
+        const uint64_t v = ta[(uint64_t)s2 * S + i];
+        if (v != acc_init_bits(A.acc[s2].kind)) acc_merge_lds(A.acc[s2].kind, acc + (uint64_t)s2 * S + pos, v);
+      }
+    }
+    __syncthreads();
+  }
+  if (__any(full) && __lane_id() == 0) atomicOr(A.flags + 2, 2u);
+  __syncthreads();
+  part_write(A, hdr, kw, acc, A.slots, A.acc_data, (uint64_t)parts[blockIdx.x] * S, true);
+}
+
 // ---------------------------------------------- exclusive scan of u32 counts
 // Three passes: per-workgroup sums of SCAN_B elements, one workgroup scanning
 // those sums, then each workgroup scanning its elements from its base.
@@ -1820,6 +2096,15 @@ void launch_join_part_build(const DevJoinTable& T, int nparts, const uint32_t* o
   join_part_build_kernel<<<nparts, 256, lds, s>>>(T, offsets, nblocks, srow, skeys, bloom, flags);
 }
 
+__global__ void strided_u32_kernel(const uint32_t* src, int64_t stride, int64_t n, uint32_t* dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i * stride];
+}
+
+void launch_strided_u32(const uint32_t* src, int64_t stride, int64_t n, uint32_t* dst, hipStream_t s) {
+  if (n > 0) strided_u32_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(src, stride, n, dst);
+}
+
 __global__ void strided_u64_kernel(const uint64_t* src, int stride, int64_t n, uint64_t* dst) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     dst[i] = src[i * stride];
@@ -1842,6 +2127,30 @@ void launch_dense_build(const uint64_t* keys, const uint8_t* key_null, int64_t n
   dense_scatter_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, rows);
   dense_verify_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, rows, flag);
   dense_bits_kernel<<<grid_for((int64_t)range, 256, 8192), 256, 0, s>>>(rows, range, bits);
+}
+
+void launch_agg_part_hist(const AggRecords& R, int64_t n, uint64_t mask, int part_shift, int nblocks,
+                          int64_t rows_per_block, int nparts, uint32_t* hist, hipStream_t s) {
+  agg_part_hist_kernel<<<nblocks, 256, (size_t)nparts * 4, s>>>(R, n, mask, part_shift, rows_per_block, nparts, hist);
+}
+
+void launch_agg_part_scatter(const AggRecords& R, int64_t n, uint64_t mask, int part_shift, int nblocks,
+                             int64_t rows_per_block, int nparts, const uint32_t* offsets, uint64_t* rkeys,
+                             uint64_t* rargs, uint32_t* rvalid, hipStream_t s) {
+  agg_part_scatter_kernel<<<nblocks, 256, (size_t)nparts * 4, s>>>(R, n, mask, part_shift, rows_per_block, nparts,
+                                                                   offsets, rkeys, rargs, rvalid);
+}
+
+void launch_agg_part_chunks(const PartAggPlan& A, const PartChunk* chunks, int nchunks, const uint64_t* rkeys,
+                            const uint64_t* rargs, const uint32_t* rvalid, int64_t n, uint64_t* partial, hipStream_t s) {
+  if (nchunks <= 0) return;
+  agg_part_chunk_kernel<<<nchunks, 256, agg_part_lds(A), s>>>(A, chunks, rkeys, rargs, rvalid, n, partial);
+}
+
+void launch_agg_part_merge(const PartAggPlan& A, const uint32_t* parts, const uint32_t* first, int nparts,
+                           const uint64_t* partial, hipStream_t s) {
+  if (nparts <= 0) return;
+  agg_part_merge_kernel<<<nparts, 256, agg_part_lds(A), s>>>(A, parts, first, partial);
 }
 
 int64_t scan_u32_blocks(int64_t n) { return (n + SCAN_B - 1) / SCAN_B; }

@@ -254,6 +254,7 @@ struct Plan {
   int32_t agg_nkw;
   int32_t nacc;
   uint64_t agg_mask;
+  uint64_t agg_wrap;         // linear probing wraps within agg_wrap + 1 slots (a partition or the table)
   uint64_t* agg_count;       // number of groups (device counter)
   uint32_t* agg_overflow;    // set when the table passes its fill limit
   uint64_t agg_fill_limit;
@@ -310,6 +311,12 @@ struct Plan {
   int32_t pf_n;
   uint16_t pf_op[2];
   const void* pf_col[2];
+  // Hash aggregate: per-workgroup LDS cache of group accumulators, keyed by
+  // global slot (ac_n entries, a power of two; 0 = off), at u64 offset ac_off
+  // of the dynamic LDS. Rows of cached groups update LDS; the cache is merged
+  // into the table once per workgroup, so hot groups cost no global atomics.
+  int32_t ac_n;
+  int32_t ac_off;
   // Segment 0 is one filter made only of fused column atoms (OP_CMPC /
   // OP_LOOKUPC on stage 0) joined by AND / OR / NOT: the atoms are evaluated
   // directly and the predicate is bit (atom outcomes) of atom_truth.

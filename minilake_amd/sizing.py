@@ -59,6 +59,7 @@ class Mlp:
         h = C.c_void_p()
         engine._check(lib().ml_mlp_create(engine.ctx, p.ctypes.data, C.byref(h)))
         self.handle = h.value
+        self._free = lib().ml_mlp_free  # bound now: module globals may be gone when __del__ runs at exit
 
     def params(self) -> np.ndarray:
         from .gpu import lib
@@ -111,14 +112,13 @@ class Mlp:
                 log(f"epoch {ep}: batch loss {last:.4f}")
         return last
 
-    def __del__(self):
-        try:
-            from . import gpu
-        except ImportError:  # interpreter shutdown
-            return
-        if getattr(self, "handle", None) and gpu._lib is not None:
-            gpu._lib.ml_mlp_free(self.handle)
+    def close(self):
+        if getattr(self, "handle", None):
+            self._free(self.handle)
             self.handle = None
+
+    def __del__(self):
+        self.close()
 
 
 def column_features(engine, keys_dev, n: int, seed: int = 0) -> np.ndarray:

@@ -346,3 +346,50 @@ def test_pipeline_filter_collect_matches_oracle(gpu_engine, oracle_engine, k):
     got = gpu_engine.collect(t, ops).to_host()
     assert_tables_equal(got, want, rtol=0.0)
     assert gpu_engine.count(t, ops) == want.num_rows
+
+
+def _agg_table(n, groups, seed, skew=False):
+    rng = np.random.default_rng(seed)
+    if skew:  # one key holds a third of the rows: its partition is split into chunks and merged
+        g = np.where(rng.random(n) < 0.33, 7, rng.integers(0, groups, n))
+    else:
+        g = rng.integers(0, groups, n)
+    k2 = (g % 3).astype(np.int32)
+    v = rng.random(n) * 100.0  # positive (prices): FP64 sums in any order agree to ~n ulp
+    x = rng.integers(-5, 5, n).astype(np.int64)
+    return HostTable([HostColumn(INT64, (g * 2654435761) % (1 << 40)), HostColumn(INT32, k2), HostColumn(FLOAT64, v),
+                      HostColumn(INT64, x)])
+
+
+@pytest.mark.parametrize("case", ["uniform", "skew", "nullable"])
+def test_hashagg_partitioned_matches_oracle(gpu_engine, oracle_engine, case):
+    """A group-count hint of >= 65,536 groups aggregates in LDS partitions:
+    every aggregate function, NULL keys / arguments (x / 0), and a hot key
+    whose partition is split into chunks, against the oracle."""
+    n, groups = 600_000, 150_000
+    t = _agg_table(n, groups, 5, skew=(case == "skew"))
+    keys = [col(0), col(1)]
+    ktypes = [INT64, INT32]
+    aggs = [Agg(AGG_COUNT_STAR), Agg(AGG_SUM, col(2), FLOAT64), Agg(AGG_MIN, col(3), INT64),
+            Agg(AGG_MAX, col(2), FLOAT64), Agg(AGG_AVG, col(3), INT64), Agg(AGG_COUNT, col(2), FLOAT64)]
+    if case == "nullable":
+        keys = [col(0), col(1) % col(3).cast(INT32)]               # NULL where x == 0
+        aggs = aggs + [Agg(AGG_SUM, col(2) / col(3).cast(FLOAT64), FLOAT64)]  # NULL where x == 0
+    want = oracle_engine.hashagg(t, [], keys, ktypes, aggs).to_host()
+    st = {}
+    got = gpu_engine.hashagg(t, [], keys, ktypes, aggs, sizing=abi.SIZING_HINT, capacity_hint=int(want.num_rows * 1.1),
+                             stats=st)
+    assert st["resizes"] == 0
+    assert_tables_equal(got.to_host(), want, rtol=1e-12)
+
+
+def test_hashagg_partitioned_underestimate(gpu_engine, oracle_engine):
+    """A hint far below the real group count overflows the partitions: HINT
+    reports ML_ERR_CAPACITY, LEARNED reruns with plain sizing and is exact."""
+    t = _agg_table(800_000, 600_000, 6)  # ~440k groups, far above a 256k-slot table
+    aggs = [Agg(AGG_COUNT_STAR), Agg(AGG_SUM, col(3), INT64)]
+    want = oracle_engine.hashagg(t, [], [col(0)], [INT64], aggs).to_host()
+    with pytest.raises(Exception):
+        gpu_engine.hashagg(t, [], [col(0)], [INT64], aggs, sizing=abi.SIZING_HINT, capacity_hint=70_000)
+    got = gpu_engine.hashagg(t, [], [col(0)], [INT64], aggs, sizing=abi.SIZING_LEARNED, capacity_hint=70_000)
+    assert_tables_equal(got.to_host(), want, rtol=0.0)
