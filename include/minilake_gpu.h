@@ -285,7 +285,9 @@ int64_t ml_join_table_num_rows(const ml_join_table* t);
 /* 1 if every build key is distinct (each probe row matches at most once). */
 int32_t ml_join_table_unique(const ml_join_table* t);
 /* The build side's columns in table row order (JoinTable.batch, join.rs:36-45);
- * build row indices returned by ml_join_probe index these. */
+ * build row indices returned by ml_join_probe index these. Row order is
+ * internal to the table: one unique integer key over a small range is stored
+ * in key order (rows with a NULL key last). */
 int32_t ml_join_table_num_columns(const ml_join_table* t);
 const ml_column* ml_join_table_column(const ml_join_table* t, int32_t i);
 void ml_join_table_free(ml_join_table* t);

@@ -251,8 +251,7 @@ struct JoinTableImpl {
   std::vector<Buffer> col_valid;
   Buffer slots;
   Buffer bloom;
-  Buffer dense_rows, dense_bits;  // dense layout (DevJoinTable::dense)
-  Buffer keys;                    // build key words [rows][nkw] (dense layout with several keys)
+  Buffer dense_dir;  // dense layout directory (DevJoinTable::dense)
   DevJoinTable dev{};
   std::vector<ml_column> views;
 };

@@ -1512,8 +1512,9 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   T.dev.nkw = (int)nk;
   uint32_t* flags = (uint32_t*)ctx->counters.as<uint64_t>();
   // Dense layout: one integer key whose values span at most
-  // max(8n, 2^20) consecutive integers, all distinct. A probe is then one
-  // presence bit and one row load, with no hashing or probing.
+  // max(8n, 2^20) consecutive integers, all distinct. The rows move into key
+  // order and a probe reads one directory entry (presence bits and rank of
+  // 64 keys), with no hashing, probing or row index array.
   const int kt0 = T.key_types[0];
   // (With several key columns the first alone would not reject rows the full
   // key does, and the hashed layout's Bloom filter does that better.)
@@ -1521,16 +1522,8 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
       (kt0 == ML_TYPE_INT32 || kt0 == ML_TYPE_INT64 || kt0 == ML_TYPE_DATE)) {
     const int nparts = (int)std::min<int64_t>(1024, (n + 4095) / 4096);
     Buffer partials(ctx->arena, (size_t)nparts * 16), mm(ctx->arena, 16);
-    // the first key word of every row, contiguous
-    Buffer k0;
-    const uint64_t* first = keys.as<uint64_t>();
-    if (nk > 1) {
-      k0 = Buffer(ctx->arena, (size_t)n * 8);
-      launch_strided_u64(keys.as<uint64_t>(), (int)nk, n, k0.as<uint64_t>(), ctx->stream);
-      first = k0.as<uint64_t>();
-    }
-    launch_key_minmax(first, key_null.as<uint8_t>(), n, partials.as<int64_t>(), nparts, mm.as<int64_t>(),
-                      ctx->stream);
+    launch_key_minmax(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, partials.as<int64_t>(), nparts,
+                      mm.as<int64_t>(), ctx->stream);
     int64_t hm[2] = {0, -1};
     d2h(ctx, hm, mm.ptr, 16);
     if (hm[0] <= hm[1]) {
@@ -1538,24 +1531,43 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
       const unsigned __int128 limit = std::max<uint64_t>((uint64_t)n * 8, 1ull << 20);
       if (range <= limit && range < ((unsigned __int128)1 << 31)) {
         const uint64_t R = (uint64_t)range;
-        Buffer rows(ctx->arena, ((R + 1) / 2) * 8), bits(ctx->arena, ((R + 63) / 64) * 8);
+        const int64_t ne = (int64_t)((R + 63) / 64);
+        Buffer dir(ctx->arena, (size_t)ne * sizeof(DenseDir));
+        zero(ctx, dir.ptr, dir.bytes);
         zero(ctx, flags, 8);
-        launch_dense_build(first, key_null.as<uint8_t>(), n, hm[0], R, rows.as<uint32_t>(),
-                           bits.as<uint64_t>(), flags, ctx->stream);
+        launch_dense_mark(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], dir.as<DenseDir>(), flags,
+                          ctx->stream);
         uint32_t hf = 0;
         d2h(ctx, &hf, flags, 4);
         if (!(hf & 1)) {
-          T.dense_rows = std::move(rows);
-          T.dense_bits = std::move(bits);
+          Buffer counts(ctx->arena, (size_t)ne * 4), offsets(ctx->arena, (size_t)(ne + 1) * 4);
+          Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(ne) + 1) * 4);
+          launch_dense_rank(dir.as<DenseDir>(), ne, counts.as<uint32_t>(), offsets.as<uint32_t>(),
+                            sums.as<uint32_t>(), ctx->stream);
+          DensePermute M{};
+          M.ncols = (int32_t)T.cols.size();
+          std::vector<Buffer> cols, valid;
+          for (size_t c = 0; c < T.cols.size(); c++) {
+            const int w = type_width(T.col_types[c]);
+            cols.emplace_back(ctx->arena, (size_t)n * w);
+            valid.emplace_back();
+            if (T.col_valid[c].ptr) valid.back() = Buffer(ctx->arena, (size_t)n);
+            M.width[c] = w;
+            M.src[c] = T.cols[c].ptr;
+            M.dst[c] = cols.back().ptr;
+            M.src_valid[c] = T.col_valid[c].ptr ? T.col_valid[c].as<uint8_t>() : nullptr;
+            M.dst_valid[c] = valid.back().ptr ? valid.back().as<uint8_t>() : nullptr;
+          }
+          zero(ctx, flags, 8);  // flags[1]: next NULL-key row
+          launch_dense_permute(M, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], dir.as<DenseDir>(),
+                               offsets.as<uint32_t>() + ne, flags + 1, ctx->stream);
+          T.cols = std::move(cols);
+          T.col_valid = std::move(valid);
+          T.dense_dir = std::move(dir);
           T.dev.dense = 1;
           T.dev.dense_min = hm[0];
           T.dev.dense_range = R;
-          T.dev.dense_rows = T.dense_rows.as<uint32_t>();
-          T.dev.dense_bits = T.dense_bits.as<uint64_t>();
-          if (nk > 1) {
-            T.keys = std::move(keys);
-            T.dev.dense_keys = T.keys.as<uint64_t>();
-          }
+          T.dev.dense_dir = T.dense_dir.as<DenseDir>();
           T.dev.unique = 1;
         }
       }

@@ -75,13 +75,26 @@ void launch_join_part_scatter(const DevJoinTable& T, const uint64_t* keys, const
 void launch_join_part_build(const DevJoinTable& T, int nparts, const uint32_t* offsets, int nblocks,
                             const uint32_t* srow, const uint64_t* skeys, uint64_t* bloom, uint32_t* flags,
                             size_t lds, hipStream_t s);
-// dense join tables: key range, then rows / presence bits (flag bit 0 = repeated key)
 void launch_strided_u32(const uint32_t* src, int64_t stride, int64_t n, uint32_t* dst, hipStream_t s);
-void launch_strided_u64(const uint64_t* src, int stride, int64_t n, uint64_t* dst, hipStream_t s);
+// dense join tables: key range; presence bits (flag bit 0 = repeated key);
+// ranks (offsets[ne] = present keys); rows moved into key order
 void launch_key_minmax(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t* partials, int nparts,
                        int64_t* mm, hipStream_t s);
-void launch_dense_build(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t range,
-                        uint32_t* rows, uint64_t* bits, uint32_t* flag, hipStream_t s);
+void launch_dense_mark(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, DenseDir* dir,
+                       uint32_t* flag, hipStream_t s);
+void launch_dense_rank(DenseDir* dir, int64_t ne, uint32_t* counts, uint32_t* offsets, uint32_t* block_sums,
+                       hipStream_t s);
+struct DensePermute {
+  int32_t ncols;
+  int32_t width[MAX_COLS];
+  const void* src[MAX_COLS];
+  void* dst[MAX_COLS];
+  const uint8_t* src_valid[MAX_COLS];
+  uint8_t* dst_valid[MAX_COLS];
+};
+void launch_dense_permute(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                          int64_t kmin, const DenseDir* dir, const uint32_t* present, uint32_t* null_next,
+                          hipStream_t s);
 // ------------------------------------------------- partitioned aggregation
 // Records of a hash aggregate, in partition order: key word w of record i at
 // keys[w * n + i], argument a at args[a * n + i], argument validity bits at

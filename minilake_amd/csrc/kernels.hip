@@ -287,12 +287,21 @@ __device__ __forceinline__ bool bloom_maybe(const DevJoinTable& T, uint64_t h) {
   return (T.bloom[bloom_word(T, h, g)] & bits) == bits;
 }
 
-// Dense table lookup of key word k.
+// Directory entry i of a dense table, one 16-byte load.
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ DenseDir dir_load(const DenseDir* d, uint64_t i) {
+  const u64x2 v = gp<u64x2>(d)[i];
+  return DenseDir{v.x, v.y};
+}
+
+// Dense table lookup of key word k: one directory entry gives presence and row.
 __device__ __forceinline__ bool dense_find(const DevJoinTable& T, uint64_t k, uint32_t& row) {
   const uint64_t idx = k - (uint64_t)T.dense_min;
   if (idx >= T.dense_range) return false;
-  if (!((gp<uint64_t>(T.dense_bits)[idx >> 6] >> (idx & 63)) & 1ull)) return false;
-  row = gp<uint32_t>(T.dense_rows)[idx];
+  const DenseDir e = dir_load(T.dense_dir, idx >> 6);
+  const uint32_t b = (uint32_t)(idx & 63);
+  if (!((e.bits >> b) & 1ull)) return false;
+  row = (uint32_t)e.rank + (uint32_t)__popcll(e.bits & ((1ull << b) - 1ull));
   return true;
 }
 
@@ -355,9 +364,7 @@ __device__ __forceinline__ bool run_steps(const Plan& P, const Code& ins, int fr
       const DevJoinTable& T = P.tables[st.table];
       if (T.dense) {
         uint32_t drow;
-        bool ok = dense_find(T, w[0], drow);
-        for (int k = 1; k < T.nkw && ok; k++) ok = gp<uint64_t>(T.dense_keys)[(uint64_t)drow * T.nkw + k] == w[k];
-        if (!ok) {
+        if (!dense_find(T, w[0], drow)) {
           alive = false;
           continue;
         }
@@ -799,33 +806,42 @@ __device__ __forceinline__ void round_rows(const Plan& P, int64_t next, int64_t 
 // Streaming first probe for FAST_U rows per lane: every row's key load, Bloom
 // word and slot are requested before any is used, so a lane keeps FAST_U
 // independent loads in flight per step.
-__device__ __forceinline__ void fast_probe(const Plan& P, int64_t next, int64_t stop, int lane, uint32_t (&src)[FAST_U],
-                                           uint32_t (&brow)[FAST_U], bool (&hit)[FAST_U]) {
+// The key words of a streaming-probe round (source rows next + u*64 + lane).
+struct FastKeys {
+  uint32_t src[FAST_U];
+  bool act[FAST_U];
+  uint64_t k0[FAST_U], k1[FAST_U];
+};
+
+__device__ __forceinline__ void fast_keys(const Plan& P, int64_t next, int64_t stop, int lane, FastKeys& K) {
+  round_rows(P, next, stop, lane, K.src, K.act);
+  load_rows(P.fast_op[0], P.fast_col[0], K.src, K.k0);
+  if (P.fast_nkeys > 1) load_rows(P.fast_op[1], P.fast_col[1], K.src, K.k1);
+}
+
+__device__ __forceinline__ void fast_probe(const Plan& P, const FastKeys& K, uint32_t (&brow)[FAST_U],
+                                           bool (&hit)[FAST_U]) {
   const DevJoinTable& T = P.tables[P.steps[0].table];
-  round_rows(P, next, stop, lane, src, hit);
-  uint64_t k0[FAST_U], k1[FAST_U], h[FAST_U];
-  load_rows(P.fast_op[0], P.fast_col[0], src, k0);
+  const auto& k0 = K.k0;
+  const auto& k1 = K.k1;
+  uint64_t h[FAST_U];
+#pragma unroll
+  for (int u = 0; u < FAST_U; u++) hit[u] = K.act[u];
   const bool two = P.fast_nkeys > 1;
-  if (two) load_rows(P.fast_op[1], P.fast_col[1], src, k1);
-  if (T.dense) {  // presence bits of all rows, then the rows of the present keys
-    uint64_t idx[FAST_U], bw[FAST_U];
+  if (T.dense) {  // one directory entry per row: presence bit and row
+    uint64_t idx[FAST_U];
+    DenseDir e[FAST_U];
 #pragma unroll
     for (int u = 0; u < FAST_U; u++) {
       idx[u] = k0[u] - (uint64_t)T.dense_min;
       hit[u] = hit[u] && idx[u] < T.dense_range;
-      bw[u] = gp<uint64_t>(T.dense_bits)[hit[u] ? (idx[u] >> 6) : 0];
+      e[u] = dir_load(T.dense_dir, hit[u] ? (idx[u] >> 6) : 0);
     }
 #pragma unroll
     for (int u = 0; u < FAST_U; u++) {
-      hit[u] = hit[u] && ((bw[u] >> (idx[u] & 63)) & 1ull);
-      brow[u] = gp<uint32_t>(T.dense_rows)[hit[u] ? idx[u] : 0];
-    }
-    if (two) {  // the second key word must match the build row's
-      uint64_t kb[FAST_U];
-#pragma unroll
-      for (int u = 0; u < FAST_U; u++) kb[u] = gp<uint64_t>(T.dense_keys)[(uint64_t)(hit[u] ? brow[u] : 0) * 2 + 1];
-#pragma unroll
-      for (int u = 0; u < FAST_U; u++) hit[u] = hit[u] && kb[u] == k1[u];
+      const uint32_t b = (uint32_t)(idx[u] & 63);
+      hit[u] = hit[u] && ((e[u].bits >> b) & 1ull);
+      brow[u] = (uint32_t)e[u].rank + (uint32_t)__popcll(e[u].bits & ((1ull << b) - 1ull));
     }
     return;
   }
@@ -955,6 +971,9 @@ __device__ __forceinline__ void load_stages(const uint32_t* base, uint64_t strid
 #ifndef ML_PF_COLS
 #define ML_PF_COLS 2     // stage-0 columns loaded ahead at level 0
 #endif
+#ifndef ML_FAST_PREFETCH
+#define ML_FAST_PREFETCH 0  // streaming probe: next round's keys requested before this round's probe
+#endif
 #if ML_FLOW_WAVES > 0
 #define FLOW_BOUNDS __launch_bounds__(256, ML_FLOW_WAVES)
 #else
@@ -990,6 +1009,10 @@ __global__ void FLOW_BOUNDS flow_kernel(Plan P) {
   uint64_t qh = 0, qn = 0;
   const int last = P.nseg - 1;
   uint64_t nout = 0;
+#if ML_FAST_PREFETCH
+  FastKeys fk;
+  if (P.fast && next < stop) fast_keys(P, next, stop, lane, fk);
+#endif
   for (;;) {
     int lvl = -1;
 #pragma unroll
@@ -1038,17 +1061,27 @@ __global__ void FLOW_BOUNDS flow_kernel(Plan P) {
 
     if (lvl == 0 && P.fast) {
       // FAST_U * 64 source rows through the streaming probe; hits to level 1 in source order
-      uint32_t src[FAST_U], brow[FAST_U];
+      uint32_t brow[FAST_U];
       bool hit[FAST_U];
-      fast_probe(P, next, stop, lane, src, brow, hit);
+#if ML_FAST_PREFETCH
+      // this round's keys were requested by the previous level-0 round (or
+      // before the loop); request the next round's before probing this one
+      const FastKeys K = fk;
       next = min(stop, next + FAST_U * 64);
+      if (next < stop) fast_keys(P, next, stop, lane, fk);
+#else
+      FastKeys K;
+      fast_keys(P, next, stop, lane, K);
+      next = min(stop, next + FAST_U * 64);
+#endif
+      fast_probe(P, K, brow, hit);
       const int stage = P.steps[0].stage;
 #pragma unroll
       for (int u = 0; u < FAST_U; u++) {
         uint32_t rows[MAX_STAGES];
 #pragma unroll
         for (int s2 = 0; s2 < MAX_STAGES; s2++) rows[s2] = 0;
-        rows[0] = src[u];
+        rows[0] = K.src[u];
         set_row(rows, stage, brow[u]);
         push(1, hit[u], rows);
       }
@@ -1346,32 +1379,76 @@ __global__ void __launch_bounds__(256) key_minmax_final_kernel(const int64_t* pa
   }
 }
 
-// rows[k - min] = r for every non-NULL key (one writer wins for a repeated key)
-__global__ void dense_scatter_kernel(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin,
-                                     uint32_t* rows) {
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
-    if (!key_null[r]) rows[(uint64_t)((int64_t)keys[r] - kmin)] = (uint32_t)r;
-}
-
-// flag bit 0: a key whose slot holds another row (a repeated key)
-__global__ void dense_verify_kernel(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin,
-                                    const uint32_t* rows, uint32_t* flag) {
-  bool dup = false;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
-    if (!key_null[r] && rows[(uint64_t)((int64_t)keys[r] - kmin)] != (uint32_t)r) dup = true;
-  if (__any(dup) && __lane_id() == 0) atomicOr(flag, 1u);
-}
-
-// presence bits from the row array (empty = 0xFFFFFFFF): one wave per 64
-// entries, read coalesced, the word is the wave's ballot
-__global__ void dense_bits_kernel(const uint32_t* rows, uint64_t range, uint64_t* bits) {
-  const uint64_t nw = (range + 63) / 64;
-  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+// Presence bits of the non-NULL keys (key k -> bit k - kmin of the
+// directory); flag bit 0: a key seen twice. Lanes whose keys fall in the same
+// directory word (runs of neighbouring lanes, as sorted keys give) OR their
+// bits together first, so each run costs one atomic.
+__global__ void dense_mark_kernel(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin,
+                                  DenseDir* dir, uint32_t* flag) {
   const int lane = __lane_id();
-  for (uint64_t w = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); w < nw; w += waves) {
-    const uint64_t i = w * 64 + (uint64_t)lane;
-    const uint64_t m = __ballot(i < range && rows[i] != 0xFFFFFFFFu);
-    if (lane == 0) bits[w] = m;
+  bool dup = false;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
+    const int64_t r = base + lane;
+    uint64_t w = ~0ull, v = 0;
+    if (r < n && !key_null[r]) {
+      const uint64_t i = (uint64_t)((int64_t)keys[r] - kmin);
+      w = i >> 6;
+      v = 1ull << (i & 63);
+    }
+    // segmented suffix OR over runs of equal words: afterwards the first lane
+    // of a run holds the run's bits
+    const uint64_t wn = __shfl_down(w, 1);
+    int end = (lane == 63 || wn != w) ? 1 : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t vn = __shfl_down(v, d);
+      const int en = __shfl_down(end, d);
+      if (!end && lane + d < 64) {
+        dup = dup || (v & vn) != 0;
+        v |= vn;
+        end = en;
+      }
+    }
+    const uint64_t wp = __shfl_up(w, 1);
+    if ((lane == 0 || wp != w) && w != ~0ull) {
+      const unsigned long long old = atomicOr((unsigned long long*)&dir[w].bits, (unsigned long long)v);
+      dup = dup || (old & v) != 0;
+    }
+  }
+  if (__any(dup) && lane == 0) atomicOr(flag, 1u);
+}
+
+__global__ void dense_count_kernel(const DenseDir* dir, int64_t ne, uint32_t* counts) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x)
+    counts[e] = (uint32_t)__popcll(dir[e].bits);
+}
+
+__global__ void dense_rank_kernel(DenseDir* dir, int64_t ne, const uint32_t* offsets) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x)
+    dir[e].rank = offsets[e];
+}
+
+// Build rows into key order: a row with key k moves to the rank of k; rows
+// with a NULL key (never matched) follow the present keys in any order.
+__global__ void dense_permute_kernel(DensePermute M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                                     int64_t kmin, const DenseDir* dir, const uint32_t* present, uint32_t* null_next) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t pos;
+    if (!key_null[r]) {
+      const uint64_t i = (uint64_t)((int64_t)keys[r] - kmin);
+      const DenseDir e = dir[i >> 6];
+      pos = e.rank + (uint64_t)__popcll(e.bits & ((1ull << (i & 63)) - 1ull));
+    } else {
+      pos = (uint64_t)*present + atomicAdd(null_next, 1u);
+    }
+    for (int c = 0; c < M.ncols; c++) {
+      const int w = M.width[c];
+      if (w == 8) ((uint64_t*)M.dst[c])[pos] = ((const uint64_t*)M.src[c])[r];
+      else if (w == 4) ((uint32_t*)M.dst[c])[pos] = ((const uint32_t*)M.src[c])[r];
+      else ((uint8_t*)M.dst[c])[pos] = ((const uint8_t*)M.src[c])[r];
+      if (M.src_valid[c]) M.dst_valid[c][pos] = M.src_valid[c][r];
+    }
   }
 }
 
@@ -1956,7 +2033,8 @@ __global__ void sort_keys_kernel(Plan P, SortKeys K, int64_t n, uint64_t* words)
 }
 
 // Bitonic-sort tiles of TILE entries (nw+1 words each) in LDS; keep the first
-// `keep` entries of every tile.
+// `keep` entries of every tile. A short tile sorts only the power of two that
+// covers its entries (the +inf padding beyond them would stay in place).
 __global__ void __launch_bounds__(256) topn_tile_kernel(const uint64_t* in, int64_t n, int nw, int keep,
                                                          uint64_t* out) {
   extern __shared__ uint64_t sh[];
@@ -1968,9 +2046,11 @@ __global__ void __launch_bounds__(256) topn_tile_kernel(const uint64_t* in, int6
       sh[e * W + k] = e < cnt ? in[(uint64_t)(tile0 + e) * W + k] : ~0ull;
   }
   __syncthreads();
-  for (int size = 2; size <= TOPN_TILE; size <<= 1) {
+  int span = 2;
+  while (span < cnt) span <<= 1;
+  for (int size = 2; size <= span; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int t = threadIdx.x; t < TOPN_TILE / 2; t += blockDim.x) {
+      for (int t = threadIdx.x; t < span / 2; t += blockDim.x) {
         const int lo = 2 * stride * (t / stride) + (t % stride);
         const int hi = lo + stride;
         const bool up = ((lo & size) == 0);
@@ -2105,28 +2185,29 @@ void launch_strided_u32(const uint32_t* src, int64_t stride, int64_t n, uint32_t
   if (n > 0) strided_u32_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(src, stride, n, dst);
 }
 
-__global__ void strided_u64_kernel(const uint64_t* src, int stride, int64_t n, uint64_t* dst) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = src[i * stride];
-}
-
-void launch_strided_u64(const uint64_t* src, int stride, int64_t n, uint64_t* dst, hipStream_t s) {
-  if (n > 0) strided_u64_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(src, stride, n, dst);
-}
-
 void launch_key_minmax(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t* partials, int nparts,
                        int64_t* mm, hipStream_t s) {
   key_minmax_kernel<<<nparts, 256, 0, s>>>(keys, key_null, n, partials);
   key_minmax_final_kernel<<<1, 256, 0, s>>>(partials, nparts, mm);
 }
 
-void launch_dense_build(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t range,
-                        uint32_t* rows, uint64_t* bits, uint32_t* flag, hipStream_t s) {
-  fill_u64_kernel<<<grid_for((int64_t)((range + 1) / 2), 256, 8192), 256, 0, s>>>((uint64_t*)rows, ~0ull,
-                                                                                   (int64_t)((range + 1) / 2));
-  dense_scatter_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, rows);
-  dense_verify_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, rows, flag);
-  dense_bits_kernel<<<grid_for((int64_t)range, 256, 8192), 256, 0, s>>>(rows, range, bits);
+void launch_dense_mark(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, DenseDir* dir,
+                       uint32_t* flag, hipStream_t s) {
+  if (n > 0) dense_mark_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, dir, flag);
+}
+
+void launch_dense_rank(DenseDir* dir, int64_t ne, uint32_t* counts, uint32_t* offsets, uint32_t* block_sums,
+                       hipStream_t s) {
+  dense_count_kernel<<<grid_for(ne, 256, 8192), 256, 0, s>>>(dir, ne, counts);
+  launch_scan_u32(counts, ne, offsets, block_sums, s);
+  dense_rank_kernel<<<grid_for(ne, 256, 8192), 256, 0, s>>>(dir, ne, offsets);
+}
+
+void launch_dense_permute(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                          int64_t kmin, const DenseDir* dir, const uint32_t* present, uint32_t* null_next,
+                          hipStream_t s) {
+  if (n > 0)
+    dense_permute_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(M, keys, key_null, n, kmin, dir, present, null_next);
 }
 
 void launch_agg_part_hist(const AggRecords& R, int64_t n, uint64_t mask, int part_shift, int nblocks,

@@ -133,6 +133,13 @@ struct Prog {
 // probe starts at slot hash & mask and wraps inside its partition. A table
 // that could not be partitioned has one partition (part_shift = log2 capacity).
 // The blocked Bloom filter has 2^bloom_shift words per partition.
+// Directory entry of a dense join table: 64 consecutive keys per entry, read
+// with one 16-byte load.
+struct alignas(16) DenseDir {
+  uint64_t bits;  // bit j: key (entry base + j) is present
+  uint64_t rank;  // present keys below the entry base
+};
+
 struct DevJoinTable {
   uint64_t* slots = nullptr;
   uint64_t mask = 0;       // capacity - 1
@@ -143,17 +150,16 @@ struct DevJoinTable {
   int32_t bloom_shift = 0; // log2 Bloom words per partition
   const uint64_t* bloom = nullptr;  // blocked Bloom filter (one u64 per block) or null
   uint64_t bloom_mask = 0; // Bloom words per partition - 1
-  // Dense layout (the first key word is an integer, unique, and its values
-  // span a small range): the only candidate row for first key word k is
-  // dense_rows[k - dense_min] when k - dense_min < dense_range and bit
-  // (k - dense_min) of dense_bits is set; further key words are compared with
-  // dense_keys[row * nkw + j]. Slots and Bloom filter are then unused.
+  // Dense layout (one integer key, unique, its values over a small range):
+  // the table's rows are in key order, so the row of key k is the number of
+  // present keys below it. With i = k - dense_min < dense_range and
+  // e = dense_dir[i / 64], k is present iff bit i % 64 of e.bits is set, and
+  // its row is e.rank + popcount(e.bits & ((1 << i % 64) - 1)). Slots and
+  // Bloom filter are then unused.
   int32_t dense = 0;
   int64_t dense_min = 0;
   uint64_t dense_range = 0;
-  const uint32_t* dense_rows = nullptr;
-  const uint64_t* dense_bits = nullptr;
-  const uint64_t* dense_keys = nullptr;
+  const DenseDir* dense_dir = nullptr;
 };
 
 // next slot of a linear probe (inside the partition)
