@@ -251,7 +251,7 @@ struct JoinTableImpl {
   std::vector<Buffer> col_valid;
   Buffer slots;
   Buffer bloom;
-  Buffer dense_dir;  // dense layout directory (DevJoinTable::dense)
+  Buffer dense_bits, dense_rank;  // dense layout (DevJoinTable::dense)
   DevJoinTable dev{};
   std::vector<ml_column> views;
 };

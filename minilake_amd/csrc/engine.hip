@@ -1513,8 +1513,8 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   uint32_t* flags = (uint32_t*)ctx->counters.as<uint64_t>();
   // Dense layout: one integer key whose values span at most
   // max(8n, 2^20) consecutive integers, all distinct. The rows move into key
-  // order and a probe reads one directory entry (presence bits and rank of
-  // 64 keys), with no hashing, probing or row index array.
+  // order; a probe reads one presence bit and, for a present key, the rank
+  // of its 64-key word: no hashing, probing or row index array.
   const int kt0 = T.key_types[0];
   // (With several key columns the first alone would not reject rows the full
   // key does, and the hashed layout's Bloom filter does that better.)
@@ -1532,17 +1532,17 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
       if (range <= limit && range < ((unsigned __int128)1 << 31)) {
         const uint64_t R = (uint64_t)range;
         const int64_t ne = (int64_t)((R + 63) / 64);
-        Buffer dir(ctx->arena, (size_t)ne * sizeof(DenseDir));
-        zero(ctx, dir.ptr, dir.bytes);
+        Buffer bits(ctx->arena, (size_t)ne * 8);
+        zero(ctx, bits.ptr, bits.bytes);
         zero(ctx, flags, 8);
-        launch_dense_mark(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], dir.as<DenseDir>(), flags,
+        launch_dense_mark(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], bits.as<uint64_t>(), flags,
                           ctx->stream);
         uint32_t hf = 0;
         d2h(ctx, &hf, flags, 4);
         if (!(hf & 1)) {
-          Buffer counts(ctx->arena, (size_t)ne * 4), offsets(ctx->arena, (size_t)(ne + 1) * 4);
+          Buffer counts(ctx->arena, (size_t)ne * 4), rank(ctx->arena, (size_t)(ne + 1) * 4);
           Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(ne) + 1) * 4);
-          launch_dense_rank(dir.as<DenseDir>(), ne, counts.as<uint32_t>(), offsets.as<uint32_t>(),
+          launch_dense_rank(bits.as<uint64_t>(), ne, counts.as<uint32_t>(), rank.as<uint32_t>(),
                             sums.as<uint32_t>(), ctx->stream);
           DensePermute M{};
           M.ncols = (int32_t)T.cols.size();
@@ -1559,15 +1559,17 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
             M.dst_valid[c] = valid.back().ptr ? valid.back().as<uint8_t>() : nullptr;
           }
           zero(ctx, flags, 8);  // flags[1]: next NULL-key row
-          launch_dense_permute(M, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], dir.as<DenseDir>(),
-                               offsets.as<uint32_t>() + ne, flags + 1, ctx->stream);
+          launch_dense_permute(M, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], bits.as<uint64_t>(),
+                               rank.as<uint32_t>(), rank.as<uint32_t>() + ne, flags + 1, ctx->stream);
           T.cols = std::move(cols);
           T.col_valid = std::move(valid);
-          T.dense_dir = std::move(dir);
+          T.dense_bits = std::move(bits);
+          T.dense_rank = std::move(rank);
           T.dev.dense = 1;
           T.dev.dense_min = hm[0];
           T.dev.dense_range = R;
-          T.dev.dense_dir = T.dense_dir.as<DenseDir>();
+          T.dev.dense_bits = T.dense_bits.as<uint64_t>();
+          T.dev.dense_rank = T.dense_rank.as<uint32_t>();
           T.dev.unique = 1;
         }
       }

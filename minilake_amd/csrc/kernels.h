@@ -80,9 +80,10 @@ void launch_strided_u32(const uint32_t* src, int64_t stride, int64_t n, uint32_t
 // ranks (offsets[ne] = present keys); rows moved into key order
 void launch_key_minmax(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t* partials, int nparts,
                        int64_t* mm, hipStream_t s);
-void launch_dense_mark(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, DenseDir* dir,
+void launch_dense_mark(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t* bits,
                        uint32_t* flag, hipStream_t s);
-void launch_dense_rank(DenseDir* dir, int64_t ne, uint32_t* counts, uint32_t* offsets, uint32_t* block_sums,
+// rank[w] = set bits in words below w, rank[nw] = all set bits
+void launch_dense_rank(const uint64_t* bits, int64_t nw, uint32_t* counts, uint32_t* rank, uint32_t* block_sums,
                        hipStream_t s);
 struct DensePermute {
   int32_t ncols;
@@ -93,8 +94,8 @@ struct DensePermute {
   uint8_t* dst_valid[MAX_COLS];
 };
 void launch_dense_permute(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
-                          int64_t kmin, const DenseDir* dir, const uint32_t* present, uint32_t* null_next,
-                          hipStream_t s);
+                          int64_t kmin, const uint64_t* bits, const uint32_t* rank, const uint32_t* present,
+                          uint32_t* null_next, hipStream_t s);
 // ------------------------------------------------- partitioned aggregation
 // Records of a hash aggregate, in partition order: key word w of record i at
 // keys[w * n + i], argument a at args[a * n + i], argument validity bits at

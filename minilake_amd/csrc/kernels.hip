@@ -287,21 +287,14 @@ __device__ __forceinline__ bool bloom_maybe(const DevJoinTable& T, uint64_t h) {
   return (T.bloom[bloom_word(T, h, g)] & bits) == bits;
 }
 
-// Directory entry i of a dense table, one 16-byte load.
-typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ DenseDir dir_load(const DenseDir* d, uint64_t i) {
-  const u64x2 v = gp<u64x2>(d)[i];
-  return DenseDir{v.x, v.y};
-}
-
-// Dense table lookup of key word k: one directory entry gives presence and row.
+// Dense table lookup of key word k: presence bit, then the rank of its word.
 __device__ __forceinline__ bool dense_find(const DevJoinTable& T, uint64_t k, uint32_t& row) {
   const uint64_t idx = k - (uint64_t)T.dense_min;
   if (idx >= T.dense_range) return false;
-  const DenseDir e = dir_load(T.dense_dir, idx >> 6);
+  const uint64_t bits = gp<uint64_t>(T.dense_bits)[idx >> 6];
   const uint32_t b = (uint32_t)(idx & 63);
-  if (!((e.bits >> b) & 1ull)) return false;
-  row = (uint32_t)e.rank + (uint32_t)__popcll(e.bits & ((1ull << b) - 1ull));
+  if (!((bits >> b) & 1ull)) return false;
+  row = gp<uint32_t>(T.dense_rank)[idx >> 6] + (uint32_t)__popcll(bits & ((1ull << b) - 1ull));
   return true;
 }
 
@@ -828,21 +821,23 @@ __device__ __forceinline__ void fast_probe(const Plan& P, const FastKeys& K, uin
 #pragma unroll
   for (int u = 0; u < FAST_U; u++) hit[u] = K.act[u];
   const bool two = P.fast_nkeys > 1;
-  if (T.dense) {  // one directory entry per row: presence bit and row
-    uint64_t idx[FAST_U];
-    DenseDir e[FAST_U];
+  if (T.dense) {  // presence bits of all rows, then the ranks of the present keys' words
+    uint64_t idx[FAST_U], bw[FAST_U];
 #pragma unroll
     for (int u = 0; u < FAST_U; u++) {
       idx[u] = k0[u] - (uint64_t)T.dense_min;
       hit[u] = hit[u] && idx[u] < T.dense_range;
-      e[u] = dir_load(T.dense_dir, hit[u] ? (idx[u] >> 6) : 0);
+      bw[u] = gp<uint64_t>(T.dense_bits)[hit[u] ? (idx[u] >> 6) : 0];
     }
+    uint32_t rk[FAST_U];
 #pragma unroll
     for (int u = 0; u < FAST_U; u++) {
-      const uint32_t b = (uint32_t)(idx[u] & 63);
-      hit[u] = hit[u] && ((e[u].bits >> b) & 1ull);
-      brow[u] = (uint32_t)e[u].rank + (uint32_t)__popcll(e[u].bits & ((1ull << b) - 1ull));
+      hit[u] = hit[u] && ((bw[u] >> (idx[u] & 63)) & 1ull);
+      rk[u] = gp<uint32_t>(T.dense_rank)[hit[u] ? (idx[u] >> 6) : 0];
     }
+#pragma unroll
+    for (int u = 0; u < FAST_U; u++)
+      brow[u] = rk[u] + (uint32_t)__popcll(bw[u] & ((1ull << (idx[u] & 63)) - 1ull));
     return;
   }
 #pragma unroll
@@ -1379,12 +1374,12 @@ __global__ void __launch_bounds__(256) key_minmax_final_kernel(const int64_t* pa
   }
 }
 
-// Presence bits of the non-NULL keys (key k -> bit k - kmin of the
-// directory); flag bit 0: a key seen twice. Lanes whose keys fall in the same
+// Presence bits of the non-NULL keys (key k -> bit k - kmin); flag bit 0: a
+// key seen twice. Lanes whose keys fall in the same
 // directory word (runs of neighbouring lanes, as sorted keys give) OR their
 // bits together first, so each run costs one atomic.
 __global__ void dense_mark_kernel(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin,
-                                  DenseDir* dir, uint32_t* flag) {
+                                  uint64_t* bits, uint32_t* flag) {
   const int lane = __lane_id();
   bool dup = false;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1412,33 +1407,28 @@ __global__ void dense_mark_kernel(const uint64_t* keys, const uint8_t* key_null,
     }
     const uint64_t wp = __shfl_up(w, 1);
     if ((lane == 0 || wp != w) && w != ~0ull) {
-      const unsigned long long old = atomicOr((unsigned long long*)&dir[w].bits, (unsigned long long)v);
+      const unsigned long long old = atomicOr((unsigned long long*)&bits[w], (unsigned long long)v);
       dup = dup || (old & v) != 0;
     }
   }
   if (__any(dup) && lane == 0) atomicOr(flag, 1u);
 }
 
-__global__ void dense_count_kernel(const DenseDir* dir, int64_t ne, uint32_t* counts) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x)
-    counts[e] = (uint32_t)__popcll(dir[e].bits);
-}
-
-__global__ void dense_rank_kernel(DenseDir* dir, int64_t ne, const uint32_t* offsets) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x)
-    dir[e].rank = offsets[e];
+__global__ void dense_count_kernel(const uint64_t* bits, int64_t nw, uint32_t* counts) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nw; e += (int64_t)gridDim.x * blockDim.x)
+    counts[e] = (uint32_t)__popcll(bits[e]);
 }
 
 // Build rows into key order: a row with key k moves to the rank of k; rows
 // with a NULL key (never matched) follow the present keys in any order.
 __global__ void dense_permute_kernel(DensePermute M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
-                                     int64_t kmin, const DenseDir* dir, const uint32_t* present, uint32_t* null_next) {
+                                     int64_t kmin, const uint64_t* bits, const uint32_t* rank, const uint32_t* present,
+                                     uint32_t* null_next) {
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     uint64_t pos;
     if (!key_null[r]) {
       const uint64_t i = (uint64_t)((int64_t)keys[r] - kmin);
-      const DenseDir e = dir[i >> 6];
-      pos = e.rank + (uint64_t)__popcll(e.bits & ((1ull << (i & 63)) - 1ull));
+      pos = rank[i >> 6] + (uint64_t)__popcll(bits[i >> 6] & ((1ull << (i & 63)) - 1ull));
     } else {
       pos = (uint64_t)*present + atomicAdd(null_next, 1u);
     }
@@ -2191,23 +2181,23 @@ void launch_key_minmax(const uint64_t* keys, const uint8_t* key_null, int64_t n,
   key_minmax_final_kernel<<<1, 256, 0, s>>>(partials, nparts, mm);
 }
 
-void launch_dense_mark(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, DenseDir* dir,
+void launch_dense_mark(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t* bits,
                        uint32_t* flag, hipStream_t s) {
-  if (n > 0) dense_mark_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, dir, flag);
+  if (n > 0) dense_mark_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, bits, flag);
 }
 
-void launch_dense_rank(DenseDir* dir, int64_t ne, uint32_t* counts, uint32_t* offsets, uint32_t* block_sums,
+void launch_dense_rank(const uint64_t* bits, int64_t nw, uint32_t* counts, uint32_t* rank, uint32_t* block_sums,
                        hipStream_t s) {
-  dense_count_kernel<<<grid_for(ne, 256, 8192), 256, 0, s>>>(dir, ne, counts);
-  launch_scan_u32(counts, ne, offsets, block_sums, s);
-  dense_rank_kernel<<<grid_for(ne, 256, 8192), 256, 0, s>>>(dir, ne, offsets);
+  dense_count_kernel<<<grid_for(nw, 256, 8192), 256, 0, s>>>(bits, nw, counts);
+  launch_scan_u32(counts, nw, rank, block_sums, s);
 }
 
 void launch_dense_permute(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
-                          int64_t kmin, const DenseDir* dir, const uint32_t* present, uint32_t* null_next,
-                          hipStream_t s) {
+                          int64_t kmin, const uint64_t* bits, const uint32_t* rank, const uint32_t* present,
+                          uint32_t* null_next, hipStream_t s) {
   if (n > 0)
-    dense_permute_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(M, keys, key_null, n, kmin, dir, present, null_next);
+    dense_permute_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(M, keys, key_null, n, kmin, bits, rank, present,
+                                                                 null_next);
 }
 
 void launch_agg_part_hist(const AggRecords& R, int64_t n, uint64_t mask, int part_shift, int nblocks,

@@ -133,13 +133,6 @@ struct Prog {
 // probe starts at slot hash & mask and wraps inside its partition. A table
 // that could not be partitioned has one partition (part_shift = log2 capacity).
 // The blocked Bloom filter has 2^bloom_shift words per partition.
-// Directory entry of a dense join table: 64 consecutive keys per entry, read
-// with one 16-byte load.
-struct alignas(16) DenseDir {
-  uint64_t bits;  // bit j: key (entry base + j) is present
-  uint64_t rank;  // present keys below the entry base
-};
-
 struct DevJoinTable {
   uint64_t* slots = nullptr;
   uint64_t mask = 0;       // capacity - 1
@@ -152,14 +145,15 @@ struct DevJoinTable {
   uint64_t bloom_mask = 0; // Bloom words per partition - 1
   // Dense layout (one integer key, unique, its values over a small range):
   // the table's rows are in key order, so the row of key k is the number of
-  // present keys below it. With i = k - dense_min < dense_range and
-  // e = dense_dir[i / 64], k is present iff bit i % 64 of e.bits is set, and
-  // its row is e.rank + popcount(e.bits & ((1 << i % 64) - 1)). Slots and
-  // Bloom filter are then unused.
+  // present keys below it. With i = k - dense_min < dense_range, k is present
+  // iff bit i % 64 of dense_bits[i / 64] is set, and its row is
+  // dense_rank[i / 64] + popcount(dense_bits[i / 64] & ((1 << i % 64) - 1)).
+  // Slots and Bloom filter are then unused.
   int32_t dense = 0;
   int64_t dense_min = 0;
   uint64_t dense_range = 0;
-  const DenseDir* dense_dir = nullptr;
+  const uint64_t* dense_bits = nullptr;  // one bit per key of the range
+  const uint32_t* dense_rank = nullptr;  // present keys below each word
 };
 
 // next slot of a linear probe (inside the partition)
