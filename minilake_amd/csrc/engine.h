@@ -108,6 +108,20 @@ struct HostDict {
 
 // ------------------------------------------------------------------ context
 struct MlpModel;
+
+// Pinned host words that receive the flags of checks left running on the
+// stream (a join build's duplicate-key or partition-overflow test), so the
+// host need not wait for them when the build is launched. Shared by a
+// context and the tables it built.
+struct CheckPool {
+  static constexpr int WORDS = 1024;
+  uint32_t* words = nullptr;
+  std::vector<int> free;
+  CheckPool();
+  ~CheckPool();
+  int take();  // -1 when every word is in use
+  void give(int w) { if (w >= 0) free.push_back(w); }
+};
 }  // namespace mlg
 
 struct ml_ctx {
@@ -121,10 +135,14 @@ struct ml_ctx {
   mlg::MlpModel* estimator = nullptr;
   // small per-context device scratch: counters and flags
   mlg::Buffer counters;  // 64 x u64
+  static constexpr int DONE_WORD = 63;  // counters word: workgroups finished (kept at 0 between launches)
   // pinned host staging for small device-to-host reads (counts, flags), so
   // they skip the pageable-memory staging path of hipMemcpy
   void* pinned = nullptr;
   static constexpr size_t PINNED_BYTES = 4096;
+  std::shared_ptr<mlg::CheckPool> checks;
+  uint64_t check_seq = 0;   // deferred checks enqueued so far
+  uint64_t synced_seq = 0;  // deferred checks the stream has completed (seen at a synchronization)
   std::map<const ml_dict*, std::shared_ptr<mlg::HostDict>> dicts;
   std::shared_ptr<mlg::HostDict> dict(const ml_dict* d);
   // launch statistics
@@ -254,14 +272,32 @@ struct JoinTableImpl {
   Buffer dense_bits, dense_rank;  // dense layout (DevJoinTable::dense)
   DevJoinTable dev{};
   std::vector<ml_column> views;
+  // A build whose last check still runs on the stream: its flags arrive in
+  // checks->words[check]; join_table_resolve finishes the table before its
+  // layout is used on the host.
+  int pending = 0;  // PENDING_*
+  int check = -1;
+  uint64_t check_seq = 0;
+  std::shared_ptr<CheckPool> checks;
+  Buffer keys, key_null;                      // key words, for a fallback build
+  std::vector<Buffer> orig_cols, orig_valid;  // dense: rows in build order
 };
+enum : int { PENDING_NONE = 0, PENDING_DENSE = 1, PENDING_PARTS = 2 };
 
 }  // namespace mlg
 
 struct ml_join_table {
   ml_ctx* ctx = nullptr;
   mlg::JoinTableImpl impl;
+  ~ml_join_table() {
+    if (impl.checks) impl.checks->give(impl.check);
+  }
 };
+
+namespace mlg {
+// Finish a table whose build check is pending (waits for the stream once).
+void join_table_resolve(ml_join_table* t);
+}  // namespace mlg
 
 // -------------------------------------------------------------- the pipeline
 namespace mlg {

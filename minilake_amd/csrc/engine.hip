@@ -59,18 +59,48 @@ DeviceArena::~DeviceArena() {
   for (auto& kv : live_) (void)hipFree(kv.first);
 }
 
+CheckPool::CheckPool() {
+  hip_check(hipHostMalloc((void**)&words, WORDS * sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
+  for (int w = WORDS - 1; w >= 0; w--) free.push_back(w);
+}
+CheckPool::~CheckPool() {
+  if (words) (void)hipHostFree(words);
+}
+int CheckPool::take() {
+  if (free.empty()) return -1;
+  const int w = free.back();
+  free.pop_back();
+  return w;
+}
+
 static void h2d(ml_ctx* ctx, void* dst, const void* src, size_t n) {
   if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, ctx->stream), "copy to device");
 }
 static void d2h(ml_ctx* ctx, void* dst, const void* src, size_t n) {
+  const uint64_t seq = ctx->check_seq;
   if (n && ctx->pinned && n <= ml_ctx::PINNED_BYTES) {
     hip_check(hipMemcpyAsync(ctx->pinned, src, n, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
     hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
     std::memcpy(dst, ctx->pinned, n);
-    return;
+  } else {
+    if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
+    hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
   }
-  if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
-  hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+  ctx->synced_seq = seq;  // every deferred check enqueued before this copy has landed
+}
+
+// Enqueue a copy of the 4-byte flag word `src` into a pinned check word of
+// T, to be read by join_table_resolve. False when no check word is free
+// (the caller then reads the flag synchronously).
+static bool defer_check(ml_ctx* ctx, JoinTableImpl& T, const uint32_t* src, int kind) {
+  const int w = ctx->checks ? ctx->checks->take() : -1;
+  if (w < 0) return false;
+  hip_check(hipMemcpyAsync(ctx->checks->words + w, src, 4, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
+  T.checks = ctx->checks;
+  T.check = w;
+  T.check_seq = ++ctx->check_seq;
+  T.pending = kind;
+  return true;
 }
 static void d2d(ml_ctx* ctx, void* dst, const void* src, size_t n) {
   if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, ctx->stream), "device copy");
@@ -156,6 +186,7 @@ void PipelineBuild::add_ops(const ml_operator* ops, int n) {
       case ML_PIPE_PROBE: {
         const ml_join_table* t = op.table;
         if (!t) fail(ML_ERR_INTERNAL, "probe before build");
+        join_table_resolve(const_cast<ml_join_table*>(t));  // the layout is final from here on
         const JoinTableImpl& J = t->impl;
         if (op.num_keys != (int)J.key_types.size()) fail(ML_ERR_PLAN, "probe and build key counts differ");
         if (P.nsteps >= MAX_STEPS || nstages >= MAX_STAGES) fail(ML_ERR_UNSUPPORTED, "too many joins in one pipeline");
@@ -1386,23 +1417,59 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
 // Open-addressing table at load <= ML_JOIN_LOAD_PCT percent, built in LDS
 // partitions when they fit (see join_part_build_kernel), else with global
 // atomics; with a blocked Bloom filter from 64K rows (or as configured).
+static uint64_t hashed_capacity(int64_t n) {
+  return std::max<uint64_t>(16, next_pow2((uint64_t)((n * 100 + ML_JOIN_LOAD_PCT - 1) / ML_JOIN_LOAD_PCT)));
+}
+static bool hashed_bloom(ml_ctx* ctx, int64_t n) {
+  const int use_bloom = ctx->cfg.use_bloom;
+  return use_bloom > 0 || (use_bloom < 0 && n >= (1 << 16));
+}
+// one 64-bit block (3 bits set per key) per ML_BLOOM_KEYS_PER_WORD keys
+static uint64_t hashed_bloom_words(int64_t n) {
+  return std::max<uint64_t>(64, next_pow2((uint64_t)n / ML_BLOOM_KEYS_PER_WORD + 1));
+}
+
+// The global-atomic build (also the fallback when a partition overflows).
+static void build_hashed_global(ml_ctx* ctx, JoinTableImpl& T, const Buffer& keys, const Buffer& key_null, int64_t n,
+                                uint32_t* flags) {
+  const uint64_t cap = T.dev.mask + 1;
+  T.dev.wrap = cap - 1;
+  T.dev.part_shift = __builtin_ctzll(cap);
+  zero(ctx, flags, 8);
+  zero(ctx, T.slots.ptr, T.slots.bytes);
+  T.bloom = Buffer();
+  T.dev.bloom = nullptr;
+  if (hashed_bloom(ctx, n)) {
+    const uint64_t bloom_words = hashed_bloom_words(n);
+    T.bloom = Buffer(ctx->arena, bloom_words * 8);
+    zero(ctx, T.bloom.ptr, T.bloom.bytes);
+    T.dev.bloom = T.bloom.as<uint64_t>();
+    T.dev.bloom_shift = __builtin_ctzll(bloom_words);
+    T.dev.bloom_mask = bloom_words - 1;
+  }
+  launch_join_insert(T.dev, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, T.bloom.as<uint64_t>(), ctx->stream);
+  launch_join_unique(T.dev, flags, ctx->stream);
+  uint32_t hdup = 0;
+  d2h(ctx, &hdup, flags, 4);
+  T.dev.unique = hdup ? 0 : 1;
+}
+
+// `defer`: leave the partitioned build's check on the stream (the table is
+// then pending and keeps `keys`/`key_null` for a fallback).
 static void build_hashed(ml_ctx* ctx, JoinTableImpl& T, Buffer& keys, Buffer& key_null, int64_t n, size_t nk,
-                         uint32_t* flags) {
-  const uint64_t cap = std::max<uint64_t>(16, next_pow2((uint64_t)((n * 100 + ML_JOIN_LOAD_PCT - 1) / ML_JOIN_LOAD_PCT)));
+                         uint32_t* flags, bool defer) {
+  const uint64_t cap = hashed_capacity(n);
   T.dev.mask = cap - 1;
   T.slots = Buffer(ctx->arena, cap * (1 + nk) * 8);
   T.dev.slots = T.slots.as<uint64_t>();
-  const int use_bloom = ctx->cfg.use_bloom;
-  const bool bloom = use_bloom > 0 || (use_bloom < 0 && n >= (1 << 16));
-  // one 64-bit block (3 bits set per key) per ML_BLOOM_KEYS_PER_WORD keys
-  uint64_t bloom_words = bloom ? std::max<uint64_t>(64, next_pow2((uint64_t)n / ML_BLOOM_KEYS_PER_WORD + 1)) : 0;
+  const bool bloom = hashed_bloom(ctx, n);
+  const uint64_t bloom_words = bloom ? hashed_bloom_words(n) : 0;
   zero(ctx, flags, 8);
   // Partitioned build: partitions of S slots, each built by one workgroup in
   // LDS (no global atomics), S chosen so a partition's slots fit in 32 KB.
   const uint64_t smax = nk == 1 ? 2048 : (nk <= 3 ? 1024 : 512);
   const uint64_t S = std::min(cap, smax);
   const uint64_t nparts = cap / S;
-  bool built = false;
   if (n > 0 && nparts <= 8192 && (uint64_t)n < 0xFFFFFFFFull) {
     T.dev.wrap = S - 1;
     T.dev.part_shift = __builtin_ctzll(S);
@@ -1428,34 +1495,70 @@ static void build_hashed(ml_ctx* ctx, JoinTableImpl& T, Buffer& keys, Buffer& ke
     launch_join_part_build(T.dev, (int)nparts, offsets.as<uint32_t>(), nblocks, srow.as<uint32_t>(),
                            skeys.as<uint64_t>(), T.dev.bloom ? T.bloom.as<uint64_t>() : nullptr, flags,
                            S * (1 + nk) * 8 + W * 8, ctx->stream);
+    if (defer && defer_check(ctx, T, flags, PENDING_PARTS)) {
+      T.dev.unique = 1;  // settled by join_table_resolve
+      T.keys = std::move(keys);
+      T.key_null = std::move(key_null);
+      return;
+    }
     uint32_t hf = 0;
     d2h(ctx, &hf, flags, 4);
     if (!(hf & 2)) {
       T.dev.unique = (hf & 1) ? 0 : 1;
-      built = true;
-    } else {
-      zero(ctx, flags, 8);  // a partition overflowed (many equal keys): build unpartitioned
+      return;
     }
+    // a partition overflowed (many equal keys): build unpartitioned
   }
-  if (!built) {
-    T.dev.wrap = cap - 1;
-    T.dev.part_shift = __builtin_ctzll(cap);
-    zero(ctx, T.slots.ptr, T.slots.bytes);
-    T.bloom = Buffer();
-    T.dev.bloom = nullptr;
-    if (bloom) {
-      T.bloom = Buffer(ctx->arena, bloom_words * 8);
-      zero(ctx, T.bloom.ptr, T.bloom.bytes);
-      T.dev.bloom = T.bloom.as<uint64_t>();
-      T.dev.bloom_shift = __builtin_ctzll(bloom_words);
-      T.dev.bloom_mask = bloom_words - 1;
-    }
-    launch_join_insert(T.dev, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, T.bloom.as<uint64_t>(), ctx->stream);
-    launch_join_unique(T.dev, flags, ctx->stream);
-    uint32_t hdup = 0;
-    d2h(ctx, &hdup, flags, 4);
-    T.dev.unique = hdup ? 0 : 1;
+  build_hashed_global(ctx, T, keys, key_null, n, flags);
+}
+
+static void refresh_views(JoinTableImpl& T) {
+  T.views.resize(T.col_types.size());
+  for (size_t c = 0; c < T.col_types.size(); c++) {
+    ml_column v{};
+    v.type = T.col_types[c];
+    v.data = T.cols[c].ptr;
+    v.len = T.nrows;
+    v.dict = T.col_dicts[c] ? &T.col_dicts[c]->view : nullptr;
+    T.views[c] = v;
   }
+}
+
+void join_table_resolve(ml_join_table* t) {
+  JoinTableImpl& T = t->impl;
+  if (!T.pending) return;
+  ml_ctx* ctx = t->ctx;
+  if (T.check_seq > ctx->synced_seq) {
+    const uint64_t seq = ctx->check_seq;
+    hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+    ctx->synced_seq = seq;
+  }
+  const uint32_t hf = T.checks->words[T.check];
+  T.checks->give(T.check);
+  T.check = -1;
+  const int kind = T.pending;
+  T.pending = PENDING_NONE;
+  uint32_t* flags = (uint32_t*)ctx->counters.as<uint64_t>();
+  const size_t nk = T.key_types.size();
+  if (kind == PENDING_DENSE && (hf & 5)) {
+    // a repeated key (or one outside the range): back to build order and the hashed layout
+    T.cols = std::move(T.orig_cols);
+    T.col_valid = std::move(T.orig_valid);
+    T.dense_bits = Buffer();
+    T.dense_rank = Buffer();
+    T.dev.dense = 0;
+    T.dev.dense_bits = nullptr;
+    T.dev.dense_rank = nullptr;
+    refresh_views(T);
+    build_hashed(ctx, T, T.keys, T.key_null, T.nrows, nk, flags, false);
+  } else if (kind == PENDING_PARTS) {
+    if (hf & 2) build_hashed_global(ctx, T, T.keys, T.key_null, T.nrows, flags);
+    else T.dev.unique = (hf & 1) ? 0 : 1;
+  }
+  T.keys = Buffer();
+  T.key_null = Buffer();
+  T.orig_cols.clear();
+  T.orig_valid.clear();
 }
 
 static ml_join_table* join_build_finalize(ml_join_build* J) {
@@ -1522,8 +1625,10 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
       (kt0 == ML_TYPE_INT32 || kt0 == ML_TYPE_INT64 || kt0 == ML_TYPE_DATE)) {
     const int nparts = (int)std::min<int64_t>(1024, (n + 4095) / 4096);
     Buffer partials(ctx->arena, (size_t)nparts * 16), mm(ctx->arena, 16);
+    // (also clears flags[0] for the presence marking and flags[1] for the permutation)
     launch_key_minmax(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, partials.as<int64_t>(), nparts,
-                      mm.as<int64_t>(), ctx->stream);
+                      mm.as<int64_t>(), (uint32_t*)(ctx->counters.as<uint64_t>() + ml_ctx::DONE_WORD), flags,
+                      ctx->stream);
     int64_t hm[2] = {0, -1};
     d2h(ctx, hm, mm.ptr, 16);
     if (hm[0] <= hm[1]) {
@@ -1534,16 +1639,17 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
         const int64_t ne = (int64_t)((R + 63) / 64);
         Buffer bits(ctx->arena, (size_t)ne * 8);
         zero(ctx, bits.ptr, bits.bytes);
-        zero(ctx, flags, 8);
-        launch_dense_mark(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], bits.as<uint64_t>(), flags,
+        launch_dense_mark(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(), flags,
                           ctx->stream);
+        // the repeated-key flag is read when the table is first used; the
+        // rank and permutation run meanwhile (a repeated key discards them)
+        const bool deferred = defer_check(ctx, T, flags, PENDING_DENSE);
         uint32_t hf = 0;
-        d2h(ctx, &hf, flags, 4);
-        if (!(hf & 1)) {
-          Buffer counts(ctx->arena, (size_t)ne * 4), rank(ctx->arena, (size_t)(ne + 1) * 4);
+        if (!deferred) d2h(ctx, &hf, flags, 4);
+        if (!(hf & 5)) {
+          Buffer rank(ctx->arena, (size_t)(ne + 1) * 4);
           Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(ne) + 1) * 4);
-          launch_dense_rank(bits.as<uint64_t>(), ne, counts.as<uint32_t>(), rank.as<uint32_t>(),
-                            sums.as<uint32_t>(), ctx->stream);
+          launch_dense_rank(bits.as<uint64_t>(), ne, rank.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
           DensePermute M{};
           M.ncols = (int32_t)T.cols.size();
           std::vector<Buffer> cols, valid;
@@ -1558,9 +1664,15 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
             M.src_valid[c] = T.col_valid[c].ptr ? T.col_valid[c].as<uint8_t>() : nullptr;
             M.dst_valid[c] = valid.back().ptr ? valid.back().as<uint8_t>() : nullptr;
           }
-          zero(ctx, flags, 8);  // flags[1]: next NULL-key row
-          launch_dense_permute(M, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], bits.as<uint64_t>(),
+          // flags[1] (next NULL-key row) was cleared by the min/max kernel
+          launch_dense_permute(M, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(),
                                rank.as<uint32_t>(), rank.as<uint32_t>() + ne, flags + 1, ctx->stream);
+          if (deferred) {
+            T.orig_cols = std::move(T.cols);
+            T.orig_valid = std::move(T.col_valid);
+            T.keys = std::move(keys);
+            T.key_null = std::move(key_null);
+          }
           T.cols = std::move(cols);
           T.col_valid = std::move(valid);
           T.dense_bits = std::move(bits);
@@ -1575,15 +1687,8 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
       }
     }
   }
-  if (!T.dev.dense) build_hashed(ctx, T, keys, key_null, n, nk, flags);
-  for (size_t c = 0; c < T.col_types.size(); c++) {
-    ml_column v{};
-    v.type = T.col_types[c];
-    v.data = T.cols[c].ptr;
-    v.len = n;
-    v.dict = T.col_dicts[c] ? &T.col_dicts[c]->view : nullptr;
-    T.views.push_back(v);
-  }
+  if (!T.dev.dense) build_hashed(ctx, T, keys, key_null, n, nk, flags, true);
+  refresh_views(T);
   return t;
 }
 
@@ -1859,6 +1964,7 @@ ml_status ml_ctx_create(const ml_exec_config* cfg, ml_ctx** out) {
     ctx->counters = Buffer(ctx->arena, 64 * 8);
     zero(ctx, ctx->counters.ptr, 64 * 8);
     hip_check(hipHostMalloc(&ctx->pinned, ml_ctx::PINNED_BYTES, hipHostMallocDefault), "hipHostMalloc");
+    ctx->checks = std::make_shared<CheckPool>();
     hip_check(hipStreamSynchronize(ctx->stream), "sync");
   });
   if (st != ML_OK) {
@@ -1876,6 +1982,7 @@ void ml_ctx_destroy(ml_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->counters.reset();
   ctx->arena.reset();
+  ctx->checks.reset();
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -2188,10 +2295,15 @@ ml_status ml_join_build_finalize(ml_join_build* b, ml_join_table** out) {
 }
 
 int64_t ml_join_table_num_rows(const ml_join_table* t) { return t ? t->impl.nrows : 0; }
-int32_t ml_join_table_unique(const ml_join_table* t) { return t ? t->impl.dev.unique : 0; }
+// Settle a pending build check for an accessor without a status code.
+static bool resolved(const ml_join_table* t) {
+  if (!t) return false;
+  return guarded(t->ctx, [&] { join_table_resolve(const_cast<ml_join_table*>(t)); }) == ML_OK;
+}
+int32_t ml_join_table_unique(const ml_join_table* t) { return resolved(t) ? t->impl.dev.unique : 0; }
 int32_t ml_join_table_num_columns(const ml_join_table* t) { return t ? (int32_t)t->impl.col_types.size() : 0; }
 const ml_column* ml_join_table_column(const ml_join_table* t, int32_t i) {
-  if (!t || i < 0 || (size_t)i >= t->impl.views.size()) return nullptr;
+  if (!resolved(t) || i < 0 || (size_t)i >= t->impl.views.size()) return nullptr;
   return &t->impl.views[(size_t)i];
 }
 void ml_join_table_free(ml_join_table* t) { delete t; }

@@ -76,15 +76,15 @@ void launch_join_part_build(const DevJoinTable& T, int nparts, const uint32_t* o
                             const uint32_t* srow, const uint64_t* skeys, uint64_t* bloom, uint32_t* flags,
                             size_t lds, hipStream_t s);
 void launch_strided_u32(const uint32_t* src, int64_t stride, int64_t n, uint32_t* dst, hipStream_t s);
-// dense join tables: key range; presence bits (flag bit 0 = repeated key);
+// dense join tables: key range; presence bits (flag bit 0 = repeated key, bit 2 = key out of range);
 // ranks (offsets[ne] = present keys); rows moved into key order
+// mm[0..1] = min/max; `done` is a zeroed counter word (left zeroed); clear[0..1] are set to 0
 void launch_key_minmax(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t* partials, int nparts,
-                       int64_t* mm, hipStream_t s);
-void launch_dense_mark(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t* bits,
-                       uint32_t* flag, hipStream_t s);
+                       int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s);
+void launch_dense_mark(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t range,
+                       uint64_t* bits, uint32_t* flag, hipStream_t s);
 // rank[w] = set bits in words below w, rank[nw] = all set bits
-void launch_dense_rank(const uint64_t* bits, int64_t nw, uint32_t* counts, uint32_t* rank, uint32_t* block_sums,
-                       hipStream_t s);
+void launch_dense_rank(const uint64_t* bits, int64_t nw, uint32_t* rank, uint32_t* block_sums, hipStream_t s);
 struct DensePermute {
   int32_t ncols;
   int32_t width[MAX_COLS];
@@ -94,8 +94,8 @@ struct DensePermute {
   uint8_t* dst_valid[MAX_COLS];
 };
 void launch_dense_permute(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
-                          int64_t kmin, const uint64_t* bits, const uint32_t* rank, const uint32_t* present,
-                          uint32_t* null_next, hipStream_t s);
+                          int64_t kmin, uint64_t range, const uint64_t* bits, const uint32_t* rank,
+                          const uint32_t* present, uint32_t* null_next, hipStream_t s);
 // ------------------------------------------------- partitioned aggregation
 // Records of a hash aggregate, in partition order: key word w of record i at
 // keys[w * n + i], argument a at args[a * n + i], argument validity bits at

@@ -1323,11 +1323,15 @@ __global__ void join_unique_kernel(DevJoinTable T, uint32_t* dup_flag) {
 }
 
 // ------------------------------------------------- dense join table build
-// min / max of the non-NULL key words: one partial pair per workgroup, then
-// one workgroup folds the partials (mm[0] = min, mm[1] = max).
+// min / max of the non-NULL key words. Every workgroup writes its partial
+// pair; the last one to finish (counted in *done) folds them into mm[0] = min,
+// mm[1] = max, sets *done back to 0 for the next launch and clears the two
+// flag words clear[0..1] that the dense build uses next.
 __global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, const uint8_t* key_null, int64_t n,
-                                                         int64_t* partials) {
+                                                         int64_t* partials, int64_t* mm, uint32_t* done,
+                                                         uint32_t* clear) {
   __shared__ int64_t lo[256], hi[256];
+  __shared__ int last;
   int64_t a = INT64_MAX, b = INT64_MIN;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     if (key_null[r]) continue;
@@ -1348,15 +1352,17 @@ __global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, c
   if (threadIdx.x == 0) {
     partials[2 * blockIdx.x] = lo[0];
     partials[2 * blockIdx.x + 1] = hi[0];
+    __threadfence();
+    last = atomicAdd(done, 1u) == gridDim.x - 1;
   }
-}
-
-__global__ void __launch_bounds__(256) key_minmax_final_kernel(const int64_t* partials, int nparts, int64_t* mm) {
-  __shared__ int64_t lo[256], hi[256];
-  int64_t a = INT64_MAX, b = INT64_MIN;
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-    a = min(a, partials[2 * i]);
-    b = max(b, partials[2 * i + 1]);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  a = INT64_MAX;
+  b = INT64_MIN;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
+    a = min(a, __hip_atomic_load(partials + 2 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    b = max(b, __hip_atomic_load(partials + 2 * i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   }
   lo[threadIdx.x] = a;
   hi[threadIdx.x] = b;
@@ -1371,25 +1377,32 @@ __global__ void __launch_bounds__(256) key_minmax_final_kernel(const int64_t* pa
   if (threadIdx.x == 0) {
     mm[0] = lo[0];
     mm[1] = hi[0];
+    *done = 0;
+    clear[0] = 0;
+    clear[1] = 0;
   }
 }
 
 // Presence bits of the non-NULL keys (key k -> bit k - kmin); flag bit 0: a
-// key seen twice. Lanes whose keys fall in the same
+// key seen twice, bit 2: a key outside [kmin, kmin + range) (not marked). Lanes whose keys fall in the same
 // directory word (runs of neighbouring lanes, as sorted keys give) OR their
 // bits together first, so each run costs one atomic.
 __global__ void dense_mark_kernel(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin,
-                                  uint64_t* bits, uint32_t* flag) {
+                                  uint64_t range, uint64_t* bits, uint32_t* flag) {
   const int lane = __lane_id();
-  bool dup = false;
+  bool dup = false, outside = false;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
     const int64_t r = base + lane;
     uint64_t w = ~0ull, v = 0;
     if (r < n && !key_null[r]) {
       const uint64_t i = (uint64_t)((int64_t)keys[r] - kmin);
-      w = i >> 6;
-      v = 1ull << (i & 63);
+      if (i < range) {
+        w = i >> 6;
+        v = 1ull << (i & 63);
+      } else {
+        outside = true;
+      }
     }
     // segmented suffix OR over runs of equal words: afterwards the first lane
     // of a run holds the run's bits
@@ -1412,25 +1425,24 @@ __global__ void dense_mark_kernel(const uint64_t* keys, const uint8_t* key_null,
     }
   }
   if (__any(dup) && lane == 0) atomicOr(flag, 1u);
-}
-
-__global__ void dense_count_kernel(const uint64_t* bits, int64_t nw, uint32_t* counts) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nw; e += (int64_t)gridDim.x * blockDim.x)
-    counts[e] = (uint32_t)__popcll(bits[e]);
+  if (__any(outside) && lane == 0) atomicOr(flag, 4u);
 }
 
 // Build rows into key order: a row with key k moves to the rank of k; rows
 // with a NULL key (never matched) follow the present keys in any order.
 __global__ void dense_permute_kernel(DensePermute M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
-                                     int64_t kmin, const uint64_t* bits, const uint32_t* rank, const uint32_t* present,
-                                     uint32_t* null_next) {
+                                     int64_t kmin, uint64_t range, const uint64_t* bits, const uint32_t* rank,
+                                     const uint32_t* present, uint32_t* null_next) {
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     uint64_t pos;
     if (!key_null[r]) {
       const uint64_t i = (uint64_t)((int64_t)keys[r] - kmin);
+      if (i >= range) continue;  // flagged by the marking; the table falls back
       pos = rank[i >> 6] + (uint64_t)__popcll(bits[i >> 6] & ((1ull << (i & 63)) - 1ull));
+      if (pos >= (uint64_t)n) continue;  // only with repeated keys, which also fall back
     } else {
       pos = (uint64_t)*present + atomicAdd(null_next, 1u);
+      if (pos >= (uint64_t)n) continue;  // only with repeated keys, which also fall back
     }
     for (int c = 0; c < M.ncols; c++) {
       const int w = M.width[c];
@@ -1930,12 +1942,17 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* red, u
   return base + incl - v;
 }
 
-__global__ void __launch_bounds__(SCAN_T) scan_u32_reduce_kernel(const uint32_t* in, int64_t n, uint32_t* sums) {
+// Scan input i: a u32 count, or (popc) the set bits of u64 word i.
+__device__ __forceinline__ uint32_t scan_in(const void* in, int64_t i, int popc) {
+  return popc ? (uint32_t)__popcll(((const uint64_t*)in)[i]) : ((const uint32_t*)in)[i];
+}
+
+__global__ void __launch_bounds__(SCAN_T) scan_u32_reduce_kernel(const void* in, int popc, int64_t n, uint32_t* sums) {
   __shared__ uint32_t red[SCAN_T / 64];
   const int64_t base = (int64_t)blockIdx.x * SCAN_B + (int64_t)threadIdx.x * SCAN_E;
   uint32_t c = 0;
 #pragma unroll
-  for (int e = 0; e < SCAN_E; e++) c += (base + e < n) ? in[base + e] : 0u;
+  for (int e = 0; e < SCAN_E; e++) c += (base + e < n) ? scan_in(in, base + e, popc) : 0u;
   uint32_t total;
   block_excl_scan(c, red, &total);
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
@@ -1956,15 +1973,15 @@ __global__ void __launch_bounds__(SCAN_T) scan_u32_top_kernel(uint32_t* sums, in
   if (threadIdx.x == 0) sums[nb] = carry;
 }
 
-__global__ void __launch_bounds__(SCAN_T) scan_u32_down_kernel(const uint32_t* in, int64_t n, const uint32_t* sums,
-                                                               uint32_t* out) {
+__global__ void __launch_bounds__(SCAN_T) scan_u32_down_kernel(const void* in, int popc, int64_t n,
+                                                               const uint32_t* sums, uint32_t* out) {
   __shared__ uint32_t red[SCAN_T / 64];
   const int64_t base = (int64_t)blockIdx.x * SCAN_B + (int64_t)threadIdx.x * SCAN_E;
   uint32_t v[SCAN_E];
   uint32_t c = 0;
 #pragma unroll
   for (int e = 0; e < SCAN_E; e++) {
-    v[e] = (base + e < n) ? in[base + e] : 0u;
+    v[e] = (base + e < n) ? scan_in(in, base + e, popc) : 0u;
     c += v[e];
   }
   uint32_t total;
@@ -2176,28 +2193,27 @@ void launch_strided_u32(const uint32_t* src, int64_t stride, int64_t n, uint32_t
 }
 
 void launch_key_minmax(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t* partials, int nparts,
-                       int64_t* mm, hipStream_t s) {
-  key_minmax_kernel<<<nparts, 256, 0, s>>>(keys, key_null, n, partials);
-  key_minmax_final_kernel<<<1, 256, 0, s>>>(partials, nparts, mm);
+                       int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s) {
+  key_minmax_kernel<<<nparts, 256, 0, s>>>(keys, key_null, n, partials, mm, done, clear);
 }
 
-void launch_dense_mark(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t* bits,
-                       uint32_t* flag, hipStream_t s) {
-  if (n > 0) dense_mark_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, bits, flag);
+void launch_dense_mark(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t range,
+                       uint64_t* bits, uint32_t* flag, hipStream_t s) {
+  if (n > 0) dense_mark_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, range, bits, flag);
 }
 
-void launch_dense_rank(const uint64_t* bits, int64_t nw, uint32_t* counts, uint32_t* rank, uint32_t* block_sums,
-                       hipStream_t s) {
-  dense_count_kernel<<<grid_for(nw, 256, 8192), 256, 0, s>>>(bits, nw, counts);
-  launch_scan_u32(counts, nw, rank, block_sums, s);
+static void scan_launch(const void* in, int popc, int64_t n, uint32_t* out, uint32_t* block_sums, hipStream_t s);
+
+void launch_dense_rank(const uint64_t* bits, int64_t nw, uint32_t* rank, uint32_t* block_sums, hipStream_t s) {
+  scan_launch(bits, 1, nw, rank, block_sums, s);
 }
 
 void launch_dense_permute(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
-                          int64_t kmin, const uint64_t* bits, const uint32_t* rank, const uint32_t* present,
-                          uint32_t* null_next, hipStream_t s) {
+                          int64_t kmin, uint64_t range, const uint64_t* bits, const uint32_t* rank,
+                          const uint32_t* present, uint32_t* null_next, hipStream_t s) {
   if (n > 0)
-    dense_permute_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(M, keys, key_null, n, kmin, bits, rank, present,
-                                                                 null_next);
+    dense_permute_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(M, keys, key_null, n, kmin, range, bits, rank,
+                                                                 present, null_next);
 }
 
 void launch_agg_part_hist(const AggRecords& R, int64_t n, uint64_t mask, int part_shift, int nblocks,
@@ -2226,15 +2242,19 @@ void launch_agg_part_merge(const PartAggPlan& A, const uint32_t* parts, const ui
 
 int64_t scan_u32_blocks(int64_t n) { return (n + SCAN_B - 1) / SCAN_B; }
 
-void launch_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, uint32_t* block_sums, hipStream_t s) {
+static void scan_launch(const void* in, int popc, int64_t n, uint32_t* out, uint32_t* block_sums, hipStream_t s) {
   const int64_t nb = scan_u32_blocks(n);
   if (nb == 0) {
     scan_u32_top_kernel<<<1, SCAN_T, 0, s>>>(out, 0);  // out[0] = 0
     return;
   }
-  scan_u32_reduce_kernel<<<(unsigned)nb, SCAN_T, 0, s>>>(in, n, block_sums);
+  scan_u32_reduce_kernel<<<(unsigned)nb, SCAN_T, 0, s>>>(in, popc, n, block_sums);
   scan_u32_top_kernel<<<1, SCAN_T, 0, s>>>(block_sums, nb);
-  scan_u32_down_kernel<<<(unsigned)nb, SCAN_T, 0, s>>>(in, n, block_sums, out);
+  scan_u32_down_kernel<<<(unsigned)nb, SCAN_T, 0, s>>>(in, popc, n, block_sums, out);
+}
+
+void launch_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, uint32_t* block_sums, hipStream_t s) {
+  scan_launch(in, 0, n, out, block_sums, s);
 }
 
 void launch_join_unique(const DevJoinTable& T, uint32_t* dup_flag, hipStream_t s) {
