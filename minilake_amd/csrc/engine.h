@@ -269,7 +269,7 @@ struct JoinTableImpl {
   std::vector<Buffer> col_valid;
   Buffer slots;
   Buffer bloom;
-  Buffer dense_bits, dense_rank;  // dense layout (DevJoinTable::dense)
+  Buffer dense_bits, dense_rank, dense_keys;  // dense layout (DevJoinTable::dense)
   DevJoinTable dev{};
   std::vector<ml_column> views;
   // A build whose last check still runs on the stream: its flags arrive in

@@ -1481,7 +1481,8 @@ static void build_hashed(ml_ctx* ctx, JoinTableImpl& T, Buffer& keys, Buffer& ke
       T.dev.bloom_shift = __builtin_ctzll(W);
       T.dev.bloom_mask = W - 1;
     }
-    const int nblocks = (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + 8191) / 8192));
+    // workgroups of >= 2048 rows (8 per thread), at most 1024 of them
+    const int nblocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 2047) / 2048));
     const int64_t rpb = (n + nblocks - 1) / nblocks;
     const int64_t nh = (int64_t)nparts * nblocks;
     Buffer hist(ctx->arena, (size_t)nh * 4), offsets(ctx->arena, (size_t)(nh + 1) * 4);
@@ -1546,9 +1547,13 @@ void join_table_resolve(ml_join_table* t) {
     T.col_valid = std::move(T.orig_valid);
     T.dense_bits = Buffer();
     T.dense_rank = Buffer();
+    T.dense_keys = Buffer();
+    T.bloom = Buffer();
     T.dev.dense = 0;
     T.dev.dense_bits = nullptr;
     T.dev.dense_rank = nullptr;
+    T.dev.dense_keys = nullptr;
+    T.dev.bloom = nullptr;
     refresh_views(T);
     build_hashed(ctx, T, T.keys, T.key_null, T.nrows, nk, flags, false);
   } else if (kind == PENDING_PARTS) {
@@ -1614,19 +1619,20 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   if (T.col_valid.size() < T.cols.size()) T.col_valid.resize(T.cols.size());
   T.dev.nkw = (int)nk;
   uint32_t* flags = (uint32_t*)ctx->counters.as<uint64_t>();
-  // Dense layout: one integer key whose values span at most
-  // max(8n, 2^20) consecutive integers, all distinct. The rows move into key
-  // order; a probe reads one presence bit and, for a present key, the rank
-  // of its 64-key word: no hashing, probing or row index array.
+  // Dense layout: a first key column of integers whose values span at most
+  // max(8n, 2^20) consecutive integers, all distinct. The rows move into
+  // first-key order; a probe reads one presence bit and, for a present key,
+  // the rank of its 64-key word: no hashing, probing or row index array.
+  // With several key columns the other key words are compared per row, and a
+  // Bloom filter over the whole key rejects most misses first (the first
+  // column alone would not reject rows the full key does).
   const int kt0 = T.key_types[0];
-  // (With several key columns the first alone would not reject rows the full
-  // key does, and the hashed layout's Bloom filter does that better.)
-  if (nk == 1 && n > 0 && (uint64_t)n < 0xFFFFFFFFull &&
+  if (n > 0 && (uint64_t)n < 0xFFFFFFFFull &&
       (kt0 == ML_TYPE_INT32 || kt0 == ML_TYPE_INT64 || kt0 == ML_TYPE_DATE)) {
     const int nparts = (int)std::min<int64_t>(1024, (n + 4095) / 4096);
     Buffer partials(ctx->arena, (size_t)nparts * 16), mm(ctx->arena, 16);
     // (also clears flags[0] for the presence marking and flags[1] for the permutation)
-    launch_key_minmax(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, partials.as<int64_t>(), nparts,
+    launch_key_minmax(keys.as<uint64_t>(), (int)nk, key_null.as<uint8_t>(), n, partials.as<int64_t>(), nparts,
                       mm.as<int64_t>(), (uint32_t*)(ctx->counters.as<uint64_t>() + ml_ctx::DONE_WORD), flags,
                       ctx->stream);
     int64_t hm[2] = {0, -1};
@@ -1639,8 +1645,8 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
         const int64_t ne = (int64_t)((R + 63) / 64);
         Buffer bits(ctx->arena, (size_t)ne * 8);
         zero(ctx, bits.ptr, bits.bytes);
-        launch_dense_mark(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(), flags,
-                          ctx->stream);
+        launch_dense_mark(keys.as<uint64_t>(), (int)nk, key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(),
+                          flags, ctx->stream);
         // the repeated-key flag is read when the table is first used; the
         // rank and permutation run meanwhile (a repeated key discards them)
         const bool deferred = defer_check(ctx, T, flags, PENDING_DENSE);
@@ -1651,6 +1657,10 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
           Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(ne) + 1) * 4);
           launch_dense_rank(bits.as<uint64_t>(), ne, rank.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
           DensePermute M{};
+          Buffer dkeys;
+          if (nk > 1) dkeys = Buffer(ctx->arena, (size_t)n * (nk - 1) * 8);
+          M.nkw = (int32_t)nk;
+          M.dst_keys = dkeys.as<uint64_t>();
           M.ncols = (int32_t)T.cols.size();
           std::vector<Buffer> cols, valid;
           for (size_t c = 0; c < T.cols.size(); c++) {
@@ -1667,6 +1677,18 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
           // flags[1] (next NULL-key row) was cleared by the min/max kernel
           launch_dense_permute(M, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(),
                                rank.as<uint32_t>(), rank.as<uint32_t>() + ne, flags + 1, ctx->stream);
+          if (nk > 1 && hashed_bloom(ctx, n)) {  // one partition: bloom_word uses the hash bits alone
+            const uint64_t words = hashed_bloom_words(n);
+            T.bloom = Buffer(ctx->arena, words * 8);
+            zero(ctx, T.bloom.ptr, T.bloom.bytes);
+            T.dev.mask = 0;
+            T.dev.part_shift = 0;
+            T.dev.bloom_shift = __builtin_ctzll(words);
+            T.dev.bloom_mask = words - 1;
+            T.dev.bloom = T.bloom.as<uint64_t>();
+            launch_dense_bloom(T.dev, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, T.bloom.as<uint64_t>(),
+                               ctx->stream);
+          }
           if (deferred) {
             T.orig_cols = std::move(T.cols);
             T.orig_valid = std::move(T.col_valid);
@@ -1677,6 +1699,8 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
           T.col_valid = std::move(valid);
           T.dense_bits = std::move(bits);
           T.dense_rank = std::move(rank);
+          T.dense_keys = std::move(dkeys);
+          T.dev.dense_keys = T.dense_keys.as<uint64_t>();
           T.dev.dense = 1;
           T.dev.dense_min = hm[0];
           T.dev.dense_range = R;

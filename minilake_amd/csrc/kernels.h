@@ -78,14 +78,20 @@ void launch_join_part_build(const DevJoinTable& T, int nparts, const uint32_t* o
 void launch_strided_u32(const uint32_t* src, int64_t stride, int64_t n, uint32_t* dst, hipStream_t s);
 // dense join tables: key range; presence bits (flag bit 0 = repeated key, bit 2 = key out of range);
 // ranks (offsets[ne] = present keys); rows moved into key order
+// Key word r of these kernels is keys[r * nkw] (the first of a row's words).
 // mm[0..1] = min/max; `done` is a zeroed counter word (left zeroed); clear[0..1] are set to 0
-void launch_key_minmax(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t* partials, int nparts,
-                       int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s);
-void launch_dense_mark(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t range,
-                       uint64_t* bits, uint32_t* flag, hipStream_t s);
+void launch_key_minmax(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, int64_t* partials,
+                       int nparts, int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s);
+void launch_dense_mark(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, int64_t kmin,
+                       uint64_t range, uint64_t* bits, uint32_t* flag, hipStream_t s);
+// Bloom bits of every non-NULL row's whole key (T: nkw, single-partition Bloom geometry)
+void launch_dense_bloom(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                        uint64_t* bloom, hipStream_t s);
 // rank[w] = set bits in words below w, rank[nw] = all set bits
 void launch_dense_rank(const uint64_t* bits, int64_t nw, uint32_t* rank, uint32_t* block_sums, hipStream_t s);
 struct DensePermute {
+  int32_t nkw;                // key words per row; words 1.. go to dst_keys
+  uint64_t* dst_keys;
   int32_t ncols;
   int32_t width[MAX_COLS];
   const void* src[MAX_COLS];

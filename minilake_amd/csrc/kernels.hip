@@ -357,7 +357,10 @@ __device__ __forceinline__ bool run_steps(const Plan& P, const Code& ins, int fr
       const DevJoinTable& T = P.tables[st.table];
       if (T.dense) {
         uint32_t drow;
-        if (!dense_find(T, w[0], drow)) {
+        bool ok = !T.bloom || bloom_maybe(T, key_hash(w, T.nkw));
+        ok = ok && dense_find(T, w[0], drow);
+        for (int k = 1; k < T.nkw && ok; k++) ok = gp<uint64_t>(T.dense_keys)[(uint64_t)drow * (T.nkw - 1) + k - 1] == w[k];
+        if (!ok) {
           alive = false;
           continue;
         }
@@ -821,7 +824,20 @@ __device__ __forceinline__ void fast_probe(const Plan& P, const FastKeys& K, uin
 #pragma unroll
   for (int u = 0; u < FAST_U; u++) hit[u] = K.act[u];
   const bool two = P.fast_nkeys > 1;
-  if (T.dense) {  // presence bits of all rows, then the ranks of the present keys' words
+  if (T.dense) {  // (Bloom words,) presence bits of all rows, then the ranks of the present keys' words
+    if (T.bloom) {
+      uint64_t word[FAST_U], bits[FAST_U];
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) {
+        uint64_t hh = tmix(k0[u] ^ TSEED);
+        if (two) hh = tmix(hh ^ k1[u]);
+        const uint64_t g = bloom_mix(hh);
+        bits[u] = bloom_bits(g);
+        word[u] = gp<uint64_t>(T.bloom)[hit[u] ? bloom_word(T, hh, g) : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) hit[u] = hit[u] && (word[u] & bits[u]) == bits[u];
+    }
     uint64_t idx[FAST_U], bw[FAST_U];
 #pragma unroll
     for (int u = 0; u < FAST_U; u++) {
@@ -838,6 +854,13 @@ __device__ __forceinline__ void fast_probe(const Plan& P, const FastKeys& K, uin
 #pragma unroll
     for (int u = 0; u < FAST_U; u++)
       brow[u] = rk[u] + (uint32_t)__popcll(bw[u] & ((1ull << (idx[u] & 63)) - 1ull));
+    if (two) {  // the second key word must match the row's
+      uint64_t kb[FAST_U];
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) kb[u] = gp<uint64_t>(T.dense_keys)[hit[u] ? brow[u] : 0];
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) hit[u] = hit[u] && kb[u] == k1[u];
+    }
     return;
   }
 #pragma unroll
@@ -1327,15 +1350,15 @@ __global__ void join_unique_kernel(DevJoinTable T, uint32_t* dup_flag) {
 // pair; the last one to finish (counted in *done) folds them into mm[0] = min,
 // mm[1] = max, sets *done back to 0 for the next launch and clears the two
 // flag words clear[0..1] that the dense build uses next.
-__global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, const uint8_t* key_null, int64_t n,
-                                                         int64_t* partials, int64_t* mm, uint32_t* done,
+__global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, int nkw, const uint8_t* key_null,
+                                                         int64_t n, int64_t* partials, int64_t* mm, uint32_t* done,
                                                          uint32_t* clear) {
   __shared__ int64_t lo[256], hi[256];
   __shared__ int last;
   int64_t a = INT64_MAX, b = INT64_MIN;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     if (key_null[r]) continue;
-    const int64_t k = (int64_t)keys[r];
+    const int64_t k = (int64_t)keys[r * nkw];
     a = k < a ? k : a;
     b = k > b ? k : b;
   }
@@ -1387,7 +1410,7 @@ __global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, c
 // key seen twice, bit 2: a key outside [kmin, kmin + range) (not marked). Lanes whose keys fall in the same
 // directory word (runs of neighbouring lanes, as sorted keys give) OR their
 // bits together first, so each run costs one atomic.
-__global__ void dense_mark_kernel(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin,
+__global__ void dense_mark_kernel(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, int64_t kmin,
                                   uint64_t range, uint64_t* bits, uint32_t* flag) {
   const int lane = __lane_id();
   bool dup = false, outside = false;
@@ -1396,7 +1419,7 @@ __global__ void dense_mark_kernel(const uint64_t* keys, const uint8_t* key_null,
     const int64_t r = base + lane;
     uint64_t w = ~0ull, v = 0;
     if (r < n && !key_null[r]) {
-      const uint64_t i = (uint64_t)((int64_t)keys[r] - kmin);
+      const uint64_t i = (uint64_t)((int64_t)keys[r * nkw] - kmin);
       if (i < range) {
         w = i >> 6;
         v = 1ull << (i & 63);
@@ -1436,7 +1459,7 @@ __global__ void dense_permute_kernel(DensePermute M, const uint64_t* keys, const
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     uint64_t pos;
     if (!key_null[r]) {
-      const uint64_t i = (uint64_t)((int64_t)keys[r] - kmin);
+      const uint64_t i = (uint64_t)((int64_t)keys[r * M.nkw] - kmin);
       if (i >= range) continue;  // flagged by the marking; the table falls back
       pos = rank[i >> 6] + (uint64_t)__popcll(bits[i >> 6] & ((1ull << (i & 63)) - 1ull));
       if (pos >= (uint64_t)n) continue;  // only with repeated keys, which also fall back
@@ -1444,6 +1467,7 @@ __global__ void dense_permute_kernel(DensePermute M, const uint64_t* keys, const
       pos = (uint64_t)*present + atomicAdd(null_next, 1u);
       if (pos >= (uint64_t)n) continue;  // only with repeated keys, which also fall back
     }
+    for (int k = 1; k < M.nkw; k++) M.dst_keys[pos * (M.nkw - 1) + k - 1] = keys[r * M.nkw + k];
     for (int c = 0; c < M.ncols; c++) {
       const int w = M.width[c];
       if (w == 8) ((uint64_t*)M.dst[c])[pos] = ((const uint64_t*)M.src[c])[r];
@@ -1451,6 +1475,19 @@ __global__ void dense_permute_kernel(DensePermute M, const uint64_t* keys, const
       else ((uint8_t*)M.dst[c])[pos] = ((const uint8_t*)M.src[c])[r];
       if (M.src_valid[c]) M.dst_valid[c][pos] = M.src_valid[c][r];
     }
+  }
+}
+
+__global__ void dense_bloom_kernel(DevJoinTable T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                                   uint64_t* bloom) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    if (key_null[r]) continue;
+    uint64_t w[MAX_KEYS + 1];
+#pragma unroll
+    for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < T.nkw ? keys[r * T.nkw + k] : 0;
+    const uint64_t h = key_hash(w, T.nkw);
+    const uint64_t g = bloom_mix(h);
+    atomicOr((unsigned long long*)&bloom[bloom_word(T, h, g)], (unsigned long long)bloom_bits(g));
   }
 }
 
@@ -1465,6 +1502,27 @@ __device__ __forceinline__ void load_key(const uint64_t* keys, int64_t r, int nk
   for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < nkw ? keys[r * nkw + k] : 0;
 }
 
+// PART_U rows per thread per step: all their key words are requested before
+// any is hashed. Rows past `hi` read row `lo` and are inactive, so every load
+// is unconditional and in bounds; NULL build keys are never inserted
+// (join.rs:182-184).
+constexpr int PART_U = 4;
+__device__ __forceinline__ void load_keys_u(const uint64_t* keys, const uint8_t* key_null, int64_t r0, int64_t lo,
+                                            int64_t hi, int nkw, uint64_t (&w)[PART_U][MAX_KEYS + 1],
+                                            int64_t (&row)[PART_U], bool (&ok)[PART_U]) {
+  uint8_t nul[PART_U];
+#pragma unroll
+  for (int u = 0; u < PART_U; u++) {
+    const int64_t r = r0 + (int64_t)u * blockDim.x;
+    row[u] = r < hi ? r : lo;
+    ok[u] = r < hi;
+    nul[u] = key_null[row[u]];
+    load_key(keys, row[u], nkw, w[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < PART_U; u++) ok[u] = ok[u] && !nul[u];
+}
+
 __global__ void __launch_bounds__(256) join_part_hist_kernel(DevJoinTable T, const uint64_t* keys,
                                                              const uint8_t* key_null, int64_t n,
                                                              int64_t rows_per_block, int nparts, uint32_t* hist) {
@@ -1472,12 +1530,14 @@ __global__ void __launch_bounds__(256) join_part_hist_kernel(DevJoinTable T, con
   for (int p = threadIdx.x; p < nparts; p += blockDim.x) cnt[p] = 0;
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * rows_per_block, hi = min(n, lo + rows_per_block);
-  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
-    if (key_null[r]) continue;  // NULL build keys are never inserted (join.rs:182-184)
-    uint64_t w[MAX_KEYS + 1];
-    load_key(keys, r, T.nkw, w);
-    const uint64_t h = key_hash(w, T.nkw);
-    atomicAdd(&cnt[(h & T.mask) >> T.part_shift], 1u);
+  for (int64_t r0 = lo + threadIdx.x; r0 < hi; r0 += (int64_t)PART_U * blockDim.x) {
+    uint64_t w[PART_U][MAX_KEYS + 1];
+    int64_t row[PART_U];
+    bool ok[PART_U];
+    load_keys_u(keys, key_null, r0, lo, hi, T.nkw, w, row, ok);
+#pragma unroll
+    for (int u = 0; u < PART_U; u++)
+      if (ok[u]) atomicAdd(&cnt[(key_hash(w[u], T.nkw) & T.mask) >> T.part_shift], 1u);
   }
   __syncthreads();
   for (int p = threadIdx.x; p < nparts; p += blockDim.x) hist[(uint64_t)p * gridDim.x + blockIdx.x] = cnt[p];
@@ -1493,14 +1553,19 @@ __global__ void __launch_bounds__(256) join_part_scatter_kernel(DevJoinTable T, 
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * rows_per_block, hi = min(n, lo + rows_per_block);
   const int nkw = T.nkw;
-  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
-    if (key_null[r]) continue;
-    uint64_t w[MAX_KEYS + 1];
-    load_key(keys, r, nkw, w);
-    const uint64_t h = key_hash(w, nkw);
-    const uint32_t at = atomicAdd(&cur[(h & T.mask) >> T.part_shift], 1u);
-    srow[at] = (uint32_t)r;
-    for (int k = 0; k < nkw; k++) skeys[(uint64_t)at * nkw + k] = w[k];
+  for (int64_t r0 = lo + threadIdx.x; r0 < hi; r0 += (int64_t)PART_U * blockDim.x) {
+    uint64_t w[PART_U][MAX_KEYS + 1];
+    int64_t row[PART_U];
+    bool ok[PART_U];
+    load_keys_u(keys, key_null, r0, lo, hi, nkw, w, row, ok);
+#pragma unroll
+    for (int u = 0; u < PART_U; u++) {
+      if (!ok[u]) continue;
+      const uint64_t h = key_hash(w[u], nkw);
+      const uint32_t at = atomicAdd(&cur[(h & T.mask) >> T.part_shift], 1u);
+      srow[at] = (uint32_t)row[u];
+      for (int k = 0; k < nkw; k++) skeys[(uint64_t)at * nkw + k] = w[u][k];
+    }
   }
 }
 
@@ -2192,14 +2257,20 @@ void launch_strided_u32(const uint32_t* src, int64_t stride, int64_t n, uint32_t
   if (n > 0) strided_u32_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(src, stride, n, dst);
 }
 
-void launch_key_minmax(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t* partials, int nparts,
-                       int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s) {
-  key_minmax_kernel<<<nparts, 256, 0, s>>>(keys, key_null, n, partials, mm, done, clear);
+void launch_key_minmax(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, int64_t* partials,
+                       int nparts, int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s) {
+  key_minmax_kernel<<<nparts, 256, 0, s>>>(keys, nkw, key_null, n, partials, mm, done, clear);
 }
 
-void launch_dense_mark(const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin, uint64_t range,
-                       uint64_t* bits, uint32_t* flag, hipStream_t s) {
-  if (n > 0) dense_mark_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, key_null, n, kmin, range, bits, flag);
+void launch_dense_bloom(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                        uint64_t* bloom, hipStream_t s) {
+  if (n > 0) dense_bloom_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(T, keys, key_null, n, bloom);
+}
+
+void launch_dense_mark(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, int64_t kmin,
+                       uint64_t range, uint64_t* bits, uint32_t* flag, hipStream_t s) {
+  if (n > 0)
+    dense_mark_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, nkw, key_null, n, kmin, range, bits, flag);
 }
 
 static void scan_launch(const void* in, int popc, int64_t n, uint32_t* out, uint32_t* block_sums, hipStream_t s);

@@ -143,17 +143,20 @@ struct DevJoinTable {
   int32_t bloom_shift = 0; // log2 Bloom words per partition
   const uint64_t* bloom = nullptr;  // blocked Bloom filter (one u64 per block) or null
   uint64_t bloom_mask = 0; // Bloom words per partition - 1
-  // Dense layout (one integer key, unique, its values over a small range):
-  // the table's rows are in key order, so the row of key k is the number of
-  // present keys below it. With i = k - dense_min < dense_range, k is present
-  // iff bit i % 64 of dense_bits[i / 64] is set, and its row is
-  // dense_rank[i / 64] + popcount(dense_bits[i / 64] & ((1 << i % 64) - 1)).
-  // Slots and Bloom filter are then unused.
+  // Dense layout (the first key column is an integer, unique, its values
+  // over a small range): the table's rows are in order of the first key word
+  // k, so the row of k is the number of present first keys below it. With
+  // i = k - dense_min < dense_range, k is present iff bit i % 64 of
+  // dense_bits[i / 64] is set, and its row is dense_rank[i / 64] +
+  // popcount(dense_bits[i / 64] & ((1 << i % 64) - 1)). Further key words
+  // must equal dense_keys[row * (nkw - 1) + j - 1]; a Bloom filter over the
+  // whole key (one partition: mask = 0) can sit in front. Slots are unused.
   int32_t dense = 0;
   int64_t dense_min = 0;
   uint64_t dense_range = 0;
   const uint64_t* dense_bits = nullptr;  // one bit per key of the range
   const uint32_t* dense_rank = nullptr;  // present keys below each word
+  const uint64_t* dense_keys = nullptr;  // key words 1.. of each row (nkw > 1)
 };
 
 // next slot of a linear probe (inside the partition)

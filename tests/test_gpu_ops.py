@@ -300,15 +300,21 @@ def test_join_build_partitioned_sizes(gpu_engine, oracle_engine, nb):
     assert_tables_equal(agg_g.to_host(), agg_o.to_host(), rtol=0.0)
 
 
-@pytest.mark.parametrize("case", ["i64_neg", "i32", "dup_falls_back", "null_keys", "two_keys_hashed"])
+@pytest.mark.parametrize("case", ["i64_neg", "i32", "dup_falls_back", "null_keys", "two_keys", "two_keys_wide",
+                                  "two_keys_dup_first"])
 def test_join_dense_layout(gpu_engine, oracle_engine, case):
     """Unique integer keys over a small range use the direct-addressed layout;
     probes below, inside and above the range, NULL build keys, and a repeated
-    key (which must fall back to the hashed layout) all match the oracle."""
+    key (which must fall back to the hashed layout) all match the oracle. Two
+    key columns: dense first key with the second compared per row (behind a
+    Bloom filter from 64K rows), a first key over a wide range (hashed), and a
+    repeated first key with distinct full keys (falls back to hashed)."""
     rng = np.random.default_rng(21)
-    nb = 50_000
+    nb = 70_000 if case.startswith("two_keys") else 50_000
     keys = rng.permutation(200_000)[:nb].astype(np.int64) - 100_000
-    if case == "dup_falls_back":
+    if case == "two_keys_wide":
+        keys = keys * 1_000_003
+    if case in ("dup_falls_back", "two_keys_dup_first"):
         keys[7] = keys[8]
     ktype = INT32 if case == "i32" else INT64
     zero = np.zeros(nb, dtype=np.int64)
@@ -323,12 +329,17 @@ def test_join_dense_layout(gpu_engine, oracle_engine, case):
     probe = HostTable([HostColumn(ktype, pk.astype(np.int32) if ktype == INT32 else pk.astype(np.int64))])
     bkey = [col(0) + (col(2) % col(3))] if case == "null_keys" else [col(0)]
     btypes, pkeys = [ktype], [col(0)]
-    if case == "two_keys_hashed":
-        # (dense unique key, small int): two key columns use the hashed layout
+    if case.startswith("two_keys"):
+        # (first key, small int tag); the tag differs on the repeated first key
         tag = (keys % 3).astype(np.int32)
+        if case == "two_keys_dup_first":
+            tag[7] = (tag[8] + 1) % 3
         build = HostTable(build.columns + [HostColumn(INT32, tag)])
         bkey, btypes, pkeys = [col(0), col(4)], [INT64, INT32], [col(0), col(1)]
-        probe = HostTable([probe.columns[0], HostColumn(INT32, (pk % 3 + (rng.random(npr) < 0.3)).astype(np.int32))])
+        if case == "two_keys_wide":
+            pk = np.where(rng.random(npr) < 0.5, keys[rng.integers(0, nb, npr)], pk)
+        probe = HostTable([HostColumn(INT64, pk.astype(np.int64)),
+                           HostColumn(INT32, (pk % 3 + (rng.random(npr) < 0.3)).astype(np.int32))])
     jo = oracle_engine.join_build(build, [], bkey, btypes)
     jg = gpu_engine.join_build(build, [], bkey, btypes)
     assert jg.unique() == (case != "dup_falls_back")
