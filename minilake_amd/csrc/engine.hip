@@ -1629,7 +1629,7 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   const int kt0 = T.key_types[0];
   if (n > 0 && (uint64_t)n < 0xFFFFFFFFull &&
       (kt0 == ML_TYPE_INT32 || kt0 == ML_TYPE_INT64 || kt0 == ML_TYPE_DATE)) {
-    const int nparts = (int)std::min<int64_t>(1024, (n + 4095) / 4096);
+    const int nparts = (int)std::min<int64_t>(1024, (n + 1023) / 1024);
     Buffer partials(ctx->arena, (size_t)nparts * 16), mm(ctx->arena, 16);
     // (also clears flags[0] for the presence marking and flags[1] for the permutation)
     launch_key_minmax(keys.as<uint64_t>(), (int)nk, key_null.as<uint8_t>(), n, partials.as<int64_t>(), nparts,

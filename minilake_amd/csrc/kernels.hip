@@ -1356,11 +1356,24 @@ __global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, i
   __shared__ int64_t lo[256], hi[256];
   __shared__ int last;
   int64_t a = INT64_MAX, b = INT64_MIN;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    if (key_null[r]) continue;
-    const int64_t k = (int64_t)keys[r * nkw];
-    a = k < a ? k : a;
-    b = k > b ? k : b;
+  // four independent rows per thread per step (loads issued before use)
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r0 < n; r0 += 4 * stride) {
+    int64_t k[4];
+    bool ok[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t r = r0 + u * stride;
+      const int64_t rr = r < n ? r : r0;
+      k[u] = (int64_t)keys[rr * nkw];
+      ok[u] = r < n && !key_null[rr];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (!ok[u]) continue;
+      a = k[u] < a ? k[u] : a;
+      b = k[u] > b ? k[u] : b;
+    }
   }
   lo[threadIdx.x] = a;
   hi[threadIdx.x] = b;
