@@ -473,7 +473,7 @@ static uint32_t sink_read_stages(const PipelineBuild& B) {
 
 // Run a ROWS flow and materialize its rows densely (sink = SINK_COLLECT or
 // SINK_JOIN_BUILD, whose outputs are already set in B.P). Returns the row count.
-static uint64_t run_flow_rows(PipelineBuild& B, int sink, const FlowLaunch& F) {
+static uint64_t run_flow_rows(PipelineBuild& B, int sink, const FlowLaunch& F, Buffer* keep_offsets = nullptr) {
   ml_ctx* ctx = B.ctx;
   Buffer tuples(ctx->arena, std::max<uint64_t>(1, (uint64_t)__builtin_popcount(B.P.tup_mask) * B.P.staging) * 4);
   Buffer counts(ctx->arena, (size_t)F.nwaves * 4);
@@ -486,6 +486,10 @@ static uint64_t run_flow_rows(PipelineBuild& B, int sink, const FlowLaunch& F) {
   B.P.sink = keep;
   launch_scan_counts(counts.as<uint32_t>(), F.nwaves, offsets.as<uint64_t>(), ctx->stream);
   launch_materialize(B.P, sink, offsets.as<uint64_t>(), F.nwaves, ctx->stream);
+  if (keep_offsets) {  // the caller reads the row count later (offsets[nwaves])
+    *keep_offsets = std::move(offsets);
+    return UINT64_MAX;
+  }
   uint64_t total = 0;
   d2h(ctx, &total, offsets.as<uint64_t>() + F.nwaves, 8);
   return total;
@@ -1321,7 +1325,10 @@ struct ml_join_build {
   struct Chunk {
     std::vector<mlg::Buffer> cols, valid;
     mlg::Buffer keys, key_null;
-    int64_t rows = 0;
+    int64_t rows = 0;            // -1: still on the device, at count_dev
+    int64_t cap = 0;             // rows the buffers hold
+    mlg::Buffer count_buf;       // keeps count_dev alive
+    const uint64_t* count_dev = nullptr;
   };
   std::vector<Chunk> chunks;
 };
@@ -1401,8 +1408,11 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
   B.P.out_overflow = (uint32_t*)(counters + 1);
   B.upload();
   FlowLaunch F;
+  ch.cap = (int64_t)cap;
   if (plan_flow(B, SINK_ROWS, sink_read_stages(B), 0, F)) {
-    ch.rows = (int64_t)run_flow_rows(B, SINK_JOIN_BUILD, F);
+    run_flow_rows(B, SINK_JOIN_BUILD, F, &ch.count_buf);  // the count is read with the key range
+    ch.rows = -1;
+    ch.count_dev = ch.count_buf.as<uint64_t>() + F.nwaves;
   } else {
     zero(ctx, counters, 16);
     ctx->launch(B.P, B.grid(), block_of(ctx), 0);
@@ -1576,10 +1586,35 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   T.col_types = J->col_types;
   T.col_dicts = J->col_dicts;
   if (T.key_dicts.size() < T.key_types.size()) T.key_dicts.resize(T.key_types.size());
+  const size_t nk = J->key_types.size();
+  uint32_t* flags = (uint32_t*)ctx->counters.as<uint64_t>();
+  const int kt0 = T.key_types[0];
+  const bool dense_type = kt0 == ML_TYPE_INT32 || kt0 == ML_TYPE_INT64 || kt0 == ML_TYPE_DATE;
+  // Row counts still on the device. One chunk with a dense-layout key type:
+  // its count comes back with the key range (one read, below).
+  int64_t hm[3] = {0, -1, 0};
+  bool have_range = false;
+  Buffer partials, mm;
+  auto key_range = [&](const uint64_t* kw, const uint8_t* kn, int64_t rows_cap, const uint64_t* count_dev) {
+    const int nparts = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (rows_cap + 1023) / 1024));
+    partials = Buffer(ctx->arena, (size_t)nparts * 16);
+    mm = Buffer(ctx->arena, 32);
+    // (also clears flags[0] for the presence marking and flags[1] for the permutation)
+    launch_key_minmax(kw, (int)nk, kn, rows_cap, count_dev, partials.as<int64_t>(), nparts, mm.as<int64_t>(),
+                      (uint32_t*)(ctx->counters.as<uint64_t>() + ml_ctx::DONE_WORD), flags, ctx->stream);
+    d2h(ctx, hm, mm.ptr, 24);
+    have_range = true;
+  };
+  if (J->chunks.size() == 1 && J->chunks[0].rows < 0 && dense_type && (uint64_t)J->chunks[0].cap < 0xFFFFFFFFull) {
+    auto& c = J->chunks[0];
+    key_range(c.keys.as<uint64_t>(), c.key_null.as<uint8_t>(), c.cap, c.count_dev);
+    c.rows = hm[2];
+  }
+  for (auto& c : J->chunks)
+    if (c.rows < 0) d2h(ctx, &c.rows, c.count_dev, 8);
   int64_t n = 0;
   for (auto& c : J->chunks) n += c.rows;
   T.nrows = n;
-  const size_t nk = J->key_types.size();
   Buffer keys, key_null;
   if (J->chunks.size() == 1) {
     auto& c = J->chunks[0];
@@ -1618,7 +1653,6 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   J->chunks.clear();
   if (T.col_valid.size() < T.cols.size()) T.col_valid.resize(T.cols.size());
   T.dev.nkw = (int)nk;
-  uint32_t* flags = (uint32_t*)ctx->counters.as<uint64_t>();
   // Dense layout: a first key column of integers whose values span at most
   // max(8n, 2^20) consecutive integers, all distinct. The rows move into
   // first-key order; a probe reads one presence bit and, for a present key,
@@ -1626,17 +1660,8 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   // With several key columns the other key words are compared per row, and a
   // Bloom filter over the whole key rejects most misses first (the first
   // column alone would not reject rows the full key does).
-  const int kt0 = T.key_types[0];
-  if (n > 0 && (uint64_t)n < 0xFFFFFFFFull &&
-      (kt0 == ML_TYPE_INT32 || kt0 == ML_TYPE_INT64 || kt0 == ML_TYPE_DATE)) {
-    const int nparts = (int)std::min<int64_t>(1024, (n + 1023) / 1024);
-    Buffer partials(ctx->arena, (size_t)nparts * 16), mm(ctx->arena, 16);
-    // (also clears flags[0] for the presence marking and flags[1] for the permutation)
-    launch_key_minmax(keys.as<uint64_t>(), (int)nk, key_null.as<uint8_t>(), n, partials.as<int64_t>(), nparts,
-                      mm.as<int64_t>(), (uint32_t*)(ctx->counters.as<uint64_t>() + ml_ctx::DONE_WORD), flags,
-                      ctx->stream);
-    int64_t hm[2] = {0, -1};
-    d2h(ctx, hm, mm.ptr, 16);
+  if (n > 0 && (uint64_t)n < 0xFFFFFFFFull && dense_type) {
+    if (!have_range) key_range(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, nullptr);
     if (hm[0] <= hm[1]) {
       const unsigned __int128 range = (unsigned __int128)((__int128)hm[1] - (__int128)hm[0]) + 1;
       const unsigned __int128 limit = std::max<uint64_t>((uint64_t)n * 8, 1ull << 20);

@@ -79,9 +79,10 @@ void launch_strided_u32(const uint32_t* src, int64_t stride, int64_t n, uint32_t
 // dense join tables: key range; presence bits (flag bit 0 = repeated key, bit 2 = key out of range);
 // ranks (offsets[ne] = present keys); rows moved into key order
 // Key word r of these kernels is keys[r * nkw] (the first of a row's words).
-// mm[0..1] = min/max; `done` is a zeroed counter word (left zeroed); clear[0..1] are set to 0
-void launch_key_minmax(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, int64_t* partials,
-                       int nparts, int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s);
+// Rows: n, or *n_dev when given (n then bounds it). mm[0..1] = min/max, mm[2] =
+// the row count; `done` is a zeroed counter word (left zeroed); clear[0..1] are set to 0
+void launch_key_minmax(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, const uint64_t* n_dev,
+                       int64_t* partials, int nparts, int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s);
 void launch_dense_mark(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, int64_t kmin,
                        uint64_t range, uint64_t* bits, uint32_t* flag, hipStream_t s);
 // Bloom bits of every non-NULL row's whole key (T: nkw, single-partition Bloom geometry)

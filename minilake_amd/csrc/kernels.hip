@@ -1351,10 +1351,11 @@ __global__ void join_unique_kernel(DevJoinTable T, uint32_t* dup_flag) {
 // mm[1] = max, sets *done back to 0 for the next launch and clears the two
 // flag words clear[0..1] that the dense build uses next.
 __global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, int nkw, const uint8_t* key_null,
-                                                         int64_t n, int64_t* partials, int64_t* mm, uint32_t* done,
-                                                         uint32_t* clear) {
+                                                         int64_t n, const uint64_t* n_dev, int64_t* partials,
+                                                         int64_t* mm, uint32_t* done, uint32_t* clear) {
   __shared__ int64_t lo[256], hi[256];
   __shared__ int last;
+  if (n_dev) n = min(n, (int64_t)*n_dev);
   int64_t a = INT64_MAX, b = INT64_MIN;
   // four independent rows per thread per step (loads issued before use)
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1413,6 +1414,7 @@ __global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, i
   if (threadIdx.x == 0) {
     mm[0] = lo[0];
     mm[1] = hi[0];
+    mm[2] = n;
     *done = 0;
     clear[0] = 0;
     clear[1] = 0;
@@ -2270,9 +2272,9 @@ void launch_strided_u32(const uint32_t* src, int64_t stride, int64_t n, uint32_t
   if (n > 0) strided_u32_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(src, stride, n, dst);
 }
 
-void launch_key_minmax(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, int64_t* partials,
-                       int nparts, int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s) {
-  key_minmax_kernel<<<nparts, 256, 0, s>>>(keys, nkw, key_null, n, partials, mm, done, clear);
+void launch_key_minmax(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, const uint64_t* n_dev,
+                       int64_t* partials, int nparts, int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s) {
+  key_minmax_kernel<<<nparts, 256, 0, s>>>(keys, nkw, key_null, n, n_dev, partials, mm, done, clear);
 }
 
 void launch_dense_bloom(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
