@@ -141,6 +141,12 @@ struct ml_ctx {
   void* pinned = nullptr;
   static constexpr size_t PINNED_BYTES = 4096;
   std::shared_ptr<mlg::CheckPool> checks;
+  // pinned ring for small uploads (instruction arrays, small result columns):
+  // a region is rewritten only after the ring wraps, which first waits for
+  // the stream, so no upload has to wait for its copy
+  uint8_t* ring = nullptr;
+  size_t ring_head = 0;
+  static constexpr size_t RING_BYTES = 1 << 20;
   uint64_t check_seq = 0;   // deferred checks enqueued so far
   uint64_t synced_seq = 0;  // deferred checks the stream has completed (seen at a synchronization)
   std::map<const ml_dict*, std::shared_ptr<mlg::HostDict>> dicts;

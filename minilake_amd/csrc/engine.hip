@@ -76,6 +76,24 @@ int CheckPool::take() {
 static void h2d(ml_ctx* ctx, void* dst, const void* src, size_t n) {
   if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, ctx->stream), "copy to device");
 }
+// Upload through the pinned ring (returns at once; `src` may be freed after
+// the call). Larger uploads copy and then wait.
+static void upload(ml_ctx* ctx, void* dst, const void* src, size_t n) {
+  if (!n) return;
+  if (!ctx->ring || n > ml_ctx::RING_BYTES / 4) {
+    h2d(ctx, dst, src, n);
+    hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+    return;
+  }
+  if (ctx->ring_head + n > ml_ctx::RING_BYTES) {
+    hip_check(hipStreamSynchronize(ctx->stream), "synchronize");  // every earlier upload has landed
+    ctx->ring_head = 0;
+  }
+  uint8_t* at = ctx->ring + ctx->ring_head;
+  std::memcpy(at, src, n);
+  hip_check(hipMemcpyAsync(dst, at, n, hipMemcpyHostToDevice, ctx->stream), "copy to device");
+  ctx->ring_head += (n + 255) & ~(size_t)255;
+}
 static void d2h(ml_ctx* ctx, void* dst, const void* src, size_t n) {
   const uint64_t seq = ctx->check_seq;
   if (n && ctx->pinned && n <= ml_ctx::PINNED_BYTES) {
@@ -260,7 +278,7 @@ void PipelineBuild::add_ops(const ml_operator* ops, int n) {
 void PipelineBuild::upload() {
   if (ins.empty()) ins.push_back(Instr{});
   dev_ins = Buffer(ctx->arena, ins.size() * sizeof(Instr));
-  h2d(ctx, dev_ins.ptr, ins.data(), ins.size() * sizeof(Instr));
+  mlg::upload(ctx, dev_ins.ptr, ins.data(), ins.size() * sizeof(Instr));
   P.ins = dev_ins.as<Instr>();
   P.nins = (int32_t)ins.size();
 }
@@ -591,6 +609,12 @@ static bool run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buf
   B.upload();
   FlowLaunch F;
   if (cap >= (uint64_t)B.active_rows && plan_flow(B, SINK_ROWS, sink_read_stages(B), 0, F)) {
+    if (B.P.nsteps == 0) {  // no filter or probe: every active row is an output row
+      Buffer offsets;
+      run_flow_rows(B, SINK_COLLECT, F, &offsets);
+      out.rows = B.active_rows;
+      return true;
+    }
     out.rows = (int64_t)run_flow_rows(B, SINK_COLLECT, F);
     return true;
   }
@@ -1067,8 +1091,7 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
       AccSlotList kinds = slot_kinds(A);
       std::vector<uint64_t> init((size_t)groups * A->slots.size());
       for (size_t c = 0; c < init.size(); c++) init[c] = acc_init_bits(kinds.kind[c % A->slots.size()]);
-      h2d(ctx, A->totals.ptr, init.data(), init.size() * 8);
-      hip_check(hipStreamSynchronize(ctx->stream), "sync");
+      upload(ctx, A->totals.ptr, init.data(), init.size() * 8);
     }
     A->flags = Buffer(ctx->arena, 16);
     zero(ctx, A->flags.ptr, 16);
@@ -1216,10 +1239,10 @@ static ml_result* agg_finalize(ml_hashagg* A) {
         else host[j] = (uint8_t)code;
       }
       c.data = Buffer(ctx->arena, host.size());
-      h2d(ctx, c.data.ptr, host.data(), host.size());
+      upload(ctx, c.data.ptr, host.data(), host.size());
       if (any_null) {
         c.valid = Buffer(ctx->arena, valid.size());
-        h2d(ctx, c.valid.ptr, valid.data(), valid.size());
+        upload(ctx, c.valid.ptr, valid.data(), valid.size());
       }
       r->cols.push_back(std::move(c));
     }
@@ -1245,14 +1268,13 @@ static ml_result* agg_finalize(ml_hashagg* A) {
         else { uint32_t x = (uint32_t)v; std::memcpy(&host[j * 4], &x, 4); }
       }
       c.data = Buffer(ctx->arena, host.size());
-      h2d(ctx, c.data.ptr, host.data(), host.size());
+      upload(ctx, c.data.ptr, host.data(), host.size());
       if (any_null) {
         c.valid = Buffer(ctx->arena, valid.size());
-        h2d(ctx, c.valid.ptr, valid.data(), valid.size());
+        upload(ctx, c.valid.ptr, valid.data(), valid.size());
       }
       r->cols.push_back(std::move(c));
     }
-    hip_check(hipStreamSynchronize(ctx->stream), "sync");
     r->finish_views();
     return r;
   }
@@ -1798,8 +1820,7 @@ static void topn_reduce(ml_topn* T, Materialized& m) {
         rank[order[r]] = rk;
       }
       Buffer tb(ctx->arena, rank.size() * 8);
-      h2d(ctx, tb.ptr, rank.data(), rank.size() * 8);
-      hip_check(hipStreamSynchronize(ctx->stream), "sync");
+      upload(ctx, tb.ptr, rank.data(), rank.size() * 8);
       p = B.comp.compile(e, -1);
       Instr in{};
       in.op = OP_LOOKUP_U64;
@@ -1852,7 +1873,6 @@ static void topn_reduce(ml_topn* T, Materialized& m) {
     m.cols[c] = std::move(d);
     m.valid[c] = std::move(v);
   }
-  hip_check(hipStreamSynchronize(ctx->stream), "sync");
   m.rows = m_out;
 }
 
@@ -2014,6 +2034,7 @@ ml_status ml_ctx_create(const ml_exec_config* cfg, ml_ctx** out) {
     zero(ctx, ctx->counters.ptr, 64 * 8);
     hip_check(hipHostMalloc(&ctx->pinned, ml_ctx::PINNED_BYTES, hipHostMallocDefault), "hipHostMalloc");
     ctx->checks = std::make_shared<CheckPool>();
+    hip_check(hipHostMalloc((void**)&ctx->ring, ml_ctx::RING_BYTES, hipHostMallocDefault), "hipHostMalloc");
     hip_check(hipStreamSynchronize(ctx->stream), "sync");
   });
   if (st != ML_OK) {
@@ -2033,6 +2054,7 @@ void ml_ctx_destroy(ml_ctx* ctx) {
   ctx->arena.reset();
   ctx->checks.reset();
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  if (ctx->ring) (void)hipHostFree(ctx->ring);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
