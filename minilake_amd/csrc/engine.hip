@@ -1357,6 +1357,8 @@ struct ml_join_build {
 
 namespace mlg {
 
+constexpr uint64_t SMALL_BUILD_ROWS = 1 << 16;  // builds up to this many rows try the one-workgroup dense build
+
 static void join_build_run(ml_join_build* J, PipelineBuild& B) {
   ml_ctx* ctx = J->ctx;
   if ((int)J->key_types.size() > MAX_KEYS) fail(ML_ERR_UNSUPPORTED, "more than 4 join keys");
@@ -1431,7 +1433,17 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
   B.upload();
   FlowLaunch F;
   ch.cap = (int64_t)cap;
-  if (plan_flow(B, SINK_ROWS, sink_read_stages(B), 0, F)) {
+  if (!B.expanding && cap <= SMALL_BUILD_ROWS) {
+    // a small build: one append kernel, its count left on the device for the
+    // one-workgroup dense build (rows in any order: the table orders them)
+    ch.count_buf = Buffer(ctx->arena, 16);
+    zero(ctx, ch.count_buf.ptr, 16);
+    B.P.out_count = ch.count_buf.as<uint64_t>();
+    B.P.out_overflow = (uint32_t*)(ch.count_buf.as<uint64_t>() + 1);
+    if (B.active_rows > 0) ctx->launch(B.P, B.grid(), block_of(ctx), 0);
+    ch.rows = -1;
+    ch.count_dev = ch.count_buf.as<uint64_t>();
+  } else if (plan_flow(B, SINK_ROWS, sink_read_stages(B), 0, F)) {
     run_flow_rows(B, SINK_JOIN_BUILD, F, &ch.count_buf);  // the count is read with the key range
     ch.rows = -1;
     ch.count_dev = ch.count_buf.as<uint64_t>() + F.nwaves;
@@ -1627,6 +1639,56 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
     d2h(ctx, hm, mm.ptr, 24);
     have_range = true;
   };
+  if (J->chunks.size() == 1 && J->chunks[0].rows < 0 && dense_type && (uint64_t)J->chunks[0].cap <= SMALL_BUILD_ROWS) {
+    // small table: the whole dense build in one workgroup, one read of its status
+    auto& c = J->chunks[0];
+    const int64_t cap = std::max<int64_t>(c.cap, 1);
+    DensePermute M{};
+    std::vector<Buffer> cols, valid;
+    Buffer dkeys;
+    if (nk > 1) dkeys = Buffer(ctx->arena, (size_t)cap * (nk - 1) * 8);
+    M.nkw = (int32_t)nk;
+    M.dst_keys = dkeys.as<uint64_t>();
+    M.ncols = (int32_t)c.cols.size();
+    for (size_t ci = 0; ci < c.cols.size(); ci++) {
+      const int w = type_width(T.col_types[ci]);
+      cols.emplace_back(ctx->arena, (size_t)cap * w);
+      valid.emplace_back();
+      const bool has_valid = ci < c.valid.size() && c.valid[ci].ptr;
+      if (has_valid) valid.back() = Buffer(ctx->arena, (size_t)cap);
+      M.width[ci] = w;
+      M.src[ci] = c.cols[ci].ptr;
+      M.dst[ci] = cols.back().ptr;
+      M.src_valid[ci] = has_valid ? c.valid[ci].as<uint8_t>() : nullptr;
+      M.dst_valid[ci] = has_valid ? valid.back().as<uint8_t>() : nullptr;
+    }
+    const int words = small_dense_words();
+    Buffer bits(ctx->arena, (size_t)words * 8), rank(ctx->arena, (size_t)(words + 1) * 4), status(ctx->arena, 32);
+    launch_small_dense(M, c.keys.as<uint64_t>(), c.key_null.as<uint8_t>(), c.cap, c.count_dev, bits.as<uint64_t>(),
+                       rank.as<uint32_t>(), status.as<int64_t>(), ctx->stream);
+    int64_t st[4] = {0, 0, 0, 2};
+    d2h(ctx, st, status.ptr, 32);
+    c.rows = st[0];
+    if (st[3] == 0) {
+      T.nrows = st[0];
+      T.cols = std::move(cols);
+      T.col_valid = std::move(valid);
+      T.dense_bits = std::move(bits);
+      T.dense_rank = std::move(rank);
+      T.dense_keys = std::move(dkeys);
+      T.dev.nkw = (int)nk;
+      T.dev.dense = 1;
+      T.dev.dense_min = st[1];
+      T.dev.dense_range = (uint64_t)st[2];
+      T.dev.dense_bits = T.dense_bits.as<uint64_t>();
+      T.dev.dense_rank = T.dense_rank.as<uint32_t>();
+      T.dev.dense_keys = T.dense_keys.as<uint64_t>();
+      T.dev.unique = 1;
+      J->chunks.clear();
+      refresh_views(T);
+      return t;
+    }
+  }
   if (J->chunks.size() == 1 && J->chunks[0].rows < 0 && dense_type && (uint64_t)J->chunks[0].cap < 0xFFFFFFFFull) {
     auto& c = J->chunks[0];
     key_range(c.keys.as<uint64_t>(), c.key_null.as<uint8_t>(), c.cap, c.count_dev);

@@ -103,6 +103,12 @@ struct DensePermute {
 void launch_dense_permute(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
                           int64_t kmin, uint64_t range, const uint64_t* bits, const uint32_t* rank,
                           const uint32_t* present, uint32_t* null_next, hipStream_t s);
+// One-workgroup dense build of a small table (see small_dense_kernel); bits
+// and rank hold small_dense_words() and small_dense_words() + 1 entries;
+// status = {rows, min, range, flags (bit 0 repeated key, bit 1 not dense)}.
+void launch_small_dense(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t cap,
+                        const uint64_t* count_dev, uint64_t* bits, uint32_t* rank, int64_t* status, hipStream_t s);
+int small_dense_words();
 // ------------------------------------------------- partitioned aggregation
 // Records of a hash aggregate, in partition order: key word w of record i at
 // keys[w * n + i], argument a at args[a * n + i], argument validity bits at

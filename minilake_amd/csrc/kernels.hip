@@ -1493,6 +1493,146 @@ __global__ void dense_permute_kernel(DensePermute M, const uint64_t* keys, const
   }
 }
 
+// Dense build of a small table in one workgroup (LDS): min/max of the first
+// key words, presence bits, ranks and the rows moved into key order, for at
+// most `cap` rows (*count_dev of them valid) and a key range of at most
+// SMALL_DENSE_WORDS * 64. status = {rows, min, range, flags}: flags bit 0 a
+// repeated key, bit 1 not dense (no rows, or the range is too wide); with
+// flags != 0 nothing but status is written.
+constexpr int SMALL_DENSE_WORDS = 4096;
+__global__ void __launch_bounds__(1024) small_dense_kernel(DensePermute M, const uint64_t* keys, const uint8_t* key_null,
+                                                           int64_t cap, const uint64_t* count_dev, uint64_t* bits_out,
+                                                           uint32_t* rank_out, int64_t* status) {
+  __shared__ uint64_t sbits[SMALL_DENSE_WORDS];
+  __shared__ uint32_t srank[SMALL_DENSE_WORDS + 1];
+  __shared__ int64_t wlo[16], whi[16];
+  __shared__ uint32_t wsum[16];
+  __shared__ int64_t smin;
+  __shared__ uint64_t srange;
+  __shared__ uint32_t sflag, snull;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int64_t n = min(cap, (int64_t)*count_dev);
+  const int nkw = M.nkw;
+  int64_t lo = INT64_MAX, hi = INT64_MIN;
+  for (int64_t r = tid; r < n; r += blockDim.x) {
+    if (key_null[r]) continue;
+    const int64_t k = (int64_t)keys[r * nkw];
+    lo = k < lo ? k : lo;
+    hi = k > hi ? k : hi;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, (int64_t)__shfl_down(lo, o));
+    hi = max(hi, (int64_t)__shfl_down(hi, o));
+  }
+  if (lane == 0) {
+    wlo[wv] = lo;
+    whi[wv] = hi;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      lo = min(lo, wlo[w]);
+      hi = max(hi, whi[w]);
+    }
+    uint32_t f = 2u;
+    uint64_t R = 0;
+    if (n > 0 && lo <= hi) {
+      const unsigned __int128 range = (unsigned __int128)((__int128)hi - (__int128)lo) + 1;
+      const unsigned __int128 limit = (unsigned __int128)max<uint64_t>((uint64_t)n * 8, 1ull << 20);
+      if (range <= limit && range <= (unsigned __int128)SMALL_DENSE_WORDS * 64) {
+        f = 0u;
+        R = (uint64_t)range;
+      }
+    }
+    smin = lo;
+    srange = R;
+    sflag = f;
+    snull = 0;
+  }
+  __syncthreads();
+  const int64_t kmin = smin;
+  const uint64_t R = srange;
+  const int ne = (int)((R + 63) / 64);
+  if (sflag == 0) {
+    for (int w = tid; w < ne; w += blockDim.x) sbits[w] = 0;
+    __syncthreads();
+    bool dup = false;
+    for (int64_t r = tid; r < n; r += blockDim.x) {
+      if (key_null[r]) continue;
+      const uint64_t i = (uint64_t)((int64_t)keys[r * nkw] - kmin);
+      const unsigned long long bit = 1ull << (i & 63);
+      dup = dup || (atomicOr((unsigned long long*)&sbits[i >> 6], bit) & bit) != 0;
+    }
+    if (dup) atomicOr(&sflag, 1u);
+    __syncthreads();
+  }
+  if (sflag != 0) {
+    if (tid == 0) {
+      status[0] = n;
+      status[1] = kmin;
+      status[2] = (int64_t)R;
+      status[3] = sflag;
+    }
+    return;
+  }
+  // ranks: thread t sums the popcounts of words 4t .. 4t+3, then a block scan
+  uint32_t c[4], local = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int w = 4 * tid + j;
+    c[j] = w < ne ? (uint32_t)__popcll(sbits[w]) : 0u;
+    local += c[j];
+  }
+  uint32_t incl = local;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) wsum[wv] = incl;
+  __syncthreads();
+  uint32_t base = 0, total = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+    if (w < wv) base += wsum[w];
+    total += wsum[w];
+  }
+  uint32_t run = base + incl - local;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int w = 4 * tid + j;
+    if (w < ne) srank[w] = run;
+    run += c[j];
+  }
+  if (tid == 0) srank[ne] = total;
+  __syncthreads();
+  for (int w = tid; w <= ne; w += blockDim.x) {
+    if (w < ne) bits_out[w] = sbits[w];
+    rank_out[w] = srank[w];
+  }
+  for (int64_t r = tid; r < n; r += blockDim.x) {
+    uint64_t pos;
+    if (!key_null[r]) {
+      const uint64_t i = (uint64_t)((int64_t)keys[r * nkw] - kmin);
+      pos = srank[i >> 6] + (uint64_t)__popcll(sbits[i >> 6] & ((1ull << (i & 63)) - 1ull));
+    } else {
+      pos = (uint64_t)total + atomicAdd(&snull, 1u);
+    }
+    for (int k = 1; k < nkw; k++) M.dst_keys[pos * (nkw - 1) + k - 1] = keys[r * nkw + k];
+    for (int col = 0; col < M.ncols; col++) {
+      const int w = M.width[col];
+      if (w == 8) ((uint64_t*)M.dst[col])[pos] = ((const uint64_t*)M.src[col])[r];
+      else if (w == 4) ((uint32_t*)M.dst[col])[pos] = ((const uint32_t*)M.src[col])[r];
+      else ((uint8_t*)M.dst[col])[pos] = ((const uint8_t*)M.src[col])[r];
+      if (M.src_valid[col]) M.dst_valid[col][pos] = M.src_valid[col][r];
+    }
+  }
+  if (tid == 0) {
+    status[0] = n;
+    status[1] = kmin;
+    status[2] = (int64_t)R;
+    status[3] = 0;
+  }
+}
+
 __global__ void dense_bloom_kernel(DevJoinTable T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
                                    uint64_t* bloom) {
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
@@ -2276,6 +2416,12 @@ void launch_key_minmax(const uint64_t* keys, int nkw, const uint8_t* key_null, i
                        int64_t* partials, int nparts, int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s) {
   key_minmax_kernel<<<nparts, 256, 0, s>>>(keys, nkw, key_null, n, n_dev, partials, mm, done, clear);
 }
+
+void launch_small_dense(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t cap,
+                        const uint64_t* count_dev, uint64_t* bits, uint32_t* rank, int64_t* status, hipStream_t s) {
+  small_dense_kernel<<<1, 1024, 0, s>>>(M, keys, key_null, cap, count_dev, bits, rank, status);
+}
+int small_dense_words() { return SMALL_DENSE_WORDS; }
 
 void launch_dense_bloom(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
                         uint64_t* bloom, hipStream_t s) {
