@@ -97,7 +97,7 @@ def main():
     for _ in range(args.warmup):
         top, _ = Q.run_q5(eng, dev)
     torch.cuda.synchronize()
-    L.ml_ctx_enable_stats(eng.ctx, 1)
+    L.ml_ctx_enable_stats(eng.ctx, fact_rows)  # HIP events around the lineitem launches only
     stats_buf = (Stat * 4096)()
     L.ml_ctx_take_stats(eng.ctx, stats_buf, 4096)  # clear
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

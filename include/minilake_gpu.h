@@ -171,6 +171,8 @@ typedef struct ml_launch_stat {
   int64_t rows;   /* source rows the launch covered */
   double ms;      /* kernel duration */
 } ml_launch_stat;
+/* on: 0 off, 1 every launch, N > 1 only launches over at least N source rows
+ * (so recording does not slow the small launches around a large one). */
 ml_status ml_ctx_enable_stats(ml_ctx* ctx, int32_t on);
 /* Synchronize, copy up to `cap` recorded launches to `out`, clear the record;
  * returns the number written. */

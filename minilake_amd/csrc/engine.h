@@ -153,12 +153,15 @@ struct ml_ctx {
   std::shared_ptr<mlg::HostDict> dict(const ml_dict* d);
   // launch statistics
   bool stats_on = false;
+  int64_t stats_min_rows = 0;  // record launches over at least this many source rows
   struct PendingStat {
     int sink;
     int64_t rows;
     hipEvent_t start, stop;
   };
   std::vector<PendingStat> stats;
+  std::vector<hipEvent_t> event_pool;  // events of taken stats, reused
+  hipEvent_t pooled_event();
   void launch(const mlg::Plan& P, int grid, int block, size_t lds);
 };
 

@@ -1998,15 +1998,24 @@ void ml_result::finish_views() {
   }
 }
 
+hipEvent_t ml_ctx::pooled_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  mlg::hip_check(hipEventCreate(&e), "hipEventCreate");
+  return e;
+}
+
 void ml_ctx::launch(const mlg::Plan& P, int grid, int block, size_t lds) {
-  if (!stats_on) {
+  if (!stats_on || P.end - P.begin < stats_min_rows) {
     if (P.nseg > 0) mlg::launch_flow(P, grid, block, lds, stream);
     else mlg::launch_pipeline(P, grid, block, lds, stream);
     return;
   }
-  PendingStat ps{P.sink, P.end - P.begin, nullptr, nullptr};
-  mlg::hip_check(hipEventCreate(&ps.start), "hipEventCreate");
-  mlg::hip_check(hipEventCreate(&ps.stop), "hipEventCreate");
+  PendingStat ps{P.sink, P.end - P.begin, pooled_event(), pooled_event()};
   mlg::hip_check(hipEventRecord(ps.start, stream), "hipEventRecord");
   if (P.nseg > 0) mlg::launch_flow(P, grid, block, lds, stream);
   else mlg::launch_pipeline(P, grid, block, lds, stream);
@@ -2112,6 +2121,11 @@ void ml_ctx_destroy(ml_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& ps : ctx->stats) {
+    (void)hipEventDestroy(ps.start);
+    (void)hipEventDestroy(ps.stop);
+  }
+  for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
   ctx->counters.reset();
   ctx->arena.reset();
   ctx->checks.reset();
@@ -2130,7 +2144,10 @@ ml_status ml_ctx_synchronize(ml_ctx* ctx) {
 void* ml_ctx_stream(ml_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 ml_status ml_ctx_enable_stats(ml_ctx* ctx, int32_t on) {
-  return guarded(ctx, [&] { ctx->stats_on = on != 0; });
+  return guarded(ctx, [&] {
+    ctx->stats_on = on != 0;
+    ctx->stats_min_rows = on > 1 ? on : 0;
+  });
 }
 
 int32_t ml_ctx_take_stats(ml_ctx* ctx, ml_launch_stat* out, int32_t cap) {
@@ -2146,8 +2163,8 @@ int32_t ml_ctx_take_stats(ml_ctx* ctx, ml_launch_stat* out, int32_t cap) {
         out[n].ms = ms;
         n++;
       }
-      (void)hipEventDestroy(ps.start);
-      (void)hipEventDestroy(ps.stop);
+      ctx->event_pool.push_back(ps.start);
+      ctx->event_pool.push_back(ps.stop);
     }
     ctx->stats.clear();
   });
