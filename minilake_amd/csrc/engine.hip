@@ -1630,7 +1630,8 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   bool have_range = false;
   Buffer partials, mm;
   auto key_range = [&](const uint64_t* kw, const uint8_t* kn, int64_t rows_cap, const uint64_t* count_dev) {
-    const int nparts = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (rows_cap + 1023) / 1024));
+    // at most 256 workgroups: each ends with one atomic on a shared counter
+    const int nparts = (int)std::min<int64_t>(256, std::max<int64_t>(1, (rows_cap + 4095) / 4096));
     partials = Buffer(ctx->arena, (size_t)nparts * 16);
     mm = Buffer(ctx->arena, 32);
     // (also clears flags[0] for the presence marking and flags[1] for the permutation)
