@@ -1597,6 +1597,7 @@ void join_table_resolve(ml_join_table* t) {
     T.dev.dense_bits = nullptr;
     T.dev.dense_rank = nullptr;
     T.dev.dense_keys = nullptr;
+    T.dev.dense_keys32 = 0;
     T.dev.bloom = nullptr;
     refresh_views(T);
     build_hashed(ctx, T, T.keys, T.key_null, T.nrows, nk, flags, false);
@@ -1624,6 +1625,15 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   uint32_t* flags = (uint32_t*)ctx->counters.as<uint64_t>();
   const int kt0 = T.key_types[0];
   const bool dense_type = kt0 == ML_TYPE_INT32 || kt0 == ML_TYPE_INT64 || kt0 == ML_TYPE_DATE;
+  // further key words fit 32 bits when every further key column is 4 bytes wide
+  bool keys32 = nk > 1;
+  uint32_t keys_sx = 0;
+  for (size_t k = 1; k < nk; k++) {
+    const int kt = T.key_types[k];
+    keys32 = keys32 && (kt == ML_TYPE_INT32 || kt == ML_TYPE_DATE || kt == ML_TYPE_UTF8);
+    if (kt == ML_TYPE_INT32 || kt == ML_TYPE_DATE) keys_sx |= 1u << k;
+  }
+  const size_t key_word_bytes = keys32 ? 4 : 8;
   // Row counts still on the device. One chunk with a dense-layout key type:
   // its count comes back with the key range (one read, below).
   int64_t hm[3] = {0, -1, 0};
@@ -1647,8 +1657,9 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
     DensePermute M{};
     std::vector<Buffer> cols, valid;
     Buffer dkeys;
-    if (nk > 1) dkeys = Buffer(ctx->arena, (size_t)cap * (nk - 1) * 8);
+    if (nk > 1) dkeys = Buffer(ctx->arena, (size_t)cap * (nk - 1) * key_word_bytes);
     M.nkw = (int32_t)nk;
+    M.keys32 = keys32 ? 1 : 0;
     M.dst_keys = dkeys.as<uint64_t>();
     M.ncols = (int32_t)c.cols.size();
     for (size_t ci = 0; ci < c.cols.size(); ci++) {
@@ -1684,6 +1695,8 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
       T.dev.dense_bits = T.dense_bits.as<uint64_t>();
       T.dev.dense_rank = T.dense_rank.as<uint32_t>();
       T.dev.dense_keys = T.dense_keys.as<uint64_t>();
+      T.dev.dense_keys32 = keys32 ? 1 : 0;
+      T.dev.dense_keys_sx = keys_sx;
       T.dev.unique = 1;
       J->chunks.clear();
       refresh_views(T);
@@ -1768,8 +1781,9 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
           launch_dense_rank(bits.as<uint64_t>(), ne, rank.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
           DensePermute M{};
           Buffer dkeys;
-          if (nk > 1) dkeys = Buffer(ctx->arena, (size_t)n * (nk - 1) * 8);
+          if (nk > 1) dkeys = Buffer(ctx->arena, (size_t)n * (nk - 1) * key_word_bytes);
           M.nkw = (int32_t)nk;
+          M.keys32 = keys32 ? 1 : 0;
           M.dst_keys = dkeys.as<uint64_t>();
           M.ncols = (int32_t)T.cols.size();
           std::vector<Buffer> cols, valid;
@@ -1811,6 +1825,8 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
           T.dense_rank = std::move(rank);
           T.dense_keys = std::move(dkeys);
           T.dev.dense_keys = T.dense_keys.as<uint64_t>();
+          T.dev.dense_keys32 = keys32 ? 1 : 0;
+          T.dev.dense_keys_sx = keys_sx;
           T.dev.dense = 1;
           T.dev.dense_min = hm[0];
           T.dev.dense_range = R;

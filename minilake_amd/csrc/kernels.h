@@ -92,6 +92,7 @@ void launch_dense_bloom(const DevJoinTable& T, const uint64_t* keys, const uint8
 void launch_dense_rank(const uint64_t* bits, int64_t nw, uint32_t* rank, uint32_t* block_sums, hipStream_t s);
 struct DensePermute {
   int32_t nkw;                // key words per row; words 1.. go to dst_keys
+  int32_t keys32;             // dst_keys holds u32 (the low halves) instead of u64
   uint64_t* dst_keys;
   int32_t ncols;
   int32_t width[MAX_COLS];

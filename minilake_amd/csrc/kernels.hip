@@ -359,7 +359,7 @@ __device__ __forceinline__ bool run_steps(const Plan& P, const Code& ins, int fr
         uint32_t drow;
         bool ok = !T.bloom || bloom_maybe(T, key_hash(w, T.nkw));
         ok = ok && dense_find(T, w[0], drow);
-        for (int k = 1; k < T.nkw && ok; k++) ok = gp<uint64_t>(T.dense_keys)[(uint64_t)drow * (T.nkw - 1) + k - 1] == w[k];
+        for (int k = 1; k < T.nkw && ok; k++) ok = dense_key_word(T, drow, k) == w[k];
         if (!ok) {
           alive = false;
           continue;
@@ -856,8 +856,16 @@ __device__ __forceinline__ void fast_probe(const Plan& P, const FastKeys& K, uin
       brow[u] = rk[u] + (uint32_t)__popcll(bw[u] & ((1ull << (idx[u] & 63)) - 1ull));
     if (two) {  // the second key word must match the row's
       uint64_t kb[FAST_U];
+      if (T.dense_keys32) {
 #pragma unroll
-      for (int u = 0; u < FAST_U; u++) kb[u] = gp<uint64_t>(T.dense_keys)[hit[u] ? brow[u] : 0];
+        for (int u = 0; u < FAST_U; u++) {
+          const uint32_t v = gp<uint32_t>(T.dense_keys)[hit[u] ? brow[u] : 0];
+          kb[u] = ((T.dense_keys_sx >> 1) & 1u) ? (uint64_t)(int64_t)(int32_t)v : (uint64_t)v;
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < FAST_U; u++) kb[u] = gp<uint64_t>(T.dense_keys)[hit[u] ? brow[u] : 0];
+      }
 #pragma unroll
       for (int u = 0; u < FAST_U; u++) hit[u] = hit[u] && kb[u] == k1[u];
     }
@@ -1482,7 +1490,11 @@ __global__ void dense_permute_kernel(DensePermute M, const uint64_t* keys, const
       pos = (uint64_t)*present + atomicAdd(null_next, 1u);
       if (pos >= (uint64_t)n) continue;  // only with repeated keys, which also fall back
     }
-    for (int k = 1; k < M.nkw; k++) M.dst_keys[pos * (M.nkw - 1) + k - 1] = keys[r * M.nkw + k];
+    for (int k = 1; k < M.nkw; k++) {
+      const uint64_t at = pos * (M.nkw - 1) + k - 1;
+      if (M.keys32) ((uint32_t*)M.dst_keys)[at] = (uint32_t)keys[r * M.nkw + k];
+      else M.dst_keys[at] = keys[r * M.nkw + k];
+    }
     for (int c = 0; c < M.ncols; c++) {
       const int w = M.width[c];
       if (w == 8) ((uint64_t*)M.dst[c])[pos] = ((const uint64_t*)M.src[c])[r];
@@ -1616,7 +1628,11 @@ __global__ void __launch_bounds__(1024) small_dense_kernel(DensePermute M, const
     } else {
       pos = (uint64_t)total + atomicAdd(&snull, 1u);
     }
-    for (int k = 1; k < nkw; k++) M.dst_keys[pos * (nkw - 1) + k - 1] = keys[r * nkw + k];
+    for (int k = 1; k < nkw; k++) {
+      const uint64_t at = pos * (nkw - 1) + k - 1;
+      if (M.keys32) ((uint32_t*)M.dst_keys)[at] = (uint32_t)keys[r * nkw + k];
+      else M.dst_keys[at] = keys[r * nkw + k];
+    }
     for (int col = 0; col < M.ncols; col++) {
       const int w = M.width[col];
       if (w == 8) ((uint64_t*)M.dst[col])[pos] = ((const uint64_t*)M.src[col])[r];

@@ -149,15 +149,27 @@ struct DevJoinTable {
   // i = k - dense_min < dense_range, k is present iff bit i % 64 of
   // dense_bits[i / 64] is set, and its row is dense_rank[i / 64] +
   // popcount(dense_bits[i / 64] & ((1 << i % 64) - 1)). Further key words
-  // must equal dense_keys[row * (nkw - 1) + j - 1]; a Bloom filter over the
-  // whole key (one partition: mask = 0) can sit in front. Slots are unused.
+  // must equal entry row * (nkw - 1) + j - 1 of dense_keys: u64 words, or u32
+  // (dense_keys32) when every further key column is 4 bytes wide, widened by
+  // sign (bit j of dense_keys_sx) or zero; a Bloom filter over the whole key
+  // (one partition: mask = 0) can sit in front. Slots are unused.
   int32_t dense = 0;
   int64_t dense_min = 0;
   uint64_t dense_range = 0;
   const uint64_t* dense_bits = nullptr;  // one bit per key of the range
   const uint32_t* dense_rank = nullptr;  // present keys below each word
   const uint64_t* dense_keys = nullptr;  // key words 1.. of each row (nkw > 1)
+  int32_t dense_keys32 = 0;
+  uint32_t dense_keys_sx = 0;
 };
+
+// Key word j (>= 1) of dense-table row `row`.
+__host__ __device__ inline uint64_t dense_key_word(const DevJoinTable& T, uint64_t row, int j) {
+  const uint64_t at = row * (uint64_t)(T.nkw - 1) + (uint64_t)(j - 1);
+  if (!T.dense_keys32) return T.dense_keys[at];
+  const uint32_t v = ((const uint32_t*)T.dense_keys)[at];
+  return ((T.dense_keys_sx >> j) & 1u) ? (uint64_t)(int64_t)(int32_t)v : (uint64_t)v;
+}
 
 // next slot of a linear probe (inside the partition)
 __host__ __device__ inline uint64_t join_step(const DevJoinTable& T, uint64_t pos) {
