@@ -1801,17 +1801,13 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
           // flags[1] (next NULL-key row) was cleared by the min/max kernel
           launch_dense_permute(M, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(),
                                rank.as<uint32_t>(), rank.as<uint32_t>() + ne, flags + 1, ctx->stream);
-          if (nk > 1 && hashed_bloom(ctx, n)) {  // one partition: bloom_word uses the hash bits alone
-            const uint64_t words = hashed_bloom_words(n);
+          if (nk > 1 && hashed_bloom(ctx, n)) {  // a word per 8 first-key values
+            const uint64_t words = (R + (1u << DENSE_BLOOM_SHIFT) - 1) >> DENSE_BLOOM_SHIFT;
             T.bloom = Buffer(ctx->arena, words * 8);
             zero(ctx, T.bloom.ptr, T.bloom.bytes);
-            T.dev.mask = 0;
-            T.dev.part_shift = 0;
-            T.dev.bloom_shift = __builtin_ctzll(words);
-            T.dev.bloom_mask = words - 1;
             T.dev.bloom = T.bloom.as<uint64_t>();
-            launch_dense_bloom(T.dev, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, T.bloom.as<uint64_t>(),
-                               ctx->stream);
+            launch_dense_bloom(T.dev, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R,
+                               T.bloom.as<uint64_t>(), ctx->stream);
           }
           if (deferred) {
             T.orig_cols = std::move(T.cols);

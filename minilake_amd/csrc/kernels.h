@@ -85,9 +85,10 @@ void launch_key_minmax(const uint64_t* keys, int nkw, const uint8_t* key_null, i
                        int64_t* partials, int nparts, int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s);
 void launch_dense_mark(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, int64_t kmin,
                        uint64_t range, uint64_t* bits, uint32_t* flag, hipStream_t s);
-// Bloom bits of every non-NULL row's whole key (T: nkw, single-partition Bloom geometry)
+// Bloom bits of every non-NULL row's whole key, in the word of its first key
+// (T: nkw; bloom holds ceil(range / 8) words, zeroed)
 void launch_dense_bloom(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
-                        uint64_t* bloom, hipStream_t s);
+                        int64_t kmin, uint64_t range, uint64_t* bloom, hipStream_t s);
 // rank[w] = set bits in words below w, rank[nw] = all set bits
 void launch_dense_rank(const uint64_t* bits, int64_t nw, uint32_t* rank, uint32_t* block_sums, hipStream_t s);
 struct DensePermute {

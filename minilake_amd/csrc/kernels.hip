@@ -357,7 +357,12 @@ __device__ __forceinline__ bool run_steps(const Plan& P, const Code& ins, int fr
       const DevJoinTable& T = P.tables[st.table];
       if (T.dense) {
         uint32_t drow;
-        bool ok = !T.bloom || bloom_maybe(T, key_hash(w, T.nkw));
+        bool ok = true;
+        if (T.bloom) {
+          const uint64_t i = w[0] - (uint64_t)T.dense_min;
+          const uint64_t bits = bloom_bits(bloom_mix(key_hash(w, T.nkw)));
+          ok = i < T.dense_range && (gp<uint64_t>(T.bloom)[i >> DENSE_BLOOM_SHIFT] & bits) == bits;
+        }
         ok = ok && dense_find(T, w[0], drow);
         for (int k = 1; k < T.nkw && ok; k++) ok = dense_key_word(T, drow, k) == w[k];
         if (!ok) {
@@ -831,9 +836,10 @@ __device__ __forceinline__ void fast_probe(const Plan& P, const FastKeys& K, uin
       for (int u = 0; u < FAST_U; u++) {
         uint64_t hh = tmix(k0[u] ^ TSEED);
         if (two) hh = tmix(hh ^ k1[u]);
-        const uint64_t g = bloom_mix(hh);
-        bits[u] = bloom_bits(g);
-        word[u] = gp<uint64_t>(T.bloom)[hit[u] ? bloom_word(T, hh, g) : 0];
+        bits[u] = bloom_bits(bloom_mix(hh));
+        const uint64_t i = k0[u] - (uint64_t)T.dense_min;
+        hit[u] = hit[u] && i < T.dense_range;
+        word[u] = gp<uint64_t>(T.bloom)[hit[u] ? (i >> DENSE_BLOOM_SHIFT) : 0];
       }
 #pragma unroll
       for (int u = 0; u < FAST_U; u++) hit[u] = hit[u] && (word[u] & bits[u]) == bits[u];
@@ -1649,16 +1655,39 @@ __global__ void __launch_bounds__(1024) small_dense_kernel(DensePermute M, const
   }
 }
 
+// Bloom bits of every non-NULL row's whole key in the word of its first key
+// (see DevJoinTable::dense): lanes with the same word (runs of neighbouring
+// lanes, as rows in key order give) OR their bits first, one atomic per run.
 __global__ void dense_bloom_kernel(DevJoinTable T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
-                                   uint64_t* bloom) {
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    if (key_null[r]) continue;
-    uint64_t w[MAX_KEYS + 1];
+                                   int64_t kmin, uint64_t range, uint64_t* bloom) {
+  const int lane = __lane_id();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
+    const int64_t r = base + lane;
+    uint64_t w = ~0ull, v = 0;
+    if (r < n && !key_null[r]) {
+      uint64_t kw[MAX_KEYS + 1];
 #pragma unroll
-    for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < T.nkw ? keys[r * T.nkw + k] : 0;
-    const uint64_t h = key_hash(w, T.nkw);
-    const uint64_t g = bloom_mix(h);
-    atomicOr((unsigned long long*)&bloom[bloom_word(T, h, g)], (unsigned long long)bloom_bits(g));
+      for (int k = 0; k < MAX_KEYS + 1; k++) kw[k] = k < T.nkw ? keys[r * T.nkw + k] : 0;
+      const uint64_t i = (uint64_t)((int64_t)kw[0] - kmin);
+      if (i < range) {
+        w = i >> DENSE_BLOOM_SHIFT;
+        v = bloom_bits(bloom_mix(key_hash(kw, T.nkw)));
+      }
+    }
+    const uint64_t wn = __shfl_down(w, 1);
+    int end = (lane == 63 || wn != w) ? 1 : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t vn = __shfl_down(v, d);
+      const int en = __shfl_down(end, d);
+      if (!end && lane + d < 64) {
+        v |= vn;
+        end = en;
+      }
+    }
+    const uint64_t wp = __shfl_up(w, 1);
+    if ((lane == 0 || wp != w) && w != ~0ull) atomicOr((unsigned long long*)&bloom[w], (unsigned long long)v);
   }
 }
 
@@ -2440,8 +2469,8 @@ void launch_small_dense(const DensePermute& M, const uint64_t* keys, const uint8
 int small_dense_words() { return SMALL_DENSE_WORDS; }
 
 void launch_dense_bloom(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,
-                        uint64_t* bloom, hipStream_t s) {
-  if (n > 0) dense_bloom_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(T, keys, key_null, n, bloom);
+                        int64_t kmin, uint64_t range, uint64_t* bloom, hipStream_t s) {
+  if (n > 0) dense_bloom_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(T, keys, key_null, n, kmin, range, bloom);
 }
 
 void launch_dense_mark(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, int64_t kmin,

@@ -151,8 +151,10 @@ struct DevJoinTable {
   // popcount(dense_bits[i / 64] & ((1 << i % 64) - 1)). Further key words
   // must equal entry row * (nkw - 1) + j - 1 of dense_keys: u64 words, or u32
   // (dense_keys32) when every further key column is 4 bytes wide, widened by
-  // sign (bit j of dense_keys_sx) or zero; a Bloom filter over the whole key
-  // (one partition: mask = 0) can sit in front. Slots are unused.
+  // sign (bit j of dense_keys_sx) or zero. A Bloom filter can sit in front:
+  // word i >> DENSE_BLOOM_SHIFT (a word per 8 consecutive first-key values,
+  // so rows in key order set runs of one word), bits from the whole key's
+  // hash. Slots are unused.
   int32_t dense = 0;
   int64_t dense_min = 0;
   uint64_t dense_range = 0;
@@ -162,6 +164,8 @@ struct DevJoinTable {
   int32_t dense_keys32 = 0;
   uint32_t dense_keys_sx = 0;
 };
+
+constexpr int DENSE_BLOOM_SHIFT = 3;
 
 // Key word j (>= 1) of dense-table row `row`.
 __host__ __device__ inline uint64_t dense_key_word(const DevJoinTable& T, uint64_t row, int j) {
