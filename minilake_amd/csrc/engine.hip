@@ -461,7 +461,7 @@ static bool plan_flow(PipelineBuild& B, int sink, uint32_t sink_stages, size_t c
   // the stages a ROWS sink records
   P.tup_mask = sink_stages;
   // one wave per resident slot, each with a contiguous chunk of source rows
-  const int bpc = flow_blocks_per_cu(sink, FLOW_BLOCK, lds);
+  const int bpc = flow_blocks_per_cu(sink, P.fast != 0, FLOW_BLOCK, lds);
   int64_t nwaves = (int64_t)B.ctx->num_cus * bpc * (FLOW_BLOCK / 64);
   int64_t chunk = (active + nwaves - 1) / nwaves;
   if (chunk < FLOW_MIN_CHUNK) {

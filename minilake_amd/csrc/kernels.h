@@ -57,7 +57,7 @@ void launch_direct_reduce(const uint64_t* partials, int nblocks, int ncells, int
                           uint64_t* out, hipStream_t s);
 // wave-queued pipeline (flow_kernel) and the kernels around it
 void launch_flow(const Plan& P, int grid, int block, size_t lds, hipStream_t s);
-int flow_blocks_per_cu(int sink, int block, size_t lds);
+int flow_blocks_per_cu(int sink, bool fast, int block, size_t lds);
 void launch_scan_counts(const uint32_t* counts, int64_t n, uint64_t* offsets, hipStream_t s);
 void launch_materialize(const Plan& P, int sink, const uint64_t* offsets, int64_t nwaves, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);

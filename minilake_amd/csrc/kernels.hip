@@ -1006,14 +1006,20 @@ __device__ __forceinline__ void load_stages(const uint32_t* base, uint64_t strid
 #ifndef ML_FAST_PREFETCH
 #define ML_FAST_PREFETCH 0  // streaming probe: next round's keys requested before this round's probe
 #endif
-#if ML_FLOW_WAVES > 0
-#define FLOW_BOUNDS __launch_bounds__(256, ML_FLOW_WAVES)
-#else
-#define FLOW_BOUNDS __launch_bounds__(256)
+#ifndef ML_FAST_FLOW_WAVES
+#define ML_FAST_FLOW_WAVES 0  // waves per SIMD asked for the streaming-probe form (0 = none; 5 measured no faster)
 #endif
+// MODE: FLOW_GENERIC (level 0 is a filter of column atoms or the interpreter)
+// or FLOW_FAST (level 0 is the streaming first probe). Each form carries only
+// its own level-0 code, so it needs fewer registers.
+enum : int { FLOW_GENERIC = 0, FLOW_FAST = 1 };
+template <int MODE>
+constexpr int flow_waves() {
+  return MODE == FLOW_FAST ? ML_FAST_FLOW_WAVES : ML_FLOW_WAVES;
+}
 
-template <int SINK>
-__global__ void FLOW_BOUNDS flow_kernel(Plan P) {
+template <int SINK, int MODE>
+__global__ void __launch_bounds__(256, flow_waves<MODE>()) flow_kernel(Plan P) {
   extern __shared__ uint64_t lds[];
   const Code ins{(CInstr*)P.ins, P.nins - 1};
   const int ncells = (SINK == SINK_DIRECT_AGG) ? P.ngroups * P.nacc : 0;
@@ -1043,7 +1049,7 @@ __global__ void FLOW_BOUNDS flow_kernel(Plan P) {
   uint64_t nout = 0;
 #if ML_FAST_PREFETCH
   FastKeys fk;
-  if (P.fast && next < stop) fast_keys(P, next, stop, lane, fk);
+  if (MODE == FLOW_FAST && next < stop) fast_keys(P, next, stop, lane, fk);
 #endif
   for (;;) {
     int lvl = -1;
@@ -1091,7 +1097,7 @@ __global__ void FLOW_BOUNDS flow_kernel(Plan P) {
       }
     };
 
-    if (lvl == 0 && P.fast) {
+    if (MODE == FLOW_FAST && lvl == 0) {
       // FAST_U * 64 source rows through the streaming probe; hits to level 1 in source order
       uint32_t brow[FAST_U];
       bool hit[FAST_U];
@@ -1117,7 +1123,7 @@ __global__ void FLOW_BOUNDS flow_kernel(Plan P) {
         set_row(rows, stage, brow[u]);
         push(1, hit[u], rows);
       }
-    } else if (lvl == 0 && P.natoms) {
+    } else if (MODE == FLOW_GENERIC && lvl == 0 && P.natoms) {
       // FAST_U * 64 source rows through a filter of column atoms
       uint32_t src[FAST_U], bits[FAST_U];
       bool act[FAST_U];
@@ -1134,7 +1140,7 @@ __global__ void FLOW_BOUNDS flow_kernel(Plan P) {
         if (last == 0) sink(alive, rows);
         else push(1, alive, rows);
       }
-    } else if (lvl == 0) {
+    } else if (MODE == FLOW_GENERIC && lvl == 0) {
       // FAST_U * 64 source rows; the stage-0 columns segment 0 reads are
       // loaded for all of them first, then the segment runs row by row
       uint32_t src[FAST_U];
@@ -2380,27 +2386,38 @@ void launch_direct_reduce(const uint64_t* partials, int nblocks, int ncells, int
   direct_tree_kernel<<<ncells, 256, 0, s>>>(partials, nblocks, ncells, nacc, accs, out);
 }
 
-void launch_flow(const Plan& P, int grid, int block, size_t lds, hipStream_t s) {
+template <int MODE>
+static void launch_flow_mode(const Plan& P, int grid, int block, size_t lds, hipStream_t s) {
   switch (P.sink) {
-    case SINK_HASH_AGG: flow_kernel<SINK_HASH_AGG><<<grid, block, lds, s>>>(P); break;
-    case SINK_DIRECT_AGG: flow_kernel<SINK_DIRECT_AGG><<<grid, block, lds, s>>>(P); break;
-    case SINK_ROWS: flow_kernel<SINK_ROWS><<<grid, block, lds, s>>>(P); break;
-    case SINK_COUNT_PAIRS: flow_kernel<SINK_COUNT_PAIRS><<<grid, block, lds, s>>>(P); break;
+    case SINK_HASH_AGG: flow_kernel<SINK_HASH_AGG, MODE><<<grid, block, lds, s>>>(P); break;
+    case SINK_DIRECT_AGG: flow_kernel<SINK_DIRECT_AGG, MODE><<<grid, block, lds, s>>>(P); break;
+    case SINK_ROWS: flow_kernel<SINK_ROWS, MODE><<<grid, block, lds, s>>>(P); break;
+    case SINK_COUNT_PAIRS: flow_kernel<SINK_COUNT_PAIRS, MODE><<<grid, block, lds, s>>>(P); break;
     default: break;
   }
 }
 
-int flow_blocks_per_cu(int sink, int block, size_t lds) {
+void launch_flow(const Plan& P, int grid, int block, size_t lds, hipStream_t s) {
+  if (P.fast) launch_flow_mode<FLOW_FAST>(P, grid, block, lds, s);
+  else launch_flow_mode<FLOW_GENERIC>(P, grid, block, lds, s);
+}
+
+template <int MODE>
+static int flow_blocks_mode(int sink, int block, size_t lds) {
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;
   switch (sink) {
-    case SINK_HASH_AGG: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flow_kernel<SINK_HASH_AGG>, block, lds); break;
-    case SINK_DIRECT_AGG: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flow_kernel<SINK_DIRECT_AGG>, block, lds); break;
-    case SINK_ROWS: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flow_kernel<SINK_ROWS>, block, lds); break;
-    case SINK_COUNT_PAIRS: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flow_kernel<SINK_COUNT_PAIRS>, block, lds); break;
+    case SINK_HASH_AGG: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flow_kernel<SINK_HASH_AGG, MODE>, block, lds); break;
+    case SINK_DIRECT_AGG: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flow_kernel<SINK_DIRECT_AGG, MODE>, block, lds); break;
+    case SINK_ROWS: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flow_kernel<SINK_ROWS, MODE>, block, lds); break;
+    case SINK_COUNT_PAIRS: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flow_kernel<SINK_COUNT_PAIRS, MODE>, block, lds); break;
     default: break;
   }
   return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+int flow_blocks_per_cu(int sink, bool fast, int block, size_t lds) {
+  return fast ? flow_blocks_mode<FLOW_FAST>(sink, block, lds) : flow_blocks_mode<FLOW_GENERIC>(sink, block, lds);
 }
 
 void launch_scan_counts(const uint32_t* counts, int64_t n, uint64_t* offsets, hipStream_t s) {
