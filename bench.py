@@ -200,7 +200,7 @@ def main():
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": profiled_traffic(),
                      "traffic_source": "profiles/flow_kernel_star_join_q5.md (rocprofv3 FETCH_SIZE + WRITE_SIZE "
                                        "per dispatch, scaled by a 2 GiB copy kernel's counts)",
-                     "kernel": "flow_kernel<SINK_DIRECT_AGG> (lineitem probe chain)",
+                     "kernel": "flow_kernel<SINK_DIRECT_AGG, FLOW_FAST> (lineitem probe chain)",
                      "algorithmic_bytes": fact_bytes, "kernel_ms": fact_ms_avg,
                      "measured_copy_gbs": copy_gbs},
         "cpu_baseline": cpu,
