@@ -255,6 +255,9 @@ typedef struct ml_agg_spec {
   int64_t capacity_hint; /* expected distinct groups (HINT / LEARNED) */
 } ml_agg_spec;
 
+/* With num_keys == 0 the sink is UngroupedAggregateSink
+ * (aggregate/ungrouped.rs:36-139): one output row, also over no input rows
+ * (COUNT 0, SUM/AVG/MIN/MAX NULL). */
 ml_status ml_hashagg_create(ml_ctx* ctx, const ml_agg_spec* spec, ml_hashagg** out);
 /* HashAggLocal::sink (aggregate/hash.rs:429-437). */
 ml_status ml_hashagg_sink(ml_hashagg* agg, const ml_batch* batch);
@@ -332,6 +335,11 @@ ml_status ml_pipeline_topn(ml_ctx* ctx, const ml_batch* source, const ml_operato
  * counting pass runs first. */
 ml_status ml_pipeline_collect(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops,
                               int32_t num_ops, ml_result** out);
+/* LimitSink (operators/limit.rs:26-117): rows [offset, offset + limit) of the
+ * pipeline's output, in source order (the reference keeps whichever rows
+ * arrive first; without ORDER BY any such rows are a correct answer). */
+ml_status ml_pipeline_limit(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops, int32_t num_ops,
+                            int64_t limit, int64_t offset, ml_result** out);
 /* The same with the output pre-sized to `capacity_hint` rows (a learned
  * estimate) instead of the counting pass; if the rows do not fit, the exact
  * path reruns (`*used_fallback` = 1). Results are identical either way. */

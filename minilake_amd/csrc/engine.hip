@@ -1061,8 +1061,9 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
   const bool first = !A->planned;
   if (first) {
     plan_agg(A, B);
-    // direct (small-domain) aggregation when every key is a dictionary or boolean
-    bool direct = !A->key_types.empty();
+    // direct (small-domain) aggregation when every key is a dictionary or
+    // boolean column, and without keys (one group: UngroupedAggregate)
+    bool direct = true;
     int64_t groups = 1;
     std::vector<int64_t> sizes;
     for (size_t k = 0; k < A->key_types.size() && direct; k++) {
@@ -1200,6 +1201,9 @@ static ml_result* agg_finalize(ml_hashagg* A) {
   r->device = ctx->device;
   const size_t nk = A->key_types.size();
   if (!A->planned) {  // no input at all: empty result of the right shape
+    // (without group keys, UngroupedAggregate's one row: COUNT 0, the others
+    // NULL, ungrouped.rs:82-91)
+    const int64_t rows = nk == 0 ? 1 : 0;
     for (size_t k = 0; k < nk; k++) {
       ResultColumn c;
       c.type = A->key_types[k];
@@ -1209,8 +1213,20 @@ static ml_result* agg_finalize(ml_hashagg* A) {
       ResultColumn c;
       c.type = (d.func == ML_AGG_COUNT || d.func == ML_AGG_COUNT_STAR) ? ML_TYPE_INT64
                : (d.func == ML_AGG_AVG ? ML_TYPE_FLOAT64 : (d.func == ML_AGG_SUM && d.arg_type != ML_TYPE_FLOAT64 ? ML_TYPE_INT64 : d.arg_type));
+      if (rows) {
+        const uint64_t zero = 0;
+        const uint8_t is_count = d.func == ML_AGG_COUNT || d.func == ML_AGG_COUNT_STAR;
+        c.data = Buffer(ctx->arena, 8);
+        upload(ctx, c.data.ptr, &zero, 8);
+        if (!is_count) {
+          const uint8_t none = 0;
+          c.valid = Buffer(ctx->arena, 1);
+          upload(ctx, c.valid.ptr, &none, 1);
+        }
+      }
       r->cols.push_back(std::move(c));
     }
+    r->num_rows = rows;
     r->finish_views();
     return r;
   }
@@ -1219,8 +1235,8 @@ static ml_result* agg_finalize(ml_hashagg* A) {
     std::vector<uint64_t> tot((size_t)A->ngroups * ns);
     d2h(ctx, tot.data(), A->totals.ptr, tot.size() * 8);
     std::vector<int64_t> groups;
-    for (int64_t g = 0; g < A->ngroups; g++)
-      if (tot[(size_t)g * ns] > 0) groups.push_back(g);
+    for (int64_t g = 0; g < A->ngroups; g++)  // without group keys the one group always (ungrouped.rs:75-96)
+      if (tot[(size_t)g * ns] > 0 || nk == 0) groups.push_back(g);
     const size_t n = groups.size();
     r->num_rows = (int64_t)n;
     for (size_t k = 0; k < nk; k++) {
@@ -2390,7 +2406,7 @@ ml_status ml_hashagg_num_groups(ml_hashagg* agg, int64_t* out) {
       std::vector<uint64_t> tot((size_t)agg->ngroups * ns);
       d2h(agg->ctx, tot.data(), agg->totals.ptr, tot.size() * 8);
       int64_t n = 0;
-      for (int64_t g = 0; g < agg->ngroups; g++) n += tot[(size_t)g * ns] > 0;
+      for (int64_t g = 0; g < agg->ngroups; g++) n += tot[(size_t)g * ns] > 0 || agg->key_types.empty();
       *out = n;
     } else {
       agg->len_known = false;
@@ -2512,6 +2528,36 @@ ml_status ml_pipeline_collect(ml_ctx* ctx, const ml_batch* source, const ml_oper
     B.add_ops(ops, num_ops);
     Materialized m;
     run_collect(B, m, true, nullptr, 0);
+    *out = make_result(ctx, m);
+  });
+}
+
+ml_status ml_pipeline_limit(ml_ctx* ctx, const ml_batch* source, const ml_operator* ops, int32_t num_ops,
+                            int64_t limit, int64_t offset, ml_result** out) {
+  return guarded(ctx, [&] {
+    if (limit < 0 || offset < 0) fail(ML_ERR_PLAN, "negative LIMIT or OFFSET");
+    PipelineBuild B(ctx);
+    B.source(source);
+    B.add_ops(ops, num_ops);
+    Materialized m;
+    run_collect(B, m, true, nullptr, 0);
+    // limit.rs:64-83: keep rows [offset, offset + limit)
+    const int64_t start = std::min(offset, m.rows);
+    const int64_t len = std::min(limit, m.rows - start);
+    if (start > 0 || len < m.rows) {
+      for (size_t c = 0; c < m.cols.size(); c++) {
+        const int w = type_width(m.types[c]);
+        Buffer d(ctx->arena, (size_t)std::max<int64_t>(len, 1) * w);
+        d2d(ctx, d.ptr, m.cols[c].as<uint8_t>() + start * w, (size_t)len * w);
+        m.cols[c] = std::move(d);
+        if (m.valid[c].ptr) {
+          Buffer v(ctx->arena, (size_t)std::max<int64_t>(len, 1));
+          d2d(ctx, v.ptr, m.valid[c].as<uint8_t>() + start, (size_t)len);
+          m.valid[c] = std::move(v);
+        }
+      }
+      m.rows = len;
+    }
     *out = make_result(ctx, m);
   });
 }

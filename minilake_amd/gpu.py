@@ -45,6 +45,7 @@ EXPORTED = [
     "ml_join_build_finalize", "ml_join_table_num_rows", "ml_join_table_unique", "ml_join_table_free",
     "ml_join_table_num_columns", "ml_join_table_column", "ml_pipeline_collect_sized", "ml_pipeline_count",
     "ml_join_probe", "ml_pipeline_hashagg", "ml_pipeline_join_build", "ml_pipeline_topn", "ml_pipeline_collect",
+    "ml_pipeline_limit",
     "ml_topn_create", "ml_topn_sink", "ml_topn_finalize", "ml_topn_free", "ml_ctx_enable_stats",
     "ml_ctx_take_stats", "ml_fragments_create", "ml_fragments_run", "ml_fragments_run_graph",
     "ml_fragments_run_per_fragment", "ml_fragments_autotune", "ml_fragments_free", "ml_mlp_create",
@@ -105,6 +106,7 @@ def lib():
             "ml_pipeline_join_build": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, vp], i32),
             "ml_pipeline_topn": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, vp], i32),
             "ml_pipeline_collect": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, P(vp)], i32),
+            "ml_pipeline_limit": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, i64, i64, P(vp)], i32),
             "ml_pipeline_collect_sized": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, i64, P(i32), P(vp)], i32),
             "ml_pipeline_count": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, P(i64)], i32),
             "ml_topn_create": ([vp, P(abi.MlSortSpec), P(vp)], i32),
@@ -437,6 +439,15 @@ class GpuEngine:
         _, b = self._src(src, keep)
         out = C.c_void_p()
         self._check(L.ml_pipeline_collect(self.ctx, C.byref(b), self._ops(ops, keep), len(ops), C.byref(out)))
+        return GpuResult(self, out.value)
+
+    def limit(self, src, ops, limit: int, offset: int = 0) -> GpuResult:
+        """LimitSink (operators/limit.rs:26-117): rows [offset, offset + limit) in source order."""
+        L, keep = lib(), abi.Keep()
+        _, b = self._src(src, keep)
+        out = C.c_void_p()
+        self._check(L.ml_pipeline_limit(self.ctx, C.byref(b), self._ops(ops, keep), len(ops), limit, offset,
+                                        C.byref(out)))
         return GpuResult(self, out.value)
 
     def count_distinct(self, src, ops, group_exprs, key_types, sizing=abi.SIZING_PLAIN, capacity_hint=0,

@@ -71,6 +71,8 @@ def lib():
                                         C.POINTER(abi.MlSortSpec), C.POINTER(MloOptions), C.POINTER(vp)]
         L.mlo_pipeline_collect.argtypes = [C.POINTER(abi.MlBatch), C.POINTER(abi.MlOperator), i32,
                                            C.POINTER(MloOptions), C.POINTER(vp)]
+        L.mlo_pipeline_limit.argtypes = [C.POINTER(abi.MlBatch), C.POINTER(abi.MlOperator), i32, i64, i64,
+                                         C.POINTER(MloOptions), C.POINTER(vp)]
         L.mlo_join_probe.argtypes = [vp, C.POINTER(abi.MlExpr), i32, C.POINTER(abi.MlBatch), C.POINTER(vp)]
         L.mlo_join_num_rows.argtypes = [vp]
         L.mlo_join_num_rows.restype = i64
@@ -196,6 +198,15 @@ class OracleEngine:
         out = C.c_void_p()
         _check(lib().mlo_pipeline_topn(C.byref(b), self._ops(ops, keep), len(ops), C.byref(spec),
                                        C.byref(self.opts), C.byref(out)))
+        return _take_table(out.value)
+
+    def limit(self, src, ops, limit: int, offset: int = 0):
+        """LimitSink (operators/limit.rs:26-117): rows [offset, offset + limit) of the pipeline's output."""
+        keep = abi.Keep()
+        b = host_batch(self._host(src), keep)
+        out = C.c_void_p()
+        _check(lib().mlo_pipeline_limit(C.byref(b), self._ops(ops, keep), len(ops), limit, offset,
+                                        C.byref(self.opts), C.byref(out)))
         return _take_table(out.value)
 
     def collect(self, src, ops):

@@ -418,6 +418,60 @@ struct HashAggSink : Sink {
   }
 };
 
+// UngroupedAggregateSink (aggregate/ungrouped.rs:36-139), Single mode: one
+// group; per-thread accumulators merged into the global ones under a mutex,
+// once per thread; always one output row (no input at all: COUNT = 0 and
+// SUM/AVG/MIN/MAX NULL, ungrouped.rs:82-91).
+struct UngroupedAggSink : Sink {
+  std::vector<AggDef> aggs;
+  std::mutex mu;
+  std::unique_ptr<std::vector<Accumulator>> global;
+  Batch output;
+
+  std::vector<Accumulator> fresh() const {  // ungrouped.rs:23-32
+    std::vector<Accumulator> a;
+    for (auto& d : aggs) {
+      a.emplace_back(d.func, d.arg_type, (bool)d.arg);
+      a.back().resize(1);
+    }
+    return a;
+  }
+  struct Local : LocalSink {
+    UngroupedAggSink* s;
+    std::vector<Accumulator> accs;
+    std::vector<uint32_t> gids;
+    Local(UngroupedAggSink* sink) : s(sink), accs(sink->fresh()) {}
+    void sink(const Batch& b) override {  // ungrouped.rs:100-121
+      Rows rows = batch_rows(b);
+      gids.assign(rows.len(), 0u);
+      for (size_t a = 0; a < s->aggs.size(); a++) {
+        ColumnRef arg;
+        if (s->aggs[a].arg) arg = accs[a].prepare_input(evaluate_to_column(*s->aggs[a].arg, b, s->aggs[a].arg_type));
+        accs[a].update(arg.get(), rows, gids);
+      }
+    }
+    void combine() override {  // ungrouped.rs:123-139
+      std::lock_guard<std::mutex> g(s->mu);
+      if (!s->global) {
+        s->global = std::make_unique<std::vector<Accumulator>>(std::move(accs));
+        return;
+      }
+      const std::vector<uint32_t> zero(1, 0u);
+      for (size_t a = 0; a < accs.size(); a++) {
+        auto st = accs[a].state();
+        (*s->global)[a].merge(st.data(), zero);
+      }
+    }
+  };
+  std::unique_ptr<LocalSink> create_local() override { return std::make_unique<Local>(this); }
+  void finalize() override {  // ungrouped.rs:75-96
+    std::vector<Accumulator> accs = global ? std::move(*global) : fresh();
+    output.columns.clear();
+    for (auto& acc : accs) output.columns.push_back(acc.finish());
+    output.num_rows = 1;
+  }
+};
+
 // ================================================================== hash join
 struct JoinTable {  // join.rs:36-45
   Batch batch;
@@ -668,6 +722,36 @@ struct CollectSink : Sink {
       std::lock_guard<std::mutex> g(s->mu);
       for (auto& b : batches) s->batches.push_back(std::move(b));
     }
+  };
+  std::unique_ptr<LocalSink> create_local() override { return std::make_unique<Local>(this); }
+  void finalize() override {}
+};
+
+// LimitSink (operators/limit.rs:26-117): workers append compacted batches
+// under a mutex until limit + offset rows are collected (a flag then stops
+// the others early); finalize concatenates and keeps rows [offset, offset +
+// limit). With one thread the morsels arrive in order, so these are the
+// first rows in source order.
+struct LimitSink : Sink {
+  size_t limit = 0, offset = 0;
+  std::mutex mu;
+  std::vector<Batch> batches;
+  size_t have = 0;
+  std::atomic<bool> done{false};
+  struct Local : LocalSink {
+    LimitSink* s;
+    explicit Local(LimitSink* sink) : s(sink) {}
+    void sink(const Batch& b) override {  // limit.rs:88-109
+      if (s->done.load(std::memory_order_relaxed)) return;
+      const size_t needed = s->limit > SIZE_MAX - s->offset ? SIZE_MAX : s->limit + s->offset;
+      std::lock_guard<std::mutex> g(s->mu);
+      if (s->have >= needed) return;
+      Batch c = compact(b);
+      s->have += c.num_rows;
+      s->batches.push_back(std::move(c));
+      if (s->have >= needed) s->done.store(true, std::memory_order_relaxed);
+    }
+    void combine() override {}
   };
   std::unique_ptr<LocalSink> create_local() override { return std::make_unique<Local>(this); }
   void finalize() override {}
@@ -1040,18 +1124,27 @@ int mlo_pipeline_hashagg(const ml_batch* src, const ml_operator* ops, int32_t no
     SourceDef s = make_source(src, opt);
     std::vector<int> schema = schema_of(s.table);
     auto opdefs = import_ops(ops, nops, schema);
-    HashAggSink sink;
-    for (int k = 0; k < spec->num_keys; k++) {
-      sink.group_exprs.push_back(build_expr(spec->group_exprs[k]));
-      sink.key_types.push_back(spec->key_types[k]);
-    }
+    std::vector<AggDef> aggs;
     for (int a = 0; a < spec->num_aggs; a++) {
       AggDef d;
       d.func = spec->aggs[a].func;
       d.arg_type = spec->aggs[a].arg_type;
       if (d.func != ML_AGG_COUNT_STAR) d.arg = build_expr(spec->aggs[a].arg);
-      sink.aggs.push_back(d);
+      aggs.push_back(d);
     }
+    if (spec->num_keys == 0) {  // no GROUP BY: the planner's UngroupedAggregate
+      UngroupedAggSink sink;
+      sink.aggs = aggs;
+      run_pipeline(s, opdefs, sink, threads_of(opt));
+      *out = export_batch(sink.output);
+      return;
+    }
+    HashAggSink sink;
+    for (int k = 0; k < spec->num_keys; k++) {
+      sink.group_exprs.push_back(build_expr(spec->group_exprs[k]));
+      sink.key_types.push_back(spec->key_types[k]);
+    }
+    sink.aggs = aggs;
     run_pipeline(s, opdefs, sink, threads_of(opt));
     *out = export_batch(sink.output);
     (*out)->resizes = sink.resizes;
@@ -1129,6 +1222,18 @@ int mlo_pipeline_topn(const ml_batch* src, const ml_operator* ops, int32_t nops,
   });
 }
 
+static Batch empty_batch(const std::vector<int>& schema) {
+  Batch empty;
+  for (int t : schema) {
+    if (t == Int32 || t == Date) empty.columns.push_back(make_column(t, std::vector<int32_t>()));
+    else if (t == Boolean) empty.columns.push_back(make_column(t, std::vector<uint8_t>()));
+    else if (t == Float64) empty.columns.push_back(make_column(t, std::vector<double>()));
+    else if (t == Utf8) empty.columns.push_back(make_column(t, std::vector<uint32_t>(), nullptr, std::make_shared<Dict>()));
+    else empty.columns.push_back(make_column(Int64, std::vector<int64_t>()));
+  }
+  return empty;
+}
+
 int mlo_pipeline_collect(const ml_batch* src, const ml_operator* ops, int32_t nops, const mlo_options* opt,
                          mlo_table** out) {
   return guarded([&] {
@@ -1137,19 +1242,29 @@ int mlo_pipeline_collect(const ml_batch* src, const ml_operator* ops, int32_t no
     auto opdefs = import_ops(ops, nops, schema);
     CollectSink sink;
     run_pipeline(s, opdefs, sink, threads_of(opt));
+    *out = export_batch(sink.batches.empty() ? empty_batch(schema) : concat(sink.batches));
+  });
+}
+
+int mlo_pipeline_limit(const ml_batch* src, const ml_operator* ops, int32_t nops, int64_t limit, int64_t offset,
+                       const mlo_options* opt, mlo_table** out) {
+  return guarded([&] {
+    if (limit < 0 || offset < 0) throw std::runtime_error("negative LIMIT or OFFSET");
+    SourceDef s = make_source(src, opt);
+    std::vector<int> schema = schema_of(s.table);
+    auto opdefs = import_ops(ops, nops, schema);
+    LimitSink sink;
+    sink.limit = (size_t)limit;
+    sink.offset = (size_t)offset;
+    run_pipeline(s, opdefs, sink, threads_of(opt));
     if (sink.batches.empty()) {
-      Batch empty;
-      for (int t : schema) {
-        if (t == Int32 || t == Date) empty.columns.push_back(make_column(t, std::vector<int32_t>()));
-        else if (t == Boolean) empty.columns.push_back(make_column(t, std::vector<uint8_t>()));
-        else if (t == Float64) empty.columns.push_back(make_column(t, std::vector<double>()));
-        else if (t == Utf8) empty.columns.push_back(make_column(t, std::vector<uint32_t>(), nullptr, std::make_shared<Dict>()));
-        else empty.columns.push_back(make_column(Int64, std::vector<int64_t>()));
-      }
-      *out = export_batch(empty);
-    } else {
-      *out = export_batch(concat(sink.batches));
+      *out = export_batch(empty_batch(schema));
+      return;
     }
+    Batch all = concat(sink.batches);  // limit.rs:64-83
+    const size_t start = std::min(sink.offset, all.num_rows);
+    const size_t len = std::min(sink.limit, all.num_rows - start);
+    *out = export_batch(slice(all, start, len));
   });
 }
 

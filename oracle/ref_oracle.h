@@ -41,6 +41,9 @@ void mlo_join_free(mlo_join* j);
 int mlo_join_probe(const mlo_join* j, const ml_expr* keys, int32_t nkeys, const ml_batch* probe, mlo_table** out);
 int mlo_pipeline_topn(const ml_batch* src, const ml_operator* ops, int32_t nops, const ml_sort_spec* spec,
                       const mlo_options* opt, mlo_table** out);
+// LimitSink (operators/limit.rs:26-117): rows [offset, offset + limit) of what the pipeline produces
+int mlo_pipeline_limit(const ml_batch* src, const ml_operator* ops, int32_t nops, int64_t limit, int64_t offset,
+                       const mlo_options* opt, mlo_table** out);
 int mlo_pipeline_collect(const ml_batch* src, const ml_operator* ops, int32_t nops, const mlo_options* opt,
                          mlo_table** out);
 int64_t mlo_table_num_rows(const mlo_table* t);

@@ -418,3 +418,30 @@ def test_hashagg_partitioned_underestimate(gpu_engine, oracle_engine):
         gpu_engine.hashagg(t, [], [col(0)], [INT64], aggs, sizing=abi.SIZING_HINT, capacity_hint=70_000)
     got = gpu_engine.hashagg(t, [], [col(0)], [INT64], aggs, sizing=abi.SIZING_LEARNED, capacity_hint=70_000)
     assert_tables_equal(got.to_host(), want, rtol=0.0)
+
+
+def test_ungrouped_aggregate_matches_oracle(gpu_engine, oracle_engine):
+    """No GROUP BY (UngroupedAggregate, ungrouped.rs:36-139): one row, also over
+    no input rows (COUNT 0, the rest NULL); NULL arguments from x / 0."""
+    t = mixed_table(30_000, seed=12)
+    aggs = [Agg(AGG_COUNT_STAR), Agg(AGG_SUM, col(0), INT32), Agg(AGG_SUM, col(2), FLOAT64), Agg(AGG_AVG, col(0), INT32),
+            Agg(AGG_MIN, col(4), DATE), Agg(AGG_MAX, col(1), INT64), Agg(AGG_COUNT, col(0) / col(6), FLOAT64),
+            Agg(AGG_SUM, col(2) / col(6), FLOAT64)]
+    for ops in ([], [abi.filter_op(col(0).lt(lit(-100)))], [abi.filter_op(col(0).lt(lit(-5000)))]):
+        want = oracle_engine.hashagg(t, ops, [], [], aggs).to_host()
+        got = gpu_engine.hashagg(t, ops, [], [], aggs).to_host()
+        assert got.num_rows == want.num_rows == 1
+        assert_tables_equal(got, want, rtol=1e-12)
+
+
+@pytest.mark.parametrize("limit,offset", [(10, 0), (7, 123), (100_000, 4_000), (5, 10_000_000), (0, 0)])
+def test_limit_matches_oracle(gpu_engine, limit, offset):
+    """LIMIT / OFFSET (limit.rs:26-117): the GPU keeps rows [offset, offset +
+    limit) in source order, which is what the oracle returns with one thread."""
+    from oracle.oracle import OracleEngine
+    t = mixed_table(20_000, seed=13)
+    ops = [abi.filter_op(col(0).lt(lit(300)) & ~col(6).eq(lit(0)))]
+    want = OracleEngine(threads=1).limit(t, ops, limit, offset).to_host()
+    got = gpu_engine.limit(t, ops, limit, offset).to_host()
+    assert got.num_rows == want.num_rows
+    assert rows_of(got) == rows_of(want)

@@ -415,3 +415,32 @@ def test_workload_generators_are_deterministic():
     counts = np.bincount(z)[1:4]
     # Zipf(1.1): P(1)/P(2) = 2^1.1 ~ 2.14
     assert 1.9 < counts[0] / counts[1] < 2.4
+
+
+# ------------------------------------------------ UngroupedAggregate, LIMIT
+def test_ungrouped_aggregate_closed_forms():
+    """UngroupedAggregateSink (ungrouped.rs:36-139): one row always; over no
+    rows COUNT = 0 and SUM/AVG/MIN NULL (ungrouped.rs:82-91); over 0..99 the
+    closed forms (sum 4950, mean of 0.5*i = 24.75), with several threads and
+    tiny morsels so the per-thread merge runs."""
+    t = HostTable([HostColumn(INT64, np.arange(100, dtype=np.int64)), HostColumn(FLOAT64, np.arange(100) * 0.5)])
+    aggs = [Agg(AGG_COUNT_STAR), Agg(AGG_SUM, col(0), INT64), Agg(AGG_AVG, col(1), FLOAT64),
+            Agg(AGG_MIN, col(1), FLOAT64)]
+    for threads, morsel in ((1, 1 << 20), (4, 7)):
+        eng = O.OracleEngine(threads=threads, morsel_rows=morsel)
+        empty = rows_of(eng.hashagg(t, [abi.filter_op(col(0).lt(lit(0)))], [], [], aggs).to_host())
+        assert empty == [(0, None, None, None)]
+        full = rows_of(eng.hashagg(t, [], [], [], aggs).to_host())
+        assert full == [(100, 4950, 24.75, 0.0)]
+
+
+def test_limit_offset_slices_source_order():
+    """LimitSink (limit.rs:26-117) with one thread: rows [offset, offset + limit)
+    of the filtered input in order; offsets past the end give no rows."""
+    t = HostTable([HostColumn(INT64, np.arange(100, dtype=np.int64))])
+    eng = O.OracleEngine(threads=1)
+    ops = [abi.filter_op(col(0).gt(lit(10)))]
+    assert eng.limit(t, ops, 5, 3).to_host().columns[0].values.tolist() == [14, 15, 16, 17, 18]
+    assert eng.limit(t, ops, 1000, 80).to_host().columns[0].values.tolist() == list(range(91, 100))
+    assert eng.limit(t, ops, 5, 500).to_host().num_rows == 0
+    assert eng.limit(t, ops, 0, 0).to_host().num_rows == 0
