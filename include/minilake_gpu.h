@@ -363,7 +363,7 @@ typedef struct ml_sort_spec {
   const ml_sort_key* keys;
   int32_t num_keys;
   int32_t _pad;
-  int64_t limit; /* ORDER BY ... LIMIT n (physical_planner.rs:150-178) */
+  int64_t limit; /* ORDER BY ... LIMIT n (physical_planner.rs:150-178); < 0: ORDER BY alone (a full sort) */
 } ml_sort_spec;
 
 ml_status ml_topn_create(ml_ctx* ctx, const ml_sort_spec* spec, ml_topn** out);

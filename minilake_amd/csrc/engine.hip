@@ -1874,7 +1874,7 @@ namespace mlg {
 static void topn_reduce(ml_topn* T, Materialized& m) {
   ml_ctx* ctx = T->ctx;
   const int64_t n = m.rows;
-  const int64_t L = T->limit;
+  const int64_t L = T->limit;  // < 0: no limit (SortSink without LIMIT)
   if (n == 0 || L == 0) {
     m.rows = 0;
     return;
@@ -1933,26 +1933,34 @@ static void topn_reduce(ml_topn* T, Materialized& m) {
   }
   K.nwords = nw;
   if (nw + 1 > 6) fail(ML_ERR_UNSUPPORTED, "sort keys too wide");
-  if (L > 256) fail(ML_ERR_UNSUPPORTED, "TopN limit above 256");
   B.upload();
   const int W = nw + 1;
   Buffer words(ctx->arena, (size_t)n * W * 8);
   launch_sort_keys(B.P, K, n, words.as<uint64_t>(), ctx->stream);
-  // repeated tile selection until one tile remains
-  int64_t cur = n;
-  Buffer a = std::move(words);
-  for (;;) {
-    const int64_t tiles = (cur + TOPN_TILE - 1) / TOPN_TILE;
-    const int keep = (int)std::min<int64_t>(L, TOPN_TILE);
-    Buffer b(ctx->arena, (size_t)tiles * keep * W * 8);
-    launch_topn_tiles(a.as<uint64_t>(), cur, nw, keep, b.as<uint64_t>(), ctx->stream);
-    a = std::move(b);
-    cur = tiles * keep;
-    if (tiles == 1) break;
-  }
-  const int64_t m_out = std::min<int64_t>(L, n);
+  const int64_t m_out = L < 0 ? n : std::min<int64_t>(L, n);
   Buffer rows(ctx->arena, (size_t)m_out * 8);
-  launch_topn_rows(a.as<uint64_t>(), nw, m_out, rows.as<int64_t>(), ctx->stream);
+  if (L < 0 || L > 256) {
+    // a full sort (sort_batches without or with a large limit, sort.rs:85-132)
+    const int64_t N = sort_padded_entries(n);
+    Buffer s1(ctx->arena, (size_t)N * W * 8), s2(ctx->arena, (size_t)N * W * 8);
+    const uint64_t* sorted = launch_sort_words(words.as<uint64_t>(), n, nw, s1.as<uint64_t>(), s2.as<uint64_t>(),
+                                               ctx->stream);
+    launch_topn_rows(sorted, nw, m_out, rows.as<int64_t>(), ctx->stream);
+  } else {
+    // repeated tile selection until one tile remains
+    int64_t cur = n;
+    Buffer a = std::move(words);
+    for (;;) {
+      const int64_t tiles = (cur + TOPN_TILE - 1) / TOPN_TILE;
+      const int keep = (int)std::min<int64_t>(L, TOPN_TILE);
+      Buffer b(ctx->arena, (size_t)tiles * keep * W * 8);
+      launch_topn_tiles(a.as<uint64_t>(), cur, nw, keep, b.as<uint64_t>(), ctx->stream);
+      a = std::move(b);
+      cur = tiles * keep;
+      if (tiles == 1) break;
+    }
+    launch_topn_rows(a.as<uint64_t>(), nw, m_out, rows.as<int64_t>(), ctx->stream);
+  }
   // gather every column of the winners
   for (size_t c = 0; c < m.types.size(); c++) {
     const int w = type_width(m.types[c]);
@@ -2607,8 +2615,7 @@ ml_status ml_topn_create(ml_ctx* ctx, const ml_sort_spec* spec, ml_topn** out) {
       T->asc.push_back(spec->keys[k].asc != 0);
       T->nulls_first.push_back(spec->keys[k].nulls_first != 0);
     }
-    if (spec->limit < 0) fail(ML_ERR_UNSUPPORTED, "unbounded sort (only ORDER BY ... LIMIT n)");
-    T->limit = spec->limit;
+    T->limit = spec->limit < 0 ? -1 : spec->limit;  // -1: ORDER BY without LIMIT
   });
   if (st != ML_OK) {
     delete T;

@@ -164,5 +164,9 @@ void launch_compact(const uint8_t* flags, int64_t n, uint32_t* block_counts, uin
 void launch_sort_keys(const Plan& P, const SortKeys& K, int64_t n, uint64_t* words, hipStream_t s);
 void launch_topn_tiles(const uint64_t* in, int64_t n, int nw, int keep, uint64_t* out, hipStream_t s);
 void launch_topn_rows(const uint64_t* words, int nw, int64_t n, int64_t* rows_out, hipStream_t s);
+// Full sort of n entries of nw + 1 words (see merge_pass_kernel): a and b
+// hold sort_padded_entries(n) entries each; returns the one with the result.
+int64_t sort_padded_entries(int64_t n);
+uint64_t* launch_sort_words(const uint64_t* words, int64_t n, int nw, uint64_t* a, uint64_t* b, hipStream_t s);
 
 }  // namespace mlg

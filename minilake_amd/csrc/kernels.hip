@@ -11,6 +11,9 @@
 //    every row value is bit-identical to an IEEE evaluation in source order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <utility>
+
 #include "kernels.h"
 #include "mlgpu.h"
 
@@ -2353,6 +2356,67 @@ __global__ void __launch_bounds__(256) topn_tile_kernel(const uint64_t* in, int6
     for (int k = 0; k < W; k++) out[((uint64_t)blockIdx.x * keep + e) * W + k] = ~0ull;
 }
 
+// ---------------------------------------------------------- full sort
+// Merge sort of entries of W words (lexicographic over all W words; the
+// last word is the row index, so entries differ): LDS-sorted tiles of
+// TOPN_TILE entries, then passes merging pairs of sorted runs of length L
+// (merge path: each workgroup finds where its 1024 outputs split between
+// the two runs, stages both pieces in LDS and merges them there).
+constexpr int MERGE_OUT = 1024;  // outputs per workgroup in a merge pass
+
+__device__ __forceinline__ bool words_le(const uint64_t* a, const uint64_t* b, int W) {
+  for (int k = 0; k < W; k++)
+    if (a[k] != b[k]) return a[k] < b[k];
+  return true;
+}
+
+// Entries of A placed among the first k outputs of merge(A, B); A goes first on ties.
+__device__ __forceinline__ int64_t merge_corank(const uint64_t* A, int64_t na, const uint64_t* B, int64_t nb,
+                                                int64_t k, int W) {
+  int64_t lo = k > nb ? k - nb : 0, hi = k < na ? k : na;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (words_le(A + mid * W, B + (k - mid - 1) * W, W)) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) merge_pass_kernel(const uint64_t* in, uint64_t* out, int64_t N, int64_t L,
+                                                         int W) {
+  extern __shared__ uint64_t sh[];
+  __shared__ int64_t split[2];
+  const int64_t o0 = (int64_t)blockIdx.x * MERGE_OUT;
+  const int64_t ps = o0 / (2 * L) * (2 * L);
+  const int64_t na = min(L, N - ps);
+  const int64_t nb = max((int64_t)0, min(L, N - ps - L));
+  const uint64_t* A = in + ps * W;
+  const uint64_t* B = A + na * W;
+  const int64_t k0 = o0 - ps, k1 = min(k0 + MERGE_OUT, na + nb);
+  if (threadIdx.x == 0) split[0] = merge_corank(A, na, B, nb, k0, W);
+  if (threadIdx.x == 1) split[1] = merge_corank(A, na, B, nb, k1, W);
+  __syncthreads();
+  const int64_t i0 = split[0], i1 = split[1];
+  const int64_t j0 = k0 - i0, j1 = k1 - i1;
+  const int ca = (int)(i1 - i0), cb = (int)(j1 - j0);
+  uint64_t* sA = sh;
+  uint64_t* sB = sh + (int64_t)ca * W;
+  for (int e = threadIdx.x; e < ca * W; e += blockDim.x) sA[e] = A[i0 * W + e];
+  for (int e = threadIdx.x; e < cb * W; e += blockDim.x) sB[e] = B[j0 * W + e];
+  __syncthreads();
+  constexpr int PER = MERGE_OUT / 256;
+  const int t0 = (int)threadIdx.x * PER;
+  if (t0 >= ca + cb) return;
+  int ia = (int)merge_corank(sA, ca, sB, cb, t0, W);
+  int ib = t0 - ia;
+  uint64_t* dst = out + (o0 + t0) * W;
+  for (int e = 0; e < PER && t0 + e < ca + cb; e++) {
+    const bool takeA = ib >= cb || (ia < ca && words_le(sA + (int64_t)ia * W, sB + (int64_t)ib * W, W));
+    const uint64_t* src = takeA ? sA + (int64_t)(ia++) * W : sB + (int64_t)(ib++) * W;
+    for (int k = 0; k < W; k++) dst[(int64_t)e * W + k] = src[k];
+  }
+}
+
 __global__ void topn_rows_kernel(const uint64_t* words, int nw, int64_t n, int64_t* rows_out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) rows_out[i] = (int64_t)words[(uint64_t)i * (nw + 1) + nw];
@@ -2594,6 +2658,22 @@ void launch_topn_tiles(const uint64_t* in, int64_t n, int nw, int keep, uint64_t
   const int64_t tiles = (n + TOPN_TILE - 1) / TOPN_TILE;
   const size_t lds = (size_t)TOPN_TILE * (nw + 1) * 8;
   topn_tile_kernel<<<(unsigned)tiles, 256, lds, s>>>(in, n, nw, keep, out);
+}
+
+int64_t sort_padded_entries(int64_t n) { return std::max<int64_t>(1, (n + TOPN_TILE - 1) / TOPN_TILE) * TOPN_TILE; }
+
+uint64_t* launch_sort_words(const uint64_t* words, int64_t n, int nw, uint64_t* a, uint64_t* b, hipStream_t s) {
+  const int W = nw + 1;
+  const int64_t N = sort_padded_entries(n);
+  // sorted tiles (the entries past n are all-ones words, which sort last)
+  launch_topn_tiles(words, n, nw, TOPN_TILE, a, s);
+  uint64_t* cur = a;
+  uint64_t* nxt = b;
+  for (int64_t L = TOPN_TILE; L < N; L *= 2) {
+    merge_pass_kernel<<<(unsigned)(N / MERGE_OUT), 256, (size_t)MERGE_OUT * W * 8, s>>>(cur, nxt, N, L, W);
+    std::swap(cur, nxt);
+  }
+  return cur;
 }
 
 void launch_topn_rows(const uint64_t* words, int nw, int64_t n, int64_t* rows_out, hipStream_t s) {
