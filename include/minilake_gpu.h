@@ -78,9 +78,8 @@ typedef struct ml_column {
   int32_t _pad;
   const void* data;          /* device pointer, `len` values of the type's width */
   int64_t len;
-  const uint64_t* validity;  /* LSB-first bitmap; NULL = all valid. The GPU path
-                                requires NULL (every configured table is NULL-free)
-                                and returns ML_ERR_UNSUPPORTED otherwise. */
+  const uint64_t* validity;  /* device LSB-first bitmap (Bitmap, bitmap.rs:9-12),
+                                ceil(len / 64) words; NULL = all valid */
   const ml_dict* dict;       /* UTF8 only */
 } ml_column;
 
@@ -181,12 +180,12 @@ int32_t ml_ctx_take_stats(ml_ctx* ctx, ml_launch_stat* out, int32_t cap);
 /* Results: library-owned device columns (host copies on request). */
 int64_t ml_result_num_rows(const ml_result* r);
 int32_t ml_result_num_columns(const ml_result* r);
-/* Column i of the result; its data pointer is device memory. */
+/* Column i of the result; its data and validity bitmap are device memory,
+ * so a result can be the source batch of another pipeline. */
 const ml_column* ml_result_column(const ml_result* r, int32_t i);
 /* Copy column i to `dst`, host or device memory (num_rows * width bytes). */
 ml_status ml_result_copy_column(ml_ctx* ctx, const ml_result* r, int32_t i, void* dst);
-/* Copy column i's validity as one byte per row (1 = valid) to host memory.
- * Result columns carry validity this way; ml_column.validity of a result is NULL. */
+/* Copy column i's validity as one byte per row (1 = valid) to host memory. */
 ml_status ml_result_copy_validity(ml_ctx* ctx, const ml_result* r, int32_t i, uint8_t* dst);
 void ml_result_free(ml_result* r);
 

@@ -270,7 +270,11 @@ def count_star() -> Agg:
 
 
 def agg_spec(group_exprs: Sequence[Expr], key_types: Sequence[int], aggs: Sequence[Agg], keep: Keep,
-             sizing: int = SIZING_PLAIN, capacity_hint: int = 0) -> MlAggSpec:
+             sizing: int = SIZING_PLAIN, capacity_hint: int = 0, mode: int = AGG_MODE_SINGLE) -> MlAggSpec:
+    """AggMode (aggregate/mod.rs:50-57): SINGLE and PARTIAL evaluate group_exprs
+    and the aggregate arguments over the input; FINAL reads the input by
+    position instead, the group keys then each aggregate's state columns (what
+    a PARTIAL aggregate with the same aggregates emits)."""
     ga = expr_array(group_exprs, keep)
     kt = int_array(key_types, keep)
     aa = keep((MlAggregate * max(1, len(aggs)))())
@@ -278,7 +282,7 @@ def agg_spec(group_exprs: Sequence[Expr], key_types: Sequence[int], aggs: Sequen
         aa[i].func = a.func
         aa[i].arg_type = a.arg_type
         aa[i].arg = to_ml_expr(a.arg, keep)
-    return MlAggSpec(ga, kt, len(group_exprs), len(aggs), aa, AGG_MODE_SINGLE, sizing, capacity_hint)
+    return MlAggSpec(ga, kt, len(group_exprs), len(aggs), aa, mode, sizing, capacity_hint)
 
 
 def join_build_spec(keys: Sequence[Expr], key_types: Sequence[int], keep: Keep) -> MlJoinBuildSpec:

@@ -4,6 +4,8 @@
 #include <cmath>
 #include <deque>
 #include <numeric>
+#include <string_view>
+#include <unordered_set>
 
 #include "engine.h"
 
@@ -164,7 +166,6 @@ void PipelineBuild::source(const ml_batch* b, bool use_selection) {
   layout.clear();
   for (int i = 0; i < b->num_columns; i++) {
     const ml_column& mc = b->columns[i];
-    if (mc.validity) fail(ML_ERR_UNSUPPORTED, "the GPU path takes NULL-free input columns (validity must be NULL)");
     if (mc.len != b->num_rows) fail(ML_ERR_INTERNAL, "column length differs from batch rows");
     if (mc.type < ML_TYPE_BOOLEAN || mc.type > ML_TYPE_UTF8) fail(ML_ERR_PLAN, "bad column type");
     VCol c;
@@ -176,6 +177,13 @@ void PipelineBuild::source(const ml_batch* b, bool use_selection) {
       c.dict = ctx->dict(mc.dict);
     }
     if (b->num_rows > 0 && !mc.data) fail(ML_ERR_INTERNAL, "null column data");
+    if (mc.validity && b->num_rows > 0) {
+      // the pipeline reads validity as one byte per physical row
+      Buffer v(ctx->arena, (size_t)b->num_rows);
+      launch_bits_to_bytes(mc.validity, b->num_rows, v.as<uint8_t>(), ctx->stream);
+      c.valid = v.as<uint8_t>();
+      tables.push_back(std::move(v));
+    }
     layout.push_back(add_col(std::move(c)));
   }
   P.sel = (use_selection && b->selection) ? b->selection : nullptr;
@@ -668,7 +676,10 @@ struct ml_hashagg {
   ml_ctx* ctx = nullptr;
   std::vector<ml_expr> group_raw;
   std::vector<int> key_types;
-  std::vector<mlg::AggDef> aggs;
+  std::vector<mlg::AggDef> aggs;       // argument sources of the slots (Final: one per state column)
+  std::vector<mlg::AggDef> user_aggs;  // the aggregates as specified (func, arg_type)
+  std::vector<int> state_col;          // Final: index in aggs of each aggregate's first state column
+  int mode = ML_AGG_MODE_SINGLE;
   int sizing = ML_SIZING_PLAIN;
   int64_t hint = 0;
   // storage of copied expressions
@@ -711,13 +722,107 @@ static uint64_t next_pow2(uint64_t x) {
 // SwissTable::with_capacity sizing (crates/minilake-hashtable/src/swiss.rs:86-96)
 static uint64_t swiss_capacity(uint64_t n) { return std::max<uint64_t>(16, next_pow2(n * 8 / 7 + 1)); }
 
-// Plan accumulator slots (accumulator.rs:104-165 state kinds) for the aggregate list.
+static bool int_like(int t) { return t == ML_TYPE_INT32 || t == ML_TYPE_INT64 || t == ML_TYPE_DATE; }
+
+// Partial-state column types of one aggregate (Accumulator::state_types,
+// accumulator.rs:177-186).
+static std::vector<int> agg_state_types(int func, int arg_type) {
+  switch (func) {
+    case ML_AGG_COUNT_STAR: case ML_AGG_COUNT: return {ML_TYPE_INT64};
+    case ML_AGG_SUM:
+      if (arg_type == ML_TYPE_FLOAT64) return {ML_TYPE_FLOAT64};
+      if (int_like(arg_type) || arg_type == ML_TYPE_BOOLEAN) return {ML_TYPE_INT64};
+      break;
+    case ML_AGG_AVG:
+      if (int_like(arg_type) || arg_type == ML_TYPE_FLOAT64) return {ML_TYPE_FLOAT64, ML_TYPE_INT64};
+      break;
+    case ML_AGG_MIN: case ML_AGG_MAX:
+      if (int_like(arg_type)) return {ML_TYPE_INT64};
+      if (arg_type == ML_TYPE_FLOAT64) return {ML_TYPE_FLOAT64};
+      if (arg_type == ML_TYPE_UTF8) fail(ML_ERR_UNSUPPORTED, "min/max of strings");
+      break;
+    default: fail(ML_ERR_PLAN, "unknown aggregate function");
+  }
+  fail(ML_ERR_PLAN, std::string("aggregate of ") + type_name(arg_type) + " not supported");
+}
+
+// Final-value type of one aggregate (Accumulator::output_type).
+static int agg_final_type(int func, int arg_type) {
+  if (func == ML_AGG_COUNT || func == ML_AGG_COUNT_STAR) return ML_TYPE_INT64;
+  if (func == ML_AGG_AVG) return ML_TYPE_FLOAT64;
+  if (func == ML_AGG_SUM) return arg_type == ML_TYPE_FLOAT64 ? ML_TYPE_FLOAT64 : ML_TYPE_INT64;
+  return arg_type;
+}
+
+// Final mode (HashAggState::merge_partial, hash.rs:125-137; Accumulator::merge,
+// accumulator.rs:407-497): every state column is folded by the accumulator
+// that merges it. COUNT adds counts (always valid); SUM and MIN/MAX skip
+// NULL states and stay NULL without a valid one; AVG adds sums and counts and
+// finishes as sum / count (count 0: NULL).
+static void plan_final(ml_hashagg* A, PipelineBuild& B) {
+  auto add_slot = [&](int kind, int arg, int cast) {
+    for (size_t s = 0; s < A->slots.size(); s++)
+      if (A->slots[s].kind == kind && A->slots[s].arg == arg && A->slot_cast[s] == cast) return (int)s;
+    A->slots.push_back({kind, arg});
+    A->slot_cast.push_back(cast);
+    return (int)A->slots.size() - 1;
+  };
+  for (size_t a = 0; a < A->user_aggs.size(); a++) {
+    const AggDef& u = A->user_aggs[a];
+    const int sc = A->state_col[a];
+    const std::vector<int> types = agg_state_types(u.func, u.arg_type);
+    bool nullable = false;
+    for (size_t j = 0; j < types.size(); j++) {
+      std::vector<Instr> tmp;
+      std::vector<Buffer> tt;
+      Compiler c(B.ctx, tmp, tt);
+      int t = -1;
+      Prog p = c.compile(B.resolve(*A->aggs[(size_t)sc + j].raw), -1, &t);
+      if (t != types[j])
+        fail(ML_ERR_INTERNAL, std::string("aggregate merge got a ") + type_name(t) + " state, wants " + type_name(types[j]));
+      if (j == 0) nullable = p.nullable != 0;
+    }
+    OutDef o{agg_final_type(u.func, u.arg_type), FIN_COPY, 0, -1};
+    switch (u.func) {
+      case ML_AGG_COUNT_STAR: case ML_AGG_COUNT:
+        o.state = add_slot(ACC_SUM_I64, sc, ML_TYPE_INT64);
+        break;
+      case ML_AGG_SUM:
+        o.state = add_slot(types[0] == ML_TYPE_FLOAT64 ? ACC_SUM_F64 : ACC_SUM_I64, sc, types[0]);
+        o.count = nullable ? add_slot(ACC_COUNT, sc, types[0]) : 0;
+        break;
+      case ML_AGG_AVG:
+        o.finish = FIN_AVG;
+        o.state = add_slot(ACC_SUM_F64, sc, ML_TYPE_FLOAT64);
+        o.count = add_slot(ACC_SUM_I64, sc + 1, ML_TYPE_INT64);
+        break;
+      default: {  // MIN / MAX
+        const bool mn = u.func == ML_AGG_MIN;
+        const bool f = types[0] == ML_TYPE_FLOAT64;
+        if (!f) o.finish = u.arg_type == ML_TYPE_INT64 ? FIN_COPY : FIN_I32;
+        o.state = add_slot(f ? (mn ? ACC_MIN_F64 : ACC_MAX_F64) : (mn ? ACC_MIN_I64 : ACC_MAX_I64), sc, types[0]);
+        o.count = nullable ? add_slot(ACC_COUNT, sc, types[0]) : 0;
+        break;
+      }
+    }
+    A->outs.push_back(o);
+  }
+  if ((int)A->slots.size() > MAX_AGG_SLOTS) fail(ML_ERR_UNSUPPORTED, "too many aggregate states");
+}
+
+// Plan accumulator slots (accumulator.rs:104-165 state kinds) for the aggregate
+// list. Partial mode keeps Single mode's slots and emits the states instead of
+// the finished values (Accumulator::state, accumulator.rs:382-404).
 static void plan_agg(ml_hashagg* A, PipelineBuild& B) {
   A->slots.clear();
   A->slot_cast.clear();
   A->outs.clear();
   A->slots.push_back({ACC_COUNT, -1});  // rows of the group
   A->slot_cast.push_back(-1);
+  if (A->mode == ML_AGG_MODE_FINAL) {
+    plan_final(A, B);
+    return;
+  }
   auto add_slot = [&](int kind, int arg, int cast) {
     for (size_t s = 0; s < A->slots.size(); s++)
       if (A->slots[s].kind == kind && A->slots[s].arg == arg && A->slot_cast[s] == cast) return (int)s;
@@ -777,6 +882,17 @@ static void plan_agg(ml_hashagg* A, PipelineBuild& B) {
         break;
       }
       default: fail(ML_ERR_PLAN, "unknown aggregate function");
+    }
+    if (A->mode == ML_AGG_MODE_PARTIAL) {
+      if (d.func == ML_AGG_AVG) {  // (sum, count), both always valid
+        A->outs.push_back({ML_TYPE_FLOAT64, FIN_COPY, o.state, -1});
+        A->outs.push_back({ML_TYPE_INT64, FIN_COPY, o.count, -1});
+        continue;
+      }
+      if (d.func == ML_AGG_MIN || d.func == ML_AGG_MAX) {  // integer states are Int64
+        o.finish = FIN_COPY;
+        if (o.out_type != ML_TYPE_FLOAT64) o.out_type = ML_TYPE_INT64;
+      }
     }
     A->outs.push_back(o);
   }
@@ -1209,22 +1325,27 @@ static ml_result* agg_finalize(ml_hashagg* A) {
       c.type = A->key_types[k];
       r->cols.push_back(std::move(c));
     }
-    for (auto& d : A->aggs) {
-      ResultColumn c;
-      c.type = (d.func == ML_AGG_COUNT || d.func == ML_AGG_COUNT_STAR) ? ML_TYPE_INT64
-               : (d.func == ML_AGG_AVG ? ML_TYPE_FLOAT64 : (d.func == ML_AGG_SUM && d.arg_type != ML_TYPE_FLOAT64 ? ML_TYPE_INT64 : d.arg_type));
-      if (rows) {
-        const uint64_t zero = 0;
-        const uint8_t is_count = d.func == ML_AGG_COUNT || d.func == ML_AGG_COUNT_STAR;
-        c.data = Buffer(ctx->arena, 8);
-        upload(ctx, c.data.ptr, &zero, 8);
-        if (!is_count) {
-          const uint8_t none = 0;
-          c.valid = Buffer(ctx->arena, 1);
-          upload(ctx, c.valid.ptr, &none, 1);
+    // Partial mode: the fresh states (COUNT 0; AVG sum 0.0 and count 0; the
+    // others NULL, accumulator.rs:382-404); otherwise the finished values
+    for (auto& d : A->user_aggs) {
+      const bool is_count = d.func == ML_AGG_COUNT || d.func == ML_AGG_COUNT_STAR;
+      std::vector<int> types{agg_final_type(d.func, d.arg_type)};
+      if (A->mode == ML_AGG_MODE_PARTIAL) types = agg_state_types(d.func, d.arg_type);
+      for (int t : types) {
+        ResultColumn c;
+        c.type = t;
+        if (rows) {
+          const uint64_t zero = 0;
+          c.data = Buffer(ctx->arena, 8);
+          upload(ctx, c.data.ptr, &zero, 8);
+          if (!is_count && !(A->mode == ML_AGG_MODE_PARTIAL && d.func == ML_AGG_AVG)) {
+            const uint8_t none = 0;
+            c.valid = Buffer(ctx->arena, 1);
+            upload(ctx, c.valid.ptr, &none, 1);
+          }
         }
+        r->cols.push_back(std::move(c));
       }
-      r->cols.push_back(std::move(c));
     }
     r->num_rows = rows;
     r->finish_views();
@@ -2030,6 +2151,11 @@ void ml_result::finish_views() {
     v.data = c.data.ptr;
     v.len = num_rows;
     v.validity = nullptr;
+    if (c.valid.ptr && num_rows > 0) {  // the exported form of the byte-per-row validity
+      c.bitmap = mlg::Buffer(ctx->arena, (size_t)((num_rows + 63) / 64) * 8);
+      mlg::launch_bytes_to_bits(c.valid.as<uint8_t>(), num_rows, c.bitmap.as<uint64_t>(), ctx->stream);
+      v.validity = c.bitmap.as<uint64_t>();
+    }
     v.dict = c.dict ? &c.dict->view : nullptr;
     views.push_back(v);
   }
@@ -2064,6 +2190,10 @@ std::shared_ptr<mlg::HostDict> ml_ctx::dict(const ml_dict* d) {
   auto it = dicts.find(d);
   auto h = mlg::HostDict::from(d);
   if (it != dicts.end() && it->second->entries == h->entries) return it->second;
+  // string equality is code equality on the device (group keys, join keys)
+  std::unordered_set<std::string_view> seen;
+  for (const std::string& e : h->entries)
+    if (!seen.insert(e).second) mlg::fail(ML_ERR_UNSUPPORTED, "dictionary entries must be distinct");
   dicts[d] = h;
   return h;
 }
@@ -2332,28 +2462,61 @@ ml_status ml_hashagg_create(ml_ctx* ctx, const ml_agg_spec* spec, ml_hashagg** o
   A->ctx = ctx;
   ml_status st = guarded(ctx, [&] {
     if (spec->num_keys < 0 || spec->num_aggs < 0) fail(ML_ERR_PLAN, "bad aggregate spec");
-    if (spec->mode != ML_AGG_MODE_SINGLE) fail(ML_ERR_UNSUPPORTED, "partial/final aggregation modes on the GPU");
+    if (spec->mode < ML_AGG_MODE_SINGLE || spec->mode > ML_AGG_MODE_FINAL) fail(ML_ERR_PLAN, "bad aggregate mode");
+    A->mode = spec->mode;
+    for (int a = 0; a < spec->num_aggs; a++) {
+      AggDef u;
+      u.func = spec->aggs[a].func;
+      u.arg_type = spec->aggs[a].arg_type;
+      u.raw = nullptr;
+      A->user_aggs.push_back(u);
+    }
+    // Final mode reads its input by position, group keys first, then each
+    // aggregate's state columns (hash.rs:125-137, ungrouped.rs:100-113): the
+    // group expressions and aggregate arguments are replaced by column refs.
+    auto column_expr = [&](int c) {
+      ml_expr_node n{};
+      n.kind = ML_EXPR_COLUMN;
+      n.column = c;
+      ml_expr e{};
+      e.nodes = &n;
+      e.num_nodes = 1;
+      return own_expr(A, e);
+    };
+    const bool final_mode = A->mode == ML_AGG_MODE_FINAL;
     for (int k = 0; k < spec->num_keys; k++) {
-      A->group_raw.push_back(own_expr(A, spec->group_exprs[k]));
+      A->group_raw.push_back(final_mode ? column_expr(k) : own_expr(A, spec->group_exprs[k]));
       parse_expr(A->group_raw.back());
       A->key_types.push_back(spec->key_types[k]);
     }
-    for (int a = 0; a < spec->num_aggs; a++) {
-      AggDef d;
-      d.func = spec->aggs[a].func;
-      d.arg_type = spec->aggs[a].arg_type;
-      d.raw = nullptr;
-      if (d.func != ML_AGG_COUNT_STAR) {
-        A->group_raw.push_back(own_expr(A, spec->aggs[a].arg));
-        d.arg = parse_expr(A->group_raw.back());
+    if (final_mode) {
+      int col = spec->num_keys;
+      for (const AggDef& u : A->user_aggs) {
+        A->state_col.push_back((int)A->aggs.size());
+        for (int t : agg_state_types(u.func, u.arg_type)) {
+          AggDef d;
+          d.func = ML_AGG_SUM;
+          d.arg_type = t;
+          A->group_raw.push_back(column_expr(col++));
+          d.arg = parse_expr(A->group_raw.back());
+          d.raw = nullptr;
+          A->aggs.push_back(d);
+        }
       }
-      A->aggs.push_back(d);
+    } else {
+      for (int a = 0; a < spec->num_aggs; a++) {
+        AggDef d = A->user_aggs[(size_t)a];
+        if (d.func != ML_AGG_COUNT_STAR) {
+          A->group_raw.push_back(own_expr(A, spec->aggs[a].arg));
+          d.arg = parse_expr(A->group_raw.back());
+        }
+        A->aggs.push_back(d);
+      }
     }
-    // point each aggregate at its stored argument (after all push_backs)
+    // point each argument source at its stored expression (after all push_backs)
     size_t gi = (size_t)spec->num_keys;
     for (auto& d : A->aggs)
-      if (d.func != ML_AGG_COUNT_STAR) d.raw = &A->group_raw[gi++];
-    A->group_raw.resize((size_t)spec->num_keys + (gi - (size_t)spec->num_keys));
+      if (final_mode || d.func != ML_AGG_COUNT_STAR) d.raw = &A->group_raw[gi++];
     A->sizing = spec->sizing;
     A->hint = spec->capacity_hint;
     if (A->sizing != ML_SIZING_PLAIN && A->hint <= 0) A->hint = 1;

@@ -61,6 +61,9 @@ int flow_blocks_per_cu(int sink, bool fast, int block, size_t lds);
 void launch_scan_counts(const uint32_t* counts, int64_t n, uint64_t* offsets, hipStream_t s);
 void launch_materialize(const Plan& P, int sink, const uint64_t* offsets, int64_t nwaves, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
+// LSB-first validity bitmap (one bit per row) <-> one byte per row (1 = valid)
+void launch_bits_to_bytes(const uint64_t* bits, int64_t n, uint8_t* out, hipStream_t s);
+void launch_bytes_to_bits(const uint8_t* valid, int64_t n, uint64_t* bits, hipStream_t s);
 void launch_gather(const void* src, int width, const int64_t* idx, int64_t n, void* dst, const uint8_t* src_valid,
                    uint8_t* dst_valid, hipStream_t s);
 void launch_join_insert(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,

@@ -2417,6 +2417,24 @@ __global__ void __launch_bounds__(256) merge_pass_kernel(const uint64_t* in, uin
   }
 }
 
+// ------------------------------------------------------- validity formats
+__global__ void bits_to_bytes_kernel(const uint64_t* bits, int64_t n, uint8_t* out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = (uint8_t)((bits[i >> 6] >> (i & 63)) & 1ull);
+}
+
+// one wave per 64-row word: the wave's ballot is the word
+__global__ void bytes_to_bits_kernel(const uint8_t* valid, int64_t n, uint64_t* bits) {
+  const int64_t nw = (n + 63) / 64;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+  for (int64_t w = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; w < nw; w += waves) {
+    const int64_t r = w * 64 + (threadIdx.x & 63);
+    const uint64_t m = __ballot(r < n && valid[r] != 0);
+    if ((threadIdx.x & 63) == 0) bits[w] = m;
+  }
+}
+
 __global__ void topn_rows_kernel(const uint64_t* words, int nw, int64_t n, int64_t* rows_out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) rows_out[i] = (int64_t)words[(uint64_t)i * (nw + 1) + nw];
@@ -2674,6 +2692,16 @@ uint64_t* launch_sort_words(const uint64_t* words, int64_t n, int nw, uint64_t* 
     std::swap(cur, nxt);
   }
   return cur;
+}
+
+void launch_bits_to_bytes(const uint64_t* bits, int64_t n, uint8_t* out, hipStream_t s) {
+  if (n <= 0) return;
+  bits_to_bytes_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(bits, n, out);
+}
+
+void launch_bytes_to_bits(const uint8_t* valid, int64_t n, uint64_t* bits, hipStream_t s) {
+  if (n <= 0) return;
+  bytes_to_bits_kernel<<<grid_for((n + 63) / 64, 4, 8192), 256, 0, s>>>(valid, n, bits);
 }
 
 void launch_topn_rows(const uint64_t* words, int nw, int64_t n, int64_t* rows_out, hipStream_t s) {

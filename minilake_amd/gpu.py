@@ -27,7 +27,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import abi
-from .table import NP_DTYPE, HostColumn, HostTable, make_dict
+from .table import NP_DTYPE, HostColumn, HostTable, bitmap_from_bools, make_dict
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # MINILAKE_GPU_LIB selects another build of the library (e.g. one compiled with
@@ -179,11 +179,14 @@ class DeviceTable:
         self.num_rows = host.num_rows
         self.types = host.types
         self.tensors = []
+        self.bitmaps = []  # device validity bitmaps (None = all valid)
         for c in host.columns:
-            if c.validity is not None and not np.all(c.validity):
-                raise GpuError(abi.ML_ERR_UNSUPPORTED, "the GPU path takes NULL-free input columns")
             v = c.values.view(np.int32) if c.type == abi.UTF8 else c.values
             self.tensors.append(torch.from_numpy(np.ascontiguousarray(v)).to(device))
+            bm = None
+            if c.validity is not None and not np.all(c.validity) and len(c) > 0:
+                bm = torch.from_numpy(bitmap_from_bools(np.asarray(c.validity, bool)).view(np.int64)).to(device)
+            self.bitmaps.append(bm)
         self.dicts = [c.dictionary for c in host.columns]
 
     @classmethod
@@ -194,6 +197,7 @@ class DeviceTable:
         t.tensors = list(tensors)
         t.types = list(types)
         t.dicts = list(dicts) if dicts else [None] * len(t.tensors)
+        t.bitmaps = [None] * len(t.tensors)
         t.num_rows = int(num_rows if num_rows is not None else (t.tensors[0].numel() if t.tensors else 0))
         return t
 
@@ -207,6 +211,8 @@ class DeviceTable:
             cols[i].type = ty
             cols[i].data = t.data_ptr()
             cols[i].len = self.num_rows
+            if self.bitmaps[i] is not None:
+                cols[i].validity = C.cast(C.c_void_p(self.bitmaps[i].data_ptr()), C.POINTER(C.c_uint64))
             if ty == abi.UTF8:
                 key = id(self.dicts[i])
                 if key not in dcache:
@@ -402,10 +408,10 @@ class GpuEngine:
         return GpuJoin(t.value)
 
     def hashagg(self, src, ops, group_exprs, key_types, aggs, sizing=abi.SIZING_PLAIN, capacity_hint=0,
-                stats: Optional[dict] = None) -> GpuResult:
+                stats: Optional[dict] = None, mode: int = abi.AGG_MODE_SINGLE) -> GpuResult:
         L, keep = lib(), abi.Keep()
         _, b = self._src(src, keep)
-        spec = abi.agg_spec(group_exprs, key_types, aggs, keep, sizing, capacity_hint)
+        spec = abi.agg_spec(group_exprs, key_types, aggs, keep, sizing, capacity_hint, mode)
         h = C.c_void_p()
         self._check(L.ml_hashagg_create(self.ctx, C.byref(spec), C.byref(h)))
         try:

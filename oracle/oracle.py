@@ -180,11 +180,12 @@ class OracleEngine:
                                              C.byref(self.opts), C.byref(out)))
         return OracleJoin(out.value)
 
-    def hashagg(self, src, ops, group_exprs, key_types, aggs, sizing=abi.SIZING_PLAIN, capacity_hint=0):
+    def hashagg(self, src, ops, group_exprs, key_types, aggs, sizing=abi.SIZING_PLAIN, capacity_hint=0,
+                mode=abi.AGG_MODE_SINGLE):
         keep = abi.Keep()
         src = self._host(src)
         b = host_batch(src, keep)
-        spec = abi.agg_spec(group_exprs, key_types, aggs, keep, sizing, capacity_hint)
+        spec = abi.agg_spec(group_exprs, key_types, aggs, keep, sizing, capacity_hint, mode)
         out = C.c_void_p()
         _check(lib().mlo_pipeline_hashagg(C.byref(b), self._ops(ops, keep), len(ops), C.byref(spec),
                                           C.byref(self.opts), C.byref(out)))

@@ -383,11 +383,14 @@ struct Sink {
   virtual void finalize() = 0;
 };
 
-// HashAggregateSink (aggregate/hash.rs:285-484), Single mode, no spill.
+// HashAggregateSink (aggregate/hash.rs:285-484), no spill. Partial mode emits
+// each group's states (partial_batch); Final mode reads rows of states,
+// keys first, and merges them (HashAggLocal::sink, hash.rs:429-437).
 struct HashAggSink : Sink {
   std::vector<ExprPtr> group_exprs;
   std::vector<int> key_types;
   std::vector<AggDef> aggs;
+  int mode = ML_AGG_MODE_SINGLE;
   std::mutex mu;
   std::unique_ptr<HashAggState> global;
   Batch output;
@@ -398,7 +401,10 @@ struct HashAggSink : Sink {
     HashAggSink* s;
     HashAggState state;
     Local(HashAggSink* sink) : s(sink), state(sink->key_types, sink->aggs) {}
-    void sink(const Batch& b) override { state.update(b, s->group_exprs, s->aggs); }
+    void sink(const Batch& b) override {
+      if (s->mode == ML_AGG_MODE_FINAL) state.merge_partial(b);
+      else state.update(b, s->group_exprs, s->aggs);
+    }
     void combine() override {  // hash.rs:439-483
       std::lock_guard<std::mutex> g(s->mu);
       s->resizes += state.table.resizes;
@@ -408,13 +414,10 @@ struct HashAggSink : Sink {
   };
   std::unique_ptr<LocalSink> create_local() override { return std::make_unique<Local>(this); }
   void finalize() override {  // hash.rs:369-425
-    if (global) {
-      output = global->final_batch();
-      capacity = global->table.capacity();
-    } else {
-      HashAggState empty(key_types, aggs);
-      output = empty.final_batch();
-    }
+    HashAggState empty(key_types, aggs);
+    const HashAggState& st = global ? *global : empty;
+    output = mode == ML_AGG_MODE_PARTIAL ? st.partial_batch() : st.final_batch();
+    if (global) capacity = global->table.capacity();
   }
 };
 
@@ -424,6 +427,7 @@ struct HashAggSink : Sink {
 // SUM/AVG/MIN/MAX NULL, ungrouped.rs:82-91).
 struct UngroupedAggSink : Sink {
   std::vector<AggDef> aggs;
+  int mode = ML_AGG_MODE_SINGLE;
   std::mutex mu;
   std::unique_ptr<std::vector<Accumulator>> global;
   Batch output;
@@ -442,6 +446,16 @@ struct UngroupedAggSink : Sink {
     std::vector<uint32_t> gids;
     Local(UngroupedAggSink* sink) : s(sink), accs(sink->fresh()) {}
     void sink(const Batch& b) override {  // ungrouped.rs:100-121
+      if (s->mode == ML_AGG_MODE_FINAL) {  // rows of states, all into the one group
+        Batch c = compact(b);
+        gids.assign(c.num_rows, 0u);
+        size_t col = 0;
+        for (auto& acc : accs) {
+          acc.merge(&c.columns[col], gids);
+          col += acc.num_state_cols();
+        }
+        return;
+      }
       Rows rows = batch_rows(b);
       gids.assign(rows.len(), 0u);
       for (size_t a = 0; a < s->aggs.size(); a++) {
@@ -467,7 +481,12 @@ struct UngroupedAggSink : Sink {
   void finalize() override {  // ungrouped.rs:75-96
     std::vector<Accumulator> accs = global ? std::move(*global) : fresh();
     output.columns.clear();
-    for (auto& acc : accs) output.columns.push_back(acc.finish());
+    for (auto& acc : accs) {
+      if (mode == ML_AGG_MODE_PARTIAL)
+        for (auto& c : acc.state()) output.columns.push_back(c);
+      else
+        output.columns.push_back(acc.finish());
+    }
     output.num_rows = 1;
   }
 };
@@ -1135,6 +1154,7 @@ int mlo_pipeline_hashagg(const ml_batch* src, const ml_operator* ops, int32_t no
     if (spec->num_keys == 0) {  // no GROUP BY: the planner's UngroupedAggregate
       UngroupedAggSink sink;
       sink.aggs = aggs;
+      sink.mode = spec->mode;
       run_pipeline(s, opdefs, sink, threads_of(opt));
       *out = export_batch(sink.output);
       return;
@@ -1145,6 +1165,7 @@ int mlo_pipeline_hashagg(const ml_batch* src, const ml_operator* ops, int32_t no
       sink.key_types.push_back(spec->key_types[k]);
     }
     sink.aggs = aggs;
+    sink.mode = spec->mode;
     run_pipeline(s, opdefs, sink, threads_of(opt));
     *out = export_batch(sink.output);
     (*out)->resizes = sink.resizes;

@@ -54,3 +54,24 @@ def assert_tables_equal(got: HostTable, want: HostTable, rtol=1e-12, atol=0.0, o
                 assert float_close(x, y, rtol, atol), f"row {k} col {i}: {x!r} != {y!r} (rel tol {rtol})"
             else:
                 assert x == y, f"row {k} col {i}: {x!r} != {y!r}"
+
+
+def concat_tables(parts: Sequence[HostTable]) -> HostTable:
+    """Row-wise concatenation of tables of one schema; string columns are
+    re-encoded over one dictionary of distinct entries."""
+    from minilake_amd.table import HostColumn
+    cols = []
+    for i, c0 in enumerate(parts[0].columns):
+        cs = [p.columns[i] for p in parts]
+        valid = None
+        if any(c.validity is not None for c in cs):
+            valid = np.concatenate([np.ones(len(c), bool) if c.validity is None else np.asarray(c.validity, bool)
+                                    for c in cs])
+        if c0.type == abi.UTF8:
+            strings = [c.dictionary[k] for c in cs for k in c.values]
+            entries = sorted(set(strings))
+            code = {e: k for k, e in enumerate(entries)}
+            cols.append(HostColumn(c0.type, np.array([code[x] for x in strings], np.uint32), entries, valid))
+        else:
+            cols.append(HostColumn(c0.type, np.concatenate([c.values for c in cs]), None, valid))
+    return HostTable(cols)

@@ -10,7 +10,7 @@ from minilake_amd import abi
 from minilake_amd.abi import (AGG_AVG, AGG_COUNT, AGG_COUNT_STAR, AGG_MAX, AGG_MIN, AGG_SUM, BOOLEAN, DATE,
                               FLOAT64, INT32, INT64, UTF8, Agg, col, date_lit, lit)
 from minilake_amd.table import HostColumn, HostTable
-from tests.compare import assert_tables_equal, rows_of
+from tests.compare import assert_tables_equal, concat_tables, rows_of
 
 pytestmark = pytest.mark.gpu
 
@@ -432,6 +432,101 @@ def test_ungrouped_aggregate_matches_oracle(gpu_engine, oracle_engine):
         got = gpu_engine.hashagg(t, ops, [], [], aggs).to_host()
         assert got.num_rows == want.num_rows == 1
         assert_tables_equal(got, want, rtol=1e-12)
+
+
+def _with_nulls(t, cols, seed, frac=0.2):
+    rng = np.random.default_rng(seed)
+    for i in cols:
+        t.columns[i].validity = rng.random(t.num_rows) >= frac
+    return t
+
+
+def test_nullable_source_columns(gpu_engine, oracle_engine):
+    """Input columns with validity bitmaps (Column, column.rs:126-200): filters,
+    projections, group keys, aggregate arguments and join keys see the NULLs
+    as the oracle does (NULL keys never join, hash.rs group NULL keys together),
+    and a device result holding NULLs keeps them as the source of the next
+    pipeline."""
+    t = _with_nulls(mixed_table(20_000, seed=41), (0, 1, 2, 5), seed=41)
+    for pred in (col(0).lt(lit(0)) | col(2).gt(lit(10.0)), col(5).eq(lit("banana")), ~(col(1).ge(lit(0)))):
+        np.testing.assert_array_equal(gpu_engine.filter(pred, t), oracle_engine.filter(pred, t))
+    for e in (col(0) + col(6), col(2) * lit(2.0), col(1) % col(6)):
+        want, got = oracle_engine.project([e], t), gpu_engine.project([e], t)
+        assert rows_of(got) == rows_of(want)
+    aggs = [Agg(AGG_SUM, col(0), INT32), Agg(AGG_COUNT, col(2), FLOAT64), Agg(AGG_AVG, col(0), INT32),
+            Agg(AGG_MAX, col(2), FLOAT64), Agg(AGG_MIN, col(1), INT64), Agg(AGG_COUNT_STAR)]
+    for keys, kt in (([col(5), col(6)], [UTF8, INT32]), ([col(0)], [INT32]), ([], [])):
+        want = oracle_engine.hashagg(t, [], keys, kt, aggs).to_host()
+        got = gpu_engine.hashagg(t, [], keys, kt, aggs).to_host()
+        assert_tables_equal(got, want, rtol=1e-12)
+    # a collected device result with NULLs as the next source
+    ops = [abi.filter_op(col(6).ne(lit(2)))]
+    mid_o = oracle_engine.collect(t, ops).to_host()
+    mid_g = gpu_engine.collect(t, ops)
+    assert rows_of(mid_g.to_host()) == rows_of(mid_o)
+    want = oracle_engine.hashagg(mid_o, [], [col(6)], [INT32], aggs).to_host()
+    got = gpu_engine.hashagg(mid_g, [], [col(6)], [INT32], aggs).to_host()
+    assert_tables_equal(got, want, rtol=1e-12)
+    # NULL join keys on both sides, duplicate build keys
+    build = HostTable([t.columns[0], HostColumn(INT32, np.arange(t.num_rows, dtype=np.int32))]).head(3000)
+    probe = HostTable([t.columns[0]]).take(np.arange(3000, 9000))
+    jo = oracle_engine.join_build(build, [], [col(0)], [INT32])
+    jg = gpu_engine.join_build(build, [], [col(0)], [INT32])
+    po, bo = oracle_engine.join_probe(jo, [col(0)], probe)
+    pg, bg = gpu_engine.join_probe(jg, [col(0)], probe)
+    ids_g = jg.column(1)
+    assert sorted(zip(pg.tolist(), ids_g[bg].tolist())) == sorted(zip(po.tolist(), bo.tolist()))
+    assert len(po) > 0
+
+
+def _modes_table(n, seed):
+    rng = np.random.default_rng(seed)
+    return HostTable([HostColumn(INT32, rng.integers(-3, 4, n).astype(np.int32)),
+                      HostColumn(FLOAT64, rng.uniform(1, 100, n)),
+                      HostColumn(INT64, rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)),
+                      HostColumn(DATE, rng.integers(8000, 9000, n).astype(np.int32)),
+                      HostColumn(UTF8, rng.integers(0, 4, n).astype(np.uint32), ["pear", "apple", "fig", "date"]),
+                      HostColumn(INT32, rng.integers(0, 1000, n).astype(np.int32))])
+
+
+_NULLY = col(1) / (col(0) * col(0))  # positive, NULL where col 0 is 0
+_MODE_AGGS = [Agg(AGG_COUNT_STAR), Agg(AGG_SUM, col(5), INT32), Agg(AGG_SUM, _NULLY, FLOAT64), Agg(AGG_AVG, col(5), INT32),
+              Agg(AGG_AVG, _NULLY, FLOAT64), Agg(AGG_MIN, col(3), DATE), Agg(AGG_MAX, col(2), INT64),
+              Agg(AGG_MIN, _NULLY, FLOAT64), Agg(AGG_COUNT, _NULLY, FLOAT64), Agg(AGG_MAX, col(0), INT32)]
+
+
+@pytest.mark.parametrize("grouped", ["none", "small", "wide"])
+def test_partial_final_aggregate_matches_oracle(gpu_engine, oracle_engine, grouped):
+    """AggMode Partial / Final (aggregate/mod.rs:50-57, split.rs:34-125): the
+    GPU's partial states of three slices equal the oracle's (Accumulator::state,
+    accumulator.rs:382-404); a Final aggregate over their concatenation equals
+    the oracle's Final over the same rows and the Single result over the whole
+    input. Keys: none; an int and a dictionary string (a direct table; the
+    group with key 0 has only NULL arguments); a wide i64 (a hashed table)."""
+    t = _modes_table(40_000, seed=31)
+    keys, kt = {"none": ([], []), "small": ([col(0), col(4)], [INT32, UTF8]),
+                "wide": ([col(2) % lit(20_000), col(0)], [INT64, INT32])}[grouped]
+    cuts = [0, 9_000, 9_001, t.num_rows]
+    parts = []
+    for a, b in zip(cuts, cuts[1:]):
+        piece = t.take(np.arange(a, b))
+        want = oracle_engine.hashagg(piece, [], keys, kt, _MODE_AGGS, mode=abi.AGG_MODE_PARTIAL).to_host()
+        got = gpu_engine.hashagg(piece, [], keys, kt, _MODE_AGGS, mode=abi.AGG_MODE_PARTIAL).to_host()
+        assert_tables_equal(got, want, rtol=1e-12)
+        parts.append(want)
+    states = concat_tables(parts)
+    want = oracle_engine.hashagg(states, [], keys, kt, _MODE_AGGS, mode=abi.AGG_MODE_FINAL).to_host()
+    got = gpu_engine.hashagg(states, [], keys, kt, _MODE_AGGS, mode=abi.AGG_MODE_FINAL).to_host()
+    assert_tables_equal(got, want, rtol=1e-12)
+    assert_tables_equal(got, oracle_engine.hashagg(t, [], keys, kt, _MODE_AGGS).to_host(), rtol=1e-12)
+    # no input rows: Partial emits the fresh states, Final the empty finish
+    none = [abi.filter_op(col(0).gt(lit(5)))]
+    for mode, src in ((abi.AGG_MODE_PARTIAL, t), (abi.AGG_MODE_FINAL, states)):
+        ops = none if mode == abi.AGG_MODE_PARTIAL else [abi.filter_op(col(len(kt)).lt(lit(0)))]
+        want = oracle_engine.hashagg(src, ops, keys, kt, _MODE_AGGS, mode=mode).to_host()
+        got = gpu_engine.hashagg(src, ops, keys, kt, _MODE_AGGS, mode=mode).to_host()
+        assert got.num_rows == want.num_rows == (1 if grouped == "none" else 0)
+        assert_tables_equal(got, want, rtol=0)
 
 
 @pytest.mark.parametrize("limit,offset", [(10, 0), (7, 123), (100_000, 4_000), (5, 10_000_000), (0, 0)])

@@ -19,7 +19,7 @@ from minilake_amd.abi import (AGG_AVG, AGG_COUNT_STAR, AGG_MIN, AGG_SUM, DATE, F
                               date_lit, lit)
 from minilake_amd.table import HostColumn, HostTable
 from oracle import oracle as O
-from tests.compare import assert_tables_equal, rows_of
+from tests.compare import assert_tables_equal, concat_tables, rows_of
 
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "hash_vectors.json")))
 M64 = (1 << 64) - 1
@@ -444,3 +444,57 @@ def test_limit_offset_slices_source_order():
     assert eng.limit(t, ops, 1000, 80).to_host().columns[0].values.tolist() == list(range(91, 100))
     assert eng.limit(t, ops, 5, 500).to_host().num_rows == 0
     assert eng.limit(t, ops, 0, 0).to_host().num_rows == 0
+
+
+def _modes_table(n=3000, seed=21):
+    rng = np.random.default_rng(seed)
+    return HostTable([HostColumn(INT32, rng.integers(-3, 4, n).astype(np.int32)),
+                      HostColumn(FLOAT64, rng.uniform(1, 100, n)),
+                      HostColumn(INT64, rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)),
+                      HostColumn(DATE, rng.integers(8000, 9000, n).astype(np.int32))])
+
+
+_MODE_AGGS = [Agg(AGG_COUNT_STAR), Agg(AGG_SUM, col(2), INT64), Agg(AGG_SUM, col(1) / col(0), FLOAT64),
+              Agg(AGG_AVG, col(1) / col(0), FLOAT64), Agg(AGG_MIN, col(3), DATE), Agg(abi.AGG_MAX, col(2), INT64),
+              Agg(abi.AGG_COUNT, col(1) / col(0), FLOAT64), Agg(abi.AGG_MAX, col(1) / col(0), FLOAT64)]
+
+
+@pytest.mark.parametrize("grouped", [True, False])
+def test_partial_then_final_equals_single(grouped):
+    """AggMode (aggregate/mod.rs:50-57; split.rs:34-125): Partial states of
+    three slices, concatenated and merged by a Final aggregate, equal the
+    Single result (SUM/COUNT/MIN/MAX/AVG are decomposable). Group 0 has only
+    NULL arguments (x / 0), so its SUM/AVG/MIN states are NULL or (0.0, 0)."""
+    t = _modes_table()
+    eng = O.OracleEngine(threads=3, morsel_rows=257)
+    keys, kt = ([col(0)], [INT32]) if grouped else ([], [])
+    single = eng.hashagg(t, [], keys, kt, _MODE_AGGS).to_host()
+    cuts = [0, 1000, 1001, t.num_rows]
+    parts = [eng.hashagg(t.take(np.arange(a, b)), [], keys, kt, _MODE_AGGS, mode=abi.AGG_MODE_PARTIAL).to_host()
+             for a, b in zip(cuts, cuts[1:])]
+    states = concat_tables(parts)
+    # group keys, then 1+1+1+2+1+1+1+1 state columns
+    assert len(states.columns) == len(kt) + 9
+    final = eng.hashagg(states, [], keys, kt, _MODE_AGGS, mode=abi.AGG_MODE_FINAL).to_host()
+    assert_tables_equal(final, single, rtol=1e-12)
+
+
+def test_partial_states_closed_forms():
+    """Accumulator::state (accumulator.rs:382-404): COUNT a count; SUM and MIN
+    the value with NULL when no argument was valid; AVG (sum, count) with no
+    NULLs; integer MIN/MAX states are Int64. Ungrouped over no rows: the fresh
+    states (COUNT 0, AVG (0.0, 0), the others NULL)."""
+    t = HostTable([HostColumn(INT32, np.array([1, 1, 2, 2], np.int32)),
+                   HostColumn(FLOAT64, np.array([1.0, 3.0, 5.0, 7.0])),
+                   HostColumn(INT32, np.array([1, 1, 0, 0], np.int32))])
+    aggs = [Agg(AGG_COUNT_STAR), Agg(AGG_SUM, col(1) / col(2), FLOAT64), Agg(AGG_AVG, col(1) / col(2), FLOAT64),
+            Agg(AGG_MIN, col(0), INT32)]
+    eng = O.OracleEngine(threads=1)
+    got = eng.hashagg(t, [], [col(0)], [INT32], aggs, mode=abi.AGG_MODE_PARTIAL).to_host()
+    assert [c.type for c in got.columns] == [INT32, INT64, FLOAT64, FLOAT64, INT64, INT64]
+    assert sorted(rows_of(got)) == [(1, 2, 4.0, 4.0, 2, 1), (2, 2, None, 0.0, 0, 2)]
+    empty = eng.hashagg(t, [abi.filter_op(col(0).gt(lit(5)))], [], [], aggs, mode=abi.AGG_MODE_PARTIAL).to_host()
+    assert rows_of(empty) == [(0, None, 0.0, 0, None)]
+    fin = eng.hashagg(empty, [], [], [], aggs, mode=abi.AGG_MODE_FINAL).to_host()
+    assert rows_of(fin) == [(0, None, None, None)]
+    assert fin.columns[3].type == INT32
