@@ -190,6 +190,22 @@ ml_status ml_result_copy_validity(ml_ctx* ctx, const ml_result* r, int32_t i, ui
 void ml_result_free(ml_result* r);
 
 /* ------------------------------------------------------------------ */
+/* MLB1 batch bytes (crates/minilake-core/src/ipc.rs:1-179): the format of
+ * aggregate spill files and of partial aggregates shipped between workers. */
+/* ------------------------------------------------------------------ */
+/* write_batch (ipc.rs:44-87) on the device: the batch's selection is applied,
+ * dictionary columns are written as plain strings, and a column carries
+ * validity words only when a kept row is NULL. *size receives the byte count.
+ * With dst != NULL (device memory, cap bytes) the bytes are written there
+ * (cap < size: ML_ERR_CAPACITY); the call returns when they are written. */
+ml_status ml_mlb1_write(ml_ctx* ctx, const ml_batch* batch, void* dst, int64_t cap, int64_t* size);
+/* read_batch (ipc.rs:111-170): one batch from `len` bytes of host memory into
+ * a device result (strings dictionary-encoded); *consumed = bytes used. Fewer
+ * than 4 bytes is a clean end of stream (*out = NULL, ML_OK); a batch cut
+ * short is ML_ERR_IO; a bad magic, type tag or string offsets ML_ERR_INTERNAL. */
+ml_status ml_mlb1_read(ml_ctx* ctx, const void* src, int64_t len, int64_t* consumed, ml_result** out);
+
+/* ------------------------------------------------------------------ */
 /* Stateless operators                                                */
 /* ------------------------------------------------------------------ */
 

@@ -2,9 +2,12 @@
 // planning and the sinks behind the C ABI (include/minilake_gpu.h).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <deque>
+#include <memory>
 #include <numeric>
 #include <string_view>
+#include <unordered_map>
 #include <unordered_set>
 
 #include "engine.h"
@@ -2730,6 +2733,198 @@ ml_status ml_pipeline_limit(ml_ctx* ctx, const ml_batch* source, const ml_operat
       m.rows = len;
     }
     *out = make_result(ctx, m);
+  });
+}
+
+// ------------------------------------------------------------- MLB1 bytes
+// write_batch (ipc.rs:44-87): per column a 2-byte header (type tag = the
+// ml_type value, validity flag), the validity words when a kept row is NULL,
+// then the payload; strings as u32 offsets, u64 byte count and the bytes.
+ml_status ml_mlb1_write(ml_ctx* ctx, const ml_batch* b, void* dst, int64_t cap, int64_t* size) {
+  return guarded(ctx, [&] {
+    if (!b || !size) fail(ML_ERR_INTERNAL, "null argument");
+    hipStream_t s = ctx->stream;
+    const uint32_t* sel = b->selection;
+    const int64_t n = sel ? b->selection_len : b->num_rows;
+    const int nc = b->num_columns;
+    struct Prep {
+      Buffer valid, offs, dict_off, dict_bytes;
+      uint32_t ndict = 0;
+    };
+    std::vector<Prep> prep((size_t)std::max(nc, 0));
+    Buffer stats(ctx->arena, (size_t)std::max(nc, 1) * 16);  // per column: NULL rows, string bytes
+    launch_fill_u64(stats.as<uint64_t>(), 0, (int64_t)std::max(nc, 1) * 2, s);
+    for (int c = 0; c < nc; c++) {
+      const ml_column& mc = b->columns[c];
+      Prep& P = prep[(size_t)c];
+      if (mc.type < ML_TYPE_BOOLEAN || mc.type > ML_TYPE_UTF8) fail(ML_ERR_PLAN, "bad column type");
+      if (mc.len != b->num_rows) fail(ML_ERR_INTERNAL, "column length differs from batch rows");
+      if (n > 0 && !mc.data) fail(ML_ERR_INTERNAL, "null column data");
+      if (mc.validity && n > 0) {
+        P.valid = Buffer(ctx->arena, (size_t)b->num_rows);
+        launch_bits_to_bytes(mc.validity, b->num_rows, P.valid.as<uint8_t>(), s);
+        launch_null_count(P.valid.as<uint8_t>(), sel, n, stats.as<uint64_t>() + 2 * c, s);
+      }
+      if (mc.type == ML_TYPE_UTF8) {
+        if (!mc.dict) fail(ML_ERR_UNSUPPORTED, "UTF8 columns must be dictionary-encoded");
+        auto d = HostDict::from(mc.dict);
+        P.ndict = (uint32_t)d->entries.size();
+        P.dict_off = Buffer(ctx->arena, d->offsets.size() * 4);
+        upload(ctx, P.dict_off.ptr, d->offsets.data(), d->offsets.size() * 4);
+        P.dict_bytes = Buffer(ctx->arena, std::max<size_t>(d->bytes.size(), 1));
+        upload(ctx, P.dict_bytes.ptr, d->bytes.data(), d->bytes.size());
+        Buffer len(ctx->arena, (size_t)std::max<int64_t>(n, 1) * 4);
+        launch_mlb1_str_len((const uint32_t*)mc.data, sel, n, P.dict_off.as<uint32_t>(), P.ndict,
+                            len.as<uint32_t>(), s);
+        P.offs = Buffer(ctx->arena, (size_t)(n + 1) * 4);
+        if (n > 0) {
+          Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(n) + 1) * 4);
+          launch_scan_u32(len.as<uint32_t>(), n, P.offs.as<uint32_t>(), sums.as<uint32_t>(), s);
+          d2d(ctx, stats.as<uint64_t>() + 2 * c + 1, P.offs.as<uint32_t>() + n, 4);  // low half (little-endian)
+        } else {
+          launch_fill_u64(stats.as<uint64_t>() + 2 * c + 1, 0, 1, s);
+          const uint32_t zero = 0;
+          upload(ctx, P.offs.ptr, &zero, 4);
+        }
+      }
+    }
+    std::vector<uint64_t> st((size_t)std::max(nc, 1) * 2);
+    d2h(ctx, st.data(), stats.ptr, st.size() * 8);
+    // layout
+    const int64_t words = (n + 63) / 64;
+    int64_t total = 16;
+    for (int c = 0; c < nc; c++) {
+      const int t = b->columns[c].type;
+      total += 2 + (st[2 * c] ? words * 8 : 0);
+      total += t == ML_TYPE_UTF8 ? (n + 1) * 4 + 8 + (int64_t)st[2 * c + 1] : n * type_width(t);
+    }
+    *size = total;
+    if (!dst) return;
+    if (cap < total) fail(ML_ERR_CAPACITY, "MLB1 destination smaller than the batch");
+    uint8_t* out = static_cast<uint8_t*>(dst);
+    uint8_t hdr[16] = {'M', 'L', 'B', '1'};
+    const uint32_t nc32 = (uint32_t)nc;
+    const uint64_t n64 = (uint64_t)n;
+    std::memcpy(hdr + 4, &nc32, 4);
+    std::memcpy(hdr + 8, &n64, 8);
+    upload(ctx, out, hdr, 16);
+    int64_t pos = 16;
+    for (int c = 0; c < nc; c++) {
+      const ml_column& mc = b->columns[c];
+      const Prep& P = prep[(size_t)c];
+      const uint8_t ch[2] = {(uint8_t)mc.type, (uint8_t)(st[2 * c] ? 1 : 0)};
+      upload(ctx, out + pos, ch, 2);
+      pos += 2;
+      if (st[2 * c]) {
+        launch_mlb1_pack_validity(P.valid.as<uint8_t>(), sel, n, out + pos, s);
+        pos += words * 8;
+      }
+      if (mc.type == ML_TYPE_UTF8) {
+        launch_mlb1_pack_fixed(P.offs.ptr, 4, false, nullptr, n + 1, out + pos, s);
+        pos += (n + 1) * 4;
+        upload(ctx, out + pos, &st[2 * c + 1], 8);
+        pos += 8;
+        launch_mlb1_pack_strings((const uint32_t*)mc.data, sel, n, P.dict_off.as<uint32_t>(),
+                                 P.dict_bytes.as<uint8_t>(), P.ndict, P.offs.as<uint32_t>(), out + pos, s);
+        pos += (int64_t)st[2 * c + 1];
+      } else {
+        const int w = type_width(mc.type);
+        launch_mlb1_pack_fixed(mc.data, w, mc.type == ML_TYPE_BOOLEAN, sel, n, out + pos, s);
+        pos += n * w;
+      }
+    }
+    if (pos != total) fail(ML_ERR_INTERNAL, "MLB1 layout mismatch");
+    hip_check(hipStreamSynchronize(s), "synchronize");
+  });
+}
+
+// read_batch (ipc.rs:111-170) from host bytes into device columns; strings
+// are dictionary-encoded again (entries in first-seen order).
+ml_status ml_mlb1_read(ml_ctx* ctx, const void* src, int64_t len, int64_t* consumed, ml_result** out) {
+  if (out) *out = nullptr;
+  if (consumed) *consumed = 0;
+  return guarded(ctx, [&] {
+    if (!out || !consumed || (len > 0 && !src)) fail(ML_ERR_INTERNAL, "null argument");
+    const uint8_t* p = static_cast<const uint8_t*>(src);
+    const size_t avail = (size_t)std::max<int64_t>(len, 0);
+    size_t pos = 0;
+    auto take = [&](size_t k) {
+      if (avail - pos < k) fail(ML_ERR_IO, "failed to fill whole buffer");
+      const uint8_t* q = p + pos;
+      pos += k;
+      return q;
+    };
+    auto u32 = [&]() { uint32_t x; std::memcpy(&x, take(4), 4); return x; };
+    auto u64 = [&]() { uint64_t x; std::memcpy(&x, take(8), 8); return x; };
+    if (avail < 4) return;  // a clean end of stream
+    if (std::memcmp(take(4), "MLB1", 4) != 0) fail(ML_ERR_INTERNAL, "bad batch magic");
+    const uint32_t nc = u32();
+    const uint64_t n64 = u64();
+    if (n64 > (uint64_t)INT64_MAX / 8) fail(ML_ERR_IO, "failed to fill whole buffer");
+    const int64_t n = (int64_t)n64;
+    std::unique_ptr<ml_result> r(new ml_result());
+    r->ctx = ctx;
+    r->device = ctx->device;
+    r->num_rows = n;
+    for (uint32_t ci = 0; ci < nc; ci++) {
+      const uint8_t* hdr = take(2);
+      if (hdr[0] > 5) fail(ML_ERR_INTERNAL, "bad type tag " + std::to_string(hdr[0]));
+      ResultColumn c;
+      c.type = hdr[0];
+      if (hdr[1] == 1) {
+        const size_t words = (size_t)(n + 63) / 64;
+        const uint8_t* q = take(words * 8);
+        if (n > 0) {
+          Buffer wb(ctx->arena, words * 8);
+          upload(ctx, wb.ptr, q, words * 8);
+          c.valid = Buffer(ctx->arena, (size_t)n);
+          launch_bits_to_bytes(wb.as<uint64_t>(), n, c.valid.as<uint8_t>(), ctx->stream);
+        }
+      }
+      if (c.type == ML_TYPE_UTF8) {
+        const uint8_t* oq = take(((size_t)n + 1) * 4);
+        std::vector<uint32_t> offs((size_t)n + 1);
+        std::memcpy(offs.data(), oq, offs.size() * 4);
+        const uint64_t blen = u64();
+        const uint8_t* bytes = take((size_t)blen);
+        bool ok = offs[0] == 0 && offs[(size_t)n] == blen;
+        for (int64_t k = 0; k < n && ok; k++) ok = offs[(size_t)k] <= offs[(size_t)k + 1];
+        if (!ok) fail(ML_ERR_INTERNAL, "corrupt string offsets");
+        auto d = std::make_shared<HostDict>();
+        std::unordered_map<std::string_view, uint32_t> code;
+        std::vector<uint32_t> codes((size_t)n);
+        for (int64_t k = 0; k < n; k++) {
+          const std::string_view v(reinterpret_cast<const char*>(bytes) + offs[(size_t)k],
+                                   offs[(size_t)k + 1] - offs[(size_t)k]);
+          auto it = code.find(v);
+          if (it == code.end()) {
+            it = code.emplace(v, (uint32_t)d->entries.size()).first;
+            d->entries.emplace_back(v);
+          }
+          codes[(size_t)k] = it->second;
+        }
+        d->finish_view();
+        c.dict = d;
+        c.data = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1) * 4);
+        upload(ctx, c.data.ptr, codes.data(), codes.size() * 4);
+      } else {
+        const int w = type_width(c.type);
+        const uint8_t* q = take((size_t)n * w);
+        c.data = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1) * w);
+        if (c.type == ML_TYPE_BOOLEAN) {
+          std::vector<uint8_t> v(q, q + n);
+          for (auto& x : v) x = x != 0;
+          upload(ctx, c.data.ptr, v.data(), v.size());
+        } else {
+          upload(ctx, c.data.ptr, q, (size_t)n * w);
+        }
+      }
+      r->cols.push_back(std::move(c));
+    }
+    r->finish_views();
+    hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+    *consumed = (int64_t)pos;
+    *out = r.release();
   });
 }
 

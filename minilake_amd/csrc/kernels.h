@@ -64,6 +64,16 @@ void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
 // LSB-first validity bitmap (one bit per row) <-> one byte per row (1 = valid)
 void launch_bits_to_bytes(const uint64_t* bits, int64_t n, uint8_t* out, hipStream_t s);
 void launch_bytes_to_bits(const uint8_t* valid, int64_t n, uint64_t* bits, hipStream_t s);
+// MLB1 batch bytes (byte-granular stores: section offsets are unaligned)
+void launch_null_count(const uint8_t* valid, const uint32_t* sel, int64_t n, uint64_t* out, hipStream_t s);
+void launch_mlb1_str_len(const uint32_t* codes, const uint32_t* sel, int64_t n, const uint32_t* dict_off,
+                         uint32_t ndict, uint32_t* len, hipStream_t s);
+void launch_mlb1_pack_fixed(const void* src, int width, bool as_bool, const uint32_t* sel, int64_t n, uint8_t* dst,
+                            hipStream_t s);
+void launch_mlb1_pack_validity(const uint8_t* valid, const uint32_t* sel, int64_t n, uint8_t* dst, hipStream_t s);
+void launch_mlb1_pack_strings(const uint32_t* codes, const uint32_t* sel, int64_t n, const uint32_t* dict_off,
+                              const uint8_t* dict_bytes, uint32_t ndict, const uint32_t* offs, uint8_t* dst,
+                              hipStream_t s);
 void launch_gather(const void* src, int width, const int64_t* idx, int64_t n, void* dst, const uint8_t* src_valid,
                    uint8_t* dst_valid, hipStream_t s);
 void launch_join_insert(const DevJoinTable& T, const uint64_t* keys, const uint8_t* key_null, int64_t n,

@@ -2435,6 +2435,66 @@ __global__ void bytes_to_bits_kernel(const uint8_t* valid, int64_t n, uint64_t* 
   }
 }
 
+// ------------------------------------------------------------ MLB1 bytes
+// Row k of a batch is physical row sel[k] (or k without a selection).
+__global__ void null_count_kernel(const uint8_t* valid, const uint32_t* sel, int64_t n, uint64_t* out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  uint64_t c = 0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+    c += valid[sel ? sel[k] : k] == 0;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd((unsigned long long*)out, (unsigned long long)c);
+}
+
+__global__ void mlb1_str_len_kernel(const uint32_t* codes, const uint32_t* sel, int64_t n, const uint32_t* dict_off,
+                                    uint32_t ndict, uint32_t* len) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const uint32_t c = codes[sel ? sel[k] : k];
+    len[k] = c < ndict ? dict_off[c + 1] - dict_off[c] : 0u;
+  }
+}
+
+// output byte p = byte (p % width) of row p / width; booleans as 0 / 1
+__global__ void mlb1_pack_fixed_kernel(const uint8_t* src, int shift, int as_bool, const uint32_t* sel, int64_t n,
+                                       uint8_t* dst) {
+  const int64_t total = n << shift;
+  const int64_t mask = ((int64_t)1 << shift) - 1;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += stride) {
+    const int64_t k = p >> shift;
+    const int64_t row = sel ? (int64_t)sel[k] : k;
+    const uint8_t x = src[(row << shift) + (p & mask)];
+    dst[p] = as_bool ? (uint8_t)(x != 0) : x;
+  }
+}
+
+// one wave per 64-row word; lanes 0-7 store its little-endian bytes
+__global__ void mlb1_pack_validity_kernel(const uint8_t* valid, const uint32_t* sel, int64_t n, uint8_t* dst) {
+  const int64_t nw = (n + 63) / 64;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+  const int lane = threadIdx.x & 63;
+  for (int64_t w = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; w < nw; w += waves) {
+    const int64_t k = w * 64 + lane;
+    const uint64_t m = __ballot(k < n && valid[sel ? sel[k] : k] != 0);
+    if (lane < 8) dst[w * 8 + lane] = (uint8_t)(m >> (8 * lane));
+  }
+}
+
+__global__ void mlb1_pack_strings_kernel(const uint32_t* codes, const uint32_t* sel, int64_t n,
+                                         const uint32_t* dict_off, const uint8_t* dict_bytes, uint32_t ndict,
+                                         const uint32_t* offs, uint8_t* dst) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const uint32_t c = codes[sel ? sel[k] : k];
+    if (c >= ndict) continue;
+    const uint8_t* src = dict_bytes + dict_off[c];
+    const uint32_t len = dict_off[c + 1] - dict_off[c];
+    uint8_t* d = dst + offs[k];
+    for (uint32_t j = 0; j < len; j++) d[j] = src[j];
+  }
+}
+
 __global__ void topn_rows_kernel(const uint64_t* words, int nw, int64_t n, int64_t* rows_out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) rows_out[i] = (int64_t)words[(uint64_t)i * (nw + 1) + nw];
@@ -2692,6 +2752,38 @@ uint64_t* launch_sort_words(const uint64_t* words, int64_t n, int nw, uint64_t* 
     std::swap(cur, nxt);
   }
   return cur;
+}
+
+void launch_null_count(const uint8_t* valid, const uint32_t* sel, int64_t n, uint64_t* out, hipStream_t s) {
+  if (n <= 0) return;
+  null_count_kernel<<<grid_for(n, 256, 2048), 256, 0, s>>>(valid, sel, n, out);
+}
+
+void launch_mlb1_str_len(const uint32_t* codes, const uint32_t* sel, int64_t n, const uint32_t* dict_off,
+                         uint32_t ndict, uint32_t* len, hipStream_t s) {
+  if (n <= 0) return;
+  mlb1_str_len_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(codes, sel, n, dict_off, ndict, len);
+}
+
+void launch_mlb1_pack_fixed(const void* src, int width, bool as_bool, const uint32_t* sel, int64_t n, uint8_t* dst,
+                            hipStream_t s) {
+  if (n <= 0) return;
+  const int shift = width == 8 ? 3 : (width == 4 ? 2 : 0);
+  mlb1_pack_fixed_kernel<<<grid_for(n << shift, 256, 8192), 256, 0, s>>>((const uint8_t*)src, shift, as_bool ? 1 : 0,
+                                                                         sel, n, dst);
+}
+
+void launch_mlb1_pack_validity(const uint8_t* valid, const uint32_t* sel, int64_t n, uint8_t* dst, hipStream_t s) {
+  if (n <= 0) return;
+  mlb1_pack_validity_kernel<<<grid_for((n + 63) / 64, 4, 8192), 256, 0, s>>>(valid, sel, n, dst);
+}
+
+void launch_mlb1_pack_strings(const uint32_t* codes, const uint32_t* sel, int64_t n, const uint32_t* dict_off,
+                              const uint8_t* dict_bytes, uint32_t ndict, const uint32_t* offs, uint8_t* dst,
+                              hipStream_t s) {
+  if (n <= 0) return;
+  mlb1_pack_strings_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(codes, sel, n, dict_off, dict_bytes, ndict, offs,
+                                                                   dst);
 }
 
 void launch_bits_to_bytes(const uint64_t* bits, int64_t n, uint8_t* out, hipStream_t s) {

@@ -45,7 +45,7 @@ EXPORTED = [
     "ml_join_build_finalize", "ml_join_table_num_rows", "ml_join_table_unique", "ml_join_table_free",
     "ml_join_table_num_columns", "ml_join_table_column", "ml_pipeline_collect_sized", "ml_pipeline_count",
     "ml_join_probe", "ml_pipeline_hashagg", "ml_pipeline_join_build", "ml_pipeline_topn", "ml_pipeline_collect",
-    "ml_pipeline_limit",
+    "ml_pipeline_limit", "ml_mlb1_write", "ml_mlb1_read",
     "ml_topn_create", "ml_topn_sink", "ml_topn_finalize", "ml_topn_free", "ml_ctx_enable_stats",
     "ml_ctx_take_stats", "ml_fragments_create", "ml_fragments_run", "ml_fragments_run_graph",
     "ml_fragments_run_per_fragment", "ml_fragments_autotune", "ml_fragments_free", "ml_mlp_create",
@@ -107,6 +107,8 @@ def lib():
             "ml_pipeline_topn": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, vp], i32),
             "ml_pipeline_collect": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, P(vp)], i32),
             "ml_pipeline_limit": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, i64, i64, P(vp)], i32),
+            "ml_mlb1_write": ([vp, P(abi.MlBatch), vp, i64, P(i64)], i32),
+            "ml_mlb1_read": ([vp, C.c_char_p, i64, P(i64), P(vp)], i32),
             "ml_pipeline_collect_sized": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, i64, P(i32), P(vp)], i32),
             "ml_pipeline_count": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, P(i64)], i32),
             "ml_topn_create": ([vp, P(abi.MlSortSpec), P(vp)], i32),
@@ -455,6 +457,22 @@ class GpuEngine:
         self._check(L.ml_pipeline_limit(self.ctx, C.byref(b), self._ops(ops, keep), len(ops), limit, offset,
                                         C.byref(out)))
         return GpuResult(self, out.value)
+
+    def mlb1_write(self, src, selection: Optional[np.ndarray] = None) -> bytes:
+        """write_batch (ipc.rs:44-87) on the device; the bytes are copied to the host."""
+        L, keep = lib(), abi.Keep()
+        _, b = self._src(src, keep, selection)
+        size = C.c_int64()
+        self._check(L.ml_mlb1_write(self.ctx, C.byref(b), None, 0, C.byref(size)))
+        buf = self.torch.empty(max(1, size.value), dtype=self.torch.uint8, device=f"cuda:{self.device}")
+        self._check(L.ml_mlb1_write(self.ctx, C.byref(b), buf.data_ptr(), buf.numel(), C.byref(size)))
+        return bytes(buf[: size.value].cpu().numpy())
+
+    def mlb1_read(self, data: bytes):
+        """read_batch (ipc.rs:111-170): (device result, or None at a clean end of stream; bytes consumed)."""
+        out, used = C.c_void_p(), C.c_int64()
+        self._check(lib().ml_mlb1_read(self.ctx, data, len(data), C.byref(used), C.byref(out)))
+        return (GpuResult(self, out.value) if out.value else None), used.value
 
     def count_distinct(self, src, ops, group_exprs, key_types, sizing=abi.SIZING_PLAIN, capacity_hint=0,
                        stats: Optional[dict] = None) -> int:

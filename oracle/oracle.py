@@ -90,6 +90,9 @@ def lib():
         L.mlo_table_capacity.argtypes = [vp]
         L.mlo_table_capacity.restype = i64
         L.mlo_table_free.argtypes = [vp]
+        L.mlo_mlb1_write.argtypes = [C.POINTER(abi.MlBatch), C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(i64)]
+        L.mlo_bytes_free.argtypes = [vp]
+        L.mlo_mlb1_read.argtypes = [C.c_char_p, i64, C.POINTER(i64), C.POINTER(vp)]
         _lib = L
     return _lib
 
@@ -209,6 +212,23 @@ class OracleEngine:
         _check(lib().mlo_pipeline_limit(C.byref(b), self._ops(ops, keep), len(ops), limit, offset,
                                         C.byref(self.opts), C.byref(out)))
         return _take_table(out.value)
+
+    def mlb1_write(self, src, selection: Optional[np.ndarray] = None) -> bytes:
+        """write_batch (ipc.rs:44-87): the MLB1 bytes of a table (selection applied)."""
+        keep = abi.Keep()
+        b = host_batch(self._host(src), keep, selection)
+        p, n = C.POINTER(C.c_uint8)(), C.c_int64()
+        _check(lib().mlo_mlb1_write(C.byref(b), C.byref(p), C.byref(n)))
+        try:
+            return C.string_at(p, n.value)
+        finally:
+            lib().mlo_bytes_free(p)
+
+    def mlb1_read(self, data: bytes):
+        """read_batch (ipc.rs:111-170): (table, or None at a clean end of stream; bytes consumed)."""
+        out, used = C.c_void_p(), C.c_int64()
+        _check(lib().mlo_mlb1_read(data, len(data), C.byref(used), C.byref(out)))
+        return (_take_table(out.value) if out.value else None), used.value
 
     def collect(self, src, ops):
         keep = abi.Keep()

@@ -1297,3 +1297,185 @@ int64_t mlo_table_capacity(const mlo_table* t) { return t->capacity; }
 void mlo_table_free(mlo_table* t) { delete t; }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- MLB1 format
+// batch := "MLB1" num_cols:u32 num_rows:u64 column*; column := type:u8
+// has_validity:u8 [validity words] payload, little-endian (ipc.rs:1-13).
+static int mlb1_tag(int t) {  // type_tag (ipc.rs:21-30)
+  switch (t) {
+    case Boolean: return 0;
+    case Int32: return 1;
+    case Int64: return 2;
+    case Float64: return 3;
+    case Date: return 4;
+    case Utf8: return 5;
+  }
+  fail(ML_ERR_INTERNAL, "bad column type");
+}
+static int mlb1_type(int tag) {  // tag_type (ipc.rs:32-42)
+  if (tag < 0 || tag > 5) fail(ML_ERR_INTERNAL, "bad type tag " + std::to_string(tag));
+  static const int types[] = {Boolean, Int32, Int64, Float64, Date, Utf8};
+  return types[tag];
+}
+template <class T>
+static void put_le(std::string& w, T x) {
+  w.append(reinterpret_cast<const char*>(&x), sizeof(T));  // x86: already little-endian
+}
+
+// write_batch (ipc.rs:44-87): the selection is applied first; a column has
+// a validity section when any kept row is NULL (Column::new drops all-valid
+// bitmaps); dictionary columns are decoded to plain strings.
+static void mlb1_write(const Batch& batch, std::string& w) {
+  Batch b = compact(batch);
+  const size_t n = b.num_rows;
+  w += "MLB1";
+  put_le<uint32_t>(w, (uint32_t)b.columns.size());
+  put_le<uint64_t>(w, (uint64_t)n);
+  for (auto& c : b.columns) {
+    w.push_back((char)mlb1_tag(c->type));
+    w.push_back(c->valid ? 1 : 0);
+    if (c->valid) {
+      std::vector<uint64_t> words((n + 63) / 64, 0);
+      for (size_t r = 0; r < n; r++)
+        if ((*c->valid)[r]) words[r / 64] |= 1ull << (r % 64);
+      for (uint64_t x : words) put_le(w, x);
+    }
+    switch (c->type) {
+      case Boolean:
+        for (size_t r = 0; r < n; r++) w.push_back(c->as<uint8_t>()[r] != 0 ? 1 : 0);
+        break;
+      case Int32: case Date:
+        for (size_t r = 0; r < n; r++) put_le(w, c->as<int32_t>()[r]);
+        break;
+      case Int64:
+        for (size_t r = 0; r < n; r++) put_le(w, c->as<int64_t>()[r]);
+        break;
+      case Float64:
+        for (size_t r = 0; r < n; r++) put_le(w, c->as<double>()[r]);
+        break;
+      case Utf8: {
+        std::string bytes;
+        put_le<uint32_t>(w, 0);
+        for (size_t r = 0; r < n; r++) {
+          bytes += c->dict->entries[c->as<uint32_t>()[r]];
+          put_le<uint32_t>(w, (uint32_t)bytes.size());
+        }
+        put_le<uint64_t>(w, (uint64_t)bytes.size());
+        w += bytes;
+        break;
+      }
+    }
+  }
+}
+
+// read_batch (ipc.rs:111-170). A stream shorter than the magic is a clean end
+// (nullopt); a batch cut short is an I/O error; a bad magic, type tag or
+// string offsets are internal errors. Strings come back as a dictionary
+// column with one entry per row.
+static bool mlb1_read(const uint8_t* p, size_t len, size_t* used, Batch* out) {
+  size_t pos = 0;
+  auto take = [&](size_t k) {
+    if (len - pos < k) fail(ML_ERR_IO, "failed to fill whole buffer");
+    const uint8_t* q = p + pos;
+    pos += k;
+    return q;
+  };
+  auto u32 = [&]() { uint32_t x; std::memcpy(&x, take(4), 4); return x; };
+  auto u64 = [&]() { uint64_t x; std::memcpy(&x, take(8), 8); return x; };
+  if (len < 4) return false;
+  if (std::memcmp(take(4), "MLB1", 4) != 0) fail(ML_ERR_INTERNAL, "bad batch magic");
+  const size_t ncols = u32();
+  const size_t n = (size_t)u64();
+  out->columns.clear();
+  out->sel = nullptr;
+  out->num_rows = n;
+  for (size_t ci = 0; ci < ncols; ci++) {
+    const uint8_t* hdr = take(2);
+    const int t = mlb1_type(hdr[0]);
+    std::shared_ptr<std::vector<uint8_t>> valid;
+    if (hdr[1] == 1) {
+      valid = std::make_shared<std::vector<uint8_t>>(n);
+      for (size_t wi = 0; wi < (n + 63) / 64; wi++) {
+        const uint64_t x = u64();
+        for (size_t r = wi * 64; r < std::min(n, wi * 64 + 64); r++) (*valid)[r] = (x >> (r % 64)) & 1;
+      }
+    }
+    ColumnRef c;
+    switch (t) {
+      case Boolean: {
+        const uint8_t* q = take(n);
+        std::vector<uint8_t> v(n);
+        for (size_t r = 0; r < n; r++) v[r] = q[r] != 0;
+        c = make_column(t, std::move(v), valid);
+        break;
+      }
+      case Int32: case Date: {
+        std::vector<int32_t> v(n);
+        if (n) std::memcpy(v.data(), take(4 * n), 4 * n);
+        c = make_column(t, std::move(v), valid);
+        break;
+      }
+      case Int64: {
+        std::vector<int64_t> v(n);
+        if (n) std::memcpy(v.data(), take(8 * n), 8 * n);
+        c = make_column(t, std::move(v), valid);
+        break;
+      }
+      case Float64: {
+        std::vector<double> v(n);
+        if (n) std::memcpy(v.data(), take(8 * n), 8 * n);
+        c = make_column(t, std::move(v), valid);
+        break;
+      }
+      default: {  // Utf8: StringVec::from_parts (column.rs:101-110) checks the offsets
+        std::vector<uint32_t> offs(n + 1);
+        for (auto& o : offs) o = u32();
+        const uint64_t blen = u64();
+        const uint8_t* bytes = take((size_t)blen);
+        bool ok = offs[0] == 0 && offs[n] == blen;
+        for (size_t r = 0; r < n && ok; r++) ok = offs[r] <= offs[r + 1];
+        if (!ok) fail(ML_ERR_INTERNAL, "corrupt string offsets");
+        auto d = std::make_shared<Dict>();
+        std::vector<uint32_t> codes(n);
+        for (size_t r = 0; r < n; r++) {
+          d->entries.emplace_back(reinterpret_cast<const char*>(bytes) + offs[r], offs[r + 1] - offs[r]);
+          codes[r] = (uint32_t)r;
+        }
+        c = make_column(Utf8, std::move(codes), valid, d);
+        break;
+      }
+    }
+    out->columns.push_back(c);
+  }
+  *used = pos;
+  return true;
+}
+
+extern "C" {
+
+int mlo_mlb1_write(const ml_batch* src, uint8_t** out, int64_t* len) {
+  return guarded([&] {
+    std::string w;
+    mlb1_write(import_batch(src), w);
+    *out = static_cast<uint8_t*>(std::malloc(std::max<size_t>(w.size(), 1)));
+    if (!*out) fail(ML_ERR_RESOURCES_EXHAUSTED, "out of memory");
+    std::memcpy(*out, w.data(), w.size());
+    *len = (int64_t)w.size();
+  });
+}
+
+void mlo_bytes_free(uint8_t* p) { std::free(p); }
+
+int mlo_mlb1_read(const uint8_t* src, int64_t len, int64_t* consumed, mlo_table** out) {
+  *out = nullptr;
+  *consumed = 0;
+  return guarded([&] {
+    Batch b;
+    size_t used = 0;
+    if (!mlb1_read(src, (size_t)std::max<int64_t>(len, 0), &used, &b)) return;
+    *out = export_batch(std::move(b));
+    *consumed = (int64_t)used;
+  });
+}
+
+}  // extern "C"

@@ -52,6 +52,11 @@ const ml_column* mlo_table_column(const mlo_table* t, int32_t i);
 int32_t mlo_table_resizes(const mlo_table* t);
 int64_t mlo_table_capacity(const mlo_table* t);
 void mlo_table_free(mlo_table* t);
+// MLB1 batch bytes (ipc.rs:44-170): write returns malloc'd bytes (free with
+// mlo_bytes_free); read returns *out = NULL at a clean end of stream.
+int mlo_mlb1_write(const ml_batch* src, uint8_t** out, int64_t* len);
+void mlo_bytes_free(uint8_t* p);
+int mlo_mlb1_read(const uint8_t* src, int64_t len, int64_t* consumed, mlo_table** out);
 
 #ifdef __cplusplus
 }

@@ -498,3 +498,43 @@ def test_partial_states_closed_forms():
     fin = eng.hashagg(empty, [], [], [], aggs, mode=abi.AGG_MODE_FINAL).to_host()
     assert rows_of(fin) == [(0, None, None, None)]
     assert fin.columns[3].type == INT32
+
+
+def test_mlb1_known_bytes_and_round_trip():
+    """MLB1 (ipc.rs:1-13): the bytes of a small batch written out by hand from
+    the format (magic, u32 columns, u64 rows; per column type tag, validity
+    flag, LSB-first words, payload; strings as u32 offsets, u64 length and the
+    bytes), with the selection applied first and a validity section only where
+    a kept row is NULL. Reading them back gives the rows; a stream shorter than
+    the magic is a clean end; a cut batch is an I/O error; two batches in a row
+    are read one after the other."""
+    B = abi.BOOLEAN
+    t = HostTable([HostColumn(INT32, np.array([5, -1, 7, 9]), validity=np.array([True, False, True, True])),
+                   HostColumn(UTF8, np.array([1, 0, 1, 2], np.uint32), ["a", "", "hello"]),
+                   HostColumn(FLOAT64, np.array([0.5, 1.5, -2.0, 3.0])),
+                   HostColumn(B, np.array([1, 0, 0, 1], np.uint8)),
+                   HostColumn(DATE, np.array([10, 20, 30, 40])), HostColumn(INT64, np.array([1, 2, 3, -4]))])
+    eng = O.OracleEngine(threads=1)
+    sel = np.array([0, 1, 3], np.uint32)
+    want = b"MLB1" + struct.pack("<IQ", 6, 3)
+    want += bytes([1, 1]) + struct.pack("<Q", 0b101) + struct.pack("<3i", 5, -1, 9)
+    want += bytes([5, 0]) + struct.pack("<4I", 0, 0, 1, 6) + struct.pack("<Q", 6) + b"ahello"
+    want += bytes([3, 0]) + struct.pack("<3d", 0.5, 1.5, 3.0)
+    want += bytes([0, 0]) + bytes([1, 0, 1])
+    want += bytes([4, 0]) + struct.pack("<3i", 10, 20, 40)
+    want += bytes([2, 0]) + struct.pack("<3q", 1, 2, -4)
+    got = eng.mlb1_write(t, sel)
+    assert got == want
+    back, used = eng.mlb1_read(got)
+    assert used == len(got)
+    assert rows_of(back.to_host()) == rows_of(t.take(sel))
+    # the NULL row not kept: no validity section
+    assert eng.mlb1_write(t, np.array([0, 2], np.uint32))[16:18] == bytes([1, 0])
+    assert eng.mlb1_read(b"ML") == (None, 0)
+    with pytest.raises(O.OracleError) as e:
+        eng.mlb1_read(got[:-1])
+    assert e.value.code == abi.ML_ERR_IO
+    two = got + eng.mlb1_write(t)
+    first, used = eng.mlb1_read(two)
+    second, used2 = eng.mlb1_read(two[used:])
+    assert first.to_host().num_rows == 3 and rows_of(second.to_host()) == rows_of(t) and used + used2 == len(two)
