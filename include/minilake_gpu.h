@@ -190,6 +190,68 @@ ml_status ml_result_copy_validity(ml_ctx* ctx, const ml_result* r, int32_t i, ui
 void ml_result_free(ml_result* r);
 
 /* ------------------------------------------------------------------ */
+/* Device Parquet scan (crates/minilake-storage/src/table.rs:59-240,
+ * parquet_reader.rs:210-358, stats.rs:53-70). Footer and page headers are
+ * parsed on the host; page bodies are decoded on the device. Flat schemas,
+ * UNCOMPRESSED chunks, data pages v1, PLAIN and dictionary encodings, RLE
+ * definition levels; other files return ML_ERR_UNSUPPORTED.              */
+/* ------------------------------------------------------------------ */
+typedef struct ml_parquet ml_parquet;
+
+/* RowGroupMeta.stats (convert_stats, parquet_reader.rs:290-358): exact
+ * bounds only; none for booleans and byte-array decimals; string bounds only
+ * when valid UTF-8 (pointers valid while the handle lives). */
+typedef struct ml_column_stats {
+  int32_t type;           /* ml_type of the column */
+  int32_t has_min, has_max, has_null_count;
+  int64_t null_count;
+  int64_t min_i64, max_i64;  /* INT32 / DATE / INT64 */
+  double min_f64, max_f64;   /* FLOAT64 */
+  const char* min_str;       /* UTF8 */
+  int64_t min_len;
+  const char* max_str;
+  int64_t max_len;
+} ml_column_stats;
+
+enum { /* PruneOp (stats.rs:24-36) */
+  ML_PRUNE_EQ = 0, ML_PRUNE_LT = 1, ML_PRUNE_LTEQ = 2, ML_PRUNE_GT = 3, ML_PRUNE_GTEQ = 4
+};
+/* PrunePredicate (stats.rs:38-47): column op literal (i64 for INT32 / INT64 /
+ * DATE / BOOLEAN, f64 for FLOAT64, str for UTF8). */
+typedef struct ml_prune_predicate {
+  int32_t column;
+  int32_t op;
+  int32_t type;
+  int32_t _pad;
+  int64_t i64;
+  double f64;
+  const char* str;
+  int64_t str_len;
+} ml_prune_predicate;
+
+/* Table::open (table.rs:59-134) over one file held in host memory; the bytes
+ * must stay valid while the handle lives. */
+ml_status ml_parquet_open(ml_ctx* ctx, const void* data, int64_t len, ml_parquet** out);
+int32_t ml_parquet_num_columns(const ml_parquet* p);
+int32_t ml_parquet_column_type(const ml_parquet* p, int32_t column);
+const char* ml_parquet_column_name(const ml_parquet* p, int32_t column);
+int32_t ml_parquet_column_nullable(const ml_parquet* p, int32_t column);
+int32_t ml_parquet_num_row_groups(const ml_parquet* p);
+int64_t ml_parquet_row_group_rows(const ml_parquet* p, int32_t row_group);
+ml_status ml_parquet_column_stats(const ml_parquet* p, int32_t row_group, int32_t column, ml_column_stats* out);
+/* Table::prune (table.rs:165-172): the row groups that may hold matching rows
+ * (out_row_groups holds at least ml_parquet_num_row_groups entries). */
+ml_status ml_parquet_prune(const ml_parquet* p, const ml_prune_predicate* preds, int32_t num_preds,
+                           int32_t* out_row_groups, int32_t* out_n);
+/* Table::read_row_group (table.rs:204-240) + read_column_chunk
+ * (parquet_reader.rs:210-287): the projected columns of one row group as a
+ * device result (strings dictionary-encoded; DATE, DECIMAL -> FLOAT64 as
+ * value / 10^scale, FLOAT -> FLOAT64). */
+ml_status ml_parquet_read_row_group(ml_parquet* p, int32_t row_group, const int32_t* projection,
+                                    int32_t num_columns, ml_result** out);
+void ml_parquet_free(ml_parquet* p);
+
+/* ------------------------------------------------------------------ */
 /* MLB1 batch bytes (crates/minilake-core/src/ipc.rs:1-179): the format of
  * aggregate spill files and of partial aggregates shipped between workers. */
 /* ------------------------------------------------------------------ */

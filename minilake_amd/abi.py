@@ -54,6 +54,21 @@ class MlDict(C.Structure):
                 ("offsets", C.POINTER(C.c_uint32)), ("bytes", C.POINTER(C.c_uint8))]
 
 
+class MlColumnStats(C.Structure):
+    _fields_ = [("type", C.c_int32), ("has_min", C.c_int32), ("has_max", C.c_int32), ("has_null_count", C.c_int32),
+                ("null_count", C.c_int64), ("min_i64", C.c_int64), ("max_i64", C.c_int64),
+                ("min_f64", C.c_double), ("max_f64", C.c_double), ("min_str", C.c_void_p), ("min_len", C.c_int64),
+                ("max_str", C.c_void_p), ("max_len", C.c_int64)]
+
+
+class MlPrunePredicate(C.Structure):
+    _fields_ = [("column", C.c_int32), ("op", C.c_int32), ("type", C.c_int32), ("_pad", C.c_int32),
+                ("i64", C.c_int64), ("f64", C.c_double), ("str", C.c_char_p), ("str_len", C.c_int64)]
+
+
+PRUNE_OPS = {"=": 0, "<": 1, "<=": 2, ">": 3, ">=": 4}
+
+
 class MlColumn(C.Structure):
     _fields_ = [("type", C.c_int32), ("_pad", C.c_int32), ("data", C.c_void_p), ("len", C.c_int64),
                 ("validity", C.POINTER(C.c_uint64)), ("dict", C.POINTER(MlDict))]

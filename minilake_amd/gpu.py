@@ -22,7 +22,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from typing import Optional, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -45,7 +45,10 @@ EXPORTED = [
     "ml_join_build_finalize", "ml_join_table_num_rows", "ml_join_table_unique", "ml_join_table_free",
     "ml_join_table_num_columns", "ml_join_table_column", "ml_pipeline_collect_sized", "ml_pipeline_count",
     "ml_join_probe", "ml_pipeline_hashagg", "ml_pipeline_join_build", "ml_pipeline_topn", "ml_pipeline_collect",
-    "ml_pipeline_limit", "ml_mlb1_write", "ml_mlb1_read",
+    "ml_pipeline_limit", "ml_mlb1_write", "ml_mlb1_read", "ml_parquet_open", "ml_parquet_num_columns",
+    "ml_parquet_column_type", "ml_parquet_column_name", "ml_parquet_column_nullable", "ml_parquet_num_row_groups",
+    "ml_parquet_row_group_rows", "ml_parquet_column_stats", "ml_parquet_prune", "ml_parquet_read_row_group",
+    "ml_parquet_free",
     "ml_topn_create", "ml_topn_sink", "ml_topn_finalize", "ml_topn_free", "ml_ctx_enable_stats",
     "ml_ctx_take_stats", "ml_fragments_create", "ml_fragments_run", "ml_fragments_run_graph",
     "ml_fragments_run_per_fragment", "ml_fragments_autotune", "ml_fragments_free", "ml_mlp_create",
@@ -109,6 +112,17 @@ def lib():
             "ml_pipeline_limit": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, i64, i64, P(vp)], i32),
             "ml_mlb1_write": ([vp, P(abi.MlBatch), vp, i64, P(i64)], i32),
             "ml_mlb1_read": ([vp, C.c_char_p, i64, P(i64), P(vp)], i32),
+            "ml_parquet_open": ([vp, C.c_char_p, i64, P(vp)], i32),
+            "ml_parquet_num_columns": ([vp], i32),
+            "ml_parquet_column_type": ([vp, i32], i32),
+            "ml_parquet_column_name": ([vp, i32], C.c_char_p),
+            "ml_parquet_column_nullable": ([vp, i32], i32),
+            "ml_parquet_num_row_groups": ([vp], i32),
+            "ml_parquet_row_group_rows": ([vp, i32], i64),
+            "ml_parquet_column_stats": ([vp, i32, i32, P(abi.MlColumnStats)], i32),
+            "ml_parquet_prune": ([vp, P(abi.MlPrunePredicate), i32, P(i32), P(i32)], i32),
+            "ml_parquet_read_row_group": ([vp, i32, P(i32), i32, P(vp)], i32),
+            "ml_parquet_free": ([vp], None),
             "ml_pipeline_collect_sized": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, i64, P(i32), P(vp)], i32),
             "ml_pipeline_count": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, P(i64)], i32),
             "ml_topn_create": ([vp, P(abi.MlSortSpec), P(vp)], i32),
@@ -295,6 +309,92 @@ class GpuResult:
         if self.handle and _lib is not None:
             _lib.ml_result_free(self.handle)
             self.handle = None
+
+
+class ParquetTable:
+    """Table::open (crates/minilake-storage/src/table.rs:59-134) over one
+    Parquet file held in host memory. Metadata, statistics and pruning are
+    host work (usable without a GPU, ctx = None); read_row_group decodes on
+    the device."""
+
+    def __init__(self, data: bytes, engine: Optional["GpuEngine"] = None):
+        self.data, self.engine = bytes(data), engine
+        ctx = engine.ctx if engine is not None else None
+        h = C.c_void_p()
+        rc = lib().ml_parquet_open(ctx, self.data, len(self.data), C.byref(h))
+        if rc != 0:
+            msg = lib().ml_last_error(ctx).decode() if ctx else ""
+            raise GpuError(rc, msg)
+        self.handle = h.value
+        L = lib()
+        self.num_columns = L.ml_parquet_num_columns(self.handle)
+        self.types = [L.ml_parquet_column_type(self.handle, i) for i in range(self.num_columns)]
+        self.names = [L.ml_parquet_column_name(self.handle, i).decode() for i in range(self.num_columns)]
+        self.nullable = [bool(L.ml_parquet_column_nullable(self.handle, i)) for i in range(self.num_columns)]
+        self.num_row_groups = L.ml_parquet_num_row_groups(self.handle)
+        self.row_group_rows = [L.ml_parquet_row_group_rows(self.handle, i) for i in range(self.num_row_groups)]
+
+    def _rc(self, rc):
+        if rc != 0:
+            ctx = self.engine.ctx if self.engine is not None else None
+            raise GpuError(rc, lib().ml_last_error(ctx).decode() if ctx else "")
+
+    def stats(self, rg: int, c: int) -> dict:
+        """ColumnStats of a row group's column: min / max (None when unknown), null_count."""
+        st = abi.MlColumnStats()
+        self._rc(lib().ml_parquet_column_stats(self.handle, rg, c, C.byref(st)))
+
+        def val(has, i, f, p, n):
+            if not has:
+                return None
+            if st.type == abi.FLOAT64:
+                return f
+            if st.type == abi.UTF8:
+                return C.string_at(p, n).decode("utf-8")
+            return i
+        return {"min": val(st.has_min, st.min_i64, st.min_f64, st.min_str, st.min_len),
+                "max": val(st.has_max, st.max_i64, st.max_f64, st.max_str, st.max_len),
+                "null_count": st.null_count if st.has_null_count else None}
+
+    def prune(self, preds) -> List[int]:
+        """Table::prune (table.rs:165-172); preds: (column, op, value), the value of the column's type."""
+        keep = abi.Keep()
+        arr = keep((abi.MlPrunePredicate * max(1, len(preds)))())
+        for k, (c, op, v) in enumerate(preds):
+            arr[k].column, arr[k].op = c, abi.PRUNE_OPS[op]
+            t = self.types[c]
+            if isinstance(v, str):
+                raw = keep(v.encode())
+                arr[k].type, arr[k].str, arr[k].str_len = abi.UTF8, raw, len(raw)
+            elif isinstance(v, float):
+                arr[k].type, arr[k].f64 = abi.FLOAT64, v
+            else:
+                arr[k].type, arr[k].i64 = (abi.DATE if t == abi.DATE else abi.INT64), int(v)
+        out = (C.c_int32 * max(1, self.num_row_groups))()
+        n = C.c_int32()
+        self._rc(lib().ml_parquet_prune(self.handle, arr, len(preds), out, C.byref(n)))
+        return list(out[: n.value])
+
+    def read_row_group(self, rg: int, projection: Optional[Sequence[int]] = None) -> "GpuResult":
+        """Table::read_row_group (table.rs:204-240), decoded on the device."""
+        if self.engine is None:
+            raise GpuError(abi.ML_ERR_INTERNAL, "reading row groups needs a GpuEngine")
+        proj = list(range(self.num_columns)) if projection is None else list(projection)
+        arr = (C.c_int32 * max(1, len(proj)))(*proj)
+        out = C.c_void_p()
+        self._rc(lib().ml_parquet_read_row_group(self.handle, rg, arr, len(proj), C.byref(out)))
+        return GpuResult(self.engine, out.value)
+
+    def close(self):
+        if self.handle:
+            lib().ml_parquet_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class GpuJoin:

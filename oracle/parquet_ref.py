@@ -1,0 +1,346 @@
+"""CPU ORACLE for the device Parquet scan (test infrastructure only: loaded by
+tests/, never by the product path).
+
+MiniLake reads Parquet through the `parquet` crate, version 60.0.0 (Cargo.lock),
+which is not part of the reference tree. This module restates the published
+Apache Parquet format for the subset the scan supports (flat schema,
+UNCOMPRESSED chunks, data pages v1, PLAIN and dictionary encodings, RLE /
+bit-packed hybrid definition levels and indices) and MiniLake's own code on
+top of it:
+
+- Conversion::for_column and the value conversions of read_column_chunk
+  (crates/minilake-storage/src/parquet_reader.rs:55-80, 210-287): DATE stays
+  days, DECIMAL -> f64 as unscaled / 10f64.powi(scale), FLOAT -> f64, strings
+  as Python str (encode_strings' dictionary choice is a representation detail);
+- convert_stats (parquet_reader.rs:290-358);
+- PrunePredicate::may_match with ScalarValue::compare (stats.rs:53-70,
+  scalar.rs:115-131) and Table::prune (table.rs:165-172).
+
+Plain Python loops: meant for files of up to a few hundred thousand values.
+"""
+
+from __future__ import annotations
+
+import struct
+from typing import List, Optional
+
+T_BOOLEAN, T_INT32, T_INT64, T_INT96, T_FLOAT, T_DOUBLE, T_BYTE_ARRAY, T_FLBA = range(8)
+CT_DECIMAL, CT_DATE = 5, 6
+
+
+class ParquetError(Exception):
+    pass
+
+
+class Unsupported(Exception):
+    pass
+
+
+class _Thrift:
+    """Thrift compact protocol reader."""
+
+    def __init__(self, b: bytes, pos: int = 0):
+        self.b, self.pos = b, pos
+
+    def byte(self) -> int:
+        if self.pos >= len(self.b):
+            raise ParquetError("metadata ends early")
+        x = self.b[self.pos]
+        self.pos += 1
+        return x
+
+    def varint(self) -> int:
+        v, s = 0, 0
+        while True:
+            x = self.byte()
+            v |= (x & 0x7F) << s
+            if not x & 0x80:
+                return v
+            s += 7
+
+    def zz(self) -> int:
+        v = self.varint()
+        return (v >> 1) ^ -(v & 1)
+
+    def binary(self) -> bytes:
+        n = self.varint()
+        out = self.b[self.pos:self.pos + n]
+        if len(out) != n:
+            raise ParquetError("binary past the end")
+        self.pos += n
+        return out
+
+    def read(self, t: int):
+        """Value of a field or element of type t (structs as {id: value})."""
+        if t in (1, 2):
+            return t == 1
+        if t == 3:
+            return struct.unpack("<b", bytes([self.byte()]))[0]
+        if t in (4, 5, 6):
+            return self.zz()
+        if t == 7:
+            v = struct.unpack_from("<d", self.b, self.pos)[0]
+            self.pos += 8
+            return v
+        if t == 8:
+            return self.binary()
+        if t in (9, 10):
+            h = self.byte()
+            n, et = h >> 4, h & 0x0F
+            if n == 15:
+                n = self.varint()
+            return [(self.byte() == 1) if et in (1, 2) else self.read(et) for _ in range(n)]
+        if t == 11:
+            n = self.varint()
+            if n == 0:
+                return {}
+            kv = self.byte()
+            return {self.read(kv >> 4): self.read(kv & 0x0F) for _ in range(n)}
+        if t == 12:
+            out, last = {}, 0
+            while True:
+                h = self.byte()
+                if h == 0:
+                    return out
+                d = h >> 4
+                fid = last + d if d else self.zz()
+                last = fid
+                out[fid] = self.read(h & 0x0F)
+        raise ParquetError(f"thrift type {t}")
+
+
+def _pow10i(s: int) -> float:
+    """10f64.powi(s) by square-and-multiply (compiler-rt __powidf2)."""
+    recip, b, a, r = s < 0, abs(s), 10.0, 1.0
+    while True:
+        if b & 1:
+            r *= a
+        b //= 2
+        if b == 0:
+            break
+        a *= a
+    return 1.0 / r if recip else r
+
+
+class Column:
+    def __init__(self, el: dict):
+        self.name = el.get(4, b"").decode()
+        self.phys = el.get(1)
+        self.type_length = el.get(2, 0)
+        self.optional = el.get(3, 0) == 1
+        self.converted = el.get(6)
+        self.scale = el.get(7, 0)
+        if el.get(3) == 2 or el.get(5, 0) > 0:
+            raise Unsupported(f"nested parquet column '{self.name}'")
+        c, p = self.converted, self.phys
+        # Conversion::for_column (parquet_reader.rs:55-80)
+        if p == T_BOOLEAN:
+            self.conv = "bool"
+        elif p == T_INT32:
+            self.conv = "date" if c == CT_DATE else ("dec_i32" if c == CT_DECIMAL else "int32")
+        elif p == T_INT64:
+            self.conv = "dec_i64" if c == CT_DECIMAL else "int64"
+        elif p == T_FLOAT:
+            self.conv = "float"
+        elif p == T_DOUBLE:
+            self.conv = "double"
+        elif p in (T_BYTE_ARRAY, T_FLBA) and c == CT_DECIMAL:
+            self.conv = "dec_bytes"
+        elif p == T_BYTE_ARRAY:
+            self.conv = "utf8"
+        else:
+            raise Unsupported(f"parquet column '{self.name}' type")
+
+    @property
+    def ml_type(self) -> str:
+        return {"bool": "bool", "int32": "int32", "date": "date", "int64": "int64", "utf8": "utf8"}.get(self.conv,
+                                                                                                     "float64")
+
+
+def _hybrid(b: bytes, pos: int, end: int, bw: int, want: int) -> List[int]:
+    """RLE / bit-packed hybrid values (at most `want`)."""
+    out: List[int] = []
+    vbytes = (bw + 7) // 8
+    t = _Thrift(b[:end], pos)
+    while len(out) < want and t.pos < end:
+        h = t.varint()
+        if h & 1:
+            groups = h >> 1
+            nbytes = groups * bw
+            chunk = int.from_bytes(b[t.pos:t.pos + nbytes], "little")
+            t.pos += nbytes
+            mask = (1 << bw) - 1
+            for j in range(groups * 8):
+                out.append((chunk >> (j * bw)) & mask if bw else 0)
+        else:
+            v = int.from_bytes(b[t.pos:t.pos + vbytes], "little")
+            t.pos += vbytes
+            out.extend([v] * (h >> 1))
+    return out[:want]
+
+
+class ParquetFile:
+    """Table::open (table.rs:59-134) over one file's bytes."""
+
+    def __init__(self, data: bytes):
+        if len(data) < 12 or data[:4] != b"PAR1" or data[-4:] != b"PAR1":
+            raise ParquetError("not a parquet file")
+        flen = struct.unpack_from("<I", data, len(data) - 8)[0]
+        meta = _Thrift(data, len(data) - 8 - flen).read(12)
+        self.data = data
+        schema = meta[2]
+        self.columns = [Column(el) for el in schema[1:]]
+        if schema[0].get(5, 0) != len(self.columns):
+            raise Unsupported("nested parquet schema")
+        self.row_groups = []
+        for rg in meta.get(4, []):
+            chunks = []
+            for cc in rg[1]:
+                if 1 in cc:
+                    raise Unsupported("column chunks in other files")
+                chunks.append(cc[3])
+            self.row_groups.append((rg[3], chunks))
+
+    # convert_stats (parquet_reader.rs:290-358)
+    def stats(self, rg: int, c: int) -> dict:
+        col = self.columns[c]
+        md = self.row_groups[rg][1][c]
+        st = md.get(12)
+        out = {"min": None, "max": None, "null_count": None}
+        if st is None:
+            return out
+        out["null_count"] = st.get(3)
+
+        def conv(raw: Optional[bytes], exact: bool):
+            if raw is None or not exact:
+                return None
+            if col.conv in ("int32", "date", "dec_i32") and md[1] == T_INT32 and len(raw) == 4:
+                x = struct.unpack("<i", raw)[0]
+                return x / _pow10i(col.scale) if col.conv == "dec_i32" else x
+            if col.conv in ("int64", "dec_i64") and md[1] == T_INT64 and len(raw) == 8:
+                x = struct.unpack("<q", raw)[0]
+                return x / _pow10i(col.scale) if col.conv == "dec_i64" else x
+            if col.conv == "double" and len(raw) == 8:
+                return struct.unpack("<d", raw)[0]
+            if col.conv == "float" and len(raw) == 4:
+                return float(struct.unpack("<f", raw)[0])
+            if col.conv == "utf8":
+                try:
+                    return raw.decode("utf-8")
+                except UnicodeDecodeError:
+                    return None
+            return None
+
+        out["min"] = conv(st.get(6), st.get(8, True))
+        out["max"] = conv(st.get(5), st.get(7, True))
+        return out
+
+    # read_column_chunk (parquet_reader.rs:210-287): Python values, None = NULL
+    def read_column(self, rg: int, c: int) -> list:
+        col = self.columns[c]
+        nrows, chunks = self.row_groups[rg]
+        md = chunks[c]
+        if md.get(4, 0) != 0:
+            raise Unsupported("compressed column chunk")
+        start = md.get(11) if md.get(11) is not None and md.get(11) < md[9] else md[9]
+        b = self.data[start:start + md[7]]
+        width = col.type_length if col.conv == "dec_bytes" else (8 if col.phys in (T_INT64, T_DOUBLE) else 4)
+        dictionary = None
+        out: list = []
+        pos = 0
+        while pos < len(b) and len(out) < nrows:
+            t = _Thrift(b, pos)
+            h = t.read(12)
+            body = t.pos
+            end = body + h[3]
+            if h[1] == 2:
+                dictionary = self._plain(col, b, body, end, h[7][1], width)
+            elif h[1] == 0:
+                dph = h[5]
+                rows = dph[1]
+                v = body
+                valid = [True] * rows
+                if col.optional:
+                    dl = struct.unpack_from("<I", b, v)[0]
+                    levels = _hybrid(b, v + 4, v + 4 + dl, 1, rows)
+                    valid = [lv == 1 for lv in levels]
+                    v += 4 + dl
+                nvalid = sum(valid)
+                if dph[2] == 0:
+                    vals = self._plain(col, b, v, end, nvalid, width)
+                elif dph[2] in (2, 8):
+                    bw = b[v]
+                    vals = [dictionary[k] for k in _hybrid(b, v + 1, end, bw, nvalid)]
+                else:
+                    raise Unsupported(f"encoding {dph[2]}")
+                it = iter(vals)
+                out.extend(next(it) if ok else None for ok in valid)
+            elif h[1] == 3:
+                raise Unsupported("data page v2")
+            pos = end
+        if len(out) != nrows:
+            raise ParquetError(f"expected {nrows} rows, decoded {len(out)}")
+        return out
+
+    def _plain(self, col: Column, b: bytes, pos: int, end: int, n: int, width: int) -> list:
+        conv = col.conv
+        if conv == "bool":
+            return [bool((b[pos + j // 8] >> (j % 8)) & 1) for j in range(n)]
+        if conv == "utf8":
+            vals = []
+            for _ in range(n):
+                ln = struct.unpack_from("<I", b, pos)[0]
+                vals.append(b[pos + 4:pos + 4 + ln].decode("utf-8", errors="surrogateescape"))
+                pos += 4 + ln
+            return vals
+        raw = [b[pos + j * width:pos + (j + 1) * width] for j in range(n)]
+        if conv in ("int32", "date"):
+            return [struct.unpack("<i", r)[0] for r in raw]
+        if conv == "int64":
+            return [struct.unpack("<q", r)[0] for r in raw]
+        if conv == "double":
+            return [struct.unpack("<d", r)[0] for r in raw]
+        if conv == "float":
+            return [float(struct.unpack("<f", r)[0]) for r in raw]
+        d = _pow10i(col.scale)
+        if conv == "dec_i32":
+            return [struct.unpack("<i", r)[0] / d for r in raw]
+        if conv == "dec_i64":
+            return [struct.unpack("<q", r)[0] / d for r in raw]
+        return [int.from_bytes(r, "big", signed=True) / d for r in raw]  # be_i128 (parquet_reader.rs:156-167)
+
+    # PrunePredicate::may_match (stats.rs:53-70) / Table::prune (table.rs:165-172)
+    def prune(self, preds) -> List[int]:
+        """preds: (column, op, value) with op in '=', '<', '<=', '>', '>='."""
+        return [rg for rg in range(len(self.row_groups))
+                if all(may_match(self.stats(rg, c), op, v) for c, op, v in preds)]
+
+
+def _compare(bound, v) -> Optional[int]:
+    """ScalarValue::compare (scalar.rs:115-131) for same-family values."""
+    if bound is None or v is None:
+        return None
+    if isinstance(bound, str) != isinstance(v, str):
+        return None
+    if isinstance(bound, str):
+        a, b = bound.encode(), v.encode()
+    else:
+        a, b = bound, v
+        if a != a or b != b:
+            return None
+    return (a > b) - (a < b)
+
+
+def may_match(stats: dict, op: str, v) -> bool:
+    lo, hi = _compare(stats["min"], v), _compare(stats["max"], v)
+    if op == "=":
+        return lo != 1 and hi != -1
+    if op == "<":
+        return lo not in (1, 0)
+    if op == "<=":
+        return lo != 1
+    if op == ">":
+        return hi not in (-1, 0)
+    if op == ">=":
+        return hi != -1
+    raise ValueError(op)
