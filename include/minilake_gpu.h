@@ -177,6 +177,16 @@ ml_status ml_ctx_enable_stats(ml_ctx* ctx, int32_t on);
  * returns the number written. */
 int32_t ml_ctx_take_stats(ml_ctx* ctx, ml_launch_stat* out, int32_t cap);
 
+/* MemoryPool (crates/minilake-exec/src/memory.rs:22-157): a budget in bytes
+ * for the context's pipeline-breaker state (hash-aggregate group tables);
+ * 0 = unlimited. Over budget, a hash aggregate returns
+ * ML_ERR_RESOURCES_EXHAUSTED, or with spill != 0 writes its groups' partial
+ * states to host memory in 16 hash partitions (aggregate/hash.rs:189-282) and
+ * merges one partition at a time at finalize. */
+ml_status ml_ctx_set_memory_limit(ml_ctx* ctx, int64_t bytes, int32_t spill);
+/* Bytes reserved now, the most reserved at once, and the limit. */
+void ml_ctx_memory_usage(const ml_ctx* ctx, int64_t* used, int64_t* peak, int64_t* limit);
+
 /* Results: library-owned device columns (host copies on request). */
 int64_t ml_result_num_rows(const ml_result* r);
 int32_t ml_result_num_columns(const ml_result* r);

@@ -151,6 +151,20 @@ struct ml_ctx {
   uint64_t synced_seq = 0;  // deferred checks the stream has completed (seen at a synchronization)
   std::map<const ml_dict*, std::shared_ptr<mlg::HostDict>> dicts;
   std::shared_ptr<mlg::HostDict> dict(const ml_dict* d);
+  // MemoryPool (memory.rs:22-76): a budget for pipeline-breaker state
+  // (group tables, join tables); 0 = unlimited. With spill on, a hash
+  // aggregate over budget writes its partial states to host memory.
+  int64_t mem_limit = 0;
+  int64_t mem_used = 0;
+  int64_t mem_peak = 0;
+  bool mem_spill = false;
+  bool mem_try_grow(int64_t bytes) {
+    if (bytes > 0 && mem_limit > 0 && mem_used + bytes > mem_limit) return false;
+    mem_used += bytes;
+    if (mem_used > mem_peak) mem_peak = mem_used;
+    return true;
+  }
+  void mem_shrink(int64_t bytes) { mem_used -= bytes < mem_used ? bytes : mem_used; }
   // launch statistics
   bool stats_on = false;
   int64_t stats_min_rows = 0;  // record launches over at least this many source rows

@@ -712,6 +712,24 @@ struct ml_hashagg {
   bool len_known = true;
   int resizes = 0;
   bool failed = false;
+  // memory budget and spill (aggregate/hash.rs:189-282)
+  int64_t reserved = 0;           // bytes of the group table held against the context's budget
+  bool emit_states = false;       // finalize emits partial states (a spilled Partial aggregate's merge)
+  bool no_spill = false;          // a spill merge: over budget is an error
+  bool spilled = false;
+  std::vector<mlg::OutDef> state_outs;  // partial-state emission (Accumulator::state)
+  struct SpillColumn {
+    int type;
+    std::vector<uint8_t> data, valid;  // valid empty: all rows valid
+  };
+  struct SpillBatch {
+    int64_t rows = 0;
+    std::vector<SpillColumn> cols;
+  };
+  std::vector<std::vector<SpillBatch>> spill;  // per partition, the spilled state batches
+  ~ml_hashagg() {
+    if (ctx) ctx->mem_shrink(reserved);
+  }
 };
 
 namespace mlg {
@@ -789,15 +807,19 @@ static void plan_final(ml_hashagg* A, PipelineBuild& B) {
     switch (u.func) {
       case ML_AGG_COUNT_STAR: case ML_AGG_COUNT:
         o.state = add_slot(ACC_SUM_I64, sc, ML_TYPE_INT64);
+        A->state_outs.push_back(o);
         break;
       case ML_AGG_SUM:
         o.state = add_slot(types[0] == ML_TYPE_FLOAT64 ? ACC_SUM_F64 : ACC_SUM_I64, sc, types[0]);
         o.count = nullable ? add_slot(ACC_COUNT, sc, types[0]) : 0;
+        A->state_outs.push_back(o);
         break;
       case ML_AGG_AVG:
         o.finish = FIN_AVG;
         o.state = add_slot(ACC_SUM_F64, sc, ML_TYPE_FLOAT64);
         o.count = add_slot(ACC_SUM_I64, sc + 1, ML_TYPE_INT64);
+        A->state_outs.push_back({ML_TYPE_FLOAT64, FIN_COPY, o.state, -1});
+        A->state_outs.push_back({ML_TYPE_INT64, FIN_COPY, o.count, -1});
         break;
       default: {  // MIN / MAX
         const bool mn = u.func == ML_AGG_MIN;
@@ -805,11 +827,13 @@ static void plan_final(ml_hashagg* A, PipelineBuild& B) {
         if (!f) o.finish = u.arg_type == ML_TYPE_INT64 ? FIN_COPY : FIN_I32;
         o.state = add_slot(f ? (mn ? ACC_MIN_F64 : ACC_MAX_F64) : (mn ? ACC_MIN_I64 : ACC_MAX_I64), sc, types[0]);
         o.count = nullable ? add_slot(ACC_COUNT, sc, types[0]) : 0;
+        A->state_outs.push_back({types[0], FIN_COPY, o.state, o.count});
         break;
       }
     }
     A->outs.push_back(o);
   }
+  if (A->emit_states) A->outs = A->state_outs;
   if ((int)A->slots.size() > MAX_AGG_SLOTS) fail(ML_ERR_UNSUPPORTED, "too many aggregate states");
 }
 
@@ -817,6 +841,7 @@ static void plan_final(ml_hashagg* A, PipelineBuild& B) {
 // list. Partial mode keeps Single mode's slots and emits the states instead of
 // the finished values (Accumulator::state, accumulator.rs:382-404).
 static void plan_agg(ml_hashagg* A, PipelineBuild& B) {
+  A->state_outs.clear();
   A->slots.clear();
   A->slot_cast.clear();
   A->outs.clear();
@@ -886,19 +911,21 @@ static void plan_agg(ml_hashagg* A, PipelineBuild& B) {
       }
       default: fail(ML_ERR_PLAN, "unknown aggregate function");
     }
-    if (A->mode == ML_AGG_MODE_PARTIAL) {
-      if (d.func == ML_AGG_AVG) {  // (sum, count), both always valid
-        A->outs.push_back({ML_TYPE_FLOAT64, FIN_COPY, o.state, -1});
-        A->outs.push_back({ML_TYPE_INT64, FIN_COPY, o.count, -1});
-        continue;
-      }
-      if (d.func == ML_AGG_MIN || d.func == ML_AGG_MAX) {  // integer states are Int64
-        o.finish = FIN_COPY;
-        if (o.out_type != ML_TYPE_FLOAT64) o.out_type = ML_TYPE_INT64;
-      }
+    // the partial states of this aggregate (accumulator.rs:382-404)
+    if (d.func == ML_AGG_AVG) {  // (sum, count), both always valid
+      A->state_outs.push_back({ML_TYPE_FLOAT64, FIN_COPY, o.state, -1});
+      A->state_outs.push_back({ML_TYPE_INT64, FIN_COPY, o.count, -1});
+    } else if (d.func == ML_AGG_MIN || d.func == ML_AGG_MAX) {  // integer states are Int64
+      OutDef st = o;
+      st.finish = FIN_COPY;
+      if (st.out_type != ML_TYPE_FLOAT64) st.out_type = ML_TYPE_INT64;
+      A->state_outs.push_back(st);
+    } else {
+      A->state_outs.push_back(o);
     }
-    A->outs.push_back(o);
+    if (A->mode != ML_AGG_MODE_PARTIAL) A->outs.push_back(o);
   }
+  if (A->mode == ML_AGG_MODE_PARTIAL) A->outs = A->state_outs;
   if ((int)A->slots.size() > MAX_AGG_SLOTS) fail(ML_ERR_UNSUPPORTED, "too many aggregate states");
 }
 
@@ -994,12 +1021,16 @@ static void init_acc(ml_hashagg* A, std::vector<Buffer>& acc, uint64_t cap) {
   }
 }
 
-// Grow the group table to hold `needed` groups (SwissTable::reserve, swiss.rs:219-232).
-static void agg_reserve(ml_hashagg* A, uint64_t needed) {
+// Grow the group table to hold `needed` groups (SwissTable::reserve,
+// swiss.rs:219-232). The new table is reserved against the context's budget
+// first (old and new coexist during the rehash); false = refused.
+static bool agg_reserve(ml_hashagg* A, uint64_t needed) {
   ml_ctx* ctx = A->ctx;
-  if (A->cap && needed * 8 <= A->cap * 7) return;
+  if (A->cap && needed * 8 <= A->cap * 7) return true;
   const uint64_t len = A->len_host;
   const uint64_t ncap = swiss_capacity(std::max(needed, len * 2));
+  const int64_t bytes = (int64_t)(ncap * (1 + A->nkw) * 8 + ncap * A->slots.size() * 8);
+  if (!ctx->mem_try_grow(bytes)) return false;
   Buffer nt(ctx->arena, ncap * (1 + A->nkw) * 8);
   zero(ctx, nt.ptr, nt.bytes);
   std::vector<Buffer> nacc;
@@ -1018,6 +1049,9 @@ static void agg_reserve(ml_hashagg* A, uint64_t needed) {
   A->acc = std::move(nacc);
   A->cap = ncap;
   A->wrap = ncap - 1;
+  ctx->mem_shrink(A->reserved);
+  A->reserved = bytes;
+  return true;
 }
 
 static uint64_t agg_len(ml_hashagg* A) {
@@ -1033,6 +1067,8 @@ static uint64_t agg_len(ml_hashagg* A) {
 static void agg_reset_table(ml_hashagg* A) {
   A->table.reset();
   A->acc.clear();
+  A->ctx->mem_shrink(A->reserved);
+  A->reserved = 0;
   A->cap = 0;
   A->len_host = 0;
   A->len_known = true;
@@ -1175,6 +1211,84 @@ static bool agg_partitioned(ml_hashagg* A, PipelineBuild& B) {
 }
 
 // One sink call: run the pipeline in B into the aggregate.
+static ml_result* agg_finalize(ml_hashagg* A, bool states);
+
+// MiniLakeError::ResourcesExhausted (memory.rs:110-121)
+[[noreturn]] static void agg_exhausted(ml_hashagg* A, uint64_t rows) {
+  const ml_ctx* ctx = A->ctx;
+  fail(ML_ERR_RESOURCES_EXHAUSTED, "HashAggregate: the group table for " + std::to_string(rows) +
+                                       " more rows does not fit the memory budget (" + std::to_string(ctx->mem_used) +
+                                       " of " + std::to_string(ctx->mem_limit) + " bytes reserved)");
+}
+
+constexpr int SPILL_PARTITIONS = 16;  // hash.rs:189-199
+
+// Spill (Inner::spill, hash.rs:220-277): the table's groups as partial states,
+// cut into 16 partitions by bits 32..35 of hash_columns over the group keys,
+// copied to host memory; the table is emptied and its reservation released.
+static void agg_spill(ml_hashagg* A) {
+  ml_ctx* ctx = A->ctx;
+  if (A->cap && agg_len(A) > 0) {
+    std::unique_ptr<ml_result> r(agg_finalize(A, true));
+    const int64_t n = r->num_rows;
+    const int nk = (int)A->key_types.size();
+    std::vector<ml_expr_node> nodes((size_t)nk);
+    std::vector<ml_expr> keys((size_t)nk);
+    for (int k = 0; k < nk; k++) {
+      nodes[(size_t)k] = ml_expr_node{};
+      nodes[(size_t)k].kind = ML_EXPR_COLUMN;
+      nodes[(size_t)k].column = k;
+      keys[(size_t)k].nodes = &nodes[(size_t)k];
+      keys[(size_t)k].num_nodes = 1;
+    }
+    ml_batch b{};
+    b.columns = r->views.data();
+    b.num_columns = nk;
+    b.num_rows = n;
+    Buffer hd(ctx->arena, (size_t)std::max<int64_t>(n, 1) * 8);
+    const ml_status st = ml_hash_columns(ctx, keys.data(), nk, &b, hd.as<uint64_t>());
+    if (st != ML_OK) fail(st, ctx->last_error);
+    std::vector<uint64_t> h((size_t)n);
+    d2h(ctx, h.data(), hd.ptr, (size_t)n * 8);
+    std::vector<ml_hashagg::SpillColumn> host;
+    for (auto& c : r->cols) {
+      ml_hashagg::SpillColumn hc;
+      hc.type = c.type;
+      hc.data.resize((size_t)n * type_width(c.type));
+      d2h(ctx, hc.data.data(), c.data.ptr, hc.data.size());
+      if (c.valid.ptr) {
+        hc.valid.resize((size_t)n);
+        d2h(ctx, hc.valid.data(), c.valid.ptr, (size_t)n);
+      }
+      host.push_back(std::move(hc));
+    }
+    if (A->spill.empty()) A->spill.resize(SPILL_PARTITIONS);
+    std::vector<std::vector<int64_t>> rows(SPILL_PARTITIONS);
+    for (int64_t i = 0; i < n; i++) rows[(size_t)((h[(size_t)i] >> 32) & (SPILL_PARTITIONS - 1))].push_back(i);
+    for (int p = 0; p < SPILL_PARTITIONS; p++) {
+      if (rows[(size_t)p].empty()) continue;
+      ml_hashagg::SpillBatch sb;
+      sb.rows = (int64_t)rows[(size_t)p].size();
+      for (auto& hc : host) {
+        ml_hashagg::SpillColumn pc;
+        pc.type = hc.type;
+        const size_t w = (size_t)type_width(hc.type);
+        pc.data.resize((size_t)sb.rows * w);
+        if (!hc.valid.empty()) pc.valid.resize((size_t)sb.rows);
+        for (size_t j = 0; j < rows[(size_t)p].size(); j++) {
+          const size_t i = (size_t)rows[(size_t)p][j];
+          std::memcpy(&pc.data[j * w], &hc.data[i * w], w);
+          if (!hc.valid.empty()) pc.valid[j] = hc.valid[i];
+        }
+        sb.cols.push_back(std::move(pc));
+      }
+      A->spill[(size_t)p].push_back(std::move(sb));
+    }
+  }
+  agg_reset_table(A);
+  A->spilled = true;
+}
+
 static void agg_run(ml_hashagg* A, PipelineBuild& B) {
   ml_ctx* ctx = A->ctx;
   const bool first = !A->planned;
@@ -1284,22 +1398,28 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
     else ctx->launch(B.P, B.grid(), block_of(ctx), 0);
     A->len_known = false;
   };
-  if (A->sizing != ML_SIZING_PLAIN && !A->cap && (uint64_t)A->hint >= PART_AGG_MIN_GROUPS &&
-      agg_partitioned(A, B))
-    return;
-  if (A->sizing == ML_SIZING_PLAIN) {
+  // under a memory budget the table grows (and spills) step by step
+  const bool plain = A->sizing == ML_SIZING_PLAIN || ctx->mem_limit > 0;
+  if (!plain && !A->cap && (uint64_t)A->hint >= PART_AGG_MIN_GROUPS && agg_partitioned(A, B)) return;
+  if (plain) {
     // reserve(len + batch) before every device batch, doubling as needed
     const int64_t batch = ctx->cfg.batch_rows > 0 ? ctx->cfg.batch_rows : (1 << 21);
     for (int64_t b = 0; b < total || (b == 0 && total == 0); b += batch) {
       const int64_t e = std::min(total, b + batch);
       if (e <= b) break;
-      agg_reserve(A, agg_len(A) + (uint64_t)(e - b));
+      if (!agg_reserve(A, agg_len(A) + (uint64_t)(e - b))) {
+        // over budget (HashAggLocal::account, hash.rs:369-390): spill the
+        // groups so far, then the batch must fit an empty table
+        if (!ctx->mem_spill || A->no_spill) agg_exhausted(A, (uint64_t)(e - b));
+        agg_spill(A);
+        if (!agg_reserve(A, (uint64_t)(e - b))) agg_exhausted(A, (uint64_t)(e - b));
+      }
       launch_range(b, e, A->cap);
     }
   } else {
     if (!A->cap) {
       A->len_host = 0;
-      agg_reserve(A, std::max<uint64_t>(1, (uint64_t)A->hint));
+      if (!agg_reserve(A, std::max<uint64_t>(1, (uint64_t)A->hint))) agg_exhausted(A, (uint64_t)A->hint);
     }
     launch_range(0, total, A->cap * 7 / 8);
   }
@@ -1313,8 +1433,9 @@ static uint32_t agg_overflow_bits(ml_hashagg* A) {
   return (uint32_t)f[1];
 }
 
-static ml_result* agg_finalize(ml_hashagg* A) {
+static ml_result* agg_finalize(ml_hashagg* A, bool states) {
   ml_ctx* ctx = A->ctx;
+  const std::vector<OutDef>& outs = states ? A->state_outs : A->outs;
   auto* r = new ml_result();
   r->ctx = ctx;
   r->device = ctx->device;
@@ -1386,7 +1507,7 @@ static ml_result* agg_finalize(ml_hashagg* A) {
       }
       r->cols.push_back(std::move(c));
     }
-    for (auto& o : A->outs) {
+    for (auto& o : outs) {
       ResultColumn c;
       c.type = o.out_type;
       const int w = type_width(c.type);
@@ -1448,9 +1569,9 @@ static ml_result* agg_finalize(ml_hashagg* A) {
     E.key_valid[k] = c.valid.ptr ? c.valid.as<uint8_t>() : nullptr;
     r->cols.push_back(std::move(c));
   }
-  E.nout = (int)A->outs.size();
-  for (size_t a = 0; a < A->outs.size(); a++) {
-    const OutDef& o = A->outs[a];
+  E.nout = (int)outs.size();
+  for (size_t a = 0; a < outs.size(); a++) {
+    const OutDef& o = outs[a];
     ResultColumn c;
     c.type = o.out_type;
     const int w = type_width(c.type);
@@ -1468,6 +1589,135 @@ static ml_result* agg_finalize(ml_hashagg* A) {
   hip_check(hipStreamSynchronize(ctx->stream), "sync");
   r->finish_views();
   return r;
+}
+
+// Rows of several results of one schema, one after another.
+static ml_result* concat_results(ml_ctx* ctx, std::vector<std::unique_ptr<ml_result>>& parts) {
+  auto* r = new ml_result();
+  r->ctx = ctx;
+  r->device = ctx->device;
+  int64_t total = 0;
+  for (auto& p : parts) total += p->num_rows;
+  r->num_rows = total;
+  for (size_t c = 0; c < parts[0]->cols.size(); c++) {
+    ResultColumn rc;
+    rc.type = parts[0]->cols[c].type;
+    rc.dict = parts[0]->cols[c].dict;
+    const int w = type_width(rc.type);
+    rc.data = Buffer(ctx->arena, (size_t)std::max<int64_t>(total, 1) * w);
+    bool any_valid = false;
+    for (auto& p : parts) any_valid = any_valid || p->cols[c].valid.ptr;
+    if (any_valid) rc.valid = Buffer(ctx->arena, (size_t)std::max<int64_t>(total, 1));
+    int64_t off = 0;
+    for (auto& p : parts) {
+      const int64_t n = p->num_rows;
+      if (n > 0) {
+        d2d(ctx, rc.data.as<uint8_t>() + off * w, p->cols[c].data.ptr, (size_t)n * w);
+        if (any_valid) {
+          if (p->cols[c].valid.ptr) {
+            d2d(ctx, rc.valid.as<uint8_t>() + off, p->cols[c].valid.ptr, (size_t)n);
+          } else {
+            const std::vector<uint8_t> ones((size_t)n, 1);
+            upload(ctx, rc.valid.as<uint8_t>() + off, ones.data(), ones.size());
+          }
+        }
+      }
+      off += n;
+    }
+    r->cols.push_back(std::move(rc));
+  }
+  r->finish_views();
+  hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+  return r;
+}
+
+// HashAggregateSink::finalize after a spill (hash.rs:394-425): the groups
+// still in the table are spilled too; then each partition's states are merged
+// by a Final aggregate of their own (about 1/16 of the groups in memory at a
+// time, over budget now an error) and the partitions' rows concatenated. A
+// Partial aggregate's merge emits states again.
+static ml_result* agg_merge_spilled(ml_hashagg* A) {
+  ml_ctx* ctx = A->ctx;
+  agg_spill(A);
+  const int nk = (int)A->key_types.size();
+  std::vector<ml_aggregate> aggs(A->user_aggs.size());
+  for (size_t a = 0; a < aggs.size(); a++) {
+    aggs[a] = ml_aggregate{};
+    aggs[a].func = A->user_aggs[a].func;
+    aggs[a].arg_type = A->user_aggs[a].arg_type;
+  }
+  ml_agg_spec spec{};
+  spec.key_types = A->key_types.data();
+  spec.num_keys = nk;
+  spec.num_aggs = (int32_t)aggs.size();
+  spec.aggs = aggs.data();
+  spec.mode = ML_AGG_MODE_FINAL;
+  spec.sizing = ML_SIZING_PLAIN;
+  auto check = [&](ml_status st) {
+    if (st != ML_OK) fail(st, ctx->last_error);
+  };
+  std::vector<std::unique_ptr<ml_result>> parts;
+  for (size_t p = 0; p < A->spill.size(); p++) {
+    if (A->spill[p].empty()) continue;
+    ml_hashagg* M = nullptr;
+    check(ml_hashagg_create(ctx, &spec, &M));
+    std::unique_ptr<ml_hashagg, void (*)(ml_hashagg*)> own(M, ml_hashagg_free);
+    M->emit_states = A->mode == ML_AGG_MODE_PARTIAL;
+    M->no_spill = true;
+    for (auto& sb : A->spill[p]) {
+      std::vector<Buffer> bufs;
+      std::vector<ml_column> cols(sb.cols.size());
+      for (size_t c = 0; c < sb.cols.size(); c++) {
+        const auto& sc = sb.cols[c];
+        Buffer d(ctx->arena, std::max<size_t>(sc.data.size(), 8));
+        upload(ctx, d.ptr, sc.data.data(), sc.data.size());
+        cols[c] = ml_column{};
+        cols[c].type = sc.type;
+        cols[c].data = d.ptr;
+        cols[c].len = sb.rows;
+        if (!sc.valid.empty()) {
+          Buffer vb(ctx->arena, sc.valid.size());
+          upload(ctx, vb.ptr, sc.valid.data(), sc.valid.size());
+          Buffer bits(ctx->arena, (size_t)((sb.rows + 63) / 64) * 8);
+          launch_bytes_to_bits(vb.as<uint8_t>(), sb.rows, bits.as<uint64_t>(), ctx->stream);
+          cols[c].validity = bits.as<uint64_t>();
+          bufs.push_back(std::move(vb));
+          bufs.push_back(std::move(bits));
+        }
+        if ((int)c < nk && sc.type == ML_TYPE_UTF8) cols[c].dict = &A->key_dicts[c]->view;
+        bufs.push_back(std::move(d));
+      }
+      ml_batch b{};
+      b.columns = cols.data();
+      b.num_columns = (int32_t)cols.size();
+      b.num_rows = sb.rows;
+      check(ml_hashagg_sink(M, &b));
+      sb = ml_hashagg::SpillBatch{};  // the host copy is no longer needed
+    }
+    ml_result* r = nullptr;
+    check(ml_hashagg_finalize(M, &r));
+    parts.emplace_back(r);
+  }
+  A->spill.clear();
+  if (parts.empty()) {  // nothing was ever held: an empty result of the aggregate's shape
+    auto* r = new ml_result();
+    r->ctx = ctx;
+    r->device = ctx->device;
+    for (int k = 0; k < nk; k++) {
+      ResultColumn c;
+      c.type = A->key_types[(size_t)k];
+      c.dict = A->key_dicts.empty() ? nullptr : A->key_dicts[(size_t)k];
+      r->cols.push_back(std::move(c));
+    }
+    for (auto& o : A->outs) {
+      ResultColumn c;
+      c.type = o.out_type;
+      r->cols.push_back(std::move(c));
+    }
+    r->finish_views();
+    return r;
+  }
+  return concat_results(ctx, parts);
 }
 
 }  // namespace mlg
@@ -2570,7 +2820,21 @@ ml_status ml_pipeline_hashagg(ml_ctx* ctx, const ml_batch* source, const ml_oper
 }
 
 ml_status ml_hashagg_finalize(ml_hashagg* agg, ml_result** out) {
-  return guarded(agg->ctx, [&] { *out = agg_finalize(agg); });
+  return guarded(agg->ctx, [&] { *out = agg->spilled ? agg_merge_spilled(agg) : agg_finalize(agg, false); });
+}
+
+ml_status ml_ctx_set_memory_limit(ml_ctx* ctx, int64_t bytes, int32_t spill) {
+  return guarded(ctx, [&] {
+    if (bytes < 0) fail(ML_ERR_PLAN, "negative memory limit");
+    ctx->mem_limit = bytes;
+    ctx->mem_spill = spill != 0;
+  });
+}
+
+void ml_ctx_memory_usage(const ml_ctx* ctx, int64_t* used, int64_t* peak, int64_t* limit) {
+  if (used) *used = ctx->mem_used;
+  if (peak) *peak = ctx->mem_peak;
+  if (limit) *limit = ctx->mem_limit;
 }
 
 ml_status ml_hashagg_num_groups(ml_hashagg* agg, int64_t* out) {

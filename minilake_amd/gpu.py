@@ -48,7 +48,7 @@ EXPORTED = [
     "ml_pipeline_limit", "ml_mlb1_write", "ml_mlb1_read", "ml_parquet_open", "ml_parquet_num_columns",
     "ml_parquet_column_type", "ml_parquet_column_name", "ml_parquet_column_nullable", "ml_parquet_num_row_groups",
     "ml_parquet_row_group_rows", "ml_parquet_column_stats", "ml_parquet_prune", "ml_parquet_read_row_group",
-    "ml_parquet_free",
+    "ml_parquet_free", "ml_ctx_set_memory_limit", "ml_ctx_memory_usage",
     "ml_topn_create", "ml_topn_sink", "ml_topn_finalize", "ml_topn_free", "ml_ctx_enable_stats",
     "ml_ctx_take_stats", "ml_fragments_create", "ml_fragments_run", "ml_fragments_run_graph",
     "ml_fragments_run_per_fragment", "ml_fragments_autotune", "ml_fragments_free", "ml_mlp_create",
@@ -123,6 +123,8 @@ def lib():
             "ml_parquet_prune": ([vp, P(abi.MlPrunePredicate), i32, P(i32), P(i32)], i32),
             "ml_parquet_read_row_group": ([vp, i32, P(i32), i32, P(vp)], i32),
             "ml_parquet_free": ([vp], None),
+            "ml_ctx_set_memory_limit": ([vp, i64, i32], i32),
+            "ml_ctx_memory_usage": ([vp, P(i64), P(i64), P(i64)], None),
             "ml_pipeline_collect_sized": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, i64, P(i32), P(vp)], i32),
             "ml_pipeline_count": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, P(i64)], i32),
             "ml_topn_create": ([vp, P(abi.MlSortSpec), P(vp)], i32),
@@ -466,6 +468,15 @@ class GpuEngine:
     def _check(self, rc):
         if rc != 0:
             raise GpuError(rc, lib().ml_last_error(self.ctx).decode())
+
+    def set_memory_limit(self, nbytes: int, spill: bool = False):
+        """MemoryPool (memory.rs:22-157) for this context; 0 = unlimited."""
+        self._check(lib().ml_ctx_set_memory_limit(self.ctx, int(nbytes), int(spill)))
+
+    def memory_usage(self) -> dict:
+        used, peak, limit = C.c_int64(), C.c_int64(), C.c_int64()
+        lib().ml_ctx_memory_usage(self.ctx, C.byref(used), C.byref(peak), C.byref(limit))
+        return {"used": used.value, "peak": peak.value, "limit": limit.value}
 
     def synchronize(self):
         self._check(lib().ml_ctx_synchronize(self.ctx))
