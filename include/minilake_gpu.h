@@ -178,8 +178,9 @@ ml_status ml_ctx_enable_stats(ml_ctx* ctx, int32_t on);
 int32_t ml_ctx_take_stats(ml_ctx* ctx, ml_launch_stat* out, int32_t cap);
 
 /* MemoryPool (crates/minilake-exec/src/memory.rs:22-157): a budget in bytes
- * for the context's pipeline-breaker state (hash-aggregate group tables);
- * 0 = unlimited. Over budget, a hash aggregate returns
+ * for the context's pipeline-breaker state (hash-aggregate group tables, join
+ * tables from ml_join_build_finalize until ml_join_table_free); 0 = unlimited.
+ * A join table over budget is ML_ERR_RESOURCES_EXHAUSTED. Over budget, a hash aggregate returns
  * ML_ERR_RESOURCES_EXHAUSTED, or with spill != 0 writes its groups' partial
  * states to host memory in 16 hash partitions (aggregate/hash.rs:189-282) and
  * merges one partition at a time at finalize. */

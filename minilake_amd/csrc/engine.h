@@ -109,6 +109,23 @@ struct HostDict {
 // ------------------------------------------------------------------ context
 struct MlpModel;
 
+// MemoryPool (memory.rs:22-76): a budget for pipeline-breaker state (group
+// tables, join tables); limit 0 = unlimited. With spill on, a hash aggregate
+// over budget writes its partial states to host memory.
+struct MemPool {
+  int64_t limit = 0;
+  int64_t used = 0;
+  int64_t peak = 0;
+  bool spill = false;
+  bool try_grow(int64_t bytes) {
+    if (bytes > 0 && limit > 0 && used + bytes > limit) return false;
+    used += bytes;
+    if (used > peak) peak = used;
+    return true;
+  }
+  void shrink(int64_t bytes) { used -= bytes < used ? bytes : used; }
+};
+
 // Pinned host words that receive the flags of checks left running on the
 // stream (a join build's duplicate-key or partition-overflow test), so the
 // host need not wait for them when the build is launched. Shared by a
@@ -151,20 +168,8 @@ struct ml_ctx {
   uint64_t synced_seq = 0;  // deferred checks the stream has completed (seen at a synchronization)
   std::map<const ml_dict*, std::shared_ptr<mlg::HostDict>> dicts;
   std::shared_ptr<mlg::HostDict> dict(const ml_dict* d);
-  // MemoryPool (memory.rs:22-76): a budget for pipeline-breaker state
-  // (group tables, join tables); 0 = unlimited. With spill on, a hash
-  // aggregate over budget writes its partial states to host memory.
-  int64_t mem_limit = 0;
-  int64_t mem_used = 0;
-  int64_t mem_peak = 0;
-  bool mem_spill = false;
-  bool mem_try_grow(int64_t bytes) {
-    if (bytes > 0 && mem_limit > 0 && mem_used + bytes > mem_limit) return false;
-    mem_used += bytes;
-    if (mem_used > mem_peak) mem_peak = mem_used;
-    return true;
-  }
-  void mem_shrink(int64_t bytes) { mem_used -= bytes < mem_used ? bytes : mem_used; }
+  // the budget for pipeline-breaker state (shared with the join tables built here)
+  std::shared_ptr<mlg::MemPool> mem = std::make_shared<mlg::MemPool>();
   // launch statistics
   bool stats_on = false;
   int64_t stats_min_rows = 0;  // record launches over at least this many source rows
@@ -312,8 +317,11 @@ enum : int { PENDING_NONE = 0, PENDING_DENSE = 1, PENDING_PARTS = 2 };
 struct ml_join_table {
   ml_ctx* ctx = nullptr;
   mlg::JoinTableImpl impl;
+  std::shared_ptr<mlg::MemPool> pool;  // the budget its bytes are reserved against
+  int64_t reserved = 0;
   ~ml_join_table() {
     if (impl.checks) impl.checks->give(impl.check);
+    if (pool) pool->shrink(reserved);
   }
 };
 

@@ -728,7 +728,7 @@ struct ml_hashagg {
   };
   std::vector<std::vector<SpillBatch>> spill;  // per partition, the spilled state batches
   ~ml_hashagg() {
-    if (ctx) ctx->mem_shrink(reserved);
+    if (ctx) ctx->mem->shrink(reserved);
   }
 };
 
@@ -1030,7 +1030,7 @@ static bool agg_reserve(ml_hashagg* A, uint64_t needed) {
   const uint64_t len = A->len_host;
   const uint64_t ncap = swiss_capacity(std::max(needed, len * 2));
   const int64_t bytes = (int64_t)(ncap * (1 + A->nkw) * 8 + ncap * A->slots.size() * 8);
-  if (!ctx->mem_try_grow(bytes)) return false;
+  if (!ctx->mem->try_grow(bytes)) return false;
   Buffer nt(ctx->arena, ncap * (1 + A->nkw) * 8);
   zero(ctx, nt.ptr, nt.bytes);
   std::vector<Buffer> nacc;
@@ -1049,7 +1049,7 @@ static bool agg_reserve(ml_hashagg* A, uint64_t needed) {
   A->acc = std::move(nacc);
   A->cap = ncap;
   A->wrap = ncap - 1;
-  ctx->mem_shrink(A->reserved);
+  ctx->mem->shrink(A->reserved);
   A->reserved = bytes;
   return true;
 }
@@ -1067,7 +1067,7 @@ static uint64_t agg_len(ml_hashagg* A) {
 static void agg_reset_table(ml_hashagg* A) {
   A->table.reset();
   A->acc.clear();
-  A->ctx->mem_shrink(A->reserved);
+  A->ctx->mem->shrink(A->reserved);
   A->reserved = 0;
   A->cap = 0;
   A->len_host = 0;
@@ -1217,8 +1217,8 @@ static ml_result* agg_finalize(ml_hashagg* A, bool states);
 [[noreturn]] static void agg_exhausted(ml_hashagg* A, uint64_t rows) {
   const ml_ctx* ctx = A->ctx;
   fail(ML_ERR_RESOURCES_EXHAUSTED, "HashAggregate: the group table for " + std::to_string(rows) +
-                                       " more rows does not fit the memory budget (" + std::to_string(ctx->mem_used) +
-                                       " of " + std::to_string(ctx->mem_limit) + " bytes reserved)");
+                                       " more rows does not fit the memory budget (" + std::to_string(ctx->mem->used) +
+                                       " of " + std::to_string(ctx->mem->limit) + " bytes reserved)");
 }
 
 constexpr int SPILL_PARTITIONS = 16;  // hash.rs:189-199
@@ -1399,7 +1399,7 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
     A->len_known = false;
   };
   // under a memory budget the table grows (and spills) step by step
-  const bool plain = A->sizing == ML_SIZING_PLAIN || ctx->mem_limit > 0;
+  const bool plain = A->sizing == ML_SIZING_PLAIN || ctx->mem->limit > 0;
   if (!plain && !A->cap && (uint64_t)A->hint >= PART_AGG_MIN_GROUPS && agg_partitioned(A, B)) return;
   if (plain) {
     // reserve(len + batch) before every device batch, doubling as needed
@@ -1410,7 +1410,7 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
       if (!agg_reserve(A, agg_len(A) + (uint64_t)(e - b))) {
         // over budget (HashAggLocal::account, hash.rs:369-390): spill the
         // groups so far, then the batch must fit an empty table
-        if (!ctx->mem_spill || A->no_spill) agg_exhausted(A, (uint64_t)(e - b));
+        if (!ctx->mem->spill || A->no_spill) agg_exhausted(A, (uint64_t)(e - b));
         agg_spill(A);
         if (!agg_reserve(A, (uint64_t)(e - b))) agg_exhausted(A, (uint64_t)(e - b));
       }
@@ -2826,15 +2826,15 @@ ml_status ml_hashagg_finalize(ml_hashagg* agg, ml_result** out) {
 ml_status ml_ctx_set_memory_limit(ml_ctx* ctx, int64_t bytes, int32_t spill) {
   return guarded(ctx, [&] {
     if (bytes < 0) fail(ML_ERR_PLAN, "negative memory limit");
-    ctx->mem_limit = bytes;
-    ctx->mem_spill = spill != 0;
+    ctx->mem->limit = bytes;
+    ctx->mem->spill = spill != 0;
   });
 }
 
 void ml_ctx_memory_usage(const ml_ctx* ctx, int64_t* used, int64_t* peak, int64_t* limit) {
-  if (used) *used = ctx->mem_used;
-  if (peak) *peak = ctx->mem_peak;
-  if (limit) *limit = ctx->mem_limit;
+  if (used) *used = ctx->mem->used;
+  if (peak) *peak = ctx->mem->peak;
+  if (limit) *limit = ctx->mem->limit;
 }
 
 ml_status ml_hashagg_num_groups(ml_hashagg* agg, int64_t* out) {
@@ -2905,7 +2905,21 @@ ml_status ml_join_build_finalize(ml_join_build* b, ml_join_table** out) {
   ml_ctx* ctx = b->ctx;
   ml_status st = guarded(ctx, [&] {
     if (!b->schema_fixed) fail(ML_ERR_EXECUTION, "join build received no batch");
-    *out = join_build_finalize(b);
+    std::unique_ptr<ml_join_table> t(join_build_finalize(b));
+    // the table (rows, key structures, filters) held against the budget
+    // (JoinBuildSink::finalize, join.rs:120-153)
+    const JoinTableImpl& J = t->impl;
+    int64_t bytes = (int64_t)(J.slots.bytes + J.bloom.bytes + J.dense_bits.bytes + J.dense_rank.bytes +
+                              J.dense_keys.bytes + J.keys.bytes + J.key_null.bytes);
+    for (auto* v : {&J.cols, &J.col_valid, &J.orig_cols, &J.orig_valid})
+      for (const Buffer& x : *v) bytes += (int64_t)x.bytes;
+    if (!ctx->mem->try_grow(bytes))
+      fail(ML_ERR_RESOURCES_EXHAUSTED, "HashJoinBuild(table): " + std::to_string(bytes) + " bytes do not fit the memory budget (" +
+                                           std::to_string(ctx->mem->used) + " of " + std::to_string(ctx->mem->limit) +
+                                           " bytes reserved)");
+    t->pool = ctx->mem;
+    t->reserved = bytes;
+    *out = t.release();
   });
   delete b;
   return st;

@@ -418,10 +418,13 @@ class GpuJoin:
             device_to_host(out.ctypes.data, mc.data, n * out.itemsize)
         return out[:n]
 
-    def __del__(self):
+    def close(self):
         if self.handle and _lib is not None:
             _lib.ml_join_table_free(self.handle)
-            self.handle = None
+        self.handle = None
+
+    def __del__(self):
+        self.close()
 
 
 class GpuEngine:

@@ -546,6 +546,28 @@ def test_memory_budget_and_spill(oracle_engine, mode):
         eng.close()
 
 
+def test_join_table_reserved_against_budget():
+    """JoinBuildSink::finalize reserves the built table (join.rs:120-153): over
+    the budget ResourcesExhausted; otherwise held until the table is freed."""
+    from minilake_amd.gpu import GpuEngine, GpuError
+    rng = np.random.default_rng(5)
+    t = HostTable([HostColumn(INT64, rng.permutation(100_000).astype(np.int64)), HostColumn(FLOAT64, rng.random(100_000))])
+    eng = GpuEngine()
+    try:
+        eng.set_memory_limit(1 << 20)
+        with pytest.raises(GpuError) as e:
+            eng.join_build(t, [], [col(0)], [INT64])
+        assert e.value.code == abi.ML_ERR_RESOURCES_EXHAUSTED
+        assert eng.memory_usage()["used"] == 0
+        eng.set_memory_limit(64 << 20)
+        j = eng.join_build(t, [], [col(0)], [INT64])
+        assert 1 << 20 < eng.memory_usage()["used"] <= 64 << 20
+        j.close()
+        assert eng.memory_usage()["used"] == 0
+    finally:
+        eng.close()
+
+
 def _modes_table(n, seed):
     rng = np.random.default_rng(seed)
     return HostTable([HostColumn(INT32, rng.integers(-3, 4, n).astype(np.int32)),
