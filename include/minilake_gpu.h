@@ -203,9 +203,10 @@ void ml_result_free(ml_result* r);
 /* ------------------------------------------------------------------ */
 /* Device Parquet scan (crates/minilake-storage/src/table.rs:59-240,
  * parquet_reader.rs:210-358, stats.rs:53-70). Footer and page headers are
- * parsed on the host; page bodies are decoded on the device. Flat schemas,
- * UNCOMPRESSED chunks, data pages v1, PLAIN and dictionary encodings, RLE
- * definition levels; other files return ML_ERR_UNSUPPORTED.              */
+ * parsed on the host (SNAPPY page bodies decompressed there); page bodies
+ * are decoded on the device. Flat schemas, UNCOMPRESSED and SNAPPY chunks,
+ * data pages v1, PLAIN and dictionary encodings, RLE definition levels;
+ * other files return ML_ERR_UNSUPPORTED.                                 */
 /* ------------------------------------------------------------------ */
 typedef struct ml_parquet ml_parquet;
 

@@ -7,7 +7,7 @@
 // convert_stats (crates/minilake-storage/src/parquet_reader.rs:210-358), whose
 // page decoding the reference delegates to the `parquet` crate (60.0.0, the
 // Apache Parquet format). Supported: flat schemas (REQUIRED / OPTIONAL
-// columns), UNCOMPRESSED chunks, data pages v1, PLAIN and dictionary
+// columns), UNCOMPRESSED and SNAPPY chunks, data pages v1, PLAIN and dictionary
 // (PLAIN_DICTIONARY / RLE_DICTIONARY) encodings, RLE definition levels.
 #include <cstring>
 #include <deque>
@@ -226,6 +226,7 @@ static void parse_row_group(TReader& r, RowGroupMeta& g) {
 
 struct PageHeader {
   int type = -1;
+  int64_t uncompressed = 0;
   int64_t compressed = 0;
   int64_t num_values = 0;
   int encoding = -1, def_encoding = -1;
@@ -236,6 +237,8 @@ static void parse_page_header(TReader& r, PageHeader& h) {
   while (r.field(&last, &id, &t)) {
     if (id == 1) {
       h.type = (int)r.integer(t);
+    } else if (id == 2) {
+      h.uncompressed = r.integer(t);
     } else if (id == 3) {
       h.compressed = r.integer(t);
     } else if ((id == 5 || id == 7) && t == 12) {  // DataPageHeader / DictionaryPageHeader
@@ -251,6 +254,72 @@ static void parse_page_header(TReader& r, PageHeader& h) {
     }
   }
 }
+
+// Snappy, from the format's published description: the uncompressed length
+// as a little-endian varint, then elements: a literal (tag & 3 == 0; length
+// - 1 in the tag's upper 6 bits, or in the 1-4 bytes after it when those bits
+// are 60-63) or a back-reference of 1-, 2- or 4-byte offset (tag & 3 = 1, 2,
+// 3), copied forward byte by byte so a reference may overlap its output.
+static void snappy_decompress(const uint8_t* in, size_t n, size_t expect, std::vector<uint8_t>& out) {
+  size_t i = 0;
+  uint64_t len = 0;
+  for (int shift = 0;; shift += 7) {
+    if (i >= n || shift > 28) corrupt("snappy length");
+    const uint8_t b = in[i++];
+    len |= (uint64_t)(b & 0x7f) << shift;
+    if (!(b & 0x80)) break;
+  }
+  if (len != expect) corrupt("snappy length differs from the page header");
+  const size_t base = out.size();
+  out.resize(base + (size_t)len);
+  uint8_t* o = out.data() + base;
+  size_t op = 0;
+  while (i < n) {
+    const uint8_t tag = in[i++];
+    size_t l, off;
+    if ((tag & 3) == 0) {
+      l = tag >> 2;
+      if (l >= 60) {
+        const size_t nb = l - 59;
+        if (i + nb > n) corrupt("snappy literal length");
+        l = 0;
+        for (size_t k = 0; k < nb; k++) l |= (size_t)in[i + k] << (8 * k);
+        i += nb;
+      }
+      l += 1;
+      if (l > n - i || l > len - op) corrupt("snappy literal past the end");
+      std::memcpy(o + op, in + i, l);
+      i += l;
+      op += l;
+      continue;
+    }
+    if ((tag & 3) == 1) {
+      if (i >= n) corrupt("snappy copy");
+      l = 4 + ((tag >> 2) & 7);
+      off = ((size_t)(tag >> 5) << 8) | in[i++];
+    } else if ((tag & 3) == 2) {
+      if (n - i < 2) corrupt("snappy copy");
+      l = (size_t)(tag >> 2) + 1;
+      off = (size_t)in[i] | ((size_t)in[i + 1] << 8);
+      i += 2;
+    } else {
+      if (n - i < 4) corrupt("snappy copy");
+      l = (size_t)(tag >> 2) + 1;
+      off = (size_t)in[i] | ((size_t)in[i + 1] << 8) | ((size_t)in[i + 2] << 16) | ((size_t)in[i + 3] << 24);
+      i += 4;
+    }
+    if (off == 0 || off > op || l > len - op) corrupt("snappy back-reference");
+    for (size_t k = 0; k < l; k++) o[op + k] = o[op + k - off];
+    op += l;
+  }
+  if (op != len) corrupt("snappy output short");
+}
+
+// A page of a column chunk: its header and where its (uncompressed) body lies.
+struct PageRef {
+  PageHeader h;
+  size_t body, end;
+};
 
 // Page table entry of a column chunk (device): rows [first_row, first_row + rows).
 struct PageDesc {
@@ -733,12 +802,36 @@ ml_status ml_parquet_read_row_group(ml_parquet* p, int32_t rg, const int32_t* pr
       const SchemaElem& e = p->leaves[(size_t)c];
       const int cv = p->conv[(size_t)c];
       const int maxd = p->max_def[(size_t)c];
-      if (m.codec != 0) fail(ML_ERR_UNSUPPORTED, "compressed parquet column chunk (codec " + std::to_string(m.codec) + ")");
+      if (m.codec != 0 && m.codec != 1)
+        fail(ML_ERR_UNSUPPORTED, "parquet column chunk codec " + std::to_string(m.codec) + " (only UNCOMPRESSED and SNAPPY)");
       const int64_t start = m.dict_page_offset >= 0 && m.dict_page_offset < m.data_page_offset ? m.dict_page_offset
                                                                                                : m.data_page_offset;
       if (start < 4 || m.total_compressed < 0 || start + m.total_compressed > p->len - 8) corrupt("column chunk range");
-      const uint8_t* chunk = p->file + start;
-      const int64_t clen = m.total_compressed;
+      const uint8_t* raw = p->file + start;
+      const int64_t rawlen = m.total_compressed;
+      // the pages; a SNAPPY chunk's bodies are decompressed into one host image
+      std::vector<PageRef> refs;
+      std::vector<uint8_t> image;
+      for (size_t pos = 0; pos < (size_t)rawlen;) {
+        PageRef pr;
+        TReader r{raw, (size_t)rawlen, pos};
+        parse_page_header(r, pr.h);
+        if (pr.h.compressed < 0 || (int64_t)r.pos + pr.h.compressed > rawlen) corrupt("page past the column chunk");
+        if (m.codec == 1) {
+          if (pr.h.uncompressed < 0) corrupt("page size");
+          pr.body = image.size();
+          snappy_decompress(raw + r.pos, (size_t)pr.h.compressed, (size_t)pr.h.uncompressed, image);
+          pr.end = image.size();
+        } else {
+          pr.body = r.pos;
+          pr.end = r.pos + (size_t)pr.h.compressed;
+        }
+        pos = r.pos + (size_t)pr.h.compressed;
+        refs.push_back(pr);
+      }
+      const int64_t clen = m.codec == 1 ? (int64_t)image.size() : rawlen;
+      if (m.codec == 1) host_keep.push_back(std::move(image));  // uploaded below; kept until the final synchronize
+      const uint8_t* chunk = m.codec == 1 ? host_keep.back().data() : raw;
       // page headers (host)
       std::vector<PageDesc> pages;
       std::vector<Run> def_runs, idx_runs;
@@ -757,14 +850,10 @@ ml_status ml_parquet_read_row_group(ml_parquet* p, int32_t rg, const int32_t* pr
       bool have_dict = false;
       const int width = cv == C_DEC_BYTES ? e.type_length
                         : (m.type == T_INT64 || m.type == T_DOUBLE) ? 8 : (m.type == T_BOOLEAN ? 0 : 4);
-      size_t pos = 0;
-      while (pos < (size_t)clen && row < N) {
-        PageHeader h;
-        TReader r{chunk, (size_t)clen, pos};
-        parse_page_header(r, h);
-        const size_t body = r.pos;
-        if (h.compressed < 0 || (int64_t)body + h.compressed > clen) corrupt("page past the column chunk");
-        const size_t end = body + (size_t)h.compressed;
+      for (const PageRef& pr : refs) {
+        if (row >= N) break;
+        const PageHeader& h = pr.h;
+        const size_t body = pr.body, end = pr.end;
         if (h.type == PAGE_DICTIONARY) {
           if (h.encoding != ENC_PLAIN && h.encoding != ENC_PLAIN_DICTIONARY) fail(ML_ERR_UNSUPPORTED, "dictionary page encoding");
           if (cv == C_BOOL) fail(ML_ERR_UNSUPPORTED, "dictionary-encoded booleans");
@@ -783,7 +872,7 @@ ml_status ml_parquet_read_row_group(ml_parquet* p, int32_t rg, const int32_t* pr
               remap.push_back(intern(std::string_view(reinterpret_cast<const char*>(chunk) + d.pos, l)));
               d.pos += l;
             }
-          } else if ((int64_t)dict_n * width > (int64_t)h.compressed) {
+          } else if ((int64_t)dict_n * width > (int64_t)(end - body)) {
             corrupt("dictionary page too short");
           }
         } else if (h.type == PAGE_DATA) {
@@ -837,7 +926,6 @@ ml_status ml_parquet_read_row_group(ml_parquet* p, int32_t rg, const int32_t* pr
         } else if (h.type == PAGE_DATA_V2) {
           fail(ML_ERR_UNSUPPORTED, "parquet data page v2");
         }
-        pos = end;
       }
       if (row != N) fail(ML_ERR_PARQUET, "expected " + std::to_string(N) + " rows, decoded " + std::to_string(row));
       // device: chunk bytes, tables, runs

@@ -4,7 +4,7 @@ tests/, never by the product path).
 MiniLake reads Parquet through the `parquet` crate, version 60.0.0 (Cargo.lock),
 which is not part of the reference tree. This module restates the published
 Apache Parquet format for the subset the scan supports (flat schema,
-UNCOMPRESSED chunks, data pages v1, PLAIN and dictionary encodings, RLE /
+UNCOMPRESSED and SNAPPY chunks, data pages v1, PLAIN and dictionary encodings, RLE /
 bit-packed hybrid definition levels and indices) and MiniLake's own code on
 top of it:
 
@@ -157,6 +157,51 @@ class Column:
                                                                                                      "float64")
 
 
+def snappy_decompress(b: bytes) -> bytes:
+    """Snappy, from the format's published description: varint length, then
+    literals (tag & 3 == 0) and back-references with 1-, 2- or 4-byte offsets
+    (tag & 3 == 1, 2, 3), copied forward byte by byte."""
+    i, n, shift = 0, 0, 0
+    while True:
+        x = b[i]
+        i += 1
+        n |= (x & 0x7F) << shift
+        if not x & 0x80:
+            break
+        shift += 7
+    out = bytearray()
+    while i < len(b):
+        tag = b[i]
+        i += 1
+        kind = tag & 3
+        if kind == 0:
+            ln = tag >> 2
+            if ln >= 60:
+                nb = ln - 59
+                ln = int.from_bytes(b[i:i + nb], "little")
+                i += nb
+            ln += 1
+            out += b[i:i + ln]
+            i += ln
+            continue
+        if kind == 1:
+            ln, off = 4 + ((tag >> 2) & 7), ((tag >> 5) << 8) | b[i]
+            i += 1
+        elif kind == 2:
+            ln, off = (tag >> 2) + 1, int.from_bytes(b[i:i + 2], "little")
+            i += 2
+        else:
+            ln, off = (tag >> 2) + 1, int.from_bytes(b[i:i + 4], "little")
+            i += 4
+        if off == 0 or off > len(out):
+            raise ParquetError("snappy back-reference")
+        for _ in range(ln):
+            out.append(out[-off])
+    if len(out) != n:
+        raise ParquetError("snappy length")
+    return bytes(out)
+
+
 def _hybrid(b: bytes, pos: int, end: int, bw: int, want: int) -> List[int]:
     """RLE / bit-packed hybrid values (at most `want`)."""
     out: List[int] = []
@@ -240,19 +285,28 @@ class ParquetFile:
         col = self.columns[c]
         nrows, chunks = self.row_groups[rg]
         md = chunks[c]
-        if md.get(4, 0) != 0:
-            raise Unsupported("compressed column chunk")
+        codec = md.get(4, 0)
+        if codec not in (0, 1):
+            raise Unsupported(f"codec {codec}")
         start = md.get(11) if md.get(11) is not None and md.get(11) < md[9] else md[9]
-        b = self.data[start:start + md[7]]
+        raw = self.data[start:start + md[7]]
+        # the pages' bodies, uncompressed, one after another (headers dropped)
+        b, bodies, pos = b"", [], 0
+        while pos < len(raw):
+            t = _Thrift(raw, pos)
+            h = t.read(12)
+            body = raw[t.pos:t.pos + h[3]]
+            if codec == 1:
+                body = snappy_decompress(body)
+            bodies.append((h, len(b), len(b) + len(body)))
+            b += body
+            pos = t.pos + h[3]
         width = col.type_length if col.conv == "dec_bytes" else (8 if col.phys in (T_INT64, T_DOUBLE) else 4)
         dictionary = None
         out: list = []
-        pos = 0
-        while pos < len(b) and len(out) < nrows:
-            t = _Thrift(b, pos)
-            h = t.read(12)
-            body = t.pos
-            end = body + h[3]
+        for h, body, end in bodies:
+            if len(out) >= nrows:
+                break
             if h[1] == 2:
                 dictionary = self._plain(col, b, body, end, h[7][1], width)
             elif h[1] == 0:
@@ -277,7 +331,6 @@ class ParquetFile:
                 out.extend(next(it) if ok else None for ok in valid)
             elif h[1] == 3:
                 raise Unsupported("data page v2")
-            pos = end
         if len(out) != nrows:
             raise ParquetError(f"expected {nrows} rows, decoded {len(out)}")
         return out

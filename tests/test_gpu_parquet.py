@@ -2,8 +2,8 @@
 parquet_reader.rs:210-287) against the oracle's decoder on files written by
 pyarrow: every column type, NULLs, dictionary and PLAIN pages, many small
 pages, decimals as fixed-length bytes and as integers, REQUIRED columns,
-projections, a scan result as the source of an aggregate, and compressed
-chunks refused."""
+SNAPPY chunks, projections, a scan result as the source of an aggregate, and
+other codecs refused."""
 
 import numpy as np
 import pytest
@@ -54,9 +54,10 @@ def test_scan_result_feeds_an_aggregate(gpu_engine, oracle_engine, files):
     assert_tables_equal(got, want, rtol=1e-12)
 
 
-def test_compressed_chunk_refused(gpu_engine):
+@pytest.mark.parametrize("codec", ["ZSTD", "GZIP"])
+def test_other_codecs_refused(gpu_engine, codec):
     from minilake_amd.gpu import GpuError, ParquetTable
-    data = F.write(F.sample_table(500, seed=4), compression="SNAPPY")
+    data = F.write(F.sample_table(500, seed=4), compression=codec)
     t = ParquetTable(data, gpu_engine)
     with pytest.raises(GpuError) as e:
         t.read_row_group(0)

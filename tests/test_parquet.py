@@ -21,6 +21,7 @@ WRITES = {
     "plain": {"use_dictionary": False},
     "small_pages": {"data_page_size": 300},
     "decimals_as_integers": {"store_decimal_as_integer": True},
+    "snappy": {"compression": "SNAPPY"},
 }
 
 
@@ -114,3 +115,14 @@ def test_unsupported_files_are_refused():
     with pytest.raises(GpuError) as e:
         ParquetTable(b"PAR1" + b"\0" * 8)
     assert e.value.code == abi.ML_ERR_PARQUET
+
+
+def test_snappy_known_answers():
+    """Snappy: a hand-encoded stream (literal "abc", then a 1-byte-offset copy of
+    6 bytes at offset 3 that overlaps its own output) and pyarrow's Snappy
+    codec round trips over literal-heavy and repetitive data."""
+    assert R.snappy_decompress(b"\x09\x08abc\x09\x03") == b"abcabcabc"
+    rng = np.random.default_rng(2)
+    for data in (rng.integers(0, 256, 5000).astype(np.uint8).tobytes(), b"MiniLake " * 3000,
+                 bytes(70_000), rng.integers(0, 4, 100_000).astype(np.uint8).tobytes()):
+        assert R.snappy_decompress(pa.compress(data, codec="snappy", asbytes=True)) == data
