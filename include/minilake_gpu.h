@@ -192,7 +192,9 @@ void ml_ctx_memory_usage(const ml_ctx* ctx, int64_t* used, int64_t* peak, int64_
 int64_t ml_result_num_rows(const ml_result* r);
 int32_t ml_result_num_columns(const ml_result* r);
 /* Column i of the result; its data and validity bitmap are device memory,
- * so a result can be the source batch of another pipeline. */
+ * so a result can be the source batch of another pipeline. They are written
+ * on the context's stream: read them on that stream (ml_ctx_stream) or after
+ * ml_ctx_synchronize. */
 const ml_column* ml_result_column(const ml_result* r, int32_t i);
 /* Copy column i to `dst`, host or device memory (num_rows * width bytes). */
 ml_status ml_result_copy_column(ml_ctx* ctx, const ml_result* r, int32_t i, void* dst);
@@ -386,7 +388,9 @@ int32_t ml_join_table_unique(const ml_join_table* t);
 /* The build side's columns in table row order (JoinTable.batch, join.rs:36-45);
  * build row indices returned by ml_join_probe index these. Row order is
  * internal to the table: one unique integer key over a small range is stored
- * in key order (rows with a NULL key last). */
+ * in key order (rows with a NULL key last). ml_join_table_column waits for the
+ * context's stream first (the build's last kernels may still be queued), so
+ * the columns can then be read from any stream. */
 int32_t ml_join_table_num_columns(const ml_join_table* t);
 const ml_column* ml_join_table_column(const ml_join_table* t, int32_t i);
 void ml_join_table_free(ml_join_table* t);

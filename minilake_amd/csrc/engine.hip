@@ -2365,8 +2365,9 @@ static void topn_reduce(ml_topn* T, Materialized& m) {
     // a full sort (sort_batches without or with a large limit, sort.rs:85-132)
     const int64_t N = sort_padded_entries(n);
     Buffer s1(ctx->arena, (size_t)N * W * 8), s2(ctx->arena, (size_t)N * W * 8);
+    Buffer split(ctx->arena, (size_t)sort_split_entries(n) * 8);
     const uint64_t* sorted = launch_sort_words(words.as<uint64_t>(), n, nw, s1.as<uint64_t>(), s2.as<uint64_t>(),
-                                               ctx->stream);
+                                               split.as<int64_t>(), ctx->stream);
     launch_topn_rows(sorted, nw, m_out, rows.as<int64_t>(), ctx->stream);
   } else {
     // repeated tile selection until one tile remains
@@ -3024,6 +3025,9 @@ int32_t ml_join_table_unique(const ml_join_table* t) { return resolved(t) ? t->i
 int32_t ml_join_table_num_columns(const ml_join_table* t) { return t ? (int32_t)t->impl.col_types.size() : 0; }
 const ml_column* ml_join_table_column(const ml_join_table* t, int32_t i) {
   if (!resolved(t) || i < 0 || (size_t)i >= t->impl.views.size()) return nullptr;
+  // the build's last kernels may still be queued: the columns are complete
+  // when this returns, so they can be read from any stream
+  if (hipStreamSynchronize(t->ctx->stream) != hipSuccess) return nullptr;
   return &t->impl.views[(size_t)i];
 }
 void ml_join_table_free(ml_join_table* t) { delete t; }

@@ -180,6 +180,8 @@ void launch_topn_rows(const uint64_t* words, int nw, int64_t n, int64_t* rows_ou
 // Full sort of n entries of nw + 1 words (see merge_pass_kernel): a and b
 // hold sort_padded_entries(n) entries each; returns the one with the result.
 int64_t sort_padded_entries(int64_t n);
-uint64_t* launch_sort_words(const uint64_t* words, int64_t n, int nw, uint64_t* a, uint64_t* b, hipStream_t s);
+int64_t sort_split_entries(int64_t n);  // i64 entries of the merge splitter array
+uint64_t* launch_sort_words(const uint64_t* words, int64_t n, int nw, uint64_t* a, uint64_t* b, int64_t* split,
+                            hipStream_t s);
 
 }  // namespace mlg

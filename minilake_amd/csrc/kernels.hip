@@ -2322,10 +2322,9 @@ __global__ void __launch_bounds__(256) topn_tile_kernel(const uint64_t* in, int6
   const int W = nw + 1;
   const int64_t tile0 = (int64_t)blockIdx.x * TOPN_TILE;
   const int cnt = (int)((n - tile0) < TOPN_TILE ? (n - tile0) : TOPN_TILE);
-  for (int e = threadIdx.x; e < TOPN_TILE; e += blockDim.x) {
-    for (int k = 0; k < W; k++)
-      sh[e * W + k] = e < cnt ? in[(uint64_t)(tile0 + e) * W + k] : ~0ull;
-  }
+  // the tile's entries are contiguous words: load them flat (coalesced), pad with +inf
+  for (int e = threadIdx.x; e < TOPN_TILE * W; e += blockDim.x)
+    sh[e] = e < cnt * W ? in[(uint64_t)tile0 * W + e] : ~0ull;
   __syncthreads();
   int span = 2;
   while (span < cnt) span <<= 1;
@@ -2348,12 +2347,10 @@ __global__ void __launch_bounds__(256) topn_tile_kernel(const uint64_t* in, int6
       __syncthreads();
     }
   }
+  // the first `keep` entries, flat (a short tile's missing entries are the +inf padding)
   const int m = keep < cnt ? keep : cnt;
-  for (int e = threadIdx.x; e < m; e += blockDim.x)
-    for (int k = 0; k < W; k++) out[((uint64_t)blockIdx.x * keep + e) * W + k] = sh[e * W + k];
-  // pad missing entries of a short tile with +inf keys
-  for (int e = m + threadIdx.x; e < keep; e += blockDim.x)
-    for (int k = 0; k < W; k++) out[((uint64_t)blockIdx.x * keep + e) * W + k] = ~0ull;
+  uint64_t* dst = out + (uint64_t)blockIdx.x * keep * W;
+  for (int e = threadIdx.x; e < keep * W; e += blockDim.x) dst[e] = e < m * W ? sh[e] : ~0ull;
 }
 
 // ---------------------------------------------------------- full sort
@@ -2382,10 +2379,22 @@ __device__ __forceinline__ int64_t merge_corank(const uint64_t* A, int64_t na, c
   return lo;
 }
 
+// Where every workgroup's first output splits between its two runs: one
+// thread per workgroup, all searches in parallel ahead of the merge.
+__global__ void merge_split_kernel(const uint64_t* in, int64_t N, int64_t L, int W, int64_t nblocks, int64_t* split) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  const int64_t o0 = b * MERGE_OUT;
+  const int64_t ps = o0 / (2 * L) * (2 * L);
+  const int64_t na = min(L, N - ps);
+  const int64_t nb = max((int64_t)0, min(L, N - ps - L));
+  const uint64_t* A = in + ps * W;
+  split[b] = merge_corank(A, na, A + na * W, nb, o0 - ps, W);
+}
+
 __global__ void __launch_bounds__(256) merge_pass_kernel(const uint64_t* in, uint64_t* out, int64_t N, int64_t L,
-                                                         int W) {
+                                                         int W, const int64_t* split) {
   extern __shared__ uint64_t sh[];
-  __shared__ int64_t split[2];
   const int64_t o0 = (int64_t)blockIdx.x * MERGE_OUT;
   const int64_t ps = o0 / (2 * L) * (2 * L);
   const int64_t na = min(L, N - ps);
@@ -2393,10 +2402,9 @@ __global__ void __launch_bounds__(256) merge_pass_kernel(const uint64_t* in, uin
   const uint64_t* A = in + ps * W;
   const uint64_t* B = A + na * W;
   const int64_t k0 = o0 - ps, k1 = min(k0 + MERGE_OUT, na + nb);
-  if (threadIdx.x == 0) split[0] = merge_corank(A, na, B, nb, k0, W);
-  if (threadIdx.x == 1) split[1] = merge_corank(A, na, B, nb, k1, W);
-  __syncthreads();
-  const int64_t i0 = split[0], i1 = split[1];
+  // a block that ends its pair of runs ends with all of A; otherwise the
+  // next block (same pair: MERGE_OUT divides 2L) starts where this one ends
+  const int64_t i0 = split[blockIdx.x], i1 = k1 == na + nb ? na : split[blockIdx.x + 1];
   const int64_t j0 = k0 - i0, j1 = k1 - i1;
   const int ca = (int)(i1 - i0), cb = (int)(j1 - j0);
   uint64_t* sA = sh;
@@ -2404,16 +2412,25 @@ __global__ void __launch_bounds__(256) merge_pass_kernel(const uint64_t* in, uin
   for (int e = threadIdx.x; e < ca * W; e += blockDim.x) sA[e] = A[i0 * W + e];
   for (int e = threadIdx.x; e < cb * W; e += blockDim.x) sB[e] = B[j0 * W + e];
   __syncthreads();
+  // each thread merges PER outputs, noting which staged entry each one is;
+  // then the workgroup writes the outputs out in order, coalesced
+  __shared__ uint16_t pos[MERGE_OUT];
   constexpr int PER = MERGE_OUT / 256;
+  const int cnt = ca + cb;
   const int t0 = (int)threadIdx.x * PER;
-  if (t0 >= ca + cb) return;
-  int ia = (int)merge_corank(sA, ca, sB, cb, t0, W);
-  int ib = t0 - ia;
-  uint64_t* dst = out + (o0 + t0) * W;
-  for (int e = 0; e < PER && t0 + e < ca + cb; e++) {
-    const bool takeA = ib >= cb || (ia < ca && words_le(sA + (int64_t)ia * W, sB + (int64_t)ib * W, W));
-    const uint64_t* src = takeA ? sA + (int64_t)(ia++) * W : sB + (int64_t)(ib++) * W;
-    for (int k = 0; k < W; k++) dst[(int64_t)e * W + k] = src[k];
+  if (t0 < cnt) {
+    int ia = (int)merge_corank(sA, ca, sB, cb, t0, W);
+    int ib = t0 - ia;
+    for (int e = 0; e < PER && t0 + e < cnt; e++) {
+      const bool takeA = ib >= cb || (ia < ca && words_le(sA + (int64_t)ia * W, sB + (int64_t)ib * W, W));
+      pos[t0 + e] = (uint16_t)(takeA ? ia++ : ca + ib++);
+    }
+  }
+  __syncthreads();
+  uint64_t* dst = out + o0 * W;
+  for (int e = threadIdx.x; e < cnt * W; e += blockDim.x) {
+    const int r = e / W;
+    dst[e] = sh[(int)pos[r] * W + (e - r * W)];
   }
 }
 
@@ -2740,15 +2757,20 @@ void launch_topn_tiles(const uint64_t* in, int64_t n, int nw, int keep, uint64_t
 
 int64_t sort_padded_entries(int64_t n) { return std::max<int64_t>(1, (n + TOPN_TILE - 1) / TOPN_TILE) * TOPN_TILE; }
 
-uint64_t* launch_sort_words(const uint64_t* words, int64_t n, int nw, uint64_t* a, uint64_t* b, hipStream_t s) {
+int64_t sort_split_entries(int64_t n) { return sort_padded_entries(n) / MERGE_OUT + 1; }
+
+uint64_t* launch_sort_words(const uint64_t* words, int64_t n, int nw, uint64_t* a, uint64_t* b, int64_t* split,
+                            hipStream_t s) {
   const int W = nw + 1;
   const int64_t N = sort_padded_entries(n);
+  const int64_t nblocks = N / MERGE_OUT;
   // sorted tiles (the entries past n are all-ones words, which sort last)
   launch_topn_tiles(words, n, nw, TOPN_TILE, a, s);
   uint64_t* cur = a;
   uint64_t* nxt = b;
   for (int64_t L = TOPN_TILE; L < N; L *= 2) {
-    merge_pass_kernel<<<(unsigned)(N / MERGE_OUT), 256, (size_t)MERGE_OUT * W * 8, s>>>(cur, nxt, N, L, W);
+    merge_split_kernel<<<(unsigned)((nblocks + 255) / 256), 256, 0, s>>>(cur, N, L, W, nblocks, split);
+    merge_pass_kernel<<<(unsigned)nblocks, 256, (size_t)MERGE_OUT * W * 8, s>>>(cur, nxt, N, L, W, split);
     std::swap(cur, nxt);
   }
   return cur;
