@@ -377,6 +377,11 @@ ml_status ml_join_build_sink(ml_join_build* b, const ml_batch* batch);
 /* JoinBuildSink::finalize + build_table (join.rs:122-203). The table keeps every
  * build column; NULL keys never match. Consumes the builder. */
 ml_status ml_join_build_finalize(ml_join_build* b, ml_join_table** out);
+/* Finalize n builds of one context together (each as ml_join_build_finalize;
+ * every builder is consumed). Their layout decisions share one read of the
+ * device, so independent build sides cost one host round trip. On failure
+ * every out[i] is NULL. */
+ml_status ml_join_build_finalize_many(ml_join_build** builds, int32_t n, ml_join_table** out);
 int64_t ml_join_table_num_rows(const ml_join_table* t);
 /* 1 if every build key is distinct (each probe row matches at most once). */
 int32_t ml_join_table_unique(const ml_join_table* t);

@@ -2001,7 +2001,91 @@ void join_table_resolve(ml_join_table* t) {
   T.orig_valid.clear();
 }
 
-static ml_join_table* join_build_finalize(ml_join_build* J) {
+// A join build's finalize runs in two parts around one read of the device:
+// the first enqueues what the host must know before it chooses the layout
+// (a small table's whole dense build and its status, or a large table's row
+// count and first-key range) with a copy into a pinned slot; the second reads
+// the slot and finishes. Several builds can share one synchronization.
+struct FinalizePrep {
+  bool small = false;   // the one-workgroup dense build ran; its status is in the slot
+  bool ranged = false;  // key_minmax ran; count and range are in the slot
+  bool batched = false; // other builds' kernels run between the two parts
+  int64_t slot[4] = {0, -1, 0, 2};
+  DensePermute M{};
+  std::vector<Buffer> cols, valid;
+  Buffer dkeys, bits, rank, status, partials, mm;
+};
+
+static bool finalize_keys32(const std::vector<int>& key_types, uint32_t* sx) {
+  bool keys32 = key_types.size() > 1;
+  *sx = 0;
+  for (size_t k = 1; k < key_types.size(); k++) {
+    const int kt = key_types[k];
+    keys32 = keys32 && (kt == ML_TYPE_INT32 || kt == ML_TYPE_DATE || kt == ML_TYPE_UTF8);
+    if (kt == ML_TYPE_INT32 || kt == ML_TYPE_DATE) *sx |= 1u << k;
+  }
+  return keys32;
+}
+
+// First part: enqueue the build's device read into `pinned` (4 x i64).
+static void join_finalize_enqueue(ml_join_build* J, FinalizePrep& P, int64_t* pinned) {
+  ml_ctx* ctx = J->ctx;
+  const size_t nk = J->key_types.size();
+  uint32_t* flags = (uint32_t*)ctx->counters.as<uint64_t>();
+  const int kt0 = J->key_types[0];
+  const bool dense_type = kt0 == ML_TYPE_INT32 || kt0 == ML_TYPE_INT64 || kt0 == ML_TYPE_DATE;
+  uint32_t keys_sx;
+  const bool keys32 = finalize_keys32(J->key_types, &keys_sx);
+  const size_t key_word_bytes = keys32 ? 4 : 8;
+  if (J->chunks.size() == 1 && J->chunks[0].rows < 0 && dense_type && (uint64_t)J->chunks[0].cap <= SMALL_BUILD_ROWS) {
+    // small table: the whole dense build in one workgroup, one read of its status
+    auto& c = J->chunks[0];
+    const int64_t cap = std::max<int64_t>(c.cap, 1);
+    DensePermute& M = P.M;
+    if (nk > 1) P.dkeys = Buffer(ctx->arena, (size_t)cap * (nk - 1) * key_word_bytes);
+    M.nkw = (int32_t)nk;
+    M.keys32 = keys32 ? 1 : 0;
+    M.dst_keys = P.dkeys.as<uint64_t>();
+    M.ncols = (int32_t)c.cols.size();
+    for (size_t ci = 0; ci < c.cols.size(); ci++) {
+      const int w = type_width(J->col_types[ci]);
+      P.cols.emplace_back(ctx->arena, (size_t)cap * w);
+      P.valid.emplace_back();
+      const bool has_valid = ci < c.valid.size() && c.valid[ci].ptr;
+      if (has_valid) P.valid.back() = Buffer(ctx->arena, (size_t)cap);
+      M.width[ci] = w;
+      M.src[ci] = c.cols[ci].ptr;
+      M.dst[ci] = P.cols.back().ptr;
+      M.src_valid[ci] = has_valid ? c.valid[ci].as<uint8_t>() : nullptr;
+      M.dst_valid[ci] = has_valid ? P.valid.back().as<uint8_t>() : nullptr;
+    }
+    const int words = small_dense_words();
+    P.bits = Buffer(ctx->arena, (size_t)words * 8);
+    P.rank = Buffer(ctx->arena, (size_t)(words + 1) * 4);
+    P.status = Buffer(ctx->arena, 32);
+    launch_small_dense(M, c.keys.as<uint64_t>(), c.key_null.as<uint8_t>(), c.cap, c.count_dev, P.bits.as<uint64_t>(),
+                       P.rank.as<uint32_t>(), P.status.as<int64_t>(), ctx->stream);
+    hip_check(hipMemcpyAsync(pinned, P.status.ptr, 32, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
+    P.small = true;
+    return;
+  }
+  if (J->chunks.size() == 1 && J->chunks[0].rows < 0 && dense_type && (uint64_t)J->chunks[0].cap < 0xFFFFFFFFull) {
+    auto& c = J->chunks[0];
+    // at most 256 workgroups: each ends with one atomic on a shared counter
+    const int nparts = (int)std::min<int64_t>(256, std::max<int64_t>(1, (c.cap + 4095) / 4096));
+    P.partials = Buffer(ctx->arena, (size_t)nparts * 16);
+    P.mm = Buffer(ctx->arena, 32);
+    // (also clears flags[0] for the presence marking and flags[1] for the permutation)
+    launch_key_minmax(c.keys.as<uint64_t>(), (int)nk, c.key_null.as<uint8_t>(), c.cap, c.count_dev,
+                      P.partials.as<int64_t>(), nparts, P.mm.as<int64_t>(),
+                      (uint32_t*)(ctx->counters.as<uint64_t>() + ml_ctx::DONE_WORD), flags, ctx->stream);
+    hip_check(hipMemcpyAsync(pinned, P.mm.ptr, 24, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
+    P.ranged = true;
+  }
+}
+
+// Second part, after the stream has passed the copy and P.slot holds it.
+static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
   ml_ctx* ctx = J->ctx;
   auto* t = new ml_join_table();
   t->ctx = ctx;
@@ -2015,69 +2099,34 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
   uint32_t* flags = (uint32_t*)ctx->counters.as<uint64_t>();
   const int kt0 = T.key_types[0];
   const bool dense_type = kt0 == ML_TYPE_INT32 || kt0 == ML_TYPE_INT64 || kt0 == ML_TYPE_DATE;
-  // further key words fit 32 bits when every further key column is 4 bytes wide
-  bool keys32 = nk > 1;
-  uint32_t keys_sx = 0;
-  for (size_t k = 1; k < nk; k++) {
-    const int kt = T.key_types[k];
-    keys32 = keys32 && (kt == ML_TYPE_INT32 || kt == ML_TYPE_DATE || kt == ML_TYPE_UTF8);
-    if (kt == ML_TYPE_INT32 || kt == ML_TYPE_DATE) keys_sx |= 1u << k;
-  }
+  uint32_t keys_sx;
+  const bool keys32 = finalize_keys32(T.key_types, &keys_sx);
   const size_t key_word_bytes = keys32 ? 4 : 8;
-  // Row counts still on the device. One chunk with a dense-layout key type:
-  // its count comes back with the key range (one read, below).
+  // the later dense steps use flags[0..1] from zero (key_minmax cleared them,
+  // unless other builds ran in between)
+  if (P.batched) launch_fill_u64(ctx->counters.as<uint64_t>(), 0, 1, ctx->stream);
   int64_t hm[3] = {0, -1, 0};
   bool have_range = false;
-  Buffer partials, mm;
   auto key_range = [&](const uint64_t* kw, const uint8_t* kn, int64_t rows_cap, const uint64_t* count_dev) {
-    // at most 256 workgroups: each ends with one atomic on a shared counter
     const int nparts = (int)std::min<int64_t>(256, std::max<int64_t>(1, (rows_cap + 4095) / 4096));
-    partials = Buffer(ctx->arena, (size_t)nparts * 16);
-    mm = Buffer(ctx->arena, 32);
-    // (also clears flags[0] for the presence marking and flags[1] for the permutation)
-    launch_key_minmax(kw, (int)nk, kn, rows_cap, count_dev, partials.as<int64_t>(), nparts, mm.as<int64_t>(),
+    P.partials = Buffer(ctx->arena, (size_t)nparts * 16);
+    P.mm = Buffer(ctx->arena, 32);
+    launch_key_minmax(kw, (int)nk, kn, rows_cap, count_dev, P.partials.as<int64_t>(), nparts, P.mm.as<int64_t>(),
                       (uint32_t*)(ctx->counters.as<uint64_t>() + ml_ctx::DONE_WORD), flags, ctx->stream);
-    d2h(ctx, hm, mm.ptr, 24);
+    d2h(ctx, hm, P.mm.ptr, 24);
     have_range = true;
   };
-  if (J->chunks.size() == 1 && J->chunks[0].rows < 0 && dense_type && (uint64_t)J->chunks[0].cap <= SMALL_BUILD_ROWS) {
-    // small table: the whole dense build in one workgroup, one read of its status
+  if (P.small) {
     auto& c = J->chunks[0];
-    const int64_t cap = std::max<int64_t>(c.cap, 1);
-    DensePermute M{};
-    std::vector<Buffer> cols, valid;
-    Buffer dkeys;
-    if (nk > 1) dkeys = Buffer(ctx->arena, (size_t)cap * (nk - 1) * key_word_bytes);
-    M.nkw = (int32_t)nk;
-    M.keys32 = keys32 ? 1 : 0;
-    M.dst_keys = dkeys.as<uint64_t>();
-    M.ncols = (int32_t)c.cols.size();
-    for (size_t ci = 0; ci < c.cols.size(); ci++) {
-      const int w = type_width(T.col_types[ci]);
-      cols.emplace_back(ctx->arena, (size_t)cap * w);
-      valid.emplace_back();
-      const bool has_valid = ci < c.valid.size() && c.valid[ci].ptr;
-      if (has_valid) valid.back() = Buffer(ctx->arena, (size_t)cap);
-      M.width[ci] = w;
-      M.src[ci] = c.cols[ci].ptr;
-      M.dst[ci] = cols.back().ptr;
-      M.src_valid[ci] = has_valid ? c.valid[ci].as<uint8_t>() : nullptr;
-      M.dst_valid[ci] = has_valid ? valid.back().as<uint8_t>() : nullptr;
-    }
-    const int words = small_dense_words();
-    Buffer bits(ctx->arena, (size_t)words * 8), rank(ctx->arena, (size_t)(words + 1) * 4), status(ctx->arena, 32);
-    launch_small_dense(M, c.keys.as<uint64_t>(), c.key_null.as<uint8_t>(), c.cap, c.count_dev, bits.as<uint64_t>(),
-                       rank.as<uint32_t>(), status.as<int64_t>(), ctx->stream);
-    int64_t st[4] = {0, 0, 0, 2};
-    d2h(ctx, st, status.ptr, 32);
+    const int64_t* st = P.slot;
     c.rows = st[0];
     if (st[3] == 0) {
       T.nrows = st[0];
-      T.cols = std::move(cols);
-      T.col_valid = std::move(valid);
-      T.dense_bits = std::move(bits);
-      T.dense_rank = std::move(rank);
-      T.dense_keys = std::move(dkeys);
+      T.cols = std::move(P.cols);
+      T.col_valid = std::move(P.valid);
+      T.dense_bits = std::move(P.bits);
+      T.dense_rank = std::move(P.rank);
+      T.dense_keys = std::move(P.dkeys);
       T.dev.nkw = (int)nk;
       T.dev.dense = 1;
       T.dev.dense_min = st[1];
@@ -2092,11 +2141,10 @@ static ml_join_table* join_build_finalize(ml_join_build* J) {
       refresh_views(T);
       return t;
     }
-  }
-  if (J->chunks.size() == 1 && J->chunks[0].rows < 0 && dense_type && (uint64_t)J->chunks[0].cap < 0xFFFFFFFFull) {
-    auto& c = J->chunks[0];
-    key_range(c.keys.as<uint64_t>(), c.key_null.as<uint8_t>(), c.cap, c.count_dev);
-    c.rows = hm[2];
+  } else if (P.ranged) {
+    std::memcpy(hm, P.slot, 24);
+    have_range = true;
+    J->chunks[0].rows = hm[2];
   }
   for (auto& c : J->chunks)
     if (c.rows < 0) d2h(ctx, &c.rows, c.count_dev, 8);
@@ -2902,27 +2950,68 @@ ml_status ml_pipeline_join_build(ml_ctx* ctx, const ml_batch* source, const ml_o
   });
 }
 
+// The table (rows, key structures, filters) held against the budget
+// (JoinBuildSink::finalize, join.rs:120-153).
+static void reserve_join_table(ml_ctx* ctx, ml_join_table* t) {
+  const JoinTableImpl& J = t->impl;
+  int64_t bytes = (int64_t)(J.slots.bytes + J.bloom.bytes + J.dense_bits.bytes + J.dense_rank.bytes +
+                            J.dense_keys.bytes + J.keys.bytes + J.key_null.bytes);
+  for (auto* v : {&J.cols, &J.col_valid, &J.orig_cols, &J.orig_valid})
+    for (const Buffer& x : *v) bytes += (int64_t)x.bytes;
+  if (!ctx->mem->try_grow(bytes))
+    fail(ML_ERR_RESOURCES_EXHAUSTED, "HashJoinBuild(table): " + std::to_string(bytes) +
+                                         " bytes do not fit the memory budget (" + std::to_string(ctx->mem->used) +
+                                         " of " + std::to_string(ctx->mem->limit) + " bytes reserved)");
+  t->pool = ctx->mem;
+  t->reserved = bytes;
+}
+
 ml_status ml_join_build_finalize(ml_join_build* b, ml_join_table** out) {
   ml_ctx* ctx = b->ctx;
   ml_status st = guarded(ctx, [&] {
     if (!b->schema_fixed) fail(ML_ERR_EXECUTION, "join build received no batch");
-    std::unique_ptr<ml_join_table> t(join_build_finalize(b));
-    // the table (rows, key structures, filters) held against the budget
-    // (JoinBuildSink::finalize, join.rs:120-153)
-    const JoinTableImpl& J = t->impl;
-    int64_t bytes = (int64_t)(J.slots.bytes + J.bloom.bytes + J.dense_bits.bytes + J.dense_rank.bytes +
-                              J.dense_keys.bytes + J.keys.bytes + J.key_null.bytes);
-    for (auto* v : {&J.cols, &J.col_valid, &J.orig_cols, &J.orig_valid})
-      for (const Buffer& x : *v) bytes += (int64_t)x.bytes;
-    if (!ctx->mem->try_grow(bytes))
-      fail(ML_ERR_RESOURCES_EXHAUSTED, "HashJoinBuild(table): " + std::to_string(bytes) + " bytes do not fit the memory budget (" +
-                                           std::to_string(ctx->mem->used) + " of " + std::to_string(ctx->mem->limit) +
-                                           " bytes reserved)");
-    t->pool = ctx->mem;
-    t->reserved = bytes;
+    FinalizePrep P;
+    join_finalize_enqueue(b, P, (int64_t*)ctx->pinned);
+    if (P.small || P.ranged) {
+      const uint64_t seq = ctx->check_seq;
+      hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+      ctx->synced_seq = seq;
+      std::memcpy(P.slot, ctx->pinned, 32);
+    }
+    std::unique_ptr<ml_join_table> t(join_finalize_finish(b, P));
+    reserve_join_table(ctx, t.get());
     *out = t.release();
   });
   delete b;
+  return st;
+}
+
+ml_status ml_join_build_finalize_many(ml_join_build** builds, int32_t n, ml_join_table** out) {
+  if (n <= 0) return ML_OK;
+  ml_ctx* ctx = builds[0]->ctx;
+  std::vector<std::unique_ptr<ml_join_table>> tables((size_t)n);
+  ml_status st = guarded(ctx, [&] {
+    if (n > (int32_t)(ml_ctx::PINNED_BYTES / 32)) fail(ML_ERR_PLAN, "too many join builds in one call");
+    std::vector<FinalizePrep> P((size_t)n);
+    for (int32_t i = 0; i < n; i++) {
+      if (builds[i]->ctx != ctx) fail(ML_ERR_PLAN, "join builds of one call must share a context");
+      if (!builds[i]->schema_fixed) fail(ML_ERR_EXECUTION, "join build received no batch");
+      P[(size_t)i].batched = n > 1;
+      join_finalize_enqueue(builds[i], P[(size_t)i], (int64_t*)ctx->pinned + 4 * i);
+    }
+    const uint64_t seq = ctx->check_seq;
+    hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+    ctx->synced_seq = seq;
+    for (int32_t i = 0; i < n; i++) std::memcpy(P[(size_t)i].slot, (int64_t*)ctx->pinned + 4 * i, 32);
+    for (int32_t i = 0; i < n; i++) {
+      tables[(size_t)i].reset(join_finalize_finish(builds[i], P[(size_t)i]));
+      reserve_join_table(ctx, tables[(size_t)i].get());
+    }
+  });
+  for (int32_t i = 0; i < n; i++) {
+    out[i] = st == ML_OK ? tables[(size_t)i].release() : nullptr;
+    delete builds[i];
+  }
   return st;
 }
 

@@ -48,7 +48,7 @@ EXPORTED = [
     "ml_pipeline_limit", "ml_mlb1_write", "ml_mlb1_read", "ml_parquet_open", "ml_parquet_num_columns",
     "ml_parquet_column_type", "ml_parquet_column_name", "ml_parquet_column_nullable", "ml_parquet_num_row_groups",
     "ml_parquet_row_group_rows", "ml_parquet_column_stats", "ml_parquet_prune", "ml_parquet_read_row_group",
-    "ml_parquet_free", "ml_ctx_set_memory_limit", "ml_ctx_memory_usage",
+    "ml_parquet_free", "ml_ctx_set_memory_limit", "ml_ctx_memory_usage", "ml_join_build_finalize_many",
     "ml_topn_create", "ml_topn_sink", "ml_topn_finalize", "ml_topn_free", "ml_ctx_enable_stats",
     "ml_ctx_take_stats", "ml_fragments_create", "ml_fragments_run", "ml_fragments_run_graph",
     "ml_fragments_run_per_fragment", "ml_fragments_autotune", "ml_fragments_free", "ml_mlp_create",
@@ -124,6 +124,7 @@ def lib():
             "ml_parquet_read_row_group": ([vp, i32, P(i32), i32, P(vp)], i32),
             "ml_parquet_free": ([vp], None),
             "ml_ctx_set_memory_limit": ([vp, i64, i32], i32),
+            "ml_join_build_finalize_many": ([vp, i32, vp], i32),
             "ml_ctx_memory_usage": ([vp, P(i64), P(i64), P(i64)], None),
             "ml_pipeline_collect_sized": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, i64, P(i32), P(vp)], i32),
             "ml_pipeline_count": ([vp, P(abi.MlBatch), P(abi.MlOperator), i32, P(i64)], i32),
@@ -522,6 +523,32 @@ class GpuEngine:
         t = C.c_void_p()
         self._check(L.ml_join_build_finalize(jb, C.byref(t)))
         return GpuJoin(t.value)
+
+    def join_build_many(self, specs) -> List[GpuJoin]:
+        """Independent join builds, (src, ops, keys, key_types) each, finalized
+        together (ml_join_build_finalize_many: one host round trip)."""
+        L = lib()
+        builders = []
+        try:
+            for src, ops, keys, key_types in specs:
+                keep = abi.Keep()
+                _, b = self._src(src, keep)
+                spec = abi.join_build_spec(keys, key_types, keep)
+                jb = C.c_void_p()
+                self._check(L.ml_join_build_create(self.ctx, C.byref(spec), C.byref(jb)))
+                builders.append(jb.value)
+                self._check(L.ml_pipeline_join_build(self.ctx, C.byref(b), self._ops(ops, keep), len(ops), jb))
+        except Exception:
+            for jb in builders:  # finalizing frees each builder
+                t = C.c_void_p()
+                L.ml_join_build_finalize(jb, C.byref(t))
+                if t.value:
+                    L.ml_join_table_free(t)
+            raise
+        arr = (C.c_void_p * len(builders))(*builders)
+        out = (C.c_void_p * len(builders))()
+        self._check(L.ml_join_build_finalize_many(arr, len(builders), out))
+        return [GpuJoin(out[i]) for i in range(len(builders))]
 
     def hashagg(self, src, ops, group_exprs, key_types, aggs, sizing=abi.SIZING_PLAIN, capacity_hint=0,
                 stats: Optional[dict] = None, mode: int = abi.AGG_MODE_SINGLE) -> GpuResult:

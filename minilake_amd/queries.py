@@ -69,16 +69,18 @@ def run_q3(engine, customer, orders, lineitem, limit=10, **agg_kw):
 
 def run_q5(engine, sj, **agg_kw):
     """Six-table star join GROUP BY n_name (q5.sql:1-24); c_nationkey = s_nationkey is a join key."""
-    cust = engine.join_build(sj.customer, [], [col(0)], [INT64])
+    # the four independent build sides together, then orders (which probes customer)
+    cust, supp, nation, region = engine.join_build_many([
+        (sj.customer, [], [col(0)], [INT64]),
+        (sj.supplier, [], [col(0), col(1)], [INT64, INT32]),
+        (sj.nation, [], [col(0)], [INT32]),
+        (sj.region, [filter_op(col(1).eq(lit("ASIA")))], [col(0)], [INT32])])
     # orders -> probe(customer): [c_custkey, c_nationkey, o_orderkey, o_custkey, o_orderdate]
     ords = engine.join_build(sj.orders,
                              [filter_op(col(2).ge(date_lit(W.D_1994_01_01)) & col(2).lt(date_lit(W.D_1995_01_01))),
                               probe_op(cust, [col(1)], build_is_left=True),
                               project_op([col(1), col(2)])],
                              [col(1)], [INT64])
-    supp = engine.join_build(sj.supplier, [], [col(0), col(1)], [INT64, INT32])
-    nation = engine.join_build(sj.nation, [], [col(0)], [INT32])
-    region = engine.join_build(sj.region, [filter_op(col(1).eq(lit("ASIA")))], [col(0)], [INT32])
     # lineitem [l_orderkey, l_suppkey, price, disc]
     #  -> probe(ords, build left)  : [c_nationkey, o_orderkey, l_orderkey, l_suppkey, price, disc]
     #  -> probe(supp)              : [..6.., s_suppkey, s_nationkey]

@@ -183,6 +183,9 @@ class OracleEngine:
                                              C.byref(self.opts), C.byref(out)))
         return OracleJoin(out.value)
 
+    def join_build_many(self, specs):
+        return [self.join_build(*s) for s in specs]
+
     def hashagg(self, src, ops, group_exprs, key_types, aggs, sizing=abi.SIZING_PLAIN, capacity_hint=0,
                 mode=abi.AGG_MODE_SINGLE):
         keep = abi.Keep()

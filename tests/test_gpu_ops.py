@@ -126,6 +126,32 @@ def test_join_probe_with_duplicate_build_keys(gpu_engine, oracle_engine):
     assert (jg.column(0)[bg] == probe.columns[0].values[pg]).all()
 
 
+def test_join_build_many_matches_oracle(gpu_engine, oracle_engine):
+    """Builds finalized together (one device read for all layout decisions):
+    a large dense table, a small one (one-workgroup build), one with duplicate
+    keys (hashed layout) and a two-key table; every table probes like the
+    oracle's, pairs compared as (probe row, build payload)."""
+    rng = np.random.default_rng(8)
+    big = HostTable([HostColumn(INT64, rng.permutation(200_000).astype(np.int64)), HostColumn(FLOAT64, rng.random(200_000))])
+    small = HostTable([HostColumn(INT32, np.arange(50, dtype=np.int32)), HostColumn(FLOAT64, rng.random(50))])
+    dup = HostTable([HostColumn(INT64, rng.integers(0, 5000, 20_000)), HostColumn(FLOAT64, rng.random(20_000))])
+    two = HostTable([HostColumn(INT64, rng.permutation(100_000).astype(np.int64)),
+                     HostColumn(FLOAT64, rng.random(100_000)), HostColumn(INT32, rng.integers(0, 25, 100_000).astype(np.int32))])
+    specs = [(big, [], [col(0)], [INT64]), (small, [], [col(0)], [INT32]), (dup, [], [col(0)], [INT64]),
+             (two, [], [col(0), col(2)], [INT64, INT32])]
+    tables = gpu_engine.join_build_many(specs)
+    probe = HostTable([HostColumn(INT64, rng.integers(-10, 210_000, 30_000)),
+                       HostColumn(INT32, rng.integers(0, 25, 30_000).astype(np.int32))])
+    for (src, ops, keys, kt), jg in zip(specs, tables):
+        jo = oracle_engine.join_build(src, ops, keys, kt)
+        pk = [col(1)] if kt[0] == INT32 else ([col(0)] if len(kt) == 1 else [col(0), col(1)])
+        po, bo = oracle_engine.join_probe(jo, pk, probe)
+        pg, bg = gpu_engine.join_probe(jg, pk, probe)
+        want = sorted(zip(po.tolist(), src.columns[1].values[bo].tolist()))
+        assert sorted(zip(pg.tolist(), jg.column(1)[bg].tolist())) == want and len(want) > 0
+    assert not tables[2].unique() and tables[0].unique()
+
+
 def test_expanding_probe_into_aggregate(gpu_engine, oracle_engine):
     rng = np.random.default_rng(9)
     build = HostTable([HostColumn(INT64, rng.integers(0, 50, 400)), HostColumn(FLOAT64, rng.random(400))])
