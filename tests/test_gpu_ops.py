@@ -126,11 +126,15 @@ def test_join_probe_with_duplicate_build_keys(gpu_engine, oracle_engine):
     assert (jg.column(0)[bg] == probe.columns[0].values[pg]).all()
 
 
-def test_join_build_many_matches_oracle(gpu_engine, oracle_engine):
+def test_join_build_many_matches_oracle(gpu_engine):
     """Builds finalized together (one device read for all layout decisions):
     a large dense table, a small one (one-workgroup build), one with duplicate
     keys (hashed layout) and a two-key table; every table probes like the
-    oracle's, pairs compared as (probe row, build payload)."""
+    oracle's, pairs compared as (probe row, build payload). The oracle runs one
+    thread so its build rows keep source order (with several threads the
+    morsels of a multi-morsel build are concatenated in completion order)."""
+    from oracle.oracle import OracleEngine
+    oracle_engine = OracleEngine(threads=1)
     rng = np.random.default_rng(8)
     big = HostTable([HostColumn(INT64, rng.permutation(200_000).astype(np.int64)), HostColumn(FLOAT64, rng.random(200_000))])
     small = HostTable([HostColumn(INT32, np.arange(50, dtype=np.int32)), HostColumn(FLOAT64, rng.random(50))])
