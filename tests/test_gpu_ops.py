@@ -538,7 +538,7 @@ def test_mlb1_bytes_match_oracle(gpu_engine, oracle_engine):
     assert gpu_engine.mlb1_read(b"") == (None, 0)
 
 
-@pytest.mark.parametrize("mode", [abi.AGG_MODE_SINGLE, abi.AGG_MODE_PARTIAL])
+@pytest.mark.parametrize("mode", [abi.AGG_MODE_SINGLE, abi.AGG_MODE_PARTIAL, abi.AGG_MODE_FINAL])
 def test_memory_budget_and_spill(oracle_engine, mode):
     """MemoryPool + spill (memory.rs:22-157, aggregate/hash.rs:189-282, 369-425):
     a GROUP BY of ~150K groups (an i64 and a dictionary string key) under a
@@ -557,6 +557,10 @@ def test_memory_budget_and_spill(oracle_engine, mode):
     aggs = [Agg(AGG_COUNT_STAR), Agg(AGG_SUM, col(2), FLOAT64), Agg(AGG_AVG, col(3), INT32),
             Agg(AGG_MIN, col(4), DATE), Agg(AGG_MAX, col(2) / col(3), FLOAT64), Agg(AGG_SUM, col(3), INT32)]
     keys, kt = [col(0), col(1)], [INT64, UTF8]
+    if mode == abi.AGG_MODE_FINAL:  # the input is the partial states of three slices
+        cuts = [0, 200_000, 400_000, n]
+        t = concat_tables([oracle_engine.hashagg(t.take(np.arange(a, b)), [], keys, kt, aggs,
+                                                 mode=abi.AGG_MODE_PARTIAL).to_host() for a, b in zip(cuts, cuts[1:])])
     want = oracle_engine.hashagg(t, [], keys, kt, aggs, mode=mode).to_host()
     assert want.num_rows > 100_000
     eng = GpuEngine(batch_rows=50_000)
