@@ -1379,21 +1379,29 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
     B.P.agg_fill_limit = fill_limit;
     for (size_t s = 0; s < A->slots.size(); s++) B.P.acc_data[s] = A->acc[s].ptr;
     FlowLaunch F;
-    // accumulator cache: as many entries (a power of two, <= 1024) as fit 16 KB
+    // accumulator cache: as many entries (a power of two) as fit its budget,
+    // 60 KB when the pipeline has no probes (nothing else is in its LDS, so a
+    // few thousand hot groups stay on chip), else 16 KB
+    bool probes = false;
+    for (int st = 0; st < B.P.nsteps; st++) probes = probes || B.P.steps[st].kind == STEP_PROBE;
     const int per = 4 + 8 * (int)A->slots.size();
-    int acn = 1024;
-    while (acn > 32 && acn * per > 16384) acn >>= 1;
-    const bool cache = acn * per <= 16384 && (B.P.agg_mask + 1) <= 0xFFFFFFFEull;
+    const bool cacheable = (B.P.agg_mask + 1) <= 0xFFFFFFFEull;
     B.P.ac_n = 0;
     // the cache sits after the queues (the kernel keeps no cells for this sink);
     // its bytes are counted in the launch's LDS so the grid matches occupancy
-    bool flow = cache && plan_flow(B, SINK_HASH_AGG, sink_read_stages(B), (size_t)acn * per, F);
-    if (flow) {
-      B.P.ac_n = acn;
-      B.P.ac_off = 2 * B.P.qwords;  // u64 index past 4 waves x qwords u32
-    } else {
-      flow = plan_flow(B, SINK_HASH_AGG, sink_read_stages(B), 0, F);
+    bool flow = false;
+    for (const int budget : {probes ? 16384 : 61440, 16384}) {
+      int acn = 4096;
+      while (acn > 32 && acn * per > budget) acn >>= 1;
+      if (!cacheable || acn * per > budget) break;
+      if (plan_flow(B, SINK_HASH_AGG, sink_read_stages(B), (size_t)acn * per, F)) {
+        flow = true;
+        B.P.ac_n = acn;
+        B.P.ac_off = 2 * B.P.qwords;  // u64 index past 4 waves x qwords u32
+        break;
+      }
     }
+    if (!flow) flow = plan_flow(B, SINK_HASH_AGG, sink_read_stages(B), 0, F);
     if (flow) ctx->launch(B.P, F.grid, FLOW_BLOCK, F.lds);
     else ctx->launch(B.P, B.grid(), block_of(ctx), 0);
     A->len_known = false;

@@ -570,14 +570,24 @@ __device__ __forceinline__ void sink_hash_agg_cached(const Plan& P, const Code& 
     return;
   }
   uint32_t* tags = (uint32_t*)(ac + (uint64_t)P.ac_n * P.nacc);
-  const uint32_t ci = (uint32_t)slot & (uint32_t)(P.ac_n - 1);
+  // a group's cache entry: the first of AC_PROBES entries from slot mod ac_n
+  // that holds it or is free (claimed with a CAS); otherwise the global table
+  constexpr int AC_PROBES = 4;
   const uint32_t want = (uint32_t)slot + 1u;
-  uint32_t tag = tags[ci];
-  if (tag == 0) {
-    const uint32_t prev = atomicCAS(&tags[ci], 0u, want);
-    tag = prev == 0 ? want : prev;
+  uint32_t ci = (uint32_t)slot & (uint32_t)(P.ac_n - 1);
+  bool cached = false;
+  for (int pr = 0; pr < AC_PROBES; pr++) {
+    uint32_t tag = tags[ci];
+    if (tag == 0) {
+      const uint32_t prev = atomicCAS(&tags[ci], 0u, want);
+      tag = prev == 0 ? want : prev;
+    }
+    if (tag == want) {
+      cached = true;
+      break;
+    }
+    ci = (ci + 1u) & (uint32_t)(P.ac_n - 1);
   }
-  const bool cached = tag == want;
   for (int a = 0; a < P.nargs; a++) {
     uint64_t v;
     const bool ok = eval(ins, P.args[a], rows, v);
