@@ -189,6 +189,19 @@ _TORCH_DT = {abi.BOOLEAN: "uint8", abi.INT32: "int32", abi.INT64: "int64", abi.F
              abi.DATE: "int32", abi.UTF8: "int32"}
 
 
+class _Selection:
+    """A device selection vector: pointer and length (an empty one keeps its pointer)."""
+
+    def __init__(self, ptr: int, n: int):
+        self.ptr, self.n = ptr, n
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def numel(self) -> int:
+        return self.n
+
+
 class DeviceTable:
     """A HostTable uploaded to HBM (one tensor per column, dictionaries kept on the host)."""
 
@@ -497,9 +510,11 @@ class GpuEngine:
         src = self.table(src)
         sel = None
         if selection is not None:
-            sel = self.torch.from_numpy(np.ascontiguousarray(selection, dtype=np.uint32).view(np.int32)).to(
-                f"cuda:{self.device}")
-            keep(sel)
+            arr = np.ascontiguousarray(selection, dtype=np.uint32).view(np.int32)
+            # an empty selection still needs a non-NULL pointer (NULL means "no selection")
+            dev = self.torch.from_numpy(arr if len(arr) else np.zeros(1, np.int32)).to(f"cuda:{self.device}")
+            keep(dev)
+            sel = _Selection(dev.data_ptr(), len(arr))
         return src, src.ml_batch(keep, sel)
 
     def _ops(self, ops, keep):

@@ -513,12 +513,13 @@ def test_mlb1_bytes_match_oracle(gpu_engine, oracle_engine):
     """MLB1 batch bytes (ipc.rs:44-170): the device writer's bytes equal the
     oracle's for every column type with NULLs and a dictionary column, with
     and without a selection (one keeping no NULL drops that validity section),
-    and for a device result as the source; reading them back on the device
-    gives the rows; a cut batch is an I/O error, an empty stream a clean end."""
+    and for a device result as the source (an empty selection: a batch of no
+    rows); reading them back on the device gives the rows; a cut batch is an
+    I/O error, an empty stream a clean end."""
     from minilake_amd.gpu import GpuError
     t = _with_nulls(mixed_table(50_000, seed=51), (0, 2, 3, 5), seed=51)
     no_null_rows = np.nonzero(t.columns[0].validity)[0].astype(np.uint32)
-    for sel in (None, np.arange(0, t.num_rows, 3, dtype=np.uint32), no_null_rows):
+    for sel in (None, np.arange(0, t.num_rows, 3, dtype=np.uint32), no_null_rows, np.array([], np.uint32)):
         want = oracle_engine.mlb1_write(t, sel)
         got = gpu_engine.mlb1_write(t, sel)
         assert got == want
