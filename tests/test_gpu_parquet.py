@@ -24,7 +24,7 @@ def files():
     return _files()
 
 
-@pytest.mark.parametrize("kind", list(WRITES) + ["required"])
+@pytest.mark.parametrize("kind", list(WRITES) + ["required", "edge"])
 def test_device_scan_matches_oracle(gpu_engine, files, kind):
     from minilake_amd.gpu import ParquetTable
     data = files[kind]

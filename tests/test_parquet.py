@@ -22,6 +22,7 @@ WRITES = {
     "small_pages": {"data_page_size": 300},
     "decimals_as_integers": {"store_decimal_as_integer": True},
     "snappy": {"compression": "SNAPPY"},
+    "dictionary_fallback": {"dictionary_pagesize_limit": 512},  # later pages PLAIN
 }
 
 
@@ -32,6 +33,12 @@ def _files():
                    schema=pa.schema([pa.field("k", pa.int64(), nullable=False),
                                      pa.field("s", pa.string(), nullable=False)]))
     out["required"] = F.write(req)
+    n = 3000
+    edge = pa.table({"nulls": pa.array([None] * n, pa.int64()),
+                     "empty": pa.array(["" if i % 3 else None for i in range(n)], pa.string()),
+                     "sparse": pa.array([float(i) if i % 97 == 0 else None for i in range(n)], pa.float64()),
+                     "flag": pa.array([None if i % 5 else bool(i % 2) for i in range(n)], pa.bool_())})
+    out["edge"] = F.write(edge, data_page_size=256)
     return out
 
 
@@ -40,7 +47,7 @@ def files():
     return _files()
 
 
-@pytest.mark.parametrize("kind", list(WRITES) + ["required"])
+@pytest.mark.parametrize("kind", list(WRITES) + ["required", "edge"])
 def test_oracle_decodes_like_pyarrow(files, kind):
     data = files[kind]
     f = R.ParquetFile(data)
@@ -76,7 +83,7 @@ def test_prune_known_answers():
         assert t.prune([(0, op, v)]) == ([0] if keep else []), (op, v)
 
 
-@pytest.mark.parametrize("kind", list(WRITES) + ["required"])
+@pytest.mark.parametrize("kind", list(WRITES) + ["required", "edge"])
 def test_library_metadata_stats_and_prune_match_oracle(files, kind):
     from minilake_amd.gpu import ParquetTable
     data = files[kind]
