@@ -727,8 +727,9 @@ struct ml_hashagg {
     std::vector<SpillColumn> cols;
   };
   std::vector<std::vector<SpillBatch>> spill;  // per partition, the spilled state batches
+  std::shared_ptr<mlg::MemPool> pool;  // the budget `reserved` is held against (outlives the context)
   ~ml_hashagg() {
-    if (ctx) ctx->mem->shrink(reserved);
+    if (pool) pool->shrink(reserved);
   }
 };
 
@@ -1030,7 +1031,7 @@ static bool agg_reserve(ml_hashagg* A, uint64_t needed) {
   const uint64_t len = A->len_host;
   const uint64_t ncap = swiss_capacity(std::max(needed, len * 2));
   const int64_t bytes = (int64_t)(ncap * (1 + A->nkw) * 8 + ncap * A->slots.size() * 8);
-  if (!ctx->mem->try_grow(bytes)) return false;
+  if (!A->pool->try_grow(bytes)) return false;
   Buffer nt(ctx->arena, ncap * (1 + A->nkw) * 8);
   zero(ctx, nt.ptr, nt.bytes);
   std::vector<Buffer> nacc;
@@ -1049,7 +1050,7 @@ static bool agg_reserve(ml_hashagg* A, uint64_t needed) {
   A->acc = std::move(nacc);
   A->cap = ncap;
   A->wrap = ncap - 1;
-  ctx->mem->shrink(A->reserved);
+  A->pool->shrink(A->reserved);
   A->reserved = bytes;
   return true;
 }
@@ -1067,7 +1068,7 @@ static uint64_t agg_len(ml_hashagg* A) {
 static void agg_reset_table(ml_hashagg* A) {
   A->table.reset();
   A->acc.clear();
-  A->ctx->mem->shrink(A->reserved);
+  A->pool->shrink(A->reserved);
   A->reserved = 0;
   A->cap = 0;
   A->len_host = 0;
@@ -2770,6 +2771,7 @@ ml_status ml_hash_columns(ml_ctx* ctx, const ml_expr* keys, int32_t num_keys, co
 ml_status ml_hashagg_create(ml_ctx* ctx, const ml_agg_spec* spec, ml_hashagg** out) {
   auto* A = new ml_hashagg();
   A->ctx = ctx;
+  A->pool = ctx->mem;
   ml_status st = guarded(ctx, [&] {
     if (spec->num_keys < 0 || spec->num_aggs < 0) fail(ML_ERR_PLAN, "bad aggregate spec");
     if (spec->mode < ML_AGG_MODE_SINGLE || spec->mode > ML_AGG_MODE_FINAL) fail(ML_ERR_PLAN, "bad aggregate mode");
