@@ -63,8 +63,9 @@ def main():
     ap.add_argument("--learned-rows", type=int, default=60_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
-                          "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the star-join benchmark runs on one GPU; sharding over several GPUs "
+                                   "(replicated build sides, row-sharded fact table, a gather of the partial "
+                                   "aggregates) is not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import ctypes as C
