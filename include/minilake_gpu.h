@@ -183,9 +183,11 @@ int32_t ml_ctx_take_stats(ml_ctx* ctx, ml_launch_stat* out, int32_t cap);
  * A join table over budget is ML_ERR_RESOURCES_EXHAUSTED. Over budget, a hash aggregate returns
  * ML_ERR_RESOURCES_EXHAUSTED, or with spill != 0 writes its groups' partial
  * states to host memory in 16 hash partitions (aggregate/hash.rs:189-282) and
- * merges one partition at a time at finalize. */
+ * merges one partition at a time at finalize. Restarts the peak below from the
+ * bytes reserved now. */
 ml_status ml_ctx_set_memory_limit(ml_ctx* ctx, int64_t bytes, int32_t spill);
-/* Bytes reserved now, the most reserved at once, and the limit. */
+/* Bytes reserved now, the most reserved at once since the limit was last set
+ * (or since the context was created), and the limit. */
 void ml_ctx_memory_usage(const ml_ctx* ctx, int64_t* used, int64_t* peak, int64_t* limit);
 
 /* Results: library-owned device columns (host copies on request). */

@@ -117,6 +117,82 @@ def main():
     sec, _ = _timed(torch, scan, a.iters)
     _line("parquet_scan", m, sec, len(data) + m * 24, file_bytes=len(data),
           note="host file -> device columns; includes the pageable host-to-device copy of every chunk")
+    del pt, data, bio, tab
+
+    # GROUP BY a high-cardinality i64 key (n/8 groups), unlimited and under a
+    # memory budget of about a quarter of the group table (spill to host in 16
+    # partitions, merged one partition at a time); 256K-row device batches, so
+    # that one batch fits the budget
+    hk = torch.randint(0, n // 8, (n,), generator=g, device="cuda", dtype=torch.int64)
+    ht = DeviceTable.from_tensors([hk, f64], [abi.INT64, FLOAT64], num_rows=n)
+    hagg = [Agg(AGG_SUM, col(1), FLOAT64), Agg(AGG_COUNT_STAR)]
+    torch.cuda.synchronize()
+    spill_eng = GpuEngine(batch_rows=1 << 18)
+    try:
+        sec_u, res = _timed(torch, lambda: spill_eng.hashagg(ht, [], [col(0)], [abi.INT64], hagg), a.iters)
+        groups = res.num_rows
+        del res
+        limit = max(32 << 20, groups * 64 // 4)
+        spill_eng.set_memory_limit(limit, spill=True)
+        sec_s, res_s = _timed(torch, lambda: spill_eng.hashagg(ht, [], [col(0)], [abi.INT64], hagg), a.iters)
+        assert res_s.num_rows == groups
+        del res_s  # results are freed before their context
+        peak = spill_eng.memory_usage()["peak"]
+        assert peak <= limit, (peak, limit)
+    finally:
+        spill_eng.close()
+    _line("hashagg_unlimited", n, sec_u, n * 16, groups=groups, note="bytes: the key and argument columns read once")
+    _line("hashagg_spill", n, sec_s, n * 16, groups=groups, memory_limit=limit, peak_reserved=peak,
+          slowdown=round(sec_s / sec_u, 2), note="same GROUP BY under a budget of ~1/4 of its table, spilling to host")
+    del ht, hk
+
+    # batched fragments: 32 independent join+aggregate fragments of 512K probe rows
+    frag_line(torch, eng, np.random.default_rng(11), 32, 1 << 19, a.iters)
+
+
+def frag_line(torch, eng, rng, nfrag, prows, iters):
+    """One line per way of running the fragments: a launch pair per fragment,
+    one persistent launch pair for all, the same replayed as a hipGraph, and
+    the persistent form after autotuning."""
+    import ctypes as C
+
+    from minilake_amd import abi
+    from minilake_amd.gpu import lib
+    arr = (abi.MlFragment * nfrag)()
+    keep, probe_rows, build_rows = [], 0, 0
+    for i in range(nfrag):
+        brows = prows // 8
+        groups = 1 << int(rng.integers(4, 12))
+        bk = torch.from_numpy(rng.permutation(1 << 24)[:brows].astype(np.int64) + 1).cuda()
+        bg = torch.from_numpy(rng.integers(0, groups, brows).astype(np.int64)).cuda()
+        pk = bk[torch.from_numpy(rng.integers(0, brows, prows)).cuda()]
+        pk[::7] = -pk[::7]
+        pv = torch.from_numpy(rng.random(prows)).cuda()
+        os_ = torch.zeros(groups, dtype=torch.float64, device="cuda")
+        oc = torch.zeros(groups, dtype=torch.int64, device="cuda")
+        keep += [bk, bg, pk, pv, os_, oc]
+        arr[i] = abi.MlFragment(bk.data_ptr(), bg.data_ptr(), brows, pk.data_ptr(), pv.data_ptr(), prows, groups,
+                                os_.data_ptr(), oc.data_ptr())
+        probe_rows += prows
+        build_rows += brows
+    torch.cuda.synchronize()
+    h = C.c_void_p()
+    tune = abi.MlFragmentTuning(256, 1024, 2, 1024)
+    eng._check(lib().ml_fragments_create(eng.ctx, arr, nfrag, C.byref(tune), C.byref(h)))
+    algo = (probe_rows + build_rows) * 16
+    try:
+        for mode, fn in [("per_fragment", lib().ml_fragments_run_per_fragment),
+                         ("persistent", lib().ml_fragments_run), ("graph", lib().ml_fragments_run_graph)]:
+            sec, _ = _timed(torch, lambda: eng._check(fn(h)), iters)
+            _line("fragments_" + mode, probe_rows + build_rows, sec, algo, fragments=nfrag,
+                  note="bytes: build keys+groups and probe keys+values read once")
+        best = abi.MlFragmentTuning()
+        eng._check(lib().ml_fragments_autotune(h, C.byref(best)))
+        sec, _ = _timed(torch, lambda: eng._check(lib().ml_fragments_run(h)), iters)
+        _line("fragments_autotuned", probe_rows + build_rows, sec, algo, fragments=nfrag,
+              tuning=[best.block_size, best.tile_rows, best.blocks_per_cu, best.lds_groups])
+    finally:
+        lib().ml_fragments_free(h)
 
 
 if __name__ == "__main__":

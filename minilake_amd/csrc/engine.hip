@@ -2887,6 +2887,7 @@ ml_status ml_ctx_set_memory_limit(ml_ctx* ctx, int64_t bytes, int32_t spill) {
     if (bytes < 0) fail(ML_ERR_PLAN, "negative memory limit");
     ctx->mem->limit = bytes;
     ctx->mem->spill = spill != 0;
+    ctx->mem->peak = ctx->mem->used;  // the peak is counted from the new limit on
   });
 }
 

@@ -567,7 +567,10 @@ def test_memory_budget_and_spill(oracle_engine, mode):
     eng = GpuEngine(batch_rows=50_000)
     try:
         limit = 6 << 20
+        assert_tables_equal(eng.hashagg(t, [], keys, kt, aggs, mode=mode).to_host(), want, rtol=1e-12)
+        assert eng.memory_usage()["peak"] > limit  # unlimited: the whole table at once
         eng.set_memory_limit(limit, spill=False)
+        assert eng.memory_usage() == {"used": 0, "peak": 0, "limit": limit}  # the peak restarts
         with pytest.raises(GpuError) as e:
             eng.hashagg(t, [], keys, kt, aggs, mode=mode)
         assert e.value.code == abi.ML_ERR_RESOURCES_EXHAUSTED
