@@ -470,6 +470,12 @@ __device__ __forceinline__ void acc_apply_lds(int kind, uint64_t* w, uint64_t v)
 }
 
 // Find or insert the group with key words w; returns its slot or -1 (table full).
+// Slot words are read with relaxed agent-scope loads, which bypass the CU's
+// non-coherent L1: an acquire load would add a full wait and an L1 invalidate
+// to every probe. The inserter stores the keys and then the header (release);
+// a reader compares keys only after its header load returned that header (the
+// key loads are issued under that branch), so the keys it reads from L2 are
+// the inserted ones.
 __device__ __forceinline__ int64_t agg_slot(const Plan& P, const uint64_t (&w)[MAX_KEYS + 1], int nkw, uint64_t h) {
   const uint64_t hdr = (h & ~3ull) | 2ull;
   const uint64_t stride = 1 + (uint64_t)nkw;
@@ -477,7 +483,7 @@ __device__ __forceinline__ int64_t agg_slot(const Plan& P, const uint64_t (&w)[M
   uint64_t probes = 0;
   while (probes <= P.agg_wrap) {
     uint64_t* s = P.agg_slots + pos * stride;
-    uint64_t cur = __hip_atomic_load(s, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t cur = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (cur == 0) {
       const uint64_t prev = atomicCAS((unsigned long long*)s, 0ull, 1ull);
       if (prev == 0) {
@@ -490,14 +496,14 @@ __device__ __forceinline__ int64_t agg_slot(const Plan& P, const uint64_t (&w)[M
         return (int64_t)pos;
       }
       cur = prev;
-      if (cur != 1) cur = __hip_atomic_load(s, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      if (cur != 1) cur = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (cur == 1) continue;  // another lane is writing this slot's key: read it again
     if (cur == hdr) {
       bool eq = true;
 #pragma unroll
       for (int k = 0; k < MAX_KEYS + 1; k++)
-        if (k < nkw) eq = eq && (s[1 + k] == w[k]);
+        if (k < nkw) eq = eq && (__hip_atomic_load(s + 1 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == w[k]);
       if (eq) return (int64_t)pos;
     }
     pos = (pos & ~P.agg_wrap) | ((pos + 1) & P.agg_wrap);
