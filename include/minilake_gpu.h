@@ -138,7 +138,7 @@ typedef struct ml_expr {
 typedef struct ml_exec_config {
   int64_t batch_rows;       /* device batch (morsel) size; default 2^21 */
   int32_t device;           /* HIP device ordinal */
-  int32_t block_size;       /* threads per workgroup for streaming kernels (0 = tuned) */
+  int32_t block_size;       /* threads per workgroup for streaming kernels: 0 = tuned, else 64, 128, 192 or 256 */
   int32_t grid_per_cu;      /* workgroups per CU for grid-stride kernels (0 = tuned) */
   int32_t use_bloom;        /* 1: join tables carry a blocked Bloom filter (-1 = auto) */
   void* stream;             /* hipStream_t to launch on; NULL = library-owned stream */

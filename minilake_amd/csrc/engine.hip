@@ -2564,6 +2564,9 @@ ml_status ml_ctx_create(const ml_exec_config* cfg, ml_ctx** out) {
     if (cfg) ctx->cfg = *cfg;
     else ml_default_config(&ctx->cfg);
     if (ctx->cfg.batch_rows <= 0) ctx->cfg.batch_rows = 1 << 21;
+    // whole waves, up to the 256 threads the kernels are compiled for
+    if (ctx->cfg.block_size != 0 && (ctx->cfg.block_size < 64 || ctx->cfg.block_size > 256 || ctx->cfg.block_size % 64))
+      fail(ML_ERR_PLAN, "block_size must be 64, 128, 192 or 256");
     ctx->device = ctx->cfg.device;
     int ndev = 0;
     hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");

@@ -711,6 +711,10 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
     for (int c = threadIdx.x; c < ncells; c += blockDim.x) lds[c] = acc_init_bits(P.acc[c % P.nacc].kind);
     __syncthreads();
   }
+  // COUNT_PAIRS: each lane counts its matches in a register and the wave adds
+  // its total to the counter once, after the loop (matches in an expand loop
+  // are scattered over lanes, and one atomic per match serializes on the counter)
+  uint64_t npairs = 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = P.begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.end; i += stride) {
     uint32_t rows[MAX_STAGES];
@@ -720,6 +724,7 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
     if (P.expand_step < 0) {
       const bool alive = run_steps(P, ins, 0, P.nsteps, rows);
       if constexpr (SINK == SINK_FLAGS) P.flag_out[i] = alive;
+      else if constexpr (SINK == SINK_COUNT_PAIRS) npairs += alive ? 1u : 0u;
       else if (alive) do_sink<SINK>(P, ins, rows, i, lds);
     } else {
       // one probe may match several build rows: the rest of the chain runs per match
@@ -736,7 +741,10 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
           if (bloom_maybe(T, h)) {
             while (join_next(T, w, h, pos, row)) {
               set_row(rows, st.stage, row);
-              if (run_steps(P, ins, e + 1, P.nsteps, rows)) do_sink<SINK>(P, ins, rows, i, lds);
+              if (run_steps(P, ins, e + 1, P.nsteps, rows)) {
+                if constexpr (SINK == SINK_COUNT_PAIRS) npairs++;
+                else do_sink<SINK>(P, ins, rows, i, lds);
+              }
               pos = join_step(T, pos);
             }
           }
@@ -748,6 +756,12 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
     __syncthreads();
     uint64_t* out = (uint64_t*)P.direct_partials + (uint64_t)blockIdx.x * ncells;
     for (int c = threadIdx.x; c < ncells; c += blockDim.x) out[c] = lds[c];
+  }
+  if constexpr (SINK == SINK_COUNT_PAIRS) {
+    // every lane of the wave is here (the loop above has no early exit)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) npairs += __shfl_down(npairs, off);
+    if (__lane_id() == 0 && npairs) atomicAdd((unsigned long long*)P.out_count, (unsigned long long)npairs);
   }
 }
 

@@ -62,3 +62,16 @@ def test_constants_match_header():
     assert abi.MLP_NUM_PARAMS == 32 * 64 + 64 + 64 * 64 + 64 + 64 * 2 + 2 == 6402
     for name, val in [("ML_ERR_CAPACITY", 10), ("ML_TYPE_UTF8", 5)]:
         assert re.search(rf"{name}\s*=\s*{val}", text)
+
+
+@pytest.mark.parametrize("block", [32, 100, 320])
+def test_ctx_rejects_partial_wave_block_sizes(block):
+    """Streaming kernels reduce across whole 64-lane waves and are compiled for
+    at most 256 threads: other block sizes are a plan error, before any device
+    is touched."""
+    L = gpu.lib()
+    cfg = abi.MlExecConfig()
+    L.ml_default_config(C.byref(cfg))
+    cfg.block_size = block
+    ctx = C.c_void_p()
+    assert L.ml_ctx_create(C.byref(cfg), C.byref(ctx)) == abi.ML_ERR_PLAN
