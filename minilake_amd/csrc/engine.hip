@@ -629,10 +629,38 @@ static bool run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buf
     out.rows = (int64_t)run_flow_rows(B, SINK_COLLECT, F);
     return true;
   }
+  // row-per-lane pipeline: a counting pass stores each thread's output rows,
+  // a scan turns them into per-thread offsets, and the collect pass writes
+  // every row at its thread's own cursor; one shared counter for all matches
+  // would serialize an expanding probe whose matches are scattered over lanes
+  const int grid = B.grid(), block = block_of(ctx);
+  const int64_t nthreads = (int64_t)grid * block;
+  Buffer tcount(ctx->arena, (size_t)nthreads * 4), toff(ctx->arena, (size_t)(nthreads + 1) * 8);
   zero(ctx, counters, 16);
-  ctx->launch(B.P, B.grid(), block_of(ctx), 0);
+  B.P.sink = SINK_COUNT_PAIRS;
+  B.P.wave_counts = tcount.as<uint32_t>();
+  ctx->launch(B.P, grid, block, 0);
+  B.P.wave_counts = nullptr;
+  B.P.sink = SINK_COLLECT;
   uint64_t hc[2] = {0, 0};
   d2h(ctx, hc, counters, 16);
+  const bool two_pass = !(hc[1] & 2) && hc[0] <= cap;
+  if (two_pass) {
+    launch_scan_counts(tcount.as<uint32_t>(), nthreads, toff.as<uint64_t>(), ctx->stream);
+    B.P.thread_offsets = toff.as<uint64_t>();
+    zero(ctx, counters, 16);
+    ctx->launch(B.P, grid, block, 0);
+    B.P.thread_offsets = nullptr;
+    const uint64_t total = hc[0];
+    d2h(ctx, hc, counters, 16);
+    hc[0] = total;
+  } else if (hc[0] <= cap) {  // a thread's count overflowed 32 bits: one shared counter
+    zero(ctx, counters, 16);
+    ctx->launch(B.P, grid, block, 0);
+    d2h(ctx, hc, counters, 16);
+  } else {
+    hc[1] |= 1;  // more rows than the capacity
+  }
   if (hc[1] & 1) {
     if (capacity_hint >= 0) return false;
     fail(ML_ERR_INTERNAL, "collect output overflowed its capacity");

@@ -679,6 +679,19 @@ __device__ __forceinline__ void sink_collect(const Plan& P, const Code& ins, con
   }
 }
 
+// COLLECT at a position the thread already owns (row-per-lane two-pass collect)
+__device__ __forceinline__ void collect_at(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES],
+                                           uint64_t pos) {
+  if (pos >= P.out_capacity) {
+    atomicOr(P.out_overflow, 1u);
+    return;
+  }
+  write_cols(P, ins, rows, pos);
+  if (P.pair_out) {
+    for (int s = 0; s < P.pair_stages; s++) P.pair_out[pos * P.pair_stages + s] = row_of(rows, s);
+  }
+}
+
 template <int SINK>
 __device__ __forceinline__ void do_sink(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES], int64_t i,
                                         uint64_t* lds) {
@@ -715,6 +728,11 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
   // its total to the counter once, after the loop (matches in an expand loop
   // are scattered over lanes, and one atomic per match serializes on the counter)
   uint64_t npairs = 0;
+  // COLLECT with thread_offsets: this thread's rows go to consecutive positions
+  // from its offset (counted by a COUNT_PAIRS pass over the same grid)
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t cursor = 0;
+  if constexpr (SINK == SINK_COLLECT) cursor = P.thread_offsets ? P.thread_offsets[tid] : 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = P.begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.end; i += stride) {
     uint32_t rows[MAX_STAGES];
@@ -725,7 +743,12 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
       const bool alive = run_steps(P, ins, 0, P.nsteps, rows);
       if constexpr (SINK == SINK_FLAGS) P.flag_out[i] = alive;
       else if constexpr (SINK == SINK_COUNT_PAIRS) npairs += alive ? 1u : 0u;
-      else if (alive) do_sink<SINK>(P, ins, rows, i, lds);
+      else if constexpr (SINK == SINK_COLLECT) {
+        if (alive) {
+          if (P.thread_offsets) collect_at(P, ins, rows, cursor++);
+          else sink_collect(P, ins, rows);
+        }
+      } else if (alive) do_sink<SINK>(P, ins, rows, i, lds);
     } else {
       // one probe may match several build rows: the rest of the chain runs per match
       const int e = P.expand_step;
@@ -743,7 +766,10 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
               set_row(rows, st.stage, row);
               if (run_steps(P, ins, e + 1, P.nsteps, rows)) {
                 if constexpr (SINK == SINK_COUNT_PAIRS) npairs++;
-                else do_sink<SINK>(P, ins, rows, i, lds);
+                else if constexpr (SINK == SINK_COLLECT) {
+                  if (P.thread_offsets) collect_at(P, ins, rows, cursor++);
+                  else sink_collect(P, ins, rows);
+                } else do_sink<SINK>(P, ins, rows, i, lds);
               }
               pos = join_step(T, pos);
             }
@@ -758,6 +784,11 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
     for (int c = threadIdx.x; c < ncells; c += blockDim.x) out[c] = lds[c];
   }
   if constexpr (SINK == SINK_COUNT_PAIRS) {
+    // per-thread counts for a two-pass collect (a count past 32 bits flags it)
+    if (P.wave_counts) {
+      if (npairs > 0xFFFFFFFFull) atomicOr(P.out_overflow, 2u);
+      P.wave_counts[tid] = (uint32_t)npairs;
+    }
     // every lane of the wave is here (the loop above has no early exit)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) npairs += __shfl_down(npairs, off);

@@ -318,7 +318,9 @@ struct Plan {
   int64_t chunk;                         // source rows per wave
   uint32_t* tuples;                      // ROWS: [k][staging] row index of the k-th recorded stage
   uint64_t staging;                      // waves * chunk
-  uint32_t* wave_counts;                 // ROWS / COUNT_PAIRS: rows produced per wave
+  uint32_t* wave_counts;                 // ROWS / COUNT_PAIRS: rows produced per wave (flow), or per
+                                         // thread (row-per-lane COUNT_PAIRS counting pass)
+  uint64_t* thread_offsets;              // row-per-lane COLLECT: each thread's first output row
   // streaming first probe: step 0 probes a unique table with plain stage-0
   // column keys; level 0 then handles FAST_U rows per lane with independent
   // key, Bloom and slot loads in flight
