@@ -1996,13 +1996,19 @@ __global__ void flag_counts_kernel(const uint8_t* flags, int64_t n, uint32_t* bl
 }
 
 // exclusive scan of block counts (single workgroup, chunked); total at offsets[nb]
+// Exclusive scan of nb u32 counts into u64 offsets, offsets[nb] = total; one
+// 1024-thread workgroup, each thread summing SCAN_PER consecutive counts so a
+// round of the workgroup scan covers 16K counts.
+constexpr int SCAN_PER = 16;
 __global__ void scan_counts_kernel(const uint32_t* counts, int64_t nb, uint64_t* offsets) {
   __shared__ uint64_t buf[1024];
   uint64_t carry = 0;
-  for (int64_t base = 0; base < nb; base += 1024) {
-    const int64_t i = base + threadIdx.x;
-    const uint64_t v = i < nb ? counts[i] : 0;
-    buf[threadIdx.x] = v;
+  for (int64_t base = 0; base < nb; base += 1024 * SCAN_PER) {
+    const int64_t first = base + (int64_t)threadIdx.x * SCAN_PER;
+    uint64_t local = 0;
+    for (int k = 0; k < SCAN_PER; k++)
+      if (first + k < nb) local += counts[first + k];
+    buf[threadIdx.x] = local;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
       const uint64_t t = threadIdx.x >= (unsigned)o ? buf[threadIdx.x - o] : 0;
@@ -2010,7 +2016,12 @@ __global__ void scan_counts_kernel(const uint32_t* counts, int64_t nb, uint64_t*
       buf[threadIdx.x] += t;
       __syncthreads();
     }
-    if (i < nb) offsets[i] = carry + buf[threadIdx.x] - v;
+    uint64_t run = carry + buf[threadIdx.x] - local;
+    for (int k = 0; k < SCAN_PER; k++)
+      if (first + k < nb) {
+        offsets[first + k] = run;
+        run += counts[first + k];
+      }
     const uint64_t total = buf[1023];
     __syncthreads();
     carry += total;
