@@ -586,12 +586,23 @@ Prog Compiler::compile(const NodePtr& n, int want, int* out_type) {
   return p;
 }
 
-Prog Compiler::compile_key(const NodePtr& n, int key_type) {
+Prog Compiler::compile_key(const NodePtr& n, int key_type, uint64_t* nan_ids) {
   Prog p = compile(n, key_type);
   if (key_type == ML_TYPE_FLOAT64) {
     depth_ = 1;
-    unary(OP_KEY_F64);
+    unary(OP_KEY_F64, (uint64_t)(uintptr_t)nan_ids);
     p.count++;
+  }
+  return p;
+}
+
+Prog Compiler::compile_join_key(const NodePtr& n, int key_type) {
+  Prog p = compile(n, key_type);
+  if (key_type == ML_TYPE_FLOAT64) {  // values_equal compares f64 with == (kernels/hash.rs:94)
+    depth_ = 1;
+    unary(OP_JKEY_F64);
+    p.count++;
+    p.nullable = 1;
   }
   return p;
 }

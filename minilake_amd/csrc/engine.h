@@ -253,8 +253,12 @@ class Compiler {
   // Compile a resolved tree; the value is cast to `want` (evaluate_to_column
   // semantics) unless want < 0. Returns the program.
   Prog compile(const NodePtr& n, int want, int* out_type = nullptr);
-  // Group/join key word: cast to `key_type`, f64 normalized.
-  Prog compile_key(const NodePtr& n, int key_type);
+  // Group key word: cast to `key_type`; f64 -0.0 -> +0.0 and each NaN a word
+  // of its own, numbered from the device counter `nan_ids` (null: one
+  // canonical NaN, for programs that are only inspected, never run).
+  Prog compile_key(const NodePtr& n, int key_type, uint64_t* nan_ids = nullptr);
+  // Join key word: cast to `key_type`; f64 -0.0 -> +0.0 and NaN -> NULL.
+  Prog compile_join_key(const NodePtr& n, int key_type);
   // hash_columns input: a word to hash, or an already-hashed value (dictionaries).
   Prog compile_hash_input(const NodePtr& n, bool* prehashed);
   // A plain load of a layout column.

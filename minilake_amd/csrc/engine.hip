@@ -233,7 +233,7 @@ void PipelineBuild::add_ops(const ml_operator* ops, int n) {
             if (!c || !same_dict(c->dict, J.key_dicts[k]))
               fail(ML_ERR_UNSUPPORTED, "string join keys must share the build side's dictionary");
           }
-          st.keys[k] = comp.compile_key(kn, J.key_types[k]);
+          st.keys[k] = comp.compile_join_key(kn, J.key_types[k]);
         }
         P.tables[ntables++] = J.dev;
         if (!J.dev.unique) {
@@ -736,6 +736,7 @@ struct ml_hashagg {
   mlg::Buffer table;
   std::vector<mlg::Buffer> acc;
   mlg::Buffer flags;   // [0] group count (u64), [1] overflow bits (u32)
+  mlg::Buffer nan_ids; // u64 counter numbering NaN group keys (each NaN row is its own group)
   uint64_t len_host = 0;
   bool len_known = true;
   int resizes = 0;
@@ -972,7 +973,7 @@ static void compile_agg(ml_hashagg* A, PipelineBuild& B, bool first) {
       d = n->col->dict;
     }
     dicts.push_back(d);
-    B.P.keys[k] = B.comp.compile_key(n, A->key_types[k]);
+    B.P.keys[k] = B.comp.compile_key(n, A->key_types[k], A->nan_ids.as<uint64_t>());
     key_nullable = key_nullable || B.P.keys[k].nullable;
   }
   if (first) {
@@ -1358,6 +1359,8 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
     }
     A->flags = Buffer(ctx->arena, 16);
     zero(ctx, A->flags.ptr, 16);
+    A->nan_ids = Buffer(ctx->arena, 8);  // numbers of NaN group keys, never reused
+    zero(ctx, A->nan_ids.ptr, 8);
     A->planned = true;
   }
   compile_agg(A, B, first);
@@ -1594,6 +1597,8 @@ static ml_result* agg_finalize(ml_hashagg* A, bool states) {
   EmitPlan E{};
   E.nkeys = (int)nk;
   E.key_nullable = A->key_nullable ? 1 : 0;
+  for (size_t k = 0; k < nk; k++)
+    if (A->key_types[k] == ML_TYPE_FLOAT64) E.key_f64 |= 1 << k;
   for (size_t k = 0; k < nk; k++) {
     ResultColumn c;
     c.type = A->key_types[k];
@@ -1807,7 +1812,7 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
       d = n->col->dict;
     }
     kd.push_back(d);
-    B.P.keys[k] = B.comp.compile_key(n, J->key_types[k]);
+    B.P.keys[k] = B.comp.compile_join_key(n, J->key_types[k]);
   }
   if (!J->schema_fixed) {
     J->col_types = types;

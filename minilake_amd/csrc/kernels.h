@@ -26,6 +26,7 @@ enum FinishKind : int32_t { FIN_COPY = 0, FIN_AVG = 1, FIN_I32 = 2 };
 struct EmitPlan {
   int32_t nkeys;
   int32_t key_nullable;
+  int32_t key_f64;  // bit k: key k is FLOAT64 (NaN key words decode to the canonical NaN)
   int32_t key_width[MAX_KEYS];
   void* key_out[MAX_KEYS];
   uint8_t* key_valid[MAX_KEYS];

@@ -199,7 +199,14 @@ __device__ __forceinline__ bool eval(const Code& ins, Prog p, const uint32_t (&r
         case OP_LOOKUP_U8: res = gp<uint8_t>((const void*)imm)[a]; break;
         case OP_LOOKUP_U64: res = gp<uint64_t>((const void*)imm)[a]; break;
         case OP_KEY_F64:
-          res = (a == 0x8000000000000000ull) ? 0ull : ((u2f(a) != u2f(a)) ? 0x7FF8000000000000ull : a);
+          if (a == 0x8000000000000000ull) res = 0ull;
+          else if (u2f(a) != u2f(a))
+            res = imm ? NAN_KEY_TAG | (atomicAdd((unsigned long long*)imm, 1ull) & NAN_KEY_IDS) : CANONICAL_NAN;
+          else res = a;
+          break;
+        case OP_JKEY_F64:
+          res = a == 0x8000000000000000ull ? 0ull : a;
+          if (u2f(a) != u2f(a)) vres = 0;
           break;
         default: res = u2f(a) == 0.0 ? 0ull : a; break;  // OP_HASH_F64
       }
@@ -1952,7 +1959,8 @@ __global__ void agg_emit_kernel(const uint64_t* slots, int nkw, const int64_t* i
     const uint64_t* s = slots + (uint64_t)slot * (1 + nkw);
     const uint64_t nullmask = E.key_nullable ? s[1 + E.nkeys] : 0;
     for (int k = 0; k < E.nkeys; k++) {
-      const uint64_t w = s[1 + k];
+      uint64_t w = s[1 + k];
+      if (((E.key_f64 >> k) & 1) && f64_bits_nan(w)) w = CANONICAL_NAN;
       if (E.key_width[k] == 8) ((uint64_t*)E.key_out[k])[j] = w;
       else if (E.key_width[k] == 4) ((uint32_t*)E.key_out[k])[j] = (uint32_t)w;
       else ((uint8_t*)E.key_out[k])[j] = (uint8_t)w;

@@ -63,6 +63,16 @@ __host__ __device__ inline uint64_t table_hash(const uint64_t* w, int n) {
 // Bloom hash of a table hash: its top bits pick the 3 bits, bits 18.. the word.
 __host__ __device__ inline uint64_t bloom_mix(uint64_t h) { return h * 0xD6E8FEB86659FD93ull; }
 
+// Group key words of NaN keys: sign + exponent + quiet bit set, the low 51
+// bits a number unique to the row. Results decode any NaN key word to the
+// canonical quiet NaN.
+constexpr uint64_t NAN_KEY_TAG = 0xFFF8000000000000ull;
+constexpr uint64_t NAN_KEY_IDS = 0x0007FFFFFFFFFFFFull;
+constexpr uint64_t CANONICAL_NAN = 0x7FF8000000000000ull;
+__host__ __device__ inline bool f64_bits_nan(uint64_t w) {
+  return ((w >> 52) & 0x7FFull) == 0x7FFull && (w & 0x000FFFFFFFFFFFFFull) != 0;
+}
+
 // ------------------------------------------------------------------ opcodes
 enum Opcode : uint16_t {
   // push (write slot `depth`)
@@ -86,8 +96,11 @@ enum Opcode : uint16_t {
   OP_I64_TO_I32,    // truncate, sign-extend
   OP_LOOKUP_U8,     // slot = table[slot] (dictionary predicate per entry)
   OP_LOOKUP_U64,    // slot = table[slot] (dictionary hash / rank per entry)
-  OP_KEY_F64,       // -0.0 -> +0.0 and one canonical NaN (group/join key words)
+  OP_KEY_F64,       // group key word: -0.0 -> +0.0; a NaN becomes a word no other row has
+                    // (a NaN tag plus a number drawn from the u64 counter at imm), so every
+                    // NaN row is its own group, as f64 `==` never matches NaN (kernels/hash.rs:94)
   OP_HASH_F64,      // hash input word: 0 if value == 0.0 else bits (kernels/hash.rs:53)
+  OP_JKEY_F64,      // join key word: -0.0 -> +0.0; a NaN key is NULL, so it never matches
   // binary: slots depth-2, depth-1 -> depth-2
   OP_ADD_I32, OP_SUB_I32, OP_MUL_I32, OP_MOD_I32,
   OP_ADD_I64, OP_SUB_I64, OP_MUL_I64, OP_MOD_I64,

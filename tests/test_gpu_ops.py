@@ -709,3 +709,64 @@ def test_full_sort_matches_oracle(gpu_engine, n):
             if j < len(got):
                 assert sorted(got[i:j]) == sorted(want[i:j])
             i = j
+
+
+def _split_nan_rows(t):
+    rows = rows_of(t)
+    import math
+    return ([r for r in rows if not (isinstance(r[0], float) and math.isnan(r[0]))],
+            [r for r in rows if isinstance(r[0], float) and math.isnan(r[0])])
+
+
+@pytest.mark.parametrize("sizing", ["plain", "partitioned"])
+def test_hashagg_nan_and_negative_zero_group_keys(gpu_engine, oracle_engine, sizing):
+    """values_equal (kernels/hash.rs:84-101): each NaN key row is a group of its
+    own, -0.0 groups with +0.0; the rest of the result equals the oracle's."""
+    import math
+    rng = np.random.default_rng(44)
+    n = 200_000
+    pool = np.array([1.5, -0.0, 0.0, np.nan, 2.5, -7.25, np.inf, -np.inf])
+    k = pool[rng.integers(0, len(pool), n)]
+    k[rng.integers(0, n, 5000)] = rng.normal(0, 1e6, 5000)  # many distinct keys too
+    v = rng.integers(-100, 100, n)
+    g = rng.integers(0, 3, n)
+    t = HostTable([HostColumn(FLOAT64, k), HostColumn(INT64, v), HostColumn(INT64, g)])
+    aggs = [Agg(AGG_SUM, col(1), INT64), count_star_agg()]
+    want = oracle_engine.hashagg(t, [], [col(0), col(2)], [FLOAT64, INT64], aggs).to_host()
+    kw = {} if sizing == "plain" else dict(sizing=abi.SIZING_HINT, capacity_hint=200_000)
+    got = gpu_engine.hashagg(t, [], [col(0), col(2)], [FLOAT64, INT64], aggs, **kw).to_host()
+    assert got.num_rows == want.num_rows
+    gn, gnan = _split_nan_rows(got)
+    wn, wnan = _split_nan_rows(want)
+    assert len(gnan) == int(np.isnan(k).sum()) == len(wnan)
+    assert sorted(r[1:] for r in gnan) == sorted(r[1:] for r in wnan)
+    # every NaN key is emitted as the canonical quiet NaN
+    kc = got.columns[0].values.view(np.uint64)
+    assert set(kc[np.isnan(got.columns[0].values)].tolist()) == {0x7FF8000000000000}
+    norm = lambda rows: sorted((0.0 if r[0] == 0 else r[0],) + tuple(r[1:]) for r in rows)
+    assert norm(gn) == norm(wn)
+    assert not any(math.copysign(1.0, r[0]) < 0 and r[0] == 0 for r in gn)  # one zero group per g: +0.0
+
+
+def test_join_nan_keys_never_match(gpu_engine, oracle_engine):
+    """A NaN join key matches nothing, -0.0 matches +0.0 (join.rs:283-300 with
+    values_equal), in the probe operator and inside a fused pipeline."""
+    rng = np.random.default_rng(45)
+    nb, npr = 3000, 20000
+    pool = np.array([np.nan, -0.0, 0.0, 1.0, 2.0, 3.5])
+    bk = np.concatenate([pool, rng.normal(0, 1e3, nb - len(pool))])
+    bk[rng.integers(0, nb, 100)] = np.nan
+    pk = np.where(rng.random(npr) < 0.5, bk[rng.integers(0, nb, npr)], pool[rng.integers(0, len(pool), npr)])
+    build = HostTable([HostColumn(FLOAT64, bk), HostColumn(INT32, np.arange(nb, dtype=np.int32))])
+    probe = HostTable([HostColumn(FLOAT64, pk), HostColumn(INT64, rng.integers(0, 10, npr))])
+    jo = oracle_engine.join_build(build, [], [col(0)], [FLOAT64])
+    jg = gpu_engine.join_build(build, [], [col(0)], [FLOAT64])
+    po, bo = oracle_engine.join_probe(jo, [col(0)], probe)
+    pg, bg = gpu_engine.join_probe(jg, [col(0)], probe)
+    ids_g = jg.column(1)
+    assert sorted(zip(pg.tolist(), ids_g[bg].tolist())) == sorted(zip(po.tolist(), bo.tolist()))
+    assert not np.isnan(pk[pg]).any()
+    aggs = [Agg(AGG_SUM, col(1), INT64), count_star_agg()]
+    want = oracle_engine.hashagg(probe, [abi.probe_op(jo, [col(0)])], [col(3)], [INT32], aggs).to_host()
+    got = gpu_engine.hashagg(probe, [abi.probe_op(jg, [col(0)])], [col(3)], [INT32], aggs).to_host()
+    assert_tables_equal(got, want)

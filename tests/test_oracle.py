@@ -538,3 +538,46 @@ def test_mlb1_known_bytes_and_round_trip():
     first, used = eng.mlb1_read(two)
     second, used2 = eng.mlb1_read(two[used:])
     assert first.to_host().num_rows == 3 and rows_of(second.to_host()) == rows_of(t) and used + used2 == len(two)
+
+
+def _nan_key_table(n=3000, seed=31):
+    rng = np.random.default_rng(seed)
+    pool = np.array([1.5, -0.0, 0.0, np.nan, 2.5, -7.25, np.inf, -np.inf])
+    k = pool[rng.integers(0, len(pool), n)]
+    v = rng.integers(-100, 100, n)
+    g = rng.integers(0, 3, n)
+    return HostTable([HostColumn(FLOAT64, k), HostColumn(INT64, v), HostColumn(INT64, g)]), k, v, g
+
+
+def test_group_keys_nan_never_equal_negative_zero_equal(oracle_engine):
+    # values_equal (kernels/hash.rs:84-101): f64 keys compare with ==, so every
+    # NaN row is a group of its own and -0.0 joins +0.0's group
+    t, k, v, g = _nan_key_table()
+    r = oracle_engine.hashagg(t, [], [col(0), col(2)], [FLOAT64, INT64],
+                              [Agg(AGG_SUM, col(1), INT64), Agg(AGG_COUNT_STAR)]).to_host()
+    rows = rows_of(r)
+    nan_rows = sorted((gg, s, c) for kk, gg, s, c in rows if math.isnan(kk))
+    assert nan_rows == sorted((int(gg), int(vv), 1) for kk, vv, gg in zip(k, v, g) if math.isnan(kk))
+    ref = {}
+    for kk, vv, gg in zip(k.tolist(), v.tolist(), g.tolist()):
+        if math.isnan(kk):
+            continue
+        key = (0.0 if kk == 0 else kk, gg)
+        s, c = ref.get(key, (0, 0))
+        ref[key] = (s + vv, c + 1)
+    got = {(0.0 if kk == 0 else kk, gg): (s, c) for kk, gg, s, c in rows if not math.isnan(kk)}
+    assert got == ref
+
+
+def test_join_keys_nan_never_match(oracle_engine):
+    # join.rs:283-300 finds candidates with values_equal: a NaN key matches
+    # nothing (not even NaN); -0.0 matches +0.0
+    bk = np.array([np.nan, -0.0, 1.0, np.nan, 2.0, 0.0])
+    pk = np.array([np.nan, 0.0, -0.0, 1.0, 3.0, np.nan, 2.0])
+    build = HostTable([HostColumn(FLOAT64, bk)])
+    probe = HostTable([HostColumn(FLOAT64, pk)])
+    jt = oracle_engine.join_build(build, [], [col(0)], [FLOAT64])
+    p, b = oracle_engine.join_probe(jt, [col(0)], probe)
+    pairs = sorted(zip(p.tolist(), b.tolist()))
+    want = sorted((i, j) for i, x in enumerate(pk.tolist()) for j, y in enumerate(bk.tolist()) if x == y)
+    assert pairs == want and len(want) == 6
