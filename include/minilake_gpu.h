@@ -204,6 +204,24 @@ ml_status ml_result_copy_column(ml_ctx* ctx, const ml_result* r, int32_t i, void
 ml_status ml_result_copy_validity(ml_ctx* ctx, const ml_result* r, int32_t i, uint8_t* dst);
 void ml_result_free(ml_result* r);
 
+/* Allocation-free variant of ml_filter. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (work under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as ml_filter, but it allocates nothing:
+ * every device buffer of the call (flags, compaction counts and offsets,
+ * expanded validity, program, lookup tables) comes from one static device
+ * array of ML_FILTER_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends. Limits: at most ML_FILTER_NOALLOC_MAX_ROWS active rows, and the call's
+ * buffers (1 byte per active row, 1 byte per row for each column with a
+ * validity bitmap, 12 bytes per 1024 active rows, the program) must fit the
+ * array; otherwise ML_ERR_RESOURCES_EXHAUSTED before any launch. Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (several
+ * streams or host threads at once) race; calls on one stream are ordered. */
+#define ML_FILTER_NOALLOC_SCRATCH_BYTES (32 * 1024 * 1024)
+#define ML_FILTER_NOALLOC_MAX_ROWS (1 << 24)
+ml_status ml_filter_noalloc(ml_ctx* ctx, const ml_expr* predicate, const ml_batch* in,
+                            uint32_t* out_sel, int64_t* out_len);
+
 /* ------------------------------------------------------------------ */
 /* Device Parquet scan (crates/minilake-storage/src/table.rs:59-240,
  * parquet_reader.rs:210-358, stats.rs:53-70). Footer and page headers are

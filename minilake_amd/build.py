@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libminilake_gpu.so")
-SOURCES = ["kernels.hip", "compile.hip", "engine.hip", "fragments.hip", "mlp.hip", "parquet.hip"]
+SOURCES = ["kernels.hip", "compile.hip", "engine.hip", "fragments.hip", "mlp.hip", "parquet.hip", "noalloc.hip"]
 ARCH = os.environ.get("MINILAKE_GPU_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

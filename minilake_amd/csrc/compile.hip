@@ -368,6 +368,12 @@ static bool compare_bytes(int op, const std::string& a, const std::string& b) { 
 const uint8_t* Compiler::dict_predicate(const HostDict& d, int op, const std::string& lit) {
   std::vector<uint8_t> per_entry(d.entries.size() + 1, 0);
   for (size_t k = 0; k < d.entries.size(); k++) per_entry[k] = compare_bytes(op, d.entries[k], lit);
+  if (bump) {
+    void* at = bump->take(per_entry.size());
+    hip_check(hipMemcpyAsync(at, per_entry.data(), per_entry.size(), hipMemcpyHostToDevice, ctx_->stream), "upload");
+    hip_check(hipStreamSynchronize(ctx_->stream), "sync");
+    return (const uint8_t*)at;
+  }
   Buffer b(ctx_->arena, per_entry.size());
   hip_check(hipMemcpyAsync(b.ptr, per_entry.data(), per_entry.size(), hipMemcpyHostToDevice, ctx_->stream), "upload");
   hip_check(hipStreamSynchronize(ctx_->stream), "sync");
@@ -631,11 +637,19 @@ Prog Compiler::compile_hash_input(const NodePtr& n, bool* prehashed) {
       }
       hs[k] = h;
     }
-    Buffer b(ctx_->arena, hs.size() * 8);
-    hip_check(hipMemcpyAsync(b.ptr, hs.data(), hs.size() * 8, hipMemcpyHostToDevice, ctx_->stream), "upload");
-    hip_check(hipStreamSynchronize(ctx_->stream), "sync");
-    const uint64_t* tp = b.as<uint64_t>();
-    tables_.push_back(std::move(b));
+    const uint64_t* tp;
+    if (bump) {
+      void* at = bump->take(hs.size() * 8);
+      hip_check(hipMemcpyAsync(at, hs.data(), hs.size() * 8, hipMemcpyHostToDevice, ctx_->stream), "upload");
+      hip_check(hipStreamSynchronize(ctx_->stream), "sync");
+      tp = (const uint64_t*)at;
+    } else {
+      Buffer b(ctx_->arena, hs.size() * 8);
+      hip_check(hipMemcpyAsync(b.ptr, hs.data(), hs.size() * 8, hipMemcpyHostToDevice, ctx_->stream), "upload");
+      hip_check(hipStreamSynchronize(ctx_->stream), "sync");
+      tp = b.as<uint64_t>();
+      tables_.push_back(std::move(b));
+    }
     Prog p = compile(n, -1);
     depth_ = 1;
     unary(OP_LOOKUP_U64, (uint64_t)(uintptr_t)tp);

@@ -106,6 +106,23 @@ struct HostDict {
   void finish_view();
 };
 
+// Device memory handed out front to back from one fixed region (the
+// allocation-free entry points carve every buffer of a call from it instead
+// of the context's pool). Offsets are 256-byte aligned.
+struct ScratchBump {
+  uint8_t* base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  void* take(size_t n) {
+    const size_t at = (used + 255) & ~(size_t)255;
+    if (at + n > cap)
+      fail(ML_ERR_RESOURCES_EXHAUSTED, "scratch region of " + std::to_string(cap) + " bytes exhausted (" +
+                                           std::to_string(at + n) + " bytes needed)");
+    used = at + n;
+    return base + at;
+  }
+};
+
 // ------------------------------------------------------------------ context
 struct MlpModel;
 
@@ -263,6 +280,8 @@ class Compiler {
   Prog compile_hash_input(const NodePtr& n, bool* prehashed);
   // A plain load of a layout column.
   Prog compile_load(const VCol& c);
+  // When set, lookup tables are carved from this region, not the context's pool.
+  ScratchBump* bump = nullptr;
 
  private:
   struct Val {
@@ -330,6 +349,9 @@ struct ml_join_table {
 };
 
 namespace mlg {
+// Look up this device's address of the allocation-free entry points' scratch
+// array (once per device; call with the device current).
+void noalloc_setup(int device);
 // Finish a table whose build check is pending (waits for the stream once).
 void join_table_resolve(ml_join_table* t);
 }  // namespace mlg
@@ -350,6 +372,10 @@ struct PipelineBuild {
   bool expanding = false;
   Compiler comp;
   Buffer dev_ins;
+  // When set, every device buffer of the plan (expanded validity, lookup
+  // tables, instructions) is carved from this region, not the context's pool.
+  ScratchBump* bump = nullptr;
+  void use_scratch(ScratchBump* b) { bump = b; comp.bump = b; }
 
   PipelineBuild(ml_ctx* c) : ctx(c), comp(c, ins, tables) {}
   void source(const ml_batch* b, bool use_selection = true);

@@ -182,10 +182,16 @@ void PipelineBuild::source(const ml_batch* b, bool use_selection) {
     if (b->num_rows > 0 && !mc.data) fail(ML_ERR_INTERNAL, "null column data");
     if (mc.validity && b->num_rows > 0) {
       // the pipeline reads validity as one byte per physical row
-      Buffer v(ctx->arena, (size_t)b->num_rows);
-      launch_bits_to_bytes(mc.validity, b->num_rows, v.as<uint8_t>(), ctx->stream);
-      c.valid = v.as<uint8_t>();
-      tables.push_back(std::move(v));
+      if (bump) {
+        uint8_t* v = (uint8_t*)bump->take((size_t)b->num_rows);
+        launch_bits_to_bytes(mc.validity, b->num_rows, v, ctx->stream);
+        c.valid = v;
+      } else {
+        Buffer v(ctx->arena, (size_t)b->num_rows);
+        launch_bits_to_bytes(mc.validity, b->num_rows, v.as<uint8_t>(), ctx->stream);
+        c.valid = v.as<uint8_t>();
+        tables.push_back(std::move(v));
+      }
     }
     layout.push_back(add_col(std::move(c)));
   }
@@ -288,9 +294,15 @@ void PipelineBuild::add_ops(const ml_operator* ops, int n) {
 
 void PipelineBuild::upload() {
   if (ins.empty()) ins.push_back(Instr{});
-  dev_ins = Buffer(ctx->arena, ins.size() * sizeof(Instr));
-  mlg::upload(ctx, dev_ins.ptr, ins.data(), ins.size() * sizeof(Instr));
-  P.ins = dev_ins.as<Instr>();
+  Instr* dst;
+  if (bump) {
+    dst = (Instr*)bump->take(ins.size() * sizeof(Instr));
+  } else {
+    dev_ins = Buffer(ctx->arena, ins.size() * sizeof(Instr));
+    dst = dev_ins.as<Instr>();
+  }
+  mlg::upload(ctx, dst, ins.data(), ins.size() * sizeof(Instr));
+  P.ins = dst;
   P.nins = (int32_t)ins.size();
 }
 
@@ -2605,6 +2617,7 @@ ml_status ml_ctx_create(const ml_exec_config* cfg, ml_ctx** out) {
     hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
     if (ctx->device < 0 || ctx->device >= ndev) fail(ML_ERR_EXECUTION, "no such HIP device");
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    noalloc_setup(ctx->device);
     hipDeviceProp_t prop;
     hip_check(hipGetDeviceProperties(&prop, ctx->device), "hipGetDeviceProperties");
     ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
