@@ -770,3 +770,22 @@ def test_join_nan_keys_never_match(gpu_engine, oracle_engine):
     want = oracle_engine.hashagg(probe, [abi.probe_op(jo, [col(0)])], [col(3)], [INT32], aggs).to_host()
     got = gpu_engine.hashagg(probe, [abi.probe_op(jg, [col(0)])], [col(3)], [INT32], aggs).to_host()
     assert_tables_equal(got, want)
+
+
+@pytest.mark.parametrize("asc0,nf0,asc1,nf1", [(True, False, True, False), (False, True, True, True),
+                                               (True, True, False, False), (False, False, False, True)])
+def test_sort_total_order_nan_zero_strings_nulls(gpu_engine, asc0, nf0, asc1, nf1):
+    """The full sort and TopN against a brute-force sort: f64 in total_cmp
+    order (NaN, -0.0, infinities), a dictionary string in byte order, NULLS
+    FIRST/LAST, ties kept in source order (sort.rs:43-132)."""
+    import math
+    from tests.test_oracle import _brute_sort, _f64_rank, _sort_table
+    t = _sort_table(n=3000, seed=62)
+    keys = [(col(0), asc0, nf0), (col(1), asc1, nf1)]
+    want = _brute_sort(rows_of(t), [(0, asc0, nf0), (1, asc1, nf1)])
+    got = rows_of(gpu_engine.topn(t, [], keys, -1).to_host())
+    assert [r[3] for r in got] == [r[3] for r in want]
+    kf = lambda r: tuple(None if v is None else (_f64_rank(v) if isinstance(v, float) else v) for v in r[:2])
+    for limit in (10, 200):
+        top = rows_of(gpu_engine.topn(t, [], keys, limit).to_host())
+        assert [kf(r) for r in top] == [kf(r) for r in want[:limit]]

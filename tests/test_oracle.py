@@ -581,3 +581,77 @@ def test_join_keys_nan_never_match(oracle_engine):
     pairs = sorted(zip(p.tolist(), b.tolist()))
     want = sorted((i, j) for i, x in enumerate(pk.tolist()) for j, y in enumerate(bk.tolist()) if x == y)
     assert pairs == want and len(want) == 6
+
+
+# ------------------------------------------- compare_rows (sort.rs:43-82)
+SORT_DICT = ["pear", "apple", "fig", "", "Zebra", "apple2", "date"]  # not in byte order
+
+
+def _sort_table(n=400, seed=61):
+    rng = np.random.default_rng(seed)
+    pool = np.array([np.nan, -np.nan, -np.inf, np.inf, -0.0, 0.0, 1.5, -2.5, 1e-300, -1e-300])
+    f = pool[rng.integers(0, len(pool), n)]
+    f_valid = rng.random(n) > 0.15
+    s = rng.integers(0, len(SORT_DICT), n).astype(np.uint32)
+    s_valid = rng.random(n) > 0.15
+    i = rng.integers(-2, 3, n).astype(np.int32)
+    return HostTable([HostColumn(FLOAT64, f, None, f_valid), HostColumn(UTF8, s, SORT_DICT, s_valid),
+                      HostColumn(INT32, i), HostColumn(INT64, np.arange(n, dtype=np.int64))])
+
+
+def _f64_rank(x):
+    # f64::total_cmp: -NaN < -inf < ... < -0.0 < +0.0 < ... < +inf < +NaN
+    if math.isnan(x):
+        return (2 if math.copysign(1.0, x) > 0 else -2, 0.0, 0.0)
+    return (0, x, math.copysign(1.0, x))
+
+
+def _brute_sort(rows, keys):
+    """Python's stable sort applied key by key from the last to the first,
+    with each key's NULL placement and direction."""
+    out = list(rows)
+    for c, asc, nulls_first in reversed(keys):
+        def k(r, c=c):
+            v = r[c]
+            if v is None:
+                return (0,)
+            if isinstance(v, float):
+                return (1,) + _f64_rank(v)
+            if isinstance(v, str):
+                return (1, v.encode())
+            return (1, v)
+        nonnull = sorted([r for r in out if r[c] is not None], key=k, reverse=not asc)
+        if not asc:  # a reversed stable sort must keep ties in input order
+            nonnull = []
+            groups = {}
+            for r in out:
+                if r[c] is not None:
+                    groups.setdefault(k(r), []).append(r)
+            for kk in sorted(groups, reverse=True):
+                nonnull.extend(groups[kk])
+        nulls = [r for r in out if r[c] is None]
+        out = nulls + nonnull if nulls_first else nonnull + nulls
+    return out
+
+
+@pytest.mark.parametrize("asc0,nf0,asc1,nf1", [(a0, n0, a1, n1) for a0 in (True, False) for n0 in (True, False)
+                                               for a1 in (True, False) for n1 in (True, False)])
+def test_compare_rows_matches_brute_force(asc0, nf0, asc1, nf1):
+    # ORDER BY f64 (total_cmp, NaN/-0/inf), then a dictionary string (byte
+    # order, not code order): the full sort is stable (sort_by keeps ties in
+    # input order; one thread, so input order is source order)
+    t = _sort_table()
+    eng = O.OracleEngine(threads=1)
+    keys = [(col(0), asc0, nf0), (col(1), asc1, nf1)]
+    got = rows_of(eng.topn(t, [], keys, -1).to_host())
+    want = _brute_sort(rows_of(t), [(0, asc0, nf0), (1, asc1, nf1)])
+    same = lambda a, b: (a is None and b is None) or (isinstance(a, float) and isinstance(b, float) and math.isnan(a)
+                                                       and math.isnan(b)) or a == b
+    assert len(got) == len(want)
+    for g, w in zip(got, want):
+        assert g[3] == w[3], (g, w)  # the same row at every position
+        assert all(same(x, y) for x, y in zip(g, w))
+    # with a limit: the first rows of the same order (ties at the cut may differ)
+    top = rows_of(eng.topn(t, [], keys, 25).to_host())
+    kf = lambda r: tuple(None if v is None else (_f64_rank(v) if isinstance(v, float) else v) for v in r[:2])
+    assert [kf(r) for r in top] == [kf(r) for r in want[:25]]
