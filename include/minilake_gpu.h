@@ -491,42 +491,57 @@ ml_status ml_topn_finalize(ml_topn* t, ml_result** out);
 void ml_topn_free(ml_topn* t);
 
 /* ------------------------------------------------------------------ */
-/* Batched fragments: many independent "SELECT g, SUM(m) FROM probe JOIN build
- * USING (key) GROUP BY g" fragments in one persistent-kernel launch, optionally
- * replayed through a captured hipGraph. No reference counterpart (its nearest
- * analog is the morsel queue, crates/minilake-exec/src/scheduler.rs:44-68).   */
+/* Batched fragments: many independent
+ *   SELECT g, SUM(v), COUNT(*) FROM probe JOIN build USING (key) GROUP BY g
+ * fragments (g = the group id of the matching build row) in one persistent-
+ * kernel launch sequence, optionally replayed through a captured hipGraph.
+ * A probe row adds to the group of every build row with an equal key (the
+ * reference's HashJoinProbe emits every chain match, join.rs:283-300); NULL
+ * keys never match. No reference counterpart as a whole: the nearest analog
+ * is the morsel queue, crates/minilake-exec/src/scheduler.rs:44-68.        */
 /* ------------------------------------------------------------------ */
 typedef struct ml_fragment {
-  const int64_t* build_keys;   /* device, unique */
-  const int64_t* build_groups; /* device, group key of each build row, in [0, num_groups) */
+  const int64_t* build_keys;       /* device */
+  const int64_t* build_groups;     /* device, group id of each build row; must lie in [0, num_groups) */
+  const uint64_t* build_validity;  /* device LSB-first bitmap of the build keys; NULL = all valid */
   int64_t build_rows;
-  const int64_t* probe_keys;   /* device */
-  const double* probe_values;  /* device */
+  const int64_t* probe_keys;       /* device */
+  const double* probe_values;      /* device */
+  const uint64_t* probe_validity;  /* device LSB-first bitmap of the probe keys; NULL = all valid */
   int64_t probe_rows;
-  int64_t num_groups;          /* group key domain (dense) */
-  double* out_sums;            /* device, num_groups */
-  int64_t* out_counts;         /* device, num_groups */
+  int64_t num_groups;              /* group id domain, >= 1 */
+  double* out_sums;                /* device, num_groups; overwritten by every run */
+  int64_t* out_counts;             /* device, num_groups; matched pairs per group */
 } ml_fragment;
 
 typedef struct ml_fragment_tuning {
-  int32_t block_size;      /* threads per workgroup */
-  int32_t tile_rows;       /* probe rows per work item */
+  int32_t block_size;      /* threads per workgroup: 256, 512 or 1024 */
+  int32_t tile_rows;       /* rows per work item */
   int32_t blocks_per_cu;   /* persistent grid = CUs * blocks_per_cu */
-  int32_t lds_groups;      /* groups up to this domain aggregate in LDS first */
+  int32_t lds_groups;      /* LDS aggregation entries (a power of two <= 2048; 0 = none): domains up
+                              to this size aggregate in an LDS array */
+  int32_t probe_width;     /* probe rows each lane keeps in flight: 1, 2, 4 or 8 */
+  int32_t lds_cache;       /* 1: larger domains aggregate through an LDS cache of lds_groups entries */
 } ml_fragment_tuning;
 
 typedef struct ml_fragments ml_fragments;
 
-/* Plan a set of fragments: allocates every hash table and the work queue. */
+/* Plan a set of fragments: allocates every hash table and the work queues.
+ * Sizes and null pointers are checked here (ML_ERR_PLAN). */
 ml_status ml_fragments_create(ml_ctx* ctx, const ml_fragment* frags, int32_t n,
                               const ml_fragment_tuning* tuning, ml_fragments** out);
-/* Run all fragments: one build launch and one probe launch (persistent kernels). */
+/* Run all fragments (reset, build and probe launches over device work
+ * queues) and wait for them. A build row whose group id lies outside
+ * [0, num_groups) is skipped on the device and the call returns
+ * ML_ERR_PLAN naming the first such fragment. */
 ml_status ml_fragments_run(ml_fragments* f);
-/* Capture ml_fragments_run into a hipGraph (once) and replay it. */
+/* The same launch sequence captured into a hipGraph (once) and replayed. */
 ml_status ml_fragments_run_graph(ml_fragments* f);
-/* Baseline: the same fragments, two kernel launches per fragment. */
+/* Baseline: the same kernels launched fragment by fragment. */
 ml_status ml_fragments_run_per_fragment(ml_fragments* f);
-/* Autotune: time a small grid of tunings and keep the fastest. */
+/* Autotune: coordinate descent over probe width, tile rows, block size,
+ * blocks per CU and the LDS aggregation mode, timing whole runs; keeps and
+ * returns the fastest tuning. */
 ml_status ml_fragments_autotune(ml_fragments* f, ml_fragment_tuning* best);
 void ml_fragments_free(ml_fragments* f);
 

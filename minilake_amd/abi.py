@@ -125,9 +125,10 @@ class MlSortSpec(C.Structure):
 
 
 class MlFragment(C.Structure):
-    _fields_ = [("build_keys", C.c_void_p), ("build_groups", C.c_void_p), ("build_rows", C.c_int64),
-                ("probe_keys", C.c_void_p), ("probe_values", C.c_void_p), ("probe_rows", C.c_int64),
-                ("num_groups", C.c_int64), ("out_sums", C.c_void_p), ("out_counts", C.c_void_p)]
+    _fields_ = [("build_keys", C.c_void_p), ("build_groups", C.c_void_p), ("build_validity", C.c_void_p),
+                ("build_rows", C.c_int64), ("probe_keys", C.c_void_p), ("probe_values", C.c_void_p),
+                ("probe_validity", C.c_void_p), ("probe_rows", C.c_int64), ("num_groups", C.c_int64),
+                ("out_sums", C.c_void_p), ("out_counts", C.c_void_p)]
 
 
 class MlLaunchStat(C.Structure):
@@ -136,7 +137,7 @@ class MlLaunchStat(C.Structure):
 
 class MlFragmentTuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("tile_rows", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("lds_groups", C.c_int32)]
+                ("lds_groups", C.c_int32), ("probe_width", C.c_int32), ("lds_cache", C.c_int32)]
 
 
 # --------------------------------------------------------------- expressions

@@ -62,6 +62,8 @@ def gpu_lib():
         L.mlw_gpu_zipf_keys.restype = C.c_int
         L.mlw_gpu_build_keys.argtypes = [vp, i64, i64, u64, u64, i64, vp]
         L.mlw_gpu_build_keys.restype = C.c_int
+        L.mlw_gpu_fragment.argtypes = [vp, vp, i64, vp, vp, i64, i64, C.c_double, u64, u64, vp]
+        L.mlw_gpu_fragment.restype = C.c_int
         _gpu_lib = L
     return _gpu_lib
 
@@ -283,3 +285,38 @@ def star_join(seed: int = 42, scale: float = 10.0) -> StarJoin:
                           HostColumn(abi.FLOAT64, pick_f64(seed, 308, n_l, DISC_TABLE))],
                          names=["l_orderkey", "l_suppkey", "l_extendedprice", "l_discount"])
     return StarJoin(region, nation, customer, orders, supplier, lineitem)
+
+
+# ------------------------------------------------------------ batched fragments
+@dataclass
+class FragmentShape:
+    probe_rows: int
+    build_rows: int
+    groups: int
+    zipf_s: float
+
+
+def fragment_shapes(seed: int = 42, n: int = 256, max_log2_probe: float = 24.0) -> List[FragmentShape]:
+    """Sizes of the batched-fragment workload: probe rows = floor(2^U[10, 24]),
+    build rows = probe rows / U[4, 40], groups = floor(2^U[0, 20]), Zipf
+    s = (0, 0.8, 1.2)[i % 3]. `max_log2_probe` < 24 shrinks the probe range's
+    upper end (the other draws are unchanged)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        lp = rng.uniform(10.0, 24.0)
+        ratio = rng.uniform(4.0, 40.0)
+        lg = rng.uniform(0.0, 20.0)
+        lp = 10.0 + (lp - 10.0) * (max_log2_probe - 10.0) / 14.0
+        prow = int(2.0 ** lp)
+        out.append(FragmentShape(prow, max(1, int(prow / ratio)), max(1, int(2.0 ** lg)), (0.0, 0.8, 1.2)[i % 3]))
+    return out
+
+
+def gpu_fragment(shape: FragmentShape, index: int, seed: int, bkeys, bgroups, pkeys, pvals, hip_stream=None):
+    """Fill device buffers (pointers) with fragment `index`: unique build keys,
+    uniform build groups, Zipf(s) probe keys over the build keys, U[0,1) values."""
+    rc = gpu_lib().mlw_gpu_fragment(bkeys, bgroups, shape.build_rows, pkeys, pvals, shape.probe_rows, shape.groups,
+                                    float(shape.zipf_s), seed, index, hip_stream)
+    if rc != 0:
+        raise RuntimeError("device fragment generation failed")

@@ -90,6 +90,25 @@ __global__ void build_keys_kernel(int64_t* out, int64_t nkeys, int64_t mult, uin
     out[j] = (int64_t)permute(pkey, half, pmask, domain, (uint64_t)(j / mult));
 }
 
+// One batched-fragment workload fragment: build keys unique (the Feistel
+// permutation of the build row over [0, 2^62)), build groups uniform in
+// [0, groups), probe keys the build key of a Zipf(s)-distributed build row
+// (s <= 0: uniform), probe values uniform in [0, 1).
+__global__ void fragment_kernel(int64_t* bkeys, int64_t* bgroups, int64_t brows, int64_t* pkeys, double* pvals,
+                                int64_t prows, uint64_t groups, uint64_t kkey, int khalf, uint64_t kmask,
+                                uint64_t kdomain, uint64_t gkey, uint64_t zkey, Zipf z, int uniform, uint64_t vkey) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < brows; j += stride) {
+    bkeys[j] = (int64_t)permute(kkey, khalf, kmask, kdomain, (uint64_t)j);
+    bgroups[j] = (int64_t)below(draw(gkey, (uint64_t)j), groups);
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < prows; i += stride) {
+    const uint64_t rank = uniform ? 1 + below(draw(zkey, (uint64_t)i), (uint64_t)brows) : zipf_sample(z, zkey, (uint64_t)i);
+    pkeys[i] = (int64_t)permute(kkey, khalf, kmask, kdomain, rank - 1);
+    pvals[i] = unit(draw(vkey, (uint64_t)i));
+  }
+}
+
 void feistel_params(uint64_t n, int* half, uint64_t* mask) {
   int bits = 2;
   while ((1ull << bits) < n) bits += 2;
@@ -128,6 +147,25 @@ int mlw_gpu_build_keys(int64_t* out, int64_t nkeys, int64_t mult, uint64_t seed,
   build_keys_kernel<<<(unsigned)blocks, 256, 0, (hipStream_t)stream_ptr>>>(out, nkeys, mult,
                                                                             stream_key(seed, perm_stream), half, mask,
                                                                             (uint64_t)domain);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// Fragment `index` of the batched-fragment workload (see fragment_kernel);
+// every column is a function of (seed, index, row) only.
+int mlw_gpu_fragment(int64_t* bkeys, int64_t* bgroups, int64_t brows, int64_t* pkeys, double* pvals, int64_t prows,
+                     int64_t groups, double s, uint64_t seed, uint64_t index, void* stream_ptr) {
+  if (brows <= 0 || prows < 0 || groups <= 0) return 1;
+  const uint64_t kdomain = 1ull << 62;
+  int half;
+  uint64_t mask;
+  feistel_params(kdomain, &half, &mask);
+  const Zipf z = make_zipf(s > 0 ? s : 1.0, (double)brows);
+  const uint64_t base = 0x4652414700000000ull + index * 8;
+  const int64_t n = brows > prows ? brows : prows;
+  const int64_t blocks = (n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384;
+  fragment_kernel<<<(unsigned)(blocks > 0 ? blocks : 1), 256, 0, (hipStream_t)stream_ptr>>>(
+      bkeys, bgroups, brows, pkeys, pvals, prows, (uint64_t)groups, stream_key(seed, base), half, mask, kdomain,
+      stream_key(seed, base + 1), stream_key(seed, base + 2), z, s <= 0 ? 1 : 0, stream_key(seed, base + 3));
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

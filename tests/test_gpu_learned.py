@@ -1,11 +1,9 @@
-"""GPU checks of the learned-sizing network and the batched-fragment runner.
+"""GPU checks of the learned-sizing network.
 
 * MLP forward and gradients from the library's HIP kernels against the float64
   numpy oracle (itself pinned by finite differences in test_mlp_oracle.py);
   tolerance: FP32 arithmetic, 2e-5 of the largest gradient.
 * ml_sizing_features' device statistics against numpy on the same rows.
-* Batched fragments (one persistent launch, a replayed hipGraph, and one
-  launch pair per fragment) against the CPU oracle's join + aggregate.
 """
 
 import ctypes as C
@@ -82,81 +80,6 @@ def test_sizing_features_statistics(gpu_engine):
     maxs = [keys[g * rg:(g + 1) * rg].max() for g in range(ng)]
     ml = np.mean([np.log2(float(b) - float(a) + 1) for a, b in zip(mins, maxs)])
     assert f[3] == pytest.approx(ml, rel=1e-5)
-
-
-def make_fragments(rng, nfrag, max_probe):
-    frags = []
-    for i in range(nfrag):
-        prow = int(2 ** rng.uniform(4, np.log2(max_probe)))
-        brow = max(1, prow // int(rng.integers(4, 41)))
-        G = int(2 ** rng.uniform(0, 12))
-        bkeys = rng.permutation(1 << 24)[:brow].astype(np.int64) + 1
-        bgroups = rng.integers(0, G, brow).astype(np.int64)
-        s = [0.0, 0.8, 1.2][i % 3]
-        if s == 0:
-            pk = bkeys[rng.integers(0, brow, prow)]
-        else:
-            ranks = np.minimum(rng.zipf(1 + s, prow), brow) - 1 if s > 0 else None
-            pk = bkeys[ranks]
-        pk = pk.copy()
-        pk[::7] = -pk[::7]  # some probe keys miss
-        vals = rng.random(prow)
-        frags.append((bkeys, bgroups, pk.astype(np.int64), vals, G))
-    return frags
-
-
-def run_fragments(engine, frags, mode, tuning=None, reps=1):
-    from minilake_amd.gpu import lib
-    torch = engine.torch
-    dev, outs = [], []
-    arr = (abi.MlFragment * len(frags))()
-    for i, (bk, bg, pk, pv, G) in enumerate(frags):
-        t = [torch.from_numpy(a).to("cuda") for a in (bk, bg, pk, pv)]
-        o_s = torch.zeros(G, dtype=torch.float64, device="cuda")
-        o_c = torch.zeros(G, dtype=torch.int64, device="cuda")
-        dev.append(t)
-        outs.append((o_s, o_c))
-        arr[i] = abi.MlFragment(t[0].data_ptr(), t[1].data_ptr(), len(bk), t[2].data_ptr(), t[3].data_ptr(), len(pk),
-                                G, o_s.data_ptr(), o_c.data_ptr())
-    h = C.c_void_p()
-    tune = tuning or abi.MlFragmentTuning(256, 1024, 2, 1024)
-    engine._check(lib().ml_fragments_create(engine.ctx, arr, len(frags), C.byref(tune), C.byref(h)))
-    results = []
-    try:
-        for _ in range(reps):
-            fn = {"persistent": lib().ml_fragments_run, "graph": lib().ml_fragments_run_graph,
-                  "per_fragment": lib().ml_fragments_run_per_fragment}[mode]
-            engine._check(fn(h))
-            engine.synchronize()
-            results.append([(s.cpu().numpy().copy(), c.cpu().numpy().copy()) for s, c in outs])
-    finally:
-        lib().ml_fragments_free(h)
-    return results
-
-
-def oracle_fragment(oracle_engine, bk, bg, pk, pv, G):
-    build = HostTable([HostColumn(INT64, bk), HostColumn(INT64, bg)])
-    probe = HostTable([HostColumn(INT64, pk), HostColumn(FLOAT64, pv)])
-    jt = oracle_engine.join_build(build, [], [col(0)], [INT64])
-    # layout: probe (key, value) ++ build (key, group)
-    r = oracle_engine.hashagg(probe, [abi.probe_op(jt, [col(0)])], [col(3)], [INT64],
-                              [Agg(AGG_SUM, col(1), FLOAT64), Agg(AGG_COUNT_STAR)]).to_host()
-    sums, counts = np.zeros(G), np.zeros(G, dtype=np.int64)
-    for g, s, c in rows_of(r):
-        sums[g], counts[g] = s, c
-    return sums, counts
-
-
-@pytest.mark.parametrize("mode", ["persistent", "graph", "per_fragment"])
-def test_fragments_match_oracle(gpu_engine, oracle_engine, mode):
-    rng = np.random.default_rng(7)
-    frags = make_fragments(rng, 12, 200_000)
-    want = [oracle_fragment(oracle_engine, *f) for f in frags]
-    reps = 3 if mode == "graph" else 1
-    for got in run_fragments(gpu_engine, frags, mode, reps=reps):
-        for (gs, gc), (ws, wc) in zip(got, want):
-            np.testing.assert_array_equal(gc, wc)
-            np.testing.assert_allclose(gs, ws, rtol=1e-12, atol=1e-9)
 
 
 def test_learned_sizing_bit_identical_and_faster(gpu_engine):
