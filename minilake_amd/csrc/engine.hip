@@ -357,7 +357,8 @@ constexpr size_t FLOW_MAX_LDS = 64 * 1024;
 // so every CU is filled. `sink_stages` are the stages the sink reads. Returns
 // false when the general kernel must run instead (a probe that can match
 // several rows, row indices beyond 32 bits, or queues too large for LDS).
-static bool plan_flow(PipelineBuild& B, int sink, uint32_t sink_stages, size_t cells_bytes, FlowLaunch& F) {
+static bool plan_flow(PipelineBuild& B, int sink, uint32_t sink_stages, size_t cells_bytes, FlowLaunch& F,
+                      bool sink_at_level0 = false) {
   Plan& P = B.P;
   P.nseg = 0;
   const int64_t active = P.end - P.begin;
@@ -365,6 +366,9 @@ static bool plan_flow(PipelineBuild& B, int sink, uint32_t sink_stages, size_t c
   if (P.end > (int64_t)UINT32_MAX) return false;
   std::vector<int> cuts;
   for (int s = 1; s <= P.nsteps; s++) cuts.push_back(s);
+  // a lone filter in front of a sink that evaluates a lane's rows together:
+  // the sink runs in the filter's round (no queue before an empty segment)
+  if (sink_at_level0 && P.nsteps == 1 && P.steps[0].kind == STEP_FILTER) cuts.clear();
   if ((int)cuts.size() > MAX_LEVELS) {
     std::vector<int> c2;
     for (int s = 1; s < P.nsteps; s++)
@@ -1047,6 +1051,66 @@ static void compile_agg(ml_hashagg* A, PipelineBuild& B, bool first) {
     }
     B.P.acc[s].arg = found;
   }
+  // stack slots of the key and argument programs (push instructions write
+  // slot depth, the others consume from the top)
+  auto slots_of = [&](const Prog& pg) {
+    int m = 0;
+    for (int i = 0; i < pg.count; i++) {
+      const Instr& in = B.ins[(size_t)pg.first + i];
+      m = std::max(m, in.op <= OP_LOOKUPC ? in.depth + 1 : (int)in.depth);
+    }
+    return m;
+  };
+  int need = 0;
+  for (int k = 0; k < B.P.nkeys; k++) need = std::max(need, slots_of(B.P.keys[k]));
+  for (int a = 0; a < B.P.nargs; a++) need = std::max(need, slots_of(B.P.args[a]));
+  B.P.src_batch = need <= 4 ? 1 : 0;
+  // the stage-0 columns those programs read, for a batched level-0 sink: at
+  // most 8, each marked in the instructions that read it
+  B.P.sp_n = 0;
+  if (B.P.src_batch) {
+    std::vector<const void*> cols;
+    std::vector<int> ops;
+    std::vector<size_t> marked;
+    bool fits = true;
+    auto mark = [&](const Prog& pg) {
+      for (int i = 0; i < pg.count && fits; i++) {
+        Instr& in = B.ins[(size_t)pg.first + i];
+        if (!reads_column(in.op) || in.stage != 0) continue;
+        const void* c = (const void*)in.imm;
+        const int lop = (int)column_load_op(in.op, in.flags);
+        size_t j = 0;
+        while (j < cols.size() && !(cols[j] == c && ops[j] == lop)) j++;
+        if (j == cols.size()) {
+          if (cols.size() == 8) {
+            fits = false;
+            break;
+          }
+          cols.push_back(c);
+          ops.push_back(lop);
+        }
+        in.flags = (in.flags & 0x00FFFFFFu) | ((uint32_t)(j + 1) << 24);
+        marked.push_back((size_t)pg.first + i);
+      }
+    };
+    for (int k = 0; k < B.P.nkeys; k++) mark(B.P.keys[k]);
+    for (int a = 0; a < B.P.nargs; a++) mark(B.P.args[a]);
+    if (fits) {
+      B.P.sp_n = (int32_t)cols.size();
+      for (size_t j = 0; j < cols.size(); j++) {
+        B.P.sp_col[j] = cols[j];
+        B.P.sp_op[j] = ops[j];
+      }
+    } else {
+      for (size_t m : marked) B.ins[m].flags &= 0x00FFFFFFu;
+    }
+  }
+  B.P.count_slots = 0;
+  for (int a = 0; a < MAX_AGG_SLOTS; a++) B.P.arg_slots[a] = 0;
+  for (size_t s = 0; s < A->slots.size(); s++) {
+    if (B.P.acc[s].arg < 0) B.P.count_slots |= 1u << s;
+    else B.P.arg_slots[B.P.acc[s].arg] |= 1u << s;
+  }
 }
 
 static AccSlotList slot_kinds(const ml_hashagg* A) {
@@ -1390,7 +1454,24 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
     FlowLaunch F;
     int grid, block;
     size_t lds;
-    if (plan_flow(B, SINK_DIRECT_AGG, sink_read_stages(B), (size_t)ncells * 8, F)) {
+    // copies of the cells for the flow kernel (each lane adds to copy lane mod
+    // copies): as many as fit 32 KB, or 8 KB when the pipeline probes (its
+    // queues and registers bound occupancy, and few rows reach the sink)
+    bool probes = false;
+    for (int st = 0; st < B.P.nsteps; st++) probes = probes || B.P.steps[st].kind == STEP_PROBE;
+    const int cstride = ncells | 1;
+    const int budget = probes ? 8192 : 32768;
+    int copies = 64;
+    while (copies > 1 && (size_t)copies * cstride * 8 > (size_t)budget) copies >>= 1;
+    bool flow = false;
+    for (; copies >= 1 && !flow; copies >>= 1) {
+      B.P.direct_copies = copies;
+      B.P.direct_stride = cstride;
+      flow = plan_flow(B, SINK_DIRECT_AGG, sink_read_stages(B), (size_t)copies * cstride * 8, F,
+                       B.P.src_batch != 0);
+      if (!flow && B.P.nseg == 0 && (B.expanding || B.P.expand_step >= 0)) break;
+    }
+    if (flow) {
       grid = F.grid;
       block = FLOW_BLOCK;
       lds = F.lds;

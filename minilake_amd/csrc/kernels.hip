@@ -128,6 +128,102 @@ struct Pf {
   uint64_t v0 = 0, v1 = 0;
 };
 
+// Column value of a load kind (OP_LOAD_*) at a row.
+__device__ __forceinline__ uint64_t load_col(uint32_t lop, uint64_t imm, uint64_t row) {
+  if (lop == OP_LOAD_I32) return sx32((uint32_t)gp<int32_t>((const void*)imm)[row]);
+  if (lop == OP_LOAD_U32) return (uint64_t)gp<uint32_t>((const void*)imm)[row];
+  if (lop == OP_LOAD_I64) return gp<uint64_t>((const void*)imm)[row];
+  return gp<uint8_t>((const void*)imm)[row] != 0;
+}
+
+// OP_CMPC: column value x compared with the constant i2.
+__device__ __forceinline__ uint64_t cmpc(uint32_t flags, uint64_t x, uint64_t i2) {
+  const uint32_t c = (flags >> 8) & 7u;
+  bool r;
+  if (flags & 0x800u) {
+    const double a = u2f(x), b = u2f(i2);
+    r = c == 0 ? a == b : c == 1 ? a != b : c == 2 ? a < b : c == 3 ? a <= b : c == 4 ? a > b : a >= b;
+  } else {
+    const int64_t a = (int64_t)x, b = (int64_t)i2;
+    r = c == 0 ? a == b : c == 1 ? a != b : c == 2 ? a < b : c == 3 ? a <= b : c == 4 ? a > b : a >= b;
+  }
+  return r;
+}
+
+// Unary instruction on value a (validity in vres, may be cleared).
+__device__ __forceinline__ uint64_t un_op(uint32_t op, uint64_t a, uint64_t imm, uint32_t& vres) {
+  switch (op) {
+    case OP_NOT: return a ^ 1ull;
+    case OP_NEG_I32: return sx32(0u - (uint32_t)a);
+    case OP_NEG_I64: return 0ull - a;
+    case OP_NEG_F64: return a ^ 0x8000000000000000ull;
+    case OP_I2F: return f2u((double)(int64_t)a);
+    case OP_F2I64: return (uint64_t)sat_i64(u2f(a));
+    case OP_F2I32: return sx32((uint32_t)sat_i32(u2f(a)));
+    case OP_I64_TO_I32: return sx32((uint32_t)a);
+    case OP_LOOKUP_U8: return gp<uint8_t>((const void*)imm)[a];
+    case OP_LOOKUP_U64: return gp<uint64_t>((const void*)imm)[a];
+    case OP_KEY_F64:
+      if (a == 0x8000000000000000ull) return 0ull;
+      if (u2f(a) != u2f(a))
+        return imm ? NAN_KEY_TAG | (atomicAdd((unsigned long long*)imm, 1ull) & NAN_KEY_IDS) : CANONICAL_NAN;
+      return a;
+    case OP_JKEY_F64:
+      if (u2f(a) != u2f(a)) vres = 0;
+      return a == 0x8000000000000000ull ? 0ull : a;
+    default: return u2f(a) == 0.0 ? 0ull : a;  // OP_HASH_F64
+  }
+}
+
+// Binary instruction on (a, b) with validities va, vb; result validity in vres.
+__device__ __forceinline__ uint64_t bin_op(uint32_t op, uint64_t a, uint64_t b, uint32_t va, uint32_t vb,
+                                           uint32_t& vres) {
+  vres = va & vb;
+  switch (op) {
+    case OP_ADD_I32: return sx32((uint32_t)a + (uint32_t)b);
+    case OP_SUB_I32: return sx32((uint32_t)a - (uint32_t)b);
+    case OP_MUL_I32: return sx32((uint32_t)a * (uint32_t)b);
+    case OP_MOD_I32: {  // 0 and NULL for x % 0; wrapping_rem for -1
+      const int32_t x = (int32_t)a, y = (int32_t)b;
+      if (y == 0) vres = 0;
+      return (y == 0 || y == -1) ? 0ull : sx32((uint32_t)(x % y));
+    }
+    case OP_ADD_I64: return a + b;
+    case OP_SUB_I64: return a - b;
+    case OP_MUL_I64: return a * b;
+    case OP_MOD_I64: {
+      const int64_t x = (int64_t)a, y = (int64_t)b;
+      if (y == 0) vres = 0;
+      return (y == 0 || y == -1) ? 0ull : (uint64_t)(x % y);
+    }
+    case OP_ADD_F64: return f2u(u2f(a) + u2f(b));
+    case OP_SUB_F64: return f2u(u2f(a) - u2f(b));
+    case OP_MUL_F64: return f2u(u2f(a) * u2f(b));
+    case OP_DIV_F64:  // x / 0 is NULL (crates/minilake-exec/src/expr/eval.rs:260-270)
+      if (u2f(b) == 0.0) vres = 0;
+      return f2u(u2f(a) / u2f(b));
+    case OP_MOD_F64: return f2u(fmod(u2f(a), u2f(b)));
+    case OP_EQ_I: return (int64_t)a == (int64_t)b;
+    case OP_NE_I: return (int64_t)a != (int64_t)b;
+    case OP_LT_I: return (int64_t)a < (int64_t)b;
+    case OP_LE_I: return (int64_t)a <= (int64_t)b;
+    case OP_GT_I: return (int64_t)a > (int64_t)b;
+    case OP_GE_I: return (int64_t)a >= (int64_t)b;
+    case OP_EQ_F: return u2f(a) == u2f(b);
+    case OP_NE_F: return u2f(a) != u2f(b);
+    case OP_LT_F: return u2f(a) < u2f(b);
+    case OP_LE_F: return u2f(a) <= u2f(b);
+    case OP_GT_F: return u2f(a) > u2f(b);
+    case OP_GE_F: return u2f(a) >= u2f(b);
+    case OP_AND:  // Kleene AND (crates/minilake-exec/src/kernels/boolean.rs:18-36)
+      vres = (va & vb) | (va & (uint32_t)(a == 0)) | (vb & (uint32_t)(b == 0));
+      return a & b;
+    default:  // OP_OR, Kleene (boolean.rs:40-58)
+      vres = (va & vb) | (va & (uint32_t)(a != 0)) | (vb & (uint32_t)(b != 0));
+      return a | b;
+  }
+}
+
 // Evaluate a program for one row; returns validity, value in `val`.
 __device__ __forceinline__ bool eval(const Code& ins, Prog p, const uint32_t (&rows)[MAX_STAGES], uint64_t& val,
                                      const Pf& pf = Pf{}) {
@@ -160,22 +256,10 @@ __device__ __forceinline__ bool eval(const Code& ins, Prog p, const uint32_t (&r
         uint64_t x;
         if (stage == 0 && pf.c0 && (const void*)imm == pf.c0) x = pf.v0;
         else if (stage == 0 && pf.c1 && (const void*)imm == pf.c1) x = pf.v1;
-        else if (lop == OP_LOAD_I32) x = sx32((uint32_t)gp<int32_t>((const void*)imm)[row]);
-        else if (lop == OP_LOAD_U32) x = (uint64_t)gp<uint32_t>((const void*)imm)[row];
-        else if (lop == OP_LOAD_I64) x = gp<uint64_t>((const void*)imm)[row];
-        else x = gp<uint8_t>((const void*)imm)[row] != 0;
+        else x = load_col(lop, imm, row);
         const uint64_t i2 = in.imm2;
         if (op == OP_CMPC) {
-          const uint32_t c = (flags >> 8) & 7u;
-          bool r;
-          if (flags & 0x800u) {
-            const double a = u2f(x), b = u2f(i2);
-            r = c == 0 ? a == b : c == 1 ? a != b : c == 2 ? a < b : c == 3 ? a <= b : c == 4 ? a > b : a >= b;
-          } else {
-            const int64_t a = (int64_t)x, b = (int64_t)i2;
-            r = c == 0 ? a == b : c == 1 ? a != b : c == 2 ? a < b : c == 3 ? a <= b : c == 4 ? a > b : a >= b;
-          }
-          res = r;
+          res = cmpc(flags, x, i2);
         } else if (op == OP_LOOKUPC) {
           res = gp<uint8_t>((const void*)i2)[x];
         } else {
@@ -185,90 +269,216 @@ __device__ __forceinline__ bool eval(const Code& ins, Prog p, const uint32_t (&r
       }
     } else if (op < OP_ADD_I32) {  // unary on the top slot
       dst = d - 1;
-      const uint64_t a = get_slot(r, dst);
       vres = (ok >> dst) & 1u;
-      switch (op) {
-        case OP_NOT: res = a ^ 1ull; break;
-        case OP_NEG_I32: res = sx32(0u - (uint32_t)a); break;
-        case OP_NEG_I64: res = 0ull - a; break;
-        case OP_NEG_F64: res = a ^ 0x8000000000000000ull; break;
-        case OP_I2F: res = f2u((double)(int64_t)a); break;
-        case OP_F2I64: res = (uint64_t)sat_i64(u2f(a)); break;
-        case OP_F2I32: res = sx32((uint32_t)sat_i32(u2f(a))); break;
-        case OP_I64_TO_I32: res = sx32((uint32_t)a); break;
-        case OP_LOOKUP_U8: res = gp<uint8_t>((const void*)imm)[a]; break;
-        case OP_LOOKUP_U64: res = gp<uint64_t>((const void*)imm)[a]; break;
-        case OP_KEY_F64:
-          if (a == 0x8000000000000000ull) res = 0ull;
-          else if (u2f(a) != u2f(a))
-            res = imm ? NAN_KEY_TAG | (atomicAdd((unsigned long long*)imm, 1ull) & NAN_KEY_IDS) : CANONICAL_NAN;
-          else res = a;
-          break;
-        case OP_JKEY_F64:
-          res = a == 0x8000000000000000ull ? 0ull : a;
-          if (u2f(a) != u2f(a)) vres = 0;
-          break;
-        default: res = u2f(a) == 0.0 ? 0ull : a; break;  // OP_HASH_F64
-      }
+      res = un_op(op, get_slot(r, dst), imm, vres);
     } else {  // binary: slots d-2 (left), d-1 (right) -> d-2
       dst = d - 2;
-      const uint64_t a = get_slot(r, dst), b = get_slot(r, d - 1);
-      const uint32_t va = (ok >> dst) & 1u, vb = (ok >> (d - 1)) & 1u;
-      vres = va & vb;
-      switch (op) {
-        case OP_ADD_I32: res = sx32((uint32_t)a + (uint32_t)b); break;
-        case OP_SUB_I32: res = sx32((uint32_t)a - (uint32_t)b); break;
-        case OP_MUL_I32: res = sx32((uint32_t)a * (uint32_t)b); break;
-        case OP_MOD_I32: {  // 0 and NULL for x % 0; wrapping_rem for -1
-          const int32_t x = (int32_t)a, y = (int32_t)b;
-          res = (y == 0 || y == -1) ? 0ull : sx32((uint32_t)(x % y));
-          if (y == 0) vres = 0;
-          break;
-        }
-        case OP_ADD_I64: res = a + b; break;
-        case OP_SUB_I64: res = a - b; break;
-        case OP_MUL_I64: res = a * b; break;
-        case OP_MOD_I64: {
-          const int64_t x = (int64_t)a, y = (int64_t)b;
-          res = (y == 0 || y == -1) ? 0ull : (uint64_t)(x % y);
-          if (y == 0) vres = 0;
-          break;
-        }
-        case OP_ADD_F64: res = f2u(u2f(a) + u2f(b)); break;
-        case OP_SUB_F64: res = f2u(u2f(a) - u2f(b)); break;
-        case OP_MUL_F64: res = f2u(u2f(a) * u2f(b)); break;
-        case OP_DIV_F64:  // x / 0 is NULL (crates/minilake-exec/src/expr/eval.rs:260-270)
-          res = f2u(u2f(a) / u2f(b));
-          if (u2f(b) == 0.0) vres = 0;
-          break;
-        case OP_MOD_F64: res = f2u(fmod(u2f(a), u2f(b))); break;
-        case OP_EQ_I: res = (int64_t)a == (int64_t)b; break;
-        case OP_NE_I: res = (int64_t)a != (int64_t)b; break;
-        case OP_LT_I: res = (int64_t)a < (int64_t)b; break;
-        case OP_LE_I: res = (int64_t)a <= (int64_t)b; break;
-        case OP_GT_I: res = (int64_t)a > (int64_t)b; break;
-        case OP_GE_I: res = (int64_t)a >= (int64_t)b; break;
-        case OP_EQ_F: res = u2f(a) == u2f(b); break;
-        case OP_NE_F: res = u2f(a) != u2f(b); break;
-        case OP_LT_F: res = u2f(a) < u2f(b); break;
-        case OP_LE_F: res = u2f(a) <= u2f(b); break;
-        case OP_GT_F: res = u2f(a) > u2f(b); break;
-        case OP_GE_F: res = u2f(a) >= u2f(b); break;
-        case OP_AND:  // Kleene AND (crates/minilake-exec/src/kernels/boolean.rs:18-36)
-          res = a & b;
-          vres = (va & vb) | (va & (uint32_t)(a == 0)) | (vb & (uint32_t)(b == 0));
-          break;
-        default:  // OP_OR, Kleene (boolean.rs:40-58)
-          res = a | b;
-          vres = (va & vb) | (va & (uint32_t)(a != 0)) | (vb & (uint32_t)(b != 0));
-          break;
-      }
+      res = bin_op(op, get_slot(r, dst), get_slot(r, d - 1), (ok >> dst) & 1u, (ok >> (d - 1)) & 1u, vres);
     }
     set_slot(r, dst, res);
     ok = (ok & ~(1u << dst)) | (vres << dst);
   }
   val = r[0];
   return (ok & 1u) != 0;
+}
+
+// The same program over 4 stage-0 rows per lane (src[u]). Each instruction
+// is decoded once for the 4 rows: one switch on its stack slot and one on its
+// opcode, with the rows handled inside the case, so the scalar unit (which
+// runs the interpreter's decode and branches for every wave of a CU) does the
+// same work for 4 rows as for one. The stack is 4 named vector registers
+// (no array, so nothing lands in scratch memory); the host runs a sink this
+// way only when every program it evaluates needs at most 4 slots
+// (Plan::src_batch). Column loads are issued for all 4 rows before use.
+typedef uint64_t u64x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+struct Stack4 {
+  u64x4 s0, s1, s2, s3;
+  u32x4 ok;  // bit k: slot k valid
+};
+
+// Slot selection by switch (whose cases copy named registers; branch trees
+// of ternaries are folded into selects of addresses, which puts the stack in
+// scratch memory); the opcode dispatch tests the hot cases first, since a
+// switch on a uniform value is lowered to a chain of compare-and-branch tests.
+__device__ __forceinline__ u64x4 st_get(const Stack4& S, int k) {
+  switch (k) {
+    case 0: return S.s0;
+    case 1: return S.s1;
+    case 2: return S.s2;
+    default: return S.s3;
+  }
+}
+__device__ __forceinline__ void st_get2(const Stack4& S, int k, u64x4& a, u64x4& b) {
+  switch (k) {
+    case 0: a = S.s0; b = S.s1; break;
+    case 1: a = S.s1; b = S.s2; break;
+    default: a = S.s2; b = S.s3; break;
+  }
+}
+__device__ __forceinline__ void st_set(Stack4& S, int k, const u64x4& v) {
+  switch (k) {
+    case 0: S.s0 = v; break;
+    case 1: S.s1 = v; break;
+    case 2: S.s2 = v; break;
+    default: S.s3 = v; break;
+  }
+}
+
+// The staged columns of a round (Plan::sp_col) for the lane's 4 rows.
+struct Staged8 {
+  u64x4 c0, c1, c2, c3, c4, c5, c6, c7;
+};
+__device__ __forceinline__ u64x4 load4(uint32_t lop, const void* col, const uint32_t (&src)[4]) {
+  if (lop == OP_LOAD_I64) {
+    const auto c = gp<uint64_t>(col);
+    return (u64x4){c[src[0]], c[src[1]], c[src[2]], c[src[3]]};
+  }
+  if (lop == OP_LOAD_I32) {
+    const auto c = gp<int32_t>(col);
+    return (u64x4){sx32((uint32_t)c[src[0]]), sx32((uint32_t)c[src[1]]), sx32((uint32_t)c[src[2]]),
+                   sx32((uint32_t)c[src[3]])};
+  }
+  if (lop == OP_LOAD_U32) {
+    const auto c = gp<uint32_t>(col);
+    return (u64x4){(uint64_t)c[src[0]], (uint64_t)c[src[1]], (uint64_t)c[src[2]], (uint64_t)c[src[3]]};
+  }
+  const auto c = gp<uint8_t>(col);
+  return (u64x4){(uint64_t)(c[src[0]] != 0), (uint64_t)(c[src[1]] != 0), (uint64_t)(c[src[2]] != 0),
+                 (uint64_t)(c[src[3]] != 0)};
+}
+// Every staged column's loads are issued before any value is used.
+__device__ __forceinline__ void stage_load(const Plan& P, const uint32_t (&src)[4], Staged8& T) {
+  const int n = P.sp_n;
+  if (n > 0) T.c0 = load4(P.sp_op[0], P.sp_col[0], src);
+  if (n > 1) T.c1 = load4(P.sp_op[1], P.sp_col[1], src);
+  if (n > 2) T.c2 = load4(P.sp_op[2], P.sp_col[2], src);
+  if (n > 3) T.c3 = load4(P.sp_op[3], P.sp_col[3], src);
+  if (n > 4) T.c4 = load4(P.sp_op[4], P.sp_col[4], src);
+  if (n > 5) T.c5 = load4(P.sp_op[5], P.sp_col[5], src);
+  if (n > 6) T.c6 = load4(P.sp_op[6], P.sp_col[6], src);
+  if (n > 7) T.c7 = load4(P.sp_op[7], P.sp_col[7], src);
+}
+__device__ __forceinline__ u64x4 staged(const Staged8& T, uint32_t j) {
+  switch (j) {
+    case 0: return T.c0;
+    case 1: return T.c1;
+    case 2: return T.c2;
+    case 3: return T.c3;
+    case 4: return T.c4;
+    case 5: return T.c5;
+    case 6: return T.c6;
+    default: return T.c7;
+  }
+}
+
+__device__ __forceinline__ void eval_src4(const Code& ins, Prog p, const uint32_t (&src)[4], const Staged8& T,
+                                          uint64_t (&val)[4], bool (&valid)[4]) {
+  Stack4 S;
+  S.s0 = S.s1 = S.s2 = S.s3 = (u64x4)(0);
+  S.ok = (u32x4)(0xFFFFFFFFu);
+  const int first = uni(p.first), count = uni(p.count), nullable = uni(p.nullable);
+#pragma unroll 1
+  for (int pc = 0; pc < count; pc++) {
+    CInstr& in = ins.at(first + pc);
+    const uint64_t head = *(CU64*)&in;
+    const uint32_t op = (uint32_t)head & 0xFFFFu;
+    const int d = (int)((head >> 16) & 0xFFu);
+    const uint32_t flags = (uint32_t)(head >> 32);
+    const uint64_t imm = in.imm;
+    u64x4 res;
+    u32x4 vres = (u32x4)(1u);
+    int dst;
+    if (op <= OP_LOOKUPC) {
+      dst = d;
+      if (op == OP_CONST) {
+        res = (u64x4)(imm);
+        vres = (u32x4)(flags & 1u);
+      } else {
+        const uint32_t lop = column_load_op(op, flags);
+        const uint64_t i2 = in.imm2;
+        const uint32_t sj = (flags >> 24) & 15u;
+        const u64x4 x = sj ? staged(T, sj - 1u) : load4(lop, (const void*)imm, src);
+        if (op == OP_CMPC) {
+          res = (u64x4){cmpc(flags, x[0], i2), cmpc(flags, x[1], i2), cmpc(flags, x[2], i2), cmpc(flags, x[3], i2)};
+        } else if (op == OP_LOOKUPC) {
+          const auto t = gp<uint8_t>((const void*)i2);
+          res = (u64x4){(uint64_t)t[x[0]], (uint64_t)t[x[1]], (uint64_t)t[x[2]], (uint64_t)t[x[3]]};
+        } else {
+          res = x;
+          if (nullable && i2) {
+            const auto vb = gp<uint8_t>((const void*)i2);
+            vres = (u32x4){(uint32_t)(vb[src[0]] != 0), (uint32_t)(vb[src[1]] != 0), (uint32_t)(vb[src[2]] != 0),
+                           (uint32_t)(vb[src[3]] != 0)};
+          }
+        }
+      }
+    } else if (op < OP_ADD_I32) {
+      dst = d - 1;
+      const u64x4 a = st_get(S, dst);
+      vres = (S.ok >> (uint32_t)dst) & 1u;
+      uint32_t v0 = vres[0], v1 = vres[1], v2 = vres[2], v3 = vres[3];
+#define ML_UN(OPC) res = (u64x4){un_op(OPC, a[0], imm, v0), un_op(OPC, a[1], imm, v1), un_op(OPC, a[2], imm, v2), un_op(OPC, a[3], imm, v3)};
+      if (op == OP_I2F) {
+        ML_UN(OP_I2F)
+      } else if (op == OP_NEG_F64) {
+        ML_UN(OP_NEG_F64)
+      } else {
+        switch (op) {
+#define ML_UNC(OPC) case OPC: ML_UN(OPC) break;
+          ML_UNC(OP_NOT) ML_UNC(OP_NEG_I32) ML_UNC(OP_NEG_I64) ML_UNC(OP_F2I64) ML_UNC(OP_F2I32)
+          ML_UNC(OP_I64_TO_I32) ML_UNC(OP_LOOKUP_U8) ML_UNC(OP_LOOKUP_U64) ML_UNC(OP_KEY_F64) ML_UNC(OP_JKEY_F64)
+          default: ML_UN(OP_HASH_F64) break;
+#undef ML_UNC
+        }
+      }
+#undef ML_UN
+      vres = (u32x4){v0, v1, v2, v3};
+    } else {
+      dst = d - 2;
+      u64x4 a, b;
+      st_get2(S, dst, a, b);
+      const u32x4 va = (S.ok >> (uint32_t)dst) & 1u, vb = (S.ok >> (uint32_t)(dst + 1)) & 1u;
+      uint32_t v0 = 1, v1 = 1, v2 = 1, v3 = 1;
+#define ML_BIN(OPC) res = (u64x4){bin_op(OPC, a[0], b[0], va[0], vb[0], v0), bin_op(OPC, a[1], b[1], va[1], vb[1], v1), bin_op(OPC, a[2], b[2], va[2], vb[2], v2), bin_op(OPC, a[3], b[3], va[3], vb[3], v3)};
+#define ML_BINC(OPC) case OPC: ML_BIN(OPC) break;
+      if (op == OP_MUL_F64) {
+        ML_BIN(OP_MUL_F64)
+      } else if (op == OP_SUB_F64) {
+        ML_BIN(OP_SUB_F64)
+      } else if (op == OP_ADD_F64) {
+        ML_BIN(OP_ADD_F64)
+      } else if (op < OP_EQ_I) {  // the other arithmetic
+        switch (op) {
+          ML_BINC(OP_ADD_I64) ML_BINC(OP_SUB_I64) ML_BINC(OP_MUL_I64) ML_BINC(OP_DIV_F64) ML_BINC(OP_ADD_I32)
+          ML_BINC(OP_SUB_I32) ML_BINC(OP_MUL_I32) ML_BINC(OP_MOD_I32) ML_BINC(OP_MOD_I64)
+          default: ML_BIN(OP_MOD_F64) break;
+        }
+      } else if (op < OP_EQ_F) {  // integer comparisons
+        switch (op) {
+          ML_BINC(OP_EQ_I) ML_BINC(OP_NE_I) ML_BINC(OP_LT_I) ML_BINC(OP_LE_I) ML_BINC(OP_GT_I)
+          default: ML_BIN(OP_GE_I) break;
+        }
+      } else {  // float comparisons, AND, OR
+        switch (op) {
+          ML_BINC(OP_EQ_F) ML_BINC(OP_NE_F) ML_BINC(OP_LT_F) ML_BINC(OP_LE_F) ML_BINC(OP_GT_F) ML_BINC(OP_GE_F)
+          ML_BINC(OP_AND)
+          default: ML_BIN(OP_OR) break;
+        }
+      }
+#undef ML_BINC
+#undef ML_BIN
+      vres = (u32x4){v0, v1, v2, v3};
+    }
+    st_set(S, dst, res);
+    S.ok = (S.ok & ~(1u << (uint32_t)dst)) | (vres << (uint32_t)dst);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    val[u] = S.s0[u];
+    valid[u] = (S.ok[u] & 1u) != 0;
+  }
 }
 
 __device__ __forceinline__ void set_word(uint64_t (&w)[MAX_KEYS + 1], int k, uint64_t v) {
@@ -427,6 +637,19 @@ __device__ __forceinline__ void atomic_max_f64(double* p, double v) {
     unsigned long long prev = atomicCAS(q, cur, (unsigned long long)__double_as_longlong(v));
     if (prev == cur) break;
     cur = prev;
+  }
+}
+
+// fold state b into state a
+__device__ __forceinline__ uint64_t acc_combine(int kind, uint64_t a, uint64_t b) {
+  switch (kind) {
+    case ACC_SUM_F64: return f2u(u2f(a) + u2f(b));
+    case ACC_SUM_I64: case ACC_COUNT: return a + b;
+    case ACC_MIN_I64: return (int64_t)b < (int64_t)a ? b : a;
+    case ACC_MAX_I64: return (int64_t)b > (int64_t)a ? b : a;
+    case ACC_MIN_F64: return u2f(b) < u2f(a) ? b : a;
+    case ACC_MAX_F64: return u2f(b) > u2f(a) ? b : a;
+    default: return a;
   }
 }
 
@@ -639,6 +862,64 @@ __device__ __forceinline__ void sink_direct_agg(const Plan& P, const Code& ins, 
   }
   for (int s = 0; s < P.nacc; s++)
     if (P.acc[s].arg < 0) atomicAdd((unsigned long long*)(st + s), 1ull);
+}
+
+// The direct aggregate for U source rows per lane (a level-0 round that ends
+// the pipeline): keys and arguments evaluated once per instruction for all U
+// rows, then each live row's states added to the lane's copy of the cells.
+template <int U>
+__device__ __forceinline__ void sink_direct_agg_src(const Plan& P, const Code& ins, const uint32_t (&src)[U],
+                                                    const Staged8& T, const bool (&alive)[U], uint64_t* cells) {
+  static_assert(U == 4, "the multi-row evaluator takes 4 rows per lane");
+  int64_t g[U];
+  uint64_t v[U];
+  bool ok[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) g[u] = 0;
+#pragma unroll 1
+  for (int k = 0; k < P.nkeys; k++) {
+    eval_src4(ins, P.keys[k], src, T, v, ok);
+    const int64_t stride = P.dim_stride[k], nullcode = P.dim_size[k] - 1;
+#pragma unroll
+    for (int u = 0; u < U; u++) g[u] += (ok[u] ? (int64_t)v[u] : nullcode) * stride;
+  }
+  const int nacc = P.nacc;
+#pragma unroll
+  for (int u = 0; u < U; u++) g[u] *= nacc;
+  // the slots an argument feeds, from the host's per-argument masks
+  auto apply = [&](uint32_t m, bool count) {
+    while (m) {
+      const int s = __builtin_ctz(m);
+      m &= m - 1u;
+      const int kind = count ? (int)ACC_COUNT : uni(P.acc[s].kind);
+      bool on[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) on[u] = alive[u] && (count || ok[u]);
+#define ML_ACC(K) _Pragma("unroll") for (int u = 0; u < U; u++) if (on[u]) acc_apply_lds(K, cells + g[u] + s, v[u]);
+      if (kind == ACC_SUM_F64) {
+        ML_ACC(ACC_SUM_F64)
+      } else if (kind == ACC_COUNT) {
+        ML_ACC(ACC_COUNT)
+      } else if (kind == ACC_SUM_I64) {
+        ML_ACC(ACC_SUM_I64)
+      } else if (kind == ACC_MIN_I64) {
+        ML_ACC(ACC_MIN_I64)
+      } else if (kind == ACC_MAX_I64) {
+        ML_ACC(ACC_MAX_I64)
+      } else if (kind == ACC_MIN_F64) {
+        ML_ACC(ACC_MIN_F64)
+      } else {
+        ML_ACC(ACC_MAX_F64)
+      }
+#undef ML_ACC
+    }
+  };
+#pragma unroll 1
+  for (int a = 0; a < P.nargs; a++) {
+    eval_src4(ins, P.args[a], src, T, v, ok);
+    apply(uni(P.arg_slots[a]), false);
+  }
+  apply(uni(P.count_slots), true);
 }
 
 // Materialize every column of the layout for one row at output position pos.
@@ -1093,14 +1374,23 @@ template <int SINK, int MODE>
 __global__ void __launch_bounds__(256, flow_waves<MODE>()) flow_kernel(Plan P) {
   extern __shared__ uint64_t lds[];
   const Code ins{(CInstr*)P.ins, P.nins - 1};
+  // direct aggregate: direct_copies copies of the cells, direct_stride words
+  // apart (odd, so lanes hit distinct LDS banks); lane l adds to copy
+  // l mod copies, so the lanes of a wave rarely add to the same word
   const int ncells = (SINK == SINK_DIRECT_AGG) ? P.ngroups * P.nacc : 0;
+  const int cstride = (SINK == SINK_DIRECT_AGG) ? P.direct_stride : 0;
+  const int ncopies = (SINK == SINK_DIRECT_AGG) ? P.direct_copies : 0;
   if constexpr (SINK == SINK_DIRECT_AGG) {
-    for (int c = threadIdx.x; c < ncells; c += blockDim.x) lds[c] = acc_init_bits(P.acc[c % P.nacc].kind);
+    for (int c = threadIdx.x; c < ncopies * cstride; c += blockDim.x) {
+      const int cc = c % cstride;
+      lds[c] = cc < ncells ? acc_init_bits(P.acc[cc % P.nacc].kind) : 0;
+    }
     __syncthreads();
   }
   const int lane = __lane_id();
   const int wv = threadIdx.x >> 6;
-  uint32_t* q = (uint32_t*)(lds + ncells) + wv * P.qwords;
+  uint64_t* cells = lds + (SINK == SINK_DIRECT_AGG ? (lane & (ncopies - 1)) * cstride : 0);
+  uint32_t* q = (uint32_t*)(lds + ncopies * cstride) + wv * P.qwords;
   uint64_t* ac = nullptr;
   if constexpr (SINK == SINK_HASH_AGG) {
     if (P.ac_n) {
@@ -1150,7 +1440,7 @@ __global__ void __launch_bounds__(256, flow_waves<MODE>()) flow_kernel(Plan P) {
     // rows that passed the last segment
     auto sink = [&](bool alive, const uint32_t (&rows)[MAX_STAGES]) {
       if constexpr (SINK == SINK_DIRECT_AGG) {
-        if (alive) sink_direct_agg(P, ins, rows, lds);
+        if (alive) sink_direct_agg(P, ins, rows, cells);
       } else if constexpr (SINK == SINK_HASH_AGG) {
         if (alive) {
           if (ac) sink_hash_agg_cached(P, ins, rows, ac);
@@ -1199,17 +1489,34 @@ __global__ void __launch_bounds__(256, flow_waves<MODE>()) flow_kernel(Plan P) {
       uint32_t src[FAST_U], bits[FAST_U];
       bool act[FAST_U];
       round_rows(P, next, stop, lane, src, act);
-      eval_atoms(P, src, bits);
+      bool batched = false;
+      if constexpr (SINK == SINK_DIRECT_AGG && FAST_U == 4) {
+        if (last == 0 && P.src_batch) {
+          // the sink evaluates all 4 rows of the lane at once, from columns
+          // requested together with the filter's before any is used
+          Staged8 T;
+          stage_load(P, src, T);
+          eval_atoms(P, src, bits);
+          bool alive[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) alive[u] = act[u] && ((P.atom_truth >> bits[u]) & 1u);
+          sink_direct_agg_src<4>(P, ins, src, T, alive, cells);
+          batched = true;
+        }
+      }
+      if (!batched) eval_atoms(P, src, bits);
       next = min(stop, next + FAST_U * 64);
+      if (!batched) {
 #pragma unroll
-      for (int u = 0; u < FAST_U; u++) {
-        uint32_t rows[MAX_STAGES];
+        for (int u = 0; u < FAST_U; u++) {
+          uint32_t rows[MAX_STAGES];
 #pragma unroll
-        for (int s2 = 0; s2 < MAX_STAGES; s2++) rows[s2] = 0;
-        rows[0] = src[u];
-        const bool alive = act[u] && ((P.atom_truth >> bits[u]) & 1u);
-        if (last == 0) sink(alive, rows);
-        else push(1, alive, rows);
+          for (int s2 = 0; s2 < MAX_STAGES; s2++) rows[s2] = 0;
+          rows[0] = src[u];
+          const bool alive = act[u] && ((P.atom_truth >> bits[u]) & 1u);
+          if (last == 0) sink(alive, rows);
+          else push(1, alive, rows);
+        }
       }
     } else if (MODE == FLOW_GENERIC && lvl == 0) {
       // FAST_U * 64 source rows; the stage-0 columns segment 0 reads are
@@ -1287,7 +1594,12 @@ __global__ void __launch_bounds__(256, flow_waves<MODE>()) flow_kernel(Plan P) {
   if constexpr (SINK == SINK_DIRECT_AGG) {
     __syncthreads();
     uint64_t* out = (uint64_t*)P.direct_partials + (uint64_t)blockIdx.x * ncells;
-    for (int c = threadIdx.x; c < ncells; c += blockDim.x) out[c] = lds[c];
+    for (int c = threadIdx.x; c < ncells; c += blockDim.x) {
+      const int kind = P.acc[c % P.nacc].kind;
+      uint64_t v = lds[c];
+      for (int k = 1; k < ncopies; k++) v = acc_combine(kind, v, lds[k * cstride + c]);
+      out[c] = v;
+    }
   }
 }
 
@@ -1332,18 +1644,6 @@ __global__ void __launch_bounds__(256) materialize_kernel(Plan P, const uint64_t
 // partials (thread t folds partials t, t+256, ... in order, then a pairwise
 // LDS tree), added to the running total of earlier batches. The shape depends
 // only on the partial count, so results are identical from run to run.
-__device__ __forceinline__ uint64_t acc_combine(int kind, uint64_t a, uint64_t b) {
-  switch (kind) {
-    case ACC_SUM_F64: return f2u(u2f(a) + u2f(b));
-    case ACC_SUM_I64: case ACC_COUNT: return a + b;
-    case ACC_MIN_I64: return (int64_t)b < (int64_t)a ? b : a;
-    case ACC_MAX_I64: return (int64_t)b > (int64_t)a ? b : a;
-    case ACC_MIN_F64: return u2f(b) < u2f(a) ? b : a;
-    case ACC_MAX_F64: return u2f(b) > u2f(a) ? b : a;
-    default: return a;
-  }
-}
-
 __global__ void __launch_bounds__(256) direct_tree_kernel(const uint64_t* partials, int nblocks, int ncells, int nacc,
                                                           AccSlotList accs, uint64_t* out) {
   __shared__ uint64_t buf[256];

@@ -293,12 +293,24 @@ struct Plan {
   AccSlot acc[MAX_AGG_SLOTS];
   int32_t nargs;
   Prog args[MAX_AGG_SLOTS];  // argument programs
+  uint32_t arg_slots[MAX_AGG_SLOTS];  // bit s of arg_slots[a]: slot s accumulates argument a
+  uint32_t count_slots;      // bit s: slot s counts rows (no argument)
+  int32_t src_batch;         // 1: every key and argument program needs at most 4 stack
+                             // slots, so a level-0 sink may evaluate several rows at once
+  // stage-0 columns a batched level-0 sink reads, loaded for the round's rows
+  // before the round's filter runs; an instruction reading column j carries
+  // j + 1 in flags bits 24..27 (ignored by the one-row interpreter)
+  int32_t sp_n;
+  int32_t sp_op[8];          // OP_LOAD_* kind of column j
+  const void* sp_col[8];
   void* acc_data[MAX_AGG_SLOTS];  // SoA arrays (f64 or i64) of capacity slots
 
   // direct (small-domain) aggregate: key k contributes (word * dim_stride[k])
   int64_t dim_stride[MAX_KEYS];
   int64_t dim_size[MAX_KEYS];
   int32_t ngroups;
+  int32_t direct_copies;     // flow kernel: copies of the cells in LDS (a power of two)
+  int32_t direct_stride;     // words between copies (odd, >= ngroups * nacc)
   void* direct_partials;     // [grid][ngroups][nacc] 8-byte words
 
   // join build / collect: materialized columns
