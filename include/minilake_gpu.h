@@ -491,6 +491,44 @@ ml_status ml_topn_finalize(ml_topn* t, ml_result** out);
 void ml_topn_free(ml_topn* t);
 
 /* ------------------------------------------------------------------ */
+/* Per-thread sink states: LocalSink (crates/minilake-exec/src/pipeline.rs:70-79),
+ * as MiniLake's run_parallel drives them (scheduler.rs:102-144): each worker
+ * thread makes a local state on its own context with create_local, sinks its
+ * morsels into it (ml_*_sink / ml_pipeline_* with the local handle) without
+ * locking, and calls combine exactly once; combine merges the local into the
+ * global sink under the global's lock (from any thread) and frees the local.
+ * finalize on the global runs once, after every combine. A global's context
+ * is used by combine and finalize: its owner does not use it meanwhile.     */
+/* ------------------------------------------------------------------ */
+/* HashAggLocal::combine (aggregate/hash.rs:439-483): the local's groups as
+ * partial states, merged by the global's Final-mode merge. The global sink
+ * may also take batches itself; finalize then merges its own groups too. */
+ml_status ml_hashagg_create_local(ml_hashagg* global, ml_ctx* ctx, ml_hashagg** out);
+ml_status ml_hashagg_combine(ml_hashagg* global, ml_hashagg* local);
+/* JoinBuildLocal::combine (join.rs:205-226): the local's build rows are
+ * appended to the global's (the string dictionaries must be the same). */
+ml_status ml_join_build_create_local(ml_join_build* global, ml_ctx* ctx, ml_join_build** out);
+ml_status ml_join_build_combine(ml_join_build* global, ml_join_build* local);
+/* SortLocal::combine (sort.rs:253-268): the local's best rows merged into
+ * the global's best (a full sort merges everything). */
+ml_status ml_topn_create_local(ml_topn* global, ml_ctx* ctx, ml_topn** out);
+ml_status ml_topn_combine(ml_topn* global, ml_topn* local);
+
+/* LimitSink (operators/limit.rs:26-117). Its local state holds nothing, so
+ * the one handle is shared: ml_limit_sink may be called from several threads
+ * at once, each with its own context (the batch is compacted there); rows are
+ * kept in arrival order until offset + limit are held, after which
+ * ml_limit_is_finished (LocalSink::is_finished, limit.rs:114-116) is 1 and
+ * further batches are ignored. finalize returns rows [offset, offset + limit)
+ * of what was kept (no columns if no batch arrived). */
+typedef struct ml_limit ml_limit;
+ml_status ml_limit_create(ml_ctx* ctx, int64_t limit, int64_t offset, ml_limit** out);
+ml_status ml_limit_sink(ml_limit* s, ml_ctx* ctx, const ml_batch* batch);
+int32_t ml_limit_is_finished(const ml_limit* s);
+ml_status ml_limit_finalize(ml_limit* s, ml_result** out);
+void ml_limit_free(ml_limit* s);
+
+/* ------------------------------------------------------------------ */
 /* Batched fragments: many independent
  *   SELECT g, SUM(v), COUNT(*) FROM probe JOIN build USING (key) GROUP BY g
  * fragments (g = the group id of the matching build row) in one persistent-

@@ -1,10 +1,12 @@
 // Host side of the gfx950 execution path: context, device memory, pipeline
 // planning and the sinks behind the C ABI (include/minilake_gpu.h).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string_view>
 #include <unordered_map>
@@ -773,8 +775,14 @@ struct ml_hashagg {
   };
   std::vector<std::vector<SpillBatch>> spill;  // per partition, the spilled state batches
   std::shared_ptr<mlg::MemPool> pool;  // the budget `reserved` is held against (outlives the context)
+  // LocalSink::combine (hash.rs:439-483): the merged states of combined
+  // per-thread aggregates, and the lock combine takes
+  ml_hashagg* merger = nullptr;
+  bool states_out = false;        // a spilled aggregate's merge emits states (it is being combined)
+  std::mutex combine_mu;
   ~ml_hashagg() {
     if (pool) pool->shrink(reserved);
+    delete merger;
   }
 };
 
@@ -1797,7 +1805,7 @@ static ml_result* agg_merge_spilled(ml_hashagg* A) {
     ml_hashagg* M = nullptr;
     check(ml_hashagg_create(ctx, &spec, &M));
     std::unique_ptr<ml_hashagg, void (*)(ml_hashagg*)> own(M, ml_hashagg_free);
-    M->emit_states = A->mode == ML_AGG_MODE_PARTIAL;
+    M->emit_states = A->mode == ML_AGG_MODE_PARTIAL || A->states_out;
     M->no_spill = true;
     for (auto& sb : A->spill[p]) {
       std::vector<Buffer> bufs;
@@ -1855,11 +1863,15 @@ static ml_result* agg_merge_spilled(ml_hashagg* A) {
   return concat_results(ctx, parts);
 }
 
+static ml_result* agg_states(ml_hashagg* A);
+static void agg_merge_states(ml_hashagg* G, ml_result* st);
+
 }  // namespace mlg
 
 // ---------------------------------------------------------------- join build
 struct ml_join_build {
   ml_ctx* ctx = nullptr;
+  std::mutex combine_mu;  // JoinBuildLocal::combine (join.rs:224) appends under the sink's lock
   std::vector<ml_expr> keys_raw;
   std::vector<int> key_types;
   std::vector<std::vector<ml_expr_node>> node_store;
@@ -2423,6 +2435,7 @@ struct ml_topn {
   std::deque<std::string> str_store;
   bool schema_fixed = false;
   mlg::Materialized best;  // the best `limit` rows seen so far
+  std::mutex combine_mu;   // SortLocal::combine (sort.rs:253-268) merges under the sink's lock
 };
 
 namespace mlg {
@@ -2533,6 +2546,8 @@ static void topn_reduce(ml_topn* T, Materialized& m) {
   m.rows = m_out;
 }
 
+static void topn_merge(ml_topn* T, Materialized& m);
+
 static void topn_run(ml_topn* T, PipelineBuild& B) {
   Materialized m;
   run_collect(B, m, true, nullptr, 0);
@@ -2547,7 +2562,12 @@ static void topn_run(ml_topn* T, PipelineBuild& B) {
     fail(ML_ERR_INTERNAL, "TopN batches with different schemas");
   }
   topn_reduce(T, m);
-  // merge with the best rows so far: concatenate, then reduce again
+  topn_merge(T, m);
+}
+
+// Merge reduced rows m (the same schema) into the best rows so far:
+// concatenate, then reduce again.
+static void topn_merge(ml_topn* T, Materialized& m) {
   Materialized& b = T->best;
   if (b.rows == 0) {
     b = std::move(m);
@@ -3009,7 +3029,17 @@ ml_status ml_pipeline_hashagg(ml_ctx* ctx, const ml_batch* source, const ml_oper
 }
 
 ml_status ml_hashagg_finalize(ml_hashagg* agg, ml_result** out) {
-  return guarded(agg->ctx, [&] { *out = agg->spilled ? agg_merge_spilled(agg) : agg_finalize(agg, false); });
+  return guarded(agg->ctx, [&] {
+    if (agg->merger) {  // combined per-thread states (and this sink's own groups)
+      std::lock_guard<std::mutex> lock(agg->combine_mu);
+      std::unique_ptr<ml_result> own(agg_states(agg));
+      if (own) agg_merge_states(agg, own.get());
+      ml_hashagg* M = agg->merger;
+      *out = M->spilled ? agg_merge_spilled(M) : agg_finalize(M, false);
+      return;
+    }
+    *out = agg->spilled ? agg_merge_spilled(agg) : agg_finalize(agg, false);
+  });
 }
 
 ml_status ml_ctx_set_memory_limit(ml_ctx* ctx, int64_t bytes, int32_t spill) {
@@ -3524,5 +3554,259 @@ ml_status ml_topn_finalize(ml_topn* t, ml_result** out) {
 }
 
 void ml_topn_free(ml_topn* t) { delete t; }
+
+// ------------------------------------------------------ per-thread sinks
+// LocalSink (crates/minilake-exec/src/pipeline.rs:70-79): create_local makes
+// a sink state on the calling thread's context; combine merges it into the
+// global sink under the global's lock and frees it.
+
+}  // extern "C"
+
+namespace mlg {
+
+// The specification a hash aggregate was created with (its copied expressions).
+static void agg_spec_of(ml_hashagg* A, std::vector<ml_aggregate>& aggs, ml_agg_spec& spec) {
+  const size_t nk = A->key_types.size();
+  aggs.assign(A->user_aggs.size(), ml_aggregate{});
+  for (size_t a = 0; a < aggs.size(); a++) {
+    aggs[a].func = A->user_aggs[a].func;
+    aggs[a].arg_type = A->user_aggs[a].arg_type;
+    if (A->mode != ML_AGG_MODE_FINAL && aggs[a].func != ML_AGG_COUNT_STAR) aggs[a].arg = *A->aggs[a].raw;
+  }
+  spec = ml_agg_spec{};
+  spec.group_exprs = A->group_raw.data();
+  spec.key_types = A->key_types.data();
+  spec.num_keys = (int32_t)nk;
+  spec.num_aggs = (int32_t)aggs.size();
+  spec.aggs = aggs.data();
+  spec.mode = A->mode;
+  spec.sizing = A->sizing;
+  spec.capacity_hint = A->sizing == ML_SIZING_PLAIN ? 0 : A->hint;
+}
+
+// The partial states of an aggregate's groups (Accumulator::state), also
+// after a spill; null when it never received a batch.
+static ml_result* agg_states(ml_hashagg* A) {
+  if (!A->planned) return nullptr;
+  if (A->spilled) {
+    A->states_out = true;
+    return agg_merge_spilled(A);
+  }
+  return agg_finalize(A, true);
+}
+
+// Sink a states result into the global's merger (a Final aggregate of the
+// same groups and aggregates, created at the first merge).
+static void agg_merge_states(ml_hashagg* G, ml_result* st) {
+  ml_ctx* ctx = G->ctx;
+  if (!G->merger) {
+    std::vector<ml_aggregate> aggs;
+    ml_agg_spec spec;
+    agg_spec_of(G, aggs, spec);
+    spec.mode = ML_AGG_MODE_FINAL;
+    ml_hashagg* M = nullptr;
+    const ml_status rc = ml_hashagg_create(ctx, &spec, &M);
+    if (rc != ML_OK) fail(rc, ctx->last_error);
+    M->emit_states = G->mode == ML_AGG_MODE_PARTIAL;
+    G->merger = M;
+  }
+  ml_batch b{};
+  b.columns = st->views.data();
+  b.num_columns = (int32_t)st->views.size();
+  b.num_rows = st->num_rows;
+  const ml_status rc = ml_hashagg_sink(G->merger, &b);
+  if (rc != ML_OK) fail(rc, ctx->last_error);
+}
+
+}  // namespace mlg
+
+// The result of a slice [start, start + len) of r's rows.
+static ml_result* slice_result(ml_ctx* ctx, ml_result* r, int64_t start, int64_t len) {
+  auto* o = new ml_result();
+  o->ctx = ctx;
+  o->device = ctx->device;
+  o->num_rows = len;
+  for (auto& c : r->cols) {
+    ResultColumn rc;
+    rc.type = c.type;
+    rc.dict = c.dict;
+    const int w = type_width(c.type);
+    rc.data = Buffer(ctx->arena, (size_t)std::max<int64_t>(len, 1) * w);
+    if (len > 0) d2d(ctx, rc.data.ptr, c.data.as<uint8_t>() + start * w, (size_t)len * w);
+    if (c.valid.ptr) {
+      rc.valid = Buffer(ctx->arena, (size_t)std::max<int64_t>(len, 1));
+      if (len > 0) d2d(ctx, rc.valid.ptr, c.valid.as<uint8_t>() + start, (size_t)len);
+    }
+    o->cols.push_back(std::move(rc));
+  }
+  o->finish_views();
+  hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+  return o;
+}
+
+// LimitSink (crates/minilake-exec/src/operators/limit.rs:26-117): compacted
+// batches in arrival order until offset + limit rows are held.
+struct ml_limit {
+  ml_ctx* ctx = nullptr;
+  int64_t limit = 0, offset = 0;
+  std::mutex mu;
+  std::atomic<bool> done{false};
+  int64_t held = 0;
+  std::vector<std::unique_ptr<ml_result>> parts;
+};
+
+extern "C" {
+
+ml_status ml_hashagg_create_local(ml_hashagg* global, ml_ctx* ctx, ml_hashagg** out) {
+  return guarded(ctx, [&] {
+    std::vector<ml_aggregate> aggs;
+    ml_agg_spec spec;
+    agg_spec_of(global, aggs, spec);
+    const ml_status rc = ml_hashagg_create(ctx, &spec, out);
+    if (rc != ML_OK) fail(rc, ctx->last_error);
+  });
+}
+
+ml_status ml_hashagg_combine(ml_hashagg* global, ml_hashagg* local) {
+  std::unique_ptr<ml_hashagg> own(local);
+  return guarded(global->ctx, [&] {
+    if (!local) return;
+    std::unique_ptr<ml_result> st(agg_states(local));  // on the local's context (synchronized)
+    if (!st) return;
+    std::lock_guard<std::mutex> lock(global->combine_mu);
+    agg_merge_states(global, st.get());
+  });
+}
+
+ml_status ml_join_build_create_local(ml_join_build* global, ml_ctx* ctx, ml_join_build** out) {
+  return guarded(ctx, [&] {
+    ml_join_build_spec spec{};
+    spec.keys = global->keys_raw.data();
+    spec.key_types = global->key_types.data();
+    spec.num_keys = (int32_t)global->key_types.size();
+    const ml_status rc = ml_join_build_create(ctx, &spec, out);
+    if (rc != ML_OK) fail(rc, ctx->last_error);
+  });
+}
+
+ml_status ml_join_build_combine(ml_join_build* global, ml_join_build* local) {
+  std::unique_ptr<ml_join_build> own(local);
+  return guarded(global->ctx, [&] {
+    if (!local) return;
+    hip_check(hipStreamSynchronize(local->ctx->stream), "synchronize");  // its chunks are complete
+    if (!local->schema_fixed) return;
+    std::lock_guard<std::mutex> lock(global->combine_mu);
+    if (!global->schema_fixed) {
+      global->col_types = local->col_types;
+      global->col_dicts = local->col_dicts;
+      global->key_dicts = local->key_dicts;
+      global->col_nullable = local->col_nullable;
+      global->schema_fixed = true;
+    } else {
+      if (global->col_types != local->col_types) fail(ML_ERR_INTERNAL, "join build batches with different schemas");
+      for (size_t c = 0; c < local->col_dicts.size(); c++)
+        if (!same_dict(global->col_dicts[c], local->col_dicts[c]))
+          fail(ML_ERR_UNSUPPORTED, "every build batch must share the string columns' dictionaries");
+      for (size_t k = 0; k < local->key_dicts.size(); k++)
+        if (!same_dict(global->key_dicts[k], local->key_dicts[k]))
+          fail(ML_ERR_UNSUPPORTED, "every build batch must share the key dictionaries");
+      for (size_t c = 0; c < local->col_nullable.size() && c < global->col_nullable.size(); c++)
+        global->col_nullable[c] = global->col_nullable[c] || local->col_nullable[c];
+    }
+    for (auto& ch : local->chunks) global->chunks.push_back(std::move(ch));
+    local->chunks.clear();
+  });
+}
+
+ml_status ml_topn_create_local(ml_topn* global, ml_ctx* ctx, ml_topn** out) {
+  return guarded(ctx, [&] {
+    std::vector<ml_sort_key> keys(global->keys_raw.size());
+    for (size_t k = 0; k < keys.size(); k++) {
+      keys[k] = ml_sort_key{};
+      keys[k].expr = global->keys_raw[k];
+      keys[k].asc = global->asc[k];
+      keys[k].nulls_first = global->nulls_first[k];
+    }
+    ml_sort_spec spec{};
+    spec.keys = keys.data();
+    spec.num_keys = (int32_t)keys.size();
+    spec.limit = global->limit;
+    const ml_status rc = ml_topn_create(ctx, &spec, out);
+    if (rc != ML_OK) fail(rc, ctx->last_error);
+  });
+}
+
+ml_status ml_topn_combine(ml_topn* global, ml_topn* local) {
+  std::unique_ptr<ml_topn> own(local);
+  return guarded(global->ctx, [&] {
+    if (!local) return;
+    hip_check(hipStreamSynchronize(local->ctx->stream), "synchronize");
+    if (!local->schema_fixed || local->best.rows == 0) return;
+    std::lock_guard<std::mutex> lock(global->combine_mu);
+    if (!global->schema_fixed) {
+      global->best.types = local->best.types;
+      global->best.dicts = local->best.dicts;
+      global->best.cols.resize(local->best.types.size());
+      global->best.valid.resize(local->best.types.size());
+      global->best.rows = 0;
+      global->schema_fixed = true;
+    } else if (global->best.types != local->best.types) {
+      fail(ML_ERR_INTERNAL, "TopN batches with different schemas");
+    }
+    topn_merge(global, local->best);
+    hip_check(hipStreamSynchronize(global->ctx->stream), "synchronize");
+  });
+}
+
+ml_status ml_limit_create(ml_ctx* ctx, int64_t limit, int64_t offset, ml_limit** out) {
+  return guarded(ctx, [&] {
+    if (limit < 0 || offset < 0) fail(ML_ERR_PLAN, "negative LIMIT or OFFSET");
+    auto* L = new ml_limit();
+    L->ctx = ctx;
+    L->limit = limit;
+    L->offset = offset;
+    *out = L;
+  });
+}
+
+ml_status ml_limit_sink(ml_limit* s, ml_ctx* ctx, const ml_batch* batch) {
+  return guarded(ctx, [&] {
+    if (s->done.load(std::memory_order_relaxed)) return;
+    const int64_t needed = s->limit > INT64_MAX - s->offset ? INT64_MAX : s->limit + s->offset;
+    ml_result* r = nullptr;  // Batch::compact on the caller's context
+    const ml_status rc = ml_pipeline_collect(ctx, batch, nullptr, 0, &r);
+    if (rc != ML_OK) fail(rc, ctx->last_error);
+    std::unique_ptr<ml_result> part(r);
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (s->held >= needed) return;
+    s->held += part->num_rows;
+    s->parts.push_back(std::move(part));
+    if (s->held >= needed) s->done.store(true, std::memory_order_relaxed);
+  });
+}
+
+int32_t ml_limit_is_finished(const ml_limit* s) { return s && s->done.load(std::memory_order_relaxed) ? 1 : 0; }
+
+ml_status ml_limit_finalize(ml_limit* s, ml_result** out) {
+  return guarded(s->ctx, [&] {
+    std::lock_guard<std::mutex> lock(s->mu);
+    *out = nullptr;
+    if (s->parts.empty()) {  // no batch: no rows and no columns
+      auto* r = new ml_result();
+      r->ctx = s->ctx;
+      r->device = s->ctx->device;
+      r->finish_views();
+      *out = r;
+      return;
+    }
+    std::unique_ptr<ml_result> all(concat_results(s->ctx, s->parts));
+    s->parts.clear();
+    const int64_t start = std::min(s->offset, all->num_rows);
+    const int64_t len = std::min(s->limit, all->num_rows - start);
+    *out = slice_result(s->ctx, all.get(), start, len);
+  });
+}
+
+void ml_limit_free(ml_limit* s) { delete s; }
 
 }  // extern "C"

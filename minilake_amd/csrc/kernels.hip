@@ -227,11 +227,28 @@ __device__ __forceinline__ uint64_t bin_op(uint32_t op, uint64_t a, uint64_t b, 
 // Evaluate a program for one row; returns validity, value in `val`.
 __device__ __forceinline__ bool eval(const Code& ins, Prog p, const uint32_t (&rows)[MAX_STAGES], uint64_t& val,
                                      const Pf& pf = Pf{}) {
+  const int first = uni(p.first), count = uni(p.count), nullable = uni(p.nullable);
+  if (count == 1) {
+    // a plain column load (most join keys, group keys and materialized
+    // columns): no decode loop
+    CInstr& in = ins.at(first);
+    const uint64_t head = *(CU64*)&in;
+    const uint32_t op = (uint32_t)head & 0xFFFFu;
+    if (op < OP_CONST) {
+      const int stage = (int)((head >> 24) & 0xFFu);
+      const uint64_t imm = in.imm;
+      const uint64_t row = row_of(rows, stage);
+      if (stage == 0 && pf.c0 && (const void*)imm == pf.c0) val = pf.v0;
+      else if (stage == 0 && pf.c1 && (const void*)imm == pf.c1) val = pf.v1;
+      else val = load_col(op, imm, row);
+      const uint64_t i2 = in.imm2;
+      return !(nullable && i2) || gp<uint8_t>((const void*)i2)[row] != 0;
+    }
+  }
   uint64_t r[MAX_DEPTH];
 #pragma unroll
   for (int i = 0; i < MAX_DEPTH; i++) r[i] = 0;
   uint32_t ok = 0xFFFFFFFFu;
-  const int first = uni(p.first), count = uni(p.count), nullable = uni(p.nullable);
 #pragma unroll 1
   for (int pc = 0; pc < count; pc++) {
     CInstr& in = ins.at(first + pc);

@@ -53,7 +53,10 @@ EXPORTED = [
     "ml_ctx_take_stats", "ml_fragments_create", "ml_fragments_run", "ml_fragments_run_graph",
     "ml_fragments_run_per_fragment", "ml_fragments_autotune", "ml_fragments_free", "ml_mlp_create",
     "ml_mlp_get_params", "ml_mlp_forward", "ml_mlp_loss_grad", "ml_mlp_train_step", "ml_mlp_free",
-    "ml_sizing_features", "ml_ctx_set_estimator", "ml_filter_noalloc",
+    "ml_sizing_features", "ml_ctx_set_estimator", "ml_filter_noalloc", "ml_hashagg_create_local",
+    "ml_hashagg_combine", "ml_join_build_create_local", "ml_join_build_combine", "ml_topn_create_local",
+    "ml_topn_combine", "ml_limit_create", "ml_limit_sink", "ml_limit_is_finished", "ml_limit_finalize",
+    "ml_limit_free",
 ]
 
 
@@ -89,6 +92,17 @@ def lib():
             "ml_expr_type": ([vp, P(abi.MlExpr), P(abi.MlBatch), P(i32)], i32),
             "ml_filter": ([vp, P(abi.MlExpr), P(abi.MlBatch), vp, P(i64)], i32),
             "ml_filter_noalloc": ([vp, P(abi.MlExpr), P(abi.MlBatch), vp, P(i64)], i32),
+            "ml_hashagg_create_local": ([vp, vp, P(vp)], i32),
+            "ml_hashagg_combine": ([vp, vp], i32),
+            "ml_join_build_create_local": ([vp, vp, P(vp)], i32),
+            "ml_join_build_combine": ([vp, vp], i32),
+            "ml_topn_create_local": ([vp, vp, P(vp)], i32),
+            "ml_topn_combine": ([vp, vp], i32),
+            "ml_limit_create": ([vp, i64, i64, P(vp)], i32),
+            "ml_limit_sink": ([vp, vp, P(abi.MlBatch)], i32),
+            "ml_limit_is_finished": ([vp], i32),
+            "ml_limit_finalize": ([vp, P(vp)], i32),
+            "ml_limit_free": ([vp], None),
             "ml_project": ([vp, P(abi.MlExpr), i32, P(abi.MlBatch), P(vp)], i32),
             "ml_hash_columns": ([vp, P(abi.MlExpr), i32, P(abi.MlBatch), vp], i32),
             "ml_hashagg_create": ([vp, P(abi.MlAggSpec), P(vp)], i32),
