@@ -243,6 +243,8 @@ def main():
     ap.add_argument("--bloom", type=int, default=-1)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain sizing comparison")
     ap.add_argument("--no-extra", action="store_true", help="only the star join (no other query shapes, fragments)")
+    ap.add_argument("--extra", default="filter_groupby,join_topn,fragments",
+                    help="comma-separated extra workloads to run after the star join")
     ap.add_argument("--learned-rows", type=int, default=60_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -296,10 +298,13 @@ def main():
     torch.cuda.empty_cache()
 
     extra = {}
-    if not args.no_extra:
+    wanted = set() if args.no_extra else {x for x in args.extra.split(",") if x}
+    if "filter_groupby" in wanted:
         extra["filter_groupby"] = bench_filter_groupby(eng, torch, W, Q, stats, args.steps, args.warmup, args.seed)
+    if "join_topn" in wanted:
         extra["join_topn"] = bench_join_topn(eng, torch, W, Q, stats, args.steps, args.warmup, args.seed)
         torch.cuda.empty_cache()
+    if "fragments" in wanted:
         extra["fragments"] = bench_fragments(eng, torch, args.seed)
         torch.cuda.empty_cache()
 

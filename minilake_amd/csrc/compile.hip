@@ -370,13 +370,11 @@ const uint8_t* Compiler::dict_predicate(const HostDict& d, int op, const std::st
   for (size_t k = 0; k < d.entries.size(); k++) per_entry[k] = compare_bytes(op, d.entries[k], lit);
   if (bump) {
     void* at = bump->take(per_entry.size());
-    hip_check(hipMemcpyAsync(at, per_entry.data(), per_entry.size(), hipMemcpyHostToDevice, ctx_->stream), "upload");
-    hip_check(hipStreamSynchronize(ctx_->stream), "sync");
+    ring_upload(ctx_, at, per_entry.data(), per_entry.size());
     return (const uint8_t*)at;
   }
   Buffer b(ctx_->arena, per_entry.size());
-  hip_check(hipMemcpyAsync(b.ptr, per_entry.data(), per_entry.size(), hipMemcpyHostToDevice, ctx_->stream), "upload");
-  hip_check(hipStreamSynchronize(ctx_->stream), "sync");
+  ring_upload(ctx_, b.ptr, per_entry.data(), per_entry.size());
   const uint8_t* p = b.as<uint8_t>();
   tables_.push_back(std::move(b));
   return p;
@@ -640,13 +638,11 @@ Prog Compiler::compile_hash_input(const NodePtr& n, bool* prehashed) {
     const uint64_t* tp;
     if (bump) {
       void* at = bump->take(hs.size() * 8);
-      hip_check(hipMemcpyAsync(at, hs.data(), hs.size() * 8, hipMemcpyHostToDevice, ctx_->stream), "upload");
-      hip_check(hipStreamSynchronize(ctx_->stream), "sync");
+      ring_upload(ctx_, at, hs.data(), hs.size() * 8);
       tp = (const uint64_t*)at;
     } else {
       Buffer b(ctx_->arena, hs.size() * 8);
-      hip_check(hipMemcpyAsync(b.ptr, hs.data(), hs.size() * 8, hipMemcpyHostToDevice, ctx_->stream), "upload");
-      hip_check(hipStreamSynchronize(ctx_->stream), "sync");
+      ring_upload(ctx_, b.ptr, hs.data(), hs.size() * 8);
       tp = b.as<uint64_t>();
       tables_.push_back(std::move(b));
     }

@@ -203,6 +203,10 @@ struct ml_ctx {
 
 namespace mlg {
 
+// Enqueue a host-to-device copy of n bytes through the context's pinned
+// upload ring: returns without waiting, and `src` may be freed at once.
+void ring_upload(ml_ctx* ctx, void* dst, const void* src, size_t n);
+
 // ------------------------------------------------------------ result column
 struct ResultColumn {
   int type = ML_TYPE_INT64;

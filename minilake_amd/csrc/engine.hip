@@ -101,6 +101,7 @@ static void upload(ml_ctx* ctx, void* dst, const void* src, size_t n) {
   hip_check(hipMemcpyAsync(dst, at, n, hipMemcpyHostToDevice, ctx->stream), "copy to device");
   ctx->ring_head += (n + 255) & ~(size_t)255;
 }
+void ring_upload(ml_ctx* ctx, void* dst, const void* src, size_t n) { upload(ctx, dst, src, n); }
 static void d2h(ml_ctx* ctx, void* dst, const void* src, size_t n) {
   const uint64_t seq = ctx->check_seq;
   if (n && ctx->pinned && n <= ml_ctx::PINNED_BYTES) {
@@ -533,6 +534,9 @@ static uint64_t run_flow_rows(PipelineBuild& B, int sink, const FlowLaunch& F, B
   B.P.sink = keep;
   launch_scan_counts(counts.as<uint32_t>(), F.nwaves, offsets.as<uint64_t>(), ctx->stream);
   launch_materialize(B.P, sink, offsets.as<uint64_t>(), F.nwaves, ctx->stream);
+  // both buffers are released on return: a later launch of B must not see them
+  B.P.tuples = nullptr;
+  B.P.wave_counts = nullptr;
   if (keep_offsets) {  // the caller reads the row count later (offsets[nwaves])
     *keep_offsets = std::move(offsets);
     return UINT64_MAX;
@@ -572,6 +576,7 @@ static uint64_t run_count(PipelineBuild& B) {
     Buffer offsets(ctx->arena, (size_t)(F.nwaves + 1) * 8);
     B.P.wave_counts = counts.as<uint32_t>();
     ctx->launch(B.P, F.grid, FLOW_BLOCK, F.lds);
+    B.P.wave_counts = nullptr;  // released on return
     launch_scan_counts(counts.as<uint32_t>(), F.nwaves, offsets.as<uint64_t>(), ctx->stream);
     uint64_t n = 0;
     d2h(ctx, &n, offsets.as<uint64_t>() + F.nwaves, 8);
@@ -592,17 +597,32 @@ static bool run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buf
   uint64_t* counters = ctx->counters.as<uint64_t>();
   // capacity: active rows, unless one probe can match several build rows
   uint64_t cap = (uint64_t)B.active_rows;
-  if (capacity_hint >= 0) {
-    cap = (uint64_t)capacity_hint;
-  } else if (B.expanding) {
+  // row-per-lane collect (below): per-thread output counts, their offsets
+  const int grid = B.grid(), block = block_of(ctx);
+  const int64_t nthreads = (int64_t)grid * block;
+  Buffer tcount, toff;
+  uint64_t hc[2] = {0, 0};
+  bool counted = false;
+  auto count_pass = [&] {  // per-thread counts into tcount, the total into hc
+    tcount = Buffer(ctx->arena, (size_t)nthreads * 4);
+    toff = Buffer(ctx->arena, (size_t)(nthreads + 1) * 8);
     B.P.sink = SINK_COUNT_PAIRS;
     B.P.out_count = counters;
     zero(ctx, counters, 16);
     B.upload();
-    ctx->launch(B.P, B.grid(), block_of(ctx), 0);
-    uint64_t n = 0;
-    d2h(ctx, &n, counters, 8);
-    cap = n;
+    B.P.wave_counts = tcount.as<uint32_t>();
+    ctx->launch(B.P, grid, block, 0);
+    B.P.wave_counts = nullptr;
+    d2h(ctx, hc, counters, 16);
+    counted = true;
+  };
+  if (capacity_hint >= 0) {
+    cap = (uint64_t)capacity_hint;
+  } else if (B.expanding) {
+    // the row-per-lane collect's counting pass sizes the output: the probe
+    // chain is walked twice (count, collect), not three times
+    count_pass();
+    cap = hc[0];
   }
   if (with_cols) {
     if ((int)B.layout.size() > MAX_COLS) fail(ML_ERR_UNSUPPORTED, "too many columns to materialize");
@@ -637,7 +657,7 @@ static bool run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buf
   B.P.out_overflow = (uint32_t*)(counters + 1);
   B.upload();
   FlowLaunch F;
-  if (cap >= (uint64_t)B.active_rows && plan_flow(B, SINK_ROWS, sink_read_stages(B), 0, F)) {
+  if (!counted && cap >= (uint64_t)B.active_rows && plan_flow(B, SINK_ROWS, sink_read_stages(B), 0, F)) {
     if (B.P.nsteps == 0) {  // no filter or probe: every active row is an output row
       Buffer offsets;
       run_flow_rows(B, SINK_COLLECT, F, &offsets);
@@ -651,17 +671,8 @@ static bool run_collect(PipelineBuild& B, Materialized& out, bool with_cols, Buf
   // a scan turns them into per-thread offsets, and the collect pass writes
   // every row at its thread's own cursor; one shared counter for all matches
   // would serialize an expanding probe whose matches are scattered over lanes
-  const int grid = B.grid(), block = block_of(ctx);
-  const int64_t nthreads = (int64_t)grid * block;
-  Buffer tcount(ctx->arena, (size_t)nthreads * 4), toff(ctx->arena, (size_t)(nthreads + 1) * 8);
-  zero(ctx, counters, 16);
-  B.P.sink = SINK_COUNT_PAIRS;
-  B.P.wave_counts = tcount.as<uint32_t>();
-  ctx->launch(B.P, grid, block, 0);
-  B.P.wave_counts = nullptr;
+  if (!counted) count_pass();
   B.P.sink = SINK_COLLECT;
-  uint64_t hc[2] = {0, 0};
-  d2h(ctx, hc, counters, 16);
   const bool two_pass = !(hc[1] & 2) && hc[0] <= cap;
   if (two_pass) {
     launch_scan_counts(tcount.as<uint32_t>(), nthreads, toff.as<uint64_t>(), ctx->stream);
@@ -2131,6 +2142,7 @@ void join_table_resolve(ml_join_table* t) {
     T.dense_keys = Buffer();
     T.bloom = Buffer();
     T.dev.dense = 0;
+    T.dev.dense_full = 0;
     T.dev.dense_bits = nullptr;
     T.dev.dense_rank = nullptr;
     T.dev.dense_keys = nullptr;
@@ -2220,13 +2232,13 @@ static void join_finalize_enqueue(ml_join_build* J, FinalizePrep& P, int64_t* pi
     auto& c = J->chunks[0];
     // at most 256 workgroups: each ends with one atomic on a shared counter
     const int nparts = (int)std::min<int64_t>(256, std::max<int64_t>(1, (c.cap + 4095) / 4096));
-    P.partials = Buffer(ctx->arena, (size_t)nparts * 16);
+    P.partials = Buffer(ctx->arena, (size_t)nparts * 24);
     P.mm = Buffer(ctx->arena, 32);
     // (also clears flags[0] for the presence marking and flags[1] for the permutation)
     launch_key_minmax(c.keys.as<uint64_t>(), (int)nk, c.key_null.as<uint8_t>(), c.cap, c.count_dev,
                       P.partials.as<int64_t>(), nparts, P.mm.as<int64_t>(),
                       (uint32_t*)(ctx->counters.as<uint64_t>() + ml_ctx::DONE_WORD), flags, ctx->stream);
-    hip_check(hipMemcpyAsync(pinned, P.mm.ptr, 24, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
+    hip_check(hipMemcpyAsync(pinned, P.mm.ptr, 32, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
     P.ranged = true;
   }
 }
@@ -2252,22 +2264,22 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
   // the later dense steps use flags[0..1] from zero (key_minmax cleared them,
   // unless other builds ran in between)
   if (P.batched) launch_fill_u64(ctx->counters.as<uint64_t>(), 0, 1, ctx->stream);
-  int64_t hm[3] = {0, -1, 0};
+  int64_t hm[4] = {0, -1, 0, 0};  // min, max, rows, non-NULL rows
   bool have_range = false;
   auto key_range = [&](const uint64_t* kw, const uint8_t* kn, int64_t rows_cap, const uint64_t* count_dev) {
     const int nparts = (int)std::min<int64_t>(256, std::max<int64_t>(1, (rows_cap + 4095) / 4096));
-    P.partials = Buffer(ctx->arena, (size_t)nparts * 16);
+    P.partials = Buffer(ctx->arena, (size_t)nparts * 24);
     P.mm = Buffer(ctx->arena, 32);
     launch_key_minmax(kw, (int)nk, kn, rows_cap, count_dev, P.partials.as<int64_t>(), nparts, P.mm.as<int64_t>(),
                       (uint32_t*)(ctx->counters.as<uint64_t>() + ml_ctx::DONE_WORD), flags, ctx->stream);
-    d2h(ctx, hm, P.mm.ptr, 24);
+    d2h(ctx, hm, P.mm.ptr, 32);
     have_range = true;
   };
   if (P.small) {
     auto& c = J->chunks[0];
     const int64_t* st = P.slot;
     c.rows = st[0];
-    if (st[3] == 0) {
+    if ((st[3] & 3) == 0) {
       T.nrows = st[0];
       T.cols = std::move(P.cols);
       T.col_valid = std::move(P.valid);
@@ -2276,6 +2288,7 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
       T.dense_keys = std::move(P.dkeys);
       T.dev.nkw = (int)nk;
       T.dev.dense = 1;
+      T.dev.dense_full = (st[3] & SMALL_DENSE_FULL) ? 1 : 0;
       T.dev.dense_min = st[1];
       T.dev.dense_range = (uint64_t)st[2];
       T.dev.dense_bits = T.dense_bits.as<uint64_t>();
@@ -2289,7 +2302,7 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
       return t;
     }
   } else if (P.ranged) {
-    std::memcpy(hm, P.slot, 24);
+    std::memcpy(hm, P.slot, 32);
     have_range = true;
     J->chunks[0].rows = hm[2];
   }
@@ -2386,7 +2399,11 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
           // flags[1] (next NULL-key row) was cleared by the min/max kernel
           launch_dense_permute(M, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(),
                                rank.as<uint32_t>(), rank.as<uint32_t>() + ne, flags + 1, ctx->stream);
-          if (nk > 1 && hashed_bloom(ctx, n)) {  // a word per 8 first-key values
+          // every key of the range present (all distinct, or the table is
+          // rebuilt hashed): a probe finds its row without bits or ranks, and
+          // a Bloom filter would cost the one load it saves
+          const bool full = hm[3] == (int64_t)R;
+          if (nk > 1 && !full && hashed_bloom(ctx, n)) {  // a word per 8 first-key values
             const uint64_t words = (R + (1u << DENSE_BLOOM_SHIFT) - 1) >> DENSE_BLOOM_SHIFT;
             T.bloom = Buffer(ctx->arena, words * 8);
             zero(ctx, T.bloom.ptr, T.bloom.bytes);
@@ -2409,6 +2426,7 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
           T.dev.dense_keys32 = keys32 ? 1 : 0;
           T.dev.dense_keys_sx = keys_sx;
           T.dev.dense = 1;
+          T.dev.dense_full = full ? 1 : 0;
           T.dev.dense_min = hm[0];
           T.dev.dense_range = R;
           T.dev.dense_bits = T.dense_bits.as<uint64_t>();

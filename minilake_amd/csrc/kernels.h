@@ -94,7 +94,8 @@ void launch_strided_u32(const uint32_t* src, int64_t stride, int64_t n, uint32_t
 // ranks (offsets[ne] = present keys); rows moved into key order
 // Key word r of these kernels is keys[r * nkw] (the first of a row's words).
 // Rows: n, or *n_dev when given (n then bounds it). mm[0..1] = min/max, mm[2] =
-// the row count; `done` is a zeroed counter word (left zeroed); clear[0..1] are set to 0
+// the row count, mm[3] = the rows with a non-NULL key; partials holds 3 words
+// per workgroup; `done` is a zeroed counter word (left zeroed); clear[0..1] are set to 0
 void launch_key_minmax(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, const uint64_t* n_dev,
                        int64_t* partials, int nparts, int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s);
 void launch_dense_mark(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, int64_t kmin,
@@ -121,7 +122,9 @@ void launch_dense_permute(const DensePermute& M, const uint64_t* keys, const uin
                           const uint32_t* present, uint32_t* null_next, hipStream_t s);
 // One-workgroup dense build of a small table (see small_dense_kernel); bits
 // and rank hold small_dense_words() and small_dense_words() + 1 entries;
-// status = {rows, min, range, flags (bit 0 repeated key, bit 1 not dense)}.
+// status = {rows, min, range, flags (bit 0 repeated key, bit 1 not dense,
+// SMALL_DENSE_FULL: dense and every key of the range present)}.
+constexpr int64_t SMALL_DENSE_FULL = 256;
 void launch_small_dense(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t cap,
                         const uint64_t* count_dev, uint64_t* bits, uint32_t* rank, int64_t* status, hipStream_t s);
 int small_dense_words();

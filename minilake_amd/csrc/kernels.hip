@@ -528,6 +528,10 @@ __device__ __forceinline__ bool bloom_maybe(const DevJoinTable& T, uint64_t h) {
 __device__ __forceinline__ bool dense_find(const DevJoinTable& T, uint64_t k, uint32_t& row) {
   const uint64_t idx = k - (uint64_t)T.dense_min;
   if (idx >= T.dense_range) return false;
+  if (T.dense_full) {
+    row = (uint32_t)idx;
+    return true;
+  }
   const uint64_t bits = gp<uint64_t>(T.dense_bits)[idx >> 6];
   const uint32_t b = (uint32_t)(idx & 63);
   if (!((bits >> b) & 1ull)) return false;
@@ -1213,22 +1217,29 @@ __device__ __forceinline__ void fast_probe(const Plan& P, const FastKeys& K, uin
 #pragma unroll
       for (int u = 0; u < FAST_U; u++) hit[u] = hit[u] && (word[u] & bits[u]) == bits[u];
     }
-    uint64_t idx[FAST_U], bw[FAST_U];
+    uint64_t idx[FAST_U];
 #pragma unroll
     for (int u = 0; u < FAST_U; u++) {
       idx[u] = k0[u] - (uint64_t)T.dense_min;
       hit[u] = hit[u] && idx[u] < T.dense_range;
-      bw[u] = gp<uint64_t>(T.dense_bits)[hit[u] ? (idx[u] >> 6) : 0];
     }
-    uint32_t rk[FAST_U];
+    if (T.dense_full) {  // every key of the range present: its row is its offset
 #pragma unroll
-    for (int u = 0; u < FAST_U; u++) {
-      hit[u] = hit[u] && ((bw[u] >> (idx[u] & 63)) & 1ull);
-      rk[u] = gp<uint32_t>(T.dense_rank)[hit[u] ? (idx[u] >> 6) : 0];
+      for (int u = 0; u < FAST_U; u++) brow[u] = (uint32_t)idx[u];
+    } else {
+      uint64_t bw[FAST_U];
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) bw[u] = gp<uint64_t>(T.dense_bits)[hit[u] ? (idx[u] >> 6) : 0];
+      uint32_t rk[FAST_U];
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++) {
+        hit[u] = hit[u] && ((bw[u] >> (idx[u] & 63)) & 1ull);
+        rk[u] = gp<uint32_t>(T.dense_rank)[hit[u] ? (idx[u] >> 6) : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < FAST_U; u++)
+        brow[u] = rk[u] + (uint32_t)__popcll(bw[u] & ((1ull << (idx[u] & 63)) - 1ull));
     }
-#pragma unroll
-    for (int u = 0; u < FAST_U; u++)
-      brow[u] = rk[u] + (uint32_t)__popcll(bw[u] & ((1ull << (idx[u] & 63)) - 1ull));
     if (two) {  // the second key word must match the row's
       uint64_t kb[FAST_U];
       if (T.dense_keys32) {
@@ -1479,12 +1490,18 @@ __global__ void __launch_bounds__(256, flow_waves<MODE>()) flow_kernel(Plan P) {
       // FAST_U * 64 source rows through the streaming probe; hits to level 1 in source order
       uint32_t brow[FAST_U];
       bool hit[FAST_U];
-#if ML_FAST_PREFETCH
+#if ML_FAST_PREFETCH == 1
       // this round's keys were requested by the previous level-0 round (or
       // before the loop); request the next round's before probing this one
       const FastKeys K = fk;
       next = min(stop, next + FAST_U * 64);
       if (next < stop) fast_keys(P, next, stop, lane, fk);
+#elif ML_FAST_PREFETCH == 2
+      // this round's keys were requested at the end of the previous level-0
+      // round (or before the loop); the next round's are requested below,
+      // after this round's table loads have returned
+      const FastKeys K = fk;
+      next = min(stop, next + FAST_U * 64);
 #else
       FastKeys K;
       fast_keys(P, next, stop, lane, K);
@@ -1501,6 +1518,13 @@ __global__ void __launch_bounds__(256, flow_waves<MODE>()) flow_kernel(Plan P) {
         set_row(rows, stage, brow[u]);
         push(1, hit[u], rows);
       }
+#if ML_FAST_PREFETCH == 2
+      // vector loads return in issue order: keys requested before the table
+      // loads would make every table wait for them, requested here they stay
+      // in flight through the deeper levels' rounds
+      __builtin_amdgcn_sched_barrier(0);
+      if (next < stop) fast_keys(P, next, stop, lane, fk);
+#endif
     } else if (MODE == FLOW_GENERIC && lvl == 0 && P.natoms) {
       // FAST_U * 64 source rows through a filter of column atoms
       uint32_t src[FAST_U], bits[FAST_U];
@@ -1754,17 +1778,18 @@ __global__ void join_unique_kernel(DevJoinTable T, uint32_t* dup_flag) {
 }
 
 // ------------------------------------------------- dense join table build
-// min / max of the non-NULL key words. Every workgroup writes its partial
-// pair; the last one to finish (counted in *done) folds them into mm[0] = min,
-// mm[1] = max, sets *done back to 0 for the next launch and clears the two
-// flag words clear[0..1] that the dense build uses next.
+// min / max and count of the non-NULL key words. Every workgroup writes its
+// partial triple; the last one to finish (counted in *done) folds them into
+// mm[0] = min, mm[1] = max, mm[2] = rows, mm[3] = non-NULL rows, sets *done
+// back to 0 for the next launch and clears the two flag words clear[0..1]
+// that the dense build uses next.
 __global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, int nkw, const uint8_t* key_null,
                                                          int64_t n, const uint64_t* n_dev, int64_t* partials,
                                                          int64_t* mm, uint32_t* done, uint32_t* clear) {
-  __shared__ int64_t lo[256], hi[256];
+  __shared__ int64_t lo[256], hi[256], cnt[256];
   __shared__ int last;
   if (n_dev) n = min(n, (int64_t)*n_dev);
-  int64_t a = INT64_MAX, b = INT64_MIN;
+  int64_t a = INT64_MAX, b = INT64_MIN, c = 0;
   // four independent rows per thread per step (loads issued before use)
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r0 < n; r0 += 4 * stride) {
@@ -1782,21 +1807,25 @@ __global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, i
       if (!ok[u]) continue;
       a = k[u] < a ? k[u] : a;
       b = k[u] > b ? k[u] : b;
+      c++;
     }
   }
   lo[threadIdx.x] = a;
   hi[threadIdx.x] = b;
+  cnt[threadIdx.x] = c;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) {
       lo[threadIdx.x] = min(lo[threadIdx.x], lo[threadIdx.x + o]);
       hi[threadIdx.x] = max(hi[threadIdx.x], hi[threadIdx.x + o]);
+      cnt[threadIdx.x] += cnt[threadIdx.x + o];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = lo[0];
-    partials[2 * blockIdx.x + 1] = hi[0];
+    partials[3 * blockIdx.x] = lo[0];
+    partials[3 * blockIdx.x + 1] = hi[0];
+    partials[3 * blockIdx.x + 2] = cnt[0];
     __threadfence();
     last = atomicAdd(done, 1u) == gridDim.x - 1;
   }
@@ -1805,17 +1834,21 @@ __global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, i
   __threadfence();
   a = INT64_MAX;
   b = INT64_MIN;
+  c = 0;
   for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
-    a = min(a, __hip_atomic_load(partials + 2 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    b = max(b, __hip_atomic_load(partials + 2 * i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    a = min(a, __hip_atomic_load(partials + 3 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    b = max(b, __hip_atomic_load(partials + 3 * i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    c += __hip_atomic_load(partials + 3 * i + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   lo[threadIdx.x] = a;
   hi[threadIdx.x] = b;
+  cnt[threadIdx.x] = c;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) {
       lo[threadIdx.x] = min(lo[threadIdx.x], lo[threadIdx.x + o]);
       hi[threadIdx.x] = max(hi[threadIdx.x], hi[threadIdx.x + o]);
+      cnt[threadIdx.x] += cnt[threadIdx.x + o];
     }
     __syncthreads();
   }
@@ -1823,6 +1856,7 @@ __global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, i
     mm[0] = lo[0];
     mm[1] = hi[0];
     mm[2] = n;
+    mm[3] = cnt[0];
     *done = 0;
     clear[0] = 0;
     clear[1] = 0;
@@ -2045,7 +2079,7 @@ __global__ void __launch_bounds__(1024) small_dense_kernel(DensePermute M, const
     status[0] = n;
     status[1] = kmin;
     status[2] = (int64_t)R;
-    status[3] = 0;
+    status[3] = (uint64_t)total == R ? SMALL_DENSE_FULL : 0;
   }
 }
 

@@ -167,8 +167,11 @@ struct DevJoinTable {
   // sign (bit j of dense_keys_sx) or zero. A Bloom filter can sit in front:
   // word i >> DENSE_BLOOM_SHIFT (a word per 8 consecutive first-key values,
   // so rows in key order set runs of one word), bits from the whole key's
-  // hash. Slots are unused.
+  // hash. Slots are unused. dense_full: every key of the range is present,
+  // so the row of k is i itself and probes read neither bits nor ranks (nor
+  // a Bloom filter: the further key words are one load away).
   int32_t dense = 0;
+  int32_t dense_full = 0;
   int64_t dense_min = 0;
   uint64_t dense_range = 0;
   const uint64_t* dense_bits = nullptr;  // one bit per key of the range

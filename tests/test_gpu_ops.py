@@ -789,3 +789,29 @@ def test_sort_total_order_nan_zero_strings_nulls(gpu_engine, asc0, nf0, asc1, nf
     for limit in (10, 200):
         top = rows_of(gpu_engine.topn(t, [], keys, limit).to_host())
         assert [kf(r) for r in top] == [kf(r) for r in want[:limit]]
+
+
+def test_collect_sized_expanding_probe_matches_oracle(gpu_engine, oracle_engine):
+    """ml_pipeline_collect_sized over a probe with duplicate build keys: a hint
+    below the true row count falls back to the exact (counted) collect, a hint
+    at or above it is used as is; the rows equal the oracle's collect either way
+    (learned sizing of a join output, join.rs:283-300 emits every match)."""
+    from minilake_amd.abi import probe_op
+    rng = np.random.default_rng(12)
+    bkeys = rng.integers(0, 2000, 6000)  # about 3 rows per key
+    build = HostTable([HostColumn(INT64, bkeys), HostColumn(INT32, np.arange(6000, dtype=np.int32))])
+    probe = HostTable([HostColumn(INT64, rng.integers(-50, 2100, 50_000)),
+                       HostColumn(FLOAT64, rng.normal(0, 1, 50_000))])
+    jo = oracle_engine.join_build(build, [], [col(0)], [INT64])
+    jg = gpu_engine.join_build(build, [], [col(0)], [INT64])
+    ops_o = [abi.probe_op(jo, [col(0)])]
+    ops_g = [abi.probe_op(jg, [col(0)])]
+    want = oracle_engine.collect(probe, ops_o).to_host()
+    exact = gpu_engine.collect(probe, ops_g).to_host()
+    assert_tables_equal(exact, want, rtol=0)
+    n = want.num_rows
+    assert n > probe.num_rows  # the probe expands
+    for hint, fallback in ((n // 3, True), (n - 1, True), (n, False), (2 * n, False)):
+        got, fell_back = gpu_engine.collect_sized(probe, ops_g, hint)
+        assert fell_back == fallback, hint
+        assert_tables_equal(got.to_host(), want, rtol=0)

@@ -41,3 +41,48 @@ def test_star_join_matches_oracle(gpu_engine, oracle_engine):
     dev = W.StarJoin(*[gpu_engine.table(t) for t in sj.tables()])
     top_g, _ = Q.run_q5(gpu_engine, dev)
     assert_tables_equal(top_g.to_host(), top_o.to_host(), rtol=RTOL, ordered=True)
+
+
+# ---------------------------------------------------------------- full sizes
+# The three query shapes at the sizes the benchmark runs them: the oracle runs
+# on the host's threads (a few seconds for the 77.5M-row star join).
+
+@pytest.fixture(scope="module")
+def wide_oracle():
+    import os
+    from oracle.oracle import OracleEngine
+    return OracleEngine(threads=max(1, min(16, len(os.sched_getaffinity(0)))))
+
+
+def test_filter_groupby_full_size_matches_oracle(gpu_engine, wide_oracle):
+    li = W.filter_groupby(seed=42)
+    assert li.num_rows == 6_001_215
+    want = Q.run_q1(wide_oracle, li).to_host()
+    got = Q.run_q1(gpu_engine, gpu_engine.table(li)).to_host()
+    assert got.num_rows == 6
+    assert_tables_equal(got, want, rtol=RTOL)
+
+
+def test_join_topn_full_size_matches_oracle(gpu_engine, wide_oracle):
+    d = W.join_topn(seed=42, scale=1.0)
+    top_o, (_, _, agg_o) = Q.run_q3(wide_oracle, d.customer, d.orders, d.lineitem)
+    top_g, (_, _, agg_g) = Q.run_q3(gpu_engine, gpu_engine.table(d.customer), gpu_engine.table(d.orders),
+                                   gpu_engine.table(d.lineitem))
+    ah, oh = agg_g.to_host(), agg_o.to_host()
+    assert 11_000 <= oh.num_rows <= 30_000  # the configured group count range
+    assert_tables_equal(ah, oh, rtol=RTOL)
+    tg, to = rows_of(top_g.to_host()), rows_of(top_o.to_host())
+    assert len(tg) == len(to) == 10
+    for a, b in zip(tg, to):
+        assert a[1] == b[1]
+        assert abs(a[3] - b[3]) <= RTOL * abs(b[3])
+
+
+def test_star_join_full_size_matches_oracle(gpu_engine, wide_oracle):
+    sj = W.star_join(seed=42, scale=10)
+    assert sj.lineitem.num_rows == 60_000_000
+    top_o, _ = Q.run_q5(wide_oracle, sj)
+    dev = W.StarJoin(*[gpu_engine.table(t) for t in sj.tables()])
+    top_g, _ = Q.run_q5(gpu_engine, dev)
+    del dev
+    assert_tables_equal(top_g.to_host(), top_o.to_host(), rtol=RTOL, ordered=True)
