@@ -231,6 +231,35 @@ def bench_fragments(eng, torch, seed, reps=5):
     return out
 
 
+# Key distributions for the sizing ablation: (rows, Zipf s (0 = uniform), key domain).
+# The first is the learned-sizing workload; in the others MiniLake's input/10
+# estimate is far off (too high, too low, too high, too low).
+ABLATION_WORKLOADS = [(1.0, 1.1, 100_000_000), (1.0, 0.0, 1_000_000), (0.5, 0.6, 1_000_000_000),
+                      (1.0, 1.4, 100_000_000), (1.0, 0.0, 100_000_000)]
+
+
+def bench_sizing_ablation(eng, torch, sizing, mlp, rows):
+    """Each sizing estimator (MiniLake's planner heuristic, the sample's GEE and
+    raw KMV estimates, the MLP, the exact counts) pre-sizing the same aggregate
+    and join on the same path, over key distributions where input/10 is right
+    and where it is far off."""
+    out = []
+    for frac, s, domain in ABLATION_WORKLOADS:
+        n = int(rows * frac)
+        nb = min(2_000_000, domain // 4)  # distinct build keys drawn from the same domain
+        fact, build = sizing.zipf_workload(eng, rows=n, s=s, domain=domain, build_keys=nb)
+        r = sizing.ablation(eng, mlp, fact, build, build_distinct=nb)
+        r["workload"] = (f"{n:,} int64 keys {'uniform' if s == 0 else f'Zipf({s:g})'} over {domain:.0e}; "
+                         f"join with {nb:,} keys x 2 rows")
+        legs = r["legs"]
+        log(f"ablation {r['workload']}: {r['true_groups']:,} groups; " +
+            ", ".join(f"{k} {v['ms']:.1f} ms (x{v['groups_ratio']:.2f})" for k, v in legs.items()))
+        out.append(r)
+        del fact, build
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +275,7 @@ def main():
     ap.add_argument("--extra", default="filter_groupby,join_topn,fragments",
                     help="comma-separated extra workloads to run after the star join")
     ap.add_argument("--learned-rows", type=int, default=60_000_000)
+    ap.add_argument("--no-ablation", action="store_true", help="skip the sizing-estimator ablation")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the star-join benchmark runs on one GPU; sharding over several GPUs "
@@ -321,7 +351,11 @@ def main():
             learned = sizing.compare_plain_learned(eng, mlp, fact, build, build_distinct=2_000_000, log=log)
             learned["workload"] = (f"{args.learned_rows:,} int64 keys Zipf(1.1) over 1e8, GROUP BY key COUNT(*), "
                                    f"SUM(key mod 1009); join with 2,000,000 keys x 2 rows")
-            del fact, build, mlp
+            del fact, build
+            torch.cuda.empty_cache()
+            if not args.no_ablation:
+                learned["ablation"] = bench_sizing_ablation(eng, torch, sizing, mlp, args.learned_rows)
+            del mlp
 
     cpu, parity = None, None
     if not args.no_cpu:

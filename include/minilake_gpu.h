@@ -344,9 +344,10 @@ enum { /* AggMode, aggregate/mod.rs:50-57 */
 enum { /* how the group table is sized */
   ML_SIZING_PLAIN = 0,   /* start small, double and rehash before a batch that could
                             exceed 7/8 load (swiss.rs:219-232 applied per device batch) */
-  ML_SIZING_HINT = 1,    /* pre-size from capacity_hint; overflow -> ML_ERR_CAPACITY */
-  ML_SIZING_LEARNED = 2  /* pre-size from the learned estimator; on overflow fall back
-                            to ML_SIZING_PLAIN automatically */
+  ML_SIZING_HINT = 1,    /* pre-size from capacity_hint (room for at least that many
+                            groups); more groups than that room -> ML_ERR_CAPACITY */
+  ML_SIZING_LEARNED = 2  /* pre-size from the learned estimator; more groups than the
+                            room it gave -> fall back to ML_SIZING_PLAIN automatically */
 };
 
 typedef struct ml_aggregate {

@@ -1521,6 +1521,7 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
     B.P.agg_mask = A->cap - 1;
     B.P.agg_wrap = A->wrap;
     B.P.agg_fill_limit = fill_limit;
+    B.P.agg_presized = fill_limit < A->cap ? 1 : 0;
     for (size_t s = 0; s < A->slots.size(); s++) B.P.acc_data[s] = A->acc[s].ptr;
     FlowLaunch F;
     // accumulator cache: as many entries (a power of two) as fit its budget,

@@ -727,11 +727,15 @@ __device__ __forceinline__ void acc_apply_lds(int kind, uint64_t* w, uint64_t v)
 // a reader compares keys only after its header load returned that header (the
 // key loads are issued under that branch), so the keys it reads from L2 are
 // the inserted ones.
+// Returns -1 when the row found no slot, -2 when an earlier row of a
+// pre-sized table already found none (the batch is rerun: nothing to record).
 __device__ __forceinline__ int64_t agg_slot(const Plan& P, const uint64_t (&w)[MAX_KEYS + 1], int nkw, uint64_t h) {
   const uint64_t hdr = (h & ~3ull) | 2ull;
   const uint64_t stride = 1 + (uint64_t)nkw;
   uint64_t pos = h & P.agg_mask;
   uint64_t probes = 0;
+  if (P.agg_presized && (__hip_atomic_load(P.agg_overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 2u))
+    return -2;
   while (probes <= P.agg_wrap) {
     uint64_t* s = P.agg_slots + pos * stride;
     uint64_t cur = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -739,7 +743,7 @@ __device__ __forceinline__ int64_t agg_slot(const Plan& P, const uint64_t (&w)[M
       const uint64_t prev = atomicCAS((unsigned long long*)s, 0ull, 1ull);
       if (prev == 0) {
         const uint64_t n = atomicAdd((unsigned long long*)P.agg_count, 1ull);
-        if (n >= P.agg_fill_limit) atomicOr(P.agg_overflow, 1u);  // past the load-factor limit
+        if (n >= P.agg_fill_limit) atomicOr(P.agg_overflow, P.agg_presized ? 3u : 1u);  // past the load-factor limit
 #pragma unroll
         for (int k = 0; k < MAX_KEYS + 1; k++)
           if (k < nkw) s[1 + k] = w[k];
@@ -787,7 +791,7 @@ __device__ __forceinline__ void sink_hash_agg(const Plan& P, const Code& ins, co
   const uint64_t h = key_hash(w, nkw);
   const int64_t slot = agg_slot(P, w, nkw, h);
   if (slot < 0) {  // table full: this row's contribution is lost, the caller must rerun
-    atomicOr(P.agg_overflow, 2u);
+    if (slot == -1) atomicOr(P.agg_overflow, 2u);
     return;
   }
   for (int a = 0; a < P.nargs; a++) {
@@ -823,7 +827,7 @@ __device__ __forceinline__ void sink_hash_agg_cached(const Plan& P, const Code& 
   const uint64_t h = key_hash(w, nkw);
   const int64_t slot = agg_slot(P, w, nkw, h);
   if (slot < 0) {
-    atomicOr(P.agg_overflow, 2u);
+    if (slot == -1) atomicOr(P.agg_overflow, 2u);
     return;
   }
   uint32_t* tags = (uint32_t*)(ac + (uint64_t)P.ac_n * P.nacc);
@@ -2547,21 +2551,27 @@ __global__ void __launch_bounds__(256) agg_part_chunk_kernel(PartAggPlan A, cons
                                                              const uint64_t* rkeys, const uint64_t* rargs,
                                                              const uint32_t* rvalid, int64_t n, uint64_t* partial) {
   extern __shared__ uint64_t lds[];
+  __shared__ uint32_t table_full;  // the batch is aggregated again: stop early
   const PartChunk C = chunks[blockIdx.x];
   const int S = A.S;
   uint64_t* hdr = lds;
   uint64_t* kw = lds + S;
   uint64_t* acc = kw + (uint64_t)S * A.nkw;
+  if (threadIdx.x == 0) table_full = 0;
   part_init(A, hdr, acc);
   bool full = false;
   for (uint32_t e = C.lo + threadIdx.x; e < C.hi; e += blockDim.x) {
+    if (__hip_atomic_load(&table_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
     uint64_t w[MAX_KEYS + 1];
 #pragma unroll
     for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < A.nkw ? rkeys[(uint64_t)k * n + e] : 0;
     const int pos = lds_group(hdr, kw, S, A.nkw, w, key_hash(w, A.nkw));
     if (pos < 0) {
+      // every later record would scan all S slots too: the overflow reruns
+      // the batch, so nothing more of this chunk is needed
       full = true;
-      continue;
+      __hip_atomic_store(&table_full, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      break;
     }
     const uint32_t vm = rvalid ? rvalid[e] : 0xFFFFFFFFu;
     for (int a = 0; a < A.nargs; a++) {
@@ -2588,10 +2598,12 @@ __global__ void __launch_bounds__(256) agg_part_chunk_kernel(PartAggPlan A, cons
 __global__ void __launch_bounds__(256) agg_part_merge_kernel(PartAggPlan A, const uint32_t* parts,
                                                              const uint32_t* first, const uint64_t* partial) {
   extern __shared__ uint64_t lds[];
+  __shared__ uint32_t table_full;  // the batch is aggregated again: stop early
   const int S = A.S;
   uint64_t* hdr = lds;
   uint64_t* kw = lds + S;
   uint64_t* acc = kw + (uint64_t)S * A.nkw;
+  if (threadIdx.x == 0) table_full = 0;
   part_init(A, hdr, acc);
   bool full = false;
   const size_t words = (size_t)S * (1 + A.nkw + A.nacc);
@@ -2600,6 +2612,7 @@ __global__ void __launch_bounds__(256) agg_part_merge_kernel(PartAggPlan A, cons
     const uint64_t* tk = t + S;
     const uint64_t* ta = tk + (uint64_t)S * A.nkw;
     for (int i = threadIdx.x; i < S; i += blockDim.x) {
+      if (__hip_atomic_load(&table_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
       if (t[i] == 0) continue;
       uint64_t w[MAX_KEYS + 1];
 #pragma unroll
@@ -2607,7 +2620,8 @@ __global__ void __launch_bounds__(256) agg_part_merge_kernel(PartAggPlan A, cons
       const int pos = lds_group(hdr, kw, S, A.nkw, w, key_hash(w, A.nkw));
       if (pos < 0) {
         full = true;
-        continue;
+        __hip_atomic_store(&table_full, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        break;
       }
       for (int s2 = 0; s2 < A.nacc; s2++) {
         const uint64_t v = ta[(uint64_t)s2 * S + i];

@@ -291,8 +291,14 @@ struct Plan {
   uint64_t agg_mask;
   uint64_t agg_wrap;         // linear probing wraps within agg_wrap + 1 slots (a partition or the table)
   uint64_t* agg_count;       // number of groups (device counter)
-  uint32_t* agg_overflow;    // set when the table passes its fill limit
+  uint32_t* agg_overflow;    // bit 0: the table passed its fill limit; bit 1: a row found no slot
   uint64_t agg_fill_limit;
+  // Pre-sized table (hint / learned sizing): passing the fill limit means the
+  // estimate was too small, which counts as a lost row (bit 1), and once that
+  // is set every later row stops at once (the caller reruns the batch or
+  // fails), so a far too small estimate never fills the table up to long
+  // probe chains. 0: a grow-and-rehash table (reserved before each batch).
+  uint64_t agg_presized;
   AccSlot acc[MAX_AGG_SLOTS];
   int32_t nargs;
   Prog args[MAX_AGG_SLOTS];  // argument programs

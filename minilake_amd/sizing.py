@@ -207,8 +207,8 @@ def labelled_table(engine, gen: KeyGen, seed: int, rows: int, domain: int, s: fl
     return x, np.array([math.log2(max(1, distinct)), math.log2(1 + join_rows)], dtype=np.float32)
 
 
-def training_set(engine, ntables: int, seed: int = 1, min_rows: float = 1e4, max_rows: float = 4e6,
-                 max_domain: float = 1e8, log=None):
+def training_set(engine, ntables: int, seed: int = 1, min_rows: float = 1e4, max_rows: float = 1e8,
+                 max_domain: float = 1e9, log=None):
     rng = np.random.default_rng(seed)
     gen = KeyGen(engine, int(max_rows))
     X = np.zeros((ntables, 32), np.float32)
@@ -221,7 +221,7 @@ def training_set(engine, ntables: int, seed: int = 1, min_rows: float = 1e4, max
         mult = int(rng.integers(1, 4))
         hot = bool(rng.integers(0, 2))
         X[t], Y[t] = labelled_table(engine, gen, seed * 1_000_003 + t, rows, domain, s, share, mult, hot)
-        if log and (t % 2000 == 0 or t == ntables - 1):
+        if log and (t % 500 == 0 or t == ntables - 1):
             log(f"training table {t}/{ntables}")
     return X, Y
 
@@ -244,6 +244,8 @@ def zipf_workload(engine, rows: int = 60_000_000, domain: int = 100_000_000, s: 
     `build_keys` random keys, each `mult` times, with payload b = key mod 7."""
     from .gpu import DeviceTable
     from .abi import INT64
+    if build_keys > domain:
+        raise ValueError(f"{build_keys} distinct build keys do not fit a key domain of {domain}")
     torch = engine.torch
     gen = KeyGen(engine, max(rows, build_keys * mult))
     keys = gen.probe(rows, seed, s, domain).clone()
@@ -285,6 +287,87 @@ def run_learned(engine, mlp: "Mlp", fact, build, build_distinct: int, margin: fl
                          sizing=abi.SIZING_LEARNED, capacity_hint=groups, stats=stats)
     joined, fell_back = engine.collect_sized(fact, [probe_op(jt, [col(0)])], out_rows)
     return agg, joined, y, fell_back
+
+
+def run_sized(engine, fact, build, jt, groups: int, join_rows: int, stats: Optional[dict] = None):
+    """The evaluation queries with the group table and the join output sized
+    from given estimates (plus margin), through the same path as run_learned:
+    a too-small estimate falls back to the exact path inside the library."""
+    from .abi import AGG_SUM, INT64, Agg, col, count_star, probe_op
+    agg = engine.hashagg(fact, [], [col(0)], [INT64], [count_star(), Agg(AGG_SUM, col(1), INT64)],
+                         sizing=abi.SIZING_LEARNED, capacity_hint=max(1, int(groups)), stats=stats)
+    joined, fell_back = engine.collect_sized(fact, [probe_op(jt, [col(0)])], max(1, int(join_rows)))
+    return agg, joined, fell_back
+
+
+# Estimators compared on the same sizing path: (name, needs the features pass,
+# f(features, fact rows, build rows) -> (log2 groups, log2 (1 + join rows))).
+def _heuristic(x, rows, build_rows):
+    """MiniLake's planner estimate (estimate_rows, physical_planner.rs:283-292):
+    an aggregate keeps 1/10 of its input, a join yields its larger input."""
+    return math.log2(max(1.0, rows / 10.0)), math.log2(1.0 + max(rows, build_rows))
+
+
+def _sample_sketch(x, rows, build_rows):
+    """The sample's own estimates, no model: GEE (sqrt(N/n) f1 + sum of f_j for
+    j >= 2, feature 15) for distinct keys, the sampled match rate scaled to the
+    table (feature 31) for the join."""
+    return float(x[15]), float(x[31])
+
+
+def _kmv_sample(x, rows, build_rows):
+    """The raw KMV sketch of the sampled keys (k = 256, feature 23): distinct
+    keys of the sample, not scaled to the table; the join as _sample_sketch."""
+    return float(x[23]), float(x[31])
+
+
+ESTIMATORS = [("reference_heuristic", False, _heuristic), ("sample_gee", True, _sample_sketch),
+              ("kmv_sample", True, _kmv_sample)]
+
+
+def ablation(engine, mlp: "Mlp", fact, build, build_distinct: int, reps: int = 3, margin: float = 1.15) -> dict:
+    """Time each estimator's sizing of the same two queries on the same path
+    (features and inference inside the timed region where they are used), the
+    MLP and exact sizing (the true counts) among them; every leg's results
+    must equal the plain run's."""
+    import time
+    from .abi import INT64, col
+    torch = engine.torch
+    ps = {}
+    agg_p, join_p = run_plain(engine, fact, build, stats=ps)
+    true_groups, true_join = agg_p.num_rows, join_p.num_rows
+    legs = list(ESTIMATORS) + [("mlp", True, None), ("exact", False, None)]
+    out = {"true_groups": true_groups, "true_join_rows": true_join, "legs": {}}
+    for name, needs_features, f in legs:
+        def one():
+            jt = engine.join_build(build, [], [col(0)], [INT64])
+            if name == "mlp":
+                y = predict(engine, mlp, fact, build.num_rows, build_distinct, jt)
+                g, j = float(y[0]), float(y[1])
+            elif name == "exact":
+                g, j = math.log2(max(1, true_groups)), math.log2(1 + true_join)
+            else:
+                x = features(engine, fact.tensors[0], fact.num_rows, jt, build.num_rows, build_distinct) \
+                    if needs_features else None
+                g, j = f(x, fact.num_rows, build.num_rows)
+            groups = capacity_from_log2(g, margin)
+            out_rows = max(1, capacity_from_log2(j, margin) - 1)
+            st = {}
+            agg, joined, fb = run_sized(engine, fact, build, jt, groups, out_rows, stats=st)
+            return agg, joined, fb, groups, out_rows, st
+        one()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            agg, joined, fb, groups, out_rows, st = one()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        out["legs"][name] = {"ms": sorted(ts)[len(ts) // 2], "groups_capacity": groups, "join_capacity": out_rows,
+                             "groups_ratio": groups / max(1, true_groups), "join_ratio": out_rows / max(1, true_join),
+                             "agg_resizes": st.get("resizes"), "join_fell_back": bool(fb),
+                             "identical": same_rows(agg, agg_p) and same_rows(joined, join_p)}
+    return out
 
 
 def same_rows(a, b) -> bool:

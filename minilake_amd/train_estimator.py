@@ -1,6 +1,6 @@
 """Train the learned-sizing network on the GPU and save its weights.
 
-  python -m minilake_amd.train_estimator --tables 20000 --epochs 300
+  python -m minilake_amd.train_estimator --tables 6000 --epochs 300
 
 Training pairs come from the plain engine (exact group and join counts over
 synthetic Zipf key columns generated in HBM); the network is trained with the
@@ -23,11 +23,12 @@ from . import sizing
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--tables", type=int, default=20000)
+    ap.add_argument("--tables", type=int, default=6000)
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--max-rows", type=float, default=4e6)
+    ap.add_argument("--min-rows", type=float, default=1e4)
+    ap.add_argument("--max-rows", type=float, default=1e8)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=sizing.WEIGHTS_PATH)
     ap.add_argument("--data-out", default="")
@@ -39,7 +40,8 @@ def main(argv=None):
 
     eng = GpuEngine()
     t0 = time.time()
-    X, Y = sizing.training_set(eng, args.tables, seed=args.seed, max_rows=args.max_rows, log=log)
+    X, Y = sizing.training_set(eng, args.tables, seed=args.seed, min_rows=args.min_rows, max_rows=args.max_rows,
+                               log=log)
     log(f"{args.tables} labelled tables in {time.time() - t0:.1f}s")
     if args.data_out:
         np.savez(args.data_out, X=X, Y=Y)

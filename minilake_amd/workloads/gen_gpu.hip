@@ -125,6 +125,7 @@ extern "C" {
 int mlw_gpu_zipf_keys(int64_t* out, int64_t n, uint64_t seed, uint64_t stream, double s, int64_t domain,
                       uint64_t perm_stream, void* stream_ptr) {
   if (n <= 0) return 0;
+  if (domain <= 0) return 1;
   int half;
   uint64_t mask;
   feistel_params((uint64_t)domain, &half, &mask);
@@ -135,10 +136,14 @@ int mlw_gpu_zipf_keys(int64_t* out, int64_t n, uint64_t seed, uint64_t stream, d
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-// out[j*mult + r] = scramble(j) for j < nkeys (each key `mult` times).
+// out[j*mult + r] = scramble(j) for j < nkeys (each key `mult` times). The
+// keys are distinct values of {0..domain-1}, so nkeys may not exceed domain
+// (the permutation is only defined on {0..domain-1}: a larger input would
+// never leave its cycle walk); such a request returns 1 without a launch.
 int mlw_gpu_build_keys(int64_t* out, int64_t nkeys, int64_t mult, uint64_t seed, uint64_t perm_stream, int64_t domain,
                        void* stream_ptr) {
   if (nkeys <= 0 || mult <= 0) return 0;
+  if (domain <= 0 || nkeys > domain) return 1;
   int half;
   uint64_t mask;
   feistel_params((uint64_t)domain, &half, &mask);

@@ -99,3 +99,34 @@ def test_learned_sizing_bit_identical_and_faster(gpu_engine):
     assert r["plain_resizes"] >= 1
     assert r["learned_resizes"] == 0
     assert r["learned_ms"] < r["plain_ms"]
+
+
+def test_learned_sizing_full_size_matches_oracle(gpu_engine):
+    """The learned-sizing evaluation at its configured size: 60M Zipf(1.1) keys
+    over 1e8 (about 6.2M groups) aggregated and joined with 2M keys x 2 rows.
+    Learned and plain sizing give bit-identical results; the aggregate (keys and
+    integer states exact) and the join's row count equal the oracle's."""
+    import os
+    from minilake_amd.abi import probe_op
+    from oracle.oracle import OracleEngine
+    params = sizing.load_params()
+    assert params is not None
+    mlp = sizing.Mlp(gpu_engine, params)
+    fact, build = sizing.zipf_workload(gpu_engine, rows=60_000_000)
+    agg_p, join_p = sizing.run_plain(gpu_engine, fact, build)
+    agg_l, join_l, _, _ = sizing.run_learned(gpu_engine, mlp, fact, build, build_distinct=2_000_000)
+    assert sizing.same_rows(agg_p, agg_l) and sizing.same_rows(join_p, join_l)
+    host_fact = HostTable([HostColumn(INT64, fact.tensors[0].cpu().numpy()),
+                           HostColumn(INT64, fact.tensors[1].cpu().numpy())])
+    host_build = HostTable([HostColumn(INT64, build.tensors[0].cpu().numpy()),
+                            HostColumn(INT64, build.tensors[1].cpu().numpy())])
+    oe = OracleEngine(threads=max(1, min(16, len(os.sched_getaffinity(0)))))
+    want = oe.hashagg(host_fact, [], [col(0)], [INT64], [Agg(AGG_COUNT_STAR, None, INT64),
+                                                         Agg(AGG_SUM, col(1), INT64)]).to_host()
+    got = agg_l.to_host()
+    assert got.num_rows == want.num_rows > 5_000_000
+    og, ow = np.argsort(got.columns[0].values), np.argsort(want.columns[0].values)
+    for c in range(3):
+        np.testing.assert_array_equal(got.columns[c].values[og], want.columns[c].values[ow])
+    jo = oe.join_build(host_build, [], [col(0)], [INT64])
+    assert join_l.num_rows == oe.count(host_fact, [probe_op(jo, [col(0)])])
