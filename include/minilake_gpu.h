@@ -59,7 +59,8 @@ enum {
   ML_TYPE_INT64 = 2,
   ML_TYPE_FLOAT64 = 3,
   ML_TYPE_DATE = 4, /* i32 days since 1970-01-01 */
-  ML_TYPE_UTF8 = 5, /* dictionary-encoded: u32 code per row + ml_dict */
+  ML_TYPE_UTF8 = 5, /* dictionary-encoded: u32 code per row + ml_dict; every code,
+                       NULL rows' included, is below the dictionary's len */
   ML_TYPE_NULL = -1 /* only for a NULL literal */
 };
 

@@ -728,7 +728,22 @@ struct OutDef {
   int finish;       // FIN_*
   int state;        // slot index
   int count;        // slot whose value > 0 means "valid" (-1: always valid)
+  int rank = -1;    // FIN_RANK: index of the aggregate's RankMap
 };
+
+// MIN/MAX of a dictionary string column (State::MinMaxStr, accumulator.rs:94,
+// 352-368): strings compare bytewise, so the accumulators fold the byte-order
+// rank of each row's entry (equal strings share a rank) and the output maps a
+// rank back to the code of an entry with that string, in the same dictionary.
+struct RankMap {
+  std::shared_ptr<HostDict> dict;
+  Buffer to_rank;                  // u64 per code (OP_LOOKUP_U64 table)
+  Buffer to_code;                  // u32 per rank
+  std::vector<uint32_t> host_code; // u32 per rank
+};
+
+// Slot "cast" of an argument folded as its dictionary entry's rank.
+constexpr int CAST_RANK = 1000;
 
 }  // namespace mlg
 
@@ -754,6 +769,8 @@ struct ml_hashagg {
   std::vector<int> slot_cast;        // accumulator input type of the slot's argument
   std::vector<mlg::OutDef> outs;
   std::vector<std::shared_ptr<mlg::HostDict>> key_dicts;
+  std::vector<mlg::RankMap> ranks;   // MIN/MAX of strings: one per argument dictionary
+  std::vector<int> agg_rank;         // per entry of aggs: its RankMap, or -1
   // direct
   int64_t ngroups = 0;
   std::vector<int64_t> dim_size, dim_stride;
@@ -825,7 +842,7 @@ static std::vector<int> agg_state_types(int func, int arg_type) {
     case ML_AGG_MIN: case ML_AGG_MAX:
       if (int_like(arg_type)) return {ML_TYPE_INT64};
       if (arg_type == ML_TYPE_FLOAT64) return {ML_TYPE_FLOAT64};
-      if (arg_type == ML_TYPE_UTF8) fail(ML_ERR_UNSUPPORTED, "min/max of strings");
+      if (arg_type == ML_TYPE_UTF8) return {ML_TYPE_UTF8};
       break;
     default: fail(ML_ERR_PLAN, "unknown aggregate function");
   }
@@ -838,6 +855,40 @@ static int agg_final_type(int func, int arg_type) {
   if (func == ML_AGG_AVG) return ML_TYPE_FLOAT64;
   if (func == ML_AGG_SUM) return arg_type == ML_TYPE_FLOAT64 ? ML_TYPE_FLOAT64 : ML_TYPE_INT64;
   return arg_type;
+}
+
+// The RankMap of dictionary d (built and uploaded once per aggregate).
+static int rank_map(ml_hashagg* A, const std::shared_ptr<HostDict>& d) {
+  for (size_t i = 0; i < A->ranks.size(); i++)
+    if (A->ranks[i].dict == d) return (int)i;
+  const auto& ent = d->entries;
+  std::vector<uint32_t> order(ent.size());
+  std::iota(order.begin(), order.end(), 0u);
+  // std::string compares as unsigned bytes, as Rust's [u8] ordering does
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return ent[a] < ent[b]; });
+  std::vector<uint64_t> rank(std::max<size_t>(ent.size(), 1), 0);
+  std::vector<uint32_t> code;
+  for (size_t r = 0; r < order.size(); r++) {
+    if (r == 0 || ent[order[r]] != ent[order[r - 1]]) code.push_back(order[r]);
+    rank[order[r]] = code.size() - 1;
+  }
+  if (code.empty()) code.push_back(0);
+  RankMap m;
+  m.dict = d;
+  m.to_rank = Buffer(A->ctx->arena, rank.size() * 8);
+  upload(A->ctx, m.to_rank.ptr, rank.data(), rank.size() * 8);
+  m.to_code = Buffer(A->ctx->arena, code.size() * 4);
+  upload(A->ctx, m.to_code.ptr, code.data(), code.size() * 4);
+  m.host_code = std::move(code);
+  A->ranks.push_back(std::move(m));
+  return (int)A->ranks.size() - 1;
+}
+
+// The dictionary of a string aggregate argument (a dictionary column).
+static std::shared_ptr<HostDict> arg_dict(PipelineBuild& B, const ml_expr* raw) {
+  NodePtr n = B.resolve(*raw);
+  if (n->kind != ML_EXPR_COLUMN || !n->col->dict) fail(ML_ERR_UNSUPPORTED, "min/max of a string needs a dictionary column");
+  return n->col->dict;
 }
 
 // Final mode (HashAggState::merge_partial, hash.rs:125-137; Accumulator::merge,
@@ -888,6 +939,15 @@ static void plan_final(ml_hashagg* A, PipelineBuild& B) {
         break;
       default: {  // MIN / MAX
         const bool mn = u.func == ML_AGG_MIN;
+        if (types[0] == ML_TYPE_UTF8) {  // a string state: fold its entry's rank
+          o.finish = FIN_RANK;
+          o.rank = rank_map(A, arg_dict(B, A->aggs[(size_t)sc].raw));
+          A->agg_rank[(size_t)sc] = o.rank;
+          o.state = add_slot(mn ? ACC_MIN_I64 : ACC_MAX_I64, sc, CAST_RANK);
+          o.count = nullable ? add_slot(ACC_COUNT, sc, CAST_RANK) : 0;
+          A->state_outs.push_back(o);
+          break;
+        }
         const bool f = types[0] == ML_TYPE_FLOAT64;
         if (!f) o.finish = u.arg_type == ML_TYPE_INT64 ? FIN_COPY : FIN_I32;
         o.state = add_slot(f ? (mn ? ACC_MIN_F64 : ACC_MAX_F64) : (mn ? ACC_MIN_I64 : ACC_MAX_I64), sc, types[0]);
@@ -910,6 +970,7 @@ static void plan_agg(ml_hashagg* A, PipelineBuild& B) {
   A->slots.clear();
   A->slot_cast.clear();
   A->outs.clear();
+  A->agg_rank.assign(A->aggs.size(), -1);
   A->slots.push_back({ACC_COUNT, -1});  // rows of the group
   A->slot_cast.push_back(-1);
   if (A->mode == ML_AGG_MODE_FINAL) {
@@ -968,10 +1029,16 @@ static void plan_agg(ml_hashagg* A, PipelineBuild& B) {
         } else if (d.arg_type == ML_TYPE_FLOAT64) {
           o.out_type = ML_TYPE_FLOAT64;
           o.state = add_slot(mn ? ACC_MIN_F64 : ACC_MAX_F64, ia, ML_TYPE_FLOAT64);
+        } else if (d.arg_type == ML_TYPE_UTF8) {
+          o.out_type = ML_TYPE_UTF8;
+          o.finish = FIN_RANK;
+          o.rank = rank_map(A, arg_dict(B, d.raw));
+          A->agg_rank[(size_t)ia] = o.rank;
+          o.state = add_slot(mn ? ACC_MIN_I64 : ACC_MAX_I64, ia, CAST_RANK);
         } else {
-          fail(ML_ERR_UNSUPPORTED, "min/max of strings");
+          fail(ML_ERR_PLAN, std::string("min/max(") + type_name(d.arg_type) + ") not supported");
         }
-        o.count = nullable ? add_slot(ACC_COUNT, ia, ML_TYPE_INT64) : 0;
+        o.count = nullable ? add_slot(ACC_COUNT, ia, o.finish == FIN_RANK ? CAST_RANK : ML_TYPE_INT64) : 0;
         break;
       }
       default: fail(ML_ERR_PLAN, "unknown aggregate function");
@@ -980,10 +1047,12 @@ static void plan_agg(ml_hashagg* A, PipelineBuild& B) {
     if (d.func == ML_AGG_AVG) {  // (sum, count), both always valid
       A->state_outs.push_back({ML_TYPE_FLOAT64, FIN_COPY, o.state, -1});
       A->state_outs.push_back({ML_TYPE_INT64, FIN_COPY, o.count, -1});
-    } else if (d.func == ML_AGG_MIN || d.func == ML_AGG_MAX) {  // integer states are Int64
+    } else if (d.func == ML_AGG_MIN || d.func == ML_AGG_MAX) {  // integer states are Int64, strings stay strings
       OutDef st = o;
-      st.finish = FIN_COPY;
-      if (st.out_type != ML_TYPE_FLOAT64) st.out_type = ML_TYPE_INT64;
+      if (st.finish != FIN_RANK) {
+        st.finish = FIN_COPY;
+        if (st.out_type != ML_TYPE_FLOAT64) st.out_type = ML_TYPE_INT64;
+      }
       A->state_outs.push_back(st);
     } else {
       A->state_outs.push_back(o);
@@ -1041,7 +1110,7 @@ static void compile_agg(ml_hashagg* A, PipelineBuild& B, bool first) {
       const AggDef& d = A->aggs[(size_t)sl.arg];
       NodePtr x = B.resolve(*d.raw);
       int want = d.arg_type;
-      if (A->slot_cast[s] >= 0 && A->slot_cast[s] != d.arg_type) {
+      if (A->slot_cast[s] >= 0 && A->slot_cast[s] != CAST_RANK && A->slot_cast[s] != d.arg_type) {
         // evaluate_to_column(arg, arg_type), then prepare_input's cast (accumulator.rs:229-236)
         auto c = std::make_shared<Node>();
         c->kind = ML_EXPR_CAST;
@@ -1050,7 +1119,22 @@ static void compile_agg(ml_hashagg* A, PipelineBuild& B, bool first) {
         x = c;
         want = A->slot_cast[s];
       }
-      Prog p = B.comp.compile(x, want);
+      Prog p;
+      if (A->slot_cast[s] == CAST_RANK) {  // the byte-order rank of the row's dictionary entry
+        const int rk = A->agg_rank[(size_t)sl.arg];
+        if (rk < 0) fail(ML_ERR_INTERNAL, "string aggregate without a rank map");
+        if (!same_dict(arg_dict(B, d.raw), A->ranks[(size_t)rk].dict))
+          fail(ML_ERR_UNSUPPORTED, "every batch must share the dictionary of a string MIN/MAX argument");
+        p = B.comp.compile(B.resolve(*d.raw), -1);
+        Instr in{};
+        in.op = OP_LOOKUP_U64;
+        in.depth = 1;
+        in.imm = (uint64_t)(uintptr_t)A->ranks[(size_t)rk].to_rank.as<uint64_t>();
+        B.ins.push_back(in);
+        p.count++;
+      } else {
+        p = B.comp.compile(x, want);
+      }
       // dedupe identical code
       for (int q = 0; q < B.P.nargs && found < 0; q++) {
         const Prog& o = B.P.args[q];
@@ -1353,6 +1437,8 @@ constexpr int SPILL_PARTITIONS = 16;  // hash.rs:189-199
 // copied to host memory; the table is emptied and its reservation released.
 static void agg_spill(ml_hashagg* A) {
   ml_ctx* ctx = A->ctx;
+  for (const OutDef& o : A->state_outs)
+    if (o.finish == FIN_RANK) fail(ML_ERR_UNSUPPORTED, "spilling the states of MIN/MAX of strings");
   if (A->cap && agg_len(A) > 0) {
     std::unique_ptr<ml_result> r(agg_finalize(A, true));
     const int64_t n = r->num_rows;
@@ -1611,6 +1697,12 @@ static ml_result* agg_finalize(ml_hashagg* A, bool states) {
       for (int t : types) {
         ResultColumn c;
         c.type = t;
+        if (t == ML_TYPE_UTF8) {  // a NULL string over an empty string, as str_column writes it
+          auto d = std::make_shared<HostDict>();
+          d->entries.push_back("");
+          d->finish_view();
+          c.dict = d;
+        }
         if (rows) {
           const uint64_t zero = 0;
           c.data = Buffer(ctx->arena, 8);
@@ -1663,6 +1755,7 @@ static ml_result* agg_finalize(ml_hashagg* A, bool states) {
     for (auto& o : outs) {
       ResultColumn c;
       c.type = o.out_type;
+      if (o.finish == FIN_RANK) c.dict = A->ranks[(size_t)o.rank].dict;
       const int w = type_width(c.type);
       std::vector<uint8_t> host(std::max<size_t>(n, 1) * w);
       std::vector<uint8_t> valid(std::max<size_t>(n, 1), 1);
@@ -1676,6 +1769,9 @@ static ml_result* agg_finalize(ml_hashagg* A, bool states) {
           std::memcpy(&s, &v, 8);
           const double avg = cnt == 0 ? 0.0 : s / (double)cnt;
           std::memcpy(&v, &avg, 8);
+        } else if (o.finish == FIN_RANK) {
+          const auto& hc = A->ranks[(size_t)o.rank].host_code;
+          v = (cnt > 0 && v < hc.size()) ? hc[v] : 0;
         }
         if (o.count >= 0 && cnt <= 0) { valid[j] = 0; any_null = true; }
         if (w == 8) std::memcpy(&host[j * 8], &v, 8);
@@ -1738,6 +1834,12 @@ static ml_result* agg_finalize(ml_hashagg* A, bool states) {
     E.count_state[a] = o.count >= 0 ? A->acc[(size_t)o.count].ptr : nullptr;
     E.out[a] = c.data.ptr;
     E.out_valid[a] = c.valid.ptr ? c.valid.as<uint8_t>() : nullptr;
+    if (o.finish == FIN_RANK) {
+      const RankMap& m = A->ranks[(size_t)o.rank];
+      c.dict = m.dict;
+      E.rank_code[a] = m.to_code.as<uint32_t>();
+      E.rank_n[a] = m.host_code.size();
+    }
     r->cols.push_back(std::move(c));
   }
   launch_agg_emit(A->table.as<uint64_t>(), A->nkw, idx.as<int64_t>(), (int64_t)n, E, ctx->stream);

@@ -20,7 +20,9 @@ struct AccMove {
   void* dst[MAX_AGG_SLOTS];
 };
 
-enum FinishKind : int32_t { FIN_COPY = 0, FIN_AVG = 1, FIN_I32 = 2 };
+// FIN_RANK: the state is the byte-order rank of a dictionary entry (MIN/MAX
+// of strings); the output is the code of an entry with that rank.
+enum FinishKind : int32_t { FIN_COPY = 0, FIN_AVG = 1, FIN_I32 = 2, FIN_RANK = 3 };
 
 // hash aggregate output: key columns then one column per aggregate
 struct EmitPlan {
@@ -37,6 +39,8 @@ struct EmitPlan {
   const void* count_state[MAX_AGG_SLOTS];  // valid-input count (AVG divisor, SUM/MIN/MAX seen)
   void* out[MAX_AGG_SLOTS];
   uint8_t* out_valid[MAX_AGG_SLOTS];
+  const uint32_t* rank_code[MAX_AGG_SLOTS];  // FIN_RANK: entry code of each rank
+  uint64_t rank_n[MAX_AGG_SLOTS];            // FIN_RANK: number of ranks
 };
 
 enum SortKeyKind : int32_t { SK_INT = 0, SK_F64 = 1, SK_UNSIGNED = 2 };

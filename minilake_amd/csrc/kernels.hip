@@ -2328,6 +2328,7 @@ __global__ void agg_emit_kernel(const uint64_t* slots, int nkw, const int64_t* i
       switch (E.finish[a]) {
         case FIN_AVG: out = f2u(cnt == 0 ? 0.0 : u2f(v) / (double)cnt); break;
         case FIN_I32: out = (uint64_t)(uint32_t)v; break;
+        case FIN_RANK: out = (cnt > 0 && v < E.rank_n[a]) ? (uint64_t)E.rank_code[a][v] : 0; break;
         default: break;
       }
       if (E.out_width[a] == 8) ((uint64_t*)E.out[a])[j] = out;

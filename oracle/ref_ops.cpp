@@ -101,7 +101,7 @@ struct GroupKeys {
 
 // =================================================================== Accumulator
 // accumulator.rs:99-544
-enum StateKind { S_COUNT, S_SUM_I64, S_SUM_F64, S_AVG, S_MINMAX_I64, S_MINMAX_F64 };
+enum StateKind { S_COUNT, S_SUM_I64, S_SUM_F64, S_AVG, S_MINMAX_I64, S_MINMAX_F64, S_MINMAX_STR };
 
 struct Accumulator {
   int func;
@@ -110,6 +110,7 @@ struct Accumulator {
   StateKind kind;
   std::vector<int64_t> i;    // count / sum i64 / min-max i64 / avg count
   std::vector<double> f;     // sum f64 / avg sum / min-max f64
+  std::vector<std::string> s; // min-max of strings (State::MinMaxStr, accumulator.rs:94)
   std::vector<uint8_t> seen;
 
   Accumulator(int fn, int arg_type, bool has_arg) : func(fn) {  // accumulator.rs:104-165
@@ -129,18 +130,23 @@ struct Accumulator {
       case ML_AGG_MIN: case ML_AGG_MAX:
         if (arg_type == Int32 || arg_type == Int64 || arg_type == Date) { input_type = Int64; output_type = arg_type; kind = S_MINMAX_I64; }
         else if (arg_type == Float64) { input_type = Float64; output_type = Float64; kind = S_MINMAX_F64; }
-        else fail(ML_ERR_UNSUPPORTED, "MIN/MAX of strings");
+        else if (arg_type == Utf8) { input_type = Utf8; output_type = Utf8; kind = S_MINMAX_STR; }  // :156-160
+        else bad();
         break;
       default: bad();
     }
   }
-  size_t num_groups() const { return kind == S_SUM_F64 || kind == S_MINMAX_F64 ? f.size() : i.size(); }
+  size_t num_groups() const {
+    if (kind == S_MINMAX_STR) return s.size();
+    return kind == S_SUM_F64 || kind == S_MINMAX_F64 ? f.size() : i.size();
+  }
   void resize(size_t n) {  // accumulator.rs:205-230
     switch (kind) {
       case S_COUNT: i.resize(n, 0); break;
       case S_SUM_I64: case S_MINMAX_I64: i.resize(n, 0); seen.resize(n, 0); break;
       case S_SUM_F64: case S_MINMAX_F64: f.resize(n, 0.0); seen.resize(n, 0); break;
       case S_AVG: f.resize(n, 0.0); i.resize(n, 0); break;
+      case S_MINMAX_STR: s.resize(n); seen.resize(n, 0); break;
     }
   }
   ColumnRef prepare_input(ColumnRef c) const {  // accumulator.rs:229-236
@@ -216,8 +222,27 @@ struct Accumulator {
         }
         break;
       }
+      case S_MINMAX_STR: {  // accumulator.rs:352-368: bytewise (std::string compares as unsigned bytes)
+        const uint32_t* v = in->as<uint32_t>();
+        for (size_t k = 0; k < rows.len(); k++) {
+          size_t r = rows.row(k);
+          if (!in->is_valid(r)) continue;
+          uint32_t g = gids[k];
+          const std::string& x = in->dict->entries[v[r]];
+          if (!seen[g] || (is_min && x < s[g]) || (!is_min && x > s[g])) { s[g] = x; seen[g] = 1; }
+        }
+        break;
+      }
       default: break;
     }
+  }
+  // str_column (accumulator.rs:546-553): the strings, "" for a group never seen, NULL there
+  ColumnRef str_column() const {
+    auto d = std::make_shared<Dict>();
+    d->entries = s;
+    std::vector<uint32_t> codes(s.size());
+    for (size_t g = 0; g < s.size(); g++) codes[g] = (uint32_t)g;
+    return make_column(Utf8, std::move(codes), std::make_shared<std::vector<uint8_t>>(seen), d);
   }
   // state (accumulator.rs:382-404)
   std::vector<ColumnRef> state() const {
@@ -227,6 +252,7 @@ struct Accumulator {
       case S_SUM_I64: case S_MINMAX_I64: return {make_column(Int64, std::vector<int64_t>(i), bm())};
       case S_SUM_F64: case S_MINMAX_F64: return {make_column(Float64, std::vector<double>(f), bm())};
       case S_AVG: return {make_column(Float64, std::vector<double>(f)), make_column(Int64, std::vector<int64_t>(i))};
+      case S_MINMAX_STR: return {str_column()};
     }
     return {};
   }
@@ -254,6 +280,12 @@ struct Accumulator {
         case S_MINMAX_F64: {
           double x = first.as<double>()[k];
           if (first.is_valid(k) && (!seen[g] || (is_min && x < f[g]) || (!is_min && x > f[g]))) { f[g] = x; seen[g] = 1; }
+          break;
+        }
+        case S_MINMAX_STR: {  // accumulator.rs:470-484
+          if (!first.is_valid(k)) break;
+          const std::string& x = first.dict->entries[first.as<uint32_t>()[k]];
+          if (!seen[g] || (is_min && x < s[g]) || (!is_min && x > s[g])) { s[g] = x; seen[g] = 1; }
           break;
         }
       }
@@ -284,6 +316,7 @@ struct Accumulator {
         return make_column(Int64, std::vector<int64_t>(i), bm());
       }
       case S_MINMAX_F64: return make_column(Float64, std::vector<double>(f), bm());
+      case S_MINMAX_STR: return str_column();
     }
     return nullptr;
   }

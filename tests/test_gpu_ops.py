@@ -815,3 +815,37 @@ def test_collect_sized_expanding_probe_matches_oracle(gpu_engine, oracle_engine)
         got, fell_back = gpu_engine.collect_sized(probe, ops_g, hint)
         assert fell_back == fallback, hint
         assert_tables_equal(got.to_host(), want, rtol=0)
+
+
+@pytest.mark.parametrize("case", ["hashed", "direct", "partitioned", "ungrouped"])
+def test_string_min_max_matches_oracle(gpu_engine, oracle_engine, case):
+    """MIN/MAX of a dictionary string (State::MinMaxStr, accumulator.rs:94,
+    352-368): the device folds each entry's byte-order rank and maps the
+    result back to an entry of the same dictionary. A hashed group table, a
+    direct one (dictionary key), the partitioned LDS path (pre-sized, >65,536
+    groups) and no keys; NULL arguments and an all-NULL group; Partial states
+    of three slices merged by Final."""
+    from tests.test_oracle import STR_AGGS, _str_agg_table
+    t = _str_agg_table(n=400_000 if case == "partitioned" else 40_000, seed=43)
+    if case == "partitioned":  # ~200k distinct keys
+        t.columns[0].values[:] = np.random.default_rng(5).integers(0, 200_000, t.num_rows)
+    keys, kt = {"hashed": ([col(0)], [INT64]), "direct": ([col(2)], [UTF8]), "partitioned": ([col(0)], [INT64]),
+                "ungrouped": ([], [])}[case]
+    kw = dict(sizing=abi.SIZING_HINT, capacity_hint=250_000) if case == "partitioned" else {}
+    want = oracle_engine.hashagg(t, [], keys, kt, STR_AGGS).to_host()
+    got = gpu_engine.hashagg(t, [], keys, kt, STR_AGGS, **kw).to_host()
+    assert [c.type for c in got.columns] == [c.type for c in want.columns]
+    assert_tables_equal(got, want, rtol=0)
+    if case == "partitioned":
+        return
+    cuts = [0, 9_000, 9_001, t.num_rows]
+    parts = []
+    for a, b in zip(cuts, cuts[1:]):
+        piece = t.take(np.arange(a, b))
+        w = oracle_engine.hashagg(piece, [], keys, kt, STR_AGGS, mode=abi.AGG_MODE_PARTIAL).to_host()
+        g = gpu_engine.hashagg(piece, [], keys, kt, STR_AGGS, mode=abi.AGG_MODE_PARTIAL).to_host()
+        assert_tables_equal(g, w, rtol=0)
+        parts.append(w)
+    states = concat_tables(parts)
+    got = gpu_engine.hashagg(states, [], keys, kt, STR_AGGS, mode=abi.AGG_MODE_FINAL).to_host()
+    assert_tables_equal(got, want, rtol=0)

@@ -655,3 +655,61 @@ def test_compare_rows_matches_brute_force(asc0, nf0, asc1, nf1):
     top = rows_of(eng.topn(t, [], keys, 25).to_host())
     kf = lambda r: tuple(None if v is None else (_f64_rank(v) if isinstance(v, float) else v) for v in r[:2])
     assert [kf(r) for r in top] == [kf(r) for r in want[:25]]
+
+
+STR_ENTRIES = ["pear", "Apple", "", "apple", "éclair", "z", "zz", "Zebra", "pea"]  # not in byte order
+
+
+def _str_agg_table(n=4000, seed=41):
+    """key i64 in [0, 60) (key 59: only NULL strings), a dictionary string with
+    10% NULLs, and a small dictionary key (direct-table grouping)."""
+    rng = np.random.default_rng(seed)
+    key = rng.integers(0, 60, n).astype(np.int64)
+    codes = rng.integers(0, len(STR_ENTRIES), n).astype(np.uint32)
+    valid = (rng.random(n) > 0.1) & (key != 59)
+    small = rng.integers(0, 3, n).astype(np.uint32)
+    return HostTable([HostColumn(INT64, key), HostColumn(UTF8, codes, STR_ENTRIES, valid),
+                      HostColumn(UTF8, small, ["x", "y", "w"])])
+
+
+def _str_minmax_brute(t, key_col):
+    keys, strs = t.columns[key_col].decoded(), t.columns[1].decoded()
+    out = {}
+    for k, s in zip(keys, strs):
+        lo, hi = out.get(k, (None, None))
+        if s is not None:  # NULL arguments are skipped; bytes compare unsigned (Vec<u8> Ord)
+            b = s.encode("utf-8")
+            lo = s if lo is None or b < lo.encode("utf-8") else lo
+            hi = s if hi is None or b > hi.encode("utf-8") else hi
+        out[k] = (lo, hi)
+    return out
+
+
+STR_AGGS = [Agg(AGG_MIN, col(1), UTF8), Agg(abi.AGG_MAX, col(1), UTF8), Agg(AGG_COUNT_STAR)]
+
+
+@pytest.mark.parametrize("key_col", [0, 2])
+def test_string_min_max_matches_brute_force(key_col):
+    """State::MinMaxStr (accumulator.rs:94, 352-368, 470-484, 546-553): MIN/MAX
+    of strings by unsigned byte order ('é' sorts after 'z', upper case before
+    lower case, '' first), NULL arguments skipped, a group with none is NULL.
+    Threads and small morsels make the partial-state merge run; Partial states
+    of three slices merged by Final equal Single."""
+    t = _str_agg_table()
+    eng = O.OracleEngine(threads=3, morsel_rows=301)
+    kt = [INT64] if key_col == 0 else [UTF8]
+    got = eng.hashagg(t, [], [col(key_col)], kt, STR_AGGS).to_host()
+    assert got.columns[1].type == UTF8 and got.columns[2].type == UTF8
+    want = _str_minmax_brute(t, key_col)
+    rows = {r[0]: (r[1], r[2]) for r in rows_of(got)}
+    assert rows == want
+    if key_col == 0:
+        assert rows[59] == (None, None)
+    cuts = [0, 1500, 1501, t.num_rows]
+    parts = [eng.hashagg(t.take(np.arange(a, b)), [], [col(key_col)], kt, STR_AGGS, mode=abi.AGG_MODE_PARTIAL).to_host()
+             for a, b in zip(cuts, cuts[1:])]
+    final = eng.hashagg(concat_tables(parts), [], [col(key_col)], kt, STR_AGGS, mode=abi.AGG_MODE_FINAL).to_host()
+    assert_tables_equal(final, got, rtol=0)
+    ungrouped = eng.hashagg(t, [], [], [], STR_AGGS).to_host()
+    allv = [s for s in t.columns[1].decoded() if s is not None]
+    assert rows_of(ungrouped) == [(min(allv, key=lambda s: s.encode()), max(allv, key=lambda s: s.encode()), t.num_rows)]
