@@ -151,10 +151,14 @@ struct CheckPool {
   static constexpr int WORDS = 1024;
   uint32_t* words = nullptr;
   std::vector<int> free;
+  // events recorded right after each word's copy: a table's check is read by
+  // waiting for its own copy, not for everything the stream holds after it
+  std::vector<hipEvent_t> events;
   CheckPool();
   ~CheckPool();
   int take();  // -1 when every word is in use
   void give(int w) { if (w >= 0) free.push_back(w); }
+  hipEvent_t event(int w);
 };
 }  // namespace mlg
 

@@ -71,7 +71,14 @@ CheckPool::CheckPool() {
   for (int w = WORDS - 1; w >= 0; w--) free.push_back(w);
 }
 CheckPool::~CheckPool() {
+  for (hipEvent_t e : events)
+    if (e) (void)hipEventDestroy(e);
   if (words) (void)hipHostFree(words);
+}
+hipEvent_t CheckPool::event(int w) {
+  if (events.empty()) events.assign(WORDS, nullptr);
+  if (!events[(size_t)w]) hip_check(hipEventCreateWithFlags(&events[(size_t)w], hipEventDisableTiming), "hipEventCreate");
+  return events[(size_t)w];
 }
 int CheckPool::take() {
   if (free.empty()) return -1;
@@ -122,6 +129,7 @@ static bool defer_check(ml_ctx* ctx, JoinTableImpl& T, const uint32_t* src, int 
   const int w = ctx->checks ? ctx->checks->take() : -1;
   if (w < 0) return false;
   hip_check(hipMemcpyAsync(ctx->checks->words + w, src, 4, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
+  hip_check(hipEventRecord(ctx->checks->event(w), ctx->stream), "record");
   T.checks = ctx->checks;
   T.check = w;
   T.check_seq = ++ctx->check_seq;
@@ -491,7 +499,15 @@ static bool plan_flow(PipelineBuild& B, int sink, uint32_t sink_stages, size_t c
   // the stages a ROWS sink records
   P.tup_mask = sink_stages;
   // one wave per resident slot, each with a contiguous chunk of source rows
+#ifdef ML_FLOW_BPC
+  const int bpc = ML_FLOW_BPC;  // a fixed workgroups-per-CU count (occupancy sweeps)
+#else
   const int bpc = flow_blocks_per_cu(sink, P.fast != 0, FLOW_BLOCK, lds);
+#endif
+#ifdef ML_FLOW_DEBUG
+  fprintf(stderr, "[flow] sink %d fast %d lds %zu blocks/CU %d cus %d rows %lld\n", sink, (int)P.fast, lds, bpc,
+          B.ctx->num_cus, (long long)active);
+#endif
   int64_t nwaves = (int64_t)B.ctx->num_cus * bpc * (FLOW_BLOCK / 64);
   int64_t chunk = (active + nwaves - 1) / nwaves;
   if (chunk < FLOW_MIN_CHUNK) {
@@ -2224,11 +2240,9 @@ void join_table_resolve(ml_join_table* t) {
   JoinTableImpl& T = t->impl;
   if (!T.pending) return;
   ml_ctx* ctx = t->ctx;
-  if (T.check_seq > ctx->synced_seq) {
-    const uint64_t seq = ctx->check_seq;
-    hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
-    ctx->synced_seq = seq;
-  }
+  // wait for this check's own copy: the kernels queued behind it keep the
+  // device busy while the host plans with the settled layout
+  if (T.check_seq > ctx->synced_seq) hip_check(hipEventSynchronize(T.checks->event(T.check)), "synchronize");
   const uint32_t hf = T.checks->words[T.check];
   T.checks->give(T.check);
   T.check = -1;
@@ -2671,7 +2685,25 @@ static void topn_merge(ml_topn* T, Materialized& m);
 
 static void topn_run(ml_topn* T, PipelineBuild& B) {
   Materialized m;
-  run_collect(B, m, true, nullptr, 0);
+  // without operators or a selection the rows to sort are the source's own
+  // columns: topn_reduce reads them in place (the caller owns them until this
+  // call returns, and the winners are gathered into new buffers before that)
+  bool in_place = B.P.nsteps == 0 && !B.P.sel;
+  for (const VCol* v : B.layout) in_place = in_place && !v->expr && v->stage == 0;
+  if (in_place) {
+    for (const VCol* v : B.layout) {
+      m.types.push_back(v->type);
+      m.dicts.push_back(v->dict);
+      Buffer d, vb;  // views: no arena, never freed
+      d.ptr = const_cast<void*>(v->data);
+      vb.ptr = const_cast<uint8_t*>(v->valid);
+      m.cols.push_back(std::move(d));
+      m.valid.push_back(std::move(vb));
+    }
+    m.rows = B.active_rows;
+  } else {
+    run_collect(B, m, true, nullptr, 0);
+  }
   if (!T->schema_fixed) {
     T->best.types = m.types;
     T->best.dicts = m.dicts;
@@ -2683,6 +2715,12 @@ static void topn_run(ml_topn* T, PipelineBuild& B) {
     fail(ML_ERR_INTERNAL, "TopN batches with different schemas");
   }
   topn_reduce(T, m);
+  // nothing may keep a view of the source beyond this call (topn_reduce
+  // replaced every column unless it kept no rows)
+  for (size_t c = 0; c < m.cols.size(); c++) {
+    if (!m.cols[c].arena) m.cols[c] = Buffer();
+    if (!m.valid[c].arena) m.valid[c] = Buffer();
+  }
   topn_merge(T, m);
 }
 
