@@ -92,7 +92,15 @@ static void h2d(ml_ctx* ctx, void* dst, const void* src, size_t n) {
 }
 // Upload through the pinned ring (returns at once; `src` may be freed after
 // the call). Larger uploads copy and then wait.
-static void upload(ml_ctx* ctx, void* dst, const void* src, size_t n) {
+#ifdef ML_COPY_DEBUG
+#define ML_CALLER , const char* caller = __builtin_FUNCTION(), int line = __builtin_LINE()
+#define ML_NOTE(kind) fprintf(stderr, "[copy] %s %zu %s:%d\n", kind, n, caller, line)
+#else
+#define ML_CALLER
+#define ML_NOTE(kind)
+#endif
+static void upload(ml_ctx* ctx, void* dst, const void* src, size_t n ML_CALLER) {
+  ML_NOTE("h2d");
   if (!n) return;
   if (!ctx->ring || n > ml_ctx::RING_BYTES / 4) {
     h2d(ctx, dst, src, n);
@@ -109,7 +117,8 @@ static void upload(ml_ctx* ctx, void* dst, const void* src, size_t n) {
   ctx->ring_head += (n + 255) & ~(size_t)255;
 }
 void ring_upload(ml_ctx* ctx, void* dst, const void* src, size_t n) { upload(ctx, dst, src, n); }
-static void d2h(ml_ctx* ctx, void* dst, const void* src, size_t n) {
+static void d2h(ml_ctx* ctx, void* dst, const void* src, size_t n ML_CALLER) {
+  ML_NOTE("d2h");
   const uint64_t seq = ctx->check_seq;
   if (n && ctx->pinned && n <= ml_ctx::PINNED_BYTES) {
     hip_check(hipMemcpyAsync(ctx->pinned, src, n, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
@@ -136,10 +145,15 @@ static bool defer_check(ml_ctx* ctx, JoinTableImpl& T, const uint32_t* src, int 
   T.pending = kind;
   return true;
 }
-static void d2d(ml_ctx* ctx, void* dst, const void* src, size_t n) {
+static void d2d(ml_ctx* ctx, void* dst, const void* src, size_t n ML_CALLER) {
+  ML_NOTE("d2d");
   if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, ctx->stream), "device copy");
 }
-static void zero(ml_ctx* ctx, void* p, size_t bytes) {  // a fill kernel, so the work stays capturable
+static void zero(ml_ctx* ctx, void* p, size_t bytes ML_CALLER) {  // a fill kernel, so the work stays capturable
+#ifdef ML_COPY_DEBUG
+  const size_t n = bytes;
+  ML_NOTE("fill");
+#endif
   launch_fill_u64((uint64_t*)p, 0, (int64_t)((bytes + 7) / 8), ctx->stream);
 }
 
@@ -2100,14 +2114,24 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
   B.upload();
   FlowLaunch F;
   ch.cap = (int64_t)cap;
-  if (!B.expanding && cap <= SMALL_BUILD_ROWS) {
-    // a small build: one append kernel, its count left on the device for the
-    // one-workgroup dense build (rows in any order: the table orders them)
+  // no filter, probe or selection: every source row is a build row, written
+  // at its own index by one row-per-lane kernel (no counting, no compaction)
+  const bool identity = !B.expanding && B.P.nsteps == 0 && !B.P.sel;
+  if (!B.expanding && (cap <= SMALL_BUILD_ROWS || identity)) {
+    // one append kernel, its count left on the device for the dense build's
+    // first kernel (rows in any order: the table orders them)
     ch.count_buf = Buffer(ctx->arena, 16);
-    zero(ctx, ch.count_buf.ptr, 16);
+    if (identity) {
+      const uint64_t cw[2] = {(uint64_t)B.active_rows, 0};
+      upload(ctx, ch.count_buf.ptr, cw, 16);
+      B.P.out_identity = 1;
+    } else {
+      zero(ctx, ch.count_buf.ptr, 16);
+    }
     B.P.out_count = ch.count_buf.as<uint64_t>();
     B.P.out_overflow = (uint32_t*)(ch.count_buf.as<uint64_t>() + 1);
     if (B.active_rows > 0) ctx->launch(B.P, B.grid(), block_of(ctx), 0);
+    B.P.out_identity = 0;
     ch.rows = -1;
     ch.count_dev = ch.count_buf.as<uint64_t>();
   } else if (plan_flow(B, SINK_ROWS, sink_read_stages(B), 0, F)) {
@@ -2491,9 +2515,17 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
         uint32_t hf = 0;
         if (!deferred) d2h(ctx, &hf, flags, 4);
         if (!(hf & 5)) {
-          Buffer rank(ctx->arena, (size_t)(ne + 1) * 4);
-          Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(ne) + 1) * 4);
-          launch_dense_rank(bits.as<uint64_t>(), ne, rank.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
+          // every key of the range present (all distinct, or the table is
+          // rebuilt hashed): a row's rank is its key's offset, so no ranks are
+          // computed, and a probe reads neither bits nor ranks; a Bloom filter
+          // would cost the one load it saves
+          const bool full = hm[3] == (int64_t)R;
+          Buffer rank;
+          if (!full) {
+            rank = Buffer(ctx->arena, (size_t)(ne + 1) * 4);
+            Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(ne) + 1) * 4);
+            launch_dense_rank(bits.as<uint64_t>(), ne, rank.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
+          }
           DensePermute M{};
           Buffer dkeys;
           if (nk > 1) dkeys = Buffer(ctx->arena, (size_t)n * (nk - 1) * key_word_bytes);
@@ -2515,11 +2547,8 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
           }
           // flags[1] (next NULL-key row) was cleared by the min/max kernel
           launch_dense_permute(M, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(),
-                               rank.as<uint32_t>(), rank.as<uint32_t>() + ne, flags + 1, ctx->stream);
-          // every key of the range present (all distinct, or the table is
-          // rebuilt hashed): a probe finds its row without bits or ranks, and
-          // a Bloom filter would cost the one load it saves
-          const bool full = hm[3] == (int64_t)R;
+                               full ? nullptr : rank.as<uint32_t>(), full ? nullptr : rank.as<uint32_t>() + ne,
+                               flags + 1, ctx->stream);
           if (nk > 1 && !full && hashed_bloom(ctx, n)) {  // a word per 8 first-key values
             const uint64_t words = (R + (1u << DENSE_BLOOM_SHIFT) - 1) >> DENSE_BLOOM_SHIFT;
             T.bloom = Buffer(ctx->arena, words * 8);

@@ -121,6 +121,8 @@ struct DensePermute {
   const uint8_t* src_valid[MAX_COLS];
   uint8_t* dst_valid[MAX_COLS];
 };
+// Rows into key order; rank and present null for a full table (every key of
+// the range present: a row's position is its key's offset).
 void launch_dense_permute(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
                           int64_t kmin, uint64_t range, const uint64_t* bits, const uint32_t* rank,
                           const uint32_t* present, uint32_t* null_next, hipStream_t s);

@@ -961,7 +961,7 @@ __device__ __forceinline__ void write_cols(const Plan& P, const Code& ins, const
 }
 
 __device__ __forceinline__ void sink_join_build(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES]) {
-  const uint64_t pos = wave_alloc(P.out_count, 1);
+  const uint64_t pos = P.out_identity ? (uint64_t)rows[0] : wave_alloc(P.out_count, 1);
   if (pos >= P.out_capacity) {
     atomicOr(P.out_overflow, 1u);
     return;
@@ -1917,15 +1917,18 @@ __global__ void dense_mark_kernel(const uint64_t* keys, int nkw, const uint8_t* 
 __global__ void dense_permute_kernel(DensePermute M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
                                      int64_t kmin, uint64_t range, const uint64_t* bits, const uint32_t* rank,
                                      const uint32_t* present, uint32_t* null_next) {
+  // rank == null: every key of the range is present (a full table), so the
+  // rank of key offset i is i itself and the NULL-key rows start at range
+  const uint64_t null_base = rank ? (uint64_t)*present : range;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     uint64_t pos;
     if (!key_null[r]) {
       const uint64_t i = (uint64_t)((int64_t)keys[r * M.nkw] - kmin);
       if (i >= range) continue;  // flagged by the marking; the table falls back
-      pos = rank[i >> 6] + (uint64_t)__popcll(bits[i >> 6] & ((1ull << (i & 63)) - 1ull));
+      pos = rank ? rank[i >> 6] + (uint64_t)__popcll(bits[i >> 6] & ((1ull << (i & 63)) - 1ull)) : i;
       if (pos >= (uint64_t)n) continue;  // only with repeated keys, which also fall back
     } else {
-      pos = (uint64_t)*present + atomicAdd(null_next, 1u);
+      pos = null_base + atomicAdd(null_next, 1u);
       if (pos >= (uint64_t)n) continue;  // only with repeated keys, which also fall back
     }
     for (int k = 1; k < M.nkw; k++) {

@@ -333,6 +333,7 @@ struct Plan {
   uint64_t* out_count;       // rows written so far (device counter)
   uint64_t out_capacity;
   uint32_t* out_overflow;
+  int32_t out_identity;      // JOIN_BUILD: every source row is output, at its own index (no counter)
   int64_t* pair_out;         // COLLECT with row ids: [rows][nstages]
   int32_t pair_stages;
 
