@@ -329,6 +329,7 @@ struct JoinTableImpl {
   Buffer slots;
   Buffer bloom;
   Buffer dense_bits, dense_rank, dense_keys;  // dense layout (DevJoinTable::dense)
+  Buffer dense_keys8;  // a byte copy of the second key words, used once they are known to fit
   DevJoinTable dev{};
   std::vector<ml_column> views;
   // A build whose last check still runs on the stream: its flags arrive in

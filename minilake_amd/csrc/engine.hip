@@ -2260,6 +2260,18 @@ static void refresh_views(JoinTableImpl& T) {
   }
 }
 
+// Second key words that all fit a byte (flag bit 8 clear): probes read the
+// byte copy (a quarter of the u32 words' footprint in L2).
+static void use_keys8(JoinTableImpl& T, uint32_t hf) {
+  if (T.dense_keys8.ptr && !(hf & 256u)) {
+    T.dense_keys = std::move(T.dense_keys8);
+    T.dev.dense_keys = T.dense_keys.as<uint64_t>();
+    T.dev.dense_keys32 = 0;
+    T.dev.dense_keys8 = 1;
+  }
+  T.dense_keys8 = Buffer();
+}
+
 void join_table_resolve(ml_join_table* t) {
   JoinTableImpl& T = t->impl;
   if (!T.pending) return;
@@ -2281,9 +2293,11 @@ void join_table_resolve(ml_join_table* t) {
     T.dense_bits = Buffer();
     T.dense_rank = Buffer();
     T.dense_keys = Buffer();
+    T.dense_keys8 = Buffer();
     T.bloom = Buffer();
     T.dev.dense = 0;
     T.dev.dense_full = 0;
+    T.dev.dense_keys8 = 0;
     T.dev.dense_bits = nullptr;
     T.dev.dense_rank = nullptr;
     T.dev.dense_keys = nullptr;
@@ -2291,6 +2305,8 @@ void join_table_resolve(ml_join_table* t) {
     T.dev.bloom = nullptr;
     refresh_views(T);
     build_hashed(ctx, T, T.keys, T.key_null, T.nrows, nk, flags, false);
+  } else if (kind == PENDING_DENSE) {
+    use_keys8(T, hf);
   } else if (kind == PENDING_PARTS) {
     if (hf & 2) build_hashed_global(ctx, T, T.keys, T.key_null, T.nrows, flags);
     else T.dev.unique = (hf & 1) ? 0 : 1;
@@ -2527,8 +2543,12 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
             launch_dense_rank(bits.as<uint64_t>(), ne, rank.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
           }
           DensePermute M{};
-          Buffer dkeys;
+          Buffer dkeys, dkeys8;
           if (nk > 1) dkeys = Buffer(ctx->arena, (size_t)n * (nk - 1) * key_word_bytes);
+          if (nk == 2) {  // also a byte per row, used if every second key word fits (flag bit 8)
+            dkeys8 = Buffer(ctx->arena, (size_t)n);
+            M.dst_keys8 = dkeys8.as<uint8_t>();
+          }
           M.nkw = (int32_t)nk;
           M.keys32 = keys32 ? 1 : 0;
           M.dst_keys = dkeys.as<uint64_t>();
@@ -2578,6 +2598,8 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
           T.dev.dense_bits = T.dense_bits.as<uint64_t>();
           T.dev.dense_rank = T.dense_rank.as<uint32_t>();
           T.dev.unique = 1;
+          T.dense_keys8 = std::move(dkeys8);
+          if (!deferred) use_keys8(T, hf);
         }
       }
     }

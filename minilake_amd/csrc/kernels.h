@@ -102,6 +102,7 @@ void launch_strided_u32(const uint32_t* src, int64_t stride, int64_t n, uint32_t
 // per workgroup; `done` is a zeroed counter word (left zeroed); clear[0..1] are set to 0
 void launch_key_minmax(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, const uint64_t* n_dev,
                        int64_t* partials, int nparts, int64_t* mm, uint32_t* done, uint32_t* clear, hipStream_t s);
+// (flag bit 8: with nkw == 2, a second key word outside [0, 255])
 void launch_dense_mark(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, int64_t kmin,
                        uint64_t range, uint64_t* bits, uint32_t* flag, hipStream_t s);
 // Bloom bits of every non-NULL row's whole key, in the word of its first key
@@ -114,6 +115,8 @@ struct DensePermute {
   int32_t nkw;                // key words per row; words 1.. go to dst_keys
   int32_t keys32;             // dst_keys holds u32 (the low halves) instead of u64
   uint64_t* dst_keys;
+  uint8_t* dst_keys8;         // or null: the low byte of key word 1 (nkw == 2), the table's
+                              // narrow copy when every second key word is in [0, 255]
   int32_t ncols;
   int32_t width[MAX_COLS];
   const void* src[MAX_COLS];

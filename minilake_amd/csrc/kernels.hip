@@ -1246,7 +1246,10 @@ __device__ __forceinline__ void fast_probe(const Plan& P, const FastKeys& K, uin
     }
     if (two) {  // the second key word must match the row's
       uint64_t kb[FAST_U];
-      if (T.dense_keys32) {
+      if (T.dense_keys8) {
+#pragma unroll
+        for (int u = 0; u < FAST_U; u++) kb[u] = (uint64_t)gp<uint8_t>(T.dense_keys)[hit[u] ? brow[u] : 0];
+      } else if (T.dense_keys32) {
 #pragma unroll
         for (int u = 0; u < FAST_U; u++) {
           const uint32_t v = gp<uint32_t>(T.dense_keys)[hit[u] ? brow[u] : 0];
@@ -1874,12 +1877,13 @@ __global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, i
 __global__ void dense_mark_kernel(const uint64_t* keys, int nkw, const uint8_t* key_null, int64_t n, int64_t kmin,
                                   uint64_t range, uint64_t* bits, uint32_t* flag) {
   const int lane = __lane_id();
-  bool dup = false, outside = false;
+  bool dup = false, outside = false, wide = false;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
     const int64_t r = base + lane;
     uint64_t w = ~0ull, v = 0;
     if (r < n && !key_null[r]) {
+      if (nkw == 2) wide = wide || keys[r * 2 + 1] > 255u;
       const uint64_t i = (uint64_t)((int64_t)keys[r * nkw] - kmin);
       if (i < range) {
         w = i >> 6;
@@ -1910,6 +1914,7 @@ __global__ void dense_mark_kernel(const uint64_t* keys, int nkw, const uint8_t* 
   }
   if (__any(dup) && lane == 0) atomicOr(flag, 1u);
   if (__any(outside) && lane == 0) atomicOr(flag, 4u);
+  if (__any(wide) && lane == 0) atomicOr(flag, 256u);
 }
 
 // Build rows into key order: a row with key k moves to the rank of k; rows
@@ -1936,6 +1941,7 @@ __global__ void dense_permute_kernel(DensePermute M, const uint64_t* keys, const
       if (M.keys32) ((uint32_t*)M.dst_keys)[at] = (uint32_t)keys[r * M.nkw + k];
       else M.dst_keys[at] = keys[r * M.nkw + k];
     }
+    if (M.dst_keys8) M.dst_keys8[pos] = (uint8_t)keys[r * 2 + 1];
     for (int c = 0; c < M.ncols; c++) {
       const int w = M.width[c];
       if (w == 8) ((uint64_t*)M.dst[c])[pos] = ((const uint64_t*)M.src[c])[r];

@@ -178,6 +178,7 @@ struct DevJoinTable {
   const uint32_t* dense_rank = nullptr;  // present keys below each word
   const uint64_t* dense_keys = nullptr;  // key words 1.. of each row (nkw > 1)
   int32_t dense_keys32 = 0;
+  int32_t dense_keys8 = 0;  // nkw == 2 and every second key word in [0, 255]: one byte per row
   uint32_t dense_keys_sx = 0;
 };
 
@@ -185,6 +186,7 @@ constexpr int DENSE_BLOOM_SHIFT = 3;
 
 // Key word j (>= 1) of dense-table row `row`.
 __host__ __device__ inline uint64_t dense_key_word(const DevJoinTable& T, uint64_t row, int j) {
+  if (T.dense_keys8) return ((const uint8_t*)T.dense_keys)[row];
   const uint64_t at = row * (uint64_t)(T.nkw - 1) + (uint64_t)(j - 1);
   if (!T.dense_keys32) return T.dense_keys[at];
   const uint32_t v = ((const uint32_t*)T.dense_keys)[at];
