@@ -390,12 +390,15 @@ __device__ __forceinline__ u64x4 staged(const Staged8& T, uint32_t j) {
   }
 }
 
-__device__ __forceinline__ void eval_src4(const Code& ins, Prog p, const uint32_t (&src)[4], const Staged8& T,
-                                          uint64_t (&val)[4], bool (&valid)[4]) {
+// NUL: the program can produce NULL (Prog::nullable); without it every value
+// is valid, so no validity is loaded or tracked.
+template <bool NUL>
+__device__ __forceinline__ void eval_src4_t(const Code& ins, Prog p, const uint32_t (&src)[4], const Staged8& T,
+                                            uint64_t (&val)[4], bool (&valid)[4]) {
   Stack4 S;
   S.s0 = S.s1 = S.s2 = S.s3 = (u64x4)(0);
   S.ok = (u32x4)(0xFFFFFFFFu);
-  const int first = uni(p.first), count = uni(p.count), nullable = uni(p.nullable);
+  const int first = uni(p.first), count = uni(p.count);
 #pragma unroll 1
   for (int pc = 0; pc < count; pc++) {
     CInstr& in = ins.at(first + pc);
@@ -424,7 +427,7 @@ __device__ __forceinline__ void eval_src4(const Code& ins, Prog p, const uint32_
           res = (u64x4){(uint64_t)t[x[0]], (uint64_t)t[x[1]], (uint64_t)t[x[2]], (uint64_t)t[x[3]]};
         } else {
           res = x;
-          if (nullable && i2) {
+          if (NUL && i2) {
             const auto vb = gp<uint8_t>((const void*)i2);
             vres = (u32x4){(uint32_t)(vb[src[0]] != 0), (uint32_t)(vb[src[1]] != 0), (uint32_t)(vb[src[2]] != 0),
                            (uint32_t)(vb[src[3]] != 0)};
@@ -434,7 +437,7 @@ __device__ __forceinline__ void eval_src4(const Code& ins, Prog p, const uint32_
     } else if (op < OP_ADD_I32) {
       dst = d - 1;
       const u64x4 a = st_get(S, dst);
-      vres = (S.ok >> (uint32_t)dst) & 1u;
+      if (NUL) vres = (S.ok >> (uint32_t)dst) & 1u;
       uint32_t v0 = vres[0], v1 = vres[1], v2 = vres[2], v3 = vres[3];
 #define ML_UN(OPC) res = (u64x4){un_op(OPC, a[0], imm, v0), un_op(OPC, a[1], imm, v1), un_op(OPC, a[2], imm, v2), un_op(OPC, a[3], imm, v3)};
       if (op == OP_I2F) {
@@ -456,7 +459,7 @@ __device__ __forceinline__ void eval_src4(const Code& ins, Prog p, const uint32_
       dst = d - 2;
       u64x4 a, b;
       st_get2(S, dst, a, b);
-      const u32x4 va = (S.ok >> (uint32_t)dst) & 1u, vb = (S.ok >> (uint32_t)(dst + 1)) & 1u;
+      const u32x4 va = NUL ? (S.ok >> (uint32_t)dst) & 1u : (u32x4)(1u), vb = NUL ? (S.ok >> (uint32_t)(dst + 1)) & 1u : (u32x4)(1u);
       uint32_t v0 = 1, v1 = 1, v2 = 1, v3 = 1;
 #define ML_BIN(OPC) res = (u64x4){bin_op(OPC, a[0], b[0], va[0], vb[0], v0), bin_op(OPC, a[1], b[1], va[1], vb[1], v1), bin_op(OPC, a[2], b[2], va[2], vb[2], v2), bin_op(OPC, a[3], b[3], va[3], vb[3], v3)};
 #define ML_BINC(OPC) case OPC: ML_BIN(OPC) break;
@@ -489,13 +492,19 @@ __device__ __forceinline__ void eval_src4(const Code& ins, Prog p, const uint32_
       vres = (u32x4){v0, v1, v2, v3};
     }
     st_set(S, dst, res);
-    S.ok = (S.ok & ~(1u << (uint32_t)dst)) | (vres << (uint32_t)dst);
+    if (NUL) S.ok = (S.ok & ~(1u << (uint32_t)dst)) | (vres << (uint32_t)dst);
   }
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     val[u] = S.s0[u];
-    valid[u] = (S.ok[u] & 1u) != 0;
+    valid[u] = !NUL || (S.ok[u] & 1u) != 0;
   }
+}
+
+__device__ __forceinline__ void eval_src4(const Code& ins, Prog p, const uint32_t (&src)[4], const Staged8& T,
+                                          uint64_t (&val)[4], bool (&valid)[4]) {
+  if (uni(p.nullable)) eval_src4_t<true>(ins, p, src, T, val, valid);
+  else eval_src4_t<false>(ins, p, src, T, val, valid);
 }
 
 __device__ __forceinline__ void set_word(uint64_t (&w)[MAX_KEYS + 1], int k, uint64_t v) {
