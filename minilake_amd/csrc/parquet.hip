@@ -7,8 +7,14 @@
 // convert_stats (crates/minilake-storage/src/parquet_reader.rs:210-358), whose
 // page decoding the reference delegates to the `parquet` crate (60.0.0, the
 // Apache Parquet format). Supported: flat schemas (REQUIRED / OPTIONAL
-// columns), UNCOMPRESSED and SNAPPY chunks, data pages v1, PLAIN and dictionary
-// (PLAIN_DICTIONARY / RLE_DICTIONARY) encodings, RLE definition levels.
+// columns), UNCOMPRESSED and SNAPPY chunks, data pages v1 and v2, the PLAIN,
+// dictionary (PLAIN_DICTIONARY / RLE_DICTIONARY), RLE (booleans),
+// DELTA_BINARY_PACKED, DELTA_LENGTH_BYTE_ARRAY, DELTA_BYTE_ARRAY and
+// BYTE_STREAM_SPLIT encodings, RLE definition levels. Byte arrays (strings and
+// BYTE_ARRAY decimals) are assembled on the host; integers of DELTA pages are
+// unpacked and prefix-summed on the device. SNAPPY is the only codec, as in
+// the reference, which builds the crate with the "snap" feature alone
+// (Cargo.toml:30).
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -27,7 +33,16 @@ namespace pq {
 enum PhysType { T_BOOLEAN = 0, T_INT32 = 1, T_INT64 = 2, T_INT96 = 3, T_FLOAT = 4, T_DOUBLE = 5, T_BYTE_ARRAY = 6, T_FLBA = 7 };
 enum Conv { C_BOOL, C_INT32, C_DATE, C_INT64, C_FLOAT, C_DOUBLE, C_UTF8, C_DEC_I32, C_DEC_I64, C_DEC_BYTES };
 enum { CT_UTF8 = 0, CT_DECIMAL = 5, CT_DATE = 6 };
-enum { ENC_PLAIN = 0, ENC_PLAIN_DICTIONARY = 2, ENC_RLE = 3, ENC_RLE_DICTIONARY = 8 };
+enum {
+  ENC_PLAIN = 0,
+  ENC_PLAIN_DICTIONARY = 2,
+  ENC_RLE = 3,
+  ENC_DELTA_BINARY_PACKED = 5,
+  ENC_DELTA_LENGTH_BYTE_ARRAY = 6,
+  ENC_DELTA_BYTE_ARRAY = 7,
+  ENC_RLE_DICTIONARY = 8,
+  ENC_BYTE_STREAM_SPLIT = 9
+};
 enum { PAGE_DATA = 0, PAGE_INDEX = 1, PAGE_DICTIONARY = 2, PAGE_DATA_V2 = 3 };
 
 [[noreturn]] static void corrupt(const char* what) { fail(ML_ERR_PARQUET, std::string("corrupt parquet: ") + what); }
@@ -230,6 +245,9 @@ struct PageHeader {
   int64_t compressed = 0;
   int64_t num_values = 0;
   int encoding = -1, def_encoding = -1;
+  // DataPageHeaderV2: rows, and the level bytes stored uncompressed before the values
+  int64_t num_rows = -1, def_len = 0, rep_len = 0;
+  bool values_compressed = true;
 };
 
 static void parse_page_header(TReader& r, PageHeader& h) {
@@ -247,6 +265,17 @@ static void parse_page_header(TReader& r, PageHeader& h) {
         if (id2 == 1) h.num_values = r.integer(t2);
         else if (id2 == 2) h.encoding = (int)r.integer(t2);
         else if (id2 == 3 && id == 5) h.def_encoding = (int)r.integer(t2);
+        else r.skip(t2);
+      }
+    } else if (id == 8 && t == 12) {  // DataPageHeaderV2
+      int l2 = 0, id2, t2;
+      while (r.field(&l2, &id2, &t2)) {
+        if (id2 == 1) h.num_values = r.integer(t2);
+        else if (id2 == 3) h.num_rows = r.integer(t2);
+        else if (id2 == 4) h.encoding = (int)r.integer(t2);
+        else if (id2 == 5) h.def_len = r.integer(t2);
+        else if (id2 == 6) h.rep_len = r.integer(t2);
+        else if (id2 == 7 && (t2 == 1 || t2 == 2)) h.values_compressed = t2 == 1;
         else r.skip(t2);
       }
     } else {
@@ -321,13 +350,38 @@ struct PageRef {
   size_t body, end;
 };
 
+// How a data page's non-NULL values are stored.
+enum PageKind : int32_t {
+  PK_PLAIN = 0,  // PLAIN bytes at values_off (strings: host-assigned codes from plain_base)
+  PK_DICT = 1,   // dictionary indices, expanded into idx
+  PK_BITS = 2,   // RLE booleans, expanded into idx
+  PK_SPLIT = 3,  // BYTE_STREAM_SPLIT: byte k of value j at values_off + k * stride + j
+  PK_DELTA = 4   // DELTA_BINARY_PACKED, decoded into the delta array
+};
+
 // Page table entry of a column chunk (device): rows [first_row, first_row + rows).
 struct PageDesc {
   int64_t first_row;
-  int64_t values_off;  // PLAIN values (bytes into the chunk buffer)
-  int64_t plain_base;  // PLAIN strings: index of the page's first value in the host-assigned codes
+  int64_t values_off;  // PLAIN / BYTE_STREAM_SPLIT values (bytes into the chunk buffer)
+  int64_t plain_base;  // strings: index of the page's first value in the host-assigned codes
+  int64_t stride;      // BYTE_STREAM_SPLIT: values in the page's stream
   int32_t rows;
-  int32_t dict;        // values are dictionary indices
+  int32_t kind;
+};
+
+// A piece of a DELTA_BINARY_PACKED stream (device): up to 64 numbers of one
+// miniblock, or (bw < 0) the page's first value. Value i of a piece is the
+// page's first value + the deltas of the page's earlier pieces + its own first
+// i + 1 deltas, each min_delta + an unpacked number, wrapping at 64 bits (the
+// low 32 bits are the INT32 result).
+struct DeltaRec {
+  int64_t out;         // output position of the piece's first value
+  int64_t bit;         // bit offset of its first packed number in the chunk
+  uint64_t min_delta;
+  uint64_t first;      // the page's first value
+  int64_t head;        // index of the page's first-value record
+  int32_t count;
+  int32_t bw;          // 0..64; -1: the page's first value
 };
 
 // A run of the RLE / bit-packed hybrid encoding, placed at out_start.
@@ -372,6 +426,106 @@ static void hybrid_runs(const uint8_t* chunk, size_t pos, size_t end, int bw, in
   }
 }
 
+static uint64_t host_unpack(const uint8_t* chunk, size_t end, int64_t bit, int bw) {
+  if (bw == 0) return 0;
+  const size_t b0 = (size_t)(bit >> 3);
+  const int sh = (int)(bit & 7);
+  unsigned __int128 x = 0;
+  for (int k = 0; k < 9; k++)
+    if (b0 + (size_t)k < end) x |= (unsigned __int128)chunk[b0 + (size_t)k] << (8 * k);
+  const uint64_t v = (uint64_t)(x >> sh);
+  return bw == 64 ? v : v & ((1ull << bw) - 1);
+}
+
+// DELTA_BINARY_PACKED, from the format's published description: a header
+// <block size> <miniblocks per block> <total count> <first value (zigzag)>,
+// then blocks of <min delta (zigzag)> <a bit width byte per miniblock>
+// <miniblocks, bit-packed LSB first>; each value is the previous one + min
+// delta + an unpacked number, wrapping at the type's width. Miniblocks the
+// last block does not need take no bytes. Walks the stream in [pos, end),
+// which may hold at most max_total values: first(v) for the first value, then
+// mini(bit offset, count, bit width, min delta) for each miniblock used.
+// Returns the position after the last miniblock read.
+template <class First, class Mini>
+static size_t delta_walk(const uint8_t* chunk, size_t pos, size_t end, int bits, int64_t max_total, int64_t* total_out,
+                         First first, Mini mini) {
+  TReader r{chunk, end, pos};
+  const uint64_t block = r.varint(), nmb = r.varint(), total = r.varint();
+  const int64_t v0 = r.zz();
+  if (block == 0 || block > (1u << 30) || block % 128 || nmb == 0 || block % nmb || (block / nmb) % 32)
+    corrupt("delta header");
+  if (total > (uint64_t)max_total) corrupt("delta value count");
+  *total_out = (int64_t)total;
+  if (total == 0) return r.pos;
+  first((uint64_t)v0);
+  const uint64_t per = block / nmb;
+  uint64_t left = total - 1;
+  while (left > 0) {
+    const uint64_t md = (uint64_t)r.zz();
+    if (nmb > end - r.pos) corrupt("delta bit widths past the page");
+    const size_t widths = r.pos;
+    r.pos += (size_t)nmb;
+    for (uint64_t i = 0; i < nmb && left > 0; i++) {
+      const int bw = chunk[widths + i];
+      if (bw > bits) corrupt("delta bit width");
+      const uint64_t bytes = per * (uint64_t)bw / 8;
+      if (bytes > end - r.pos) corrupt("delta miniblock past the page");
+      const uint64_t n = std::min(per, left);
+      mini((int64_t)r.pos * 8, (int64_t)n, bw, md);
+      r.pos += (size_t)bytes;
+      left -= n;
+    }
+  }
+  return r.pos;
+}
+
+// A DELTA_BINARY_PACKED stream of 32-bit numbers (byte array lengths), decoded
+// on the host as signed values.
+static size_t delta_host32(const uint8_t* chunk, size_t pos, size_t end, int64_t max_total, std::vector<int64_t>& out) {
+  out.clear();
+  uint64_t v = 0;
+  int64_t total = 0;
+  return delta_walk(chunk, pos, end, 32, max_total, &total,
+                    [&](uint64_t f) {
+                      v = f;
+                      out.push_back((int32_t)(uint32_t)v);
+                    },
+                    [&](int64_t bit, int64_t n, int bw, uint64_t md) {
+                      for (int64_t q = 0; q < n; q++) {
+                        v += md + host_unpack(chunk, end, bit + q * bw, bw);
+                        out.push_back((int32_t)(uint32_t)v);
+                      }
+                    });
+}
+
+// DELTA_LENGTH_BYTE_ARRAY: the lengths as DELTA_BINARY_PACKED, then the bytes
+// back to back. DELTA_BYTE_ARRAY (prefixed): prefix lengths as
+// DELTA_BINARY_PACKED, then the suffixes as DELTA_LENGTH_BYTE_ARRAY; value k
+// is the first prefix[k] bytes of value k - 1, then suffix k. Exactly `want`
+// values, passed to emit(view, the view stays valid while the chunk does).
+template <class F>
+static void delta_byte_arrays(const uint8_t* chunk, size_t pos, size_t end, bool prefixed, int64_t want, F emit) {
+  std::vector<int64_t> pre, lens;
+  if (prefixed) pos = delta_host32(chunk, pos, end, want, pre);
+  pos = delta_host32(chunk, pos, end, want, lens);
+  if ((int64_t)lens.size() != want || (prefixed && pre.size() != lens.size())) corrupt("delta value count");
+  std::string prev;
+  for (size_t k = 0; k < lens.size(); k++) {
+    const int64_t l = lens[k];
+    if (l < 0 || (uint64_t)l > end - pos) corrupt("delta length");
+    const char* v = reinterpret_cast<const char*>(chunk) + pos;
+    if (prefixed) {
+      if (pre[k] < 0 || (size_t)pre[k] > prev.size()) corrupt("delta prefix");
+      prev.resize((size_t)pre[k]);
+      prev.append(v, (size_t)l);
+      emit(std::string_view(prev), false);
+    } else {
+      emit(std::string_view(v, (size_t)l), true);
+    }
+    pos += (size_t)l;
+  }
+}
+
 // ---------------------------------------------------------------- kernels
 __device__ __forceinline__ uint32_t packed_value(const uint8_t* chunk, int64_t len, int64_t off, int64_t j, int bw) {
   if (bw == 0) return 0;
@@ -408,18 +562,90 @@ __global__ void expand_runs_kernel(const Run* runs, const int64_t* cum, int nrun
   }
 }
 
+__device__ __forceinline__ uint64_t unpack64(const uint8_t* chunk, int64_t len, int64_t bit, int bw) {
+  if (bw == 0) return 0;
+  const int64_t b0 = bit >> 3;
+  const int sh = (int)(bit & 7);
+  uint64_t lo = 0;
+  for (int k = 0; k < 8; k++)
+    if (b0 + k < len) lo |= (uint64_t)chunk[b0 + k] << (8 * k);
+  uint64_t v = lo >> sh;
+  if (sh + bw > 64 && b0 + 8 < len) v |= (uint64_t)chunk[b0 + 8] << (64 - sh);  // sh >= 1 here
+  return bw == 64 ? v : v & ((1ull << bw) - 1);
+}
+
+// The sum of each piece's deltas (0 for a page's first value).
+__global__ void delta_sum_kernel(const DeltaRec* recs, int64_t n, const uint8_t* chunk, int64_t len, uint64_t* sums) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += stride) {
+    const DeltaRec R = recs[m];
+    uint64_t sum = 0;
+    if (R.bw >= 0) {
+      sum = (uint64_t)R.count * R.min_delta;
+      for (int i = 0; i < R.count; i++) sum += unpack64(chunk, len, R.bit + (int64_t)i * R.bw, R.bw);
+    }
+    sums[m] = sum;
+  }
+}
+
+// Exclusive prefix sums of n 64-bit numbers, wrapping; one workgroup of 1024
+// threads walks the input 1024 numbers at a time.
+__global__ __launch_bounds__(1024) void scan_u64_one_block(const uint64_t* in, int64_t n, uint64_t* out) {
+  __shared__ uint64_t wsum[16];
+  const int t = (int)threadIdx.x, lane = t & 63, w = t >> 6;
+  uint64_t carry = 0;
+  for (int64_t base = 0; base < n; base += 1024) {
+    const int64_t i = base + t;
+    const uint64_t x = i < n ? in[i] : 0;
+    uint64_t v = x;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t y = __shfl_up(v, d, 64);
+      if (lane >= d) v += y;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    uint64_t before = carry, total = 0;
+    for (int k = 0; k < 16; k++) {
+      if (k < w) before += wsum[k];
+      total += wsum[k];
+    }
+    if (i < n) out[i] = before + v - x;
+    carry += total;
+    __syncthreads();
+  }
+}
+
+// Each piece's values into out (one thread per piece, at most 64 values).
+__global__ void delta_decode_kernel(const DeltaRec* recs, int64_t n, const uint8_t* chunk, int64_t len,
+                                    const uint64_t* pre, uint64_t* out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += stride) {
+    const DeltaRec R = recs[m];
+    if (R.bw < 0) {
+      out[R.out] = R.first;
+      continue;
+    }
+    uint64_t v = R.first + (pre[m] - pre[R.head]);
+    for (int i = 0; i < R.count; i++) {
+      v += R.min_delta + unpack64(chunk, len, R.bit + (int64_t)i * R.bw, R.bw);
+      out[R.out + i] = v;
+    }
+  }
+}
+
 struct Gather {
   int conv;
-  int width;           // physical value bytes (FLBA: its type length)
+  int width;           // physical value bytes (decimal bytes: the FLBA length; 16 for BYTE_ARRAY)
   double div;          // DECIMAL: 10^scale
   const uint8_t* chunk;
   int64_t len;
   const uint8_t* valid;  // null: REQUIRED
   const uint32_t* rank;  // exclusive scan of valid
   const uint32_t* idx;
+  const uint64_t* delta;  // DELTA_BINARY_PACKED values, at the positions of idx
   const PageDesc* pages;
   int npages;
-  int64_t dict_off;    // dictionary page values (bytes into the chunk)
+  int64_t dict_off;    // dictionary values (bytes into the chunk)
   int64_t dict_n;
   const uint32_t* plain_codes;
   int64_t plain_n;
@@ -428,10 +654,21 @@ struct Gather {
   void* out;
 };
 
-__device__ __forceinline__ uint64_t load_le(const uint8_t* p, int w) {
-  uint64_t x = 0;
-  for (int k = 0; k < w; k++) x |= (uint64_t)p[k] << (8 * k);
-  return x;
+__device__ __forceinline__ uint64_t f64_bits(double d) {
+  uint64_t b;
+  __builtin_memcpy(&b, &d, 8);
+  return b;
+}
+
+// An integer of the column's physical type (INT32 in the low 32 bits) in the
+// column's conversion.
+__device__ __forceinline__ uint64_t convert_int(const Gather& G, uint64_t x) {
+  switch (G.conv) {
+    case C_INT32: case C_DATE: return (uint32_t)x;
+    case C_DEC_I32: return f64_bits((double)(int32_t)(uint32_t)x / G.div);
+    case C_DEC_I64: return f64_bits((double)(int64_t)x / G.div);
+    default: return x;
+  }
 }
 
 __global__ void gather_kernel(Gather G) {
@@ -449,53 +686,48 @@ __global__ void gather_kernel(Gather G) {
       const PageDesc& P = G.pages[lo];
       const int64_t j = G.rank ? (int64_t)(G.rank[r] - G.rank[P.first_row]) : r - P.first_row;
       if (G.conv == C_UTF8) {
-        if (P.dict) {
+        if (P.kind == PK_DICT) {
           const uint32_t k = G.idx[P.first_row + j];
           code = k < (uint64_t)G.dict_n ? G.remap[k] : 0u;
         } else {
           code = P.plain_base + j < G.plain_n ? G.plain_codes[P.plain_base + j] : 0u;
         }
+      } else if (P.kind == PK_BITS) {
+        bits = G.idx[P.first_row + j] & 1u;
+      } else if (P.kind == PK_DELTA) {
+        bits = convert_int(G, G.delta[P.first_row + j]);
       } else if (G.conv == C_BOOL) {
         const int64_t bit = P.values_off * 8 + j;
         bits = (bit >> 3) < G.len ? (G.chunk[bit >> 3] >> (bit & 7)) & 1 : 0;
       } else {
-        int64_t at;
-        if (P.dict) {
+        // the value's bytes: base + k * step for k < width
+        int64_t base = -1, step = 1;
+        if (P.kind == PK_DICT) {
           const uint32_t k = G.idx[P.first_row + j];
-          at = k < (uint64_t)G.dict_n ? G.dict_off + (int64_t)k * G.width : -1;
+          if (k < (uint64_t)G.dict_n) base = G.dict_off + (int64_t)k * G.width;
+        } else if (P.kind == PK_SPLIT) {
+          if (j < P.stride) base = P.values_off + j;
+          step = P.stride;
         } else {
-          at = P.values_off + j * G.width;
+          base = P.values_off + j * G.width;
         }
-        if (at >= 0 && at + G.width <= G.len) {
-          const uint8_t* v = G.chunk + at;
-          switch (G.conv) {
-            case C_INT32: case C_DATE: bits = (uint32_t)load_le(v, 4); break;
-            case C_INT64: bits = load_le(v, 8); break;
-            case C_DOUBLE: bits = load_le(v, 8); break;
-            case C_FLOAT: {
-              const uint32_t u = (uint32_t)load_le(v, 4);
+        const int nb = G.conv == C_DEC_BYTES ? (G.width < 16 ? G.width : 16) : G.width;
+        if (base >= 0 && nb > 0 && base + (int64_t)(nb - 1) * step < G.len) {
+          const uint8_t* v = G.chunk + base;
+          if (G.conv == C_DEC_BYTES) {  // be_i128 of the first 16 bytes at most
+            unsigned __int128 x = (v[0] & 0x80) ? ~(unsigned __int128)0 : 0;
+            for (int k = 0; k < nb; k++) x = (x << 8) | v[(int64_t)k * step];
+            bits = f64_bits((double)(__int128)x / G.div);
+          } else {
+            uint64_t x = 0;
+            for (int k = 0; k < nb; k++) x |= (uint64_t)v[(int64_t)k * step] << (8 * k);
+            if (G.conv == C_FLOAT) {
+              const uint32_t u = (uint32_t)x;
               float f;
               __builtin_memcpy(&f, &u, 4);
-              const double d = (double)f;
-              __builtin_memcpy(&bits, &d, 8);
-              break;
-            }
-            case C_DEC_I32: {
-              const double d = (double)(int32_t)load_le(v, 4) / G.div;
-              __builtin_memcpy(&bits, &d, 8);
-              break;
-            }
-            case C_DEC_I64: {
-              const double d = (double)(int64_t)load_le(v, 8) / G.div;
-              __builtin_memcpy(&bits, &d, 8);
-              break;
-            }
-            default: {  // big-endian two's complement, at most 8 bytes
-              int64_t x = (v[0] & 0x80) ? -1 : 0;
-              for (int k = 0; k < G.width; k++) x = (int64_t)(((uint64_t)x << 8) | v[k]);
-              const double d = (double)x / G.div;
-              __builtin_memcpy(&bits, &d, 8);
-              break;
+              bits = f64_bits((double)f);
+            } else {
+              bits = convert_int(G, x);
             }
           }
         }
@@ -691,8 +923,7 @@ ml_status ml_parquet_open(ml_ctx* ctx, const void* data, int64_t len, ml_parquet
       const SchemaElem& e = schema[i];
       if (e.num_children > 0 || e.repetition == 2) fail(ML_ERR_UNSUPPORTED, "nested parquet column '" + e.name + "'");
       const int cv = conversion_of(e);
-      if (cv == C_DEC_BYTES && (e.type != T_FLBA || e.type_length < 1 || e.type_length > 8))
-        fail(ML_ERR_UNSUPPORTED, "decimal column '" + e.name + "' wider than 8 bytes");
+      if (cv == C_DEC_BYTES && e.type == T_FLBA && e.type_length < 0) corrupt("fixed-length byte array length");
       P->leaves.push_back(e);
       P->conv.push_back(cv);
       P->max_def.push_back(e.repetition == 1 ? 1 : 0);
@@ -810,6 +1041,7 @@ ml_status ml_parquet_read_row_group(ml_parquet* p, int32_t rg, const int32_t* pr
       const uint8_t* raw = p->file + start;
       const int64_t rawlen = m.total_compressed;
       // the pages; a SNAPPY chunk's bodies are decompressed into one host image
+      // (a v2 page keeps its level bytes, stored uncompressed, in front of its values)
       std::vector<PageRef> refs;
       std::vector<uint8_t> image;
       for (size_t pos = 0; pos < (size_t)rawlen;) {
@@ -817,10 +1049,18 @@ ml_status ml_parquet_read_row_group(ml_parquet* p, int32_t rg, const int32_t* pr
         TReader r{raw, (size_t)rawlen, pos};
         parse_page_header(r, pr.h);
         if (pr.h.compressed < 0 || (int64_t)r.pos + pr.h.compressed > rawlen) corrupt("page past the column chunk");
+        const int64_t levels = pr.h.type == PAGE_DATA_V2 ? pr.h.def_len + pr.h.rep_len : 0;
+        if (pr.h.def_len < 0 || pr.h.rep_len < 0 || levels > pr.h.compressed) corrupt("v2 level lengths");
         if (m.codec == 1) {
-          if (pr.h.uncompressed < 0) corrupt("page size");
+          if (pr.h.uncompressed < levels) corrupt("page size");
           pr.body = image.size();
-          snappy_decompress(raw + r.pos, (size_t)pr.h.compressed, (size_t)pr.h.uncompressed, image);
+          image.insert(image.end(), raw + r.pos, raw + r.pos + levels);
+          const uint8_t* vals = raw + r.pos + levels;
+          const size_t vlen = (size_t)(pr.h.compressed - levels);
+          if (pr.h.type != PAGE_DATA_V2 || pr.h.values_compressed)
+            snappy_decompress(vals, vlen, (size_t)(pr.h.uncompressed - levels), image);
+          else
+            image.insert(image.end(), vals, vals + vlen);
           pr.end = image.size();
         } else {
           pr.body = r.pos;
@@ -835,20 +1075,59 @@ ml_status ml_parquet_read_row_group(ml_parquet* p, int32_t rg, const int32_t* pr
       // page headers (host)
       std::vector<PageDesc> pages;
       std::vector<Run> def_runs, idx_runs;
+      std::vector<DeltaRec> deltas;
       std::vector<uint32_t> plain_codes, remap;
+      std::vector<uint8_t> extra;  // values decoded on the host, fixed width, placed after the chunk's bytes
+      std::deque<std::string> decoded;  // strings assembled on the host (stable keys for code_of)
       auto dict = std::make_shared<HostDict>();
       std::unordered_map<std::string_view, uint32_t> code_of;
-      auto intern = [&](std::string_view v) {
+      auto intern = [&](std::string_view v, bool stable) {
         auto it = code_of.find(v);
         if (it != code_of.end()) return it->second;
+        if (!stable) {
+          decoded.emplace_back(v);
+          v = decoded.back();
+        }
         const uint32_t k = (uint32_t)dict->entries.size();
         dict->entries.emplace_back(v);
         code_of.emplace(v, k);
         return k;
       };
+      // BYTE_ARRAY decimals: each value as the 16 big-endian bytes with the same be_i128
+      auto put16 = [&](std::string_view v) {
+        uint8_t o[16];
+        const size_t L = v.size();
+        if (L >= 16) {
+          std::memcpy(o, v.data(), 16);
+        } else {
+          std::memset(o, (L && ((uint8_t)v[0] & 0x80)) ? 0xFF : 0, 16 - L);
+          std::memcpy(o + 16 - L, v.data(), L);
+        }
+        extra.insert(extra.end(), o, o + 16);
+      };
+      const bool byte_dec = cv == C_DEC_BYTES && m.type == T_BYTE_ARRAY;
+      // length-prefixed PLAIN byte arrays in [v, end)
+      auto plain_arrays = [&](size_t v, size_t end, int64_t limit, auto&& emit) {
+        for (int64_t k = 0; k < limit && v < end; k++) {
+          if (end - v < 4) corrupt("byte array value");
+          uint32_t l;
+          std::memcpy(&l, chunk + v, 4);
+          v += 4;
+          if (l > end - v) corrupt("byte array past the page");
+          emit(std::string_view(reinterpret_cast<const char*>(chunk) + v, l));
+          v += l;
+        }
+      };
+      // values held by the runs appended since `before`
+      auto run_values = [&](size_t before) {
+        int64_t got = 0;
+        for (size_t k = before; k < idx_runs.size(); k++) got += idx_runs[k].count;
+        return got;
+      };
       int64_t dict_off = 0, dict_n = 0, row = 0;
       bool have_dict = false;
-      const int width = cv == C_DEC_BYTES ? e.type_length
+      const int width = byte_dec ? 16
+                        : cv == C_DEC_BYTES ? e.type_length
                         : (m.type == T_INT64 || m.type == T_DOUBLE) ? 8 : (m.type == T_BOOLEAN ? 0 : 4);
       for (const PageRef& pr : refs) {
         if (row >= N) break;
@@ -861,76 +1140,144 @@ ml_status ml_parquet_read_row_group(ml_parquet* p, int32_t rg, const int32_t* pr
           dict_n = h.num_values;
           have_dict = true;
           if (dict_n < 0) corrupt("dictionary size");
-          if (cv == C_UTF8) {
-            TReader d{chunk, end, body};
-            for (int64_t k = 0; k < dict_n; k++) {
-              if (end - d.pos < 4) corrupt("dictionary page");
-              uint32_t l;
-              std::memcpy(&l, chunk + d.pos, 4);
-              d.pos += 4;
-              if (l > end - d.pos) corrupt("dictionary entry past the page");
-              remap.push_back(intern(std::string_view(reinterpret_cast<const char*>(chunk) + d.pos, l)));
-              d.pos += l;
-            }
+          if (cv == C_UTF8 || byte_dec) {
+            dict_off = clen + (int64_t)extra.size();
+            int64_t got = 0;
+            plain_arrays(body, end, dict_n, [&](std::string_view v) {
+              if (byte_dec) put16(v);
+              else remap.push_back(intern(v, true));
+              got++;
+            });
+            if (got != dict_n) corrupt("dictionary page");
           } else if ((int64_t)dict_n * width > (int64_t)(end - body)) {
             corrupt("dictionary page too short");
           }
-        } else if (h.type == PAGE_DATA) {
-          const int64_t rows = h.num_values;
+        } else if (h.type == PAGE_DATA || h.type == PAGE_DATA_V2) {
+          const bool v2 = h.type == PAGE_DATA_V2;
+          const int64_t rows = v2 ? h.num_rows : h.num_values;
+          if (v2 && h.num_values != h.num_rows) fail(ML_ERR_UNSUPPORTED, "parquet page with repeated values");
           if (rows < 0 || row + rows > N) fail(ML_ERR_PARQUET, "expected " + std::to_string(N) + " rows, decoded more");
           size_t vpos = body;
-          if (maxd > 0) {  // definition levels: u32 length, then RLE / bit-packed hybrid of bit width 1
-            if (h.def_encoding != ENC_RLE) fail(ML_ERR_UNSUPPORTED, "definition level encoding");
-            if (end - vpos < 4) corrupt("definition levels");
-            uint32_t dl;
-            std::memcpy(&dl, chunk + vpos, 4);
-            vpos += 4;
+          int64_t nvalid = rows;
+          if (v2) vpos += (size_t)h.rep_len;
+          if (maxd > 0) {  // definition levels: RLE / bit-packed hybrid of bit width 1 (v1: after a u32 length)
+            size_t dl;
+            if (v2) {
+              dl = (size_t)h.def_len;
+            } else {
+              if (h.def_encoding != ENC_RLE) fail(ML_ERR_UNSUPPORTED, "definition level encoding");
+              if (end - vpos < 4) corrupt("definition levels");
+              uint32_t l;
+              std::memcpy(&l, chunk + vpos, 4);
+              vpos += 4;
+              dl = l;
+            }
             if (dl > end - vpos) corrupt("definition levels past the page");
             const size_t before = def_runs.size();
             hybrid_runs(chunk, vpos, vpos + dl, 1, rows, row, def_runs);
             int64_t got = 0;
-            for (size_t k = before; k < def_runs.size(); k++) got += def_runs[k].count;
+            nvalid = 0;
+            for (size_t k = before; k < def_runs.size(); k++) {
+              const Run& R = def_runs[k];
+              got += R.count;
+              if (!R.packed) {
+                nvalid += R.value == (uint32_t)maxd ? R.count : 0;
+              } else {
+                for (int64_t q = 0; q < R.count; q++) nvalid += (chunk[R.byte_off + (q >> 3)] >> (q & 7)) & 1;
+              }
+            }
             if (got != rows) corrupt("definition levels short");
             vpos += dl;
+          } else if (v2) {
+            vpos += (size_t)h.def_len;
           }
+          if (vpos > end) corrupt("page levels past the page");
           PageDesc pd{};
           pd.first_row = row;
           pd.rows = (int32_t)rows;
-          if (h.encoding == ENC_PLAIN) {
+          const int enc = h.encoding;
+          const bool ints = m.type == T_INT32 || m.type == T_INT64;
+          const bool fixed = cv != C_BOOL && cv != C_UTF8 && !byte_dec;
+          if (enc == ENC_PLAIN) {
             pd.values_off = (int64_t)vpos;
             if (cv == C_UTF8) {
               pd.plain_base = (int64_t)plain_codes.size();
-              size_t v = vpos;
-              while (v < end) {
-                if (end - v < 4) corrupt("byte array value");
-                uint32_t l;
-                std::memcpy(&l, chunk + v, 4);
-                v += 4;
-                if (l > end - v) corrupt("byte array past the page");
-                plain_codes.push_back(intern(std::string_view(reinterpret_cast<const char*>(chunk) + v, l)));
-                v += l;
-              }
+              plain_arrays(vpos, end, INT64_MAX, [&](std::string_view v) { plain_codes.push_back(intern(v, true)); });
+            } else if (byte_dec) {
+              pd.values_off = clen + (int64_t)extra.size();
+              plain_arrays(vpos, end, nvalid, put16);
+              if (pd.values_off + nvalid * 16 != clen + (int64_t)extra.size()) corrupt("byte array values short");
             }
-          } else if (h.encoding == ENC_RLE_DICTIONARY || h.encoding == ENC_PLAIN_DICTIONARY) {
+          } else if (enc == ENC_RLE_DICTIONARY || enc == ENC_PLAIN_DICTIONARY) {
             if (!have_dict) corrupt("dictionary indices without a dictionary page");
-            pd.dict = 1;
+            pd.kind = PK_DICT;
             if (end - vpos < 1) corrupt("dictionary indices");
             const int bw = chunk[vpos];
             if (bw > 32) corrupt("dictionary index bit width");
+            const size_t before = idx_runs.size();
             hybrid_runs(chunk, vpos + 1, end, bw, rows, row, idx_runs);
+            if (run_values(before) < nvalid) corrupt("dictionary indices short");
+          } else if (enc == ENC_RLE && cv == C_BOOL) {  // u32 length, then the hybrid of bit width 1
+            pd.kind = PK_BITS;
+            if (end - vpos < 4) corrupt("boolean RLE length");
+            uint32_t l;
+            std::memcpy(&l, chunk + vpos, 4);
+            if (l > end - vpos - 4) corrupt("boolean RLE past the page");
+            const size_t before = idx_runs.size();
+            hybrid_runs(chunk, vpos + 4, vpos + 4 + l, 1, nvalid, row, idx_runs);
+            if (run_values(before) < nvalid) corrupt("boolean values short");
+          } else if (enc == ENC_DELTA_BINARY_PACKED && ints) {
+            pd.kind = PK_DELTA;
+            const int64_t head = (int64_t)deltas.size();
+            int64_t total = 0, at = row;
+            delta_walk(chunk, vpos, end, m.type == T_INT32 ? 32 : 64, nvalid, &total,
+                       [&](uint64_t f) { deltas.push_back(DeltaRec{at++, 0, 0, f, head, 1, -1}); },
+                       [&](int64_t bit, int64_t n, int bw, uint64_t md) {
+                         for (int64_t q = 0; q < n; q += 64) {
+                           const int32_t c = (int32_t)std::min<int64_t>(64, n - q);
+                           deltas.push_back(DeltaRec{at, bit + q * bw, md, deltas[(size_t)head].first, head, c, bw});
+                           at += c;
+                         }
+                       });
+            if (total != nvalid) corrupt("delta value count");
+          } else if ((enc == ENC_DELTA_LENGTH_BYTE_ARRAY && m.type == T_BYTE_ARRAY) ||
+                     (enc == ENC_DELTA_BYTE_ARRAY && (m.type == T_BYTE_ARRAY || m.type == T_FLBA))) {
+            pd.plain_base = (int64_t)plain_codes.size();
+            pd.values_off = clen + (int64_t)extra.size();
+            delta_byte_arrays(chunk, vpos, end, enc == ENC_DELTA_BYTE_ARRAY, nvalid, [&](std::string_view v, bool stable) {
+              if (cv == C_UTF8) {
+                plain_codes.push_back(intern(v, stable));
+              } else if (byte_dec) {
+                put16(v);
+              } else {
+                if ((int64_t)v.size() != width) corrupt("fixed-length value of another length");
+                extra.insert(extra.end(), v.begin(), v.end());
+              }
+            });
+          } else if (enc == ENC_BYTE_STREAM_SPLIT && fixed) {
+            pd.kind = PK_SPLIT;
+            pd.values_off = (int64_t)vpos;
+            if (width <= 0 || (end - vpos) % (size_t)width) corrupt("byte stream split size");
+            pd.stride = (int64_t)((end - vpos) / (size_t)width);
+            if (pd.stride < nvalid) corrupt("byte stream split count");
           } else {
-            fail(ML_ERR_UNSUPPORTED, "parquet data page encoding " + std::to_string(h.encoding));
+            fail(ML_ERR_UNSUPPORTED, "parquet data page encoding " + std::to_string(enc));
           }
           pages.push_back(pd);
           row += rows;
-        } else if (h.type == PAGE_DATA_V2) {
-          fail(ML_ERR_UNSUPPORTED, "parquet data page v2");
         }
       }
       if (row != N) fail(ML_ERR_PARQUET, "expected " + std::to_string(N) + " rows, decoded " + std::to_string(row));
       // device: chunk bytes, tables, runs
-      Buffer dchunk(ctx->arena, (size_t)std::max<int64_t>(clen, 1));
+      const int64_t blen = clen + (int64_t)extra.size();
+      Buffer dchunk(ctx->arena, (size_t)std::max<int64_t>(blen, 1));
       hip_check(hipMemcpyAsync(dchunk.ptr, chunk, (size_t)clen, hipMemcpyHostToDevice, s), "upload column chunk");
+      if (!extra.empty()) {
+        host_keep.push_back(std::move(extra));
+        hip_check(hipMemcpyAsync(dchunk.as<uint8_t>() + clen, host_keep.back().data(), host_keep.back().size(),
+                                 hipMemcpyHostToDevice, s),
+                  "upload decoded values");
+      }
       auto put = [&](const void* src, size_t bytes) {  // the host copy lives until the final synchronize
         Buffer b(ctx->arena, std::max<size_t>(bytes, 8));
         host_keep.emplace_back(static_cast<const uint8_t*>(src), static_cast<const uint8_t*>(src) + bytes);
@@ -946,7 +1293,7 @@ ml_status ml_parquet_read_row_group(ml_parquet* p, int32_t rg, const int32_t* pr
       g.width = width;
       g.div = (cv == C_DEC_I32 || cv == C_DEC_I64 || cv == C_DEC_BYTES) ? pow10i(e.scale) : 1.0;
       g.chunk = dchunk.as<uint8_t>();
-      g.len = clen;
+      g.len = blen;
       g.pages = (const PageDesc*)dpages.ptr;
       g.npages = (int)pages.size();
       g.dict_off = dict_off;
@@ -984,6 +1331,22 @@ ml_status ml_parquet_read_row_group(ml_parquet* p, int32_t rg, const int32_t* pr
         expand(idx_runs, nullptr, nullptr, didx.as<uint32_t>());
         g.idx = didx.as<uint32_t>();
       }
+      Buffer ddelta;
+      if (!deltas.empty()) {  // DELTA_BINARY_PACKED: piece sums, their prefix sums, then the values
+        const int64_t nrec = (int64_t)deltas.size();
+        ddelta = Buffer(ctx->arena, (size_t)N * 8);
+        Buffer drecs = put(deltas.data(), deltas.size() * sizeof(DeltaRec));
+        Buffer dsums(ctx->arena, (size_t)nrec * 8), dpre(ctx->arena, (size_t)nrec * 8);
+        const DeltaRec* recs = (const DeltaRec*)drecs.ptr;
+        delta_sum_kernel<<<grid_of(nrec), 256, 0, s>>>(recs, nrec, g.chunk, clen, dsums.as<uint64_t>());
+        scan_u64_one_block<<<1, 1024, 0, s>>>(dsums.as<uint64_t>(), nrec, dpre.as<uint64_t>());
+        delta_decode_kernel<<<grid_of(nrec), 256, 0, s>>>(recs, nrec, g.chunk, clen, dpre.as<uint64_t>(),
+                                                            ddelta.as<uint64_t>());
+        g.delta = ddelta.as<uint64_t>();
+        keep.push_back(std::move(drecs));
+        keep.push_back(std::move(dsums));
+        keep.push_back(std::move(dpre));
+      }
       Buffer dplain = put(plain_codes.data(), plain_codes.size() * 4);
       Buffer dremap = put(remap.data(), remap.size() * 4);
       g.plain_codes = dplain.as<uint32_t>();
@@ -997,6 +1360,7 @@ ml_status ml_parquet_read_row_group(ml_parquet* p, int32_t rg, const int32_t* pr
       keep.push_back(std::move(dchunk));
       keep.push_back(std::move(dpages));
       keep.push_back(std::move(didx));
+      keep.push_back(std::move(ddelta));
       keep.push_back(std::move(dplain));
       keep.push_back(std::move(dremap));
       res->cols.push_back(std::move(rc));

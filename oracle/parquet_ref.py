@@ -4,13 +4,15 @@ tests/, never by the product path).
 MiniLake reads Parquet through the `parquet` crate, version 60.0.0 (Cargo.lock),
 which is not part of the reference tree. This module restates the published
 Apache Parquet format for the subset the scan supports (flat schema,
-UNCOMPRESSED and SNAPPY chunks, data pages v1, PLAIN and dictionary encodings, RLE /
-bit-packed hybrid definition levels and indices) and MiniLake's own code on
-top of it:
+UNCOMPRESSED and SNAPPY chunks, data pages v1 and v2, the PLAIN, dictionary, RLE
+(booleans), DELTA_BINARY_PACKED, DELTA_LENGTH_BYTE_ARRAY, DELTA_BYTE_ARRAY and
+BYTE_STREAM_SPLIT encodings, RLE / bit-packed hybrid definition levels and
+indices) and MiniLake's own code on top of it:
 
 - Conversion::for_column and the value conversions of read_column_chunk
   (crates/minilake-storage/src/parquet_reader.rs:55-80, 210-287): DATE stays
-  days, DECIMAL -> f64 as unscaled / 10f64.powi(scale), FLOAT -> f64, strings
+  days, DECIMAL -> f64 as unscaled / 10f64.powi(scale) (decimal bytes through
+  be_i128, parquet_reader.rs:156-167), FLOAT -> f64, strings
   as Python str (encode_strings' dictionary choice is a representation detail);
 - convert_stats (parquet_reader.rs:290-358);
 - PrunePredicate::may_match with ScalarValue::compare (stats.rs:53-70,
@@ -224,6 +226,84 @@ def _hybrid(b: bytes, pos: int, end: int, bw: int, want: int) -> List[int]:
     return out[:want]
 
 
+def be_i128(r: bytes) -> int:
+    """be_i128 (parquet_reader.rs:156-167): big-endian two's complement of the
+    first 16 bytes at most, the sign taken from the first byte; no bytes -> 0."""
+    v = -1 if r and r[0] & 0x80 else 0
+    for x in r[:16]:
+        v = (v << 8) | x
+    v &= (1 << 128) - 1
+    return v - (1 << 128) if v >> 127 else v
+
+
+def delta_binary_packed(b: bytes, pos: int, end: int, bits: int):
+    """DELTA_BINARY_PACKED, from the format's published description: a header
+    <block size> <miniblocks per block> <total count> <first value (zigzag)>,
+    then blocks of <min delta (zigzag)> <a bit width byte per miniblock>
+    <miniblocks, bit-packed LSB first>; each value is the previous one + min
+    delta + the unpacked number, wrapping at `bits` bits. Miniblocks the last
+    block does not need take no bytes. Returns the values as unsigned `bits`-bit
+    integers and the position after the last miniblock read."""
+    t = _Thrift(b[:end], pos)
+    block, nmb, total = t.varint(), t.varint(), t.varint()
+    first = t.zz()
+    if block == 0 or block % 128 or nmb == 0 or block % nmb or (block // nmb) % 32:
+        raise ParquetError("delta header")
+    per, mask = block // nmb, (1 << bits) - 1
+    out: List[int] = []
+    if total == 0:
+        return out, t.pos
+    v = first & mask
+    out.append(v)
+    while len(out) < total:
+        md = t.zz()
+        widths = [t.byte() for _ in range(nmb)]
+        for w in widths:
+            if len(out) >= total:
+                break
+            if w > bits:
+                raise ParquetError("delta bit width")
+            nbytes = per * w // 8
+            if t.pos + nbytes > end:
+                raise ParquetError("delta miniblock past the page")
+            chunk = int.from_bytes(b[t.pos:t.pos + nbytes], "little")
+            t.pos += nbytes
+            for j in range(min(per, total - len(out))):
+                v = (v + md + ((chunk >> (j * w)) & ((1 << w) - 1))) & mask
+                out.append(v)
+    return out, t.pos
+
+
+def delta_length_byte_array(b: bytes, pos: int, end: int):
+    """DELTA_LENGTH_BYTE_ARRAY: the lengths as DELTA_BINARY_PACKED (32-bit),
+    then the values' bytes back to back."""
+    lens, p = delta_binary_packed(b, pos, end, 32)
+    out = []
+    for ln in lens:
+        if ln >> 31 or p + ln > end:
+            raise ParquetError("delta length")
+        out.append(b[p:p + ln])
+        p += ln
+    return out, p
+
+
+def delta_byte_array(b: bytes, pos: int, end: int):
+    """DELTA_BYTE_ARRAY: prefix lengths as DELTA_BINARY_PACKED, then the
+    suffixes as DELTA_LENGTH_BYTE_ARRAY; value k = the first prefix[k] bytes of
+    value k-1, then suffix k."""
+    pre, p = delta_binary_packed(b, pos, end, 32)
+    suf, p = delta_length_byte_array(b, p, end)
+    if len(pre) != len(suf):
+        raise ParquetError("delta prefix count")
+    out, prev = [], b""
+    for k, s in zip(pre, suf):
+        if k > len(prev):
+            raise ParquetError("delta prefix")
+        prev = prev[:k] + s
+        out.append(prev)
+    return out, p
+
+
 class ParquetFile:
     """Table::open (table.rs:59-134) over one file's bytes."""
 
@@ -290,17 +370,25 @@ class ParquetFile:
             raise Unsupported(f"codec {codec}")
         start = md.get(11) if md.get(11) is not None and md.get(11) < md[9] else md[9]
         raw = self.data[start:start + md[7]]
-        # the pages' bodies, uncompressed, one after another (headers dropped)
-        b, bodies, pos = b"", [], 0
+        # the pages' bodies, uncompressed, one after another (headers dropped);
+        # a v2 page keeps its levels uncompressed in front of its values
+        b, bodies, pos = bytearray(), [], 0
         while pos < len(raw):
             t = _Thrift(raw, pos)
             h = t.read(12)
             body = raw[t.pos:t.pos + h[3]]
-            if codec == 1:
+            if h[1] == 3:
+                lv = h[8].get(5, 0) + h[8].get(6, 0)
+                vals = body[lv:]
+                if codec == 1 and h[8].get(7, True):
+                    vals = snappy_decompress(vals)
+                body = body[:lv] + vals
+            elif codec == 1:
                 body = snappy_decompress(body)
             bodies.append((h, len(b), len(b) + len(body)))
             b += body
             pos = t.pos + h[3]
+        b = bytes(b)
         width = col.type_length if col.conv == "dec_bytes" else (8 if col.phys in (T_INT64, T_DOUBLE) else 4)
         dictionary = None
         out: list = []
@@ -309,58 +397,97 @@ class ParquetFile:
                 break
             if h[1] == 2:
                 dictionary = self._plain(col, b, body, end, h[7][1], width)
-            elif h[1] == 0:
-                dph = h[5]
-                rows = dph[1]
-                v = body
-                valid = [True] * rows
-                if col.optional:
-                    dl = struct.unpack_from("<I", b, v)[0]
-                    levels = _hybrid(b, v + 4, v + 4 + dl, 1, rows)
-                    valid = [lv == 1 for lv in levels]
-                    v += 4 + dl
-                nvalid = sum(valid)
-                if dph[2] == 0:
-                    vals = self._plain(col, b, v, end, nvalid, width)
-                elif dph[2] in (2, 8):
-                    bw = b[v]
-                    vals = [dictionary[k] for k in _hybrid(b, v + 1, end, bw, nvalid)]
-                else:
-                    raise Unsupported(f"encoding {dph[2]}")
+            elif h[1] in (0, 3):
+                if h[1] == 0:
+                    rows, enc, v = h[5][1], h[5][2], body
+                    if col.optional:
+                        dl = struct.unpack_from("<I", b, v)[0]
+                        levels = _hybrid(b, v + 4, v + 4 + dl, 1, rows)
+                        v += 4 + dl
+                else:  # v2: repetition levels, definition levels (no length prefix), values
+                    v2 = h[8]
+                    rows, enc = v2[3], v2[4]
+                    if v2[1] != rows:
+                        raise Unsupported("repeated values")
+                    v = body + v2.get(6, 0)
+                    if col.optional:
+                        levels = _hybrid(b, v, v + v2.get(5, 0), 1, rows)
+                    v += v2.get(5, 0)
+                valid = [lv == 1 for lv in levels] if col.optional else [True] * rows
+                vals = self._values(col, enc, b, v, end, sum(valid), width, dictionary)
                 it = iter(vals)
                 out.extend(next(it) if ok else None for ok in valid)
-            elif h[1] == 3:
-                raise Unsupported("data page v2")
         if len(out) != nrows:
             raise ParquetError(f"expected {nrows} rows, decoded {len(out)}")
         return out
+
+    def _values(self, col: Column, enc: int, b: bytes, v: int, end: int, n: int, width: int, dictionary) -> list:
+        """The `n` non-NULL values of a data page in encoding `enc`."""
+        conv = col.conv
+        if enc == 0:
+            return self._plain(col, b, v, end, n, width)
+        if enc in (2, 8):
+            return [dictionary[k] for k in _hybrid(b, v + 1, end, b[v], n)]
+        if enc == 3 and conv == "bool":  # RLE: u32 length, then the hybrid of bit width 1
+            ln = struct.unpack_from("<I", b, v)[0]
+            return [x == 1 for x in _hybrid(b, v + 4, v + 4 + ln, 1, n)]
+        if enc == 5 and col.phys in (T_INT32, T_INT64):
+            bits = 32 if col.phys == T_INT32 else 64
+            ints, _ = delta_binary_packed(b, v, end, bits)
+            if len(ints) != n:
+                raise ParquetError("delta value count")
+            return self._ints(col, [x - (1 << bits) if x >> (bits - 1) else x for x in ints])
+        if (enc == 6 and col.phys == T_BYTE_ARRAY) or (enc == 7 and col.phys in (T_BYTE_ARRAY, T_FLBA)):
+            raws, _ = (delta_length_byte_array if enc == 6 else delta_byte_array)(b, v, end)
+            if len(raws) != n:
+                raise ParquetError("delta value count")
+            if conv == "utf8":
+                return [r.decode("utf-8", errors="surrogateescape") for r in raws]
+            if col.phys == T_FLBA and any(len(r) != width for r in raws):
+                raise ParquetError("fixed-length value of another length")
+            return [float(be_i128(r)) / _pow10i(col.scale) for r in raws]
+        if enc == 9 and conv not in ("bool", "utf8") and col.phys != T_BYTE_ARRAY:
+            if (end - v) % width:
+                raise ParquetError("byte stream split size")
+            m = (end - v) // width
+            if m < n:
+                raise ParquetError("byte stream split count")
+            return self._fixed(col, [bytes(b[v + k * m + j] for k in range(width)) for j in range(n)])
+        raise Unsupported(f"encoding {enc}")
 
     def _plain(self, col: Column, b: bytes, pos: int, end: int, n: int, width: int) -> list:
         conv = col.conv
         if conv == "bool":
             return [bool((b[pos + j // 8] >> (j % 8)) & 1) for j in range(n)]
-        if conv == "utf8":
+        if conv == "utf8" or col.phys == T_BYTE_ARRAY:
             vals = []
             for _ in range(n):
                 ln = struct.unpack_from("<I", b, pos)[0]
-                vals.append(b[pos + 4:pos + 4 + ln].decode("utf-8", errors="surrogateescape"))
+                vals.append(b[pos + 4:pos + 4 + ln])
                 pos += 4 + ln
-            return vals
-        raw = [b[pos + j * width:pos + (j + 1) * width] for j in range(n)]
-        if conv in ("int32", "date"):
-            return [struct.unpack("<i", r)[0] for r in raw]
-        if conv == "int64":
-            return [struct.unpack("<q", r)[0] for r in raw]
+            if conv == "utf8":
+                return [r.decode("utf-8", errors="surrogateescape") for r in vals]
+            return [float(be_i128(r)) / _pow10i(col.scale) for r in vals]
+        return self._fixed(col, [b[pos + j * width:pos + (j + 1) * width] for j in range(n)])
+
+    def _fixed(self, col: Column, raw: list) -> list:
+        """Fixed-width little-endian values (big-endian for decimal bytes)."""
+        conv = col.conv
         if conv == "double":
             return [struct.unpack("<d", r)[0] for r in raw]
         if conv == "float":
             return [float(struct.unpack("<f", r)[0]) for r in raw]
-        d = _pow10i(col.scale)
-        if conv == "dec_i32":
-            return [struct.unpack("<i", r)[0] / d for r in raw]
-        if conv == "dec_i64":
-            return [struct.unpack("<q", r)[0] / d for r in raw]
-        return [int.from_bytes(r, "big", signed=True) / d for r in raw]  # be_i128 (parquet_reader.rs:156-167)
+        if conv == "dec_bytes":
+            d = _pow10i(col.scale)
+            return [float(be_i128(r)) / d for r in raw]
+        fmt = "<i" if col.phys == T_INT32 else "<q"
+        return self._ints(col, [struct.unpack(fmt, r)[0] for r in raw])
+
+    def _ints(self, col: Column, xs: list) -> list:
+        if col.conv in ("dec_i32", "dec_i64"):
+            d = _pow10i(col.scale)
+            return [x / d for x in xs]
+        return xs
 
     # PrunePredicate::may_match (stats.rs:53-70) / Table::prune (table.rs:165-172)
     def prune(self, preds) -> List[int]:

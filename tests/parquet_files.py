@@ -4,6 +4,7 @@ writer of the format) in memory."""
 import datetime
 import decimal
 import io
+import struct
 
 import numpy as np
 
@@ -43,6 +44,40 @@ def write(table, **kw) -> bytes:
     buf = io.BytesIO()
     pq.write_table(table, buf, **opts)
     return buf.getvalue()
+
+
+def byte_array_decimals(values, precision, scale, **kw) -> bytes:
+    """A file of one OPTIONAL BYTE_ARRAY DECIMAL(precision, scale) column "dec"
+    holding `values` (big-endian two's complement bytes, or None). pyarrow
+    writes DECIMAL only as fixed-length bytes, so it writes a binary column and
+    the footer's schema element of it gets converted_type DECIMAL, scale and
+    precision (Thrift compact fields 6, 7, 8) spliced in after its name."""
+    import pyarrow as pa
+    assert 0 <= scale < 64 and 0 < precision < 64
+    data = write(pa.table({"dec": pa.array(values, pa.binary())}), **kw)
+    flen = struct.unpack_from("<I", data, len(data) - 8)[0]
+    start = len(data) - 8 - flen
+    foot = data[start:len(data) - 8]
+    name = b"\x18\x03dec"  # field 4 (name) after field 3, 3 bytes
+    at = foot.index(name) + len(name)
+    assert foot[at] == 0  # the element ends after its name
+    add = bytes([0x25, 2 * 5, 0x15, 2 * scale, 0x15, 2 * precision])  # i32 fields, zigzag values
+    foot = foot[:at] + add + foot[at:]
+    return data[:start] + foot + struct.pack("<I", len(foot)) + b"PAR1"
+
+
+def decimal_bytes_sample(n, seed, min_len=1, max_len=16):
+    """Big-endian two's complement values of every length in [min_len,
+    max_len], some NULL."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        if k % 13 == 5:
+            out.append(None)
+            continue
+        ln = int(rng.integers(min_len, max_len + 1))
+        out.append(bytes(rng.integers(0, 256, ln).astype(np.uint8).tobytes()))
+    return out
 
 
 def as_engine_value(v):

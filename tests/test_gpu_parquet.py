@@ -1,9 +1,11 @@
 """Device Parquet scan (read_row_group, table.rs:204-240; read_column_chunk,
 parquet_reader.rs:210-287) against the oracle's decoder on files written by
 pyarrow: every column type, NULLs, dictionary and PLAIN pages, many small
-pages, decimals as fixed-length bytes and as integers, REQUIRED columns,
-SNAPPY chunks, projections, a scan result as the source of an aggregate, and
-other codecs refused."""
+pages, data pages v2, the RLE / DELTA / BYTE_STREAM_SPLIT encodings, decimals
+as fixed-length bytes (up to 16), as BYTE_ARRAY (0 to 20 bytes) and as
+integers, REQUIRED columns, SNAPPY chunks, projections, a scan result as the
+source of an aggregate, a 300,000-row row group of DELTA pages against
+pyarrow, and other codecs refused."""
 
 import numpy as np
 import pytest
@@ -13,7 +15,7 @@ from minilake_amd.abi import AGG_COUNT_STAR, AGG_MAX, AGG_SUM, FLOAT64, INT64, U
 from oracle import parquet_ref as R
 from tests import parquet_files as F
 from tests.compare import assert_tables_equal
-from tests.test_parquet import WRITES, _files
+from tests.test_parquet import ENCODINGS, EXTRA_FILES, ORACLE_ONLY, WRITES, _files
 
 pytestmark = pytest.mark.gpu
 pa = pytest.importorskip("pyarrow")
@@ -24,7 +26,7 @@ def files():
     return _files()
 
 
-@pytest.mark.parametrize("kind", list(WRITES) + ["required", "edge"])
+@pytest.mark.parametrize("kind", list(WRITES) + EXTRA_FILES + ORACLE_ONLY)
 def test_device_scan_matches_oracle(gpu_engine, files, kind):
     from minilake_amd.gpu import ParquetTable
     data = files[kind]
@@ -39,6 +41,29 @@ def test_device_scan_matches_oracle(gpu_engine, files, kind):
     proj = [t.num_columns - 1, 0]
     got = t.read_row_group(0, proj).to_host()
     assert [c.decoded() for c in got.columns] == [f.read_column(0, c) for c in proj]
+
+
+@pytest.mark.parametrize("version", ["1.0", "2.0"])
+def test_device_scan_of_one_large_row_group(gpu_engine, version):
+    """300,000 rows in one row group (thousands of DELTA miniblocks per column
+    chunk, so the prefix sums cross many 1024-number rounds), against
+    pyarrow's reading of the same file."""
+    from minilake_amd.gpu import ParquetTable
+    import pyarrow.parquet as pq
+    import io
+    n = 300_000
+    t = F.sample_table(n, seed=9)
+    data = F.write(t, use_dictionary=False, column_encoding=ENCODINGS, row_group_size=n, data_page_size=1 << 20,
+                   data_page_version=version, compression="SNAPPY")
+    pt = ParquetTable(data, gpu_engine)
+    assert pt.num_row_groups == 1
+    got = pt.read_row_group(0).to_host()
+    want = pq.ParquetFile(io.BytesIO(data)).read_row_group(0)
+    for c, name in enumerate(pt.names):
+        if name in ("price", "qty"):
+            continue  # decimals: the oracle-sized tests cover them
+        exp = [F.as_engine_value(v) for v in want.column(c).to_pylist()]
+        assert got.columns[c].decoded() == exp, name
 
 
 def test_scan_result_feeds_an_aggregate(gpu_engine, oracle_engine, files):
