@@ -373,11 +373,8 @@ const uint8_t* Compiler::dict_predicate(const HostDict& d, int op, const std::st
     ring_upload(ctx_, at, per_entry.data(), per_entry.size());
     return (const uint8_t*)at;
   }
-  Buffer b(ctx_->arena, per_entry.size());
-  ring_upload(ctx_, b.ptr, per_entry.data(), per_entry.size());
-  const uint8_t* p = b.as<uint8_t>();
-  tables_.push_back(std::move(b));
-  return p;
+  consts.push_back(ctx_->const_upload(per_entry.data(), per_entry.size()));
+  return consts.back()->as<uint8_t>();
 }
 
 Compiler::Val Compiler::gen(const NodePtr& n) {
@@ -641,10 +638,8 @@ Prog Compiler::compile_hash_input(const NodePtr& n, bool* prehashed) {
       ring_upload(ctx_, at, hs.data(), hs.size() * 8);
       tp = (const uint64_t*)at;
     } else {
-      Buffer b(ctx_->arena, hs.size() * 8);
-      ring_upload(ctx_, b.ptr, hs.data(), hs.size() * 8);
-      tp = b.as<uint64_t>();
-      tables_.push_back(std::move(b));
+      consts.push_back(ctx_->const_upload(hs.data(), hs.size() * 8));
+      tp = consts.back()->as<uint64_t>();
     }
     Prog p = compile(n, -1);
     depth_ = 1;

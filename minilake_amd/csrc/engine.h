@@ -9,6 +9,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/minilake_gpu.h"
@@ -203,6 +204,15 @@ struct ml_ctx {
   std::vector<hipEvent_t> event_pool;  // events of taken stats, reused
   hipEvent_t pooled_event();
   void launch(const mlg::Plan& P, int grid, int block, size_t lds);
+  // Immutable device copies of small host arrays (instruction arrays, lookup
+  // tables), keyed by their bytes: a plan compiled again from the same
+  // expressions over the same columns reuses them instead of copying again.
+  // Holders keep an entry alive after the cache drops it.
+  std::shared_ptr<mlg::Buffer> const_upload(const void* p, size_t n);
+  std::unordered_map<std::string, std::shared_ptr<mlg::Buffer>> const_cache;
+  size_t const_cache_bytes = 0;
+  static constexpr size_t CONST_CACHE_BYTES = 4 << 20;
+  static constexpr size_t CONST_CACHE_ENTRIES = 4096;
 };
 
 namespace mlg {
@@ -290,6 +300,8 @@ class Compiler {
   Prog compile_load(const VCol& c);
   // When set, lookup tables are carved from this region, not the context's pool.
   ScratchBump* bump = nullptr;
+  // lookup tables shared through the context's constant cache
+  std::vector<std::shared_ptr<Buffer>> consts;
 
  private:
   struct Val {
@@ -380,7 +392,7 @@ struct PipelineBuild {
   int64_t active_rows = 0;
   bool expanding = false;
   Compiler comp;
-  Buffer dev_ins;
+  std::shared_ptr<Buffer> dev_ins;  // the instructions, from the context's constant cache
   // When set, every device buffer of the plan (expanded validity, lookup
   // tables, instructions) is carved from this region, not the context's pool.
   ScratchBump* bump = nullptr;

@@ -117,6 +117,24 @@ static void upload(ml_ctx* ctx, void* dst, const void* src, size_t n ML_CALLER) 
   ctx->ring_head += (n + 255) & ~(size_t)255;
 }
 void ring_upload(ml_ctx* ctx, void* dst, const void* src, size_t n) { upload(ctx, dst, src, n); }
+}  // namespace mlg
+
+std::shared_ptr<mlg::Buffer> ml_ctx::const_upload(const void* p, size_t n) {
+  std::string key(static_cast<const char*>(p), n);
+  auto it = const_cache.find(key);
+  if (it != const_cache.end()) return it->second;
+  if (const_cache_bytes + n > CONST_CACHE_BYTES || const_cache.size() >= CONST_CACHE_ENTRIES) {
+    const_cache.clear();
+    const_cache_bytes = 0;
+  }
+  auto b = std::make_shared<mlg::Buffer>(arena, std::max<size_t>(n, 1));
+  mlg::upload(this, b->ptr, p, n);
+  const_cache_bytes += n;
+  const_cache.emplace(std::move(key), b);
+  return b;
+}
+
+namespace mlg {
 static void d2h(ml_ctx* ctx, void* dst, const void* src, size_t n ML_CALLER) {
   ML_NOTE("d2h");
   const uint64_t seq = ctx->check_seq;
@@ -322,11 +340,11 @@ void PipelineBuild::upload() {
   Instr* dst;
   if (bump) {
     dst = (Instr*)bump->take(ins.size() * sizeof(Instr));
+    mlg::upload(ctx, dst, ins.data(), ins.size() * sizeof(Instr));
   } else {
-    dev_ins = Buffer(ctx->arena, ins.size() * sizeof(Instr));
-    dst = dev_ins.as<Instr>();
+    dev_ins = ctx->const_upload(ins.data(), ins.size() * sizeof(Instr));
+    dst = dev_ins->as<Instr>();
   }
-  mlg::upload(ctx, dst, ins.data(), ins.size() * sizeof(Instr));
   P.ins = dst;
   P.nins = (int32_t)ins.size();
 }
@@ -2121,13 +2139,8 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
     // one append kernel, its count left on the device for the dense build's
     // first kernel (rows in any order: the table orders them)
     ch.count_buf = Buffer(ctx->arena, 16);
-    if (identity) {
-      const uint64_t cw[2] = {(uint64_t)B.active_rows, 0};
-      upload(ctx, ch.count_buf.ptr, cw, 16);
-      B.P.out_identity = 1;
-    } else {
-      zero(ctx, ch.count_buf.ptr, 16);
-    }
+    if (identity && B.active_rows > 0) B.P.out_identity = 1;  // the kernel writes the count
+    else zero(ctx, ch.count_buf.ptr, 16);
     B.P.out_count = ch.count_buf.as<uint64_t>();
     B.P.out_overflow = (uint32_t*)(ch.count_buf.as<uint64_t>() + 1);
     if (B.active_rows > 0) ctx->launch(B.P, B.grid(), block_of(ctx), 0);

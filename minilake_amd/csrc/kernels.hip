@@ -1055,6 +1055,14 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t cursor = 0;
   if constexpr (SINK == SINK_COLLECT) cursor = P.thread_offsets ? P.thread_offsets[tid] : 0;
+  if constexpr (SINK == SINK_JOIN_BUILD) {
+    // every source row is written at its own index: the count is the range
+    // (and the overflow word stays clear)
+    if (P.out_identity && tid == 0) {
+      P.out_count[0] = (uint64_t)(P.end - P.begin);
+      P.out_count[1] = 0;
+    }
+  }
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = P.begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.end; i += stride) {
     uint32_t rows[MAX_STAGES];
