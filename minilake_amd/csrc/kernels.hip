@@ -729,13 +729,22 @@ __device__ __forceinline__ void acc_apply_lds(int kind, uint64_t* w, uint64_t v)
   }
 }
 
+// Wait until this thread's earlier stores have completed. After agent-scope
+// atomic stores (sc1) this orders them before the thread's later stores at
+// agent scope, which is what an agent-scope release does for atomic stores;
+// the release would also write back the whole L2 (buffer_wbl2), which it
+// needs only for non-atomic stores and which costs ~20x more per publication.
+__device__ __forceinline__ void wait_for_stores() {
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt and lgkmcnt not waited on
+}
+
 // Find or insert the group with key words w; returns its slot or -1 (table full).
 // Slot words are read with relaxed agent-scope loads, which bypass the CU's
 // non-coherent L1: an acquire load would add a full wait and an L1 invalidate
-// to every probe. The inserter stores the keys and then the header (release);
-// a reader compares keys only after its header load returned that header (the
-// key loads are issued under that branch), so the keys it reads from L2 are
-// the inserted ones.
+// to every probe. The inserter stores the keys as agent-scope atomics, waits
+// for them, then stores the header; a reader compares keys only after its
+// header load returned that header (the key loads are issued under that
+// branch), so the keys it reads are the inserted ones.
 // Returns -1 when the row found no slot, -2 when an earlier row of a
 // pre-sized table already found none (the batch is rerun: nothing to record).
 __device__ __forceinline__ int64_t agg_slot(const Plan& P, const uint64_t (&w)[MAX_KEYS + 1], int nkw, uint64_t h) {
@@ -755,8 +764,9 @@ __device__ __forceinline__ int64_t agg_slot(const Plan& P, const uint64_t (&w)[M
         if (n >= P.agg_fill_limit) atomicOr(P.agg_overflow, P.agg_presized ? 3u : 1u);  // past the load-factor limit
 #pragma unroll
         for (int k = 0; k < MAX_KEYS + 1; k++)
-          if (k < nkw) s[1 + k] = w[k];
-        __hip_atomic_store(s, hdr, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          if (k < nkw) __hip_atomic_store(s + 1 + k, w[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        wait_for_stores();
+        __hip_atomic_store(s, hdr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return (int64_t)pos;
       }
       cur = prev;
@@ -1847,15 +1857,16 @@ __global__ void __launch_bounds__(256) key_minmax_kernel(const uint64_t* keys, i
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    partials[3 * blockIdx.x] = lo[0];
-    partials[3 * blockIdx.x + 1] = hi[0];
-    partials[3 * blockIdx.x + 2] = cnt[0];
-    __threadfence();
+    // agent-scope stores, completed before the count that publishes them
+    __hip_atomic_store(partials + 3 * blockIdx.x, lo[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(partials + 3 * blockIdx.x + 1, hi[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(partials + 3 * blockIdx.x + 2, cnt[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    wait_for_stores();
     last = atomicAdd(done, 1u) == gridDim.x - 1;
   }
   __syncthreads();
   if (!last) return;
-  __threadfence();
+  // the partials are read with agent-scope loads below
   a = INT64_MAX;
   b = INT64_MIN;
   c = 0;
