@@ -2717,14 +2717,18 @@ static void topn_reduce(ml_topn* T, Materialized& m) {
                                                split.as<int64_t>(), ctx->stream);
     launch_topn_rows(sorted, nw, m_out, rows.as<int64_t>(), ctx->stream);
   } else {
-    // repeated tile selection until one tile remains
+    // repeated tile selection until one tile remains: rank selection over
+    // 256-entry tiles for a small limit, sorted 1024-entry tiles otherwise
+    const bool by_rank = L <= TOPN_RANK_LIMIT;
+    const int tile = by_rank ? TOPN_RANK_TILE : TOPN_TILE;
     int64_t cur = n;
     Buffer a = std::move(words);
     for (;;) {
-      const int64_t tiles = (cur + TOPN_TILE - 1) / TOPN_TILE;
-      const int keep = (int)std::min<int64_t>(L, TOPN_TILE);
+      const int64_t tiles = (cur + tile - 1) / tile;
+      const int keep = (int)std::min<int64_t>(L, tile);
       Buffer b(ctx->arena, (size_t)tiles * keep * W * 8);
-      launch_topn_tiles(a.as<uint64_t>(), cur, nw, keep, b.as<uint64_t>(), ctx->stream);
+      if (by_rank) launch_topn_rank(a.as<uint64_t>(), cur, nw, keep, b.as<uint64_t>(), ctx->stream);
+      else launch_topn_tiles(a.as<uint64_t>(), cur, nw, keep, b.as<uint64_t>(), ctx->stream);
       a = std::move(b);
       cur = tiles * keep;
       if (tiles == 1) break;

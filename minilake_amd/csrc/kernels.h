@@ -189,6 +189,12 @@ void launch_compact(const uint8_t* flags, int64_t n, uint32_t* block_counts, uin
                     void* out, int out_u32, hipStream_t s);
 void launch_sort_keys(const Plan& P, const SortKeys& K, int64_t n, uint64_t* words, hipStream_t s);
 void launch_topn_tiles(const uint64_t* in, int64_t n, int nw, int keep, uint64_t* out, hipStream_t s);
+// The first `keep` (at most TOPN_RANK_LIMIT) entries of each TOPN_RANK_TILE-entry tile, by rank.
+constexpr int TOPN_RANK_TILE = 256;
+constexpr int TOPN_RANK_SPLIT = 4;  // threads counting one entry's rank
+constexpr int TOPN_RANK_LIMIT = 32;
+constexpr int MAX_SORT_WORDS = 6;  // key words + the row index
+void launch_topn_rank(const uint64_t* in, int64_t n, int nw, int keep, uint64_t* out, hipStream_t s);
 void launch_topn_rows(const uint64_t* words, int nw, int64_t n, int64_t* rows_out, hipStream_t s);
 // Full sort of n entries of nw + 1 words (see merge_pass_kernel): a and b
 // hold sort_padded_entries(n) entries each; returns the one with the result.

@@ -2836,6 +2836,58 @@ __global__ void __launch_bounds__(256) topn_tile_kernel(const uint64_t* in, int6
   for (int e = threadIdx.x; e < keep * W; e += blockDim.x) dst[e] = e < m * W ? sh[e] : ~0ull;
 }
 
+// Rank selection for a small limit: tiles of TOPN_RANK_TILE entries; the
+// rank of an entry is the number of entries of its tile that order before it
+// (sort_less, ties by position, so every entry gets its own rank), counted by
+// TOPN_RANK_SPLIT threads over disjoint quarters of the tile and summed; the
+// entry is written at its rank when that is below `keep`. A short tile's
+// missing ranks are the +inf padding. No sorting network: two barriers, and
+// the tile's entries are read as LDS broadcasts. W (words per entry) is a
+// template parameter, so the comparison has no per-word branches.
+template <int W>
+__global__ void __launch_bounds__(TOPN_RANK_TILE * TOPN_RANK_SPLIT) topn_rank_kernel(const uint64_t* in, int64_t n,
+                                                                                      int keep, uint64_t* out) {
+  __shared__ uint64_t sh[TOPN_RANK_TILE * W];
+  __shared__ int part[TOPN_RANK_SPLIT][TOPN_RANK_TILE];
+  const int64_t tile0 = (int64_t)blockIdx.x * TOPN_RANK_TILE;
+  const int cnt = (int)((n - tile0) < TOPN_RANK_TILE ? (n - tile0) : TOPN_RANK_TILE);
+  for (int e = threadIdx.x; e < cnt * W; e += blockDim.x) sh[e] = in[(uint64_t)tile0 * W + e];
+  __syncthreads();
+  const int i = (int)threadIdx.x % TOPN_RANK_TILE;  // the entry
+  const int q = (int)threadIdx.x / TOPN_RANK_TILE;  // its quarter of the tile
+  if (i < cnt) {
+    uint64_t mine[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) mine[k] = sh[i * W + k];
+    const int j0 = cnt * q / TOPN_RANK_SPLIT, j1 = cnt * (q + 1) / TOPN_RANK_SPLIT;
+    int rank = 0;
+#pragma unroll 4
+    for (int j = j0; j < j1; j++) {
+      bool lt = false, eq = true;
+#pragma unroll
+      for (int k = 0; k < W; k++) {
+        const uint64_t x = sh[j * W + k];
+        lt = lt || (eq && x < mine[k]);
+        eq = eq && x == mine[k];
+      }
+      rank += (lt || (eq && j < i)) ? 1 : 0;
+    }
+    part[q][i] = rank;
+  }
+  __syncthreads();
+  uint64_t* dst = out + (uint64_t)blockIdx.x * keep * W;
+  if (q == 0 && i < cnt) {
+    int rank = 0;
+#pragma unroll
+    for (int p = 0; p < TOPN_RANK_SPLIT; p++) rank += part[p][i];
+    if (rank < keep) {
+#pragma unroll
+      for (int k = 0; k < W; k++) dst[(uint64_t)rank * W + k] = sh[i * W + k];
+    }
+  }
+  for (int e = cnt * W + (int)threadIdx.x; e < keep * W; e += blockDim.x) dst[e] = ~0ull;
+}
+
 // ---------------------------------------------------------- full sort
 // Merge sort of entries of W words (lexicographic over all W words; the
 // last word is the row index, so entries differ): LDS-sorted tiles of
@@ -3236,6 +3288,18 @@ void launch_topn_tiles(const uint64_t* in, int64_t n, int nw, int keep, uint64_t
   const int64_t tiles = (n + TOPN_TILE - 1) / TOPN_TILE;
   const size_t lds = (size_t)TOPN_TILE * (nw + 1) * 8;
   topn_tile_kernel<<<(unsigned)tiles, 256, lds, s>>>(in, n, nw, keep, out);
+}
+
+void launch_topn_rank(const uint64_t* in, int64_t n, int nw, int keep, uint64_t* out, hipStream_t s) {
+  const unsigned tiles = (unsigned)((n + TOPN_RANK_TILE - 1) / TOPN_RANK_TILE);
+  switch (nw + 1) {
+    case 2: topn_rank_kernel<2><<<tiles, TOPN_RANK_TILE * TOPN_RANK_SPLIT, 0, s>>>(in, n, keep, out); break;
+    case 3: topn_rank_kernel<3><<<tiles, TOPN_RANK_TILE * TOPN_RANK_SPLIT, 0, s>>>(in, n, keep, out); break;
+    case 4: topn_rank_kernel<4><<<tiles, TOPN_RANK_TILE * TOPN_RANK_SPLIT, 0, s>>>(in, n, keep, out); break;
+    case 5: topn_rank_kernel<5><<<tiles, TOPN_RANK_TILE * TOPN_RANK_SPLIT, 0, s>>>(in, n, keep, out); break;
+    case 6: topn_rank_kernel<6><<<tiles, TOPN_RANK_TILE * TOPN_RANK_SPLIT, 0, s>>>(in, n, keep, out); break;
+    default: break;  // the caller refuses wider keys
+  }
 }
 
 int64_t sort_padded_entries(int64_t n) { return std::max<int64_t>(1, (n + TOPN_TILE - 1) / TOPN_TILE) * TOPN_TILE; }
