@@ -1,6 +1,7 @@
 """Throughput of the operators around the star-join path on one MI355X:
-full sort, Partial -> Final aggregation, MLB1 encoding and the device Parquet
-scan. Inputs are generated on the device (the Parquet file in host memory),
+full sort, Partial -> Final aggregation, MLB1 encoding, the device Parquet
+scan and a high-cardinality GROUP BY with and without a memory budget (the
+batched fragments are measured by bench.py at their full workload). Inputs are generated on the device (the Parquet file in host memory),
 each operator runs `--iters` times after one warm-up, and one JSON line per
 operator gives rows/s and the algorithmic bytes per second against the
 8 TB/s HBM peak.
@@ -146,53 +147,6 @@ def main():
           slowdown=round(sec_s / sec_u, 2), note="same GROUP BY under a budget of ~1/4 of its table, spilling to host")
     del ht, hk
 
-    # batched fragments: 32 independent join+aggregate fragments of 512K probe rows
-    frag_line(torch, eng, np.random.default_rng(11), 32, 1 << 19, a.iters)
-
-
-def frag_line(torch, eng, rng, nfrag, prows, iters):
-    """One line per way of running the fragments: a launch pair per fragment,
-    one persistent launch pair for all, the same replayed as a hipGraph, and
-    the persistent form after autotuning."""
-    import ctypes as C
-
-    from minilake_amd import abi
-    from minilake_amd.gpu import lib
-    arr = (abi.MlFragment * nfrag)()
-    keep, probe_rows, build_rows = [], 0, 0
-    for i in range(nfrag):
-        brows = prows // 8
-        groups = 1 << int(rng.integers(4, 12))
-        bk = torch.from_numpy(rng.permutation(1 << 24)[:brows].astype(np.int64) + 1).cuda()
-        bg = torch.from_numpy(rng.integers(0, groups, brows).astype(np.int64)).cuda()
-        pk = bk[torch.from_numpy(rng.integers(0, brows, prows)).cuda()]
-        pk[::7] = -pk[::7]
-        pv = torch.from_numpy(rng.random(prows)).cuda()
-        os_ = torch.zeros(groups, dtype=torch.float64, device="cuda")
-        oc = torch.zeros(groups, dtype=torch.int64, device="cuda")
-        keep += [bk, bg, pk, pv, os_, oc]
-        arr[i] = abi.MlFragment(bk.data_ptr(), bg.data_ptr(), brows, pk.data_ptr(), pv.data_ptr(), prows, groups,
-                                os_.data_ptr(), oc.data_ptr())
-        probe_rows += prows
-        build_rows += brows
-    torch.cuda.synchronize()
-    h = C.c_void_p()
-    tune = abi.MlFragmentTuning(256, 1024, 2, 1024)
-    eng._check(lib().ml_fragments_create(eng.ctx, arr, nfrag, C.byref(tune), C.byref(h)))
-    algo = (probe_rows + build_rows) * 16
-    try:
-        for mode, fn in [("per_fragment", lib().ml_fragments_run_per_fragment),
-                         ("persistent", lib().ml_fragments_run), ("graph", lib().ml_fragments_run_graph)]:
-            sec, _ = _timed(torch, lambda: eng._check(fn(h)), iters)
-            _line("fragments_" + mode, probe_rows + build_rows, sec, algo, fragments=nfrag,
-                  note="bytes: build keys+groups and probe keys+values read once")
-        best = abi.MlFragmentTuning()
-        eng._check(lib().ml_fragments_autotune(h, C.byref(best)))
-        sec, _ = _timed(torch, lambda: eng._check(lib().ml_fragments_run(h)), iters)
-        _line("fragments_autotuned", probe_rows + build_rows, sec, algo, fragments=nfrag,
-              tuning=[best.block_size, best.tile_rows, best.blocks_per_cu, best.lds_groups])
-    finally:
-        lib().ml_fragments_free(h)
 
 
 if __name__ == "__main__":

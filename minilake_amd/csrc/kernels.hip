@@ -2752,13 +2752,7 @@ __global__ void __launch_bounds__(SCAN_T) scan_u32_down_kernel(const void* in, i
 }
 
 // ------------------------------------------------------------------- TopN
-// Sort words: lexicographic u64 words, then row index ascending.
-__device__ __forceinline__ bool sort_less(const uint64_t* a, const uint64_t* b, int nw) {
-  for (int k = 0; k < nw; k++) {
-    if (a[k] != b[k]) return a[k] < b[k];
-  }
-  return a[nw] < b[nw];  // row index tie-break
-}
+// Sort words: lexicographic u64 words, then row index ascending (lex_less).
 
 __global__ void sort_keys_kernel(Plan P, SortKeys K, int64_t n, uint64_t* words) {
   const Code ins{(CInstr*)P.ins, P.nins - 1};
@@ -2799,10 +2793,21 @@ __global__ void sort_keys_kernel(Plan P, SortKeys K, int64_t n, uint64_t* words)
 // Bitonic-sort tiles of TILE entries (nw+1 words each) in LDS; keep the first
 // `keep` entries of every tile. A short tile sorts only the power of two that
 // covers its entries (the +inf padding beyond them would stay in place).
-__global__ void __launch_bounds__(256) topn_tile_kernel(const uint64_t* in, int64_t n, int nw, int keep,
-                                                         uint64_t* out) {
+// Lexicographic order of two W-word entries (key words, then the row index).
+template <int W>
+__device__ __forceinline__ bool lex_less(const uint64_t (&a)[W], const uint64_t (&b)[W]) {
+  bool lt = false, eq = true;
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    lt = lt || (eq && a[k] < b[k]);
+    eq = eq && a[k] == b[k];
+  }
+  return lt;
+}
+
+template <int W>
+__global__ void __launch_bounds__(256) topn_tile_kernel(const uint64_t* in, int64_t n, int keep, uint64_t* out) {
   extern __shared__ uint64_t sh[];
-  const int W = nw + 1;
   const int64_t tile0 = (int64_t)blockIdx.x * TOPN_TILE;
   const int cnt = (int)((n - tile0) < TOPN_TILE ? (n - tile0) : TOPN_TILE);
   // the tile's entries are contiguous words: load them flat (coalesced), pad with +inf
@@ -2813,19 +2818,25 @@ __global__ void __launch_bounds__(256) topn_tile_kernel(const uint64_t* in, int6
   while (span < cnt) span <<= 1;
   for (int size = 2; size <= span; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int lg = __builtin_ctz(stride);
       for (int t = threadIdx.x; t < span / 2; t += blockDim.x) {
-        const int lo = 2 * stride * (t / stride) + (t % stride);
+        const int lo = ((t >> lg) << (lg + 1)) + (t & (stride - 1));
         const int hi = lo + stride;
         const bool up = ((lo & size) == 0);
-        uint64_t* a = sh + lo * W;
-        uint64_t* b = sh + hi * W;
-        const bool swap = up ? sort_less(b, a, nw) : sort_less(a, b, nw);
-        if (swap)
+        uint64_t a[W], b[W];
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+          a[k] = sh[lo * W + k];
+          b[k] = sh[hi * W + k];
+        }
+        const bool swap = up ? lex_less(b, a) : lex_less(a, b);
+        if (swap) {
+#pragma unroll
           for (int k = 0; k < W; k++) {
-            const uint64_t x = a[k];
-            a[k] = b[k];
-            b[k] = x;
+            sh[lo * W + k] = b[k];
+            sh[hi * W + k] = a[k];
           }
+        }
       }
       __syncthreads();
     }
@@ -2838,7 +2849,7 @@ __global__ void __launch_bounds__(256) topn_tile_kernel(const uint64_t* in, int6
 
 // Rank selection for a small limit: tiles of TOPN_RANK_TILE entries; the
 // rank of an entry is the number of entries of its tile that order before it
-// (sort_less, ties by position, so every entry gets its own rank), counted by
+// (lex_less, ties by position, so every entry gets its own rank), counted by
 // TOPN_RANK_SPLIT threads over disjoint quarters of the tile and summed; the
 // entry is written at its rank when that is below `keep`. A short tile's
 // missing ranks are the +inf padding. No sorting network: two barriers, and
@@ -2896,19 +2907,25 @@ __global__ void __launch_bounds__(TOPN_RANK_TILE * TOPN_RANK_SPLIT) topn_rank_ke
 // the two runs, stages both pieces in LDS and merges them there).
 constexpr int MERGE_OUT = 1024;  // outputs per workgroup in a merge pass
 
-__device__ __forceinline__ bool words_le(const uint64_t* a, const uint64_t* b, int W) {
-  for (int k = 0; k < W; k++)
-    if (a[k] != b[k]) return a[k] < b[k];
-  return true;
+template <int W>
+__device__ __forceinline__ bool words_le(const uint64_t* a, const uint64_t* b) {
+  uint64_t x[W], y[W];
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    x[k] = a[k];
+    y[k] = b[k];
+  }
+  return !lex_less(y, x);
 }
 
 // Entries of A placed among the first k outputs of merge(A, B); A goes first on ties.
+template <int W>
 __device__ __forceinline__ int64_t merge_corank(const uint64_t* A, int64_t na, const uint64_t* B, int64_t nb,
-                                                int64_t k, int W) {
+                                                int64_t k) {
   int64_t lo = k > nb ? k - nb : 0, hi = k < na ? k : na;
   while (lo < hi) {
     const int64_t mid = (lo + hi) >> 1;
-    if (words_le(A + mid * W, B + (k - mid - 1) * W, W)) lo = mid + 1;
+    if (words_le<W>(A + mid * W, B + (k - mid - 1) * W)) lo = mid + 1;
     else hi = mid;
   }
   return lo;
@@ -2916,7 +2933,8 @@ __device__ __forceinline__ int64_t merge_corank(const uint64_t* A, int64_t na, c
 
 // Where every workgroup's first output splits between its two runs: one
 // thread per workgroup, all searches in parallel ahead of the merge.
-__global__ void merge_split_kernel(const uint64_t* in, int64_t N, int64_t L, int W, int64_t nblocks, int64_t* split) {
+template <int W>
+__global__ void merge_split_kernel(const uint64_t* in, int64_t N, int64_t L, int64_t nblocks, int64_t* split) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblocks) return;
   const int64_t o0 = b * MERGE_OUT;
@@ -2924,11 +2942,12 @@ __global__ void merge_split_kernel(const uint64_t* in, int64_t N, int64_t L, int
   const int64_t na = min(L, N - ps);
   const int64_t nb = max((int64_t)0, min(L, N - ps - L));
   const uint64_t* A = in + ps * W;
-  split[b] = merge_corank(A, na, A + na * W, nb, o0 - ps, W);
+  split[b] = merge_corank<W>(A, na, A + na * W, nb, o0 - ps);
 }
 
+template <int W>
 __global__ void __launch_bounds__(256) merge_pass_kernel(const uint64_t* in, uint64_t* out, int64_t N, int64_t L,
-                                                         int W, const int64_t* split) {
+                                                         const int64_t* split) {
   extern __shared__ uint64_t sh[];
   const int64_t o0 = (int64_t)blockIdx.x * MERGE_OUT;
   const int64_t ps = o0 / (2 * L) * (2 * L);
@@ -2954,10 +2973,10 @@ __global__ void __launch_bounds__(256) merge_pass_kernel(const uint64_t* in, uin
   const int cnt = ca + cb;
   const int t0 = (int)threadIdx.x * PER;
   if (t0 < cnt) {
-    int ia = (int)merge_corank(sA, ca, sB, cb, t0, W);
+    int ia = (int)merge_corank<W>(sA, ca, sB, cb, t0);
     int ib = t0 - ia;
     for (int e = 0; e < PER && t0 + e < cnt; e++) {
-      const bool takeA = ib >= cb || (ia < ca && words_le(sA + (int64_t)ia * W, sB + (int64_t)ib * W, W));
+      const bool takeA = ib >= cb || (ia < ca && words_le<W>(sA + (int64_t)ia * W, sB + (int64_t)ib * W));
       pos[t0 + e] = (uint16_t)(takeA ? ia++ : ca + ib++);
     }
   }
@@ -3287,7 +3306,14 @@ void launch_sort_keys(const Plan& P, const SortKeys& K, int64_t n, uint64_t* wor
 void launch_topn_tiles(const uint64_t* in, int64_t n, int nw, int keep, uint64_t* out, hipStream_t s) {
   const int64_t tiles = (n + TOPN_TILE - 1) / TOPN_TILE;
   const size_t lds = (size_t)TOPN_TILE * (nw + 1) * 8;
-  topn_tile_kernel<<<(unsigned)tiles, 256, lds, s>>>(in, n, nw, keep, out);
+  switch (nw + 1) {
+    case 2: topn_tile_kernel<2><<<(unsigned)tiles, 256, lds, s>>>(in, n, keep, out); break;
+    case 3: topn_tile_kernel<3><<<(unsigned)tiles, 256, lds, s>>>(in, n, keep, out); break;
+    case 4: topn_tile_kernel<4><<<(unsigned)tiles, 256, lds, s>>>(in, n, keep, out); break;
+    case 5: topn_tile_kernel<5><<<(unsigned)tiles, 256, lds, s>>>(in, n, keep, out); break;
+    case 6: topn_tile_kernel<6><<<(unsigned)tiles, 256, lds, s>>>(in, n, keep, out); break;
+    default: break;  // the caller refuses wider keys
+  }
 }
 
 void launch_topn_rank(const uint64_t* in, int64_t n, int nw, int keep, uint64_t* out, hipStream_t s) {
@@ -3316,8 +3342,18 @@ uint64_t* launch_sort_words(const uint64_t* words, int64_t n, int nw, uint64_t* 
   uint64_t* cur = a;
   uint64_t* nxt = b;
   for (int64_t L = TOPN_TILE; L < N; L *= 2) {
-    merge_split_kernel<<<(unsigned)((nblocks + 255) / 256), 256, 0, s>>>(cur, N, L, W, nblocks, split);
-    merge_pass_kernel<<<(unsigned)nblocks, 256, (size_t)MERGE_OUT * W * 8, s>>>(cur, nxt, N, L, W, split);
+    const unsigned gs = (unsigned)((nblocks + 255) / 256), gp = (unsigned)nblocks;
+    const size_t lds = (size_t)MERGE_OUT * W * 8;
+    switch (W) {
+#define ML_MERGE(w)                                                                 \
+  case w:                                                                           \
+    merge_split_kernel<w><<<gs, 256, 0, s>>>(cur, N, L, nblocks, split);            \
+    merge_pass_kernel<w><<<gp, 256, lds, s>>>(cur, nxt, N, L, split);               \
+    break;
+      ML_MERGE(2) ML_MERGE(3) ML_MERGE(4) ML_MERGE(5) ML_MERGE(6)
+#undef ML_MERGE
+      default: break;  // the caller refuses wider keys
+    }
     std::swap(cur, nxt);
   }
   return cur;
