@@ -342,6 +342,15 @@ struct JoinTableImpl {
   Buffer bloom;
   Buffer dense_bits, dense_rank, dense_keys;  // dense layout (DevJoinTable::dense)
   Buffer dense_keys8;  // a byte copy of the second key words, used once they are known to fit
+  // Dense, not every key present, rows held at their key offset (no rank
+  // directory, dev.dense_rank null): positions 0..dense_range-1 by key, then
+  // the NULL-key rows, offset_rows in all. The C ABI numbers build rows in key
+  // order; that numbering comes from a rank of the presence bits computed on
+  // first use (probe pairs, column views), with compacted column copies.
+  bool dense_offset = false;
+  int64_t offset_rows = 0;
+  Buffer offset_rank;
+  std::vector<Buffer> compact_cols, compact_valid;
   DevJoinTable dev{};
   std::vector<ml_column> views;
   // A build whose last check still runs on the stream: its flags arrive in

@@ -2266,7 +2266,8 @@ static void refresh_views(JoinTableImpl& T) {
   for (size_t c = 0; c < T.col_types.size(); c++) {
     ml_column v{};
     v.type = T.col_types[c];
-    v.data = T.cols[c].ptr;
+    // an offset-indexed table's views are its compacted copies (made on first access)
+    v.data = T.dense_offset ? (c < T.compact_cols.size() ? T.compact_cols[c].ptr : nullptr) : T.cols[c].ptr;
     v.len = T.nrows;
     v.dict = T.col_dicts[c] ? &T.col_dicts[c]->view : nullptr;
     T.views[c] = v;
@@ -2308,6 +2309,8 @@ void join_table_resolve(ml_join_table* t) {
     T.dense_keys = Buffer();
     T.dense_keys8 = Buffer();
     T.bloom = Buffer();
+    T.dense_offset = false;
+    T.offset_rows = 0;
     T.dev.dense = 0;
     T.dev.dense_full = 0;
     T.dev.dense_keys8 = 0;
@@ -2549,29 +2552,41 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
           // computed, and a probe reads neither bits nor ranks; a Bloom filter
           // would cost the one load it saves
           const bool full = hm[3] == (int64_t)R;
+          // otherwise the rows are held at their key offset when that costs at
+          // most 4x the rows' bytes or 256 MB: a probe then reads the presence
+          // bit and the row at the key's offset, with no rank directory
+          // between them (one dependent load fewer per hit)
+          const int64_t nnull = n - hm[3];
+          const uint64_t offset_rows = R + (uint64_t)nnull;
+          uint64_t row_bytes = (nk - 1) * key_word_bytes + (nk == 2 ? 1 : 0);
+          for (size_t c = 0; c < T.cols.size(); c++)
+            row_bytes += (uint64_t)type_width(T.col_types[c]) + (T.col_valid[c].ptr ? 1 : 0);
+          const bool offset = !full && (offset_rows <= 4 * (uint64_t)n || offset_rows * row_bytes <= (256ull << 20));
+          const int64_t held = (full || offset) ? (int64_t)offset_rows : n;  // full: R + NULL-key rows = n
           Buffer rank;
-          if (!full) {
+          if (!full && !offset) {
             rank = Buffer(ctx->arena, (size_t)(ne + 1) * 4);
             Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(ne) + 1) * 4);
             launch_dense_rank(bits.as<uint64_t>(), ne, rank.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
           }
           DensePermute M{};
           Buffer dkeys, dkeys8;
-          if (nk > 1) dkeys = Buffer(ctx->arena, (size_t)n * (nk - 1) * key_word_bytes);
+          if (nk > 1) dkeys = Buffer(ctx->arena, (size_t)std::max<int64_t>(held, 1) * (nk - 1) * key_word_bytes);
           if (nk == 2) {  // also a byte per row, used if every second key word fits (flag bit 8)
-            dkeys8 = Buffer(ctx->arena, (size_t)n);
+            dkeys8 = Buffer(ctx->arena, (size_t)std::max<int64_t>(held, 1));
             M.dst_keys8 = dkeys8.as<uint8_t>();
           }
           M.nkw = (int32_t)nk;
           M.keys32 = keys32 ? 1 : 0;
           M.dst_keys = dkeys.as<uint64_t>();
+          M.held = held;
           M.ncols = (int32_t)T.cols.size();
           std::vector<Buffer> cols, valid;
           for (size_t c = 0; c < T.cols.size(); c++) {
             const int w = type_width(T.col_types[c]);
-            cols.emplace_back(ctx->arena, (size_t)n * w);
+            cols.emplace_back(ctx->arena, (size_t)std::max<int64_t>(held, 1) * w);
             valid.emplace_back();
-            if (T.col_valid[c].ptr) valid.back() = Buffer(ctx->arena, (size_t)n);
+            if (T.col_valid[c].ptr) valid.back() = Buffer(ctx->arena, (size_t)std::max<int64_t>(held, 1));
             M.width[c] = w;
             M.src[c] = T.cols[c].ptr;
             M.dst[c] = cols.back().ptr;
@@ -2580,7 +2595,7 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
           }
           // flags[1] (next NULL-key row) was cleared by the min/max kernel
           launch_dense_permute(M, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(),
-                               full ? nullptr : rank.as<uint32_t>(), full ? nullptr : rank.as<uint32_t>() + ne,
+                               rank.ptr ? rank.as<uint32_t>() : nullptr, rank.ptr ? rank.as<uint32_t>() + ne : nullptr,
                                flags + 1, ctx->stream);
           if (nk > 1 && !full && hashed_bloom(ctx, n)) {  // a word per 8 first-key values
             const uint64_t words = (R + (1u << DENSE_BLOOM_SHIFT) - 1) >> DENSE_BLOOM_SHIFT;
@@ -2609,8 +2624,10 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
           T.dev.dense_min = hm[0];
           T.dev.dense_range = R;
           T.dev.dense_bits = T.dense_bits.as<uint64_t>();
-          T.dev.dense_rank = T.dense_rank.as<uint32_t>();
+          T.dev.dense_rank = T.dense_rank.ptr ? T.dense_rank.as<uint32_t>() : nullptr;
           T.dev.unique = 1;
+          T.dense_offset = offset;
+          T.offset_rows = offset ? (int64_t)offset_rows : 0;
           T.dense_keys8 = std::move(dkeys8);
           if (!deferred) use_keys8(T, hf);
         }
@@ -3419,10 +3436,43 @@ static bool resolved(const ml_join_table* t) {
   if (!t) return false;
   return guarded(t->ctx, [&] { join_table_resolve(const_cast<ml_join_table*>(t)); }) == ML_OK;
 }
+
+namespace mlg {
+// The rank of an offset-indexed table's presence bits (row numbers in key order).
+static const uint32_t* offset_rank(ml_ctx* ctx, JoinTableImpl& T) {
+  if (!T.offset_rank.ptr) {
+    const int64_t ne = (int64_t)((T.dev.dense_range + 63) / 64);
+    T.offset_rank = Buffer(ctx->arena, (size_t)(ne + 1) * 4);
+    Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(ne) + 1) * 4);
+    launch_dense_rank(T.dense_bits.as<uint64_t>(), ne, T.offset_rank.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
+  }
+  return T.offset_rank.as<uint32_t>();
+}
+
+// Columns of an offset-indexed table in key order (the ABI's row numbering), made once.
+static void compact_offset_table(ml_ctx* ctx, JoinTableImpl& T) {
+  if (!T.dense_offset || !T.compact_cols.empty()) return;
+  const uint32_t* rank = offset_rank(ctx, T);
+  for (size_t c = 0; c < T.cols.size(); c++) {
+    const int w = type_width(T.col_types[c]);
+    T.compact_cols.emplace_back(ctx->arena, (size_t)std::max<int64_t>(T.nrows, 1) * w);
+    T.compact_valid.emplace_back();
+    if (T.col_valid[c].ptr) T.compact_valid.back() = Buffer(ctx->arena, (size_t)std::max<int64_t>(T.nrows, 1));
+    launch_dense_compact(T.cols[c].ptr, w, T.col_valid[c].ptr ? T.col_valid[c].as<uint8_t>() : nullptr,
+                         T.offset_rows, T.dense_bits.as<uint64_t>(), rank, T.dev.dense_range,
+                         T.compact_cols.back().ptr,
+                         T.compact_valid.back().ptr ? T.compact_valid.back().as<uint8_t>() : nullptr, ctx->stream);
+  }
+  refresh_views(T);
+}
+}  // namespace mlg
 int32_t ml_join_table_unique(const ml_join_table* t) { return resolved(t) ? t->impl.dev.unique : 0; }
 int32_t ml_join_table_num_columns(const ml_join_table* t) { return t ? (int32_t)t->impl.col_types.size() : 0; }
 const ml_column* ml_join_table_column(const ml_join_table* t, int32_t i) {
   if (!resolved(t) || i < 0 || (size_t)i >= t->impl.views.size()) return nullptr;
+  if (t->impl.dense_offset &&
+      guarded(t->ctx, [&] { compact_offset_table(t->ctx, const_cast<ml_join_table*>(t)->impl); }) != ML_OK)
+    return nullptr;
   // the build's last kernels may still be queued: the columns are complete
   // when this returns, so they can be read from any stream
   if (hipStreamSynchronize(t->ctx->stream) != hipSuccess) return nullptr;
@@ -3444,6 +3494,11 @@ ml_status ml_join_probe(ml_ctx* ctx, const ml_join_table* t, const ml_expr* prob
     Materialized m;
     Buffer pairs;
     run_collect(B, m, false, &pairs, 2);
+    // an offset-indexed table's rows are key offsets here: renumber them in key order
+    JoinTableImpl& J = const_cast<ml_join_table*>(t)->impl;
+    if (J.dense_offset && m.rows > 0)
+      launch_dense_translate(pairs.as<int64_t>() + 1, 2, (int64_t)m.rows, J.dense_bits.as<uint64_t>(),
+                             offset_rank(ctx, J), J.dev.dense_range, ctx->stream);
     // split the (probe row, build row) pairs into two INT64 columns
     Materialized o;
     o.rows = m.rows;

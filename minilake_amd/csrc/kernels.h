@@ -111,6 +111,15 @@ void launch_dense_bloom(const DevJoinTable& T, const uint64_t* keys, const uint8
                         int64_t kmin, uint64_t range, uint64_t* bloom, hipStream_t s);
 // rank[w] = set bits in words below w, rank[nw] = all set bits
 void launch_dense_rank(const uint64_t* bits, int64_t nw, uint32_t* rank, uint32_t* block_sums, hipStream_t s);
+// Offset-indexed dense tables (rows at their key offset, then the NULL-key
+// rows from `range` on): row numbers in key order, given the rank of the
+// presence bits (launch_dense_rank; rank[(range + 63) / 64] = keyed rows).
+// translate: rows[i * stride] for i < n, in place; compact: every held
+// position of a column (npos = range + NULL-key rows) moved to its row.
+void launch_dense_translate(int64_t* rows, int64_t stride, int64_t n, const uint64_t* bits, const uint32_t* rank,
+                            uint64_t range, hipStream_t s);
+void launch_dense_compact(const void* src, int width, const uint8_t* src_valid, int64_t npos, const uint64_t* bits,
+                          const uint32_t* rank, uint64_t range, void* dst, uint8_t* dst_valid, hipStream_t s);
 struct DensePermute {
   int32_t nkw;                // key words per row; words 1.. go to dst_keys
   int32_t keys32;             // dst_keys holds u32 (the low halves) instead of u64
@@ -123,9 +132,11 @@ struct DensePermute {
   void* dst[MAX_COLS];
   const uint8_t* src_valid[MAX_COLS];
   uint8_t* dst_valid[MAX_COLS];
+  int64_t held;               // rows the destinations hold (0: the source's row count)
 };
-// Rows into key order; rank and present null for a full table (every key of
-// the range present: a row's position is its key's offset).
+// Rows into key order; rank and present null for a table whose rows sit at
+// their key's offset (a full table, where every key of the range is present,
+// or an offset-indexed one), the NULL-key rows from `range` on.
 void launch_dense_permute(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
                           int64_t kmin, uint64_t range, const uint64_t* bits, const uint32_t* rank,
                           const uint32_t* present, uint32_t* null_next, hipStream_t s);

@@ -533,7 +533,9 @@ __device__ __forceinline__ bool bloom_maybe(const DevJoinTable& T, uint64_t h) {
   return (T.bloom[bloom_word(T, h, g)] & bits) == bits;
 }
 
-// Dense table lookup of key word k: presence bit, then the rank of its word.
+// Dense table lookup of key word k: presence bit, then the row: the rank of
+// the key's bit (a rank directory), or its offset itself (rows held at their
+// key offset, no directory).
 __device__ __forceinline__ bool dense_find(const DevJoinTable& T, uint64_t k, uint32_t& row) {
   const uint64_t idx = k - (uint64_t)T.dense_min;
   if (idx >= T.dense_range) return false;
@@ -544,7 +546,8 @@ __device__ __forceinline__ bool dense_find(const DevJoinTable& T, uint64_t k, ui
   const uint64_t bits = gp<uint64_t>(T.dense_bits)[idx >> 6];
   const uint32_t b = (uint32_t)(idx & 63);
   if (!((bits >> b) & 1ull)) return false;
-  row = gp<uint32_t>(T.dense_rank)[idx >> 6] + (uint32_t)__popcll(bits & ((1ull << b) - 1ull));
+  row = T.dense_rank ? gp<uint32_t>(T.dense_rank)[idx >> 6] + (uint32_t)__popcll(bits & ((1ull << b) - 1ull))
+                     : (uint32_t)idx;
   return true;
 }
 
@@ -1261,15 +1264,23 @@ __device__ __forceinline__ void fast_probe(const Plan& P, const FastKeys& K, uin
       uint64_t bw[FAST_U];
 #pragma unroll
       for (int u = 0; u < FAST_U; u++) bw[u] = gp<uint64_t>(T.dense_bits)[hit[u] ? (idx[u] >> 6) : 0];
-      uint32_t rk[FAST_U];
+      if (T.dense_rank) {
+        uint32_t rk[FAST_U];
 #pragma unroll
-      for (int u = 0; u < FAST_U; u++) {
-        hit[u] = hit[u] && ((bw[u] >> (idx[u] & 63)) & 1ull);
-        rk[u] = gp<uint32_t>(T.dense_rank)[hit[u] ? (idx[u] >> 6) : 0];
+        for (int u = 0; u < FAST_U; u++) {
+          hit[u] = hit[u] && ((bw[u] >> (idx[u] & 63)) & 1ull);
+          rk[u] = gp<uint32_t>(T.dense_rank)[hit[u] ? (idx[u] >> 6) : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < FAST_U; u++)
+          brow[u] = rk[u] + (uint32_t)__popcll(bw[u] & ((1ull << (idx[u] & 63)) - 1ull));
+      } else {  // rows at their key offset: the bit decides, no rank load
+#pragma unroll
+        for (int u = 0; u < FAST_U; u++) {
+          hit[u] = hit[u] && ((bw[u] >> (idx[u] & 63)) & 1ull);
+          brow[u] = (uint32_t)idx[u];
+        }
       }
-#pragma unroll
-      for (int u = 0; u < FAST_U; u++)
-        brow[u] = rk[u] + (uint32_t)__popcll(bw[u] & ((1ull << (idx[u] & 63)) - 1ull));
     }
     if (two) {  // the second key word must match the row's
       uint64_t kb[FAST_U];
@@ -1950,19 +1961,20 @@ __global__ void dense_mark_kernel(const uint64_t* keys, int nkw, const uint8_t* 
 __global__ void dense_permute_kernel(DensePermute M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
                                      int64_t kmin, uint64_t range, const uint64_t* bits, const uint32_t* rank,
                                      const uint32_t* present, uint32_t* null_next) {
-  // rank == null: every key of the range is present (a full table), so the
-  // rank of key offset i is i itself and the NULL-key rows start at range
+  // rank == null: rows sit at their key's offset (a full table, or an
+  // offset-indexed one) and the NULL-key rows start at range
   const uint64_t null_base = rank ? (uint64_t)*present : range;
+  const uint64_t held = M.held > 0 ? (uint64_t)M.held : (uint64_t)n;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     uint64_t pos;
     if (!key_null[r]) {
       const uint64_t i = (uint64_t)((int64_t)keys[r * M.nkw] - kmin);
       if (i >= range) continue;  // flagged by the marking; the table falls back
       pos = rank ? rank[i >> 6] + (uint64_t)__popcll(bits[i >> 6] & ((1ull << (i & 63)) - 1ull)) : i;
-      if (pos >= (uint64_t)n) continue;  // only with repeated keys, which also fall back
+      if (pos >= held) continue;  // only with repeated keys, which also fall back
     } else {
       pos = null_base + atomicAdd(null_next, 1u);
-      if (pos >= (uint64_t)n) continue;  // only with repeated keys, which also fall back
+      if (pos >= held) continue;  // only with repeated keys, which also fall back
     }
     for (int k = 1; k < M.nkw; k++) {
       const uint64_t at = pos * (M.nkw - 1) + k - 1;
@@ -3210,6 +3222,46 @@ void launch_dense_mark(const uint64_t* keys, int nkw, const uint8_t* key_null, i
 }
 
 static void scan_launch(const void* in, int popc, int64_t n, uint32_t* out, uint32_t* block_sums, hipStream_t s);
+
+// Rows of an offset-indexed dense table (position p: a key offset below
+// `range`, or range + i for the i-th NULL-key row) -> their row numbers in key
+// order: the rank of the key's bit, the NULL-key rows after all keyed ones.
+__device__ __forceinline__ uint64_t dense_compact_row(const uint64_t* bits, const uint32_t* rank, uint64_t range,
+                                                      uint64_t p) {
+  if (p >= range) return (uint64_t)rank[(range + 63) >> 6] + (p - range);
+  return (uint64_t)rank[p >> 6] + (uint64_t)__popcll(bits[p >> 6] & ((1ull << (p & 63)) - 1ull));
+}
+
+__global__ void dense_translate_kernel(int64_t* rows, int64_t stride, int64_t n, const uint64_t* bits,
+                                       const uint32_t* rank, uint64_t range) {
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step)
+    rows[i * stride] = (int64_t)dense_compact_row(bits, rank, range, (uint64_t)rows[i * stride]);
+}
+
+__global__ void dense_compact_kernel(const uint8_t* src, int width, const uint8_t* src_valid, int64_t npos,
+                                     const uint64_t* bits, const uint32_t* rank, uint64_t range, uint8_t* dst,
+                                     uint8_t* dst_valid) {
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npos; p += step) {
+    if ((uint64_t)p < range && !((bits[p >> 6] >> (p & 63)) & 1ull)) continue;  // no row at this offset
+    const uint64_t q = dense_compact_row(bits, rank, range, (uint64_t)p);
+    for (int b = 0; b < width; b++) dst[q * width + b] = src[(uint64_t)p * width + b];
+    if (dst_valid) dst_valid[q] = src_valid ? src_valid[p] : 1;
+  }
+}
+
+void launch_dense_translate(int64_t* rows, int64_t stride, int64_t n, const uint64_t* bits, const uint32_t* rank,
+                            uint64_t range, hipStream_t s) {
+  if (n > 0) dense_translate_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(rows, stride, n, bits, rank, range);
+}
+
+void launch_dense_compact(const void* src, int width, const uint8_t* src_valid, int64_t npos, const uint64_t* bits,
+                          const uint32_t* rank, uint64_t range, void* dst, uint8_t* dst_valid, hipStream_t s) {
+  if (npos > 0)
+    dense_compact_kernel<<<grid_for(npos, 256, 8192), 256, 0, s>>>((const uint8_t*)src, width, src_valid, npos, bits,
+                                                                   rank, range, (uint8_t*)dst, dst_valid);
+}
 
 void launch_dense_rank(const uint64_t* bits, int64_t nw, uint32_t* rank, uint32_t* block_sums, hipStream_t s) {
   scan_launch(bits, 1, nw, rank, block_sums, s);

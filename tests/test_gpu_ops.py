@@ -156,6 +156,51 @@ def test_join_build_many_matches_oracle(gpu_engine):
     assert not tables[2].unique() and tables[0].unique()
 
 
+@pytest.mark.parametrize("spread", [3, 6])
+def test_offset_indexed_dense_table(gpu_engine, spread):
+    """A dense table with gaps in its key range (keys spread x3 and x6, some
+    NULL keys and NULL payload values) keeps its rows at their key offset; the
+    probe pairs and the column views still number the build rows one way,
+    which must agree with each other and with the oracle's pairs (compared as
+    (probe row, build payload)); one- and two-column keys."""
+    from oracle.oracle import OracleEngine
+    oracle_engine = OracleEngine(threads=1)
+    rng = np.random.default_rng(21 + spread)
+    nb = 90_000
+    k64 = rng.permutation(nb).astype(np.int64) * spread - 77
+    kv = rng.random(nb) >= 0.01  # NULL build keys
+    ids = np.arange(nb, dtype=np.int64)
+    pay = HostColumn(FLOAT64, rng.random(nb))
+    pay.validity = rng.random(nb) >= 0.05
+    k32 = rng.integers(0, 9, nb).astype(np.int32)
+    kcol = HostColumn(INT64, k64)
+    kcol.validity = kv
+    build = HostTable([kcol, HostColumn(INT64, ids), pay, HostColumn(INT32, k32)])
+    npr = 200_000
+    pk64 = rng.integers(-200, nb * spread + 200, npr).astype(np.int64)
+    pk32 = np.where(rng.random(npr) < 0.7, k32[rng.integers(0, nb, npr)], rng.integers(0, 9, npr)).astype(np.int32)
+    probe = HostTable([HostColumn(INT64, pk64), HostColumn(INT32, pk32)])
+    for keys, kt, pkeys in (([col(0)], [INT64], [col(0)]), ([col(0), col(3)], [INT64, INT32], [col(0), col(1)])):
+        jo = oracle_engine.join_build(build, [], keys, kt)
+        jg = gpu_engine.join_build(build, [], keys, kt)
+        assert jg.unique() and jg.num_rows() == nb
+        po, bo = oracle_engine.join_probe(jo, pkeys, probe)
+        pg, bg = gpu_engine.join_probe(jg, pkeys, probe)
+        want = sorted(zip(po.tolist(), ids[bo].tolist()))
+        assert len(want) > 1000
+        assert sorted(zip(pg.tolist(), jg.column(1)[bg].tolist())) == want
+        # the views hold every build row once
+        assert sorted(jg.column(1).tolist()) == ids.tolist()
+        got_k = jg.column(0)
+        by_id = np.empty(nb, dtype=np.int64)
+        by_id[jg.column(1)] = got_k
+        assert (by_id[kv] == k64[kv]).all()
+        # a pipeline reading the build payload through the probe
+        got = gpu_engine.collect(probe, [abi.probe_op(jg, pkeys)]).to_host()
+        exp = oracle_engine.collect(probe, [abi.probe_op(jo, pkeys)]).to_host()
+        assert_tables_equal(got, exp, rtol=0.0)
+
+
 def test_expanding_probe_into_aggregate(gpu_engine, oracle_engine):
     rng = np.random.default_rng(9)
     build = HostTable([HostColumn(INT64, rng.integers(0, 50, 400)), HostColumn(FLOAT64, rng.random(400))])
