@@ -1329,7 +1329,7 @@ static void agg_reset_table(ml_hashagg* A) {
   A->cap = 0;
   A->len_host = 0;
   A->len_known = true;
-  zero(A->ctx, A->flags.ptr, 16);
+  zero(A->ctx, A->flags.ptr, 32);
 }
 
 // Partitioned aggregation of a first batch whose group count is known
@@ -1342,6 +1342,7 @@ static void agg_reset_table(ml_hashagg* A) {
 // false when the pipeline or geometry does not fit (the caller then uses the
 // global table).
 constexpr uint64_t PART_AGG_MIN_GROUPS = 1 << 16;
+constexpr uint64_t PART_AGG_MAX_PARTS = 8192;
 constexpr uint32_t PART_CHUNK_ROWS = 1 << 16;
 
 static bool agg_partitioned(ml_hashagg* A, PipelineBuild& B) {
@@ -1349,12 +1350,13 @@ static bool agg_partitioned(ml_hashagg* A, PipelineBuild& B) {
   Plan& P = B.P;
   const int nkw = A->nkw, nacc = (int)A->slots.size(), nargs = P.nargs, nkeys = P.nkeys;
   if (nkeys + nargs > MAX_COLS || nkeys < 1) return false;
-  const uint64_t cap = std::max<uint64_t>(1024, next_pow2((uint64_t)A->hint * 2));
+  uint64_t cap = std::max<uint64_t>(1024, next_pow2((uint64_t)A->hint * 2));
   uint64_t S = 2048;
   while (S > 256 && S * (1 + nkw + nacc) * 8 > 65536) S >>= 1;
   S = std::min(S, cap);
-  const uint64_t nparts = cap / S;
-  if (S * (1 + nkw + nacc) * 8 > 65536 || nparts > 8192 || cap >= (1ull << 32)) return false;
+  // partitions of S slots each, at most PART_AGG_MAX_PARTS of them
+  auto fits = [&](uint64_t c) { return S * (1 + nkw + nacc) * 8 <= 65536 && c / S <= PART_AGG_MAX_PARTS && c < (1ull << 32); };
+  if (!fits(cap)) return false;
 
   // 1. key and argument values of the surviving rows, in source order
   P.ncols = nkeys + nargs;
@@ -1381,89 +1383,112 @@ static bool agg_partitioned(ml_hashagg* A, PipelineBuild& B) {
   const int64_t n = (int64_t)run_flow_rows(B, SINK_COLLECT, F);
   P.ncols = 0;
 
-  // 2. partition order
-  AggRecords R{};
-  R.nkeys = nkeys;
-  R.nkw = nkw;
-  R.nargs = nargs;
-  R.key_nullable = A->key_nullable ? 1 : 0;
-  bool args_nullable = false;
-  for (int k = 0; k < nkeys; k++) {
-    R.key_val[k] = vals[k].as<uint64_t>();
-    R.key_ok[k] = oks[k].ptr ? oks[k].as<uint8_t>() : nullptr;
-  }
-  for (int a = 0; a < nargs; a++) {
-    R.arg_val[a] = vals[nkeys + a].as<uint64_t>();
-    R.arg_ok[a] = oks[nkeys + a].ptr ? oks[nkeys + a].as<uint8_t>() : nullptr;
-    args_nullable = args_nullable || R.arg_ok[a];
-  }
-  const int part_shift = __builtin_ctzll(S);
-  const int nblocks = (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + 16383) / 16384));
-  const int64_t rpb = std::max<int64_t>(1, (n + nblocks - 1) / nblocks);
-  const int64_t nh = (int64_t)nparts * nblocks;
-  Buffer hist(ctx->arena, (size_t)nh * 4), offsets(ctx->arena, (size_t)(nh + 1) * 4);
-  Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(nh) + 1) * 4);
-  Buffer rkeys(ctx->arena, (size_t)std::max<int64_t>(n, 1) * nkw * 8);
-  Buffer rargs(ctx->arena, (size_t)std::max<int64_t>(n, 1) * std::max(nargs, 1) * 8);
-  Buffer rvalid;
-  if (args_nullable) rvalid = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1) * 4);
-  launch_agg_part_hist(R, n, cap - 1, part_shift, nblocks, rpb, (int)nparts, hist.as<uint32_t>(), ctx->stream);
-  launch_scan_u32(hist.as<uint32_t>(), nh, offsets.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
-  launch_agg_part_scatter(R, n, cap - 1, part_shift, nblocks, rpb, (int)nparts, offsets.as<uint32_t>(),
-                          rkeys.as<uint64_t>(), rargs.as<uint64_t>(), rvalid.as<uint32_t>(), ctx->stream);
-  // partition starts (every nblocks-th offset, and the total)
-  Buffer starts(ctx->arena, (size_t)(nparts + 1) * 4);
-  launch_strided_u32(offsets.as<uint32_t>(), nblocks, (int64_t)nparts + 1, starts.as<uint32_t>(), ctx->stream);
-  std::vector<uint32_t> st((size_t)nparts + 1);
-  d2h(ctx, st.data(), starts.ptr, st.size() * 4);
-  vals.clear();
-  oks.clear();
-
-  // 3. chunks: one per partition, several for partitions with many rows
-  std::vector<PartChunk> chunks;
-  std::vector<uint32_t> heavy, first;
-  uint32_t q = 0;
-  for (uint32_t p = 0; p < nparts; p++) {
-    const uint32_t lo = st[p], hi = st[p + 1];
-    if (hi - lo <= PART_CHUNK_ROWS) {
-      chunks.push_back(PartChunk{p, lo, hi, 0xFFFFFFFFu});
-      continue;
+  // Steps 2-3 run again with a larger capacity while a learned estimate
+  // proves too small (an LDS table filled up: flags u32 2, bit 1); the
+  // collected rows are partitioned anew, not collected again. A hinted
+  // capacity is never grown.
+  const bool grow = A->sizing == ML_SIZING_LEARNED;
+  for (;;) {
+    const uint64_t nparts = cap / S;
+    // 2. partition order
+    AggRecords R{};
+    R.nkeys = nkeys;
+    R.nkw = nkw;
+    R.nargs = nargs;
+    R.key_nullable = A->key_nullable ? 1 : 0;
+    bool args_nullable = false;
+    for (int k = 0; k < nkeys; k++) {
+      R.key_val[k] = vals[k].as<uint64_t>();
+      R.key_ok[k] = oks[k].ptr ? oks[k].as<uint8_t>() : nullptr;
     }
-    heavy.push_back(p);
+    for (int a = 0; a < nargs; a++) {
+      R.arg_val[a] = vals[nkeys + a].as<uint64_t>();
+      R.arg_ok[a] = oks[nkeys + a].ptr ? oks[nkeys + a].as<uint8_t>() : nullptr;
+      args_nullable = args_nullable || R.arg_ok[a];
+    }
+    const int part_shift = __builtin_ctzll(S);
+    const int nblocks = (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + 16383) / 16384));
+    const int64_t rpb = std::max<int64_t>(1, (n + nblocks - 1) / nblocks);
+    const int64_t nh = (int64_t)nparts * nblocks;
+    Buffer hist(ctx->arena, (size_t)nh * 4), offsets(ctx->arena, (size_t)(nh + 1) * 4);
+    Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(nh) + 1) * 4);
+    Buffer rkeys(ctx->arena, (size_t)std::max<int64_t>(n, 1) * nkw * 8);
+    Buffer rargs(ctx->arena, (size_t)std::max<int64_t>(n, 1) * std::max(nargs, 1) * 8);
+    Buffer rvalid;
+    if (args_nullable) rvalid = Buffer(ctx->arena, (size_t)std::max<int64_t>(n, 1) * 4);
+    launch_agg_part_hist(R, n, cap - 1, part_shift, nblocks, rpb, (int)nparts, hist.as<uint32_t>(), ctx->stream);
+    launch_scan_u32(hist.as<uint32_t>(), nh, offsets.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
+    launch_agg_part_scatter(R, n, cap - 1, part_shift, nblocks, rpb, (int)nparts, offsets.as<uint32_t>(),
+                            rkeys.as<uint64_t>(), rargs.as<uint64_t>(), rvalid.as<uint32_t>(), ctx->stream);
+    // partition starts (every nblocks-th offset, and the total)
+    Buffer starts(ctx->arena, (size_t)(nparts + 1) * 4);
+    launch_strided_u32(offsets.as<uint32_t>(), nblocks, (int64_t)nparts + 1, starts.as<uint32_t>(), ctx->stream);
+    std::vector<uint32_t> st((size_t)nparts + 1);
+    d2h(ctx, st.data(), starts.ptr, st.size() * 4);
+
+    // 3. chunks: one per partition, several for partitions with many rows
+    std::vector<PartChunk> chunks;
+    std::vector<uint32_t> heavy, first;
+    uint32_t q = 0;
+    for (uint32_t p = 0; p < nparts; p++) {
+      const uint32_t lo = st[p], hi = st[p + 1];
+      if (hi - lo <= PART_CHUNK_ROWS) {
+        chunks.push_back(PartChunk{p, lo, hi, 0xFFFFFFFFu});
+        continue;
+      }
+      heavy.push_back(p);
+      first.push_back(q);
+      for (uint32_t c = lo; c < hi; c += PART_CHUNK_ROWS) chunks.push_back(PartChunk{p, c, std::min(hi, c + PART_CHUNK_ROWS), q++});
+    }
     first.push_back(q);
-    for (uint32_t c = lo; c < hi; c += PART_CHUNK_ROWS) chunks.push_back(PartChunk{p, c, std::min(hi, c + PART_CHUNK_ROWS), q++});
+    A->table = Buffer(ctx->arena, cap * (1 + nkw) * 8);
+    init_acc(A, A->acc, cap);
+    PartAggPlan G{};
+    G.S = (int32_t)S;
+    G.nkw = nkw;
+    G.nargs = nargs;
+    G.nacc = nacc;
+    G.mask = cap - 1;
+    for (int s2 = 0; s2 < nacc; s2++) {
+      G.acc[s2] = P.acc[s2];  // the compiled slots: `arg` indexes the (deduplicated) argument programs
+      G.acc_data[s2] = A->acc[s2].ptr;
+    }
+    G.slots = A->table.as<uint64_t>();
+    G.flags = (uint32_t*)A->flags.ptr;
+    Buffer dchunks(ctx->arena, chunks.size() * sizeof(PartChunk));
+    h2d(ctx, dchunks.ptr, chunks.data(), chunks.size() * sizeof(PartChunk));
+    Buffer partial(ctx->arena, std::max<size_t>(1, (size_t)q * S * (1 + nkw + nacc) * 8));
+    Buffer dheavy(ctx->arena, std::max<size_t>(1, heavy.size()) * 4), dfirst(ctx->arena, first.size() * 4);
+    if (!heavy.empty()) {
+      h2d(ctx, dheavy.ptr, heavy.data(), heavy.size() * 4);
+      h2d(ctx, dfirst.ptr, first.data(), first.size() * 4);
+    }
+    launch_agg_part_chunks(G, dchunks.as<PartChunk>(), (int)chunks.size(), rkeys.as<uint64_t>(), rargs.as<uint64_t>(),
+                           rvalid.as<uint32_t>(), n, partial.as<uint64_t>(), ctx->stream);
+    launch_agg_part_merge(G, dheavy.as<uint32_t>(), dfirst.as<uint32_t>(), (int)heavy.size(), partial.as<uint64_t>(),
+                          ctx->stream);
+    A->cap = cap;
+    A->wrap = S - 1;
+    A->len_known = false;
+    if (!grow || !fits(cap * 2)) break;
+    // groups written; overflow bits | filled partitions << 32; their projected groups
+    uint64_t f[3] = {0, 0, 0};
+    d2h(ctx, f, A->flags.ptr, 24);
+    if (!(f[1] & 2u)) break;
+    // An LDS table filled up: everything from the partitioning on again, from
+    // an empty table sized for the groups the attempt saw (the partitions that
+    // held, exactly; a filled one projected from how far into its records it
+    // got) with the initial 2x margin, and at least twice the capacity. More
+    // groups than the partitions can hold: the plain path.
+    const uint64_t over = std::min<uint64_t>(f[1] >> 32, nparts);
+    const uint64_t seen = f[0] - std::min<uint64_t>(f[0], over * S) + f[2];
+    const uint64_t next = std::max<uint64_t>(cap * 2, next_pow2(2 * seen + 1));
+    if (!fits(next)) break;
+    zero(ctx, A->flags.ptr, 32);
+    cap = next;
+    A->resizes++;
   }
-  first.push_back(q);
-  A->table = Buffer(ctx->arena, cap * (1 + nkw) * 8);
-  init_acc(A, A->acc, cap);
-  PartAggPlan G{};
-  G.S = (int32_t)S;
-  G.nkw = nkw;
-  G.nargs = nargs;
-  G.nacc = nacc;
-  G.mask = cap - 1;
-  for (int s2 = 0; s2 < nacc; s2++) {
-    G.acc[s2] = P.acc[s2];  // the compiled slots: `arg` indexes the (deduplicated) argument programs
-    G.acc_data[s2] = A->acc[s2].ptr;
-  }
-  G.slots = A->table.as<uint64_t>();
-  G.flags = (uint32_t*)A->flags.ptr;
-  Buffer dchunks(ctx->arena, chunks.size() * sizeof(PartChunk));
-  h2d(ctx, dchunks.ptr, chunks.data(), chunks.size() * sizeof(PartChunk));
-  Buffer partial(ctx->arena, std::max<size_t>(1, (size_t)q * S * (1 + nkw + nacc) * 8));
-  Buffer dheavy(ctx->arena, std::max<size_t>(1, heavy.size()) * 4), dfirst(ctx->arena, first.size() * 4);
-  if (!heavy.empty()) {
-    h2d(ctx, dheavy.ptr, heavy.data(), heavy.size() * 4);
-    h2d(ctx, dfirst.ptr, first.data(), first.size() * 4);
-  }
-  launch_agg_part_chunks(G, dchunks.as<PartChunk>(), (int)chunks.size(), rkeys.as<uint64_t>(), rargs.as<uint64_t>(),
-                         rvalid.as<uint32_t>(), n, partial.as<uint64_t>(), ctx->stream);
-  launch_agg_part_merge(G, dheavy.as<uint32_t>(), dfirst.as<uint32_t>(), (int)heavy.size(), partial.as<uint64_t>(),
-                        ctx->stream);
   hip_check(hipStreamSynchronize(ctx->stream), "partitioned aggregation");
-  A->cap = cap;
-  A->wrap = S - 1;
-  A->len_known = false;
   return true;
 }
 
@@ -1586,8 +1611,8 @@ static void agg_run(ml_hashagg* A, PipelineBuild& B) {
       for (size_t c = 0; c < init.size(); c++) init[c] = acc_init_bits(kinds.kind[c % A->slots.size()]);
       upload(ctx, A->totals.ptr, init.data(), init.size() * 8);
     }
-    A->flags = Buffer(ctx->arena, 16);
-    zero(ctx, A->flags.ptr, 16);
+    A->flags = Buffer(ctx->arena, 32);
+    zero(ctx, A->flags.ptr, 32);
     A->nan_ids = Buffer(ctx->arena, 8);  // numbers of NaN group keys, never reused
     zero(ctx, A->nan_ids.ptr, 8);
     A->planned = true;
@@ -2050,6 +2075,8 @@ struct ml_join_build {
     int64_t cap = 0;             // rows the buffers hold
     mlg::Buffer count_buf;       // keeps count_dev alive
     const uint64_t* count_dev = nullptr;
+    mlg::Buffer partials;        // per-workgroup key ranges of the kernel that wrote the rows, or empty
+    int nparts = 0;
   };
   std::vector<Chunk> chunks;
 };
@@ -2135,6 +2162,12 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
   // no filter, probe or selection: every source row is a build row, written
   // at its own index by one row-per-lane kernel (no counting, no compaction)
   const bool identity = !B.expanding && B.P.nsteps == 0 && !B.P.sel;
+  // a table too large for the one-workgroup build whose first key may make it
+  // dense: the kernel that writes the rows also folds their first key words'
+  // range per workgroup, so no separate pass over the keys finds it
+  const int kt0 = J->key_types[0];
+  const bool ranged = !B.expanding && cap > SMALL_BUILD_ROWS &&
+                      (kt0 == ML_TYPE_INT32 || kt0 == ML_TYPE_INT64 || kt0 == ML_TYPE_DATE);
   if (!B.expanding && (cap <= SMALL_BUILD_ROWS || identity)) {
     // one append kernel, its count left on the device for the dense build's
     // first kernel (rows in any order: the table orders them)
@@ -2143,12 +2176,25 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
     else zero(ctx, ch.count_buf.ptr, 16);
     B.P.out_count = ch.count_buf.as<uint64_t>();
     B.P.out_overflow = (uint32_t*)(ch.count_buf.as<uint64_t>() + 1);
-    if (B.active_rows > 0) ctx->launch(B.P, B.grid(), block_of(ctx), 0);
+    const int grid = B.grid();
+    if (ranged) {
+      ch.nparts = grid;
+      ch.partials = Buffer(ctx->arena, (size_t)grid * 24);
+      B.P.key_partials = ch.partials.as<int64_t>();
+    }
+    if (B.active_rows > 0) ctx->launch(B.P, grid, block_of(ctx), 0);
     B.P.out_identity = 0;
+    B.P.key_partials = nullptr;
     ch.rows = -1;
     ch.count_dev = ch.count_buf.as<uint64_t>();
   } else if (plan_flow(B, SINK_ROWS, sink_read_stages(B), 0, F)) {
+    if (ranged) {  // one partial per wave region (the materialization's workgroups)
+      ch.nparts = (int)F.nwaves;
+      ch.partials = Buffer(ctx->arena, (size_t)F.nwaves * 24);
+      B.P.key_partials = ch.partials.as<int64_t>();
+    }
     run_flow_rows(B, SINK_JOIN_BUILD, F, &ch.count_buf);  // the count is read with the key range
+    B.P.key_partials = nullptr;
     ch.rows = -1;
     ch.count_dev = ch.count_buf.as<uint64_t>() + F.nwaves;
   } else {
@@ -2403,14 +2449,20 @@ static void join_finalize_enqueue(ml_join_build* J, FinalizePrep& P, int64_t* pi
   }
   if (J->chunks.size() == 1 && J->chunks[0].rows < 0 && dense_type && (uint64_t)J->chunks[0].cap < 0xFFFFFFFFull) {
     auto& c = J->chunks[0];
-    // at most 256 workgroups: each ends with one atomic on a shared counter
-    const int nparts = (int)std::min<int64_t>(256, std::max<int64_t>(1, (c.cap + 4095) / 4096));
-    P.partials = Buffer(ctx->arena, (size_t)nparts * 24);
     P.mm = Buffer(ctx->arena, 32);
-    // (also clears flags[0] for the presence marking and flags[1] for the permutation)
-    launch_key_minmax(c.keys.as<uint64_t>(), (int)nk, c.key_null.as<uint8_t>(), c.cap, c.count_dev,
-                      P.partials.as<int64_t>(), nparts, P.mm.as<int64_t>(),
-                      (uint32_t*)(ctx->counters.as<uint64_t>() + ml_ctx::DONE_WORD), flags, ctx->stream);
+    // (both also clear flags[0] for the presence marking and flags[1] for the permutation)
+    if (c.partials.ptr) {
+      // the rows' kernel left per-workgroup ranges: one workgroup folds them
+      launch_key_range_reduce(c.partials.as<int64_t>(), c.nparts, c.cap, c.count_dev, P.mm.as<int64_t>(), flags,
+                              ctx->stream);
+    } else {
+      // at most 256 workgroups: each ends with one atomic on a shared counter
+      const int nparts = (int)std::min<int64_t>(256, std::max<int64_t>(1, (c.cap + 4095) / 4096));
+      P.partials = Buffer(ctx->arena, (size_t)nparts * 24);
+      launch_key_minmax(c.keys.as<uint64_t>(), (int)nk, c.key_null.as<uint8_t>(), c.cap, c.count_dev,
+                        P.partials.as<int64_t>(), nparts, P.mm.as<int64_t>(),
+                        (uint32_t*)(ctx->counters.as<uint64_t>() + ml_ctx::DONE_WORD), flags, ctx->stream);
+    }
     hip_check(hipMemcpyAsync(pinned, P.mm.ptr, 32, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
     P.ranged = true;
   }
@@ -2539,38 +2591,26 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
         const int64_t ne = (int64_t)((R + 63) / 64);
         Buffer bits(ctx->arena, (size_t)ne * 8);
         zero(ctx, bits.ptr, bits.bytes);
-        launch_dense_mark(keys.as<uint64_t>(), (int)nk, key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(),
-                          flags, ctx->stream);
-        // the repeated-key flag is read when the table is first used; the
-        // rank and permutation run meanwhile (a repeated key discards them)
-        const bool deferred = defer_check(ctx, T, flags, PENDING_DENSE);
-        uint32_t hf = 0;
-        if (!deferred) d2h(ctx, &hf, flags, 4);
-        if (!(hf & 5)) {
-          // every key of the range present (all distinct, or the table is
-          // rebuilt hashed): a row's rank is its key's offset, so no ranks are
-          // computed, and a probe reads neither bits nor ranks; a Bloom filter
-          // would cost the one load it saves
-          const bool full = hm[3] == (int64_t)R;
-          // otherwise the rows are held at their key offset when that costs at
-          // most 4x the rows' bytes or 256 MB: a probe then reads the presence
-          // bit and the row at the key's offset, with no rank directory
-          // between them (one dependent load fewer per hit)
-          const int64_t nnull = n - hm[3];
-          const uint64_t offset_rows = R + (uint64_t)nnull;
-          uint64_t row_bytes = (nk - 1) * key_word_bytes + (nk == 2 ? 1 : 0);
-          for (size_t c = 0; c < T.cols.size(); c++)
-            row_bytes += (uint64_t)type_width(T.col_types[c]) + (T.col_valid[c].ptr ? 1 : 0);
-          const bool offset = !full && (offset_rows <= 4 * (uint64_t)n || offset_rows * row_bytes <= (256ull << 20));
-          const int64_t held = (full || offset) ? (int64_t)offset_rows : n;  // full: R + NULL-key rows = n
-          Buffer rank;
-          if (!full && !offset) {
-            rank = Buffer(ctx->arena, (size_t)(ne + 1) * 4);
-            Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(ne) + 1) * 4);
-            launch_dense_rank(bits.as<uint64_t>(), ne, rank.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
-          }
-          DensePermute M{};
-          Buffer dkeys, dkeys8;
+        // every key of the range present (all distinct, or the table is
+        // rebuilt hashed): a row's rank is its key's offset, so no ranks are
+        // computed, and a probe reads neither bits nor ranks; a Bloom filter
+        // would cost the one load it saves
+        const bool full = hm[3] == (int64_t)R;
+        // otherwise the rows are held at their key offset when that costs at
+        // most 4x the rows' bytes or 256 MB: a probe then reads the presence
+        // bit and the row at the key's offset, with no rank directory
+        // between them (one dependent load fewer per hit)
+        const int64_t nnull = n - hm[3];
+        const uint64_t offset_rows = R + (uint64_t)nnull;
+        uint64_t row_bytes = (nk - 1) * key_word_bytes + (nk == 2 ? 1 : 0);
+        for (size_t c = 0; c < T.cols.size(); c++)
+          row_bytes += (uint64_t)type_width(T.col_types[c]) + (T.col_valid[c].ptr ? 1 : 0);
+        const bool offset = !full && (offset_rows <= 4 * (uint64_t)n || offset_rows * row_bytes <= (256ull << 20));
+        const int64_t held = (full || offset) ? (int64_t)offset_rows : n;  // full: R + NULL-key rows = n
+        DensePermute M{};
+        Buffer dkeys, dkeys8;
+        std::vector<Buffer> cols, valid;
+        auto prepare_rows = [&] {
           if (nk > 1) dkeys = Buffer(ctx->arena, (size_t)std::max<int64_t>(held, 1) * (nk - 1) * key_word_bytes);
           if (nk == 2) {  // also a byte per row, used if every second key word fits (flag bit 8)
             dkeys8 = Buffer(ctx->arena, (size_t)std::max<int64_t>(held, 1));
@@ -2581,7 +2621,6 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
           M.dst_keys = dkeys.as<uint64_t>();
           M.held = held;
           M.ncols = (int32_t)T.cols.size();
-          std::vector<Buffer> cols, valid;
           for (size_t c = 0; c < T.cols.size(); c++) {
             const int w = type_width(T.col_types[c]);
             cols.emplace_back(ctx->arena, (size_t)std::max<int64_t>(held, 1) * w);
@@ -2593,10 +2632,35 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
             M.src_valid[c] = T.col_valid[c].ptr ? T.col_valid[c].as<uint8_t>() : nullptr;
             M.dst_valid[c] = valid.back().ptr ? valid.back().as<uint8_t>() : nullptr;
           }
-          // flags[1] (next NULL-key row) was cleared by the min/max kernel
-          launch_dense_permute(M, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(),
-                               rank.ptr ? rank.as<uint32_t>() : nullptr, rank.ptr ? rank.as<uint32_t>() + ne : nullptr,
-                               flags + 1, ctx->stream);
+        };
+        // rows at key offsets: marking and moves in one pass (flags[0..1]
+        // were cleared by the range kernel). Ranked: the marking, then the
+        // ranks and the moves once the repeated-key flag allows them.
+        const bool place = full || offset;
+        if (place) {
+          prepare_rows();
+          launch_dense_place(M, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(), flags,
+                             flags + 1, ctx->stream);
+        } else {
+          launch_dense_mark(keys.as<uint64_t>(), (int)nk, key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(),
+                            flags, ctx->stream);
+        }
+        // the repeated-key flag is read when the table is first used; the
+        // rank and permutation run meanwhile (a repeated key discards them)
+        const bool deferred = defer_check(ctx, T, flags, PENDING_DENSE);
+        uint32_t hf = 0;
+        if (!deferred) d2h(ctx, &hf, flags, 4);
+        if (!(hf & 5)) {
+          Buffer rank;
+          if (!place) {
+            rank = Buffer(ctx->arena, (size_t)(ne + 1) * 4);
+            Buffer sums(ctx->arena, (size_t)(scan_u32_blocks(ne) + 1) * 4);
+            launch_dense_rank(bits.as<uint64_t>(), ne, rank.as<uint32_t>(), sums.as<uint32_t>(), ctx->stream);
+            prepare_rows();
+            // flags[1] (next NULL-key row) was cleared by the range kernel
+            launch_dense_permute(M, keys.as<uint64_t>(), key_null.as<uint8_t>(), n, hm[0], R, bits.as<uint64_t>(),
+                                 rank.as<uint32_t>(), rank.as<uint32_t>() + ne, flags + 1, ctx->stream);
+          }
           if (nk > 1 && !full && hashed_bloom(ctx, n)) {  // a word per 8 first-key values
             const uint64_t words = (R + (1u << DENSE_BLOOM_SHIFT) - 1) >> DENSE_BLOOM_SHIFT;
             T.bloom = Buffer(ctx->arena, words * 8);

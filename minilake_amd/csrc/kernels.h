@@ -137,6 +137,14 @@ struct DensePermute {
 // Rows into key order; rank and present null for a table whose rows sit at
 // their key's offset (a full table, where every key of the range is present,
 // or an offset-indexed one), the NULL-key rows from `range` on.
+// Presence bits and rows of a table held at key offsets in one pass (M.held
+// positions; flag bits as launch_dense_mark; *null_next from 0).
+void launch_dense_place(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin,
+                        uint64_t range, uint64_t* bits, uint32_t* flag, uint32_t* null_next, hipStream_t s);
+// mm = {min, max, rows, non-NULL rows} from nparts {min, max, count} partials
+// of a join build's row kernel (rows = min(n, *n_dev)); clears clear[0..1].
+void launch_key_range_reduce(const int64_t* partials, int nparts, int64_t n, const uint64_t* n_dev, int64_t* mm,
+                             uint32_t* clear, hipStream_t s);
 void launch_dense_permute(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
                           int64_t kmin, uint64_t range, const uint64_t* bits, const uint32_t* rank,
                           const uint32_t* present, uint32_t* null_next, hipStream_t s);
@@ -171,7 +179,8 @@ struct PartAggPlan {
   AccSlot acc[MAX_AGG_SLOTS];
   uint64_t* slots;            // final table: [slot][1 + nkw]
   void* acc_data[MAX_AGG_SLOTS];
-  uint32_t* flags;            // [0..1] group count (u64), [2] overflow bits
+  uint32_t* flags;            // [0..1] group count (u64), [2] overflow bits, [3] filled partitions,
+                              // [4..5] projected groups of the filled partitions (u64)
 };
 
 struct PartChunk {

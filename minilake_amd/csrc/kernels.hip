@@ -1000,13 +1000,61 @@ __device__ __forceinline__ void write_cols(const Plan& P, const Code& ins, const
   }
 }
 
-__device__ __forceinline__ void sink_join_build(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES]) {
+// Range and count of the first key words a thread wrote (rows without a NULL
+// key), folded per workgroup for the dense layout's choice.
+struct KeyRange {
+  int64_t lo = INT64_MAX, hi = INT64_MIN;
+  int64_t n = 0;
+  __device__ __forceinline__ void add(int64_t k) {
+    lo = k < lo ? k : lo;
+    hi = k > hi ? k : hi;
+    n++;
+  }
+};
+
+// Fold every thread's KeyRange into thread 0's (all threads of the workgroup
+// call it; at most 1024 threads).
+__device__ __forceinline__ void block_key_range(KeyRange& kr) {
+  __shared__ int64_t s_lo[16], s_hi[16], s_n[16];
+  for (int o = 32; o > 0; o >>= 1) {
+    kr.lo = min(kr.lo, (int64_t)__shfl_down(kr.lo, o));
+    kr.hi = max(kr.hi, (int64_t)__shfl_down(kr.hi, o));
+    kr.n += (int64_t)__shfl_down(kr.n, o);
+  }
+  const int wv = threadIdx.x >> 6;
+  if (__lane_id() == 0) {
+    s_lo[wv] = kr.lo;
+    s_hi[wv] = kr.hi;
+    s_n[wv] = kr.n;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < (int)((blockDim.x + 63) >> 6); w++) {
+      kr.lo = min(kr.lo, s_lo[w]);
+      kr.hi = max(kr.hi, s_hi[w]);
+      kr.n += s_n[w];
+    }
+}
+
+// The workgroup's {min, max, count} at out[0..2] (read by the next kernel).
+__device__ __forceinline__ void write_key_range(KeyRange kr, int64_t* out) {
+  block_key_range(kr);
+  if (threadIdx.x == 0) {
+    out[0] = kr.lo;
+    out[1] = kr.hi;
+    out[2] = kr.n;
+  }
+}
+
+__device__ __forceinline__ void sink_join_build(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES],
+                                                KeyRange& kr) {
   const uint64_t pos = P.out_identity ? (uint64_t)rows[0] : wave_alloc(P.out_count, 1);
   if (pos >= P.out_capacity) {
     atomicOr(P.out_overflow, 1u);
     return;
   }
   bool any_null = false;
+  uint64_t k0 = 0;
 #pragma unroll 1
   for (int k = 0; k < P.nkeys; k++) {
     uint64_t v;
@@ -1014,9 +1062,11 @@ __device__ __forceinline__ void sink_join_build(const Plan& P, const Code& ins, 
       any_null = true;
       v = 0;
     }
+    if (k == 0) k0 = v;
     P.key_out[pos * P.nkeys + k] = v;
   }
   P.key_null[pos] = any_null;
+  if (!any_null) kr.add((int64_t)k0);
   write_cols(P, ins, rows, pos);
 }
 
@@ -1047,10 +1097,10 @@ __device__ __forceinline__ void collect_at(const Plan& P, const Code& ins, const
 
 template <int SINK>
 __device__ __forceinline__ void do_sink(const Plan& P, const Code& ins, const uint32_t (&rows)[MAX_STAGES], int64_t i,
-                                        uint64_t* lds) {
+                                        uint64_t* lds, KeyRange& kr) {
   if constexpr (SINK == SINK_HASH_AGG) sink_hash_agg(P, ins, rows);
   else if constexpr (SINK == SINK_DIRECT_AGG) sink_direct_agg(P, ins, rows, lds);
-  else if constexpr (SINK == SINK_JOIN_BUILD) sink_join_build(P, ins, rows);
+  else if constexpr (SINK == SINK_JOIN_BUILD) sink_join_build(P, ins, rows, kr);
   else if constexpr (SINK == SINK_COLLECT) sink_collect(P, ins, rows);
   else if constexpr (SINK == SINK_COUNT_PAIRS) wave_alloc(P.out_count, 1);
   else if constexpr (SINK == SINK_PROJECT) write_cols(P, ins, rows, (uint64_t)rows[0]);
@@ -1094,6 +1144,7 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
       P.out_count[1] = 0;
     }
   }
+  KeyRange kr;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = P.begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.end; i += stride) {
     uint32_t rows[MAX_STAGES];
@@ -1109,7 +1160,7 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
           if (P.thread_offsets) collect_at(P, ins, rows, cursor++);
           else sink_collect(P, ins, rows);
         }
-      } else if (alive) do_sink<SINK>(P, ins, rows, i, lds);
+      } else if (alive) do_sink<SINK>(P, ins, rows, i, lds, kr);
     } else {
       // one probe may match several build rows: the rest of the chain runs per match
       const int e = P.expand_step;
@@ -1130,7 +1181,7 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
                 else if constexpr (SINK == SINK_COLLECT) {
                   if (P.thread_offsets) collect_at(P, ins, rows, cursor++);
                   else sink_collect(P, ins, rows);
-                } else do_sink<SINK>(P, ins, rows, i, lds);
+                } else do_sink<SINK>(P, ins, rows, i, lds, kr);
               }
               pos = join_step(T, pos);
             }
@@ -1143,6 +1194,9 @@ __global__ void __launch_bounds__(256) pipeline_kernel(Plan P) {
     __syncthreads();
     uint64_t* out = (uint64_t*)P.direct_partials + (uint64_t)blockIdx.x * ncells;
     for (int c = threadIdx.x; c < ncells; c += blockDim.x) out[c] = lds[c];
+  }
+  if constexpr (SINK == SINK_JOIN_BUILD) {
+    if (P.key_partials) write_key_range(kr, P.key_partials + 3 * (int64_t)blockIdx.x);
   }
   if constexpr (SINK == SINK_COUNT_PAIRS) {
     // per-thread counts for a two-pass collect (a count past 32 bits flags it)
@@ -1715,6 +1769,7 @@ __global__ void __launch_bounds__(256) materialize_kernel(Plan P, const uint64_t
   const int64_t w = blockIdx.x;
   const uint32_t cnt = P.wave_counts[w];
   const uint64_t base = offsets[w];
+  KeyRange kr;
   for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
     const uint64_t src = (uint64_t)w * P.chunk + k;
     uint32_t rows[MAX_STAGES];
@@ -1724,6 +1779,7 @@ __global__ void __launch_bounds__(256) materialize_kernel(Plan P, const uint64_t
     const uint64_t pos = base + k;
     if constexpr (SINK == SINK_JOIN_BUILD) {
       bool any_null = false;
+      uint64_t k0 = 0;
 #pragma unroll 1
       for (int kk = 0; kk < P.nkeys; kk++) {
         uint64_t v;
@@ -1731,15 +1787,20 @@ __global__ void __launch_bounds__(256) materialize_kernel(Plan P, const uint64_t
           any_null = true;
           v = 0;
         }
+        if (kk == 0) k0 = v;
         P.key_out[pos * P.nkeys + kk] = v;
       }
       P.key_null[pos] = any_null;
+      if (!any_null) kr.add((int64_t)k0);
     }
     write_cols(P, ins, rows, pos);
     if constexpr (SINK == SINK_COLLECT) {
       if (P.pair_out)
         for (int s = 0; s < P.pair_stages; s++) P.pair_out[pos * P.pair_stages + s] = row_of(rows, s);
     }
+  }
+  if constexpr (SINK == SINK_JOIN_BUILD) {
+    if (P.key_partials) write_key_range(kr, P.key_partials + 3 * w);
   }
 }
 
@@ -1974,6 +2035,23 @@ __global__ void dense_mark_kernel(const uint64_t* keys, int nkw, const uint8_t* 
   if (__any(wide) && lane == 0) atomicOr(flag, 256u);
 }
 
+// Build row r's key words 1.. and columns, written at table position pos.
+__device__ __forceinline__ void dense_write_row(const DensePermute& M, const uint64_t* keys, int64_t r, uint64_t pos) {
+  for (int k = 1; k < M.nkw; k++) {
+    const uint64_t at = pos * (M.nkw - 1) + k - 1;
+    if (M.keys32) ((uint32_t*)M.dst_keys)[at] = (uint32_t)keys[r * M.nkw + k];
+    else M.dst_keys[at] = keys[r * M.nkw + k];
+  }
+  if (M.dst_keys8) M.dst_keys8[pos] = (uint8_t)keys[r * 2 + 1];
+  for (int c = 0; c < M.ncols; c++) {
+    const int w = M.width[c];
+    if (w == 8) ((uint64_t*)M.dst[c])[pos] = ((const uint64_t*)M.src[c])[r];
+    else if (w == 4) ((uint32_t*)M.dst[c])[pos] = ((const uint32_t*)M.src[c])[r];
+    else ((uint8_t*)M.dst[c])[pos] = ((const uint8_t*)M.src[c])[r];
+    if (M.src_valid[c]) M.dst_valid[c][pos] = M.src_valid[c][r];
+  }
+}
+
 // Build rows into key order: a row with key k moves to the rank of k; rows
 // with a NULL key (never matched) follow the present keys in any order.
 __global__ void dense_permute_kernel(DensePermute M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
@@ -1994,19 +2072,86 @@ __global__ void dense_permute_kernel(DensePermute M, const uint64_t* keys, const
       pos = null_base + atomicAdd(null_next, 1u);
       if (pos >= held) continue;  // only with repeated keys, which also fall back
     }
-    for (int k = 1; k < M.nkw; k++) {
-      const uint64_t at = pos * (M.nkw - 1) + k - 1;
-      if (M.keys32) ((uint32_t*)M.dst_keys)[at] = (uint32_t)keys[r * M.nkw + k];
-      else M.dst_keys[at] = keys[r * M.nkw + k];
+    dense_write_row(M, keys, r, pos);
+  }
+}
+
+// A table whose rows sit at their key's offset (a full table, or an
+// offset-indexed one) in one pass: each row sets its key's presence bit, with
+// the flags of dense_mark_kernel, and moves to that offset; NULL-key rows
+// follow from `range` on, in any order (*null_next from 0). A repeated key
+// leaves overwritten rows behind, and its flag sends the table to the hashed
+// layout.
+__global__ void dense_place_kernel(DensePermute M, const uint64_t* keys, const uint8_t* key_null, int64_t n,
+                                   int64_t kmin, uint64_t range, uint64_t* bits, uint32_t* flag, uint32_t* null_next) {
+  const int lane = __lane_id();
+  const int nkw = M.nkw;
+  const uint64_t held = M.held > 0 ? (uint64_t)M.held : (uint64_t)n;
+  bool dup = false, outside = false, wide = false;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // every lane of a wave takes the same trips (the shuffles below need all 64)
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
+    const int64_t r = base + lane;
+    uint64_t w = ~0ull, v = 0, pos = ~0ull;
+    const bool live = r < n;
+    const bool is_null = live && key_null[r];
+    if (live && !is_null) {
+      if (nkw == 2) wide = wide || keys[r * 2 + 1] > 255u;
+      const uint64_t i = (uint64_t)((int64_t)keys[r * nkw] - kmin);
+      if (i < range) {
+        w = i >> 6;
+        v = 1ull << (i & 63);
+        pos = i;
+      } else {
+        outside = true;
+      }
     }
-    if (M.dst_keys8) M.dst_keys8[pos] = (uint8_t)keys[r * 2 + 1];
-    for (int c = 0; c < M.ncols; c++) {
-      const int w = M.width[c];
-      if (w == 8) ((uint64_t*)M.dst[c])[pos] = ((const uint64_t*)M.src[c])[r];
-      else if (w == 4) ((uint32_t*)M.dst[c])[pos] = ((const uint32_t*)M.src[c])[r];
-      else ((uint8_t*)M.dst[c])[pos] = ((const uint8_t*)M.src[c])[r];
-      if (M.src_valid[c]) M.dst_valid[c][pos] = M.src_valid[c][r];
+    if (is_null) pos = range + atomicAdd(null_next, 1u);
+    // lanes whose keys share a presence word OR their bits first (as in
+    // dense_mark_kernel): one atomic per run of a word
+    const uint64_t wn = __shfl_down(w, 1);
+    int end = (lane == 63 || wn != w) ? 1 : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t vn = __shfl_down(v, d);
+      const int en = __shfl_down(end, d);
+      if (!end && lane + d < 64) {
+        dup = dup || (v & vn) != 0;
+        v |= vn;
+        end = en;
+      }
     }
+    const uint64_t wp = __shfl_up(w, 1);
+    if ((lane == 0 || wp != w) && w != ~0ull) {
+      const unsigned long long old = atomicOr((unsigned long long*)&bits[w], (unsigned long long)v);
+      dup = dup || (old & v) != 0;
+    }
+    if (pos < held) dense_write_row(M, keys, r, pos);  // beyond held only with repeated keys
+  }
+  if (__any(dup) && lane == 0) atomicOr(flag, 1u);
+  if (__any(outside) && lane == 0) atomicOr(flag, 4u);
+  if (__any(wide) && lane == 0) atomicOr(flag, 256u);
+}
+
+// Fold the per-workgroup {min, max, count} partials a join build's row
+// kernel wrote into mm = {min, max, rows, non-NULL rows} (as key_minmax_kernel)
+// and clear the two flag words clear[0..1] that the dense build uses next.
+__global__ void __launch_bounds__(256) key_range_reduce_kernel(const int64_t* partials, int nparts, int64_t n,
+                                                               const uint64_t* n_dev, int64_t* mm, uint32_t* clear) {
+  KeyRange kr;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    kr.lo = min(kr.lo, partials[3 * i]);
+    kr.hi = max(kr.hi, partials[3 * i + 1]);
+    kr.n += partials[3 * i + 2];
+  }
+  block_key_range(kr);
+  if (threadIdx.x == 0) {
+    mm[0] = kr.lo;
+    mm[1] = kr.hi;
+    mm[2] = n_dev ? min(n, (int64_t)*n_dev) : n;
+    mm[3] = kr.n;
+    clear[0] = 0;
+    clear[1] = 0;
   }
 }
 
@@ -2620,12 +2765,16 @@ __global__ void __launch_bounds__(256) agg_part_chunk_kernel(PartAggPlan A, cons
                                                              const uint32_t* rvalid, int64_t n, uint64_t* partial) {
   extern __shared__ uint64_t lds[];
   __shared__ uint32_t table_full;  // the batch is aggregated again: stop early
+  __shared__ uint32_t fail_at;     // the first record that found the table full
   const PartChunk C = chunks[blockIdx.x];
   const int S = A.S;
   uint64_t* hdr = lds;
   uint64_t* kw = lds + S;
   uint64_t* acc = kw + (uint64_t)S * A.nkw;
-  if (threadIdx.x == 0) table_full = 0;
+  if (threadIdx.x == 0) {
+    table_full = 0;
+    fail_at = C.hi;
+  }
   part_init(A, hdr, acc);
   bool full = false;
   for (uint32_t e = C.lo + threadIdx.x; e < C.hi; e += blockDim.x) {
@@ -2639,6 +2788,7 @@ __global__ void __launch_bounds__(256) agg_part_chunk_kernel(PartAggPlan A, cons
       // the batch, so nothing more of this chunk is needed
       full = true;
       __hip_atomic_store(&table_full, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      atomicMin(&fail_at, e);
       break;
     }
     const uint32_t vm = rvalid ? rvalid[e] : 0xFFFFFFFFu;
@@ -2653,6 +2803,14 @@ __global__ void __launch_bounds__(256) agg_part_chunk_kernel(PartAggPlan A, cons
   }
   if (__any(full) && __lane_id() == 0) atomicOr(A.flags + 2, 2u);
   __syncthreads();
+  // partitions whose table filled up (a split partition is counted by its
+  // merge), each with its projected groups: S groups after the records before
+  // fail_at, scaled to all of the partition's records
+  if (threadIdx.x == 0 && table_full && C.partial == 0xFFFFFFFFu) {
+    atomicAdd(A.flags + 3, 1u);
+    const uint64_t seen = max(1u, fail_at - C.lo);
+    atomicAdd((unsigned long long*)(A.flags + 4), (unsigned long long)((uint64_t)S * (C.hi - C.lo) / seen));
+  }
   if (C.partial == 0xFFFFFFFFu) {
     part_write(A, hdr, kw, acc, A.slots, A.acc_data, (uint64_t)C.part * S, true);
   } else {  // a partial table: [S] headers, [S * nkw] keys, [nacc * S] states
@@ -2700,6 +2858,10 @@ __global__ void __launch_bounds__(256) agg_part_merge_kernel(PartAggPlan A, cons
   }
   if (__any(full) && __lane_id() == 0) atomicOr(A.flags + 2, 2u);
   __syncthreads();
+  if (threadIdx.x == 0 && table_full) {  // no projection here: at least twice the table
+    atomicAdd(A.flags + 3, 1u);
+    atomicAdd((unsigned long long*)(A.flags + 4), (unsigned long long)(2 * S));
+  }
   part_write(A, hdr, kw, acc, A.slots, A.acc_data, (uint64_t)parts[blockIdx.x] * S, true);
 }
 
@@ -3283,6 +3445,17 @@ void launch_dense_compact(const void* src, int width, const uint8_t* src_valid, 
 
 void launch_dense_rank(const uint64_t* bits, int64_t nw, uint32_t* rank, uint32_t* block_sums, hipStream_t s) {
   scan_launch(bits, 1, nw, rank, block_sums, s);
+}
+
+void launch_dense_place(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin,
+                        uint64_t range, uint64_t* bits, uint32_t* flag, uint32_t* null_next, hipStream_t s) {
+  if (n > 0)
+    dense_place_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(M, keys, key_null, n, kmin, range, bits, flag, null_next);
+}
+
+void launch_key_range_reduce(const int64_t* partials, int nparts, int64_t n, const uint64_t* n_dev, int64_t* mm,
+                             uint32_t* clear, hipStream_t s) {
+  key_range_reduce_kernel<<<1, 256, 0, s>>>(partials, nparts, n, n_dev, mm, clear);
 }
 
 void launch_dense_permute(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n,

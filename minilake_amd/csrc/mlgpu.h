@@ -332,6 +332,7 @@ struct Plan {
   uint8_t* col_valid[MAX_COLS];  // u8 validity or null
   uint64_t* key_out;         // JOIN_BUILD: key words [rows][nkeys]
   uint8_t* key_null;         // JOIN_BUILD: 1 if any key is NULL
+  int64_t* key_partials;     // JOIN_BUILD: per workgroup {min, max, count} of key word 0 over non-NULL rows, or null
   uint64_t* out_count;       // rows written so far (device counter)
   uint64_t out_capacity;
   uint32_t* out_overflow;

@@ -268,6 +268,25 @@ def run_plain(engine, fact, build, stats: Optional[dict] = None):
     return agg, joined
 
 
+# The group table plain sizing reserves before its first device batch (2^21
+# rows: reserve(len + batch), the same 4M-slot capacity).
+PLAIN_FIRST_RESERVE = 1 << 21
+
+
+def run_plain_partitioned(engine, fact, build, stats: Optional[dict] = None):
+    """run_plain's queries with the aggregate on the learned path's algorithm
+    (the partitioned LDS aggregation) but no estimate: it starts from the
+    capacity plain sizing reserves for its first device batch and doubles
+    whenever a partition's table fills up (the rows partitioned again), so
+    learned vs this compares sizing alone. The join as in run_plain."""
+    from .abi import AGG_SUM, INT64, Agg, col, count_star, probe_op
+    agg = engine.hashagg(fact, [], [col(0)], [INT64], [count_star(), Agg(AGG_SUM, col(1), INT64)],
+                         sizing=abi.SIZING_LEARNED, capacity_hint=PLAIN_FIRST_RESERVE, stats=stats)
+    jt = engine.join_build(build, [], [col(0)], [INT64])
+    joined = engine.collect(fact, [probe_op(jt, [col(0)])])
+    return agg, joined
+
+
 def predict(engine, mlp: "Mlp", fact, build_rows: int, build_distinct: int, jt, seed: int = 0):
     x = features(engine, fact.tensors[0], fact.num_rows, jt, build_rows, build_distinct, seed=seed)
     return mlp.forward(x[None, :])[0]
@@ -394,17 +413,23 @@ def compare_plain_learned(engine, mlp: "Mlp", fact, build, build_distinct: int, 
             ts.append((time.perf_counter() - t0) * 1e3)
         return sorted(ts)[len(ts) // 2], out
 
-    ps, ls = {}, {}
+    ps, pps, ls = {}, {}, {}
     plain_ms, (agg_p, join_p) = timed(lambda: run_plain(engine, fact, build, stats=ps))
+    part_ms, (agg_pp, join_pp) = timed(lambda: run_plain_partitioned(engine, fact, build, stats=pps))
     learned_ms, (agg_l, join_l, y, fell_back) = timed(lambda: run_learned(engine, mlp, fact, build, build_distinct,
                                                                           stats=ls))
     out = {"plain_ms": plain_ms, "learned_ms": learned_ms, "speedup": plain_ms / learned_ms,
+           # the same partitioned algorithm without an estimate: sizing alone
+           "plain_partitioned_ms": part_ms, "speedup_vs_plain_partitioned": part_ms / learned_ms,
+           "plain_partitioned_resizes": pps.get("resizes"),
            "groups": agg_p.num_rows, "predicted_groups": float(2.0 ** y[0]),
            "join_rows": join_p.num_rows, "predicted_join_rows": float(2.0 ** y[1] - 1),
            "plain_resizes": ps.get("resizes"), "learned_resizes": ls.get("resizes"),
            "join_fell_back": bool(fell_back),
-           "identical": same_rows(agg_p, agg_l) and same_rows(join_p, join_l)}
+           "identical": (same_rows(agg_p, agg_l) and same_rows(join_p, join_l) and same_rows(agg_p, agg_pp)
+                         and same_rows(join_p, join_pp))}
     if log:
-        log(f"learned sizing: plain {plain_ms:.2f} ms, learned {learned_ms:.2f} ms "
-            f"({out['speedup']:.2f}x), identical={out['identical']}")
+        log(f"learned sizing: plain {plain_ms:.2f} ms, plain partitioned {part_ms:.2f} ms "
+            f"({out['plain_partitioned_resizes']} doublings), learned {learned_ms:.2f} ms "
+            f"({out['speedup']:.2f}x / {out['speedup_vs_plain_partitioned']:.2f}x), identical={out['identical']}")
     return out

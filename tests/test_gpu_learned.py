@@ -90,7 +90,7 @@ def test_learned_sizing_bit_identical_and_faster(gpu_engine):
     params = sizing.load_params()
     assert params is not None, "trained weights (minilake_amd/estimator_weights.npy) are missing"
     mlp = sizing.Mlp(gpu_engine, params)
-    # 16M Zipf(1.1) keys over 1e8 values: ~1e7 groups, so plain sizing grows its
+    # 16M Zipf(1.1) keys over 1e8 values: ~2.2M groups, so plain sizing grows its
     # table (2^21-row device batches reserve len + batch) while learned does not
     fact, build = sizing.zipf_workload(gpu_engine, rows=16_000_000, domain=100_000_000, build_keys=1_000_000)
     r = sizing.compare_plain_learned(gpu_engine, mlp, fact, build, build_distinct=1_000_000, reps=3)
@@ -98,6 +98,9 @@ def test_learned_sizing_bit_identical_and_faster(gpu_engine):
     assert r["identical"]
     assert r["plain_resizes"] >= 1
     assert r["learned_resizes"] == 0
+    # the partitioned aggregation started from plain sizing's first reservation
+    # (4M slots) holds ~2.2M groups without doubling
+    assert r["plain_partitioned_resizes"] == 0
     assert r["learned_ms"] < r["plain_ms"]
 
 

@@ -485,14 +485,19 @@ def test_hashagg_partitioned_matches_oracle(gpu_engine, oracle_engine, case):
 
 def test_hashagg_partitioned_underestimate(gpu_engine, oracle_engine):
     """A hint far below the real group count overflows the partitions: HINT
-    reports ML_ERR_CAPACITY, LEARNED reruns with plain sizing and is exact."""
+    reports ML_ERR_CAPACITY; LEARNED doubles the partitioned table (the rows
+    partitioned again) until every partition fits, and is exact."""
     t = _agg_table(800_000, 600_000, 6)  # ~440k groups, far above a 256k-slot table
     aggs = [Agg(AGG_COUNT_STAR), Agg(AGG_SUM, col(3), INT64)]
     want = oracle_engine.hashagg(t, [], [col(0)], [INT64], aggs).to_host()
     with pytest.raises(Exception):
         gpu_engine.hashagg(t, [], [col(0)], [INT64], aggs, sizing=abi.SIZING_HINT, capacity_hint=70_000)
-    got = gpu_engine.hashagg(t, [], [col(0)], [INT64], aggs, sizing=abi.SIZING_LEARNED, capacity_hint=70_000)
+    st = {}
+    got = gpu_engine.hashagg(t, [], [col(0)], [INT64], aggs, sizing=abi.SIZING_LEARNED, capacity_hint=70_000,
+                             stats=st)
     assert_tables_equal(got.to_host(), want, rtol=0.0)
+    # 256k slots -> 512k -> (at most) 1M for ~440k groups
+    assert 1 <= st["resizes"] <= 2 and st["capacity"] >= 512 * 1024
 
 
 def test_ungrouped_aggregate_matches_oracle(gpu_engine, oracle_engine):
