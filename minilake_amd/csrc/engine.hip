@@ -1481,7 +1481,7 @@ static bool agg_partitioned(ml_hashagg* A, PipelineBuild& B) {
     // got) with the initial 2x margin, and at least twice the capacity. More
     // groups than the partitions can hold: the plain path.
     const uint64_t over = std::min<uint64_t>(f[1] >> 32, nparts);
-    const uint64_t seen = f[0] - std::min<uint64_t>(f[0], over * S) + f[2];
+    const uint64_t seen = f[0] - std::min<uint64_t>(f[0], over * (S - S / 8)) + f[2];
     const uint64_t next = std::max<uint64_t>(cap * 2, next_pow2(2 * seen + 1));
     if (!fits(next)) break;
     zero(ctx, A->flags.ptr, 32);

@@ -2700,9 +2700,14 @@ __device__ __forceinline__ void acc_merge_lds(int kind, uint64_t* w, uint64_t v)
 }
 
 // Find or insert the group of key words w (table hash h) in an LDS table of S
-// slots (headers hdr, keys kw). Returns the slot, or -1 when the table is full.
+// slots (headers hdr, keys kw). Returns the slot, or -1 when the table is
+// full. A new group also counts itself in *groups; past `limit` groups it is
+// still completed (lanes may wait on its header) but -1 is returned: a table
+// stops at its fill limit, before linear probes over a nearly full table get
+// long.
 __device__ __forceinline__ int lds_group(uint64_t* hdr, uint64_t* kw, int S, int nkw,
-                                         const uint64_t (&w)[MAX_KEYS + 1], uint64_t h) {
+                                         const uint64_t (&w)[MAX_KEYS + 1], uint64_t h, uint32_t* groups,
+                                         uint32_t limit) {
   const uint64_t tag = (h & ~3ull) | 2ull;
   int pos = (int)(h & (uint64_t)(S - 1));
   int probes = 0;
@@ -2713,7 +2718,7 @@ __device__ __forceinline__ int lds_group(uint64_t* hdr, uint64_t* kw, int S, int
       if (prev == 0) {
         for (int k = 0; k < nkw; k++) kw[(uint64_t)pos * nkw + k] = w[k];
         __hip_atomic_store(&hdr[pos], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-        return pos;
+        return atomicAdd(groups, 1u) < limit ? pos : -1;
       }
       c = prev;
     }
@@ -2729,6 +2734,10 @@ __device__ __forceinline__ int lds_group(uint64_t* hdr, uint64_t* kw, int S, int
   }
   return -1;
 }
+
+// Groups an LDS table of S slots takes before it counts as full (7/8, the
+// fill limit of a pre-sized global table).
+__device__ __forceinline__ uint32_t lds_fill_limit(int S) { return (uint32_t)S - (uint32_t)S / 8u; }
 
 // LDS table layout: hdr[S], kw[S * nkw], acc[nacc * S] (u64 each)
 size_t agg_part_lds(const PartAggPlan& A) { return (size_t)A.S * (1 + A.nkw + A.nacc) * 8; }
@@ -2766,6 +2775,7 @@ __global__ void __launch_bounds__(256) agg_part_chunk_kernel(PartAggPlan A, cons
   extern __shared__ uint64_t lds[];
   __shared__ uint32_t table_full;  // the batch is aggregated again: stop early
   __shared__ uint32_t fail_at;     // the first record that found the table full
+  __shared__ uint32_t groups;      // groups in the table
   const PartChunk C = chunks[blockIdx.x];
   const int S = A.S;
   uint64_t* hdr = lds;
@@ -2774,6 +2784,7 @@ __global__ void __launch_bounds__(256) agg_part_chunk_kernel(PartAggPlan A, cons
   if (threadIdx.x == 0) {
     table_full = 0;
     fail_at = C.hi;
+    groups = 0;
   }
   part_init(A, hdr, acc);
   bool full = false;
@@ -2782,10 +2793,10 @@ __global__ void __launch_bounds__(256) agg_part_chunk_kernel(PartAggPlan A, cons
     uint64_t w[MAX_KEYS + 1];
 #pragma unroll
     for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < A.nkw ? rkeys[(uint64_t)k * n + e] : 0;
-    const int pos = lds_group(hdr, kw, S, A.nkw, w, key_hash(w, A.nkw));
+    const int pos = lds_group(hdr, kw, S, A.nkw, w, key_hash(w, A.nkw), &groups, lds_fill_limit(S));
     if (pos < 0) {
-      // every later record would scan all S slots too: the overflow reruns
-      // the batch, so nothing more of this chunk is needed
+      // the table reached its fill limit: the overflow reruns the batch, so
+      // nothing more of this chunk is needed
       full = true;
       __hip_atomic_store(&table_full, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       atomicMin(&fail_at, e);
@@ -2804,12 +2815,13 @@ __global__ void __launch_bounds__(256) agg_part_chunk_kernel(PartAggPlan A, cons
   if (__any(full) && __lane_id() == 0) atomicOr(A.flags + 2, 2u);
   __syncthreads();
   // partitions whose table filled up (a split partition is counted by its
-  // merge), each with its projected groups: S groups after the records before
-  // fail_at, scaled to all of the partition's records
+  // merge), each with its projected groups: the fill limit's groups after the
+  // records before fail_at, scaled to all of the partition's records
   if (threadIdx.x == 0 && table_full && C.partial == 0xFFFFFFFFu) {
     atomicAdd(A.flags + 3, 1u);
     const uint64_t seen = max(1u, fail_at - C.lo);
-    atomicAdd((unsigned long long*)(A.flags + 4), (unsigned long long)((uint64_t)S * (C.hi - C.lo) / seen));
+    atomicAdd((unsigned long long*)(A.flags + 4),
+              (unsigned long long)((uint64_t)lds_fill_limit(S) * (C.hi - C.lo) / seen));
   }
   if (C.partial == 0xFFFFFFFFu) {
     part_write(A, hdr, kw, acc, A.slots, A.acc_data, (uint64_t)C.part * S, true);
@@ -2825,11 +2837,15 @@ __global__ void __launch_bounds__(256) agg_part_merge_kernel(PartAggPlan A, cons
                                                              const uint32_t* first, const uint64_t* partial) {
   extern __shared__ uint64_t lds[];
   __shared__ uint32_t table_full;  // the batch is aggregated again: stop early
+  __shared__ uint32_t groups;      // groups in the table
   const int S = A.S;
   uint64_t* hdr = lds;
   uint64_t* kw = lds + S;
   uint64_t* acc = kw + (uint64_t)S * A.nkw;
-  if (threadIdx.x == 0) table_full = 0;
+  if (threadIdx.x == 0) {
+    table_full = 0;
+    groups = 0;
+  }
   part_init(A, hdr, acc);
   bool full = false;
   const size_t words = (size_t)S * (1 + A.nkw + A.nacc);
@@ -2843,7 +2859,7 @@ __global__ void __launch_bounds__(256) agg_part_merge_kernel(PartAggPlan A, cons
       uint64_t w[MAX_KEYS + 1];
 #pragma unroll
       for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < A.nkw ? tk[(uint64_t)i * A.nkw + k] : 0;
-      const int pos = lds_group(hdr, kw, S, A.nkw, w, key_hash(w, A.nkw));
+      const int pos = lds_group(hdr, kw, S, A.nkw, w, key_hash(w, A.nkw), &groups, lds_fill_limit(S));
       if (pos < 0) {
         full = true;
         __hip_atomic_store(&table_full, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
