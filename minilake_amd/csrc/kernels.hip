@@ -2656,7 +2656,12 @@ __device__ __forceinline__ void record_keys(const AggRecords& R, int64_t i, uint
   if (R.key_nullable) set_word(w, R.nkeys, nullmask);
 }
 
-__global__ void __launch_bounds__(256) agg_part_hist_kernel(AggRecords R, int64_t n, uint64_t mask, int part_shift,
+// Both passes run PART_PASS_THREADS threads per workgroup: a workgroup owns
+// one slice of the per-partition offsets, so its threads (not more
+// workgroups, which would multiply the histogram) carry the records in flight.
+constexpr int PART_PASS_THREADS = 1024;
+
+__global__ void __launch_bounds__(PART_PASS_THREADS) agg_part_hist_kernel(AggRecords R, int64_t n, uint64_t mask, int part_shift,
                                                             int64_t rows_per_block, int nparts, uint32_t* hist) {
   extern __shared__ uint32_t cnt[];
   for (int p = threadIdx.x; p < nparts; p += blockDim.x) cnt[p] = 0;
@@ -2671,7 +2676,7 @@ __global__ void __launch_bounds__(256) agg_part_hist_kernel(AggRecords R, int64_
   for (int p = threadIdx.x; p < nparts; p += blockDim.x) hist[(uint64_t)p * gridDim.x + blockIdx.x] = cnt[p];
 }
 
-__global__ void __launch_bounds__(256) agg_part_scatter_kernel(AggRecords R, int64_t n, uint64_t mask, int part_shift,
+__global__ void __launch_bounds__(PART_PASS_THREADS) agg_part_scatter_kernel(AggRecords R, int64_t n, uint64_t mask, int part_shift,
                                                                int64_t rows_per_block, int nparts,
                                                                const uint32_t* offsets, uint64_t* rkeys,
                                                                uint64_t* rargs, uint32_t* rvalid) {
@@ -3484,14 +3489,16 @@ void launch_dense_permute(const DensePermute& M, const uint64_t* keys, const uin
 
 void launch_agg_part_hist(const AggRecords& R, int64_t n, uint64_t mask, int part_shift, int nblocks,
                           int64_t rows_per_block, int nparts, uint32_t* hist, hipStream_t s) {
-  agg_part_hist_kernel<<<nblocks, 256, (size_t)nparts * 4, s>>>(R, n, mask, part_shift, rows_per_block, nparts, hist);
+  agg_part_hist_kernel<<<nblocks, PART_PASS_THREADS, (size_t)nparts * 4, s>>>(R, n, mask, part_shift, rows_per_block,
+                                                                              nparts, hist);
 }
 
 void launch_agg_part_scatter(const AggRecords& R, int64_t n, uint64_t mask, int part_shift, int nblocks,
                              int64_t rows_per_block, int nparts, const uint32_t* offsets, uint64_t* rkeys,
                              uint64_t* rargs, uint32_t* rvalid, hipStream_t s) {
-  agg_part_scatter_kernel<<<nblocks, 256, (size_t)nparts * 4, s>>>(R, n, mask, part_shift, rows_per_block, nparts,
-                                                                   offsets, rkeys, rargs, rvalid);
+  agg_part_scatter_kernel<<<nblocks, PART_PASS_THREADS, (size_t)nparts * 4, s>>>(R, n, mask, part_shift,
+                                                                                 rows_per_block, nparts, offsets,
+                                                                                 rkeys, rargs, rvalid);
 }
 
 void launch_agg_part_chunks(const PartAggPlan& A, const PartChunk* chunks, int nchunks, const uint64_t* rkeys,
