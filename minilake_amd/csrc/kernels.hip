@@ -2838,44 +2838,64 @@ __global__ void __launch_bounds__(256) agg_part_chunk_kernel(PartAggPlan A, cons
 
 // One workgroup per partition that was split into several chunks: merge the
 // chunks' partial tables first[j] .. first[j+1]-1 and write the partition.
-__global__ void __launch_bounds__(256) agg_part_merge_kernel(PartAggPlan A, const uint32_t* parts,
-                                                             const uint32_t* first, const uint64_t* partial) {
+// The (partial, slot) pairs are spread over all threads with no barrier
+// between partials (inserts and merges into the LDS table are atomic), and
+// each thread requests the headers and keys of MERGE_U slots before using
+// any: a partition of a hot key can have ~100 partials, and reading them one
+// dependent load at a time left one workgroup running long after the rest.
+constexpr int MERGE_THREADS = 1024;
+constexpr int MERGE_U = 4;
+__global__ void __launch_bounds__(MERGE_THREADS) agg_part_merge_kernel(PartAggPlan A, const uint32_t* parts,
+                                                                       const uint32_t* first, const uint64_t* partial) {
   extern __shared__ uint64_t lds[];
   __shared__ uint32_t table_full;  // the batch is aggregated again: stop early
   __shared__ uint32_t groups;      // groups in the table
-  const int S = A.S;
+  const int S = A.S, nkw = A.nkw;
   uint64_t* hdr = lds;
   uint64_t* kw = lds + S;
-  uint64_t* acc = kw + (uint64_t)S * A.nkw;
+  uint64_t* acc = kw + (uint64_t)S * nkw;
   if (threadIdx.x == 0) {
     table_full = 0;
     groups = 0;
   }
   part_init(A, hdr, acc);
   bool full = false;
-  const size_t words = (size_t)S * (1 + A.nkw + A.nacc);
-  for (uint32_t q = first[blockIdx.x]; q < first[blockIdx.x + 1]; q++) {
-    const uint64_t* t = partial + (uint64_t)q * words;
-    const uint64_t* tk = t + S;
-    const uint64_t* ta = tk + (uint64_t)S * A.nkw;
-    for (int i = threadIdx.x; i < S; i += blockDim.x) {
-      if (__hip_atomic_load(&table_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-      if (t[i] == 0) continue;
-      uint64_t w[MAX_KEYS + 1];
+  const size_t words = (size_t)S * (1 + nkw + A.nacc);
+  const uint64_t q0 = first[blockIdx.x];
+  const uint64_t total = (uint64_t)(first[blockIdx.x + 1] - q0) * (uint64_t)S;
+  const int log_s = __builtin_ctz((unsigned)S);
+  for (uint64_t base = threadIdx.x; base < total; base += (uint64_t)MERGE_U * blockDim.x) {
+    if (__hip_atomic_load(&table_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+    const uint64_t* t[MERGE_U];
+    int slot[MERGE_U];
+    uint64_t h[MERGE_U];
+    uint64_t w[MERGE_U][MAX_KEYS + 1];
 #pragma unroll
-      for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < A.nkw ? tk[(uint64_t)i * A.nkw + k] : 0;
-      const int pos = lds_group(hdr, kw, S, A.nkw, w, key_hash(w, A.nkw), &groups, lds_fill_limit(S));
+    for (int u = 0; u < MERGE_U; u++) {
+      const uint64_t e = base + (uint64_t)u * blockDim.x;
+      const bool in = e < total;
+      const uint64_t ee = in ? e : base;
+      t[u] = partial + (q0 + (ee >> log_s)) * words;
+      slot[u] = (int)(ee & (uint64_t)(S - 1));
+      h[u] = in ? t[u][slot[u]] : 0;
+#pragma unroll
+      for (int k = 0; k < MAX_KEYS + 1; k++) w[u][k] = k < nkw ? t[u][S + (uint64_t)slot[u] * nkw + k] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < MERGE_U; u++) {
+      if (h[u] == 0 || full) continue;
+      const int pos = lds_group(hdr, kw, S, nkw, w[u], key_hash(w[u], nkw), &groups, lds_fill_limit(S));
       if (pos < 0) {
         full = true;
         __hip_atomic_store(&table_full, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        break;
+        continue;
       }
+      const uint64_t* ta = t[u] + S + (uint64_t)S * nkw;
       for (int s2 = 0; s2 < A.nacc; s2++) {
-        const uint64_t v = ta[(uint64_t)s2 * S + i];
+        const uint64_t v = ta[(uint64_t)s2 * S + slot[u]];
         if (v != acc_init_bits(A.acc[s2].kind)) acc_merge_lds(A.acc[s2].kind, acc + (uint64_t)s2 * S + pos, v);
       }
     }
-    __syncthreads();
   }
   if (__any(full) && __lane_id() == 0) atomicOr(A.flags + 2, 2u);
   __syncthreads();
@@ -3510,7 +3530,7 @@ void launch_agg_part_chunks(const PartAggPlan& A, const PartChunk* chunks, int n
 void launch_agg_part_merge(const PartAggPlan& A, const uint32_t* parts, const uint32_t* first, int nparts,
                            const uint64_t* partial, hipStream_t s) {
   if (nparts <= 0) return;
-  agg_part_merge_kernel<<<nparts, 256, agg_part_lds(A), s>>>(A, parts, first, partial);
+  agg_part_merge_kernel<<<nparts, MERGE_THREADS, agg_part_lds(A), s>>>(A, parts, first, partial);
 }
 
 int64_t scan_u32_blocks(int64_t n) { return (n + SCAN_B - 1) / SCAN_B; }
