@@ -3366,8 +3366,90 @@ int flow_blocks_per_cu(int sink, bool fast, int block, size_t lds) {
   return fast ? flow_blocks_mode<FLOW_FAST>(sink, block, lds) : flow_blocks_mode<FLOW_GENERIC>(sink, block, lds);
 }
 
+// Exclusive scan of a workgroup's u64 values (SCAN_T threads); *total = the sum.
+__device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* red, uint64_t* total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint64_t incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) red[wv] = incl;
+  __syncthreads();
+  uint64_t base = 0, all = 0;
+  for (int w = 0; w < SCAN_T / 64; w++) {
+    if (w < wv) base += red[w];
+    all += red[w];
+  }
+  *total = all;
+  __syncthreads();
+  return base + incl - v;
+}
+
+// Large count scans in three passes over SCAN_B-count blocks, with no buffer
+// besides the output: pass 1 parks block b's sum in offsets[b * SCAN_B], pass
+// 2 (one workgroup) turns those into the blocks' exclusive bases and writes
+// the total at offsets[n], pass 3 writes every block's offsets from its base.
+__global__ void __launch_bounds__(SCAN_T) count_scan_reduce_kernel(const uint32_t* counts, int64_t n,
+                                                                   uint64_t* offsets) {
+  __shared__ uint64_t red[SCAN_T / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_B + (int64_t)threadIdx.x * SCAN_E;
+  uint64_t c = 0;
+#pragma unroll
+  for (int e = 0; e < SCAN_E; e++) c += (base + e < n) ? counts[base + e] : 0u;
+  uint64_t total;
+  block_excl_scan64(c, red, &total);
+  if (threadIdx.x == 0) offsets[(int64_t)blockIdx.x * SCAN_B] = total;
+}
+
+__global__ void __launch_bounds__(SCAN_T) count_scan_top_kernel(uint64_t* offsets, int64_t n, int64_t nb) {
+  __shared__ uint64_t red[SCAN_T / 64];
+  uint64_t carry = 0;
+  for (int64_t b0 = 0; b0 < nb; b0 += SCAN_T) {
+    const int64_t b = b0 + threadIdx.x;
+    const uint64_t v = b < nb ? offsets[b * SCAN_B] : 0;
+    uint64_t total;
+    const uint64_t ex = block_excl_scan64(v, red, &total);
+    if (b < nb) offsets[b * SCAN_B] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) offsets[n] = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_T) count_scan_down_kernel(const uint32_t* counts, int64_t n,
+                                                                 uint64_t* offsets) {
+  __shared__ uint64_t red[SCAN_T / 64];
+  __shared__ uint64_t block_base;
+  const int64_t first = (int64_t)blockIdx.x * SCAN_B;
+  if (threadIdx.x == 0) block_base = offsets[first];
+  const int64_t base = first + (int64_t)threadIdx.x * SCAN_E;
+  uint32_t v[SCAN_E];
+  uint64_t c = 0;
+#pragma unroll
+  for (int e = 0; e < SCAN_E; e++) {
+    v[e] = (base + e < n) ? counts[base + e] : 0u;
+    c += v[e];
+  }
+  uint64_t total;
+  const uint64_t ex = block_excl_scan64(c, red, &total);  // its barriers order block_base's read first
+  uint64_t run = block_base + ex;
+#pragma unroll
+  for (int e = 0; e < SCAN_E; e++) {
+    if (base + e < n) offsets[base + e] = run;
+    run += v[e];
+  }
+}
+
 void launch_scan_counts(const uint32_t* counts, int64_t n, uint64_t* offsets, hipStream_t s) {
-  scan_counts_kernel<<<1, 1024, 0, s>>>(counts, n, offsets);
+  if (n <= 4 * SCAN_B) {  // a few thousand wave counts: one workgroup
+    scan_counts_kernel<<<1, 1024, 0, s>>>(counts, n, offsets);
+    return;
+  }
+  const int64_t nb = (n + SCAN_B - 1) / SCAN_B;
+  count_scan_reduce_kernel<<<(unsigned)nb, SCAN_T, 0, s>>>(counts, n, offsets);
+  count_scan_top_kernel<<<1, SCAN_T, 0, s>>>(offsets, n, nb);
+  count_scan_down_kernel<<<(unsigned)nb, SCAN_T, 0, s>>>(counts, n, offsets);
 }
 
 void launch_materialize(const Plan& P, int sink, const uint64_t* offsets, int64_t nwaves, hipStream_t s) {
