@@ -1478,12 +1478,13 @@ static bool agg_partitioned(ml_hashagg* A, PipelineBuild& B) {
     // An LDS table filled up: everything from the partitioning on again, from
     // an empty table sized for the groups the attempt saw (the partitions that
     // held, exactly; a filled one projected from how far into its records it
-    // got) with the initial 2x margin, and at least twice the capacity. More
-    // groups than the partitions can hold: the plain path.
+    // got, an overestimate when new keys thin out as keys repeat) with the
+    // initial 2x margin, and at least twice the capacity; at most the largest
+    // partitioned table, and the plain path once that one has filled up.
     const uint64_t over = std::min<uint64_t>(f[1] >> 32, nparts);
     const uint64_t seen = f[0] - std::min<uint64_t>(f[0], over * (S - S / 8)) + f[2];
-    const uint64_t next = std::max<uint64_t>(cap * 2, next_pow2(2 * seen + 1));
-    if (!fits(next)) break;
+    uint64_t next = std::max<uint64_t>(cap * 2, next_pow2(2 * seen + 1));
+    while (next > cap * 2 && !fits(next)) next >>= 1;
     zero(ctx, A->flags.ptr, 32);
     cap = next;
     A->resizes++;
