@@ -1482,7 +1482,7 @@ static bool agg_partitioned(ml_hashagg* A, PipelineBuild& B) {
     // initial 2x margin, and at least twice the capacity; at most the largest
     // partitioned table, and the plain path once that one has filled up.
     const uint64_t over = std::min<uint64_t>(f[1] >> 32, nparts);
-    const uint64_t seen = f[0] - std::min<uint64_t>(f[0], over * (S - S / 8)) + f[2];
+    const uint64_t seen = f[0] - std::min<uint64_t>(f[0], over * part_fill_limit((uint32_t)S)) + f[2];
     uint64_t next = std::max<uint64_t>(cap * 2, next_pow2(2 * seen + 1));
     while (next > cap * 2 && !fits(next)) next >>= 1;
     zero(ctx, A->flags.ptr, 32);

@@ -183,6 +183,11 @@ struct PartAggPlan {
                               // [4..5] projected groups of the filled partitions (u64)
 };
 
+// Groups an LDS partition table of S slots takes before it counts as full:
+// 3/4 (linear probes for a new key stay short, and a table sized with the
+// 2x margin sits near 1/2 full).
+__host__ __device__ constexpr uint32_t part_fill_limit(uint32_t S) { return S - S / 4u; }
+
 struct PartChunk {
   uint32_t part, lo, hi, partial;  // partial = index of its partial table, or ~0 (final)
 };

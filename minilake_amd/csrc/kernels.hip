@@ -2740,9 +2740,6 @@ __device__ __forceinline__ int lds_group(uint64_t* hdr, uint64_t* kw, int S, int
   return -1;
 }
 
-// Groups an LDS table of S slots takes before it counts as full (7/8, the
-// fill limit of a pre-sized global table).
-__device__ __forceinline__ uint32_t lds_fill_limit(int S) { return (uint32_t)S - (uint32_t)S / 8u; }
 
 // LDS table layout: hdr[S], kw[S * nkw], acc[nacc * S] (u64 each)
 size_t agg_part_lds(const PartAggPlan& A) { return (size_t)A.S * (1 + A.nkw + A.nacc) * 8; }
@@ -2798,7 +2795,7 @@ __global__ void __launch_bounds__(256) agg_part_chunk_kernel(PartAggPlan A, cons
     uint64_t w[MAX_KEYS + 1];
 #pragma unroll
     for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = k < A.nkw ? rkeys[(uint64_t)k * n + e] : 0;
-    const int pos = lds_group(hdr, kw, S, A.nkw, w, key_hash(w, A.nkw), &groups, lds_fill_limit(S));
+    const int pos = lds_group(hdr, kw, S, A.nkw, w, key_hash(w, A.nkw), &groups, part_fill_limit((uint32_t)S));
     if (pos < 0) {
       // the table reached its fill limit: the overflow reruns the batch, so
       // nothing more of this chunk is needed
@@ -2826,7 +2823,7 @@ __global__ void __launch_bounds__(256) agg_part_chunk_kernel(PartAggPlan A, cons
     atomicAdd(A.flags + 3, 1u);
     const uint64_t seen = max(1u, fail_at - C.lo);
     atomicAdd((unsigned long long*)(A.flags + 4),
-              (unsigned long long)((uint64_t)lds_fill_limit(S) * (C.hi - C.lo) / seen));
+              (unsigned long long)((uint64_t)part_fill_limit((uint32_t)S) * (C.hi - C.lo) / seen));
   }
   if (C.partial == 0xFFFFFFFFu) {
     part_write(A, hdr, kw, acc, A.slots, A.acc_data, (uint64_t)C.part * S, true);
@@ -2884,7 +2881,7 @@ __global__ void __launch_bounds__(MERGE_THREADS) agg_part_merge_kernel(PartAggPl
 #pragma unroll
     for (int u = 0; u < MERGE_U; u++) {
       if (h[u] == 0 || full) continue;
-      const int pos = lds_group(hdr, kw, S, nkw, w[u], key_hash(w[u], nkw), &groups, lds_fill_limit(S));
+      const int pos = lds_group(hdr, kw, S, nkw, w[u], key_hash(w[u], nkw), &groups, part_fill_limit((uint32_t)S));
       if (pos < 0) {
         full = true;
         __hip_atomic_store(&table_full, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
