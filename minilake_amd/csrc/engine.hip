@@ -2576,7 +2576,8 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
   if (T.col_valid.size() < T.cols.size()) T.col_valid.resize(T.cols.size());
   T.dev.nkw = (int)nk;
   // Dense layout: a first key column of integers whose values span at most
-  // max(8n, 2^20) consecutive integers, all distinct. The rows move into
+  // max(8n, 2^24) consecutive integers, all distinct (at the floor the
+  // presence bits are 2 MB, held in L2 like a Bloom filter). The rows move into
   // first-key order; a probe reads one presence bit and, for a present key,
   // the rank of its 64-key word: no hashing, probing or row index array.
   // With several key columns the other key words are compared per row, and a
@@ -2586,7 +2587,7 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
     if (!have_range) key_range(keys.as<uint64_t>(), key_null.as<uint8_t>(), n, nullptr);
     if (hm[0] <= hm[1]) {
       const unsigned __int128 range = (unsigned __int128)((__int128)hm[1] - (__int128)hm[0]) + 1;
-      const unsigned __int128 limit = std::max<uint64_t>((uint64_t)n * 8, 1ull << 20);
+      const unsigned __int128 limit = std::max<uint64_t>((uint64_t)n * 8, 1ull << 24);
       if (range <= limit && range < ((unsigned __int128)1 << 31)) {
         const uint64_t R = (uint64_t)range;
         const int64_t ne = (int64_t)((R + 63) / 64);
