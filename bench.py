@@ -263,8 +263,8 @@ def bench_sizing_ablation(eng, torch, sizing, mlp, rows):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scale", type=float, default=10.0, help="TPC-H-like scale; 10 = 60M lineitem rows")
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-scale", type=float, default=1.0)
