@@ -349,6 +349,10 @@ struct JoinTableImpl {
   // first use (probe pairs, column views), with compacted column copies.
   bool dense_offset = false;
   int64_t offset_rows = 0;
+  // A full or offset-indexed table's build column that is its first key
+  // (the same program): not stored, since row p holds key min + p. Probes
+  // read the probe key in its place; the views rebuild it on first access.
+  int implied_col = -1;
   Buffer offset_rank;
   std::vector<Buffer> compact_cols, compact_valid;
   DevJoinTable dev{};

@@ -275,8 +275,10 @@ void PipelineBuild::add_ops(const ml_operator* ops, int n) {
         st.stage = stage;
         st.table = ntables;
         st.nkeys = op.num_keys;
+        NodePtr key0;
         for (int k = 0; k < op.num_keys; k++) {
           NodePtr kn = resolve(op.keys[k]);
+          if (k == 0) key0 = kn;
           if (J.key_types[k] == ML_TYPE_UTF8) {
             const VCol* c = kn->kind == ML_EXPR_COLUMN ? kn->col : nullptr;
             if (!c || !same_dict(c->dict, J.key_dicts[k]))
@@ -299,6 +301,21 @@ void PipelineBuild::add_ops(const ml_operator* ops, int n) {
           v.data = J.cols[c].ptr;
           v.valid = J.col_valid[c].ptr ? J.col_valid[c].as<uint8_t>() : nullptr;
           v.dict = J.col_dicts[c];
+          if ((int)c == J.implied_col) {
+            // the table's first-key column, not stored: a matched row's value
+            // is the probe key (equal key words), in the column's type
+            NodePtr e = key0;
+            if (comp.type_of(key0) != v.type) {
+              auto cast = std::make_shared<Node>();
+              cast->kind = ML_EXPR_CAST;
+              cast->type = v.type;
+              cast->left = key0;
+              e = cast;
+            }
+            v.data = nullptr;
+            v.valid = nullptr;
+            v.expr = e;
+          }
           build.push_back(add_col(std::move(v)));
         }
         // output layout: build ++ probe or probe ++ build (join.rs:331-335)
@@ -2069,6 +2086,7 @@ struct ml_join_build {
   std::vector<std::shared_ptr<mlg::HostDict>> col_dicts;
   std::vector<std::shared_ptr<mlg::HostDict>> key_dicts;
   std::vector<bool> col_nullable;
+  int implied_col = -2;  // the build column equal to the first key (-1 none, -2 no batch yet)
   struct Chunk {
     std::vector<mlg::Buffer> cols, valid;
     mlg::Buffer keys, key_null;
@@ -2149,6 +2167,21 @@ static void join_build_run(ml_join_build* J, PipelineBuild& B) {
   if (J->col_nullable.empty()) J->col_nullable = nullable;
   else
     for (size_t c = 0; c < nullable.size(); c++) J->col_nullable[c] = J->col_nullable[c] || nullable[c];
+  // a column computed by exactly the first key's program (the key column
+  // itself): a dense table held at key offsets need not store it
+  int implied = -1;
+  {
+    const int kt = J->key_types[0];
+    const Prog& kp = B.P.keys[0];
+    if (kt == ML_TYPE_INT32 || kt == ML_TYPE_INT64 || kt == ML_TYPE_DATE)
+      for (size_t c = 0; c < B.layout.size() && implied < 0; c++) {
+        const Prog& cp = B.P.cols[c];
+        if (B.layout[c]->type == kt && cp.count == kp.count && cp.nullable == kp.nullable && kp.count > 0 &&
+            std::memcmp(&B.ins[(size_t)cp.first], &B.ins[(size_t)kp.first], sizeof(Instr) * (size_t)kp.count) == 0)
+          implied = (int)c;
+      }
+  }
+  J->implied_col = J->implied_col == -2 || J->implied_col == implied ? implied : -1;
   ch.keys = Buffer(ctx->arena, std::max<uint64_t>(cap, 1) * J->key_types.size() * 8);
   ch.key_null = Buffer(ctx->arena, std::max<uint64_t>(cap, 1));
   B.P.key_out = ch.keys.as<uint64_t>();
@@ -2358,6 +2391,7 @@ void join_table_resolve(ml_join_table* t) {
     T.bloom = Buffer();
     T.dense_offset = false;
     T.offset_rows = 0;
+    T.implied_col = -1;
     T.dev.dense = 0;
     T.dev.dense_full = 0;
     T.dev.dense_keys8 = 0;
@@ -2606,9 +2640,11 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
         const uint64_t offset_rows = R + (uint64_t)nnull;
         uint64_t row_bytes = (nk - 1) * key_word_bytes + (nk == 2 ? 1 : 0);
         for (size_t c = 0; c < T.cols.size(); c++)
-          row_bytes += (uint64_t)type_width(T.col_types[c]) + (T.col_valid[c].ptr ? 1 : 0);
+          if ((int)c != J->implied_col)  // not stored when the rows sit at key offsets
+            row_bytes += (uint64_t)type_width(T.col_types[c]) + (T.col_valid[c].ptr ? 1 : 0);
         const bool offset = !full && (offset_rows <= 4 * (uint64_t)n || offset_rows * row_bytes <= (256ull << 20));
         const int64_t held = (full || offset) ? (int64_t)offset_rows : n;  // full: R + NULL-key rows = n
+        const int implied = (full || offset) ? J->implied_col : -1;
         DensePermute M{};
         Buffer dkeys, dkeys8;
         std::vector<Buffer> cols, valid;
@@ -2625,10 +2661,17 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
           M.ncols = (int32_t)T.cols.size();
           for (size_t c = 0; c < T.cols.size(); c++) {
             const int w = type_width(T.col_types[c]);
+            M.width[c] = w;
+            if ((int)c == implied) {  // not written (dst null): row p holds key min + p
+              cols.emplace_back();
+              valid.emplace_back();
+              M.src[c] = M.dst[c] = nullptr;
+              M.src_valid[c] = M.dst_valid[c] = nullptr;
+              continue;
+            }
             cols.emplace_back(ctx->arena, (size_t)std::max<int64_t>(held, 1) * w);
             valid.emplace_back();
             if (T.col_valid[c].ptr) valid.back() = Buffer(ctx->arena, (size_t)std::max<int64_t>(held, 1));
-            M.width[c] = w;
             M.src[c] = T.cols[c].ptr;
             M.dst[c] = cols.back().ptr;
             M.src_valid[c] = T.col_valid[c].ptr ? T.col_valid[c].as<uint8_t>() : nullptr;
@@ -2694,6 +2737,7 @@ static ml_join_table* join_finalize_finish(ml_join_build* J, FinalizePrep& P) {
           T.dev.unique = 1;
           T.dense_offset = offset;
           T.offset_rows = offset ? (int64_t)offset_rows : 0;
+          T.implied_col = implied;
           T.dense_keys8 = std::move(dkeys8);
           if (!deferred) use_keys8(T, hf);
         }
@@ -3516,6 +3560,18 @@ static const uint32_t* offset_rank(ml_ctx* ctx, JoinTableImpl& T) {
 }
 
 // Columns of an offset-indexed table in key order (the ABI's row numbering), made once.
+// The first-key column of a table that does not store it, rebuilt at its
+// row positions (p < range: min + p; NULL-key rows after: 0) for the views.
+static void materialize_implied(ml_ctx* ctx, JoinTableImpl& T) {
+  if (T.implied_col < 0 || T.cols[(size_t)T.implied_col].ptr) return;
+  const size_t c = (size_t)T.implied_col;
+  const int w = type_width(T.col_types[c]);
+  const int64_t held = T.dense_offset ? T.offset_rows : T.nrows;  // a full table holds nrows = range + NULL rows
+  T.cols[c] = Buffer(ctx->arena, (size_t)std::max<int64_t>(held, 1) * w);
+  launch_dense_key_fill(T.cols[c].ptr, w, held, T.dev.dense_range, T.dev.dense_min, ctx->stream);
+  refresh_views(T);
+}
+
 static void compact_offset_table(ml_ctx* ctx, JoinTableImpl& T) {
   if (!T.dense_offset || !T.compact_cols.empty()) return;
   const uint32_t* rank = offset_rank(ctx, T);
@@ -3536,8 +3592,11 @@ int32_t ml_join_table_unique(const ml_join_table* t) { return resolved(t) ? t->i
 int32_t ml_join_table_num_columns(const ml_join_table* t) { return t ? (int32_t)t->impl.col_types.size() : 0; }
 const ml_column* ml_join_table_column(const ml_join_table* t, int32_t i) {
   if (!resolved(t) || i < 0 || (size_t)i >= t->impl.views.size()) return nullptr;
-  if (t->impl.dense_offset &&
-      guarded(t->ctx, [&] { compact_offset_table(t->ctx, const_cast<ml_join_table*>(t)->impl); }) != ML_OK)
+  if (guarded(t->ctx, [&] {
+        JoinTableImpl& T = const_cast<ml_join_table*>(t)->impl;
+        materialize_implied(t->ctx, T);
+        compact_offset_table(t->ctx, T);
+      }) != ML_OK)
     return nullptr;
   // the build's last kernels may still be queued: the columns are complete
   // when this returns, so they can be read from any stream

@@ -137,6 +137,9 @@ struct DensePermute {
 // Rows into key order; rank and present null for a table whose rows sit at
 // their key's offset (a full table, where every key of the range is present,
 // or an offset-indexed one), the NULL-key rows from `range` on.
+// A first-key column rebuilt at row positions: dst[p] = kmin + p below
+// `range`, 0 for the NULL-key rows after (width 4 or 8).
+void launch_dense_key_fill(void* dst, int width, int64_t held, uint64_t range, int64_t kmin, hipStream_t s);
 // Presence bits and rows of a table held at key offsets in one pass (M.held
 // positions; flag bits as launch_dense_mark; *null_next from 0).
 void launch_dense_place(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin,

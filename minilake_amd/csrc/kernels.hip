@@ -2037,6 +2037,7 @@ __global__ void dense_mark_kernel(const uint64_t* keys, int nkw, const uint8_t* 
 
 // Build row r's key words 1.. and columns, written at table position pos.
 __device__ __forceinline__ void dense_write_row(const DensePermute& M, const uint64_t* keys, int64_t r, uint64_t pos) {
+  // (a column with dst null is not stored: the table's first-key column)
   for (int k = 1; k < M.nkw; k++) {
     const uint64_t at = pos * (M.nkw - 1) + k - 1;
     if (M.keys32) ((uint32_t*)M.dst_keys)[at] = (uint32_t)keys[r * M.nkw + k];
@@ -2044,6 +2045,7 @@ __device__ __forceinline__ void dense_write_row(const DensePermute& M, const uin
   }
   if (M.dst_keys8) M.dst_keys8[pos] = (uint8_t)keys[r * 2 + 1];
   for (int c = 0; c < M.ncols; c++) {
+    if (!M.dst[c]) continue;
     const int w = M.width[c];
     if (w == 8) ((uint64_t*)M.dst[c])[pos] = ((const uint64_t*)M.src[c])[r];
     else if (w == 4) ((uint32_t*)M.dst[c])[pos] = ((const uint32_t*)M.src[c])[r];
@@ -3565,6 +3567,20 @@ void launch_dense_compact(const void* src, int width, const uint8_t* src_valid, 
 
 void launch_dense_rank(const uint64_t* bits, int64_t nw, uint32_t* rank, uint32_t* block_sums, hipStream_t s) {
   scan_launch(bits, 1, nw, rank, block_sums, s);
+}
+
+__global__ void dense_key_fill_kernel(uint8_t* dst, int width, int64_t held, uint64_t range, int64_t kmin) {
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < held; p += step) {
+    const int64_t v = (uint64_t)p < range ? kmin + p : 0;
+    if (width == 8) ((int64_t*)dst)[p] = v;
+    else ((int32_t*)dst)[p] = (int32_t)v;
+  }
+}
+
+void launch_dense_key_fill(void* dst, int width, int64_t held, uint64_t range, int64_t kmin, hipStream_t s) {
+  if (held > 0)
+    dense_key_fill_kernel<<<grid_for(held, 256, 8192), 256, 0, s>>>((uint8_t*)dst, width, held, range, kmin);
 }
 
 void launch_dense_place(const DensePermute& M, const uint64_t* keys, const uint8_t* key_null, int64_t n, int64_t kmin,
