@@ -581,15 +581,15 @@ __device__ __forceinline__ bool probe_keys(const Plan& P, const Code& ins, const
 #pragma unroll
   for (int k = 0; k < MAX_KEYS + 1; k++) w[k] = 0;
   if (uni(st.nkeys) == 2 && uni(st.keys[0].count) == 1 && uni(st.keys[1].count) == 1 && !pf.c0) {
-    // two keys, each a plain 8-byte column load: both loads issued before either is used
+    // two keys, each a plain column load (any width, as the single-load path
+    // of eval): both loads issued before either is used
     CInstr& i0 = ins.at(st.keys[0].first);
     CInstr& i1 = ins.at(st.keys[1].first);
     const uint64_t h0 = *(CU64*)&i0, h1 = *(CU64*)&i1;
-    if ((h0 & 0xFFFFu) == OP_LOAD_I64 && (h1 & 0xFFFFu) == OP_LOAD_I64) {
+    const uint32_t op0 = (uint32_t)h0 & 0xFFFFu, op1 = (uint32_t)h1 & 0xFFFFu;
+    if (op0 < OP_CONST && op1 < OP_CONST) {
       const uint64_t r0 = row_of(rows, (int)((h0 >> 24) & 0xFFu)), r1 = row_of(rows, (int)((h1 >> 24) & 0xFFu));
-      const auto c0 = gp<uint64_t>((const void*)i0.imm);
-      const auto c1 = gp<uint64_t>((const void*)i1.imm);
-      const uint64_t a = c0[r0], b = c1[r1];
+      const uint64_t a = load_col(op0, i0.imm, r0), b = load_col(op1, i1.imm, r1);
       const uint64_t v0 = uni(st.keys[0].nullable) ? i0.imm2 : 0, v1 = uni(st.keys[1].nullable) ? i1.imm2 : 0;
       const bool ok0 = !v0 || gp<uint8_t>((const void*)v0)[r0] != 0;
       const bool ok1 = !v1 || gp<uint8_t>((const void*)v1)[r1] != 0;
