@@ -1219,6 +1219,17 @@ static void compile_agg(ml_hashagg* A, PipelineBuild& B, bool first) {
     }
     B.P.acc[s].arg = found;
   }
+  // an argument whose code begins with all of the previous argument's code
+  // (price * (1 - disc) * (1 + tax) after price * (1 - disc)) is evaluated
+  // from that argument's values by the 4-row sinks
+  B.P.arg_chain = 0;
+  for (int a = 1; a < B.P.nargs; a++) {
+    const Prog& pr = B.P.args[a - 1];
+    const Prog& cu = B.P.args[a];
+    if (cu.count > pr.count && pr.count > 0 && (cu.nullable || !pr.nullable) &&
+        std::memcmp(&B.ins[(size_t)cu.first], &B.ins[(size_t)pr.first], sizeof(Instr) * (size_t)pr.count) == 0)
+      B.P.arg_chain |= 1u << a;
+  }
   // stack slots of the key and argument programs (push instructions write
   // slot depth, the others consume from the top)
   auto slots_of = [&](const Prog& pg) {

@@ -392,15 +392,23 @@ __device__ __forceinline__ u64x4 staged(const Staged8& T, uint32_t j) {
 
 // NUL: the program can produce NULL (Prog::nullable); without it every value
 // is valid, so no validity is loaded or tracked.
+// start > 0: the first `start` instructions are those of a program already
+// evaluated, whose result (val, valid) becomes stack slot 0.
 template <bool NUL>
-__device__ __forceinline__ void eval_src4_t(const Code& ins, Prog p, const uint32_t (&src)[4], const Staged8& T,
-                                            uint64_t (&val)[4], bool (&valid)[4]) {
+__device__ __forceinline__ void eval_src4_t(const Code& ins, Prog p, int start, const uint32_t (&src)[4],
+                                            const Staged8& T, uint64_t (&val)[4], bool (&valid)[4]) {
   Stack4 S;
   S.s0 = S.s1 = S.s2 = S.s3 = (u64x4)(0);
   S.ok = (u32x4)(0xFFFFFFFFu);
+  if (start > 0) {
+    S.s0 = (u64x4){val[0], val[1], val[2], val[3]};
+    if (NUL)
+      S.ok = (u32x4){valid[0] ? 0xFFFFFFFFu : 0xFFFFFFFEu, valid[1] ? 0xFFFFFFFFu : 0xFFFFFFFEu,
+                     valid[2] ? 0xFFFFFFFFu : 0xFFFFFFFEu, valid[3] ? 0xFFFFFFFFu : 0xFFFFFFFEu};
+  }
   const int first = uni(p.first), count = uni(p.count);
 #pragma unroll 1
-  for (int pc = 0; pc < count; pc++) {
+  for (int pc = start; pc < count; pc++) {
     CInstr& in = ins.at(first + pc);
     const uint64_t head = *(CU64*)&in;
     const uint32_t op = (uint32_t)head & 0xFFFFu;
@@ -502,9 +510,9 @@ __device__ __forceinline__ void eval_src4_t(const Code& ins, Prog p, const uint3
 }
 
 __device__ __forceinline__ void eval_src4(const Code& ins, Prog p, const uint32_t (&src)[4], const Staged8& T,
-                                          uint64_t (&val)[4], bool (&valid)[4]) {
-  if (uni(p.nullable)) eval_src4_t<true>(ins, p, src, T, val, valid);
-  else eval_src4_t<false>(ins, p, src, T, val, valid);
+                                          uint64_t (&val)[4], bool (&valid)[4], int start = 0) {
+  if (uni(p.nullable)) eval_src4_t<true>(ins, p, start, src, T, val, valid);
+  else eval_src4_t<false>(ins, p, start, src, T, val, valid);
 }
 
 __device__ __forceinline__ void set_word(uint64_t (&w)[MAX_KEYS + 1], int k, uint64_t v) {
@@ -981,7 +989,10 @@ __device__ __forceinline__ void sink_direct_agg_src(const Plan& P, const Code& i
   };
 #pragma unroll 1
   for (int a = 0; a < P.nargs; a++) {
-    eval_src4(ins, P.args[a], src, T, v, ok);
+    // an argument that begins with the previous one's code continues from
+    // its values, still in v / ok
+    const int from = ((uni(P.arg_chain) >> a) & 1u) ? uni(P.args[a - 1].count) : 0;
+    eval_src4(ins, P.args[a], src, T, v, ok, from);
     apply(uni(P.arg_slots[a]), false);
   }
   apply(uni(P.count_slots), true);

@@ -306,6 +306,8 @@ struct Plan {
   Prog args[MAX_AGG_SLOTS];  // argument programs
   uint32_t arg_slots[MAX_AGG_SLOTS];  // bit s of arg_slots[a]: slot s accumulates argument a
   uint32_t count_slots;      // bit s: slot s counts rows (no argument)
+  uint32_t arg_chain;        // bit a: args[a] begins with all of args[a-1]'s code, and a
+                             // 4-row evaluation continues from that result
   int32_t src_batch;         // 1: every key and argument program needs at most 4 stack
                              // slots, so a level-0 sink may evaluate several rows at once
   // stage-0 columns a batched level-0 sink reads, loaded for the round's rows
