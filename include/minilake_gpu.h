@@ -45,8 +45,8 @@ enum {
   ML_ERR_CANCELLED = 8,
   ML_ERR_INTERNAL = 9,
   /* Not an error of the reference: a pre-sized hash table was too small.
-   * Raised only by the learned-sizing path, which then falls back to the
-   * plain resize-on-overflow path. */
+   * Returned by ML_SIZING_HINT; the learned-sizing path grows or falls back
+   * instead (see ML_SIZING_LEARNED). */
   ML_ERR_CAPACITY = 10
 };
 
@@ -348,7 +348,10 @@ enum { /* how the group table is sized */
   ML_SIZING_HINT = 1,    /* pre-size from capacity_hint (room for at least that many
                             groups); more groups than that room -> ML_ERR_CAPACITY */
   ML_SIZING_LEARNED = 2  /* pre-size from the learned estimator; more groups than the
-                            room it gave -> fall back to ML_SIZING_PLAIN automatically */
+                            room it gave -> a partitioned table is partitioned again at
+                            the size its filled partitions project (at most the largest
+                            partitioned table), a global one falls back to
+                            ML_SIZING_PLAIN; both automatically, with identical results */
 };
 
 typedef struct ml_aggregate {
